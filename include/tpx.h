@@ -1,0 +1,162 @@
+/*
+ * tpx.h -- C ABI of the MI355X (gfx950) implementation of the RK4 time step
+ * of the triply periodic, staggered-grid, low-Mach, particle-laden solver.
+ *
+ * The reference exposes this path as a C++ class API called by its driver
+ * (simulation.cpp:57-93): grid's substep methods (grid.h:112-134),
+ * poisson::Solve (poisson.h:79), particle's methods (particle.h:121-138) and
+ * the tensor0/tensor1 reductions.  Each entry point below replaces one of
+ * those calls; the C++ classes in triplyperiodic-modified_amd/host forward
+ * to them with the reference's names.
+ *
+ * Conventions
+ *  - Every call returns 0 on success and a negative TPX_E* code on failure;
+ *    tpx_last_error() gives the message.  No call exits the process or lets a
+ *    C++ exception escape.
+ *  - Host arrays are caller-owned.  Grid arrays use the reference's global
+ *    C order [nz][ny][nx] (x fastest).  With bs > 0 they are the reference's
+ *    ghosted local arrays (nx+2bs)(ny+2bs)(nz+2bs): ghosts are ignored on
+ *    upload and filled with periodic images on download.
+ *  - All device state stays resident between calls.  Calls on one context
+ *    are ordered on its HIP stream; contexts are independent; one context
+ *    must not be used from two host threads at once.
+ *  - Grid sizes must be powers of two in [8, 1024] (the FFT Poisson solve).
+ */
+#ifndef TPX_H
+#define TPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TPX_OK 0
+#define TPX_E_ARG (-1)      /* invalid argument */
+#define TPX_E_HIP (-2)      /* HIP runtime error */
+#define TPX_E_STATE (-3)    /* call not valid in the current state */
+#define TPX_E_UNSUPP (-4)   /* feature outside this build (HYPRE, scalar, ...) */
+#define TPX_E_OOM (-5)      /* device allocation failed */
+
+/* The input-file parameters the step reads (params.cpp:23-59, params.h:8-158).
+ * Field meaning and units are the reference's. */
+typedef struct tpx_params {
+  int nx, ny, nz;                 /* Nx_tot Ny_tot Nz_tot */
+  double lx, ly, lz;              /* Lx Ly Lz */
+  int bs;                         /* bordersize of host arrays */
+  double dt, threshold;
+  int iteration1, iteration2;
+  double epsilon;
+  int twc, particle_gravity, cooling;
+  double a1, a2, rho0;
+  double u0, v0, w0;              /* mean velocity kept by make_mean_U0 */
+  double t0, mu0, k;
+  double gx1, gy1, gz1, gx2, gy2, gz2;
+  double cp, cv, np0, dp, rhop, cvp, epsilonp, nu, i01, i02;
+} tpx_params;
+
+typedef struct tpx_ctx tpx_ctx;
+
+/* field identifiers (grid.h:56-98) */
+typedef enum tpx_field {
+  TPX_RU = 0, TPX_RU_INT, TPX_RU_NEW, TPX_RU_NP1,  /* tensor1, comp 0..2 */
+  TPX_RHS_RU,                                       /* tensor1: deposited drag */
+  TPX_P = 16, TPX_RHS_POIS, TPX_C, TPX_RHO, TPX_RHO_INT, TPX_RHO_NEW,
+  TPX_RHO_NP1, TPX_DIVERGENCE, TPX_RHS_PART_TEMP
+} tpx_field;
+
+/* particle stage identifiers (particle.h:63-113) */
+typedef enum tpx_stage { TPX_BASE = 0, TPX_INT = 1, TPX_NEW = 2, TPX_NP1 = 3 } tpx_stage;
+
+/* scalar identifiers for tpx_get_scalars */
+typedef enum tpx_scalar {
+  TPX_S_P0 = 0, TPX_S_P0_INT, TPX_S_P0_NEW, TPX_S_P0_NP1, TPX_S_T_CUR,
+  TPX_S_NUM_TIMESTEP, TPX_S_NUM_ITERATION, TPX_S_MEAN_ENERGY, TPX_S_A,
+  TPX_S_I0, TPX_S_ITERATION, TPX_S_GX, TPX_S_GY, TPX_S_GZ, TPX_S_DP0_DT,
+  TPX_S_COUNT
+} tpx_scalar;
+
+/* ---- lifetime: replaces the grid/poisson/particle constructors
+ * (grid.cpp:12-59, poisson.cpp:11-92 incl. fft_3d_create_plan at :87,
+ * particle.cpp:14-42).  np_capacity bounds the particle count. */
+int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out);
+int tpx_destroy(tpx_ctx *ctx);
+const char *tpx_last_error(const tpx_ctx *ctx);
+/* the last error of a failed tpx_create (no context exists) */
+const char *tpx_create_error(void);
+
+/* ---- data movement: replaces communicator::read/write (communicator.cpp:65-106)
+ * and the public tensor pointers.  comp selects a tensor1 component. */
+int tpx_upload_field(tpx_ctx *ctx, int field, int comp, const double *host, int bs);
+int tpx_download_field(tpx_ctx *ctx, int field, int comp, double *host, int bs);
+
+/* replaces particle::Load_All / load_random (particle.cpp:899-963): sets the
+ * base state; tau / eps (per-particle response time and absorption
+ * efficiency) may be NULL to use the scalar Tp / epsilonp of the params. */
+int tpx_upload_particles(tpx_ctx *ctx, int64_t n, const double *x, const double *y,
+                         const double *z, const double *u, const double *v, const double *w,
+                         const double *T, const int32_t *flag, const double *tau,
+                         const double *eps);
+int tpx_num_particles(tpx_ctx *ctx, int64_t *n);
+/* one stage of x,y,z,u,v,w,T (any pointer NULL to skip) and the flags, in the
+ * device's current particle order (compare by flag). */
+int tpx_download_particles(tpx_ctx *ctx, int stage, double *x, double *y, double *z,
+                           double *u, double *v, double *w, double *T, int32_t *flag);
+/* per-particle RHS_T (particle.h:113) in the same order */
+int tpx_download_particle_rhs_t(tpx_ctx *ctx, double *rhs_t);
+
+/* restart numbers (grid.cpp:232-236): P0, T_cur, num_timestep */
+int tpx_set_restart_numbers(tpx_ctx *ctx, double P0, double T_cur, int num_timestep);
+int tpx_get_scalars(tpx_ctx *ctx, double *out /* TPX_S_COUNT */);
+
+/* ---- grid methods (grid.h:112-134).  rk is grid::RK4_count. */
+int tpx_initialize(tpx_ctx *ctx);                    /* grid.cpp:460-486 */
+int tpx_params_update(tpx_ctx *ctx);                 /* params.cpp:75-86 at T_cur */
+int tpx_update_rho(tpx_ctx *ctx, int rk);            /* grid.cpp:608-614 */
+int tpx_update_p0(tpx_ctx *ctx, int rk);             /* grid.cpp:816-830 */
+int tpx_update_particle(tpx_ctx *ctx, int rk);       /* grid.cpp:865-888 */
+int tpx_update_ru_wop(tpx_ctx *ctx, int rk);         /* grid.cpp:776-814 */
+int tpx_compute_div_u_new(tpx_ctx *ctx, int rk);     /* grid.cpp:832-839 */
+int tpx_compute_rhs_pois(tpx_ctx *ctx, int rk);      /* grid.cpp:841-849 */
+/* grid.cpp:851-855 -> poisson::Solve/FFT (poisson.cpp:115-117,391-462);
+ * *iterations receives poisson::num_iteration() (may be NULL) */
+int tpx_solve_poisson(tpx_ctx *ctx, int rk, int *iterations);
+int tpx_update_ru_wp(tpx_ctx *ctx, int rk);          /* grid.cpp:857-863 */
+int tpx_time_advance_rk4(tpx_ctx *ctx);              /* grid.cpp:890-900 */
+/* tensor1::make_mean_U0 on RU_np1 (tensor1.cpp:406-411), a = rho0*(U0,V0,W0) */
+int tpx_make_mean_u0(tpx_ctx *ctx, const double a[3]);
+int tpx_time_advance(tpx_ctx *ctx);                  /* grid.cpp:595-606 */
+
+/* ---- the whole step: simulation.cpp:58-91 with Update_Particle enabled,
+ * nsteps times, with the substeps fused where no result changes. */
+int tpx_step(tpx_ctx *ctx, int nsteps);
+
+/* ---- reductions of tensor0 (tensor0.cpp:364-396,466-478) on a field */
+int tpx_field_sum(tpx_ctx *ctx, int field, int comp, double *out);
+int tpx_field_max_abs(tpx_ctx *ctx, int field, int comp, double *out);
+/* particle::part2gas_concentration (particle.cpp:522-555) into TPX_C, at x_int */
+int tpx_deposit_concentration(tpx_ctx *ctx);
+
+/* ---- timing support: the HIP stream of the context (a hipStream_t) and
+ * per-kernel-class device times accumulated with HIP events since the last
+ * reset, in milliseconds, indexed by tpx_kclass. */
+typedef enum tpx_kclass {
+  TPX_K_GRID_RHS = 0,   /* density + momentum right-hand side (G1+G4) */
+  TPX_K_DIV_RHS,        /* low-Mach divergence + Poisson RHS (G5+G6) */
+  TPX_K_FFT,            /* Poisson iteration passes (G8/G9) */
+  TPX_K_PROJECT,        /* pressure-gradient projection (G10) */
+  TPX_K_PARTICLE,       /* particle RK4 advance and gathers (P1-P4, P8) */
+  TPX_K_DEPOSIT,        /* deposits (P5, P6, P9) */
+  TPX_K_SORT,           /* cell sort of the particles */
+  TPX_K_OTHER,          /* reductions, scalars, rotations */
+  TPX_K_COUNT
+} tpx_kclass;
+int tpx_get_stream(tpx_ctx *ctx, void **stream);
+int tpx_timing_enable(tpx_ctx *ctx, int on);
+int tpx_timing_read(tpx_ctx *ctx, double *ms /* TPX_K_COUNT */, int64_t *launches /* TPX_K_COUNT */);
+int tpx_synchronize(tpx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1018 @@
+/*
+ * or_core.c -- oracle for the RK4 step (TEST INFRASTRUCTURE ONLY; see
+ * oracle.h).  Every routine names the reference lines it restates.  Loops
+ * whose iterations are independent may run in any order (and in parallel);
+ * reductions and particle deposits keep the reference's order so the
+ * rounding matches it.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "oracle.h"
+
+#define OR_PI 3.141592653589793 /* params.h:5 */
+#define NT1 9
+#define NT0 11
+
+struct or_state {
+  or_params p;
+  int nx, ny, nz, bs, NX, NY, NZ;
+  long NXNY, n;
+  double dx, dy, dz, idx, idy, idz, idx2, idy2, idz2;
+  double R, P0par, Tp, mp, Vcell, xh, yh, zh;
+  double A, gx, gy, gz, I0;
+  int Iteration;
+  double pre[4], post[4];
+  double *t1[NT1][3];
+  double *t0[NT0];
+  double *cc, *rhs_new, *pd[3], *fft;
+  int num_iteration;
+  double P0, P0_int, P0_new, P0_np1, dP0_dt, met, T_cur;
+  int num_timestep, rk;
+  int np, cap, has_tau, has_eps;
+  double *q[7][4];
+  double *aux[10];
+  int *flag;
+  double *scratch[4];
+};
+
+#define G(s, i, j, k) ((long)(k) * (s)->NXNY + (long)(j) * (s)->NX + (long)(i))
+
+/* ------------------------------------------------------------------ */
+/* allocation                                                          */
+/* ------------------------------------------------------------------ */
+static double *fld(const or_state *s) { return (double *)calloc((size_t)s->n, sizeof(double)); }
+
+or_state *or_create(const or_params *p, int np_capacity)
+{
+  or_state *s = (or_state *)calloc(1, sizeof(or_state));
+  s->p = *p;
+  s->nx = p->nx; s->ny = p->ny; s->nz = p->nz; s->bs = p->bs;
+  s->NX = p->nx + 2 * p->bs; s->NY = p->ny + 2 * p->bs; s->NZ = p->nz + 2 * p->bs;
+  s->NXNY = (long)s->NX * s->NY;
+  s->n = s->NXNY * s->NZ;
+  /* gridsize.cpp:23-25, tensor0.cpp:12-32 */
+  s->dx = p->lx / p->nx; s->dy = p->ly / p->ny; s->dz = p->lz / p->nz;
+  s->idx = 1. / s->dx; s->idy = 1. / s->dy; s->idz = 1. / s->dz;
+  s->idx2 = 1. / (s->dx * s->dx); s->idy2 = 1. / (s->dy * s->dy); s->idz2 = 1. / (s->dz * s->dz);
+  s->Vcell = s->dx * s->dy * s->dz;                 /* gridsize.h:31 */
+  s->xh = (double)p->nx * s->dx; s->yh = (double)p->ny * s->dy; s->zh = (double)p->nz * s->dz; /* gridsize.cpp:67-77 */
+  /* params.cpp:61-72 */
+  s->R = p->cp - p->cv;
+  s->P0par = p->rho0 * p->t0 * s->R;
+  s->Tp = p->rhop * p->dp * p->dp / (18 * p->mu0);
+  s->mp = p->rhop * OR_PI * p->dp * p->dp * p->dp / 6.;
+  s->Iteration = p->iteration1; s->A = p->a1;
+  s->gx = p->gx1; s->gy = p->gy1; s->gz = p->gz1; s->I0 = p->i01;
+  /* grid.cpp:18-25 */
+  s->pre[0] = 0.5; s->pre[1] = 0.5; s->pre[2] = 1.; s->pre[3] = 1. / 6.;
+  s->post[0] = 1. / 6.; s->post[1] = 1. / 3.; s->post[2] = 1. / 3.; s->post[3] = 1. / 6.;
+  for (int a = 0; a < NT1; ++a) for (int c = 0; c < 3; ++c) s->t1[a][c] = fld(s);
+  for (int a = 0; a < NT0; ++a) s->t0[a] = fld(s);
+  s->cc = fld(s); s->rhs_new = fld(s);
+  for (int c = 0; c < 3; ++c) s->pd[c] = fld(s);
+  for (int c = 0; c < 4; ++c) s->scratch[c] = fld(s);
+  s->fft = (double *)calloc(2 * (size_t)p->nx * p->ny * p->nz, sizeof(double));
+  s->cap = np_capacity > 0 ? np_capacity : 1;
+  for (int a = 0; a < 7; ++a) for (int b = 0; b < 4; ++b) s->q[a][b] = (double *)calloc((size_t)s->cap, sizeof(double));
+  for (int a = 0; a < 10; ++a) s->aux[a] = (double *)calloc((size_t)s->cap, sizeof(double));
+  s->flag = (int *)calloc((size_t)s->cap, sizeof(int));
+  return s;
+}
+
+void or_destroy(or_state *s)
+{
+  if (!s) return;
+  for (int a = 0; a < NT1; ++a) for (int c = 0; c < 3; ++c) free(s->t1[a][c]);
+  for (int a = 0; a < NT0; ++a) free(s->t0[a]);
+  free(s->cc); free(s->rhs_new);
+  for (int c = 0; c < 3; ++c) free(s->pd[c]);
+  for (int c = 0; c < 4; ++c) free(s->scratch[c]);
+  free(s->fft);
+  for (int a = 0; a < 7; ++a) for (int b = 0; b < 4; ++b) free(s->q[a][b]);
+  for (int a = 0; a < 10; ++a) free(s->aux[a]);
+  free(s->flag);
+  free(s);
+}
+
+int or_nthreads(int n)
+{
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+static double **T1(or_state *s, int id) { return s->t1[id]; }
+static double *T0(or_state *s, int id) { return s->t0[id - OR_P]; }
+
+/* ------------------------------------------------------------------ */
+/* ghost exchange semantics on one rank                                */
+/* ------------------------------------------------------------------ */
+/* communicator.cpp:108-216: x sweep (interior y,z), then y (all x, interior
+ * z), then z (all x,y).  Each ghost becomes its periodic image. */
+static void fill(const or_state *s, double *f)
+{
+  int bs = s->bs, NX = s->NX, NY = s->NY, NZ = s->NZ;
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+#pragma omp parallel for schedule(static)
+  for (int k = bs; k < NZ - bs; ++k)
+    for (int j = bs; j < NY - bs; ++j)
+      for (int g = 0; g < bs; ++g) {
+        f[G(s, NX - bs + g, j, k)] = f[G(s, bs + g, j, k)];
+        f[G(s, g, j, k)] = f[G(s, g + nx, j, k)];
+      }
+#pragma omp parallel for schedule(static)
+  for (int k = bs; k < NZ - bs; ++k)
+    for (int g = 0; g < bs; ++g)
+      for (int i = 0; i < NX; ++i) {
+        f[G(s, i, NY - bs + g, k)] = f[G(s, i, bs + g, k)];
+        f[G(s, i, g, k)] = f[G(s, i, g + ny, k)];
+      }
+#pragma omp parallel for schedule(static)
+  for (int g = 0; g < bs; ++g)
+    for (int j = 0; j < NY; ++j)
+      for (int i = 0; i < NX; ++i) {
+        f[G(s, i, j, NZ - bs + g)] = f[G(s, i, j, bs + g)];
+        f[G(s, i, j, g)] = f[G(s, i, j, g + nz)];
+      }
+}
+
+/* communicator.cpp:228-317: sweeps z (all x,y), y (all x, interior z), x
+ * (interior y,z); each ghost deposit is ADDED to its periodic owner. */
+static void fold(const or_state *s, double *f)
+{
+  int bs = s->bs, NX = s->NX, NY = s->NY, NZ = s->NZ;
+  for (int i = 0; i < NX; ++i)
+    for (int j = 0; j < NY; ++j)
+      for (int g = 0; g < bs; ++g) {
+        double lo = f[G(s, i, j, g)], hi = f[G(s, i, j, NZ - bs + g)];
+        f[G(s, i, j, NZ - 2 * bs + g)] += lo;
+        f[G(s, i, j, bs + g)] += hi;
+      }
+  for (int i = 0; i < NX; ++i)
+    for (int g = 0; g < bs; ++g)
+      for (int k = bs; k < NZ - bs; ++k) {
+        double lo = f[G(s, i, g, k)], hi = f[G(s, i, NY - bs + g, k)];
+        f[G(s, i, NY - 2 * bs + g, k)] += lo;
+        f[G(s, i, bs + g, k)] += hi;
+      }
+  for (int g = 0; g < bs; ++g)
+    for (int j = bs; j < NY - bs; ++j)
+      for (int k = bs; k < NZ - bs; ++k) {
+        double lo = f[G(s, g, j, k)], hi = f[G(s, NX - bs + g, j, k)];
+        f[G(s, NX - 2 * bs + g, j, k)] += lo;
+        f[G(s, bs + g, j, k)] += hi;
+      }
+}
+
+/* ------------------------------------------------------------------ */
+/* element-wise operations over all elements incl. ghosts              */
+/* (tensor0.cpp:54-201)                                                */
+/* ------------------------------------------------------------------ */
+#define EW_LOOP for (long e = 0; e < s->n; ++e)
+static void ew_set(const or_state *s, double *o, double a) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] = a; }
+static void ew_copy(const or_state *s, double *o, const double *a) { memcpy(o, a, sizeof(double) * (size_t)s->n); }
+static void ew_add_s(const or_state *s, double *o, double a) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] += a; }
+static void ew_sub_s(const or_state *s, double *o, double a) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] -= a; }
+static void ew_mul(const or_state *s, double *o, const double *a) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] *= a[e]; }
+static void ew_sub(const or_state *s, double *o, const double *a) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] -= a[e]; }
+static void ew_add(const or_state *s, double *o, const double *a) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] += a[e]; }
+static void ew_mul_s(const or_state *s, double *o, double a) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] *= a; }
+/* PlusEqual_Mult(double a, tensor0 b): o += a*b */
+static void ew_pem(const or_state *s, double *o, double a, const double *b) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] += a * b[e]; }
+/* Equal_Divide(tensor0 a, tensor0 b): o = a/b */
+static void ew_div(const or_state *s, double *o, const double *a, const double *b) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] = a[e] / b[e]; }
+/* Equal_Divide(double a, tensor0 b): o = a/b */
+static void ew_sdiv(const or_state *s, double *o, double a, const double *b) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] = a / b[e]; }
+/* Equal_Mult(tensor0 a, double b) */
+static void ew_mulc(const or_state *s, double *o, const double *a, double b) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] = a[e] * b; }
+/* Equal_LinComb(a,T,b,R): o = a*T + b*R */
+static void ew_lc2(const or_state *s, double *o, double a, const double *T, double b, const double *Rr) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] = a * T[e] + b * Rr[e]; }
+/* Equal_LinComb(a,T,b,R,c,S): o = a*T + b*R + c*S */
+static void ew_lc3(const or_state *s, double *o, double a, const double *T, double b, const double *Rr, double c, const double *S) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] = a * T[e] + b * Rr[e] + c * S[e]; }
+
+/* ------------------------------------------------------------------ */
+/* stencils on the interior, followed by a ghost fill                  */
+/* (tensor0.cpp:204-296,354-362; tensor1.cpp:259-311)                   */
+/* ------------------------------------------------------------------ */
+#define IN_LOOP(s) \
+  _Pragma("omp parallel for schedule(static)") \
+  for (int k = (s)->bs; k < (s)->NZ - (s)->bs; ++k) \
+    for (int j = (s)->bs; j < (s)->NY - (s)->bs; ++j) \
+      for (int i = (s)->bs; i < (s)->NX - (s)->bs; ++i)
+
+/* S5: Equal_Div_F2C (tensor0.cpp:249-256) */
+static void div_f2c(or_state *s, double *o, double *const *T)
+{
+  long sx = 1, sy = s->NX, sz = s->NXNY;
+  IN_LOOP(s) {
+    long c = G(s, i, j, k);
+    o[c] = (T[0][c + sx] - T[0][c]) * s->idx + (T[1][c + sy] - T[1][c]) * s->idy +
+           (T[2][c + sz] - T[2][c]) * s->idz;
+  }
+  fill(s, o);
+}
+
+/* S6: Equal_Del2 (tensor0.cpp:258-266), d2dx2 (:234-247) */
+static void del2(or_state *s, double *o, const double *f)
+{
+  long sx = 1, sy = s->NX, sz = s->NXNY;
+  IN_LOOP(s) {
+    long c = G(s, i, j, k);
+    o[c] = (f[c + sx] - 2 * f[c] + f[c - sx]) * s->idx2 +
+           (f[c + sy] - 2 * f[c] + f[c - sy]) * s->idy2 +
+           (f[c + sz] - 2 * f[c] + f[c - sz]) * s->idz2;
+  }
+  fill(s, o);
+}
+
+/* S7: tensor1::Equal_Grad_C2F (tensor1.cpp:259-270) */
+static void grad_c2f(or_state *s, double *const *o, const double *f)
+{
+  long sx = 1, sy = s->NX, sz = s->NXNY;
+  IN_LOOP(s) {
+    long c = G(s, i, j, k);
+    o[0][c] = (f[c] - f[c - sx]) * s->idx;
+    o[1][c] = (f[c] - f[c - sy]) * s->idy;
+    o[2][c] = (f[c] - f[c - sz]) * s->idz;
+  }
+  fill(s, o[0]); fill(s, o[1]); fill(s, o[2]);
+}
+
+/* S4: I?_C2F (tensor0.cpp:288-296) into o, then fill */
+static void i_c2f_axis(or_state *s, double *o, const double *f, int axis)
+{
+  long st = axis == 0 ? 1 : (axis == 1 ? s->NX : s->NXNY);
+  IN_LOOP(s) {
+    long c = G(s, i, j, k);
+    o[c] = (f[c] + f[c - st]) * 0.5;
+  }
+  fill(s, o);
+}
+/* S4: I?_F2C (tensor0.cpp:268-286) */
+static void i_f2c_axis(or_state *s, double *o, const double *f, int axis)
+{
+  long st = axis == 0 ? 1 : (axis == 1 ? s->NX : s->NXNY);
+  IN_LOOP(s) {
+    long c = G(s, i, j, k);
+    o[c] = (f[c] + f[c + st]) * 0.5;
+  }
+  fill(s, o);
+}
+/* S8: tensor1::Equal_I_C2F(tensor0) (tensor1.cpp:278-283) */
+static void i_c2f(or_state *s, double *const *o, const double *f)
+{
+  i_c2f_axis(s, o[0], f, 0); i_c2f_axis(s, o[1], f, 1); i_c2f_axis(s, o[2], f, 2);
+}
+/* S8: tensor1::Equal_I{x,y,z}_C2F(tensor1) (tensor1.cpp:292-311) */
+static void iaxis_c2f_t1(or_state *s, double *const *o, double *const *T, int axis)
+{
+  i_c2f_axis(s, o[0], T[0], axis); i_c2f_axis(s, o[1], T[1], axis); i_c2f_axis(s, o[2], T[2], axis);
+}
+/* S9: tensor0::Equal_I_F2C(tensor1) (tensor0.cpp:354-362) */
+static void i_f2c_mean(or_state *s, double *o, double *const *T)
+{
+  double third = 1. / 3.;
+  long sx = 1, sy = s->NX, sz = s->NXNY;
+  IN_LOOP(s) {
+    long c = G(s, i, j, k);
+    o[c] = third * ((T[0][c] + T[0][c + sx]) * 0.5 + (T[1][c] + T[1][c + sy]) * 0.5 +
+                    (T[2][c] + T[2][c + sz]) * 0.5);
+  }
+  fill(s, o);
+}
+
+/* ------------------------------------------------------------------ */
+/* reductions (tensor0.cpp:364-396, 466-478): interior, i->j->k order  */
+/* ------------------------------------------------------------------ */
+static double red_sum(const or_state *s, const double *f)
+{
+  double acc = 0;
+  for (int i = s->bs; i < s->NX - s->bs; ++i)
+    for (int j = s->bs; j < s->NY - s->bs; ++j)
+      for (int k = s->bs; k < s->NZ - s->bs; ++k) acc += f[G(s, i, j, k)];
+  return acc;
+}
+static double red_mean(const or_state *s, const double *f)
+{
+  return red_sum(s, f) / (double)(s->nx * s->ny * s->nz); /* sum()/size_tot() */
+}
+static double red_max_abs(const or_state *s, const double *f)
+{
+  double m = fabs(f[G(s, s->bs, s->bs, s->bs)]);
+  for (int k = s->bs; k < s->NZ - s->bs; ++k)
+    for (int j = s->bs; j < s->NY - s->bs; ++j)
+      for (int i = s->bs; i < s->NX - s->bs; ++i) {
+        double v = fabs(f[G(s, i, j, k)]);
+        if (v > m) m = v;
+      }
+  return m;
+}
+/* tensor0::make_mean_zero (tensor0.cpp:380-384) */
+static void make_mean_zero(or_state *s, double *f) { double m = red_mean(s, f); ew_sub_s(s, f, m); }
+
+/* ------------------------------------------------------------------ */
+/* field access                                                        */
+/* ------------------------------------------------------------------ */
+static double *field_ptr(or_state *s, int id, int comp)
+{
+  if (id >= 0 && id < NT1 && comp >= 0 && comp < 3) return s->t1[id][comp];
+  if (id >= OR_P && id < OR_P + NT0) return s->t0[id - OR_P];
+  return NULL;
+}
+
+int or_set_field(or_state *s, int id, int comp, const double *in)
+{
+  double *f = field_ptr(s, id, comp);
+  if (!f) return -1;
+  long m = 0;
+  for (int k = s->bs; k < s->NZ - s->bs; ++k)
+    for (int j = s->bs; j < s->NY - s->bs; ++j)
+      for (int i = s->bs; i < s->NX - s->bs; ++i) f[G(s, i, j, k)] = in[m++];
+  fill(s, f);
+  return 0;
+}
+
+int or_get_field(const or_state *s, int id, int comp, double *out)
+{
+  double *f = field_ptr((or_state *)s, id, comp);
+  if (!f) return -1;
+  long m = 0;
+  for (int k = s->bs; k < s->NZ - s->bs; ++k)
+    for (int j = s->bs; j < s->NY - s->bs; ++j)
+      for (int i = s->bs; i < s->NX - s->bs; ++i) out[m++] = f[G(s, i, j, k)];
+  return 0;
+}
+
+int or_get_field_ghosted(const or_state *s, int id, int comp, double *out)
+{
+  double *f = field_ptr((or_state *)s, id, comp);
+  if (!f) return -1;
+  memcpy(out, f, sizeof(double) * (size_t)s->n);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* particles                                                           */
+/* ------------------------------------------------------------------ */
+enum { QX = 0, QY, QZ, QU, QV, QW, QT };
+
+int or_set_particles(or_state *s, int n, const double *x, const double *y, const double *z,
+                     const double *u, const double *v, const double *w, const double *T,
+                     const int *flag, const double *tau, const double *eps)
+{
+  if (n > s->cap || n < 0) return -1;
+  const double *src[7] = {x, y, z, u, v, w, T};
+  for (int a = 0; a < 7; ++a) memcpy(s->q[a][OR_BASE], src[a], sizeof(double) * (size_t)n);
+  memcpy(s->flag, flag, sizeof(int) * (size_t)n);
+  s->has_tau = tau != NULL; s->has_eps = eps != NULL;
+  if (tau) memcpy(s->aux[OR_AUX_TAU], tau, sizeof(double) * (size_t)n);
+  if (eps) memcpy(s->aux[OR_AUX_EPS], eps, sizeof(double) * (size_t)n);
+  s->np = n;
+  return 0;
+}
+
+int or_get_np(const or_state *s) { return s->np; }
+
+int or_get_particles(const or_state *s, int stage, double *const *q, int *flag)
+{
+  if (stage < 0 || stage > 3) return -1;
+  for (int a = 0; a < 7; ++a)
+    if (q && q[a]) memcpy(q[a], s->q[a][stage], sizeof(double) * (size_t)s->np);
+  if (flag) memcpy(flag, s->flag, sizeof(int) * (size_t)s->np);
+  return 0;
+}
+
+int or_get_particle_aux(const or_state *s, int which, double *out)
+{
+  if (which < 0 || which >= 10) return -1;
+  memcpy(out, s->aux[which], sizeof(double) * (size_t)s->np);
+  return 0;
+}
+
+int or_set_particle_aux(or_state *s, int which, const double *in)
+{
+  if (which < 0 || which >= 10) return -1;
+  memcpy(s->aux[which], in, sizeof(double) * (size_t)s->np);
+  return 0;
+}
+
+/* particle.cpp:106-134 (single rank: il = jl = kl = 0) */
+static int x2i(const or_state *s, double x) { return (int)(x * s->idx) + s->bs; }
+static int y2j(const or_state *s, double y) { return (int)(y * s->idy) + s->bs; }
+static int z2k(const or_state *s, double z) { return (int)(z * s->idz) + s->bs; }
+
+/* P1: particle::gas2part_velocity (particle.cpp:340-360) at x_int */
+void or_p_gather_velocity(or_state *s)
+{
+  double **U = T1(s, OR_U);
+  double *xi = s->q[QX][OR_INT], *yi = s->q[QY][OR_INT], *zi = s->q[QZ][OR_INT];
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < s->np; ++p) {
+    int ii = x2i(s, xi[p]), jj = y2j(s, yi[p]), kk = z2k(s, zi[p]);
+    double xl = xi[p] * s->idx - (int)(xi[p] * s->idx);
+    double yl = yi[p] * s->idy - (int)(yi[p] * s->idy);
+    double zl = zi[p] * s->idz - (int)(zi[p] * s->idz);
+    s->aux[OR_AUX_UG][p] = xl * U[0][G(s, ii + 1, jj, kk)] + (1 - xl) * U[0][G(s, ii, jj, kk)];
+    s->aux[OR_AUX_VG][p] = yl * U[1][G(s, ii, jj + 1, kk)] + (1 - yl) * U[1][G(s, ii, jj, kk)];
+    s->aux[OR_AUX_WG][p] = zl * U[2][G(s, ii, jj, kk + 1)] + (1 - zl) * U[2][G(s, ii, jj, kk)];
+  }
+}
+
+/* P2: particle::update_position (particle.cpp:314-338) */
+static void update_position(or_state *s)
+{
+  double dt = s->p.dt, post = s->post[s->rk], pre = s->pre[s->rk];
+  for (int a = QX; a <= QZ; ++a) {
+    double *np1 = s->q[a][OR_NP1], *nw = s->q[a][OR_NEW], *b = s->q[a][OR_BASE];
+    const double *ui = s->q[a + 3][OR_INT];
+    for (int p = 0; p < s->np; ++p) np1[p] += dt * post * ui[p];
+    if (s->rk != 3) for (int p = 0; p < s->np; ++p) nw[p] = b[p] + dt * pre * ui[p];
+    else for (int p = 0; p < s->np; ++p) nw[p] = np1[p];
+  }
+}
+
+/* P3: particle::update_velocity (particle.cpp:362-399); per-particle tau is
+ * the build's generalisation of the scalar Tp. */
+static void update_velocity(or_state *s)
+{
+  double iTp = s->Tp != 0 ? 1. / s->Tp : 0.;
+  double dt = s->p.dt, post = s->post[s->rk], pre = s->pre[s->rk];
+  double g[3] = {(double)s->p.particle_gravity * s->gx, (double)s->p.particle_gravity * s->gy,
+                 (double)s->p.particle_gravity * s->gz};
+  double *gas[3] = {s->aux[OR_AUX_UG], s->aux[OR_AUX_VG], s->aux[OR_AUX_WG]};
+  double *rhs[3] = {s->aux[OR_AUX_RHS_U], s->aux[OR_AUX_RHS_V], s->aux[OR_AUX_RHS_W]};
+  for (int a = 0; a < 3; ++a) {
+    const double *ui = s->q[QU + a][OR_INT];
+    for (int p = 0; p < s->np; ++p) {
+      double itp = s->has_tau ? 1. / s->aux[OR_AUX_TAU][p] : iTp;
+      rhs[a][p] = (gas[a][p] - ui[p]) * itp;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    double *np1 = s->q[QU + a][OR_NP1], *nw = s->q[QU + a][OR_NEW], *b = s->q[QU + a][OR_BASE];
+    for (int p = 0; p < s->np; ++p) np1[p] += dt * post * (rhs[a][p] + g[a]);
+    if (s->rk != 3) for (int p = 0; p < s->np; ++p) nw[p] = b[p] + dt * pre * (rhs[a][p] + g[a]);
+    else for (int p = 0; p < s->np; ++p) nw[p] = np1[p];
+  }
+}
+
+/* P4: particle::update_Temp (particle.cpp:477-492); per-particle epsilon is
+ * the build's generalisation of the scalar epsilonp. */
+static void update_temp(or_state *s)
+{
+  double dt = s->p.dt, post = s->post[s->rk], pre = s->pre[s->rk];
+  double iS = 1. / (s->mp * s->p.cvp);
+  double *np1 = s->q[QT][OR_NP1], *nw = s->q[QT][OR_NEW], *b = s->q[QT][OR_BASE];
+  const double *rt = s->aux[OR_AUX_RHS_T];
+  for (int p = 0; p < s->np; ++p) {
+    double epsp = s->has_eps ? s->aux[OR_AUX_EPS][p] : s->p.epsilonp;
+    double alpha = 0.25 * epsp * OR_PI * s->p.dp * s->p.dp * s->I0;
+    np1[p] += dt * post * iS * (rt[p] + alpha);
+  }
+  if (s->rk != 3)
+    for (int p = 0; p < s->np; ++p) {
+      double epsp = s->has_eps ? s->aux[OR_AUX_EPS][p] : s->p.epsilonp;
+      double alpha = 0.25 * epsp * OR_PI * s->p.dp * s->p.dp * s->I0;
+      nw[p] = b[p] + dt * pre * iS * (rt[p] + alpha);
+    }
+  else
+    for (int p = 0; p < s->np; ++p) nw[p] = np1[p];
+}
+
+/* CIC index/weight helper (particle.cpp:409-420, 437-448): ii=round(x/dx),
+ * xl = x/dx - ii + 0.5, then made local with +bs. */
+static void cic(const or_state *s, double x, double y, double z, int *ii, int *jj, int *kk,
+                double *xl, double *yl, double *zl)
+{
+  int i = (int)(round(x * s->idx)), j = (int)(round(y * s->idy)), k = (int)(round(z * s->idz));
+  *xl = x * s->idx - i + 0.5; *yl = y * s->idy - j + 0.5; *zl = z * s->idz - k + 0.5;
+  *ii = i + s->bs; *jj = j + s->bs; *kk = k + s->bs;
+}
+
+/* P5: particle::part2gas_concentration (particle.cpp:522-555) */
+void or_p_deposit_concentration(or_state *s)
+{
+  double *A = T0(s, OR_C);
+  ew_set(s, A, 0.);
+  double *xi = s->q[QX][OR_INT], *yi = s->q[QY][OR_INT], *zi = s->q[QZ][OR_INT];
+  for (int p = 0; p < s->np; ++p) {
+    int ii, jj, kk; double xl, yl, zl;
+    cic(s, xi[p], yi[p], zi[p], &ii, &jj, &kk, &xl, &yl, &zl);
+    A[G(s, ii, jj, kk)] += (xl * yl * zl);
+    A[G(s, ii - 1, jj, kk)] += (1 - xl) * yl * zl;
+    A[G(s, ii, jj - 1, kk)] += xl * (1 - yl) * zl;
+    A[G(s, ii, jj, kk - 1)] += xl * yl * (1 - zl);
+    A[G(s, ii - 1, jj - 1, kk)] += (1 - xl) * (1 - yl) * zl;
+    A[G(s, ii - 1, jj, kk - 1)] += (1 - xl) * yl * (1 - zl);
+    A[G(s, ii, jj - 1, kk - 1)] += xl * (1 - yl) * (1 - zl);
+    A[G(s, ii - 1, jj - 1, kk - 1)] += (1 - xl) * (1 - yl) * (1 - zl);
+  }
+  fold(s, A);
+}
+
+/* P6: particle::part2gas_velocity (particle.cpp:494-520) into RHS_RU */
+void or_p_deposit_velocity(or_state *s)
+{
+  double **A = T1(s, OR_RHS_RU);
+  for (int c = 0; c < 3; ++c) ew_set(s, A[c], 0.);
+  double *xi = s->q[QX][OR_INT], *yi = s->q[QY][OR_INT], *zi = s->q[QZ][OR_INT];
+  const double *ru = s->aux[OR_AUX_RHS_U], *rv = s->aux[OR_AUX_RHS_V], *rw = s->aux[OR_AUX_RHS_W];
+  for (int p = 0; p < s->np; ++p) {
+    int ii = x2i(s, xi[p]), jj = y2j(s, yi[p]), kk = z2k(s, zi[p]);
+    double xl = xi[p] * s->idx - (int)(xi[p] * s->idx);
+    double yl = yi[p] * s->idy - (int)(yi[p] * s->idy);
+    double zl = zi[p] * s->idz - (int)(zi[p] * s->idz);
+    A[0][G(s, ii + 1, jj, kk)] += xl * ru[p];
+    A[0][G(s, ii, jj, kk)] += (1 - xl) * ru[p];
+    A[1][G(s, ii, jj + 1, kk)] += yl * rv[p];
+    A[1][G(s, ii, jj, kk)] += (1 - yl) * rv[p];
+    A[2][G(s, ii, jj, kk + 1)] += zl * rw[p];
+    A[2][G(s, ii, jj, kk)] += (1 - zl) * rw[p];
+  }
+  for (int c = 0; c < 3; ++c) { fold(s, A[c]); fill(s, A[c]); }
+}
+
+/* P8/P10 gather: particle::gas2part_Temp_{int,new} (particle.cpp:402-456) */
+void or_p_gather_temp(or_state *s, int stage)
+{
+  const double *T = T0(s, OR_T);
+  double *x = s->q[QX][stage], *y = s->q[QY][stage], *z = s->q[QZ][stage];
+  double *Tg = s->aux[OR_AUX_TG];
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < s->np; ++p) {
+    int ii, jj, kk; double xl, yl, zl;
+    cic(s, x[p], y[p], z[p], &ii, &jj, &kk, &xl, &yl, &zl);
+    Tg[p] = xl * yl * zl * T[G(s, ii, jj, kk)] +
+            (1 - xl) * yl * zl * T[G(s, ii - 1, jj, kk)] + xl * (1 - yl) * zl * T[G(s, ii, jj - 1, kk)] +
+            xl * yl * (1 - zl) * T[G(s, ii, jj, kk - 1)] +
+            (1 - xl) * (1 - yl) * zl * T[G(s, ii - 1, jj - 1, kk)] +
+            (1 - xl) * yl * (1 - zl) * T[G(s, ii - 1, jj, kk - 1)] +
+            xl * (yl - 1) * (zl - 1) * T[G(s, ii, jj - 1, kk - 1)] +
+            (1 - xl) * (1 - yl) * (1 - zl) * T[G(s, ii - 1, jj - 1, kk - 1)];
+  }
+}
+
+/* P8/P10: particle::Compute_RHS_Temp_{int,new} (particle.cpp:460-474) */
+static void compute_rhs_temp(or_state *s, int stage)
+{
+  double iTth = s->p.nu * OR_PI * s->p.dp * s->p.k;
+  const double *Tp = s->q[QT][stage];
+  for (int p = 0; p < s->np; ++p) s->aux[OR_AUX_RHS_T][p] = (s->aux[OR_AUX_TG][p] - Tp[p]) * iTth;
+}
+
+/* P9/P10 deposit: particle::part2gas_Temp_{int,new} (particle.cpp:582-654) */
+void or_p_deposit_temp(or_state *s, int stage)
+{
+  double *A = T0(s, OR_RHS_PART_TEMP);
+  ew_set(s, A, 0.);
+  double *x = s->q[QX][stage], *y = s->q[QY][stage], *z = s->q[QZ][stage];
+  const double *rt = s->aux[OR_AUX_RHS_T];
+  for (int p = 0; p < s->np; ++p) {
+    int ii, jj, kk; double xl, yl, zl;
+    cic(s, x[p], y[p], z[p], &ii, &jj, &kk, &xl, &yl, &zl);
+    A[G(s, ii, jj, kk)] += xl * yl * zl * rt[p];
+    A[G(s, ii - 1, jj, kk)] += (1 - xl) * yl * zl * rt[p];
+    A[G(s, ii, jj - 1, kk)] += xl * (1 - yl) * zl * rt[p];
+    A[G(s, ii, jj, kk - 1)] += xl * yl * (1 - zl) * rt[p];
+    A[G(s, ii - 1, jj - 1, kk)] += (1 - xl) * (1 - yl) * zl * rt[p];
+    A[G(s, ii - 1, jj, kk - 1)] += (1 - xl) * yl * (1 - zl) * rt[p];
+    A[G(s, ii, jj - 1, kk - 1)] += xl * (1 - yl) * (1 - zl) * rt[p];
+    A[G(s, ii - 1, jj - 1, kk - 1)] += (1 - xl) * (1 - yl) * (1 - zl) * rt[p];
+  }
+  fold(s, A);
+  fill(s, A);
+}
+
+/* P7: particle::Send_Recv on one rank (particle.cpp:58-104, 150-311,
+ * 772-846).  A particle whose x_new leaves [0, L] is wrapped (x_new, x and
+ * x_np1 shift by L; x_int does not), removed by swapping with the last
+ * particle, and re-appended: first the ones that left below 0, then the ones
+ * that left above L.  Per-particle tau/eps travel with the particle. */
+#define NREC 31
+static void rec_store(const or_state *s, int p, double *r)
+{
+  int m = 0;
+  for (int a = 0; a < 7; ++a) for (int b = 0; b < 4; ++b) r[m++] = s->q[a][b][p];
+  r[m++] = (double)s->flag[p];
+  r[m++] = s->aux[OR_AUX_TAU][p];
+  r[m++] = s->aux[OR_AUX_EPS][p];
+}
+static void rec_load(or_state *s, int p, const double *r)
+{
+  int m = 0;
+  for (int a = 0; a < 7; ++a) for (int b = 0; b < 4; ++b) s->q[a][b][p] = r[m++];
+  s->flag[p] = (int)r[m++];
+  s->aux[OR_AUX_TAU][p] = r[m++];
+  s->aux[OR_AUX_EPS][p] = r[m++];
+}
+static void swap_p(or_state *s, int i, int j)
+{
+  for (int a = 0; a < 7; ++a)
+    for (int b = 0; b < 4; ++b) { double t = s->q[a][b][i]; s->q[a][b][i] = s->q[a][b][j]; s->q[a][b][j] = t; }
+  int t = s->flag[i]; s->flag[i] = s->flag[j]; s->flag[j] = t;
+  for (int a = OR_AUX_TAU; a <= OR_AUX_EPS; ++a) { double u = s->aux[a][i]; s->aux[a][i] = s->aux[a][j]; s->aux[a][j] = u; }
+}
+static void mode_l(or_state *s, int axis, int p, double L)
+{
+  double *xn = s->q[axis][OR_NEW], *x = s->q[axis][OR_BASE], *x1 = s->q[axis][OR_NP1];
+  if (xn[p] > L) { xn[p] -= L; x[p] -= L; x1[p] -= L; }
+  else if (xn[p] < 0) { xn[p] += L; x[p] += L; x1[p] += L; }
+}
+static void sort_valid(or_state *s, int axis, double lo, double hi, double L)
+{
+  double *buf_hi = (double *)malloc(sizeof(double) * NREC * (size_t)(s->np + 1));
+  double *buf_lo = (double *)malloc(sizeof(double) * NREC * (size_t)(s->np + 1));
+  int nhi = 0, nlo = 0, i = 0;
+  while (i < s->np) {
+    double xn = s->q[axis][OR_NEW][i];
+    if (xn > hi) {
+      mode_l(s, axis, i, L); rec_store(s, i, buf_hi + NREC * nhi++);
+      swap_p(s, i, s->np - 1); s->np--; i--;
+    } else if (xn < lo) {
+      mode_l(s, axis, i, L); rec_store(s, i, buf_lo + NREC * nlo++);
+      swap_p(s, i, s->np - 1); s->np--; i--;
+    }
+    i++;
+  }
+  for (int m = 0; m < nlo; ++m) rec_load(s, s->np++, buf_lo + NREC * m);
+  for (int m = 0; m < nhi; ++m) rec_load(s, s->np++, buf_hi + NREC * m);
+  free(buf_hi); free(buf_lo);
+}
+void or_p_send_recv(or_state *s)
+{
+  sort_valid(s, QX, 0.0, s->xh, s->p.lx);
+  sort_valid(s, QY, 0.0, s->yh, s->p.ly);
+  sort_valid(s, QZ, 0.0, s->zh, s->p.lz);
+}
+
+/* ------------------------------------------------------------------ */
+/* Poisson: poisson::FFT and poisson::CCP_FFT (poisson.cpp:391-462)    */
+/* ------------------------------------------------------------------ */
+static void ccp_fft(or_state *s, const double *rhs, double *out)
+{
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+  double tp = 2 * 3.141592653589793;
+  double tpx = tp / s->p.lx, tpy = tp / s->p.ly, tpz = tp / s->p.lz;
+  double *d = s->fft;
+  long m = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i, ++m) {
+        d[2 * m + 1] = 0;
+        d[2 * m] = rhs[G(s, i + s->bs, j + s->bs, k + s->bs)];
+      }
+  or_fft3d(d, nx, ny, nz, -1);
+  double dx2 = 2. / (s->dx * s->dx), dy2 = 2. / (s->dy * s->dy), dz2 = 2. / (s->dz * s->dz);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        long c = ((long)k * ny + j) * nx + i;
+        if (i + j + k > 0) {
+          double ii = ((i < nx / 2) ? i : i - nx) * tpx;
+          double jj = ((j < ny / 2) ? j : j - ny) * tpy;
+          double kk = ((k < nz / 2) ? k : k - nz) * tpz;
+          double lam = (dx2) * (1 - cos(ii * s->dx)) + (dy2) * (1 - cos(jj * s->dy)) + (dz2) * (1 - cos(kk * s->dz));
+          d[2 * c + 1] /= lam;
+          d[2 * c] /= lam;
+        }
+      }
+  or_fft3d(d, nx, ny, nz, +1);
+  double tot = (double)(nx * ny * nz);
+  m = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i, ++m)
+        out[G(s, i + s->bs, j + s->bs, k + s->bs)] = (-1) * d[2 * m] / (tot);
+}
+
+/* poisson::FFT (poisson.cpp:391-411) with C = coefficient tensor1 */
+static void poisson_fft(or_state *s, double *const *C, double *P, const double *RHS)
+{
+  i_f2c_mean(s, s->cc, C);
+  ew_sdiv(s, s->cc, -1., s->cc);
+  int it = 0;
+  double err = 100;
+  while ((it < s->Iteration) && (err > s->p.epsilon)) {
+    grad_c2f(s, s->pd, P);
+    for (int c = 0; c < 3; ++c) ew_mul(s, s->pd[c], C[c]);
+    div_f2c(s, s->rhs_new, s->pd);
+    ew_sub(s, s->rhs_new, RHS);
+    ew_mul(s, s->rhs_new, s->cc);
+    ccp_fft(s, s->rhs_new, s->pd[0]);
+    ew_add(s, P, s->pd[0]);
+    fill(s, P);
+    err = red_max_abs(s, s->pd[0]);
+    it++;
+  }
+  s->num_iteration = it;
+}
+
+/* ------------------------------------------------------------------ */
+/* grid methods                                                        */
+/* ------------------------------------------------------------------ */
+void or_set_rk4_count(or_state *s, int rk) { s->rk = rk; }
+
+/* params::update (params.cpp:75-86) */
+void or_params_update(or_state *s)
+{
+  if (s->T_cur > s->p.threshold) {
+    s->Iteration = s->p.iteration2; s->A = s->p.a2;
+    s->gx = s->p.gx2; s->gy = s->p.gy2; s->gz = s->p.gz2; s->I0 = s->p.i02;
+  }
+}
+
+/* G14: prep tail of grid::Initialize (grid.cpp:460-486) */
+void or_initialize(or_state *s)
+{
+  for (int c = 0; c < 3; ++c) {
+    ew_copy(s, T1(s, OR_RU_INT)[c], T1(s, OR_RU)[c]);
+    ew_copy(s, T1(s, OR_RU_NP1)[c], T1(s, OR_RU)[c]);
+  }
+  ew_copy(s, T0(s, OR_RHO_INT), T0(s, OR_RHO));
+  ew_copy(s, T0(s, OR_RHO_NP1), T0(s, OR_RHO));
+  s->P0_int = s->P0; s->P0_np1 = s->P0;
+  i_c2f(s, T1(s, OR_RHO_FACE), T0(s, OR_RHO));
+  for (int a = 0; a < 7; ++a) {
+    memcpy(s->q[a][OR_INT], s->q[a][OR_BASE], sizeof(double) * (size_t)s->np);
+    memcpy(s->q[a][OR_NP1], s->q[a][OR_BASE], sizeof(double) * (size_t)s->np);
+  }
+  ew_sdiv(s, T0(s, OR_T), s->P0_int / s->R, T0(s, OR_RHO_INT));
+  or_p_gather_temp(s, OR_INT);
+  compute_rhs_temp(s, OR_INT);
+  or_p_deposit_temp(s, OR_INT);
+}
+
+void or_set_restart_numbers(or_state *s, double P0, double T_cur, int num_timestep)
+{
+  s->P0 = P0; s->T_cur = T_cur; s->num_timestep = num_timestep;
+}
+
+/* G1: grid::Update_Rho (grid.cpp:608-614) */
+void or_update_rho(or_state *s)
+{
+  double dt = s->p.dt;
+  div_f2c(s, T0(s, OR_RHS_RHO), T1(s, OR_RU_INT));
+  ew_pem(s, T0(s, OR_RHO_NP1), -(dt * s->post[s->rk]), T0(s, OR_RHS_RHO));
+  if (s->rk != 3) ew_lc2(s, T0(s, OR_RHO_NEW), 1, T0(s, OR_RHO), -dt * s->pre[s->rk], T0(s, OR_RHS_RHO));
+  else ew_copy(s, T0(s, OR_RHO_NEW), T0(s, OR_RHO_NP1));
+}
+
+/* G2: grid::Update_P0 (grid.cpp:816-830) */
+void or_update_p0(or_state *s)
+{
+  s->met = red_mean(s, T0(s, OR_RHS_PART_TEMP));
+  if (s->p.cooling) s->dP0_dt = 0;
+  else s->dP0_dt = -s->R / s->p.cv * s->met / s->Vcell;
+  s->P0_np1 += s->p.dt * s->post[s->rk] * s->dP0_dt;
+  if (s->rk != 3) s->P0_new = s->P0 + s->p.dt * s->pre[s->rk] * s->dP0_dt;
+  else s->P0_new = s->P0_np1;
+}
+
+/* G3: grid::Update_Particle (grid.cpp:865-888) */
+void or_update_particle(or_state *s)
+{
+  for (int c = 0; c < 3; ++c) ew_div(s, T1(s, OR_U)[c], T1(s, OR_RU_INT)[c], T1(s, OR_RHO_FACE)[c]);
+  or_p_gather_velocity(s);
+  update_position(s);
+  update_velocity(s);
+  update_temp(s);
+  or_p_deposit_concentration(s);
+  if (s->p.twc) or_p_deposit_velocity(s);
+  else for (int c = 0; c < 3; ++c) ew_set(s, T1(s, OR_RHS_RU)[c], 0.);
+  or_p_send_recv(s);
+  ew_sdiv(s, T0(s, OR_T), s->P0_new / s->R, T0(s, OR_RHO_NEW));
+  or_p_gather_temp(s, OR_NEW);
+  compute_rhs_temp(s, OR_NEW);
+  or_p_deposit_temp(s, OR_NEW);
+}
+
+/* G4: grid::Update_RU_WOP (grid.cpp:776-814) */
+void or_update_ru_wop(or_state *s)
+{
+  double **U = T1(s, OR_U), **RUi = T1(s, OR_RU_INT), **RHS = T1(s, OR_RHS_RU);
+  double **d = T1(s, OR_DUMMY), **d2 = T1(s, OR_DUMMY2), **Rf = T1(s, OR_RHO_FACE);
+  double *dv = T0(s, OR_DIVERGENCE);
+  double dt = s->p.dt;
+  div_f2c(s, dv, U);
+  grad_c2f(s, d2, dv);
+  for (int c = 0; c < 3; ++c) del2(s, d[c], U[c]);
+  for (int c = 0; c < 3; ++c)
+    ew_lc3(s, RHS[c], -s->mp / s->Vcell, RHS[c], s->p.mu0 / 3., d2[c], s->p.mu0, d[c]);
+  for (int a = 0; a < 3; ++a) {          /* convection of component a */
+    i_c2f(s, d, U[a]);
+    iaxis_c2f_t1(s, d2, RUi, a);
+    for (int c = 0; c < 3; ++c) ew_mul(s, d[c], d2[c]);
+    div_f2c(s, d2[a], d);
+    ew_sub(s, RHS[a], d2[a]);
+  }
+  for (int c = 0; c < 3; ++c) ew_pem(s, RHS[c], s->A, RUi[c]);
+  for (int c = 0; c < 3; ++c) ew_sub_s(s, Rf[c], s->p.rho0);
+  ew_mulc(s, d[0], Rf[0], s->gx);
+  ew_mulc(s, d[1], Rf[1], s->gy);
+  ew_mulc(s, d[2], Rf[2], s->gz);
+  for (int c = 0; c < 3; ++c) ew_add(s, RHS[c], d[c]);
+  for (int c = 0; c < 3; ++c) {
+    ew_pem(s, T1(s, OR_RU_NP1)[c], dt * s->post[s->rk], RHS[c]);
+    if (s->rk != 3) ew_lc2(s, T1(s, OR_RU_NEW)[c], 1, T1(s, OR_RU)[c], dt * s->pre[s->rk], RHS[c]);
+    else ew_copy(s, T1(s, OR_RU_NEW)[c], T1(s, OR_RU_NP1)[c]);
+  }
+}
+
+/* G5: grid::Compute_Div_U_new (grid.cpp:832-839) */
+void or_compute_div_u_new(or_state *s)
+{
+  double **d = T1(s, OR_DUMMY);
+  ew_sdiv(s, d[0], 1., T0(s, OR_RHO_NEW));
+  del2(s, d[1], d[0]);
+  ew_copy(s, d[2], T0(s, OR_RHS_PART_TEMP));
+  if (s->p.cooling) make_mean_zero(s, d[2]);
+  ew_lc2(s, T0(s, OR_DIVERGENCE), s->p.k / s->p.cp, d[1], -s->R / (s->p.cp * s->P0_new * s->Vcell), d[2]);
+}
+
+/* G6: grid::Compute_RHS_Pois (grid.cpp:841-849) */
+void or_compute_rhs_pois(or_state *s)
+{
+  double **Rf = T1(s, OR_RHO_FACE), **U = T1(s, OR_U);
+  double *R = T0(s, OR_RHS_POIS);
+  i_c2f(s, Rf, T0(s, OR_RHO_NEW));
+  for (int c = 0; c < 3; ++c) ew_div(s, U[c], T1(s, OR_RU_NEW)[c], Rf[c]);
+  div_f2c(s, R, U);
+  ew_sub(s, R, T0(s, OR_DIVERGENCE));
+  ew_mul_s(s, R, (1. / (s->p.dt * s->pre[s->rk])));
+  make_mean_zero(s, R);
+}
+
+/* G7: grid::Solve_Poisson (grid.cpp:851-855) -> poisson::Solve (Solver==0) */
+void or_solve_poisson(or_state *s)
+{
+  double **d = T1(s, OR_DUMMY);
+  for (int c = 0; c < 3; ++c) ew_sdiv(s, d[c], 1., T1(s, OR_RHO_FACE)[c]);
+  poisson_fft(s, d, T0(s, OR_P), T0(s, OR_RHS_POIS));
+}
+
+/* G10: grid::Update_RU_WP (grid.cpp:857-863) */
+void or_update_ru_wp(or_state *s)
+{
+  double **d = T1(s, OR_DUMMY), dt = s->p.dt;
+  grad_c2f(s, d, T0(s, OR_P));
+  for (int c = 0; c < 3; ++c) {
+    ew_pem(s, T1(s, OR_RU_NP1)[c], -(dt * s->post[s->rk]), d[c]);
+    if (s->rk != 3) ew_pem(s, T1(s, OR_RU_NEW)[c], -(dt * s->pre[s->rk]), d[c]);
+    else ew_copy(s, T1(s, OR_RU_NEW)[c], T1(s, OR_RU_NP1)[c]);
+  }
+}
+
+/* G11: grid::TimeAdvance_RK4 (grid.cpp:890-900) */
+void or_time_advance_rk4(or_state *s)
+{
+  ew_copy(s, T0(s, OR_RHO_INT), T0(s, OR_RHO_NEW));
+  for (int c = 0; c < 3; ++c) ew_copy(s, T1(s, OR_RU_INT)[c], T1(s, OR_RU_NEW)[c]);
+  s->P0_int = s->P0_new;
+  for (int a = 0; a < 7; ++a) memcpy(s->q[a][OR_INT], s->q[a][OR_NEW], sizeof(double) * (size_t)s->np);
+}
+
+/* G12: tensor1::make_mean_U0 on RU_np1 (tensor1.cpp:406-411, tensor0.cpp:386-396) */
+void or_make_mean_u0(or_state *s, const double a[3])
+{
+  for (int c = 0; c < 3; ++c) {
+    double *f = T1(s, OR_RU_NP1)[c];
+    double m = red_mean(s, f);
+    ew_sub_s(s, f, m);
+    ew_add_s(s, f, a[c]);
+  }
+}
+
+/* G13: grid::TimeAdvance (grid.cpp:595-606) */
+void or_time_advance(or_state *s)
+{
+  ew_copy(s, T0(s, OR_RHO), T0(s, OR_RHO_NP1));
+  for (int c = 0; c < 3; ++c) ew_copy(s, T1(s, OR_RU)[c], T1(s, OR_RU_NP1)[c]);
+  s->P0 = s->P0_np1;
+  for (int a = 0; a < 7; ++a) memcpy(s->q[a][OR_BASE], s->q[a][OR_NP1], sizeof(double) * (size_t)s->np);
+  s->T_cur += s->p.dt;
+  s->num_timestep++;
+}
+
+/* simulation.cpp:57-93 with Update_Particle enabled at :64 (scalar, vector,
+ * elongated box, statistics and touch switch off). */
+void or_step(or_state *s, int nsteps)
+{
+  double a[3] = {s->p.u0 * s->p.rho0, s->p.v0 * s->p.rho0, s->p.w0 * s->p.rho0};
+  for (int n = 0; n < nsteps; ++n) {
+    or_params_update(s);
+    for (s->rk = 0; s->rk < 4; s->rk++) {
+      or_update_rho(s);
+      or_update_p0(s);
+      or_update_particle(s);
+      or_update_ru_wop(s);
+      or_compute_div_u_new(s);
+      or_compute_rhs_pois(s);
+      or_solve_poisson(s);
+      or_update_ru_wp(s);
+      or_time_advance_rk4(s);
+    }
+    or_make_mean_u0(s, a);
+    or_time_advance(s);
+  }
+}
+
+void or_get_scalars(const or_state *s, double *o)
+{
+  o[OR_S_P0] = s->P0; o[OR_S_P0_INT] = s->P0_int; o[OR_S_P0_NEW] = s->P0_new; o[OR_S_P0_NP1] = s->P0_np1;
+  o[OR_S_T_CUR] = s->T_cur; o[OR_S_NUM_TIMESTEP] = s->num_timestep;
+  o[OR_S_NUM_ITERATION] = s->num_iteration; o[OR_S_MEAN_ENERGY] = s->met; o[OR_S_A] = s->A;
+  o[OR_S_I0] = s->I0; o[OR_S_ITERATION] = s->Iteration; o[OR_S_GX] = s->gx; o[OR_S_GY] = s->gy;
+  o[OR_S_GZ] = s->gz; o[OR_S_DP0_DT] = s->dP0_dt;
+}
+
+/* ------------------------------------------------------------------ */
+/* primitives for closed-form pins                                     */
+/* ------------------------------------------------------------------ */
+static void load_int(or_state *s, double *f, const double *in)
+{
+  long m = 0;
+  for (int k = s->bs; k < s->NZ - s->bs; ++k)
+    for (int j = s->bs; j < s->NY - s->bs; ++j)
+      for (int i = s->bs; i < s->NX - s->bs; ++i) f[G(s, i, j, k)] = in[m++];
+  fill(s, f);
+}
+static void store_int(const or_state *s, const double *f, double *out)
+{
+  long m = 0;
+  for (int k = s->bs; k < s->NZ - s->bs; ++k)
+    for (int j = s->bs; j < s->NY - s->bs; ++j)
+      for (int i = s->bs; i < s->NX - s->bs; ++i) out[m++] = f[G(s, i, j, k)];
+}
+
+int or_apply(or_state *s, int op, const double *a0, const double *a1, const double *a2,
+             double *o0, double *o1, double *o2)
+{
+  double *in[3] = {s->scratch[0], s->scratch[1], s->scratch[2]};
+  double *out[3] = {s->pd[0], s->pd[1], s->pd[2]};
+  if (a0 && op != OR_OP_FOLD) load_int(s, in[0], a0);
+  if (a1) load_int(s, in[1], a1);
+  if (a2) load_int(s, in[2], a2);
+  switch (op) {
+  case OR_OP_DIV_F2C: div_f2c(s, out[0], in); store_int(s, out[0], o0); break;
+  case OR_OP_GRAD_C2F: grad_c2f(s, out, in[0]); store_int(s, out[0], o0); store_int(s, out[1], o1); store_int(s, out[2], o2); break;
+  case OR_OP_DEL2: del2(s, out[0], in[0]); store_int(s, out[0], o0); break;
+  case OR_OP_I_C2F: i_c2f(s, out, in[0]); store_int(s, out[0], o0); store_int(s, out[1], o1); store_int(s, out[2], o2); break;
+  case OR_OP_I_F2C: i_f2c_mean(s, out[0], in); store_int(s, out[0], o0); break;
+  case OR_OP_IX_C2F: case OR_OP_IY_C2F: case OR_OP_IZ_C2F:
+    i_c2f_axis(s, out[0], in[0], op - OR_OP_IX_C2F); store_int(s, out[0], o0); break;
+  case OR_OP_IX_F2C: case OR_OP_IY_F2C: case OR_OP_IZ_F2C:
+    i_f2c_axis(s, out[0], in[0], op - OR_OP_IX_F2C); store_int(s, out[0], o0); break;
+  case OR_OP_FOLD:
+    memcpy(in[0], a0, sizeof(double) * (size_t)s->n); fold(s, in[0]); store_int(s, in[0], o0); break;
+  case OR_OP_CCP_FFT: ccp_fft(s, in[0], out[0]); store_int(s, out[0], o0); break;
+  default: return -1;
+  }
+  return 0;
+}
+
+double or_sum(or_state *s, const double *interior)
+{
+  load_int(s, s->scratch[3], interior);
+  return red_sum(s, s->scratch[3]);
+}
+
+double or_max_abs(or_state *s, const double *interior)
+{
+  load_int(s, s->scratch[3], interior);
+  return red_max_abs(s, s->scratch[3]);
+}

@@ -1,0 +1,505 @@
+// k_grid.hip -- staggered-grid kernels of the RK4 substep (gfx950).
+//
+// One thread per cell (i,j,k); it owns the cell centre and the three faces
+// i, j, k of that cell (x face at x = i dx, etc.).  Periodic neighbours are
+// formed by masking indices (grid sizes are powers of two).  Every
+// expression keeps the operand order of the reference operator it fuses, and
+// the library is built without FP contraction, so a fused kernel rounds
+// exactly like the reference's sequence of whole-array passes.
+#include <stdexcept>
+
+#include "tpx_internal.h"
+
+namespace tpx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Ix {
+  int mx, my, mz, nx;
+  long nxy;
+  __device__ __forceinline__ long operator()(int i, int j, int k) const {
+    return (long)(k & mz) * nxy + (long)(j & my) * nx + (i & mx);
+  }
+};
+
+__device__ __forceinline__ Ix make_ix(const Geo &g) {
+  Ix r;
+  r.mx = g.nx - 1; r.my = g.ny - 1; r.mz = g.nz - 1; r.nx = g.nx; r.nxy = g.nxy;
+  return r;
+}
+
+// cell (i,j,k) handled by this thread, with the XCD-contiguous block order
+__device__ __forceinline__ bool cell_of(const Geo &g, int &i, int &j, int &k, long &c) {
+  unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  c = (long)b * blockDim.x + threadIdx.x;
+  if (c >= g.n) return false;
+  i = (int)(c % g.nx);
+  long r = c / g.nx;
+  j = (int)(r % g.ny);
+  k = (int)(r / g.ny);
+  return true;
+}
+
+// deterministic block sum (fixed tree); result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__device__ __forceinline__ double block_max(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// U = RU / Rho_face on face a of cell (i,j,k): Equal_Divide(RU, Rho_face)
+// with Rho_face = I_C2F(Rho) (tensor1.cpp:278-283, tensor0.cpp:288-296)
+template <int A>
+__device__ __forceinline__ double Uface(const Ix &ix, const double *__restrict__ ru,
+                                        const double *__restrict__ rho, int i, int j, int k) {
+  long p = ix(i, j, k);
+  long q = ix(i - (A == 0), j - (A == 1), k - (A == 2));
+  return ru[p] / ((rho[p] + rho[q]) * 0.5);
+}
+
+struct RU3 { const double *__restrict__ a[3]; };
+
+// div_F2C(U) at cell centre (i,j,k) (tensor0.cpp:249-256)
+__device__ __forceinline__ double divU(const Geo &g, const Ix &ix, const RU3 &ru,
+                                       const double *__restrict__ rho, int i, int j, int k) {
+  return (Uface<0>(ix, ru.a[0], rho, i + 1, j, k) - Uface<0>(ix, ru.a[0], rho, i, j, k)) * g.idx +
+         (Uface<1>(ix, ru.a[1], rho, i, j + 1, k) - Uface<1>(ix, ru.a[1], rho, i, j, k)) * g.idy +
+         (Uface<2>(ix, ru.a[2], rho, i, j, k + 1) - Uface<2>(ix, ru.a[2], rho, i, j, k)) * g.idz;
+}
+
+// The right-hand side of momentum component A on face A of cell (i,j,k):
+// grid.cpp:780-809 (viscous + convective + forcing + body force), with the
+// drag deposit S_A already known.
+template <int A>
+__device__ __forceinline__ double rhs_face(const Geo &g, const Ix &ix, const Coef &kc,
+                                           const RU3 &ru, const double *__restrict__ rho,
+                                           double S_A, int i, int j, int k) {
+  constexpr int di = (A == 0), dj = (A == 1), dk = (A == 2);
+  const double id[3] = {g.idx, g.idy, g.idz};
+  const double *__restrict__ ra = ru.a[A];
+  double Up = Uface<A>(ix, ra, rho, i, j, k);
+  double Uxm = Uface<A>(ix, ra, rho, i - 1, j, k), Uxp = Uface<A>(ix, ra, rho, i + 1, j, k);
+  double Uym = Uface<A>(ix, ra, rho, i, j - 1, k), Uyp = Uface<A>(ix, ra, rho, i, j + 1, k);
+  double Uzm = Uface<A>(ix, ra, rho, i, j, k - 1), Uzp = Uface<A>(ix, ra, rho, i, j, k + 1);
+  // mu/3 grad(div U) and mu del2 U (grid.cpp:780-783)
+  double gdiv = (divU(g, ix, ru, rho, i, j, k) - divU(g, ix, ru, rho, i - di, j - dj, k - dk)) * id[A];
+  double lap = (Uxp - 2 * Up + Uxm) * g.idx2 + (Uyp - 2 * Up + Uym) * g.idy2 +
+               (Uzp - 2 * Up + Uzm) * g.idz2;
+  double r = kc.a_drag * S_A + kc.b_visc * gdiv + kc.c_visc * lap;
+  // convection (grid.cpp:784-801): div_F2C( I_d^{C2F}(U_A) * I_A^{C2F}(RU_d) )
+  const double *__restrict__ r0 = ru.a[0];
+  const double *__restrict__ r1 = ru.a[1];
+  const double *__restrict__ r2 = ru.a[2];
+  double fxp = ((Uxp + Up) * 0.5) * ((r0[ix(i + 1, j, k)] + r0[ix(i + 1 - di, j - dj, k - dk)]) * 0.5);
+  double fxm = ((Up + Uxm) * 0.5) * ((r0[ix(i, j, k)] + r0[ix(i - di, j - dj, k - dk)]) * 0.5);
+  double fyp = ((Uyp + Up) * 0.5) * ((r1[ix(i, j + 1, k)] + r1[ix(i - di, j + 1 - dj, k - dk)]) * 0.5);
+  double fym = ((Up + Uym) * 0.5) * ((r1[ix(i, j, k)] + r1[ix(i - di, j - dj, k - dk)]) * 0.5);
+  double fzp = ((Uzp + Up) * 0.5) * ((r2[ix(i, j, k + 1)] + r2[ix(i - di, j - dj, k + 1 - dk)]) * 0.5);
+  double fzm = ((Up + Uzm) * 0.5) * ((r2[ix(i, j, k)] + r2[ix(i - di, j - dj, k - dk)]) * 0.5);
+  double conv = (fxp - fxm) * g.idx + (fyp - fym) * g.idy + (fzp - fzm) * g.idz;
+  r = r - conv;
+  // linear forcing A*RU_int (grid.cpp:803)
+  r = r + kc.A * ra[ix(i, j, k)];
+  // body force (Rho_face - Rho0) g (grid.cpp:805-809)
+  double rf = (rho[ix(i, j, k)] + rho[ix(i - di, j - dj, k - dk)]) * 0.5;
+  r = r + (rf - kc.rho0) * kc.g[A];
+  return r;
+}
+
+struct RhsArgs {
+  RU3 ru_int;
+  const double *rho_int;
+  const double *ru_base[3];
+  const double *ru_np1_in[3];
+  const double *S[3];
+  double *ru_np1[3];
+  double *ru_new[3];
+  const double *rho_base;
+  const double *rho_np1_in;
+  double *rho_np1;
+  double *rho_new;
+  double c_np1, c_new, c_rho_np1, c_rho_new;
+};
+
+// K1: Update_RU_WOP (grid.cpp:776-814), optionally fused with Update_Rho
+// (grid.cpp:608-614), which reads the same RU_int.
+template <bool WITH_RHO, bool HAS_S, bool RK3>
+__global__ __launch_bounds__(kBlock) void k_rhs_ru(Geo g, Coef kc, RhsArgs a) {
+  int i, j, k;
+  long c;
+  if (!cell_of(g, i, j, k, c)) return;
+  Ix ix = make_ix(g);
+  const double *__restrict__ rho = a.rho_int;
+  double r0 = rhs_face<0>(g, ix, kc, a.ru_int, rho, HAS_S ? a.S[0][c] : 0.0, i, j, k);
+  double r1 = rhs_face<1>(g, ix, kc, a.ru_int, rho, HAS_S ? a.S[1][c] : 0.0, i, j, k);
+  double r2 = rhs_face<2>(g, ix, kc, a.ru_int, rho, HAS_S ? a.S[2][c] : 0.0, i, j, k);
+  double rr[3] = {r0, r1, r2};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double np1 = a.ru_np1_in[d][c] + a.c_np1 * rr[d];     // RU_np1 += dt*post*RHS
+    a.ru_np1[d][c] = np1;
+    if (!RK3) a.ru_new[d][c] = 1 * a.ru_base[d][c] + a.c_new * rr[d];  // RU_new = RU + dt*pre*RHS
+  }
+  if (WITH_RHO) {
+    const double *__restrict__ u0 = a.ru_int.a[0];
+    const double *__restrict__ u1 = a.ru_int.a[1];
+    const double *__restrict__ u2 = a.ru_int.a[2];
+    double R = (u0[ix(i + 1, j, k)] - u0[c]) * g.idx + (u1[ix(i, j + 1, k)] - u1[c]) * g.idy +
+               (u2[ix(i, j, k + 1)] - u2[c]) * g.idz;
+    double np1 = a.rho_np1_in[c] + a.c_rho_np1 * R;
+    a.rho_np1[c] = np1;
+    if (!RK3) a.rho_new[c] = 1 * a.rho_base[c] + a.c_rho_new * R;
+  }
+}
+
+// G1 alone: Update_Rho (grid.cpp:608-614)
+template <bool RK3>
+__global__ __launch_bounds__(kBlock) void k_update_rho(Geo g, RhsArgs a) {
+  int i, j, k;
+  long c;
+  if (!cell_of(g, i, j, k, c)) return;
+  Ix ix = make_ix(g);
+  const double *__restrict__ u0 = a.ru_int.a[0];
+  const double *__restrict__ u1 = a.ru_int.a[1];
+  const double *__restrict__ u2 = a.ru_int.a[2];
+  double R = (u0[ix(i + 1, j, k)] - u0[c]) * g.idx + (u1[ix(i, j + 1, k)] - u1[c]) * g.idy +
+             (u2[ix(i, j, k + 1)] - u2[c]) * g.idz;
+  double np1 = a.rho_np1_in[c] + a.c_rho_np1 * R;
+  a.rho_np1[c] = np1;
+  if (!RK3) a.rho_new[c] = 1 * a.rho_base[c] + a.c_rho_new * R;
+}
+
+struct DivArgs {
+  RU3 ru_new;
+  const double *rho_new;
+  const double *Q;
+  double *divergence;
+  double *rhs;
+  double *partials;
+  const double *scal;
+  double inv_dtpre;
+  double ncell;
+};
+
+// (k/Cp) del2(1/Rho_new) - R/(Cp P0_new Vcell) Q' (grid.cpp:832-839)
+__device__ __forceinline__ double low_mach_div(const Geo &g, const Ix &ix, const Coef &kc,
+                                               const DivArgs &a, int i, int j, int k, long c) {
+  const double *__restrict__ rho = a.rho_new;
+  double fc = 1. / rho[c];
+  double lap = ((1. / rho[ix(i + 1, j, k)]) - 2 * fc + (1. / rho[ix(i - 1, j, k)])) * g.idx2 +
+               ((1. / rho[ix(i, j + 1, k)]) - 2 * fc + (1. / rho[ix(i, j - 1, k)])) * g.idy2 +
+               ((1. / rho[ix(i, j, k + 1)]) - 2 * fc + (1. / rho[ix(i, j, k - 1)])) * g.idz2;
+  double q = a.Q ? a.Q[c] : 0.0;
+  if (kc.cooling && a.Q) q -= a.scal[SC_SUM_Q] / a.ncell;  // make_mean_zero
+  double P0n = a.scal[SC_P0_NEW];
+  return kc.k_over_cp * lap + -kc.R / (kc.cp * P0n * kc.Vcell) * q;
+}
+
+// G5+G6: Compute_Div_U_new + Compute_RHS_Pois (grid.cpp:832-849).  MODE 0
+// fuses both, MODE 1 stores the divergence field, MODE 2 reads it.  The raw
+// right-hand side is stored; its mean is removed where it is consumed.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_div_rhs(Geo g, Coef kc, DivArgs a) {
+  __shared__ double sh[kBlock];
+  int i, j, k;
+  long c;
+  bool ok = cell_of(g, i, j, k, c);
+  Ix ix = make_ix(g);
+  double v = 0.0;
+  if (ok) {
+    double dv;
+    if (MODE == 2) dv = a.divergence[c];
+    else dv = low_mach_div(g, ix, kc, a, i, j, k, c);
+    if (MODE == 1) {
+      a.divergence[c] = dv;
+    } else {
+      const double *__restrict__ rho = a.rho_new;
+      double du = (Uface<0>(ix, a.ru_new.a[0], rho, i + 1, j, k) - Uface<0>(ix, a.ru_new.a[0], rho, i, j, k)) * g.idx +
+                  (Uface<1>(ix, a.ru_new.a[1], rho, i, j + 1, k) - Uface<1>(ix, a.ru_new.a[1], rho, i, j, k)) * g.idy +
+                  (Uface<2>(ix, a.ru_new.a[2], rho, i, j, k + 1) - Uface<2>(ix, a.ru_new.a[2], rho, i, j, k)) * g.idz;
+      v = (du - dv) * a.inv_dtpre;
+      a.rhs[c] = v;
+    }
+  }
+  if (MODE != 1) {
+    double s = block_sum(v, sh);
+    if (threadIdx.x == 0) a.partials[xcd_block(blockIdx.x, gridDim.x)] = s;
+  }
+}
+
+struct ProjArgs {
+  const double *P;
+  double *np1[3];
+  double *nw[3];
+  double *partials;
+  long npart;
+  double c_np1, c_new;
+};
+
+// G10: Update_RU_WP (grid.cpp:857-863); at RK4_count==3 RU_new aliases
+// RU_np1 and the block sums of RU_np1 feed make_mean_U0.
+template <bool RK3>
+__global__ __launch_bounds__(kBlock) void k_project(Geo g, ProjArgs a) {
+  __shared__ double sh[kBlock];
+  int i, j, k;
+  long c;
+  bool ok = cell_of(g, i, j, k, c);
+  Ix ix = make_ix(g);
+  double s[3] = {0, 0, 0};
+  if (ok) {
+    const double *__restrict__ P = a.P;
+    double pc = P[c];
+    double gr[3] = {(pc - P[ix(i - 1, j, k)]) * g.idx, (pc - P[ix(i, j - 1, k)]) * g.idy,
+                    (pc - P[ix(i, j, k - 1)]) * g.idz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double v = a.np1[d][c] + a.c_np1 * gr[d];
+      a.np1[d][c] = v;
+      s[d] = v;
+      if (!RK3) a.nw[d][c] = a.nw[d][c] + a.c_new * gr[d];
+    }
+  }
+  if (RK3) {
+    unsigned b = xcd_block(blockIdx.x, gridDim.x);
+    for (int d = 0; d < 3; ++d) {
+      double t = block_sum(s[d], sh);
+      if (threadIdx.x == 0) a.partials[d * a.npart + b] = t;
+      __syncthreads();
+    }
+  }
+}
+
+// tensor0::make_mean_U0 (tensor0.cpp:386-396): out = (in - mean) + a
+__global__ __launch_bounds__(kBlock) void k_shift_mean(long n, const double *__restrict__ in,
+                                                       double *__restrict__ out, const double *scal,
+                                                       int slot, double ncell, double a) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double mean = scal[slot] / ncell;  // mean() = sum()/size_tot()
+  out[c] = (in[c] - mean) + a;
+}
+
+// generic reduction of one field into per-block partials
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_reduce(long n, const double *__restrict__ f,
+                                                   double *partials) {
+  __shared__ double sh[kBlock];
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (c < n) v = OP == 0 ? f[c] : fabs(f[c]);
+  double s = OP == 0 ? block_sum(v, sh) : block_max(v, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// sum (OP 0) or max (OP 1) of nparts partials into scal[slot], fixed order
+template <int OP>
+__global__ __launch_bounds__(1024) void k_finish(const double *__restrict__ partials, long nparts,
+                                                 double *scal, int slot) {
+  __shared__ double sh[1024];
+  long per = (nparts + 1023) / 1024;
+  long b = (long)threadIdx.x * per, e = b + per < nparts ? b + per : nparts;
+  double v = 0.0;
+  for (long q = b; q < e; ++q) v = OP == 0 ? v + partials[q] : fmax(v, partials[q]);
+  double s = OP == 0 ? block_sum(v, sh) : block_max(v, sh);
+  if (threadIdx.x == 0) scal[slot] = s;
+}
+
+// grid::Update_P0 (grid.cpp:816-830) on the device scalars
+__global__ void k_update_p0(Coef kc, double *scal, double ncell, double c_np1, double c_new) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double met = scal[SC_SUM_Q] / ncell;
+  scal[SC_MET] = met;
+  double dP0 = kc.cooling ? 0.0 : -kc.R / kc.cv * met / kc.Vcell;
+  scal[SC_DP0_DT] = dP0;
+  scal[SC_P0_NP1] += c_np1 * dP0;
+  if (kc.rk != 3) scal[SC_P0_NEW] = scal[SC_P0] + c_new * dP0;
+  else scal[SC_P0_NEW] = scal[SC_P0_NP1];
+}
+
+__global__ void k_scalar_copy(double *scal, int dst, int src) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) scal[dst] = scal[src];
+}
+
+__global__ void k_fill(double *f, double v, long n) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) f[c] = v;
+}
+
+__global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, long n) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) d[c] = s[c];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static unsigned cells_grid(const tpx_ctx *c) { return grid1d(c->g.n, kBlock); }
+
+void launch_update_rho(tpx_ctx *c, const Coef &k, const double *const ru_int[3],
+                       const double *rho_base, const double *rho_np1_in, double *rho_np1,
+                       double *rho_new) {
+  RhsArgs a{};
+  for (int d = 0; d < 3; ++d) a.ru_int.a[d] = ru_int[d];
+  a.rho_base = rho_base; a.rho_np1_in = rho_np1_in; a.rho_np1 = rho_np1; a.rho_new = rho_new;
+  a.c_rho_np1 = -(k.dt * k.post);
+  a.c_rho_new = -k.dt * k.pre;
+  timing_begin(c, TPX_K_GRID_RHS);
+  if (k.rk == 3) k_update_rho<true><<<cells_grid(c), kBlock, 0, c->stream>>>(c->g, a);
+  else k_update_rho<false><<<cells_grid(c), kBlock, 0, c->stream>>>(c->g, a);
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_GRID_RHS);
+}
+
+void launch_rhs_ru(tpx_ctx *c, const Coef &k, bool with_rho, const double *const ru_int[3],
+                   const double *rho_int, const double *const ru_base[3],
+                   const double *const ru_np1_in[3], const double *const S[3],
+                   double *const ru_np1[3], double *const ru_new[3], const double *rho_base,
+                   const double *rho_np1_in, double *rho_np1, double *rho_new) {
+  RhsArgs a{};
+  for (int d = 0; d < 3; ++d) {
+    a.ru_int.a[d] = ru_int[d];
+    a.ru_base[d] = ru_base[d];
+    a.ru_np1_in[d] = ru_np1_in[d];
+    a.S[d] = S ? S[d] : nullptr;
+    a.ru_np1[d] = ru_np1[d];
+    a.ru_new[d] = ru_new[d];
+  }
+  a.rho_int = rho_int;
+  a.rho_base = rho_base; a.rho_np1_in = rho_np1_in; a.rho_np1 = rho_np1; a.rho_new = rho_new;
+  a.c_np1 = k.dt * k.post;
+  a.c_new = k.dt * k.pre;
+  a.c_rho_np1 = -(k.dt * k.post);
+  a.c_rho_new = -k.dt * k.pre;
+  bool hs = S != nullptr, r3 = k.rk == 3;
+  dim3 gr(cells_grid(c)), bl(kBlock);
+  timing_begin(c, TPX_K_GRID_RHS);
+#define TPX_K1(WR, HS, R3) k_rhs_ru<WR, HS, R3><<<gr, bl, 0, c->stream>>>(c->g, k, a)
+  if (with_rho) {
+    if (hs) { if (r3) TPX_K1(true, true, true); else TPX_K1(true, true, false); }
+    else { if (r3) TPX_K1(true, false, true); else TPX_K1(true, false, false); }
+  } else {
+    if (hs) { if (r3) TPX_K1(false, true, true); else TPX_K1(false, true, false); }
+    else { if (r3) TPX_K1(false, false, true); else TPX_K1(false, false, false); }
+  }
+#undef TPX_K1
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_GRID_RHS);
+}
+
+void launch_div_rhs(tpx_ctx *c, const Coef &k, int mode, const double *const ru_new[3],
+                    const double *rho_new, const double *Q, double *divergence, double *rhs) {
+  DivArgs a{};
+  for (int d = 0; d < 3; ++d) a.ru_new.a[d] = ru_new[d];
+  a.rho_new = rho_new; a.Q = Q; a.divergence = divergence; a.rhs = rhs;
+  a.partials = c->partials; a.scal = c->scal;
+  a.inv_dtpre = (1. / (k.dt * k.pre));
+  a.ncell = (double)c->g.n;
+  unsigned nb = cells_grid(c);
+  timing_begin(c, TPX_K_DIV_RHS);
+  if (mode == 0) k_div_rhs<0><<<nb, kBlock, 0, c->stream>>>(c->g, k, a);
+  else if (mode == 1) k_div_rhs<1><<<nb, kBlock, 0, c->stream>>>(c->g, k, a);
+  else k_div_rhs<2><<<nb, kBlock, 0, c->stream>>>(c->g, k, a);
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_DIV_RHS);
+  if (mode != 1) {
+    timing_begin(c, TPX_K_OTHER);
+    k_finish<0><<<1, 1024, 0, c->stream>>>(c->partials, nb, c->scal, SC_SUM_RHS);
+    TPX_LAUNCH_CHECK();
+    timing_end(c, TPX_K_OTHER);
+  }
+}
+
+void launch_project(tpx_ctx *c, const Coef &k, const double *P, double *const ru_np1[3],
+                    double *const ru_new[3], bool copy_np1_to_new, bool want_sums) {
+  ProjArgs a{};
+  a.P = P;
+  for (int d = 0; d < 3; ++d) { a.np1[d] = ru_np1[d]; a.nw[d] = ru_new[d]; }
+  a.partials = c->partials;
+  unsigned nb = cells_grid(c);
+  a.npart = nb;
+  a.c_np1 = -(k.dt * k.post);
+  a.c_new = -(k.dt * k.pre);
+  timing_begin(c, TPX_K_PROJECT);
+  if (copy_np1_to_new) k_project<true><<<nb, kBlock, 0, c->stream>>>(c->g, a);
+  else k_project<false><<<nb, kBlock, 0, c->stream>>>(c->g, a);
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_PROJECT);
+  if (copy_np1_to_new && want_sums) {
+    timing_begin(c, TPX_K_OTHER);
+    for (int d = 0; d < 3; ++d)
+      k_finish<0><<<1, 1024, 0, c->stream>>>(c->partials + (long)d * nb, nb, c->scal, SC_MEAN_RU0 + d);
+    TPX_LAUNCH_CHECK();
+    timing_end(c, TPX_K_OTHER);
+  }
+}
+
+void launch_shift_mean(tpx_ctx *c, const double *const in[3], double *const out[3],
+                       const double a[3]) {
+  timing_begin(c, TPX_K_OTHER);
+  for (int d = 0; d < 3; ++d)
+    k_shift_mean<<<cells_grid(c), kBlock, 0, c->stream>>>(c->g.n, in[d], out[d], c->scal,
+                                                         SC_MEAN_RU0 + d, (double)c->g.n, a[d]);
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_OTHER);
+}
+
+void launch_reduce(tpx_ctx *c, const double *f, int op, int slot) {
+  unsigned nb = cells_grid(c);
+  timing_begin(c, TPX_K_OTHER);
+  if (op == 0) {
+    k_reduce<0><<<nb, kBlock, 0, c->stream>>>(c->g.n, f, c->partials);
+    k_finish<0><<<1, 1024, 0, c->stream>>>(c->partials, nb, c->scal, slot);
+  } else {
+    k_reduce<1><<<nb, kBlock, 0, c->stream>>>(c->g.n, f, c->partials);
+    k_finish<1><<<1, 1024, 0, c->stream>>>(c->partials, nb, c->scal, slot);
+  }
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_OTHER);
+}
+
+void launch_finish_sum(tpx_ctx *c, int nparts, int part_offset, int slot) {
+  k_finish<0><<<1, 1024, 0, c->stream>>>(c->partials + part_offset, nparts, c->scal, slot);
+  TPX_LAUNCH_CHECK();
+}
+
+void launch_update_p0(tpx_ctx *c, const Coef &k) {
+  timing_begin(c, TPX_K_OTHER);
+  k_update_p0<<<1, 64, 0, c->stream>>>(k, c->scal, (double)c->g.n, k.dt * k.post, k.dt * k.pre);
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_OTHER);
+}
+
+void launch_scalar_copy(tpx_ctx *c, int dst, int src) {
+  k_scalar_copy<<<1, 64, 0, c->stream>>>(c->scal, dst, src);
+  TPX_LAUNCH_CHECK();
+}
+
+void launch_fill(tpx_ctx *c, double *f, double v, long n) {
+  k_fill<<<grid1d(n, kBlock), kBlock, 0, c->stream>>>(f, v, n);
+  TPX_LAUNCH_CHECK();
+}
+
+void launch_copy(tpx_ctx *c, double *dst, const double *src, long n) {
+  k_copy<<<grid1d(n, kBlock), kBlock, 0, c->stream>>>(dst, src, n);
+  TPX_LAUNCH_CHECK();
+}
+
+}  // namespace tpx

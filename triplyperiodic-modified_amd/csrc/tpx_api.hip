@@ -1,0 +1,842 @@
+// tpx_api.hip -- the C ABI (include/tpx.h): device state, buffer rotation
+// and the order of kernel launches for each reference method and for the
+// fused step.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <initializer_list>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tpx_internal.h"
+
+using namespace tpx;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+template <class F>
+int guarded(tpx_ctx *c, F &&f) {
+  if (!c) return TPX_E_ARG;
+  try {
+    return f();
+  } catch (const std::invalid_argument &e) {
+    c->err = e.what();
+    return TPX_E_ARG;
+  } catch (const std::logic_error &e) {
+    c->err = e.what();
+    return TPX_E_STATE;
+  } catch (const std::bad_alloc &e) {
+    c->err = "out of memory";
+    return TPX_E_OOM;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return TPX_E_HIP;
+  }
+}
+
+// coefficient pack for substep rk (grid.cpp:18-25 tables and params)
+Coef coef(const tpx_ctx *c, int rk) {
+  Coef k{};
+  k.dt = c->p.dt;
+  k.pre = c->pre[rk];
+  k.post = c->post[rk];
+  k.a_drag = -c->mp / c->Vcell;
+  k.b_visc = c->p.mu0 / 3.;
+  k.c_visc = c->p.mu0;
+  k.A = c->A;
+  k.rho0 = c->p.rho0;
+  for (int d = 0; d < 3; ++d) k.g[d] = c->gv[d];
+  k.k_over_cp = c->p.k / c->p.cp;
+  k.R = c->R;
+  k.cp = c->p.cp;
+  k.cv = c->p.cv;
+  k.Vcell = c->Vcell;
+  k.rk = rk;
+  k.cooling = c->p.cooling;
+  return k;
+}
+
+// ---- buffer bookkeeping -------------------------------------------------
+int pick_free(std::initializer_list<int> used, int nbuf) {
+  for (int b = 0; b < nbuf; ++b)
+    if (std::find(used.begin(), used.end(), b) == used.end()) return b;
+  throw std::logic_error("no free field buffer");
+}
+
+int ru_np1_read(const tpx_ctx *c) { return c->ru_np1_is_base ? c->ru_base : c->ru_np1; }
+int rho_np1_read(const tpx_ctx *c) { return c->rho_np1_is_base ? c->rho_base : c->rho_np1; }
+
+// make RU_np1 / RU_new writable buffers for substep rk
+void prepare_ru_outputs(tpx_ctx *c, int rk) {
+  if (c->ru_np1_is_base) {
+    c->ru_np1 = pick_free({c->ru_base, c->ru_int, c->ru_new}, 5);
+    c->ru_np1_is_base = false;
+    c->ru_new = -1;
+  }
+  if (rk == 3) c->ru_new = c->ru_np1;
+  else if (c->ru_new < 0 || c->ru_new == c->ru_int || c->ru_new == c->ru_base || c->ru_new == c->ru_np1)
+    c->ru_new = pick_free({c->ru_base, c->ru_int, c->ru_np1}, 5);
+}
+
+void prepare_rho_outputs(tpx_ctx *c, int rk) {
+  if (c->rho_np1_is_base) {
+    c->rho_np1 = pick_free({c->rho_base, c->rho_int, c->rho_new}, 5);
+    c->rho_np1_is_base = false;
+    c->rho_new = -1;
+  }
+  if (rk == 3) c->rho_new = c->rho_np1;
+  else if (c->rho_new < 0 || c->rho_new == c->rho_int || c->rho_new == c->rho_base || c->rho_new == c->rho_np1)
+    c->rho_new = pick_free({c->rho_base, c->rho_int, c->rho_np1}, 5);
+}
+
+void ru_ptrs(tpx_ctx *c, int b, const double *out[3]) {
+  for (int d = 0; d < 3; ++d) out[d] = c->ru[b][d];
+}
+void ru_wptrs(tpx_ctx *c, int b, double *out[3]) {
+  for (int d = 0; d < 3; ++d) out[d] = c->ru[b][d];
+}
+
+double *field_buffer(tpx_ctx *c, int field, int comp) {
+  switch (field) {
+  case TPX_RU: return c->ru[c->ru_base][comp];
+  case TPX_RU_INT: return c->ru[c->ru_int][comp];
+  case TPX_RU_NEW: return c->ru_new >= 0 ? c->ru[c->ru_new][comp] : nullptr;
+  case TPX_RU_NP1: return c->ru[ru_np1_read(c)][comp];
+  case TPX_RHS_RU: return c->S[comp];
+  case TPX_P: return c->P;
+  case TPX_RHS_POIS: return c->rhs;
+  case TPX_C: return c->C;
+  case TPX_RHO: return c->rho[c->rho_base];
+  case TPX_RHO_INT: return c->rho[c->rho_int];
+  case TPX_RHO_NEW: return c->rho_new >= 0 ? c->rho[c->rho_new] : nullptr;
+  case TPX_RHO_NP1: return c->rho[rho_np1_read(c)];
+  case TPX_DIVERGENCE: return c->divergence;
+  case TPX_RHS_PART_TEMP: return c->Q;
+  default: return nullptr;
+  }
+}
+
+bool is_vector(int field) { return field >= TPX_RU && field <= TPX_RHS_RU; }
+
+void sync(tpx_ctx *c) { hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
+
+double read_scalar(tpx_ctx *c, int slot) {
+  hip_check(hipMemcpyAsync(c->h_scal + slot, c->scal + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpyAsync");
+  sync(c);
+  return c->h_scal[slot];
+}
+
+void write_scalar(tpx_ctx *c, int slot, double v) {
+  c->h_scal[slot] = v;
+  hip_check(hipMemcpyAsync(c->scal + slot, c->h_scal + slot, sizeof(double), hipMemcpyHostToDevice, c->stream),
+            "hipMemcpyAsync");
+  sync(c);
+}
+
+bool heat_active(const tpx_ctx *c) { return c->p.nu * TPX_PI * c->p.dp * c->p.k != 0.0; }
+
+// ---- method bodies ------------------------------------------------------
+void do_update_rho(tpx_ctx *c, int rk) {
+  Coef k = coef(c, rk);
+  int np1_in = rho_np1_read(c);
+  prepare_rho_outputs(c, rk);
+  const double *ri[3];
+  ru_ptrs(c, c->ru_int, ri);
+  launch_update_rho(c, k, ri, c->rho[c->rho_base], c->rho[np1_in], c->rho[c->rho_np1], c->rho[c->rho_new]);
+}
+
+void zero_particle_fields(tpx_ctx *c) {
+  for (int d = 0; d < 3; ++d) launch_fill(c, c->S[d], 0.0, c->g.n);
+  launch_fill(c, c->Q, 0.0, c->g.n);
+  launch_fill(c, c->scal + SC_SUM_Q, 0.0, 1);
+  c->q_zero = true;
+}
+
+// Update_Particle (grid.cpp:865-888) as one method: includes the
+// statistics-only concentration deposit at x_int.
+void do_update_particle(tpx_ctx *c, int rk) {
+  Coef k = coef(c, rk);
+  Particles &pt = c->pt;
+  if (pt.n == 0) {
+    zero_particle_fields(c);
+    return;
+  }
+  const double *ri[3];
+  ru_ptrs(c, c->ru_int, ri);
+  launch_concentration(c);
+  launch_particle_advance(c, k, ri, c->rho[c->rho_int]);
+  if (c->p.twc) launch_velocity_deposit(c);
+  else for (int d = 0; d < 3; ++d) launch_fill(c, c->S[d], 0.0, c->g.n);
+  launch_sort_new(c);
+  if (heat_active(c)) {
+    launch_temp_gather(c, TPX_NEW, c->rho[c->rho_new], SC_P0_NEW);
+    launch_temp_deposit(c, TPX_NEW, true);
+    c->q_zero = false;
+  } else {
+    launch_fill(c, pt.rhs_t, 0.0, pt.n);
+    if (!c->q_zero) {
+      launch_fill(c, c->Q, 0.0, c->g.n);
+      launch_fill(c, c->scal + SC_SUM_Q, 0.0, 1);
+      c->q_zero = true;
+    }
+  }
+}
+
+void do_update_ru_wop(tpx_ctx *c, int rk, bool with_rho) {
+  Coef k = coef(c, rk);
+  int np1_in = ru_np1_read(c);
+  int rho_np1_in = rho_np1_read(c);
+  prepare_ru_outputs(c, rk);
+  if (with_rho) prepare_rho_outputs(c, rk);
+  const double *ri[3], *rb[3], *rn[3];
+  double *on[3], *ow[3];
+  ru_ptrs(c, c->ru_int, ri);
+  ru_ptrs(c, c->ru_base, rb);
+  ru_ptrs(c, np1_in, rn);
+  ru_wptrs(c, c->ru_np1, on);
+  ru_wptrs(c, c->ru_new, ow);
+  bool has_s = c->p.twc && c->pt.n > 0;
+  const double *S[3] = {c->S[0], c->S[1], c->S[2]};
+  launch_rhs_ru(c, k, with_rho, ri, c->rho[c->rho_int], rb, rn, has_s ? S : nullptr, on, ow,
+                c->rho[c->rho_base], c->rho[rho_np1_in],
+                with_rho ? c->rho[c->rho_np1] : nullptr, with_rho ? c->rho[c->rho_new] : nullptr);
+}
+
+void do_solve_poisson(tpx_ctx *c, int *iters) {
+  int i = 0;
+  double err = 100;
+  while ((i < c->Iteration) && (err > c->p.epsilon)) {
+    launch_poisson_iteration(c, c->rho[c->rho_new]);
+    err = read_scalar(c, SC_MAX_DP);
+    i++;
+  }
+  c->num_iteration = i;
+  if (iters) *iters = i;
+}
+
+void do_update_ru_wp(tpx_ctx *c, int rk, bool want_sums) {
+  Coef k = coef(c, rk);
+  double *on[3], *ow[3];
+  ru_wptrs(c, c->ru_np1, on);
+  ru_wptrs(c, c->ru_new, ow);
+  launch_project(c, k, c->P, on, ow, rk == 3, want_sums);
+}
+
+void do_time_advance_rk4(tpx_ctx *c) {
+  c->rho_int = c->rho_new;
+  c->ru_int = c->ru_new;
+  c->rho_new = -1;
+  c->ru_new = -1;
+  launch_scalar_copy(c, SC_P0_INT, SC_P0_NEW);
+  Particles &pt = c->pt;
+  std::swap(pt.st_int, pt.st_new);
+  std::swap(pt.perm_int, pt.perm_new);
+  std::swap(pt.start_int, pt.start_new);
+}
+
+void do_make_mean_u0(tpx_ctx *c, const double a[3], bool sums_ready) {
+  int src = ru_np1_read(c);
+  if (!sums_ready)
+    for (int d = 0; d < 3; ++d) launch_reduce(c, c->ru[src][d], 0, SC_MEAN_RU0 + d);
+  int dst = pick_free({c->ru_base, c->ru_int, c->ru_new, src}, 5);
+  const double *in[3];
+  double *out[3];
+  ru_ptrs(c, src, in);
+  ru_wptrs(c, dst, out);
+  launch_shift_mean(c, in, out, a);
+  c->ru_np1 = dst;
+  c->ru_np1_is_base = false;
+}
+
+void do_time_advance(tpx_ctx *c) {
+  c->rho_base = rho_np1_read(c);
+  c->rho_np1_is_base = true;
+  c->ru_base = ru_np1_read(c);
+  c->ru_np1_is_base = true;
+  launch_scalar_copy(c, SC_P0, SC_P0_NP1);
+  c->T_cur += c->p.dt;
+  c->num_timestep++;
+  Particles &pt = c->pt;
+  if (!pt.np1_is_base) std::swap(pt.st_base, pt.st_np1);
+  pt.np1_is_base = true;
+  if (pt.n > 0 && (c->p.twc || heat_active(c))) launch_particle_permute(c);
+}
+
+void do_params_update(tpx_ctx *c) {
+  if (c->T_cur > c->p.threshold) {  // params.cpp:75-86
+    c->Iteration = c->p.iteration2;
+    c->A = c->p.a2;
+    c->gv[0] = c->p.gx2; c->gv[1] = c->p.gy2; c->gv[2] = c->p.gz2;
+    c->I0 = c->p.i02;
+  }
+}
+
+// The fused substep: the reference's method sequence with Update_Rho moved
+// after the particle advance (it reads nothing the particles write) and
+// merged into the momentum kernel, and Compute_Div_U_new merged into
+// Compute_RHS_Pois.  Results are identical to the method sequence.
+void fused_substep(tpx_ctx *c, int rk) {
+  Coef k = coef(c, rk);
+  launch_update_p0(c, k);
+  Particles &pt = c->pt;
+  bool parts = pt.n > 0;
+  if (parts) {
+    const double *ri[3];
+    ru_ptrs(c, c->ru_int, ri);
+    launch_particle_advance(c, k, ri, c->rho[c->rho_int]);
+    if (c->p.twc) launch_velocity_deposit(c);
+  }
+  do_update_ru_wop(c, rk, true);
+  if (parts && (c->p.twc || heat_active(c))) launch_sort_new(c);
+  if (parts && heat_active(c)) {
+    launch_temp_gather(c, TPX_NEW, c->rho[c->rho_new], SC_P0_NEW);
+    launch_temp_deposit(c, TPX_NEW, true);
+    c->q_zero = false;
+  } else if (!c->q_zero) {
+    if (parts) launch_fill(c, pt.rhs_t, 0.0, pt.n);
+    launch_fill(c, c->Q, 0.0, c->g.n);
+    launch_fill(c, c->scal + SC_SUM_Q, 0.0, 1);
+    c->q_zero = true;
+  }
+  const double *rn[3];
+  ru_ptrs(c, c->ru_new, rn);
+  launch_div_rhs(c, k, 0, rn, c->rho[c->rho_new], c->q_zero ? nullptr : c->Q, nullptr, c->rhs);
+  do_solve_poisson(c, nullptr);
+  do_update_ru_wp(c, rk, rk == 3);
+  do_time_advance_rk4(c);
+}
+
+void release(tpx_ctx *c) {
+  if (!c) return;
+  auto f = [](void *p) { if (p) (void)hipFree(p); };
+  for (auto &b : c->ru) for (auto &p : b) f(p);
+  for (auto &p : c->rho) f(p);
+  f(c->P); f(c->rhs); f(c->Q); f(c->C); f(c->divergence);
+  for (auto &p : c->S) f(p);
+  for (auto &p : c->spec) f(p);
+  for (auto &p : c->lam) f(p);
+  for (auto &p : c->tw) f(p);
+  f(c->scal); f(c->partials); f(c->scan_tmp);
+  if (c->h_scal) (void)hipHostFree(c->h_scal);
+  Particles &pt = c->pt;
+  for (auto &q : pt.q) for (auto &p : q) f(p);
+  f(pt.rhs_t); f(pt.tau); f(pt.eps); f(pt.flag); f(pt.contrib); f(pt.key);
+  f(pt.perm_int); f(pt.perm_new); f(pt.start_int); f(pt.start_new); f(pt.cursor);
+  for (auto &p : pt.spare) f(p);
+  f(pt.itmp);
+  for (auto &e : c->tm.pool) (void)hipEventDestroy(e);
+  for (auto &r : c->tm.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+std::string validate(const tpx_params *p) {
+  if (!is_pow2(p->nx) || !is_pow2(p->ny) || !is_pow2(p->nz))
+    return "grid sizes must be powers of two (FFT Poisson solve)";
+  if (p->nx < 8 || p->ny < 8 || p->nz < 8 || p->nx > 512 || p->ny > 512 || p->nz > 512)
+    return "grid sizes must lie in [8, 512]";
+  if ((long)p->nx * p->ny * p->nz >= (1L << 31)) return "grid too large";
+  if (!(p->lx > 0 && p->ly > 0 && p->lz > 0)) return "box lengths must be positive";
+  if (!(p->dt > 0)) return "dt must be positive";
+  if (p->bs < 0) return "bs must be >= 0";
+  if (!(p->mu0 > 0)) return "Mu0 must be positive (it divides Tp)";
+  return "";
+}
+
+}  // namespace
+
+namespace tpx {
+
+void timing_begin(tpx_ctx *c, int cls) {
+  if (!c->tm.on) return;
+  Timing &t = c->tm;
+  hipEvent_t a, b;
+  if (t.pool.size() >= 2) {
+    a = t.pool.back(); t.pool.pop_back();
+    b = t.pool.back(); t.pool.pop_back();
+  } else {
+    hip_check(hipEventCreate(&a), "hipEventCreate");
+    hip_check(hipEventCreate(&b), "hipEventCreate");
+  }
+  hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+  t.recs.push_back({cls, a, b});
+}
+
+void timing_end(tpx_ctx *c, int cls) {
+  if (!c->tm.on) return;
+  Timing &t = c->tm;
+  for (auto it = t.recs.rbegin(); it != t.recs.rend(); ++it)
+    if (it->cls == cls) {
+      hip_check(hipEventRecord(it->b, c->stream), "hipEventRecord");
+      return;
+    }
+}
+
+}  // namespace tpx
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *tpx_create_error(void) { return g_create_error.c_str(); }
+
+const char *tpx_last_error(const tpx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
+  if (!p || !out) { g_create_error = "null argument"; return TPX_E_ARG; }
+  *out = nullptr;
+  std::string why = validate(p);
+  if (!why.empty()) { g_create_error = why; return TPX_E_ARG; }
+  if (np_capacity < 0 || np_capacity > (int64_t)INT32_MAX) {
+    g_create_error = "np_capacity out of range";
+    return TPX_E_ARG;
+  }
+  tpx_ctx *c = new tpx_ctx();
+  try {
+    c->p = *p;
+    hip_check(hipGetDevice(&c->device), "hipGetDevice");
+    hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
+    Geo &g = c->g;
+    g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
+    g.nxy = (long)g.nx * g.ny;
+    g.n = g.nxy * g.nz;
+    // gridsize.cpp:23-25, tensor0.cpp:12-32
+    g.dx = p->lx / p->nx; g.dy = p->ly / p->ny; g.dz = p->lz / p->nz;
+    g.idx = 1. / g.dx; g.idy = 1. / g.dy; g.idz = 1. / g.dz;
+    g.idx2 = 1. / (g.dx * g.dx); g.idy2 = 1. / (g.dy * g.dy); g.idz2 = 1. / (g.dz * g.dz);
+    c->Vcell = g.dx * g.dy * g.dz;
+    // params.cpp:61-72
+    c->R = p->cp - p->cv;
+    c->P0par = p->rho0 * p->t0 * c->R;
+    c->Tp = p->rhop * p->dp * p->dp / (18 * p->mu0);
+    c->mp = p->rhop * TPX_PI * p->dp * p->dp * p->dp / 6.;
+    c->Iteration = p->iteration1;
+    c->A = p->a1;
+    c->gv[0] = p->gx1; c->gv[1] = p->gy1; c->gv[2] = p->gz1;
+    c->I0 = p->i01;
+    c->pre[0] = 0.5; c->pre[1] = 0.5; c->pre[2] = 1.; c->pre[3] = 1. / 6.;
+    c->post[0] = 1. / 6.; c->post[1] = 1. / 3.; c->post[2] = 1. / 3.; c->post[3] = 1. / 6.;
+    auto dalloc = [&](double **ptr, size_t n) {
+      if (hipMalloc(ptr, sizeof(double) * (n ? n : 1)) != hipSuccess) throw std::bad_alloc();
+    };
+    auto ialloc = [&](int32_t **ptr, size_t n) {
+      if (hipMalloc(ptr, sizeof(int32_t) * (n ? n : 1)) != hipSuccess) throw std::bad_alloc();
+    };
+    size_t n = (size_t)g.n;
+    for (auto &b : c->ru) for (auto &f : b) dalloc(&f, n);
+    for (auto &f : c->rho) dalloc(&f, n);
+    dalloc(&c->P, n); dalloc(&c->rhs, n); dalloc(&c->Q, n); dalloc(&c->C, n); dalloc(&c->divergence, n);
+    for (auto &f : c->S) dalloc(&f, n);
+    dalloc(&c->scal, SC_COUNT);
+    c->n_partials = (long)grid1d(g.n, 256) + 1024;
+    dalloc(&c->partials, 3 * (size_t)c->n_partials);
+    ialloc(&c->scan_tmp, (size_t)(g.n / 2048 + 1024));
+    if (hipHostMalloc((void **)&c->h_scal, sizeof(double) * SC_COUNT, hipHostMallocDefault) != hipSuccess)
+      throw std::bad_alloc();
+    memset(c->h_scal, 0, sizeof(double) * SC_COUNT);
+    hip_check(hipMemset(c->scal, 0, sizeof(double) * SC_COUNT), "hipMemset");
+    for (auto &f : c->S) hip_check(hipMemset(f, 0, sizeof(double) * n), "hipMemset");
+    hip_check(hipMemset(c->Q, 0, sizeof(double) * n), "hipMemset");
+    hip_check(hipMemset(c->P, 0, sizeof(double) * n), "hipMemset");
+    int rc = fft_setup(c);
+    if (rc != TPX_OK) throw std::runtime_error("FFT setup failed");
+    // particles
+    Particles &pt = c->pt;
+    pt.cap = np_capacity;
+    size_t cap = (size_t)np_capacity;
+    for (auto &q : pt.q) for (auto &a : q) dalloc(&a, cap);
+    dalloc(&pt.rhs_t, cap); dalloc(&pt.tau, cap); dalloc(&pt.eps, cap);
+    for (auto &a : pt.spare) dalloc(&a, cap);
+    ialloc(&pt.flag, cap); ialloc(&pt.itmp, cap);
+    dalloc(&pt.contrib, 6 * cap);
+    ialloc(&pt.key, cap);
+    ialloc(&pt.perm_int, cap); ialloc(&pt.perm_new, cap);
+    ialloc(&pt.start_int, n + 1); ialloc(&pt.start_new, n + 1); ialloc(&pt.cursor, n + 1);
+    hip_check(hipMemset(pt.start_int, 0, sizeof(int32_t) * (n + 1)), "hipMemset");
+    hip_check(hipMemset(pt.rhs_t, 0, sizeof(double) * (cap ? cap : 1)), "hipMemset");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    // buffer roles: int aliases base until the first substep writes new
+    c->ru_base = 0; c->ru_int = 0; c->ru_new = -1; c->ru_np1 = -1; c->ru_np1_is_base = true;
+    c->rho_base = 0; c->rho_int = 0; c->rho_new = -1; c->rho_np1 = -1; c->rho_np1_is_base = true;
+    c->q_zero = true;
+  } catch (const std::bad_alloc &) {
+    g_create_error = "device allocation failed";
+    release(c);
+    return TPX_E_OOM;
+  } catch (const std::exception &e) {
+    g_create_error = e.what();
+    release(c);
+    return TPX_E_HIP;
+  }
+  *out = c;
+  return TPX_OK;
+}
+
+int tpx_destroy(tpx_ctx *ctx) {
+  if (!ctx) return TPX_E_ARG;
+  (void)hipStreamSynchronize(ctx->stream);
+  release(ctx);
+  return TPX_OK;
+}
+
+int tpx_upload_field(tpx_ctx *ctx, int field, int comp, const double *host, int bs) {
+  return guarded(ctx, [&]() {
+    if (!host || bs < 0) throw std::invalid_argument("bad host array or bs");
+    if (is_vector(field) ? (comp < 0 || comp > 2) : comp != 0) throw std::invalid_argument("bad component");
+    if (!(field == TPX_RU || field == TPX_RHO || field == TPX_P || field == TPX_RHS_PART_TEMP ||
+          field == TPX_RHS_RU))
+      throw std::invalid_argument("only RU, Rho, P, RHS_Part_Temp and RHS_RU can be uploaded");
+    const Geo &g = ctx->g;
+    // uploading the base state resets the aliasing of int/np1 to the base
+    if (field == TPX_RU && ctx->ru_int != ctx->ru_base) throw std::logic_error("upload RU before the first substep only");
+    if (field == TPX_RHO && ctx->rho_int != ctx->rho_base) throw std::logic_error("upload Rho before the first substep only");
+    double *dst = field_buffer(ctx, field, comp);
+    std::vector<double> tmp((size_t)g.n);
+    int GX = g.nx + 2 * bs, GY = g.ny + 2 * bs;
+    for (int k = 0; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i)
+          tmp[((size_t)k * g.ny + j) * g.nx + i] = host[((size_t)(k + bs) * GY + (j + bs)) * GX + (i + bs)];
+    hip_check(hipMemcpyAsync(dst, tmp.data(), sizeof(double) * g.n, hipMemcpyHostToDevice, ctx->stream), "upload");
+    if (field == TPX_RHS_PART_TEMP) {
+      launch_reduce(ctx, ctx->Q, 0, SC_SUM_Q);
+      ctx->q_zero = false;
+    }
+    sync(ctx);
+    return TPX_OK;
+  });
+}
+
+int tpx_download_field(tpx_ctx *ctx, int field, int comp, double *host, int bs) {
+  return guarded(ctx, [&]() {
+    if (!host || bs < 0) throw std::invalid_argument("bad host array or bs");
+    if (is_vector(field) ? (comp < 0 || comp > 2) : comp != 0) throw std::invalid_argument("bad component");
+    const double *src = field_buffer(ctx, field, comp);
+    if (!src) throw std::logic_error("field not available in the current state");
+    const Geo &g = ctx->g;
+    std::vector<double> tmp((size_t)g.n);
+    hip_check(hipMemcpyAsync(tmp.data(), src, sizeof(double) * g.n, hipMemcpyDeviceToHost, ctx->stream), "download");
+    sync(ctx);
+    if (field == TPX_RHS_POIS) {  // make_mean_zero applied where the solve reads it
+      double mean = read_scalar(ctx, SC_SUM_RHS) / (double)(g.n);
+      for (auto &v : tmp) v -= mean;
+    }
+    int GX = g.nx + 2 * bs, GY = g.ny + 2 * bs, GZ = g.nz + 2 * bs;
+    for (int k = 0; k < GZ; ++k)
+      for (int j = 0; j < GY; ++j)
+        for (int i = 0; i < GX; ++i) {
+          int si = ((i - bs) % g.nx + g.nx) % g.nx, sj = ((j - bs) % g.ny + g.ny) % g.ny,
+              sk = ((k - bs) % g.nz + g.nz) % g.nz;
+          host[((size_t)k * GY + j) * GX + i] = tmp[((size_t)sk * g.ny + sj) * g.nx + si];
+        }
+    return TPX_OK;
+  });
+}
+
+int tpx_upload_particles(tpx_ctx *ctx, int64_t n, const double *x, const double *y, const double *z,
+                         const double *u, const double *v, const double *w, const double *T,
+                         const int32_t *flag, const double *tau, const double *eps) {
+  return guarded(ctx, [&]() {
+    Particles &pt = ctx->pt;
+    if (n < 0 || n > pt.cap) throw std::invalid_argument("particle count exceeds the capacity");
+    if (n > 0 && (!x || !y || !z || !u || !v || !w || !T || !flag)) throw std::invalid_argument("null particle array");
+    const double *src[7] = {x, y, z, u, v, w, T};
+    pt.st_base = 0; pt.st_int = 1; pt.st_new = 2; pt.st_np1 = 3;
+    pt.np1_is_base = true;
+    for (int q = 0; q < 7; ++q)
+      if (n) hip_check(hipMemcpyAsync(pt.q[q][pt.st_base], src[q], sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
+    if (n) hip_check(hipMemcpyAsync(pt.flag, flag, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
+    pt.has_tau = tau != nullptr;
+    pt.has_eps = eps != nullptr;
+    if (tau && n) hip_check(hipMemcpyAsync(pt.tau, tau, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
+    if (eps && n) hip_check(hipMemcpyAsync(pt.eps, eps, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
+    pt.n = n;
+    sync(ctx);
+    return TPX_OK;
+  });
+}
+
+int tpx_num_particles(tpx_ctx *ctx, int64_t *n) {
+  return guarded(ctx, [&]() {
+    if (!n) throw std::invalid_argument("null");
+    *n = ctx->pt.n;
+    return TPX_OK;
+  });
+}
+
+int tpx_download_particles(tpx_ctx *ctx, int stage, double *x, double *y, double *z, double *u, double *v,
+                           double *w, double *T, int32_t *flag) {
+  return guarded(ctx, [&]() {
+    Particles &pt = ctx->pt;
+    int s;
+    switch (stage) {
+    case TPX_BASE: s = pt.st_base; break;
+    case TPX_INT: s = pt.st_int; break;
+    case TPX_NEW: s = pt.st_new; break;
+    case TPX_NP1: s = pt.np1_is_base ? pt.st_base : pt.st_np1; break;
+    default: throw std::invalid_argument("bad stage");
+    }
+    double *dst[7] = {x, y, z, u, v, w, T};
+    for (int q = 0; q < 7; ++q)
+      if (dst[q] && pt.n)
+        hip_check(hipMemcpyAsync(dst[q], pt.q[q][s], sizeof(double) * pt.n, hipMemcpyDeviceToHost, ctx->stream), "download");
+    if (flag && pt.n) hip_check(hipMemcpyAsync(flag, pt.flag, sizeof(int32_t) * pt.n, hipMemcpyDeviceToHost, ctx->stream), "download");
+    sync(ctx);
+    return TPX_OK;
+  });
+}
+
+int tpx_download_particle_rhs_t(tpx_ctx *ctx, double *rhs_t) {
+  return guarded(ctx, [&]() {
+    if (!rhs_t) throw std::invalid_argument("null");
+    if (ctx->pt.n)
+      hip_check(hipMemcpyAsync(rhs_t, ctx->pt.rhs_t, sizeof(double) * ctx->pt.n, hipMemcpyDeviceToHost, ctx->stream), "download");
+    sync(ctx);
+    return TPX_OK;
+  });
+}
+
+int tpx_set_restart_numbers(tpx_ctx *ctx, double P0, double T_cur, int num_timestep) {
+  return guarded(ctx, [&]() {
+    write_scalar(ctx, SC_P0, P0);
+    ctx->T_cur = T_cur;
+    ctx->num_timestep = num_timestep;
+    return TPX_OK;
+  });
+}
+
+int tpx_get_scalars(tpx_ctx *ctx, double *o) {
+  return guarded(ctx, [&]() {
+    if (!o) throw std::invalid_argument("null");
+    hip_check(hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, ctx->stream), "download");
+    sync(ctx);
+    const double *h = ctx->h_scal;
+    o[TPX_S_P0] = h[SC_P0]; o[TPX_S_P0_INT] = h[SC_P0_INT]; o[TPX_S_P0_NEW] = h[SC_P0_NEW];
+    o[TPX_S_P0_NP1] = h[SC_P0_NP1]; o[TPX_S_T_CUR] = ctx->T_cur; o[TPX_S_NUM_TIMESTEP] = ctx->num_timestep;
+    o[TPX_S_NUM_ITERATION] = ctx->num_iteration; o[TPX_S_MEAN_ENERGY] = h[SC_MET]; o[TPX_S_A] = ctx->A;
+    o[TPX_S_I0] = ctx->I0; o[TPX_S_ITERATION] = ctx->Iteration;
+    o[TPX_S_GX] = ctx->gv[0]; o[TPX_S_GY] = ctx->gv[1]; o[TPX_S_GZ] = ctx->gv[2];
+    o[TPX_S_DP0_DT] = h[SC_DP0_DT];
+    return TPX_OK;
+  });
+}
+
+// G14: prep tail of grid::Initialize (grid.cpp:460-486)
+int tpx_initialize(tpx_ctx *ctx) {
+  return guarded(ctx, [&]() {
+    tpx_ctx *c = ctx;
+    c->ru_int = c->ru_base; c->ru_new = -1; c->ru_np1 = -1; c->ru_np1_is_base = true;
+    c->rho_int = c->rho_base; c->rho_new = -1; c->rho_np1 = -1; c->rho_np1_is_base = true;
+    launch_scalar_copy(c, SC_P0_INT, SC_P0);
+    launch_scalar_copy(c, SC_P0_NP1, SC_P0);
+    Particles &pt = c->pt;
+    pt.np1_is_base = true;
+    if (pt.n > 0) {
+      // bin the particles by cell and store them in that order (base and int)
+      launch_sort_stage(c, TPX_BASE, true);
+      launch_particle_permute(c);
+    }
+    if (pt.n > 0 && heat_active(c)) {
+      launch_temp_gather(c, TPX_INT, c->rho[c->rho_int], SC_P0_INT);
+      launch_temp_deposit(c, TPX_INT, false);
+      c->q_zero = false;
+    } else {
+      if (pt.n > 0) launch_fill(c, pt.rhs_t, 0.0, pt.n);
+      launch_fill(c, c->Q, 0.0, c->g.n);
+      launch_fill(c, c->scal + SC_SUM_Q, 0.0, 1);
+      c->q_zero = true;
+    }
+    sync(c);
+    return TPX_OK;
+  });
+}
+
+int tpx_params_update(tpx_ctx *ctx) {
+  return guarded(ctx, [&]() { do_params_update(ctx); return TPX_OK; });
+}
+
+#define TPX_RK_CHECK(rk) \
+  if ((rk) < 0 || (rk) > 3) throw std::invalid_argument("RK4_count must be 0..3")
+
+int tpx_update_rho(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() { TPX_RK_CHECK(rk); do_update_rho(ctx, rk); return TPX_OK; });
+}
+
+int tpx_update_p0(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() { TPX_RK_CHECK(rk); launch_update_p0(ctx, coef(ctx, rk)); return TPX_OK; });
+}
+
+int tpx_update_particle(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (ctx->rho_new < 0) throw std::logic_error("Update_Particle needs Rho_new (call Update_Rho first)");
+    do_update_particle(ctx, rk);
+    return TPX_OK;
+  });
+}
+
+int tpx_update_ru_wop(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() { TPX_RK_CHECK(rk); do_update_ru_wop(ctx, rk, false); return TPX_OK; });
+}
+
+int tpx_compute_div_u_new(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (ctx->rho_new < 0) throw std::logic_error("Compute_Div_U_new needs Rho_new");
+    const double *rn[3] = {nullptr, nullptr, nullptr};
+    launch_div_rhs(ctx, coef(ctx, rk), 1, rn, ctx->rho[ctx->rho_new], ctx->q_zero ? nullptr : ctx->Q,
+                   ctx->divergence, nullptr);
+    return TPX_OK;
+  });
+}
+
+int tpx_compute_rhs_pois(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (ctx->rho_new < 0 || ctx->ru_new < 0) throw std::logic_error("Compute_RHS_Pois needs RU_new and Rho_new");
+    const double *rn[3];
+    ru_ptrs(ctx, ctx->ru_new, rn);
+    launch_div_rhs(ctx, coef(ctx, rk), 2, rn, ctx->rho[ctx->rho_new], nullptr, ctx->divergence, ctx->rhs);
+    return TPX_OK;
+  });
+}
+
+int tpx_solve_poisson(tpx_ctx *ctx, int rk, int *iterations) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (ctx->rho_new < 0) throw std::logic_error("Solve_Poisson needs Rho_new");
+    do_solve_poisson(ctx, iterations);
+    return TPX_OK;
+  });
+}
+
+int tpx_update_ru_wp(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (ctx->ru_new < 0 || ctx->ru_np1_is_base) throw std::logic_error("Update_RU_WP needs RU_new/RU_np1");
+    do_update_ru_wp(ctx, rk, false);
+    return TPX_OK;
+  });
+}
+
+int tpx_time_advance_rk4(tpx_ctx *ctx) {
+  return guarded(ctx, [&]() {
+    if (ctx->ru_new < 0 || ctx->rho_new < 0) throw std::logic_error("TimeAdvance_RK4 before a substep");
+    do_time_advance_rk4(ctx);
+    return TPX_OK;
+  });
+}
+
+int tpx_make_mean_u0(tpx_ctx *ctx, const double a[3]) {
+  return guarded(ctx, [&]() {
+    if (!a) throw std::invalid_argument("null");
+    do_make_mean_u0(ctx, a, false);
+    return TPX_OK;
+  });
+}
+
+int tpx_time_advance(tpx_ctx *ctx) {
+  return guarded(ctx, [&]() { do_time_advance(ctx); sync(ctx); return TPX_OK; });
+}
+
+int tpx_step(tpx_ctx *ctx, int nsteps) {
+  return guarded(ctx, [&]() {
+    if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
+    double a[3] = {ctx->p.u0 * ctx->p.rho0, ctx->p.v0 * ctx->p.rho0, ctx->p.w0 * ctx->p.rho0};
+    for (int s = 0; s < nsteps; ++s) {
+      do_params_update(ctx);
+      for (int rk = 0; rk < 4; ++rk) fused_substep(ctx, rk);
+      do_make_mean_u0(ctx, a, true);
+      do_time_advance(ctx);
+    }
+    sync(ctx);
+    return TPX_OK;
+  });
+}
+
+int tpx_field_sum(tpx_ctx *ctx, int field, int comp, double *out) {
+  return guarded(ctx, [&]() {
+    const double *f = field_buffer(ctx, field, comp);
+    if (!f || !out) throw std::invalid_argument("bad field");
+    launch_reduce(ctx, f, 0, SC_RED);
+    *out = read_scalar(ctx, SC_RED);
+    return TPX_OK;
+  });
+}
+
+int tpx_field_max_abs(tpx_ctx *ctx, int field, int comp, double *out) {
+  return guarded(ctx, [&]() {
+    const double *f = field_buffer(ctx, field, comp);
+    if (!f || !out) throw std::invalid_argument("bad field");
+    launch_reduce(ctx, f, 1, SC_RED);
+    *out = read_scalar(ctx, SC_RED);
+    return TPX_OK;
+  });
+}
+
+int tpx_deposit_concentration(tpx_ctx *ctx) {
+  return guarded(ctx, [&]() {
+    if (ctx->pt.n == 0) launch_fill(ctx, ctx->C, 0.0, ctx->g.n);
+    else launch_concentration(ctx);
+    sync(ctx);
+    return TPX_OK;
+  });
+}
+
+int tpx_get_stream(tpx_ctx *ctx, void **stream) {
+  return guarded(ctx, [&]() {
+    if (!stream) throw std::invalid_argument("null");
+    *stream = (void *)ctx->stream;
+    return TPX_OK;
+  });
+}
+
+int tpx_timing_enable(tpx_ctx *ctx, int on) {
+  return guarded(ctx, [&]() {
+    sync(ctx);
+    ctx->tm.on = on != 0;
+    for (int k = 0; k < TPX_K_COUNT; ++k) { ctx->tm.ms[k] = 0; ctx->tm.launches[k] = 0; }
+    for (auto &r : ctx->tm.recs) { ctx->tm.pool.push_back(r.a); ctx->tm.pool.push_back(r.b); }
+    ctx->tm.recs.clear();
+    return TPX_OK;
+  });
+}
+
+int tpx_timing_read(tpx_ctx *ctx, double *ms, int64_t *launches) {
+  return guarded(ctx, [&]() {
+    sync(ctx);
+    Timing &t = ctx->tm;
+    for (auto &r : t.recs) {
+      float e = 0;
+      hip_check(hipEventElapsedTime(&e, r.a, r.b), "hipEventElapsedTime");
+      t.ms[r.cls] += e;
+      t.launches[r.cls] += 1;
+      t.pool.push_back(r.a);
+      t.pool.push_back(r.b);
+    }
+    t.recs.clear();
+    for (int k = 0; k < TPX_K_COUNT; ++k) {
+      if (ms) ms[k] = t.ms[k];
+      if (launches) launches[k] = t.launches[k];
+    }
+    return TPX_OK;
+  });
+}
+
+int tpx_synchronize(tpx_ctx *ctx) {
+  return guarded(ctx, [&]() { sync(ctx); return TPX_OK; });
+}
+
+}  // extern "C"

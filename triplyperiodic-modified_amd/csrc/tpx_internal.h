@@ -1,0 +1,184 @@
+// tpx_internal.h -- device state and kernel launchers of libtpx (gfx950).
+//
+// Device layout: every grid field is a dense nz*ny*nx FP64 array with no
+// ghost cells, x fastest.  Periodic neighbours are formed inside the kernels
+// by index wrap, which replaces the reference's ghost fill after every
+// stencil operation (communicator.cpp:108-216).  Vector fields are three such
+// arrays.  The RK4 stage copies of the reference (grid.cpp:461-479, 595-606,
+// 890-900) are pointer rotations between four buffer sets per field.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/tpx.h"
+
+#define TPX_PI 3.141592653589793  // params.h:5
+
+namespace tpx {
+
+// device scalar slots (ctx->scal)
+enum Scal {
+  SC_P0 = 0, SC_P0_INT, SC_P0_NEW, SC_P0_NP1, SC_DP0_DT, SC_MET,  // Update_P0 state
+  SC_SUM_Q,        // sum of RHS_Part_Temp (interior)
+  SC_SUM_RHS,      // sum of the raw Poisson right-hand side
+  SC_MAX_DP,       // max |dp| of the last Poisson iteration
+  SC_MEAN_RU0, SC_MEAN_RU1, SC_MEAN_RU2,
+  SC_RED,          // generic reduction result
+  SC_SUM_C,
+  SC_COUNT = 32
+};
+
+// everything a grid kernel needs, passed by value
+struct Geo {
+  int nx, ny, nz;
+  long nxy, n;
+  double dx, dy, dz, idx, idy, idz, idx2, idy2, idz2;
+};
+
+struct Coef {             // per-substep constants of the RK4 update
+  double dt, pre, post;
+  double a_drag, b_visc, c_visc;  // -mp/Vcell, Mu0/3., Mu0
+  double A, rho0, g[3];
+  double k_over_cp, R, cp, Vcell, cv;
+  int rk, cooling;
+};
+
+struct Particles {
+  int64_t n = 0, cap = 0;
+  // q[quantity][array] with quantity x,y,z,u,v,w,T; arrays are four stage
+  // buffers whose roles rotate (see st_*)
+  double *q[7][4] = {};
+  int st_base = 0, st_int = 1, st_new = 2, st_np1 = 3;
+  bool np1_is_base = false;        // np1 equals base (start of a step)
+  double *rhs_t = nullptr, *tau = nullptr, *eps = nullptr;
+  int32_t *flag = nullptr;
+  double *contrib = nullptr;      // 6 per particle: velocity deposit shares at x_int
+  int32_t *key = nullptr;         // floor cell of x_new (after the wrap)
+  int32_t *perm_int = nullptr, *perm_new = nullptr;     // sorted position -> particle
+  int32_t *start_int = nullptr, *start_new = nullptr;   // CSR over cells (n_cell+1)
+  int32_t *cursor = nullptr;
+  int32_t *wrap_flag = nullptr;   // set when a particle moved more than one cell
+  bool has_tau = false, has_eps = false;
+  // spare arrays the step-end permutation writes into (rhs_t, tau, eps, flag)
+  double *spare[3] = {};
+  int32_t *itmp = nullptr;
+};
+
+struct Timing {
+  bool on = false;
+  struct Rec { int cls; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double ms[TPX_K_COUNT] = {};
+  int64_t launches[TPX_K_COUNT] = {};
+};
+
+}  // namespace tpx
+
+struct tpx_ctx {
+  tpx_params p;
+  tpx::Geo g;
+  double R, P0par, Tp, mp, Vcell;
+  double pre[4], post[4];
+  double A, gv[3], I0;
+  int Iteration;
+  double T_cur = 0;
+  int num_timestep = 0;
+  int num_iteration = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // grid fields
+  double *ru[5][3] = {};          // buffer sets for RU
+  int ru_base = 0, ru_int = 1, ru_new = 2, ru_np1 = 3;
+  bool ru_np1_is_base = false;
+  double *rho[5] = {};
+  int rho_base = 0, rho_int = 1, rho_new = 2, rho_np1 = 3;
+  bool rho_np1_is_base = false;
+  double *P = nullptr, *rhs = nullptr, *S[3] = {}, *Q = nullptr, *C = nullptr,
+         *divergence = nullptr;
+  bool rhs_has_mean = false;      // rhs holds the raw RHS, SC_SUM_RHS its sum
+  bool q_zero = false;            // Q is known to be identically zero
+  double2 *spec[2] = {};          // half-spectrum ping-pong (nx/2+1)*ny*nz
+  double *lam[3] = {};            // modified-wavenumber tables per axis
+  double2 *tw[3] = {};            // twiddles per axis (forward)
+  double *scal = nullptr;         // device scalars
+  double *partials = nullptr;     // block partial results, 3 x n_partials
+  long n_partials = 0;
+  double *h_scal = nullptr;       // pinned host mirror of scal
+  int32_t *scan_tmp = nullptr;    // scan block sums
+  tpx::Particles pt;
+  tpx::Timing tm;
+  std::string err;
+};
+
+namespace tpx {
+
+
+// ---- launchers (k_grid.hip) ----
+void launch_update_rho(tpx_ctx *c, const Coef &k, const double *const ru_int[3],
+                       const double *rho_base, const double *rho_np1_in, double *rho_np1,
+                       double *rho_new);
+void launch_rhs_ru(tpx_ctx *c, const Coef &k, bool with_rho, const double *const ru_int[3],
+                   const double *rho_int, const double *const ru_base[3],
+                   const double *const ru_np1_in[3], const double *const S[3],
+                   double *const ru_np1[3], double *const ru_new[3],
+                   const double *rho_base, const double *rho_np1_in, double *rho_np1,
+                   double *rho_new);
+// mode 0: fused (write rhs + partial sums); 1: write divergence only;
+// 2: read divergence, write rhs + partial sums
+void launch_div_rhs(tpx_ctx *c, const Coef &k, int mode, const double *const ru_new[3],
+                    const double *rho_new, const double *Q, double *divergence, double *rhs);
+void launch_project(tpx_ctx *c, const Coef &k, const double *P, double *const ru_np1[3],
+                    double *const ru_new[3], bool copy_np1_to_new, bool want_sums);
+void launch_shift_mean(tpx_ctx *c, const double *const in[3], double *const out[3],
+                       const double a[3]);
+void launch_reduce(tpx_ctx *c, const double *f, int op, int slot);  // op 0 sum, 1 max|.|
+void launch_finish_sum(tpx_ctx *c, int nparts, int part_offset, int slot);
+void launch_update_p0(tpx_ctx *c, const Coef &k);
+void launch_fill(tpx_ctx *c, double *f, double v, long n);
+void launch_scalar_copy(tpx_ctx *c, int dst, int src);  // scal[dst] = scal[src]
+void launch_copy(tpx_ctx *c, double *dst, const double *src, long n);
+
+// ---- launchers (k_fft.hip) ----
+int fft_setup(tpx_ctx *c);
+void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
+
+// ---- launchers (k_particle.hip) ----
+void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_int[3],
+                             const double *rho_int);
+void launch_velocity_deposit(tpx_ctx *c);
+void launch_sort_new(tpx_ctx *c);
+void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot);
+void launch_temp_deposit(tpx_ctx *c, int stage, bool use_sorted_new);
+void launch_concentration(tpx_ctx *c);
+void launch_particle_permute(tpx_ctx *c);
+void launch_sort_stage(tpx_ctx *c, int stage, bool into_int);
+void launch_iota(tpx_ctx *c, int32_t *p, int64_t n);
+void launch_exclusive_scan(tpx_ctx *c, int32_t *data, int64_t n);  // in place, n+1 output
+
+// ---- timing helpers (tpx_api.hip) ----
+void timing_begin(tpx_ctx *c, int cls);
+void timing_end(tpx_ctx *c, int cls);
+
+inline unsigned grid1d(long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X.  Renumber
+// them so that each XCD receives one contiguous range of work items, which
+// keeps the neighbouring planes a stencil re-reads in that XCD's L2.
+__device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned nxcd = 8;
+  unsigned per = nb / nxcd;
+  if (per == 0 || b >= per * nxcd) return b;
+  return (b % nxcd) * per + (b / nxcd);
+}
+
+}  // namespace tpx
+
+#define TPX_LAUNCH_CHECK()                                  \
+  do {                                                      \
+    hipError_t e_ = hipGetLastError();                      \
+    if (e_ != hipSuccess) throw std::runtime_error(hipGetErrorString(e_)); \
+  } while (0)

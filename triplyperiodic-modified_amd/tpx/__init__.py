@@ -1,0 +1,290 @@
+"""Python access to libtpx, the MI355X implementation of the RK4 time step.
+
+This is a thin ctypes layer over the C ABI of include/tpx.h (the same entry
+points the C++ host classes call).  It fails loudly when the HIP library is
+missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from .params import STRUCT_FIELDS, Params  # noqa: F401
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libtpx.so")
+
+TPX_OK, TPX_E_ARG, TPX_E_HIP, TPX_E_STATE, TPX_E_UNSUPP, TPX_E_OOM = 0, -1, -2, -3, -4, -5
+
+FIELD = {"RU": 0, "RU_int": 1, "RU_new": 2, "RU_np1": 3, "RHS_RU": 4,
+         "P": 16, "RHS_Pois": 17, "C": 18, "Rho": 19, "Rho_int": 20, "Rho_new": 21,
+         "Rho_np1": 22, "divergence": 23, "RHS_Part_Temp": 24}
+VECTOR = {"RU", "RU_int", "RU_new", "RU_np1", "RHS_RU"}
+STAGE = {"base": 0, "int": 1, "new": 2, "np1": 3}
+SCALARS = ["P0", "P0_int", "P0_new", "P0_np1", "T_cur", "num_timestep",
+           "num_iteration", "mean_energy_transferred", "A", "I0", "Iteration",
+           "gx", "gy", "gz", "dP0_dt"]
+KCLASS = ["grid_rhs", "div_rhs", "fft", "project", "particle", "deposit", "sort", "other"]
+
+# C entry points of include/tpx.h, in declaration order
+EXPORTS = [
+    "tpx_create", "tpx_destroy", "tpx_last_error", "tpx_create_error",
+    "tpx_upload_field", "tpx_download_field", "tpx_upload_particles", "tpx_num_particles",
+    "tpx_download_particles", "tpx_download_particle_rhs_t", "tpx_set_restart_numbers",
+    "tpx_get_scalars", "tpx_initialize", "tpx_params_update", "tpx_update_rho",
+    "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop", "tpx_compute_div_u_new",
+    "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
+    "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_field_sum", "tpx_field_max_abs",
+    "tpx_deposit_concentration", "tpx_get_stream", "tpx_timing_enable", "tpx_timing_read",
+    "tpx_synchronize",
+]
+
+
+class TpxParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int if t == "i" else ctypes.c_double) for n, t in STRUCT_FIELDS]
+
+
+class TpxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libtpx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise TpxError(f"libtpx.so not built: {path} (run __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    DP = ctypes.POINTER(ctypes.c_double)
+    IP = ctypes.POINTER(ctypes.c_int32)
+    I64 = ctypes.c_int64
+    L.tpx_create.argtypes = [ctypes.POINTER(TpxParams), I64, ctypes.POINTER(P)]
+    L.tpx_destroy.argtypes = [P]
+    L.tpx_last_error.argtypes = [P]
+    L.tpx_last_error.restype = ctypes.c_char_p
+    L.tpx_create_error.restype = ctypes.c_char_p
+    L.tpx_upload_field.argtypes = [P, I, I, DP, I]
+    L.tpx_download_field.argtypes = [P, I, I, DP, I]
+    L.tpx_upload_particles.argtypes = [P, I64] + [DP] * 7 + [IP, DP, DP]
+    L.tpx_num_particles.argtypes = [P, ctypes.POINTER(I64)]
+    L.tpx_download_particles.argtypes = [P, I] + [DP] * 7 + [IP]
+    L.tpx_download_particle_rhs_t.argtypes = [P, DP]
+    L.tpx_set_restart_numbers.argtypes = [P, D, D, I]
+    L.tpx_get_scalars.argtypes = [P, DP]
+    for fn in ("tpx_initialize", "tpx_params_update", "tpx_time_advance_rk4", "tpx_time_advance",
+               "tpx_deposit_concentration", "tpx_synchronize"):
+        getattr(L, fn).argtypes = [P]
+    for fn in ("tpx_update_rho", "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop",
+               "tpx_compute_div_u_new", "tpx_compute_rhs_pois", "tpx_update_ru_wp", "tpx_step"):
+        getattr(L, fn).argtypes = [P, I]
+    L.tpx_solve_poisson.argtypes = [P, I, ctypes.POINTER(I)]
+    L.tpx_make_mean_u0.argtypes = [P, DP]
+    L.tpx_field_sum.argtypes = [P, I, I, DP]
+    L.tpx_field_max_abs.argtypes = [P, I, I, DP]
+    L.tpx_get_stream.argtypes = [P, ctypes.POINTER(P)]
+    L.tpx_timing_enable.argtypes = [P, I]
+    L.tpx_timing_read.argtypes = [P, DP, ctypes.POINTER(I64)]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Solver:
+    """One device context: grid fields, particles and the step."""
+
+    def __init__(self, params: Params, np_capacity: int = 0):
+        self.L = load()
+        self.params = params
+        self.st = TpxParams(**params.struct_fields())
+        self.shape = (params.Nz_tot, params.Ny_tot, params.Nx_tot)
+        h = ctypes.c_void_p()
+        rc = self.L.tpx_create(ctypes.byref(self.st), int(np_capacity), ctypes.byref(h))
+        if rc != TPX_OK:
+            raise TpxError(f"tpx_create failed ({rc}): {self.L.tpx_create_error().decode()}")
+        self.h = h
+
+    def _ok(self, rc: int, what: str) -> None:
+        if rc != TPX_OK:
+            msg = self.L.tpx_last_error(self.h).decode()
+            raise TpxError(f"{what} failed ({rc}): {msg}")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.tpx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- fields ----
+    def set(self, name: str, arr: np.ndarray) -> None:
+        fid = FIELD[name]
+        if name in VECTOR:
+            arr = _c(arr).reshape((3,) + self.shape)
+            for c in range(3):
+                a = np.ascontiguousarray(arr[c])
+                self._ok(self.L.tpx_upload_field(self.h, fid, c, _dp(a), 0), f"upload {name}")
+        else:
+            a = _c(arr).reshape(self.shape)
+            self._ok(self.L.tpx_upload_field(self.h, fid, 0, _dp(a), 0), f"upload {name}")
+
+    def get(self, name: str, bs: int = 0) -> np.ndarray:
+        fid = FIELD[name]
+        sh = tuple(n + 2 * bs for n in self.shape)
+        if name in VECTOR:
+            out = np.empty((3,) + sh)
+            for c in range(3):
+                a = np.empty(sh)
+                self._ok(self.L.tpx_download_field(self.h, fid, c, _dp(a), bs), f"download {name}")
+                out[c] = a
+            return out
+        out = np.empty(sh)
+        self._ok(self.L.tpx_download_field(self.h, fid, 0, _dp(out), bs), f"download {name}")
+        return out
+
+    # ---- particles ----
+    def set_particles(self, x, y, z, u, v, w, T, flag, tau=None, eps=None) -> None:
+        arrs = [_c(a) for a in (x, y, z, u, v, w, T)]
+        fl = np.ascontiguousarray(flag, dtype=np.int32)
+        tau = None if tau is None else _c(tau)
+        eps = None if eps is None else _c(eps)
+        rc = self.L.tpx_upload_particles(self.h, len(arrs[0]), *[_dp(a) for a in arrs],
+                                         fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _dp(tau), _dp(eps))
+        self._ok(rc, "upload particles")
+
+    @property
+    def np(self) -> int:
+        n = ctypes.c_int64()
+        self._ok(self.L.tpx_num_particles(self.h, ctypes.byref(n)), "num_particles")
+        return n.value
+
+    def particles(self, stage: str = "base") -> Dict[str, np.ndarray]:
+        n = self.np
+        qs = [np.empty(n) for _ in range(7)]
+        fl = np.empty(n, dtype=np.int32)
+        self._ok(self.L.tpx_download_particles(self.h, STAGE[stage], *[_dp(a) for a in qs],
+                                               fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                 "download particles")
+        out = dict(zip("x y z u v w T".split(), qs))
+        out["flag"] = fl
+        return out
+
+    def rhs_t(self) -> np.ndarray:
+        out = np.empty(self.np)
+        self._ok(self.L.tpx_download_particle_rhs_t(self.h, _dp(out)), "download RHS_T")
+        return out
+
+    # ---- scalars ----
+    def set_restart_numbers(self, P0: float, T_cur: float = 0.0, num_timestep: int = 0) -> None:
+        self._ok(self.L.tpx_set_restart_numbers(self.h, P0, T_cur, num_timestep), "restart numbers")
+
+    def scalars(self) -> Dict[str, float]:
+        buf = np.empty(len(SCALARS))
+        self._ok(self.L.tpx_get_scalars(self.h, _dp(buf)), "get scalars")
+        return dict(zip(SCALARS, buf.tolist()))
+
+    # ---- methods ----
+    def initialize(self): self._ok(self.L.tpx_initialize(self.h), "initialize")
+    def params_update(self): self._ok(self.L.tpx_params_update(self.h), "params_update")
+    def update_rho(self, rk): self._ok(self.L.tpx_update_rho(self.h, rk), "update_rho")
+    def update_p0(self, rk): self._ok(self.L.tpx_update_p0(self.h, rk), "update_p0")
+    def update_particle(self, rk): self._ok(self.L.tpx_update_particle(self.h, rk), "update_particle")
+    def update_ru_wop(self, rk): self._ok(self.L.tpx_update_ru_wop(self.h, rk), "update_ru_wop")
+    def compute_div_u_new(self, rk): self._ok(self.L.tpx_compute_div_u_new(self.h, rk), "compute_div_u_new")
+    def compute_rhs_pois(self, rk): self._ok(self.L.tpx_compute_rhs_pois(self.h, rk), "compute_rhs_pois")
+
+    def solve_poisson(self, rk) -> int:
+        it = ctypes.c_int(0)
+        self._ok(self.L.tpx_solve_poisson(self.h, rk, ctypes.byref(it)), "solve_poisson")
+        return it.value
+
+    def update_ru_wp(self, rk): self._ok(self.L.tpx_update_ru_wp(self.h, rk), "update_ru_wp")
+    def time_advance_rk4(self): self._ok(self.L.tpx_time_advance_rk4(self.h), "time_advance_rk4")
+    def time_advance(self): self._ok(self.L.tpx_time_advance(self.h), "time_advance")
+
+    def make_mean_u0(self, a: Sequence[float]) -> None:
+        arr = _c(a)
+        self._ok(self.L.tpx_make_mean_u0(self.h, _dp(arr)), "make_mean_u0")
+
+    def step(self, n: int = 1) -> None:
+        self._ok(self.L.tpx_step(self.h, int(n)), "step")
+
+    def deposit_concentration(self) -> None:
+        self._ok(self.L.tpx_deposit_concentration(self.h), "deposit_concentration")
+
+    def field_sum(self, name: str, comp: int = 0) -> float:
+        out = ctypes.c_double()
+        self._ok(self.L.tpx_field_sum(self.h, FIELD[name], comp, ctypes.byref(out)), "field_sum")
+        return out.value
+
+    def field_max_abs(self, name: str, comp: int = 0) -> float:
+        out = ctypes.c_double()
+        self._ok(self.L.tpx_field_max_abs(self.h, FIELD[name], comp, ctypes.byref(out)), "field_max_abs")
+        return out.value
+
+    # ---- timing ----
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        self._ok(self.L.tpx_get_stream(self.h, ctypes.byref(s)), "get_stream")
+        return s.value or 0
+
+    def timing(self, on: bool) -> None:
+        self._ok(self.L.tpx_timing_enable(self.h, 1 if on else 0), "timing_enable")
+
+    def timing_read(self):
+        ms = np.zeros(len(KCLASS))
+        cnt = np.zeros(len(KCLASS), dtype=np.int64)
+        self._ok(self.L.tpx_timing_read(self.h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+                 "timing_read")
+        return dict(zip(KCLASS, ms.tolist())), dict(zip(KCLASS, cnt.tolist()))
+
+    def synchronize(self) -> None:
+        self._ok(self.L.tpx_synchronize(self.h), "synchronize")
+
+    # ---- convenience ----
+    def load_case(self, case) -> None:
+        """Upload a synthetic case (tpx.ic.Case) and run Initialize's tail."""
+        self.set("RU", case.RU)
+        self.set("Rho", case.Rho)
+        self.set("P", case.P)
+        pp = case.particles
+        self.set_particles(pp["x"], pp["y"], pp["z"], pp["u"], pp["v"], pp["w"], pp["T"], pp["flag"],
+                           tau=case.tau, eps=case.eps)
+        self.set_restart_numbers(case.params.P0, 0.0, 0)
+        self.initialize()
+
+    SUBSTEP_METHODS = ("update_rho", "update_p0", "update_particle", "update_ru_wop",
+                       "compute_div_u_new", "compute_rhs_pois", "solve_poisson",
+                       "update_ru_wp", "time_advance_rk4")
+
+    def method_step(self) -> None:
+        """One step driven method by method, in simulation.cpp:57-91's order."""
+        p = self.params
+        self.params_update()
+        for rk in range(4):
+            for m in self.SUBSTEP_METHODS:
+                fn = getattr(self, m)
+                if m == "time_advance_rk4":
+                    fn()
+                else:
+                    fn(rk)
+        self.make_mean_u0([p.U0 * p.Rho0, p.V0 * p.Rho0, p.W0 * p.Rho0])
+        self.time_advance()
