@@ -242,9 +242,10 @@ class Oracle:
 
     # ---- operator primitives ----
     def apply(self, op: str, a0=None, a1=None, a2=None):
-        ins = [None if a is None else _c(a).reshape(self.shape) for a in (a0, a1, a2)]
         if op == "fold":
-            ins[0] = _c(a0)
+            ins = [_c(a0).ravel(), None, None]
+        else:
+            ins = [None if a is None else _c(a).reshape(self.shape) for a in (a0, a1, a2)]
         outs = [np.zeros(self.shape) for _ in range(3)]
         rc = self.L.or_apply(self.h, OPS[op], *[_dp(a) for a in ins], *[_dp(o) for o in outs])
         assert rc == 0

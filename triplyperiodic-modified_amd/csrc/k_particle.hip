@@ -478,6 +478,7 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
   k_advance<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(a);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_PARTICLE);
+  pt.np1_is_base = false;  // the np1 arrays now hold x_np1 = x + dt*post*u_int
 }
 
 void launch_velocity_deposit(tpx_ctx *c) {
