@@ -1,0 +1,191 @@
+"""Benchmark of the RK4 time step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 256^3 FP64
+staggered grid, 1.6e7 two-way-coupled particles with Stokes number 1 and mass
+loading 1.0, low-Mach heating (Nusselt exchange, absorption efficiency drawn
+uniformly in [0.5, 1]), gamma 1.4, Pr 0.7, seeded spectral initial condition
+(k0 = 4, u' = 1), dt from CFL 0.3.  A step is one full RK4 step: 4 substeps,
+each with the particle path, the momentum/density update and the FFT Poisson
+projection, followed by make_mean_U0 and TimeAdvance.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Prints one JSON line.  `value` is steps/s with all state resident in HBM.
+`roofline` reports the dominant kernel (the density + momentum right-hand
+side) with its algorithmic bytes per launch over its event-timed launch
+duration; `cpu_baseline` times the oracle (the CPU restatement of the
+reference step) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "triplyperiodic-modified_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "RK4 steps/s at 256^3+16M particles"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def grid_rhs_bytes_per_launch(ncell, rk, has_s=True):
+    """Algorithmic bytes of one density + momentum RHS launch.  Per cell it
+    reads RU_int(3), Rho_int, RU(3), RU_np1(3), Rho, Rho_np1, S(3) and writes
+    RU_new(3), RU_np1(3), Rho_new, Rho_np1 (23 FP64 values); at the first
+    substep RU_np1/Rho_np1 are the base state (read once), at the last one
+    RU_new/Rho_new are RU_np1/Rho_np1 and RU/Rho are not read."""
+    s = 3 if has_s else 0
+    if rk == 0:
+        n = 3 + 1 + 3 + 1 + s + 8
+    elif rk == 3:
+        n = 3 + 1 + 3 + 1 + s + 4
+    else:
+        n = 3 + 1 + 3 + 3 + 1 + 1 + s + 8
+    return 8.0 * ncell * n
+
+
+def step_bytes(ncell, npart, n_it):
+    """Algorithmic bytes of one RK4 step: 4 substeps of 46 grid field passes,
+    26 per Poisson iteration plus 4 per extra iteration, and 65 FP64 per
+    particle (state reads and writes, face/corner gathers and deposits)."""
+    per_sub = 8.0 * ncell * (46 + 26 * n_it + 4 * (n_it - 1)) + 8.0 * npart * 65
+    return 4 * per_sub
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
+
+
+def cpu_baseline(case, threads):
+    """Time the oracle on a bounded sample: the first RK4 substep of the same
+    workload (initialisation excluded), scaled to steps/s."""
+    from oracle.oracle import Oracle, threads as set_threads
+    from tpx import ic  # noqa: F401
+    n_used = set_threads(threads)
+    p = case.params
+    o = Oracle(p.struct_fields(), np_capacity=len(case.particles["x"]))
+    o.set("RU", case.RU)
+    o.set("Rho", case.Rho)
+    o.set("P", case.P)
+    pp = case.particles
+    o.set_particles(pp["x"], pp["y"], pp["z"], pp["u"], pp["v"], pp["w"], pp["T"], pp["flag"],
+                    tau=case.tau, eps=case.eps)
+    o.set_restart_numbers(p.P0, 0.0, 0)
+    o.initialize()
+    o.params_update()
+    o.set_rk4_count(0)
+    t0 = time.perf_counter()
+    for m in o.SUBSTEP_METHODS:
+        getattr(o, m)()
+    dt = time.perf_counter() - t0
+    n_it = int(o.scalars()["num_iteration"])
+    o.close()
+    return {"value": 1.0 / (4.0 * dt), "unit": "steps/s", "cores": n_used, "kind": "port",
+            "sample": f"oracle (C restatement of the reference step, OpenMP over {n_used} host threads) "
+                      f"timed over the first RK4 substep of the same 256^3 + 1.6e7-particle workload "
+                      f"({dt:.2f} s, {n_it} Poisson iterations), value = 1/(4 x substep time)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="heat256", choices=["heat256", "plain128", "plain32"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--timing-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the step is measured at N=1 only; "
+                                   "multi-GPU decomposition is a design (see the sharding notes), "
+                                   f"not a runnable mode (asked for --gpus {args.gpus})"}))
+        sys.exit(2)
+
+    import tpx
+    from tpx import ic
+
+    t0 = time.perf_counter()
+    case = ic.config_case(args.workload, seed=0)
+    p = case.params
+    npart = len(case.particles["x"])
+    ncell = p.Nx_tot * p.Ny_tot * p.Nz_tot
+    log(f"[bench] case {args.workload}: {p.Nx_tot}^3, {npart} particles, dt={p.dt:.5g} "
+        f"(built in {time.perf_counter() - t0:.1f} s)")
+    s = tpx.Solver(p, np_capacity=npart)
+    s.load_case(case)
+    log("[bench] uploaded and initialised")
+    s.step(args.warmup)
+    s.synchronize()
+    log(f"[bench] warmup {args.warmup} steps done")
+
+    # timed region: K whole steps, device synchronised on both sides
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = args.steps / elapsed
+    n_it_last = int(s.scalars()["num_iteration"])
+    log(f"[bench] timed {args.steps} steps in {elapsed:.3f} s -> {value:.3f} steps/s")
+
+    # per-kernel-class device times with HIP events on the library's stream
+    s.timing(True)
+    s.step(args.timing_steps)
+    ms, launches = s.timing_read()
+    s.timing(False)
+    per_step = {k: ms[k] / args.timing_steps for k in ms}
+    log("[bench] device ms per step by kernel class: " +
+        ", ".join(f"{k}={v:.3f}" for k, v in per_step.items()))
+    avg_launch_ms = ms["grid_rhs"] / max(launches["grid_rhs"], 1)
+    bytes_launch = sum(grid_rhs_bytes_per_launch(ncell, rk, p.TWC and npart > 0) for rk in range(4)) / 4.0
+    achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic = load_traffic("k_rhs_ru")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
+                "kernel": "k_rhs_ru (density + momentum right-hand side, fused RK4 update)",
+                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_launch_ms, 4)}
+
+    # whole-step algorithmic bandwidth with the iteration count of the run
+    its = s.scalars()["num_iteration"]
+    whole = step_bytes(ncell, npart, max(int(its), 1)) * value / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle) ...")
+        cpu = cpu_baseline(case, args.cpu_threads)
+        log(f"[bench] CPU baseline {cpu['value']:.4f} steps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000.0 / value, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {p.Nx_tot}^3 FP64 staggered grid, {npart} particles, "
+                               f"St=1, mass loading {case.info.get('phi')}, heat "
+                               f"{'on' if p.Nu > 0 else 'off'}, FFT Poisson (epsilon {p.epsilon:g})",
+                   "grid": p.Nx_tot, "particles": npart, "dt": p.dt,
+                   "poisson_iterations_last_substep": n_it_last},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
+        "step_algorithmic_GBps": round(whole, 1),
+    }
+    if cpu:
+        out["speedup_vs_cpu_baseline"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

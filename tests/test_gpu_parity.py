@@ -175,3 +175,19 @@ def test_mass_and_divergence_at_scale(gpu_available):
     assert np.array_equal(np.sort(p["flag"]), np.arange(200_000))
     L = case.params.Lx
     assert np.all((p["x"] >= 0) & (p["x"] <= L)) and np.all((p["z"] >= 0) & (p["z"] <= L))
+
+
+def test_fast_particles_take_the_general_binning_path(gpu_available):
+    """Particles launched at several cells per step leave the 27 cells around
+    their base cell; binning must fall back to the general sort and stay
+    exact."""
+    case = small_case(n=16, npart=2000, heat=True, seed=17)
+    dt, dx = case.params.dt, case.params.Lx / 16
+    rng = np.random.default_rng(3)
+    for q in "uvw":
+        case.particles[q] = rng.choice([-1.0, 1.0], size=2000) * 3.0 * dx / dt
+    o, s = oracle_from_case(case), solver_from_case(case)
+    o.step(2)
+    s.step(2)
+    e = state_errors(o, s, case)
+    assert max(e.values()) <= 1e-9, e

@@ -26,7 +26,6 @@ namespace tpx {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kTile = 8;  // kx columns per block in the y/z passes
 
 struct Ix {
   int mx, my, mz, nx;
@@ -37,31 +36,99 @@ struct Ix {
 };
 
 __device__ __forceinline__ int ilog2(int n) { return 31 - __clz(n); }
-__device__ __forceinline__ int bitrev(int v, int lg) { return (int)(__brev((unsigned)v) >> (32 - lg)); }
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// In-place radix-2 DIT transforms of `lines` lines of length n held in LDS
-// (line l at buf + l*n, already in bit-reversed order).  tw[m] = e^{-2 pi i m/n}
-// for m < n/2; INV uses the conjugates.
-template <bool INV>
+// Stockham autosort FFT of `lines` lines of length n held in LDS, natural
+// order in and out: one radix-2 pass when log2(n) is odd, then radix-4
+// passes.  Line-major layout (ILV false): element j of line l at l*n + j;
+// interleaved layout (ILV true): at j*lines + l, with consecutive threads on
+// consecutive lines, so both layouts are read and written without LDS bank
+// conflicts.  Each pass reads its butterfly inputs into registers,
+// waits, and writes the outputs in place.  tw[t] = e^{-2 pi i t/n}, t < n;
+// INV uses the conjugates.  At most kMaxBf butterflies per thread per pass.
+constexpr int kMaxBf = 4;
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+template <bool INV, bool ILV>
 __device__ void lds_fft(double2 *buf, int n, int lines, const double2 *__restrict__ tw) {
-  int half_n = n >> 1;
-  int total = half_n * lines;
-  for (int len = 2; len <= n; len <<= 1) {
-    int half = len >> 1, step = n / len;
-    for (int b = threadIdx.x; b < total; b += blockDim.x) {
-      int l = b / half_n, bf = b - l * half_n;
-      int grp = bf / half, pos = bf - grp * half;
-      int i0 = grp * len + pos, i1 = i0 + half;
-      double2 w = tw[pos * step];
-      if (INV) w.y = -w.y;
-      double2 *a = buf + (long)l * n;
-      double2 u = a[i0], v = cmul(a[i1], w);
-      a[i0] = make_double2(u.x + v.x, u.y + v.y);
-      a[i1] = make_double2(u.x - v.x, u.y - v.y);
+  // element j of line l
+  auto at = [&](int l, int j) -> double2 & { return ILV ? buf[j * lines + l] : buf[l * n + j]; };
+  int lg = ilog2(n);
+  int Ns = 1;
+  if (lg & 1) {  // radix-2 pass with Ns = 1 (no twiddles)
+    int per = n >> 1, total = per * lines;
+    double2 y0[kMaxBf], y1[kMaxBf];
+    int dst[kMaxBf];
+#pragma unroll
+    for (int q = 0; q < kMaxBf; ++q) {
+      int b = threadIdx.x + q * blockDim.x;
+      if (b < total) {
+        int l = ILV ? b % lines : b / per, j = ILV ? b / lines : b - (b / per) * per;
+        double2 a0 = at(l, j), a1 = at(l, j + per);
+        y0[q] = cadd(a0, a1);
+        y1[q] = csub(a0, a1);
+        dst[q] = l | ((2 * j) << 16);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxBf; ++q) {
+      int b = threadIdx.x + q * blockDim.x;
+      if (b < total) {
+        int l = dst[q] & 0xffff, d = dst[q] >> 16;
+        at(l, d) = y0[q];
+        at(l, d + 1) = y1[q];
+      }
+    }
+    __syncthreads();
+    Ns = 2;
+  }
+  int per = n >> 2, total = per * lines;
+  for (; Ns < n; Ns <<= 2) {
+    double2 y[kMaxBf][4];
+    int dst[kMaxBf];
+    int tstep = n / (4 * Ns);
+#pragma unroll
+    for (int q = 0; q < kMaxBf; ++q) {
+      int b = threadIdx.x + q * blockDim.x;
+      if (b < total) {
+        int l = ILV ? b % lines : b / per, j = ILV ? b / lines : b - (b / per) * per;
+        int k = j & (Ns - 1);
+        double2 a0 = at(l, j), a1 = at(l, j + per), a2 = at(l, j + 2 * per), a3 = at(l, j + 3 * per);
+        if (Ns > 1) {
+          double2 w1 = tw[k * tstep], w2 = tw[2 * k * tstep], w3 = tw[3 * k * tstep];
+          if (INV) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+          a1 = cmul(a1, w1); a2 = cmul(a2, w2); a3 = cmul(a3, w3);
+        }
+        double2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = csub(a1, a3);
+        y[q][0] = cadd(t0, t2);
+        y[q][2] = csub(t0, t2);
+        if (!INV) {  // multiply t3 by -i
+          y[q][1] = make_double2(t1.x + t3.y, t1.y - t3.x);
+          y[q][3] = make_double2(t1.x - t3.y, t1.y + t3.x);
+        } else {     // by +i
+          y[q][1] = make_double2(t1.x - t3.y, t1.y + t3.x);
+          y[q][3] = make_double2(t1.x + t3.y, t1.y - t3.x);
+        }
+        dst[q] = l | (((j - k) * 4 + k) << 16);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxBf; ++q) {
+      int b = threadIdx.x + q * blockDim.x;
+      if (b < total) {
+        int l = dst[q] & 0xffff, d = dst[q] >> 16;
+        at(l, d) = y[q][0];
+        at(l, d + Ns) = y[q][1];
+        at(l, d + 2 * Ns) = y[q][2];
+        at(l, d + 3 * Ns) = y[q][3];
+      }
     }
     __syncthreads();
   }
@@ -105,27 +172,29 @@ __device__ __forceinline__ double residual(const XArgs &a, const Ix &ix, int i, 
   return rn * cc;
 }
 
-// X forward pass: one x line per group of threads
-__global__ __launch_bounds__(kThreads) void k_xfwd(XArgs a, int tpl, int lpb) {
+// X forward pass: lpb consecutive x lines per block
+__global__ __launch_bounds__(kThreads) void k_xfwd(XArgs a, int lpb) {
   extern __shared__ double2 lds[];
   const Geo &g = a.g;
-  int nx = g.nx, lg = ilog2(nx), mxh = nx / 2 + 1;
+  int nx = g.nx, mxh = nx / 2 + 1;
   long nlines = (long)g.ny * g.nz;
   Ix ix{g.nx - 1, g.ny - 1, g.nz - 1, g.nx, g.nxy};
   double mean = a.scal[SC_SUM_RHS] / a.ncell;
-  int lt = threadIdx.x / tpl, t = threadIdx.x - lt * tpl;
   unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  long line = (long)blk * lpb + lt;
-  double2 *buf = lds + (long)lt * nx;
-  if (line < nlines) {
-    int j = (int)(line % g.ny), k = (int)(line / g.ny);
-    for (int i = t; i < nx; i += tpl) buf[bitrev(i, lg)] = make_double2(residual(a, ix, i, j, k, mean), 0.0);
+  long line0 = (long)blk * lpb;
+  for (int e = threadIdx.x; e < lpb * nx; e += blockDim.x) {
+    int l = e / nx, i = e - l * nx;
+    long line = line0 + l;
+    double r = 0.0;
+    if (line < nlines) r = residual(a, ix, i, (int)(line % g.ny), (int)(line / g.ny), mean);
+    lds[e] = make_double2(r, 0.0);
   }
   __syncthreads();
-  lds_fft<false>(lds, nx, lpb, a.tw);
-  if (line < nlines) {
-    double2 *out = a.spec + line * mxh;
-    for (int m = t; m < mxh; m += tpl) out[m] = buf[m];
+  lds_fft<false, false>(lds, nx, lpb, a.tw);
+  for (int e = threadIdx.x; e < lpb * mxh; e += blockDim.x) {
+    int l = e / mxh, m = e - l * mxh;
+    long line = line0 + l;
+    if (line < nlines) a.spec[line * mxh + m] = lds[l * nx + m];
   }
 }
 
@@ -139,33 +208,32 @@ struct XInvArgs {
 };
 
 // X inverse pass: Hermitian extension, inverse FFT, P += dp, block max|dp|
-__global__ __launch_bounds__(kThreads) void k_xinv(XInvArgs a, int tpl, int lpb) {
+__global__ __launch_bounds__(kThreads) void k_xinv(XInvArgs a, int lpb) {
   extern __shared__ double2 lds[];
   __shared__ double red[kThreads];
   const Geo &g = a.g;
-  int nx = g.nx, lg = ilog2(nx), mxh = nx / 2 + 1;
+  int nx = g.nx, mxh = nx / 2 + 1;
   long nlines = (long)g.ny * g.nz;
-  int lt = threadIdx.x / tpl, t = threadIdx.x - lt * tpl;
   unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-  long line = (long)blk * lpb + lt;
-  double2 *buf = lds + (long)lt * nx;
-  if (line < nlines) {
-    const double2 *in = a.spec + line * mxh;
-    for (int m = t; m < mxh; m += tpl) {
-      double2 v = in[m];
-      if (m == 0 || m == nx / 2) v.y = 0.0;
-      buf[bitrev(m, lg)] = v;
-      if (m > 0 && m < nx / 2) buf[bitrev(nx - m, lg)] = make_double2(v.x, -v.y);
-    }
+  long line0 = (long)blk * lpb;
+  for (int e = threadIdx.x; e < lpb * mxh; e += blockDim.x) {
+    int l = e / mxh, m = e - l * mxh;
+    long line = line0 + l;
+    double2 v = line < nlines ? a.spec[line * mxh + m] : make_double2(0.0, 0.0);
+    if (m == 0 || m == nx / 2) v.y = 0.0;
+    lds[l * nx + m] = v;
+    if (m > 0 && m < nx / 2) lds[l * nx + nx - m] = make_double2(v.x, -v.y);
   }
   __syncthreads();
-  lds_fft<true>(lds, nx, lpb, a.tw);
+  lds_fft<true, false>(lds, nx, lpb, a.tw);
   double mx = 0.0;
-  if (line < nlines) {
-    double *Pl = a.P + line * nx;
-    for (int i = t; i < nx; i += tpl) {
-      double dp = (-1) * buf[i].x / (a.ncell);
-      Pl[i] = Pl[i] + dp;
+  for (int e = threadIdx.x; e < lpb * nx; e += blockDim.x) {
+    int l = e / nx, i = e - l * nx;
+    long line = line0 + l;
+    if (line < nlines) {
+      double dp = (-1) * lds[e].x / (a.ncell);
+      double *Pp = a.P + line * nx + i;
+      *Pp = *Pp + dp;
       mx = fmax(mx, fabs(dp));
     }
   }
@@ -181,25 +249,24 @@ __global__ __launch_bounds__(kThreads) void k_xinv(XInvArgs a, int tpl, int lpb)
 // Y pass (forward or inverse) on a tile of kTile kx columns at fixed z
 template <bool INV>
 __global__ __launch_bounds__(kThreads) void k_ypass(double2 *spec, int mxh, int ny, int nz,
-                                                    const double2 *__restrict__ tw) {
+                                                    const double2 *__restrict__ tw, int kTile) {
   extern __shared__ double2 lds[];
   int ntile = (mxh + kTile - 1) / kTile;
   unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   int tile = blk % ntile, k = blk / ntile;
   int m0 = tile * kTile;
-  int lg = ilog2(ny);
   int cols = min(kTile, mxh - m0);
   double2 *base = spec + (long)k * ny * mxh + m0;
   for (int e = threadIdx.x; e < ny * kTile; e += blockDim.x) {
     int j = e / kTile, col = e - j * kTile;
     double2 v = col < cols ? base[(long)j * mxh + col] : make_double2(0.0, 0.0);
-    lds[col * ny + bitrev(j, lg)] = v;
+    lds[e] = v;  // element j of column col at j*kTile + col
   }
   __syncthreads();
-  lds_fft<INV>(lds, ny, kTile, tw);
+  lds_fft<INV, true>(lds, ny, kTile, tw);
   for (int e = threadIdx.x; e < ny * kTile; e += blockDim.x) {
     int j = e / kTile, col = e - j * kTile;
-    if (col < cols) base[(long)j * mxh + col] = lds[col * ny + j];
+    if (col < cols) base[(long)j * mxh + col] = lds[e];
   }
 }
 
@@ -208,50 +275,38 @@ __global__ __launch_bounds__(kThreads) void k_zpass(double2 *spec, int mxh, int 
                                                     const double2 *__restrict__ tw,
                                                     const double *__restrict__ lamx,
                                                     const double *__restrict__ lamy,
-                                                    const double *__restrict__ lamz) {
+                                                    const double *__restrict__ lamz, int kTile) {
   extern __shared__ double2 lds[];
   int ntile = (mxh + kTile - 1) / kTile;
   unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   int tile = blk % ntile, j = blk / ntile;
   int m0 = tile * kTile;
-  int lg = ilog2(nz);
   int cols = min(kTile, mxh - m0);
   long stride = (long)ny * mxh;
   double2 *base = spec + (long)j * mxh + m0;
   for (int e = threadIdx.x; e < nz * kTile; e += blockDim.x) {
     int k = e / kTile, col = e - k * kTile;
     double2 v = col < cols ? base[(long)k * stride + col] : make_double2(0.0, 0.0);
-    lds[col * nz + bitrev(k, lg)] = v;
+    lds[e] = v;  // element k of column col at k*kTile + col
   }
   __syncthreads();
-  lds_fft<false>(lds, nz, kTile, tw);
+  lds_fft<false, true>(lds, nz, kTile, tw);
   for (int e = threadIdx.x; e < nz * kTile; e += blockDim.x) {
-    int col = e / nz, k = e - col * nz;
+    int k = e / kTile, col = e - k * kTile;
     int m = m0 + col;
     if (col < cols && (m + j + k > 0)) {
       double lam = lamx[m] + lamy[j] + lamz[k];
-      double2 v = lds[col * nz + k];
+      double2 v = lds[e];
       v.y /= lam;
       v.x /= lam;
-      lds[col * nz + k] = v;
+      lds[e] = v;
     }
   }
   __syncthreads();
-  // bit-reverse each line again for the inverse DIT transform
-  for (int e = threadIdx.x; e < nz * kTile; e += blockDim.x) {
-    int col = e / nz, k = e - col * nz;
-    int r = bitrev(k, lg);
-    if (r > k) {
-      double2 t0 = lds[col * nz + k];
-      lds[col * nz + k] = lds[col * nz + r];
-      lds[col * nz + r] = t0;
-    }
-  }
-  __syncthreads();
-  lds_fft<true>(lds, nz, kTile, tw);
+  lds_fft<true, true>(lds, nz, kTile, tw);
   for (int e = threadIdx.x; e < nz * kTile; e += blockDim.x) {
     int k = e / kTile, col = e - k * kTile;
-    if (col < cols) base[(long)k * stride + col] = lds[col * nz + k];
+    if (col < cols) base[(long)k * stride + col] = lds[e];
   }
 }
 
@@ -271,10 +326,12 @@ __global__ __launch_bounds__(1024) void k_finish_max(const double *__restrict__ 
   if (threadIdx.x == 0) scal[slot] = sh[0];
 }
 
-void line_shape(int n, int &tpl, int &lpb) {
-  tpl = n / 2 < kThreads ? n / 2 : kThreads;
-  if (tpl < 1) tpl = 1;
-  lpb = kThreads / tpl;
+// lines per block of the x passes and kx columns per block of the y/z
+// passes: 2048 points per block keeps every FFT pass within kMaxBf
+// butterflies per thread and the LDS tile at 32 KB
+int lines_per_block(int n) {
+  int l = 2048 / n;
+  return l > 8 ? 8 : (l < 1 ? 1 : l);
 }
 
 }  // namespace
@@ -285,8 +342,8 @@ int fft_setup(tpx_ctx *c) {
   double L[3] = {c->p.lx, c->p.ly, c->p.lz};
   double d[3] = {g.dx, g.dy, g.dz};
   for (int a = 0; a < 3; ++a) {
-    std::vector<double2> tw(n[a] / 2 > 0 ? n[a] / 2 : 1);
-    for (int m = 0; m < n[a] / 2; ++m) {
+    std::vector<double2> tw(n[a]);
+    for (int m = 0; m < n[a]; ++m) {
       double ang = 2.0 * 3.141592653589793 * (double)m / (double)n[a];
       tw[m] = make_double2(cos(ang), -sin(ang));
     }
@@ -313,25 +370,25 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
   const Geo &g = c->g;
   int mxh = g.nx / 2 + 1;
   long nlines = (long)g.ny * g.nz;
-  int tpl, lpb;
-  line_shape(g.nx, tpl, lpb);
+  int lpb = lines_per_block(g.nx);
   unsigned nbx = (unsigned)((nlines + lpb - 1) / lpb);
   size_t shx = sizeof(double2) * (size_t)g.nx * lpb;
   timing_begin(c, TPX_K_FFT);
   XArgs xa{g, c->P, rho_new, c->rhs, c->scal, c->spec[0], c->tw[0], (double)g.n};
-  k_xfwd<<<nbx, kThreads, shx, c->stream>>>(xa, tpl, lpb);
+  k_xfwd<<<nbx, kThreads, shx, c->stream>>>(xa, lpb);
   TPX_LAUNCH_CHECK();
-  int ntile = (mxh + kTile - 1) / kTile;
-  unsigned nby = (unsigned)(ntile * g.nz), nbz = (unsigned)(ntile * g.ny);
-  k_ypass<false><<<nby, kThreads, sizeof(double2) * kTile * g.ny, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1]);
+  int ty = lines_per_block(g.ny), tz = lines_per_block(g.nz);
+  unsigned nby = (unsigned)(((mxh + ty - 1) / ty) * g.nz), nbz = (unsigned)(((mxh + tz - 1) / tz) * g.ny);
+  size_t shy = sizeof(double2) * (size_t)ty * g.ny, shz = sizeof(double2) * (size_t)tz * g.nz;
+  k_ypass<false><<<nby, kThreads, shy, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], ty);
   TPX_LAUNCH_CHECK();
-  k_zpass<<<nbz, kThreads, sizeof(double2) * kTile * g.nz, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[2],
-                                                                               c->lam[0], c->lam[1], c->lam[2]);
+  k_zpass<<<nbz, kThreads, shz, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[2], c->lam[0], c->lam[1],
+                                              c->lam[2], tz);
   TPX_LAUNCH_CHECK();
-  k_ypass<true><<<nby, kThreads, sizeof(double2) * kTile * g.ny, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1]);
+  k_ypass<true><<<nby, kThreads, shy, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], ty);
   TPX_LAUNCH_CHECK();
   XInvArgs xi{g, c->P, c->spec[0], c->tw[0], c->partials, (double)g.n};
-  k_xinv<<<nbx, kThreads, shx, c->stream>>>(xi, tpl, lpb);
+  k_xinv<<<nbx, kThreads, shx, c->stream>>>(xi, lpb);
   TPX_LAUNCH_CHECK();
   k_finish_max<<<1, 1024, 0, c->stream>>>(c->partials, nbx, c->scal, SC_MAX_DP);
   TPX_LAUNCH_CHECK();

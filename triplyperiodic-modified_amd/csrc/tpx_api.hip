@@ -179,9 +179,10 @@ void do_update_particle(tpx_ctx *c, int rk) {
   if (c->p.twc) launch_velocity_deposit(c);
   else for (int d = 0; d < 3; ++d) launch_fill(c, c->S[d], 0.0, c->g.n);
   launch_sort_new(c);
+  c->binned_this_step = true;
   if (heat_active(c)) {
     launch_temp_gather(c, TPX_NEW, c->rho[c->rho_new], SC_P0_NEW);
-    launch_temp_deposit(c, TPX_NEW, true);
+    launch_temp_deposit(c, TPX_NEW);
     c->q_zero = false;
   } else {
     launch_fill(c, pt.rhs_t, 0.0, pt.n);
@@ -242,6 +243,7 @@ void do_time_advance_rk4(tpx_ctx *c) {
   Particles &pt = c->pt;
   std::swap(pt.st_int, pt.st_new);
   std::swap(pt.perm_int, pt.perm_new);
+  std::swap(pt.inv_int, pt.inv_new);
   std::swap(pt.start_int, pt.start_new);
 }
 
@@ -270,7 +272,9 @@ void do_time_advance(tpx_ctx *c) {
   Particles &pt = c->pt;
   if (!pt.np1_is_base) std::swap(pt.st_base, pt.st_np1);
   pt.np1_is_base = true;
-  if (pt.n > 0 && (c->p.twc || heat_active(c))) launch_particle_permute(c);
+  if (pt.n > 0 && c->binned_this_step) launch_particle_permute(c);
+  else pt.base_sorted = false;
+  c->binned_this_step = false;
 }
 
 void do_params_update(tpx_ctx *c) {
@@ -298,10 +302,13 @@ void fused_substep(tpx_ctx *c, int rk) {
     if (c->p.twc) launch_velocity_deposit(c);
   }
   do_update_ru_wop(c, rk, true);
-  if (parts && (c->p.twc || heat_active(c))) launch_sort_new(c);
+  if (parts && (c->p.twc || heat_active(c))) {
+    launch_sort_new(c);
+    c->binned_this_step = true;
+  }
   if (parts && heat_active(c)) {
     launch_temp_gather(c, TPX_NEW, c->rho[c->rho_new], SC_P0_NEW);
-    launch_temp_deposit(c, TPX_NEW, true);
+    launch_temp_deposit(c, TPX_NEW);
     c->q_zero = false;
   } else if (!c->q_zero) {
     if (parts) launch_fill(c, pt.rhs_t, 0.0, pt.n);
@@ -333,6 +340,7 @@ void release(tpx_ctx *c) {
   for (auto &q : pt.q) for (auto &p : q) f(p);
   f(pt.rhs_t); f(pt.tau); f(pt.eps); f(pt.flag); f(pt.contrib); f(pt.key);
   f(pt.perm_int); f(pt.perm_new); f(pt.start_int); f(pt.start_new); f(pt.cursor);
+  f(pt.inv_int); f(pt.inv_new); f(pt.start_base); f(pt.base_key); f(pt.moved); f(pt.rkey); f(pt.rec);
   for (auto &p : pt.spare) f(p);
   f(pt.itmp);
   for (auto &e : c->tm.pool) (void)hipEventDestroy(e);
@@ -440,7 +448,8 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
     dalloc(&c->P, n); dalloc(&c->rhs, n); dalloc(&c->Q, n); dalloc(&c->C, n); dalloc(&c->divergence, n);
     for (auto &f : c->S) dalloc(&f, n);
     dalloc(&c->scal, SC_COUNT);
-    c->n_partials = (long)grid1d(g.n, 256) + 1024;
+    // one partial per 256-cell block or per x-row, whichever is more
+    c->n_partials = std::max((long)grid1d(g.n, 256), (long)g.ny * g.nz) + 1024;
     dalloc(&c->partials, 3 * (size_t)c->n_partials);
     ialloc(&c->scan_tmp, (size_t)(g.n / 2048 + 1024));
     if (hipHostMalloc((void **)&c->h_scal, sizeof(double) * SC_COUNT, hipHostMallocDefault) != hipSuccess)
@@ -463,7 +472,11 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
     dalloc(&pt.contrib, 6 * cap);
     ialloc(&pt.key, cap);
     ialloc(&pt.perm_int, cap); ialloc(&pt.perm_new, cap);
+    ialloc(&pt.inv_int, cap); ialloc(&pt.inv_new, cap);
     ialloc(&pt.start_int, n + 1); ialloc(&pt.start_new, n + 1); ialloc(&pt.cursor, n + 1);
+    ialloc(&pt.start_base, n + 1); ialloc(&pt.base_key, cap); ialloc(&pt.moved, 1);
+    ialloc(&pt.rkey, cap);
+    if (hipMalloc(&pt.rec, sizeof(double4) * (cap ? cap : 1)) != hipSuccess) throw std::bad_alloc();
     hip_check(hipMemset(pt.start_int, 0, sizeof(int32_t) * (n + 1)), "hipMemset");
     hip_check(hipMemset(pt.rhs_t, 0, sizeof(double) * (cap ? cap : 1)), "hipMemset");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -563,6 +576,7 @@ int tpx_upload_particles(tpx_ctx *ctx, int64_t n, const double *x, const double 
     if (tau && n) hip_check(hipMemcpyAsync(pt.tau, tau, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
     if (eps && n) hip_check(hipMemcpyAsync(pt.eps, eps, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
     pt.n = n;
+    pt.base_sorted = false;
     sync(ctx);
     return TPX_OK;
   });
@@ -650,7 +664,7 @@ int tpx_initialize(tpx_ctx *ctx) {
     }
     if (pt.n > 0 && heat_active(c)) {
       launch_temp_gather(c, TPX_INT, c->rho[c->rho_int], SC_P0_INT);
-      launch_temp_deposit(c, TPX_INT, false);
+      launch_temp_deposit(c, TPX_INT);
       c->q_zero = false;
     } else {
       if (pt.n > 0) launch_fill(c, pt.rhs_t, 0.0, pt.n);

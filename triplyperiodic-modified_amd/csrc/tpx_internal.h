@@ -55,12 +55,18 @@ struct Particles {
   bool np1_is_base = false;        // np1 equals base (start of a step)
   double *rhs_t = nullptr, *tau = nullptr, *eps = nullptr;
   int32_t *flag = nullptr;
-  double *contrib = nullptr;      // 6 per particle: velocity deposit shares at x_int
-  int32_t *key = nullptr;         // floor cell of x_new (after the wrap)
+  double *contrib = nullptr;      // 6 per sorted position: velocity deposit shares at x_int
+  int32_t *key = nullptr;         // floor cell of x_new (after the wrap), by particle
   int32_t *perm_int = nullptr, *perm_new = nullptr;     // sorted position -> particle
+  int32_t *inv_int = nullptr, *inv_new = nullptr;       // particle -> sorted position
   int32_t *start_int = nullptr, *start_new = nullptr;   // CSR over cells (n_cell+1)
+  int32_t *start_base = nullptr;  // CSR of the physical order (cells of the step's base positions)
+  int32_t *base_key = nullptr;    // cell of each particle in the physical order
+  bool base_sorted = false;       // physical order is sorted by base cell (start_base valid)
   int32_t *cursor = nullptr;
-  int32_t *wrap_flag = nullptr;   // set when a particle moved more than one cell
+  int32_t *moved = nullptr;       // device flag: some particle left the 27 cells around its base cell
+  double4 *rec = nullptr;         // heat deposit records in sorted order: x_l, y_l, z_l, RHS_T
+  int32_t *rkey = nullptr;        // packed rounded (cloud-in-cell) cell of each record
   bool has_tau = false, has_eps = false;
   // spare arrays the step-end permutation writes into (rhs_t, tau, eps, flag)
   double *spare[3] = {};
@@ -101,6 +107,7 @@ struct tpx_ctx {
          *divergence = nullptr;
   bool rhs_has_mean = false;      // rhs holds the raw RHS, SC_SUM_RHS its sum
   bool q_zero = false;            // Q is known to be identically zero
+  bool binned_this_step = false;  // particles were re-binned during the current step
   double2 *spec[2] = {};          // half-spectrum ping-pong (nx/2+1)*ny*nz
   double *lam[3] = {};            // modified-wavenumber tables per axis
   double2 *tw[3] = {};            // twiddles per axis (forward)
@@ -152,7 +159,7 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
 void launch_velocity_deposit(tpx_ctx *c);
 void launch_sort_new(tpx_ctx *c);
 void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot);
-void launch_temp_deposit(tpx_ctx *c, int stage, bool use_sorted_new);
+void launch_temp_deposit(tpx_ctx *c, int stage);
 void launch_concentration(tpx_ctx *c);
 void launch_particle_permute(tpx_ctx *c);
 void launch_sort_stage(tpx_ctx *c, int stage, bool into_int);
