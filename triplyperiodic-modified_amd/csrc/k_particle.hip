@@ -2,13 +2,14 @@
 //
 // Particles are SoA arrays kept physically ordered by the cell of their base
 // position (re-sorted once per step).  Every substep they are re-binned by
-// the floor cell of x_new.  Because a particle moves less than one cell per
-// substep, the new bins are built without atomics from the 27 base cells
-// around each cell: a count pass, an exclusive scan, and a placement pass
-// that visits the neighbours in increasing physical index, so the order is
-// stable and identical on every run.  A device flag set by the advance
-// kernel switches to a general counting sort (integer atomics, then a
-// per-cell insertion sort) if any particle moved further.
+// the floor cell of x_new with a counting sort: the advance kernel counts
+// each particle into its cell with an integer atomic and keeps the returned
+// slot, an exclusive scan turns the counts into CSR offsets, a placement
+// pass writes every particle to its offset plus slot, and a per-cell
+// insertion sort puts each cell's particles in increasing physical index.
+// Integer counts do not depend on the order the atomics took, and the final
+// sort removes the one thing that does, so the bins are identical on every
+// run.
 //
 // Deposits are pull-form segment reductions, never floating-point atomics:
 //   velocity (particle.cpp:494-520): face i of cell c sums the (1 - x_l)
@@ -17,10 +18,15 @@
 //     particles' sorted positions;
 //   heat (particle.cpp:620-654): the gather kernel runs in sorted order and
 //     writes one record per particle (cloud-in-cell fractions, RHS_T and the
-//     rounded cell); centre c sums the records of the 27 cells around it
-//     whose rounded cell puts c among their 8 targets.
+//     rounded cell).  The rounded cell of a particle in floor cell B is B or
+//     B + 1 along each axis, so its 8 targets lie in the 3x3x3 block around
+//     B.  Each thread sums the 27 weighted moments of its own cell's
+//     particles (every record is read once), the x shifts are added through
+//     LDS, the z shifts in a register window while the block marches along
+//     z, and a last pass adds the y shifts.
 #include <math.h>
 
+#include <algorithm>
 #include <stdexcept>
 
 #include "tpx_internal.h"
@@ -61,8 +67,8 @@ struct AdvArgs {
   double *contrib;           // shares at sorted positions, 6 arrays (null: no two-way coupling)
   long cstride;
   const int32_t *inv_int;    // particle -> sorted position of x_int
-  const int32_t *base_key;   // base cell (null: no move check)
-  int32_t *moved;
+  int32_t *cnt;              // per-cell counts of x_new (null: no binning this substep)
+  int32_t *rank;             // slot of the particle among its cell's count
   int32_t *key;
   long n;
   double c_post, c_pre;     // dt*post, dt*pre
@@ -150,21 +156,18 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a) {
   int ki = (int)(nv[0] * g.idx), kj = (int)(nv[1] * g.idy), kz = (int)(nv[2] * g.idz);
   long nk = ix(ki, kj, kz);
   a.key[p] = (int32_t)nk;
-  if (a.base_key) {  // has the particle left the 27 cells around its base cell?
-    long bk = a.base_key[p];
-    int bi = (int)(bk % g.nx), bj = (int)((bk / g.nx) % g.ny), bz = (int)(bk / g.nxy);
-    int di = (ki - bi) & ix.mx, dj = (kj - bj) & ix.my, dz = (kz - bz) & ix.mz;
-    bool far = (di > 1 && di < ix.mx) || (dj > 1 && dj < ix.my) || (dz > 1 && dz < ix.mz);
-    if (far) a.moved[0] = 1;
-  }
+  if (a.cnt) a.rank[p] = atomicAdd(&a.cnt[nk], 1);
 }
 
-// floor-cell key of one stage of positions
-__global__ void k_keys(Geo g, const double *x, const double *y, const double *z, int32_t *key, long n) {
+// floor-cell key of one stage of positions, counted into its cell
+__global__ void k_keys(Geo g, const double *x, const double *y, const double *z, int32_t *key,
+                       int32_t *cnt, int32_t *rank, long n) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   Ix ix = make_ix(g);
-  key[p] = (int32_t)ix((int)(x[p] * g.idx), (int)(y[p] * g.idy), (int)(z[p] * g.idz));
+  int32_t k = (int32_t)ix((int)(x[p] * g.idx), (int)(y[p] * g.idy), (int)(z[p] * g.idz));
+  key[p] = k;
+  rank[p] = atomicAdd(&cnt[k], 1);
 }
 
 __global__ void k_set_i(int32_t *a, int32_t v, long n) {
@@ -172,39 +175,16 @@ __global__ void k_set_i(int32_t *a, int32_t v, long n) {
   if (p < n) a[p] = v;
 }
 
-// ---- general path (gated on *moved != 0) ----
-__global__ void k_zero_gated(int32_t *a, long n, const int32_t *moved) {
-  if (!moved[0]) return;
+// placement: CSR offset of the cell plus the slot the counting atomic gave
+__global__ void k_place(const int32_t *__restrict__ key, const int32_t *__restrict__ rank,
+                        const int32_t *__restrict__ start, int32_t *perm, long n) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) a[p] = 0;
-}
-
-__global__ void k_count(const int32_t *__restrict__ key, int32_t *cnt, long n, const int32_t *moved) {
-  if (!moved[0]) return;
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) atomicAdd(&cnt[key[p]], 1);
-}
-
-__global__ void k_copy_i(int32_t *__restrict__ d, const int32_t *__restrict__ s, long n) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) d[p] = s[p];
-}
-
-__global__ void k_place(const int32_t *__restrict__ key, int32_t *cursor, int32_t *perm, long n,
-                        const int32_t *moved) {
-  if (!moved[0]) return;
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) {
-    int32_t pos = atomicAdd(&cursor[key[p]], 1);
-    perm[pos] = (int32_t)p;
-  }
+  if (p < n) perm[start[key[p]] + rank[p]] = (int32_t)p;
 }
 
 // per-cell insertion sort of the placed indices: the result is the stable
-// order (by physical index) whatever order the placement atomics took
-__global__ void k_fixup(const int32_t *__restrict__ start, int32_t *perm, int32_t *inv, long ncell,
-                        const int32_t *moved) {
-  if (!moved[0]) return;
+// order (by physical index) whatever order the counting atomics took
+__global__ void k_fixup(const int32_t *__restrict__ start, int32_t *perm, int32_t *inv, long ncell) {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   int32_t b = start[c], e = start[c + 1];
@@ -216,16 +196,6 @@ __global__ void k_fixup(const int32_t *__restrict__ start, int32_t *perm, int32_
   }
   for (int32_t q = b; q < e; ++q) inv[perm[q]] = q;
 }
-
-// ---- incremental path (gated on *moved == 0) ----
-// order of {i-1, i, i+1} (mod n) by increasing index, as offsets
-__device__ __forceinline__ void sorted_offsets(int i, int n, int o[3]) {
-  if (i == 0) { o[0] = 0; o[1] = 1; o[2] = -1; }
-  else if (i == n - 1) { o[0] = 1; o[1] = -1; o[2] = 0; }
-  else { o[0] = -1; o[1] = 0; o[2] = 1; }
-}
-
-
 
 __global__ void k_iota(int32_t *p, long n) {
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -318,85 +288,109 @@ __global__ __launch_bounds__(kBlock) void k_temp_gather(TempArgs a) {
 }
 
 
-// ---- row-streaming pulls --------------------------------------------------
-// One block per x-row (j,k) of target cells.  The block walks the 9 source
-// rows (j+dj, k+dk); for each it stages the row's CSR and then its records,
-// in chunks of kChunk, in LDS with coalesced loads, and every thread scans
-// the source cells i-1, i, i+1 of its own cells from LDS.  Sources are
-// visited in a fixed order (rows, then increasing position), so the sums are
-// the same on every run and for any chunking.
-constexpr int kChunk = 1024;
-constexpr int kMaxNx = 512;
+// ---- heat deposit by per-cell moments (P9/P10) -----------------------------
+// Stage 1: one block covers `rows` whole x-rows and marches over `zc` output
+// planes (plus one halo plane on each side).  For base plane z every thread
+// sums, over the records of its cell (i, j, z) in sorted order, the 27
+// products w_x(a) w_y(b) w_z(c) RHS_T for the target offsets a, b, c in
+// {-1, 0, 1}.  The x offsets are exchanged through LDS (left, own, right), the
+// z offsets accumulate in a three-plane register window, and the result for
+// output cell (i, j + b, z) from this row is stored as t3[b + 1].
+struct HeatArgs {
+  Geo g;
+  const int32_t *start;
+  const double4 *rec;
+  const int32_t *rkey;
+  double *t3;                // 3 x n: contribution of row j to row j - 1, j, j + 1
+  int rows, zc;
+};
 
-__device__ __forceinline__ void row_of_block(const Geo &g, int &j, int &k, unsigned &b) {
-  b = xcd_block(blockIdx.x, gridDim.x);
-  j = (int)(b % g.ny);
-  k = (int)(b / g.ny);
+constexpr int kHeatMaxBlock = 512;
+
+__global__ __launch_bounds__(kHeatMaxBlock) void k_heat_planes(HeatArgs a) {
+  extern __shared__ double xs[];  // 18 x blockDim: offsets -1 (0..8) and +1 (9..17)
+  const Geo &g = a.g;
+  Ix ix = make_ix(g);
+  const int nx = g.nx, bd = (int)blockDim.x, t = (int)threadIdx.x;
+  const int i = t & ix.mx, r = t / nx;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int nyb = g.ny / a.rows;
+  const int j = (int)(b % (unsigned)nyb) * a.rows + r;
+  const int z0 = (int)(b / (unsigned)nyb) * a.zc;
+  const int il = r * nx + ((i - 1) & ix.mx), ir = r * nx + ((i + 1) & ix.mx);
+  double sa[3] = {0., 0., 0.}, sb[3] = {0., 0., 0.};
+  long c = ((long)((z0 - 1) & ix.mz) * g.ny + j) * nx + i;
+  int32_t q0 = a.start[c], q1 = a.start[c + 1];
+  for (int zz = -1; zz <= a.zc; ++zz) {
+    const int k = (z0 + zz) & ix.mz;
+    // prefetch the next plane's segment bounds
+    const long cn = ((long)((k + 1) & ix.mz) * g.ny + j) * nx + i;
+    const int32_t n0 = a.start[cn], n1 = a.start[cn + 1];
+    double m[3][3][3];
+#pragma unroll
+    for (int u = 0; u < 27; ++u) m[u / 9][(u / 3) % 3][u % 3] = 0.0;
+    for (int32_t q = q0; q < q1; ++q) {
+      const int32_t rk = a.rkey[q];
+      const double4 v = a.rec[q];
+      // the rounded cell is B + 1 (upper) or B along each axis
+      const bool ux = (((rk & 1023) - i) & ix.mx) != 0;
+      const bool uy = ((((rk >> 10) & 1023) - j) & ix.my) != 0;
+      const bool uz = ((((rk >> 20) & 1023) - k) & ix.mz) != 0;
+      const double wx[3] = {ux ? 0.0 : 1 - v.x, ux ? 1 - v.x : v.x, ux ? v.x : 0.0};
+      const double wy[3] = {uy ? 0.0 : 1 - v.y, uy ? 1 - v.y : v.y, uy ? v.y : 0.0};
+      const double wz[3] = {uz ? 0.0 : 1 - v.z, uz ? 1 - v.z : v.z, uz ? v.z : 0.0};
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+        for (int ay = 0; ay < 3; ++ay)
+#pragma unroll
+          for (int az = 0; az < 3; ++az) m[ax][ay][az] += wx[ax] * wy[ay] * wz[az] * v.w;
+    }
+    __syncthreads();  // the previous plane's exchange has been read
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+      xs[s * bd + t] = m[0][s / 3][s % 3];
+      xs[(9 + s) * bd + t] = m[2][s / 3][s % 3];
+    }
+    __syncthreads();
+    double R[3][3];
+#pragma unroll
+    for (int s = 0; s < 9; ++s) R[s / 3][s % 3] = (xs[(9 + s) * bd + il] + m[1][s / 3][s % 3]) + xs[s * bd + ir];
+    if (zz >= 1) {
+      const long co = ((long)(z0 + zz - 1) * g.ny + j) * nx + i;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) a.t3[e * g.n + co] = sa[e] + R[e][0];
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      sa[e] = sb[e] + R[e][1];
+      sb[e] = R[e][2];
+    }
+    q0 = n0;
+    q1 = n1;
+  }
 }
 
-// heat deposit (P9/P10): Q(c) = sum of w_c(p) RHS_T(p) over the records whose
-// rounded cell r has c in {r-1, r} along every axis
-__global__ __launch_bounds__(kBlock) void k_heat_rows(Geo g, const int32_t *__restrict__ start,
-                                                      const double4 *__restrict__ rec,
-                                                      const int32_t *__restrict__ rkey, double *out,
-                                                      double *partials) {
-  __shared__ int32_t sst[kMaxNx + 1];
-  __shared__ double4 srec[kChunk];
-  __shared__ int32_t skey[kChunk];
+// Stage 2: Q(i, j, k) = t3[2](i, j-1, k) + t3[1](i, j, k) + t3[0](i, j+1, k),
+// with block partial sums for the mean of Q
+__global__ __launch_bounds__(kBlock) void k_heat_combine(Geo g, const double *__restrict__ t3, double *out,
+                                                         double *partials) {
   __shared__ double red[kBlock];
-  int j, k;
-  unsigned b;
-  row_of_block(g, j, k, b);
-  Ix ix = make_ix(g);
-  const int nx = g.nx;
-  double acc[kMaxNx / kBlock] = {};
-  for (int a = 0; a < 9; ++a) {
-    int dk = a / 3 - 1, dj = a % 3 - 1;
-    long R = ((long)((k + dk) & ix.mz) * g.ny + ((j + dj) & ix.my)) * nx;
-    for (int t = threadIdx.x; t <= nx; t += blockDim.x) sst[t] = start[R + t];
-    __syncthreads();
-    int32_t rb = sst[0], re = sst[nx];
-    for (int32_t c0 = rb; c0 < re; c0 += kChunk) {
-      int cnt = re - c0 < kChunk ? re - c0 : kChunk;
-      for (int t = threadIdx.x; t < cnt; t += blockDim.x) { srec[t] = rec[c0 + t]; skey[t] = rkey[c0 + t]; }
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < kMaxNx / kBlock; ++u) {
-        int i = threadIdx.x + u * kBlock;
-        if (i >= nx) continue;
-        int oi[3];
-        sorted_offsets(i, nx, oi);
-        for (int e = 0; e < 3; ++e) {
-          int f = (i + oi[e]) & ix.mx;
-          int32_t q0 = sst[f] > c0 ? sst[f] : c0, q1 = sst[f + 1] < c0 + cnt ? sst[f + 1] : c0 + cnt;
-          for (int32_t q = q0; q < q1; ++q) {
-            int32_t rk = skey[q - c0];
-            int ox = (i - (rk & 1023) + 1) & ix.mx;  // 1: c is rx, 0: c is rx - 1
-            int oy = (j - ((rk >> 10) & 1023) + 1) & ix.my;
-            int oz = (k - ((rk >> 20) & 1023) + 1) & ix.mz;
-            if (ox > 1 || oy > 1 || oz > 1) continue;
-            double4 r4 = srec[q - c0];
-            double wx = ox ? r4.x : (1 - r4.x), wy = oy ? r4.y : (1 - r4.y), wz = oz ? r4.z : (1 - r4.z);
-            acc[u] += wx * wy * wz * r4.w;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const long c = (long)b * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (c < g.n) {
+    const int my = g.ny - 1;
+    const long row = c / g.nx;
+    const int j = (int)(row % g.ny);
+    const long base = c - (long)j * g.nx;
+    v = (t3[2 * g.n + base + (long)((j - 1) & my) * g.nx] + t3[g.n + c]) + t3[base + (long)((j + 1) & my) * g.nx];
+    out[c] = v;
   }
-  double mine = 0.0;
-#pragma unroll
-  for (int u = 0; u < kMaxNx / kBlock; ++u) {
-    int i = threadIdx.x + u * kBlock;
-    if (i < nx) {
-      out[((long)k * g.ny + j) * nx + i] = acc[u];
-      mine += acc[u];
-    }
-  }
-  red[threadIdx.x] = mine;
+  red[threadIdx.x] = v;
   __syncthreads();
-  for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
-    if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) partials[b] = red[0];
@@ -406,140 +400,35 @@ __global__ __launch_bounds__(kBlock) void k_heat_rows(Geo g, const int32_t *__re
 // (1 - frac_a) RHS_a] + [sum over cell c - e_a of frac_a RHS_a], each part in
 // increasing sorted position.  Component 2a of contrib holds frac_a RHS_a,
 // component 2a+1 holds (1 - frac_a) RHS_a.
-__global__ __launch_bounds__(kBlock) void k_vel_rows(Geo g, const int32_t *__restrict__ start,
-                                                     const double *__restrict__ contrib, long cs,
-                                                     double *Sx, double *Sy, double *Sz) {
-  __shared__ int32_t sst[kMaxNx + 1];
-  __shared__ double sc[4][kChunk];
-  int j, k;
-  unsigned b;
-  row_of_block(g, j, k, b);
+__global__ __launch_bounds__(kBlock) void k_vel_cells(Geo g, const int32_t *__restrict__ start,
+                                                      const double *__restrict__ contrib, long cs,
+                                                      double *Sx, double *Sy, double *Sz) {
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const long c = (long)b * blockDim.x + threadIdx.x;
+  if (c >= g.n) return;
   Ix ix = make_ix(g);
-  const int nx = g.nx;
-  constexpr int U = kMaxNx / kBlock;
-  double own[U][3] = {}, nb[U][3] = {};
-  // own row: components 0 (from cell i-1), 1, 3, 5 (from cell i)
-  // row j-1: component 2 (from cell i); row k-1: component 4 (from cell i)
-  for (int ph = 0; ph < 3; ++ph) {
-    long R = ph == 0 ? ((long)k * g.ny + j) * nx
-           : ph == 1 ? ((long)k * g.ny + ((j - 1) & ix.my)) * nx
-                     : ((long)((k - 1) & ix.mz) * g.ny + j) * nx;
-    for (int t = threadIdx.x; t <= nx; t += blockDim.x) sst[t] = start[R + t];
-    __syncthreads();
-    int32_t rb = sst[0], re = sst[nx];
-    for (int32_t c0 = rb; c0 < re; c0 += kChunk) {
-      int cnt = re - c0 < kChunk ? re - c0 : kChunk;
-      for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-        if (ph == 0) {
-          sc[0][t] = contrib[0 * cs + c0 + t];
-          sc[1][t] = contrib[1 * cs + c0 + t];
-          sc[2][t] = contrib[3 * cs + c0 + t];
-          sc[3][t] = contrib[5 * cs + c0 + t];
-        } else {
-          sc[0][t] = contrib[(ph == 1 ? 2 : 4) * cs + c0 + t];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        int i = threadIdx.x + u * kBlock;
-        if (i >= nx) continue;
-        int32_t q0 = sst[i] > c0 ? sst[i] : c0, q1 = sst[i + 1] < c0 + cnt ? sst[i + 1] : c0 + cnt;
-        if (ph == 0) {
-          for (int32_t q = q0; q < q1; ++q) {
-            own[u][0] += sc[1][q - c0];
-            own[u][1] += sc[2][q - c0];
-            own[u][2] += sc[3][q - c0];
-          }
-          int f = (i - 1) & ix.mx;
-          int32_t r0 = sst[f] > c0 ? sst[f] : c0, r1 = sst[f + 1] < c0 + cnt ? sst[f + 1] : c0 + cnt;
-          for (int32_t q = r0; q < r1; ++q) nb[u][0] += sc[0][q - c0];
-        } else {
-          for (int32_t q = q0; q < q1; ++q) nb[u][ph] += sc[0][q - c0];
-        }
-      }
-      __syncthreads();
-    }
+  const int i = (int)(c & ix.mx);
+  const long row = c / g.nx;
+  const int j = (int)(row % g.ny), k = (int)(row / g.ny);
+  const double *__restrict__ c0 = contrib;
+  const double *__restrict__ c1 = contrib + cs;
+  const double *__restrict__ c2 = contrib + 2 * cs;
+  const double *__restrict__ c3 = contrib + 3 * cs;
+  const double *__restrict__ c4 = contrib + 4 * cs;
+  const double *__restrict__ c5 = contrib + 5 * cs;
+  double o0 = 0., o1 = 0., o2 = 0., n0 = 0., n1 = 0., n2 = 0.;
+  for (int32_t q = start[c], e = start[c + 1]; q < e; ++q) {
+    o0 += c1[q];
+    o1 += c3[q];
+    o2 += c5[q];
   }
-  long rowc = ((long)k * g.ny + j) * nx;
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    int i = threadIdx.x + u * kBlock;
-    if (i < nx) {
-      Sx[rowc + i] = own[u][0] + nb[u][0];
-      Sy[rowc + i] = own[u][1] + nb[u][1];
-      Sz[rowc + i] = own[u][2] + nb[u][2];
-    }
-  }
-}
-
-// incremental binning: count (PLACE = false) or place (PLACE = true) the
-// particles of the 27 base cells around each cell whose new key is that cell.
-// Placement visits the base cells in increasing index and each cell's
-// particles in increasing physical index, so the bins are stable.
-template <bool PLACE>
-__global__ __launch_bounds__(kBlock) void k_bin_rows(Geo g, const int32_t *__restrict__ start_base,
-                                                     const int32_t *__restrict__ key, int32_t *cnt,
-                                                     const int32_t *__restrict__ start_new, int32_t *perm,
-                                                     int32_t *inv, const int32_t *moved) {
-  if (moved[0]) return;
-  __shared__ int32_t sst[kMaxNx + 1];
-  __shared__ int32_t skey[kChunk];
-  int j, k;
-  unsigned b;
-  row_of_block(g, j, k, b);
-  Ix ix = make_ix(g);
-  const int nx = g.nx;
-  int32_t m[kMaxNx / kBlock] = {};
-  int32_t w[kMaxNx / kBlock] = {};
-  long rowc = ((long)k * g.ny + j) * nx;
-  if (PLACE) {
-#pragma unroll
-    for (int u = 0; u < kMaxNx / kBlock; ++u) {
-      int i = threadIdx.x + u * kBlock;
-      if (i < nx) w[u] = start_new[rowc + i];
-    }
-  }
-  int oj[3], ok[3];
-  sorted_offsets(j, g.ny, oj);
-  sorted_offsets(k, g.nz, ok);
-  for (int a = 0; a < 9; ++a) {
-    long R = ((long)((k + ok[a / 3]) & ix.mz) * g.ny + ((j + oj[a % 3]) & ix.my)) * nx;
-    for (int t = threadIdx.x; t <= nx; t += blockDim.x) sst[t] = start_base[R + t];
-    __syncthreads();
-    int32_t rb = sst[0], re = sst[nx];
-    for (int32_t c0 = rb; c0 < re; c0 += kChunk) {
-      int cntc = re - c0 < kChunk ? re - c0 : kChunk;
-      for (int t = threadIdx.x; t < cntc; t += blockDim.x) skey[t] = key[c0 + t];
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < kMaxNx / kBlock; ++u) {
-        int i = threadIdx.x + u * kBlock;
-        if (i >= nx) continue;
-        int32_t me = (int32_t)(rowc + i);
-        int oi[3];
-        sorted_offsets(i, nx, oi);
-        for (int e = 0; e < 3; ++e) {
-          int f = (i + oi[e]) & ix.mx;
-          int32_t q0 = sst[f] > c0 ? sst[f] : c0, q1 = sst[f + 1] < c0 + cntc ? sst[f + 1] : c0 + cntc;
-          for (int32_t q = q0; q < q1; ++q) {
-            if (skey[q - c0] != me) continue;
-            if (PLACE) { perm[w[u]] = q; inv[q] = w[u]; ++w[u]; }
-            else ++m[u];
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  if (!PLACE) {
-#pragma unroll
-    for (int u = 0; u < kMaxNx / kBlock; ++u) {
-      int i = threadIdx.x + u * kBlock;
-      if (i < nx) cnt[rowc + i] = m[u];
-    }
-    if (b == 0 && threadIdx.x == 0) cnt[g.n] = 0;
-  }
+  const long fx = ix(i - 1, j, k), fy = ix(i, j - 1, k), fz = ix(i, j, k - 1);
+  for (int32_t q = start[fx], e = start[fx + 1]; q < e; ++q) n0 += c0[q];
+  for (int32_t q = start[fy], e = start[fy + 1]; q < e; ++q) n1 += c2[q];
+  for (int32_t q = start[fz], e = start[fz + 1]; q < e; ++q) n2 += c4[q];
+  Sx[c] = o0 + n0;
+  Sy[c] = o1 + n1;
+  Sz[c] = o2 + n2;
 }
 
 struct CicDepArgs {
@@ -592,7 +481,6 @@ __global__ __launch_bounds__(kBlock) void k_concentration(CicDepArgs a) {
 
 struct PermArgs {
   const int32_t *perm;
-  const int32_t *key;        // cell of each particle in the order being applied
   const double *src[7];
   double *dst_a[7];
   double *dst_b[7];
@@ -600,7 +488,6 @@ struct PermArgs {
   double *d1[3];
   const int32_t *flag_s;
   int32_t *flag_d;
-  int32_t *base_key;
   long n;
 };
 
@@ -620,7 +507,6 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermArgs a) {
   for (int d = 0; d < 3; ++d)
     if (a.s1[d]) a.d1[d][q] = a.s1[d][s];
   a.flag_d[q] = a.flag_s[s];
-  a.base_key[q] = a.key[s];
 }
 
 }  // namespace
@@ -664,37 +550,24 @@ void launch_exclusive_scan(tpx_ctx *c, int32_t *data, int64_t n) {
 
 // Stable bins of the particles by key (one int per particle): perm (sorted
 // position -> particle), inv (particle -> sorted position), start (CSR).
-// Uses the incremental pass when the physical order is sorted by base cell
-// and no particle left the 27 cells around its base cell; otherwise the
-// general atomic counting sort.  Both paths are enqueued; the device flag
-// `moved` selects which one does the work.
-static void bin_particles(tpx_ctx *c, const int32_t *key, int32_t *perm, int32_t *inv, int32_t *start,
-                          bool incremental_possible) {
+// On entry `start` holds the per-cell counts (start[n_cell] = 0) and
+// pt.rank each particle's slot among its cell's count.
+static void bin_particles(tpx_ctx *c, const int32_t *key, int32_t *perm, int32_t *inv, int32_t *start) {
   Particles &pt = c->pt;
   long n = pt.n, nc = c->g.n;
-  unsigned gp = grid1d(n, kBlock), gc = grid1d(nc, kBlock), gc1 = grid1d(nc + 1, kBlock);
-  if (!incremental_possible) k_set_i<<<1, 64, 0, c->stream>>>(pt.moved, 1, 1);
-  // counts
-  unsigned rows = (unsigned)((long)c->g.ny * c->g.nz);
-  if (incremental_possible)
-    k_bin_rows<false><<<rows, kBlock, 0, c->stream>>>(c->g, pt.start_base, key, pt.cursor, nullptr, nullptr,
-                                                      nullptr, pt.moved);
-  k_zero_gated<<<gc1, kBlock, 0, c->stream>>>(pt.cursor, nc + 1, pt.moved);
-  k_count<<<gp, kBlock, 0, c->stream>>>(key, pt.cursor, n, pt.moved);
+  launch_exclusive_scan(c, start, nc + 1);
+  k_place<<<grid1d(n, kBlock), kBlock, 0, c->stream>>>(key, pt.rank, start, perm, n);
+  k_fixup<<<grid1d(nc, kBlock), kBlock, 0, c->stream>>>(start, perm, inv, nc);
   TPX_LAUNCH_CHECK();
-  launch_exclusive_scan(c, pt.cursor, nc + 1);
-  k_copy_i<<<gc1, kBlock, 0, c->stream>>>(start, pt.cursor, nc + 1);
-  // placement
-  if (incremental_possible)
-    k_bin_rows<true><<<rows, kBlock, 0, c->stream>>>(c->g, pt.start_base, key, nullptr, start, perm, inv,
-                                                     pt.moved);
-  k_place<<<gp, kBlock, 0, c->stream>>>(key, pt.cursor, perm, n, pt.moved);
-  k_fixup<<<gc, kBlock, 0, c->stream>>>(start, perm, inv, nc, pt.moved);
+}
+
+static void zero_counts(tpx_ctx *c, int32_t *cnt) {
+  k_set_i<<<grid1d(c->g.n + 1, kBlock), kBlock, 0, c->stream>>>(cnt, 0, c->g.n + 1);
   TPX_LAUNCH_CHECK();
 }
 
 void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_int[3],
-                             const double *rho_int) {
+                             const double *rho_int, bool count_new) {
   Particles &pt = c->pt;
   if (pt.n == 0) return;
   AdvArgs a{};
@@ -717,8 +590,8 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
   a.contrib = c->p.twc ? pt.contrib : nullptr;
   a.cstride = pt.cap;
   a.inv_int = pt.inv_int;
-  a.base_key = pt.base_sorted ? pt.base_key : nullptr;
-  a.moved = pt.moved;
+  a.cnt = count_new ? pt.start_new : nullptr;
+  a.rank = pt.rank;
   a.key = pt.key;
   a.n = pt.n;
   a.c_post = k.dt * k.post;
@@ -731,42 +604,51 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
   a.epsp = c->p.epsilonp;
   a.L[0] = c->p.lx; a.L[1] = c->p.ly; a.L[2] = c->p.lz;
   a.rk3 = k.rk == 3;
+  if (count_new) {
+    timing_begin(c, TPX_K_SORT);
+    zero_counts(c, pt.start_new);
+    timing_end(c, TPX_K_SORT);
+  }
   timing_begin(c, TPX_K_PARTICLE);
-  k_set_i<<<1, 64, 0, c->stream>>>(pt.moved, 0, 1);
   k_advance<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(a);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_PARTICLE);
   pt.np1_is_base = false;  // the np1 arrays now hold x_np1 = x + dt*post*u_int
+  pt.new_counted = count_new;
 }
 
 void launch_velocity_deposit(tpx_ctx *c) {
   Particles &pt = c->pt;
-  unsigned rows = (unsigned)((long)c->g.ny * c->g.nz);
   timing_begin(c, TPX_K_DEPOSIT);
-  k_vel_rows<<<rows, kBlock, 0, c->stream>>>(c->g, pt.start_int, pt.contrib, pt.cap, c->S[0], c->S[1], c->S[2]);
+  k_vel_cells<<<grid1d(c->g.n, kBlock), kBlock, 0, c->stream>>>(c->g, pt.start_int, pt.contrib, pt.cap, c->S[0],
+                                                                 c->S[1], c->S[2]);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_DEPOSIT);
 }
 
-// bin by the floor cell of x_new (keys written by the advance kernel)
+// bin by the floor cell of x_new (keys and counts written by the advance kernel)
 void launch_sort_new(tpx_ctx *c) {
   Particles &pt = c->pt;
   if (pt.n == 0) return;
+  if (!pt.new_counted) throw std::logic_error("x_new was not counted by the particle advance");
   timing_begin(c, TPX_K_SORT);
-  bin_particles(c, pt.key, pt.perm_new, pt.inv_new, pt.start_new, pt.base_sorted);
+  bin_particles(c, pt.key, pt.perm_new, pt.inv_new, pt.start_new);
   timing_end(c, TPX_K_SORT);
+  pt.new_counted = false;
 }
 
-// bin by the floor cell of one stage's positions with the general sort
+// bin by the floor cell of one stage's positions
 void launch_sort_stage(tpx_ctx *c, int stage, bool into_int) {
   Particles &pt = c->pt;
   if (pt.n == 0) return;
   int s = stage == TPX_BASE ? pt.st_base : stage == TPX_INT ? pt.st_int : stage == TPX_NEW ? pt.st_new : pt.st_np1;
+  int32_t *start = into_int ? pt.start_int : pt.start_new;
   timing_begin(c, TPX_K_SORT);
-  k_keys<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(c->g, pt.q[0][s], pt.q[1][s], pt.q[2][s], pt.key, pt.n);
+  zero_counts(c, start);
+  k_keys<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(c->g, pt.q[0][s], pt.q[1][s], pt.q[2][s], pt.key, start,
+                                                         pt.rank, pt.n);
   TPX_LAUNCH_CHECK();
-  bin_particles(c, pt.key, into_int ? pt.perm_int : pt.perm_new, into_int ? pt.inv_int : pt.inv_new,
-                into_int ? pt.start_int : pt.start_new, false);
+  bin_particles(c, pt.key, into_int ? pt.perm_int : pt.perm_new, into_int ? pt.inv_int : pt.inv_new, start);
   timing_end(c, TPX_K_SORT);
 }
 
@@ -795,16 +677,43 @@ void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot) {
   timing_end(c, TPX_K_PARTICLE);
 }
 
+// Launch shape of the heat moment kernel: whole x-rows per block (256
+// threads, or nx threads when nx > 256) and up to 32 output planes.
+struct HeatShape { int threads, rows, zc; unsigned blocks; };
+static HeatShape heat_shape(const Geo &g) {
+  HeatShape h;
+  h.rows = g.nx >= 256 ? 1 : std::min(256 / g.nx, g.ny);
+  h.threads = g.nx * h.rows;
+  h.zc = std::min(32, g.nz);
+  h.blocks = (unsigned)((g.ny / h.rows) * (g.nz / h.zc));
+  return h;
+}
+
 // heat deposit from the records of the last temperature gather
 void launch_temp_deposit(tpx_ctx *c, int stage) {
   Particles &pt = c->pt;
   const int32_t *start = stage == TPX_INT ? pt.start_int : pt.start_new;
   timing_begin(c, TPX_K_DEPOSIT);
   if (pt.n > 0) {
-    unsigned rows = (unsigned)((long)c->g.ny * c->g.nz);
-    k_heat_rows<<<rows, kBlock, 0, c->stream>>>(c->g, start, pt.rec, pt.rkey, c->Q, c->partials);
+    const Geo &g = c->g;
+    HeatShape h = heat_shape(g);
+    if (h.threads > kHeatMaxBlock || g.ny % h.rows || g.nz % h.zc)
+      throw std::logic_error("heat deposit launch shape does not tile the grid");
+    HeatArgs a{};
+    a.g = g;
+    a.start = start;
+    a.rec = pt.rec;
+    a.rkey = pt.rkey;
+    a.t3 = c->dep_tmp;
+    a.rows = h.rows;
+    a.zc = h.zc;
+    size_t lds = sizeof(double) * 18 * (size_t)h.threads;
+    k_heat_planes<<<h.blocks, h.threads, lds, c->stream>>>(a);
     TPX_LAUNCH_CHECK();
-    launch_finish_sum(c, (int)rows, 0, SC_SUM_Q);
+    unsigned nb = grid1d(g.n, kBlock);
+    k_heat_combine<<<nb, kBlock, 0, c->stream>>>(g, c->dep_tmp, c->Q, c->partials);
+    TPX_LAUNCH_CHECK();
+    launch_finish_sum(c, (int)nb, 0, SC_SUM_Q);
   }
   timing_end(c, TPX_K_DEPOSIT);
 }
@@ -831,7 +740,7 @@ void launch_concentration(tpx_ctx *c) {
 // perm_int (the bins of the current base positions), duplicate it into the
 // int stage (equal to the base at a step boundary), and make that order the
 // base order of the next step: perm_int/inv_int become the identity and
-// start_base the CSR of start_int.
+// start_int stays the CSR of that order.
 void launch_particle_permute(tpx_ctx *c) {
   Particles &pt = c->pt;
   if (pt.n == 0) return;
@@ -839,7 +748,6 @@ void launch_particle_permute(tpx_ctx *c) {
   int free_a = pt.st_np1, free_b = pt.st_int;  // np1 is stale (np1_is_base), int == base
   PermArgs a{};
   a.perm = pt.perm_int;
-  a.key = pt.key;
   for (int q = 0; q < 7; ++q) {
     a.src[q] = pt.q[q][sb];
     a.dst_a[q] = pt.q[q][free_a];
@@ -853,7 +761,6 @@ void launch_particle_permute(tpx_ctx *c) {
   }
   a.flag_s = pt.flag;
   a.flag_d = pt.itmp;
-  a.base_key = pt.base_key;
   a.n = pt.n;
   timing_begin(c, TPX_K_SORT);
   k_permute<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(a);
@@ -867,9 +774,6 @@ void launch_particle_permute(tpx_ctx *c) {
   pt.np1_is_base = true;
   launch_iota(c, pt.perm_int, pt.n);
   launch_iota(c, pt.inv_int, pt.n);
-  k_copy_i<<<grid1d(c->g.n + 1, kBlock), kBlock, 0, c->stream>>>(pt.start_base, pt.start_int, c->g.n + 1);
-  TPX_LAUNCH_CHECK();
-  pt.base_sorted = true;
   timing_end(c, TPX_K_SORT);
   check(hipGetLastError());
 }

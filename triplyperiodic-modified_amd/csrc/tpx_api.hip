@@ -175,7 +175,7 @@ void do_update_particle(tpx_ctx *c, int rk) {
   const double *ri[3];
   ru_ptrs(c, c->ru_int, ri);
   launch_concentration(c);
-  launch_particle_advance(c, k, ri, c->rho[c->rho_int]);
+  launch_particle_advance(c, k, ri, c->rho[c->rho_int], true);
   if (c->p.twc) launch_velocity_deposit(c);
   else for (int d = 0; d < 3; ++d) launch_fill(c, c->S[d], 0.0, c->g.n);
   launch_sort_new(c);
@@ -273,7 +273,6 @@ void do_time_advance(tpx_ctx *c) {
   if (!pt.np1_is_base) std::swap(pt.st_base, pt.st_np1);
   pt.np1_is_base = true;
   if (pt.n > 0 && c->binned_this_step) launch_particle_permute(c);
-  else pt.base_sorted = false;
   c->binned_this_step = false;
 }
 
@@ -295,14 +294,15 @@ void fused_substep(tpx_ctx *c, int rk) {
   launch_update_p0(c, k);
   Particles &pt = c->pt;
   bool parts = pt.n > 0;
+  bool rebin = parts && (c->p.twc || heat_active(c));
   if (parts) {
     const double *ri[3];
     ru_ptrs(c, c->ru_int, ri);
-    launch_particle_advance(c, k, ri, c->rho[c->rho_int]);
+    launch_particle_advance(c, k, ri, c->rho[c->rho_int], rebin);
     if (c->p.twc) launch_velocity_deposit(c);
   }
   do_update_ru_wop(c, rk, true);
-  if (parts && (c->p.twc || heat_active(c))) {
+  if (rebin) {
     launch_sort_new(c);
     c->binned_this_step = true;
   }
@@ -329,7 +329,7 @@ void release(tpx_ctx *c) {
   auto f = [](void *p) { if (p) (void)hipFree(p); };
   for (auto &b : c->ru) for (auto &p : b) f(p);
   for (auto &p : c->rho) f(p);
-  f(c->P); f(c->rhs); f(c->Q); f(c->C); f(c->divergence);
+  f(c->P); f(c->rhs); f(c->Q); f(c->C); f(c->divergence); f(c->dep_tmp);
   for (auto &p : c->S) f(p);
   for (auto &p : c->spec) f(p);
   for (auto &p : c->lam) f(p);
@@ -339,8 +339,8 @@ void release(tpx_ctx *c) {
   Particles &pt = c->pt;
   for (auto &q : pt.q) for (auto &p : q) f(p);
   f(pt.rhs_t); f(pt.tau); f(pt.eps); f(pt.flag); f(pt.contrib); f(pt.key);
-  f(pt.perm_int); f(pt.perm_new); f(pt.start_int); f(pt.start_new); f(pt.cursor);
-  f(pt.inv_int); f(pt.inv_new); f(pt.start_base); f(pt.base_key); f(pt.moved); f(pt.rkey); f(pt.rec);
+  f(pt.perm_int); f(pt.perm_new); f(pt.start_int); f(pt.start_new); f(pt.rank);
+  f(pt.inv_int); f(pt.inv_new); f(pt.rkey); f(pt.rec);
   for (auto &p : pt.spare) f(p);
   f(pt.itmp);
   for (auto &e : c->tm.pool) (void)hipEventDestroy(e);
@@ -446,6 +446,7 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
     for (auto &b : c->ru) for (auto &f : b) dalloc(&f, n);
     for (auto &f : c->rho) dalloc(&f, n);
     dalloc(&c->P, n); dalloc(&c->rhs, n); dalloc(&c->Q, n); dalloc(&c->C, n); dalloc(&c->divergence, n);
+    if (np_capacity > 0) dalloc(&c->dep_tmp, 3 * n);
     for (auto &f : c->S) dalloc(&f, n);
     dalloc(&c->scal, SC_COUNT);
     // one partial per 256-cell block or per x-row, whichever is more
@@ -473,8 +474,7 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
     ialloc(&pt.key, cap);
     ialloc(&pt.perm_int, cap); ialloc(&pt.perm_new, cap);
     ialloc(&pt.inv_int, cap); ialloc(&pt.inv_new, cap);
-    ialloc(&pt.start_int, n + 1); ialloc(&pt.start_new, n + 1); ialloc(&pt.cursor, n + 1);
-    ialloc(&pt.start_base, n + 1); ialloc(&pt.base_key, cap); ialloc(&pt.moved, 1);
+    ialloc(&pt.start_int, n + 1); ialloc(&pt.start_new, n + 1); ialloc(&pt.rank, cap);
     ialloc(&pt.rkey, cap);
     if (hipMalloc(&pt.rec, sizeof(double4) * (cap ? cap : 1)) != hipSuccess) throw std::bad_alloc();
     hip_check(hipMemset(pt.start_int, 0, sizeof(int32_t) * (n + 1)), "hipMemset");
@@ -576,7 +576,7 @@ int tpx_upload_particles(tpx_ctx *ctx, int64_t n, const double *x, const double 
     if (tau && n) hip_check(hipMemcpyAsync(pt.tau, tau, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
     if (eps && n) hip_check(hipMemcpyAsync(pt.eps, eps, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
     pt.n = n;
-    pt.base_sorted = false;
+    pt.new_counted = false;
     sync(ctx);
     return TPX_OK;
   });

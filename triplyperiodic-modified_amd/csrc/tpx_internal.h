@@ -60,11 +60,8 @@ struct Particles {
   int32_t *perm_int = nullptr, *perm_new = nullptr;     // sorted position -> particle
   int32_t *inv_int = nullptr, *inv_new = nullptr;       // particle -> sorted position
   int32_t *start_int = nullptr, *start_new = nullptr;   // CSR over cells (n_cell+1)
-  int32_t *start_base = nullptr;  // CSR of the physical order (cells of the step's base positions)
-  int32_t *base_key = nullptr;    // cell of each particle in the physical order
-  bool base_sorted = false;       // physical order is sorted by base cell (start_base valid)
-  int32_t *cursor = nullptr;
-  int32_t *moved = nullptr;       // device flag: some particle left the 27 cells around its base cell
+  int32_t *rank = nullptr;        // slot of each particle among its cell's count
+  bool new_counted = false;       // the last advance counted x_new into start_new
   double4 *rec = nullptr;         // heat deposit records in sorted order: x_l, y_l, z_l, RHS_T
   int32_t *rkey = nullptr;        // packed rounded (cloud-in-cell) cell of each record
   bool has_tau = false, has_eps = false;
@@ -105,6 +102,7 @@ struct tpx_ctx {
   bool rho_np1_is_base = false;
   double *P = nullptr, *rhs = nullptr, *S[3] = {}, *Q = nullptr, *C = nullptr,
          *divergence = nullptr;
+  double *dep_tmp = nullptr;      // 3 x n: row partials of the heat deposit
   bool rhs_has_mean = false;      // rhs holds the raw RHS, SC_SUM_RHS its sum
   bool q_zero = false;            // Q is known to be identically zero
   bool binned_this_step = false;  // particles were re-binned during the current step
@@ -154,8 +152,9 @@ int fft_setup(tpx_ctx *c);
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
 
 // ---- launchers (k_particle.hip) ----
+// count_new: zero start_new and count x_new into it for launch_sort_new
 void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_int[3],
-                             const double *rho_int);
+                             const double *rho_int, bool count_new);
 void launch_velocity_deposit(tpx_ctx *c);
 void launch_sort_new(tpx_ctx *c);
 void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot);
