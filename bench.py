@@ -96,16 +96,36 @@ def cpu_baseline(case, threads):
                       f"({dt:.2f} s, {n_it} Poisson iterations), value = 1/(4 x substep time)"}
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size (read `mib` MiB, write
+    `mib` MiB), five times: the calibration kernel for the FETCH_SIZE and
+    WRITE_SIZE counters that the traffic figures are scaled by."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="heat256", choices=["heat256", "plain128", "plain32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the step is measured at N=1 only; "
                                    "multi-GPU decomposition is a design (see the sharding notes), "
