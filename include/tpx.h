@@ -20,7 +20,7 @@
  *  - All device state stays resident between calls.  Calls on one context
  *    are ordered on its HIP stream; contexts are independent; one context
  *    must not be used from two host threads at once.
- *  - Grid sizes must be powers of two in [8, 1024] (the FFT Poisson solve).
+ *  - Grid sizes must be powers of two in [8, 512] (the FFT Poisson solve).
  */
 #ifndef TPX_H
 #define TPX_H
@@ -145,10 +145,11 @@ typedef enum tpx_kclass {
   TPX_K_DIV_RHS,        /* low-Mach divergence + Poisson RHS (G5+G6) */
   TPX_K_FFT,            /* Poisson iteration passes (G8/G9) */
   TPX_K_PROJECT,        /* pressure-gradient projection (G10) */
-  TPX_K_PARTICLE,       /* particle RK4 advance and gathers (P1-P4, P8) */
+  TPX_K_PARTICLE,       /* particle temperature gather (P8) */
   TPX_K_DEPOSIT,        /* deposits (P5, P6, P9) */
   TPX_K_SORT,           /* cell sort of the particles */
   TPX_K_OTHER,          /* reductions, scalars, rotations */
+  TPX_K_ADVANCE,        /* particle RK4 advance with its velocity gather (P1-P4, P7) */
   TPX_K_COUNT
 } tpx_kclass;
 int tpx_get_stream(tpx_ctx *ctx, void **stream);

@@ -310,6 +310,295 @@ __global__ __launch_bounds__(kThreads) void k_zpass(double2 *spec, int mxh, int 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Four-step register FFT for lines of n = R1 * R2 points (R1 >= R2, both in
+// {4, 8, 16}).  With i = t + R1 m and k = k2 + R2 k1,
+//   X[k2 + R2 k1] = sum_t W_R1^{t k1} W_n^{t k2} sum_m x[t + R1 m] W_R2^{m k2}.
+// Step A: the work item for t holds x[t + R1 m] (m < R2) in registers,
+//         transforms over m and applies W_n^{t k2}.
+// Exchange through LDS (one write, one barrier, one read).
+// Step B: the work item for k2 gathers the R1 values over t and transforms
+//         them: it ends holding X[k2 + R2 k1] for k1 < R1.
+// Step B's output layout is step A's input layout of the transposed
+// factorisation (R2, R1), so a forward transform can feed an inverse one
+// without another exchange.
+// ---------------------------------------------------------------------------
+// e^{-2 pi i k/16}, k < 8, from cos/sin of multiples of pi/8
+__device__ __forceinline__ double2 w16(int k) {
+  constexpr double c1 = 0.9238795325112867, s1 = 0.3826834323650898, r = 0.7071067811865476;
+  switch (k) {
+    case 0: return make_double2(1.0, 0.0);
+    case 1: return make_double2(c1, -s1);
+    case 2: return make_double2(r, -r);
+    case 3: return make_double2(s1, -c1);
+    case 4: return make_double2(0.0, -1.0);
+    case 5: return make_double2(-s1, -c1);
+    case 6: return make_double2(-r, -r);
+    default: return make_double2(-c1, -s1);
+  }
+}
+
+// multiply by W_R^k (forward) or its conjugate (INV); k < R/2 is a constant
+template <int R, bool INV>
+__device__ __forceinline__ double2 twr(double2 a, int k) {
+  if (k == 0) return a;
+  if (4 * k == R) return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+  double2 w = w16(k * (16 / R));
+  if (INV) w.y = -w.y;
+  return cmul(a, w);
+}
+
+// in-register DFT of R points, natural order in and out (radix-2 DIT)
+template <int R, bool INV>
+struct RegDft {
+  static __device__ __forceinline__ void run(double2 *v) {
+    double2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) { e[q] = v[2 * q]; o[q] = v[2 * q + 1]; }
+    RegDft<R / 2, INV>::run(e);
+    RegDft<R / 2, INV>::run(o);
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+      double2 t = twr<R, INV>(o[q], q);
+      v[q] = cadd(e[q], t);
+      v[q + R / 2] = csub(e[q], t);
+    }
+  }
+};
+template <bool INV>
+struct RegDft<1, INV> {
+  static __device__ __forceinline__ void run(double2 *) {}
+};
+
+// step A on values v[m] = x[t + R1 m]: transform over m, then W_n^{t k2}
+template <int R2, bool INV>
+__device__ __forceinline__ void step_a(double2 (&v)[R2], int t, const double2 *__restrict__ tw) {
+  RegDft<R2, INV>::run(v);
+#pragma unroll
+  for (int k2 = 1; k2 < R2; ++k2) {
+    double2 w = tw[t * k2];
+    if (INV) w.y = -w.y;
+    v[k2] = cmul(v[k2], w);
+  }
+}
+
+// X forward, four-step: C lines per block, R1 work items per line in step A
+template <int R1, int R2>
+__global__ __launch_bounds__(256) void k_xfwd4(XArgs a, int C) {
+  extern __shared__ double2 lds[];
+  constexpr int n = R1 * R2, P = R1 + 1;  // padded row of the exchange
+  const Geo &g = a.g;
+  const int mxh = n / 2 + 1;
+  const long nlines = (long)g.ny * g.nz;
+  Ix ix{g.nx - 1, g.ny - 1, g.nz - 1, g.nx, g.nxy};
+  const double mean = a.scal[SC_SUM_RHS] / a.ncell;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const long line0 = (long)blk * C;
+  {
+    const int t = threadIdx.x % R1, c = threadIdx.x / R1;
+    const long line = line0 + c;
+    double2 v[R2];
+    const bool live = line < nlines;
+    const int j = (int)(line % g.ny), k = (int)(line / g.ny);
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = make_double2(live ? residual(a, ix, t + R1 * m, j, k, mean) : 0.0, 0.0);
+    step_a<R2, false>(v, t, a.tw);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) lds[(c * R2 + k2) * P + t] = v[k2];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < C * R2) {
+    const int k2 = threadIdx.x % R2, c = threadIdx.x / R2;
+    const long line = line0 + c;
+    double2 u[R1];
+#pragma unroll
+    for (int t = 0; t < R1; ++t) u[t] = lds[(c * R2 + k2) * P + t];
+    RegDft<R1, false>::run(u);
+    if (line < nlines) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        int m = k2 + R2 * k1;
+        if (m < mxh) a.spec[line * mxh + m] = u[k1];
+      }
+    }
+  }
+}
+
+// X inverse, four-step: Hermitian extension, P += dp, block max |dp|
+template <int R1, int R2>
+__global__ __launch_bounds__(256) void k_xinv4(XInvArgs a, int C) {
+  extern __shared__ double2 lds[];
+  __shared__ double red[256];
+  constexpr int n = R1 * R2, P = R1 + 1;
+  const Geo &g = a.g;
+  const int mxh = n / 2 + 1;
+  const long nlines = (long)g.ny * g.nz;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const long line0 = (long)blk * C;
+  {
+    const int t = threadIdx.x % R1, c = threadIdx.x / R1;
+    const long line = line0 + c;
+    double2 v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) {
+      int i = t + R1 * m;
+      double2 s = make_double2(0.0, 0.0);
+      if (line < nlines) {
+        if (i < mxh) {
+          s = a.spec[line * mxh + i];
+          if (i == 0 || i == n / 2) s.y = 0.0;
+        } else {
+          s = a.spec[line * mxh + (n - i)];
+          s.y = -s.y;
+        }
+      }
+      v[m] = s;
+    }
+    step_a<R2, true>(v, t, a.tw);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) lds[(c * R2 + k2) * P + t] = v[k2];
+  }
+  __syncthreads();
+  double mx = 0.0;
+  if ((int)threadIdx.x < C * R2) {
+    const int k2 = threadIdx.x % R2, c = threadIdx.x / R2;
+    const long line = line0 + c;
+    double2 u[R1];
+#pragma unroll
+    for (int t = 0; t < R1; ++t) u[t] = lds[(c * R2 + k2) * P + t];
+    RegDft<R1, true>::run(u);
+    if (line < nlines) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        double dp = (-1) * u[k1].x / (a.ncell);
+        double *Pp = a.P + line * n + k2 + R2 * k1;
+        *Pp = *Pp + dp;
+        mx = fmax(mx, fabs(dp));
+      }
+    }
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.partials[blk] = red[0];
+}
+
+// Strided lines (the y and z passes): C kx columns per block, consecutive
+// threads on consecutive columns.  `base` points at element 0 of column 0,
+// `stride` is the distance between consecutive line elements.
+template <int R1, int R2, bool INV>
+__device__ __forceinline__ void strided_fwd_half(double2 (&v)[R2], double2 *lds, const double2 *base, long stride,
+                                                 int C, int cols, const double2 *__restrict__ tw) {
+  const int col = threadIdx.x % C, t = threadIdx.x / C;
+#pragma unroll
+  for (int m = 0; m < R2; ++m)
+    v[m] = col < cols ? base[(long)(t + R1 * m) * stride + col] : make_double2(0.0, 0.0);
+  step_a<R2, INV>(v, t, tw);
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) lds[(k2 * R1 + t) * C + col] = v[k2];
+}
+
+template <int R1, int R2, bool INV>
+__global__ __launch_bounds__(256) void k_ypass4(double2 *spec, int mxh, int ny, int nz,
+                                                const double2 *__restrict__ tw, int C) {
+  extern __shared__ double2 lds[];
+  const int ntile = (mxh + C - 1) / C;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int tile = blk % ntile, k = blk / ntile;
+  const int m0 = tile * C, cols = min(C, mxh - m0);
+  double2 *base = spec + (long)k * ny * mxh + m0;
+  {
+    double2 v[R2];
+    strided_fwd_half<R1, R2, INV>(v, lds, base, mxh, C, cols, tw);
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < C * R2) {
+    const int col = threadIdx.x % C, k2 = threadIdx.x / C;
+    double2 u[R1];
+#pragma unroll
+    for (int t = 0; t < R1; ++t) u[t] = lds[(k2 * R1 + t) * C + col];
+    RegDft<R1, INV>::run(u);
+    if (col < cols) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) base[(long)(k2 + R2 * k1) * mxh + col] = u[k1];
+    }
+  }
+}
+
+// Z pass: forward FFT along z, divide by lambda (poisson.cpp:433-451), inverse
+// FFT along z with the transposed factorisation, all on one load and store.
+template <int R1, int R2>
+__global__ __launch_bounds__(256) void k_zpass4(double2 *spec, int mxh, int ny, int nz,
+                                                const double2 *__restrict__ tw,
+                                                const double *__restrict__ lamx,
+                                                const double *__restrict__ lamy,
+                                                const double *__restrict__ lamz, int C) {
+  extern __shared__ double2 lds[];
+  const int ntile = (mxh + C - 1) / C;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int tile = blk % ntile, j = blk / ntile;
+  const int m0 = tile * C, cols = min(C, mxh - m0);
+  const long stride = (long)ny * mxh;
+  double2 *base = spec + (long)j * mxh + m0;
+  {
+    double2 v[R2];
+    strided_fwd_half<R1, R2, false>(v, lds, base, stride, C, cols, tw);
+  }
+  __syncthreads();
+  const bool b_item = (int)threadIdx.x < C * R2;
+  const int col = threadIdx.x % C;
+  double2 u[R1];
+  if (b_item) {
+    const int k2 = threadIdx.x / C;
+#pragma unroll
+    for (int t = 0; t < R1; ++t) u[t] = lds[(k2 * R1 + t) * C + col];
+    RegDft<R1, false>::run(u);
+    // u[k1] = X[k2 + R2 k1]: divide by lambda except the global zero mode
+    const int m = m0 + col;
+    const double lxy = lamx[m < mxh ? m : 0] + lamy[j];
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      int kz = k2 + R2 * k1;
+      if (m + j + kz > 0) {
+        double lam = lxy + lamz[kz];
+        u[k1].y /= lam;
+        u[k1].x /= lam;
+      }
+    }
+    // inverse step A with factorisation (R2, R1): the work item for t' = k2
+    // holds x[t' + R2 m'] = u[m']
+    RegDft<R1, true>::run(u);
+#pragma unroll
+    for (int q = 1; q < R1; ++q) {
+      double2 w = tw[k2 * q];
+      w.y = -w.y;
+      u[q] = cmul(u[q], w);
+    }
+  }
+  __syncthreads();  // the forward exchange has been read
+  if (b_item) {
+    const int k2 = threadIdx.x / C;
+#pragma unroll
+    for (int q = 0; q < R1; ++q) lds[(q * R2 + k2) * C + col] = u[q];
+  }
+  __syncthreads();
+  {
+    // inverse step B: the work item for q < R1 gathers over t' < R2
+    const int q = threadIdx.x / C;
+    double2 w2[R2];
+#pragma unroll
+    for (int t = 0; t < R2; ++t) w2[t] = lds[(q * R2 + t) * C + col];
+    RegDft<R2, true>::run(w2);
+    if (col < cols) {
+#pragma unroll
+      for (int k1 = 0; k1 < R2; ++k1) base[(long)(q + R1 * k1) * stride + col] = w2[k1];
+    }
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_finish_max(const double *__restrict__ partials, long nparts,
                                                      double *scal, int slot) {
   __shared__ double sh[1024];
@@ -366,29 +655,102 @@ int fft_setup(tpx_ctx *c) {
   return TPX_OK;
 }
 
+// (R1, R2) of the four-step kernels for an axis length, or false when the
+// length uses the LDS Stockham kernels (8 and 512)
+static bool four_step(int n, int &r1, int &r2) {
+  switch (n) {
+    case 16: r1 = 4; r2 = 4; return true;
+    case 32: r1 = 8; r2 = 4; return true;
+    case 64: r1 = 8; r2 = 8; return true;
+    case 128: r1 = 16; r2 = 8; return true;
+    case 256: r1 = 16; r2 = 16; return true;
+    default: return false;
+  }
+}
+
+#define TPX_FOUR_STEP_SWITCH(n, LAUNCH) \
+  switch (n) {                           \
+    case 16: LAUNCH(4, 4); break;        \
+    case 32: LAUNCH(8, 4); break;        \
+    case 64: LAUNCH(8, 8); break;        \
+    case 128: LAUNCH(16, 8); break;      \
+    default: LAUNCH(16, 16); break;      \
+  }
+
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
   const Geo &g = c->g;
   int mxh = g.nx / 2 + 1;
   long nlines = (long)g.ny * g.nz;
-  int lpb = lines_per_block(g.nx);
-  unsigned nbx = (unsigned)((nlines + lpb - 1) / lpb);
-  size_t shx = sizeof(double2) * (size_t)g.nx * lpb;
+  int r1, r2;
   timing_begin(c, TPX_K_FFT);
   XArgs xa{g, c->P, rho_new, c->rhs, c->scal, c->spec[0], c->tw[0], (double)g.n};
-  k_xfwd<<<nbx, kThreads, shx, c->stream>>>(xa, lpb);
-  TPX_LAUNCH_CHECK();
-  int ty = lines_per_block(g.ny), tz = lines_per_block(g.nz);
-  unsigned nby = (unsigned)(((mxh + ty - 1) / ty) * g.nz), nbz = (unsigned)(((mxh + tz - 1) / tz) * g.ny);
-  size_t shy = sizeof(double2) * (size_t)ty * g.ny, shz = sizeof(double2) * (size_t)tz * g.nz;
-  k_ypass<false><<<nby, kThreads, shy, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], ty);
-  TPX_LAUNCH_CHECK();
-  k_zpass<<<nbz, kThreads, shz, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[2], c->lam[0], c->lam[1],
-                                              c->lam[2], tz);
-  TPX_LAUNCH_CHECK();
-  k_ypass<true><<<nby, kThreads, shy, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], ty);
-  TPX_LAUNCH_CHECK();
   XInvArgs xi{g, c->P, c->spec[0], c->tw[0], c->partials, (double)g.n};
-  k_xinv<<<nbx, kThreads, shx, c->stream>>>(xi, lpb);
+  const bool x4 = four_step(g.nx, r1, r2);
+  const int cx = x4 ? 128 / r1 : lines_per_block(g.nx);
+  const unsigned nbx = (unsigned)((nlines + cx - 1) / cx);
+  const size_t shx4 = x4 ? sizeof(double2) * (size_t)cx * r2 * (r1 + 1) : 0;
+  const size_t shx = sizeof(double2) * (size_t)g.nx * cx;
+  if (x4) {
+#define L_XF(A, B) k_xfwd4<A, B><<<nbx, cx * A, shx4, c->stream>>>(xa, cx)
+    TPX_FOUR_STEP_SWITCH(g.nx, L_XF)
+#undef L_XF
+  } else {
+    k_xfwd<<<nbx, kThreads, shx, c->stream>>>(xa, cx);
+  }
+  TPX_LAUNCH_CHECK();
+  const bool y4 = four_step(g.ny, r1, r2);
+  if (y4) {
+    const int C = 128 / r1;
+    const unsigned nb = (unsigned)(((mxh + C - 1) / C) * g.nz);
+    const size_t sh = sizeof(double2) * (size_t)C * g.ny;
+#define L_YF(A, B) k_ypass4<A, B, false><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], C)
+    TPX_FOUR_STEP_SWITCH(g.ny, L_YF)
+#undef L_YF
+  } else {
+    int ty = lines_per_block(g.ny);
+    unsigned nby = (unsigned)(((mxh + ty - 1) / ty) * g.nz);
+    k_ypass<false><<<nby, kThreads, sizeof(double2) * (size_t)ty * g.ny, c->stream>>>(c->spec[0], mxh, g.ny, g.nz,
+                                                                                      c->tw[1], ty);
+  }
+  TPX_LAUNCH_CHECK();
+  if (four_step(g.nz, r1, r2)) {
+    const int C = 128 / r1;
+    const unsigned nb = (unsigned)(((mxh + C - 1) / C) * g.ny);
+    const size_t sh = sizeof(double2) * (size_t)C * g.nz;
+#define L_Z(A, B)                                                                                          \
+  k_zpass4<A, B><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[2], c->lam[0], c->lam[1], \
+                                               c->lam[2], C)
+    TPX_FOUR_STEP_SWITCH(g.nz, L_Z)
+#undef L_Z
+  } else {
+    int tz = lines_per_block(g.nz);
+    unsigned nbz = (unsigned)(((mxh + tz - 1) / tz) * g.ny);
+    k_zpass<<<nbz, kThreads, sizeof(double2) * (size_t)tz * g.nz, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[2],
+                                                                               c->lam[0], c->lam[1], c->lam[2], tz);
+  }
+  TPX_LAUNCH_CHECK();
+  if (y4) {
+    four_step(g.ny, r1, r2);
+    const int C = 128 / r1;
+    const unsigned nb = (unsigned)(((mxh + C - 1) / C) * g.nz);
+    const size_t sh = sizeof(double2) * (size_t)C * g.ny;
+#define L_YI(A, B) k_ypass4<A, B, true><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], C)
+    TPX_FOUR_STEP_SWITCH(g.ny, L_YI)
+#undef L_YI
+  } else {
+    int ty = lines_per_block(g.ny);
+    unsigned nby = (unsigned)(((mxh + ty - 1) / ty) * g.nz);
+    k_ypass<true><<<nby, kThreads, sizeof(double2) * (size_t)ty * g.ny, c->stream>>>(c->spec[0], mxh, g.ny, g.nz,
+                                                                                     c->tw[1], ty);
+  }
+  TPX_LAUNCH_CHECK();
+  if (x4) {
+#define L_XI(A, B) k_xinv4<A, B><<<nbx, cx * A, shx4, c->stream>>>(xi, cx)
+    TPX_FOUR_STEP_SWITCH(g.nx, L_XI)
+#undef L_XI
+  } else {
+    k_xinv<<<nbx, kThreads, shx, c->stream>>>(xi, cx);
+  }
   TPX_LAUNCH_CHECK();
   k_finish_max<<<1, 1024, 0, c->stream>>>(c->partials, nbx, c->scal, SC_MAX_DP);
   TPX_LAUNCH_CHECK();

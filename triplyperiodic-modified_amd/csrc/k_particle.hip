@@ -609,10 +609,10 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
     zero_counts(c, pt.start_new);
     timing_end(c, TPX_K_SORT);
   }
-  timing_begin(c, TPX_K_PARTICLE);
+  timing_begin(c, TPX_K_ADVANCE);
   k_advance<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(a);
   TPX_LAUNCH_CHECK();
-  timing_end(c, TPX_K_PARTICLE);
+  timing_end(c, TPX_K_ADVANCE);
   pt.np1_is_base = false;  // the np1 arrays now hold x_np1 = x + dt*post*u_int
   pt.new_counted = count_new;
 }

@@ -1,0 +1,65 @@
+// grid.h -- host mirror of the reference's grid class (grid.h:14-146).
+//
+// The same constructor, public fields and substep methods, so that the
+// reference's driver loop (simulation.cpp:57-93) runs unchanged on top of
+// the device library: every method forwards to one C ABI entry point, the
+// public tensors are handles to device-resident fields, and the scalars are
+// refreshed from the device after each method that changes them.
+#pragma once
+
+#include <fstream>
+
+#include "fields.h"
+
+class grid {
+  params *param_;
+  proc *pc_;
+  gridsize *size_;
+  communicator *com_;
+  tpx_ctx *ctx_ = nullptr;
+  poisson PS_;
+  bool Is_touch_ = false;
+  std::ofstream stat_P0, stat_NumIteration, stat_RhoMean, stat_RhoMax;
+  void pull_scalars();
+  void ok(int rc, const char *what) { tpx_check(ctx_, rc, what); }
+  [[noreturn]] void unsupported(const char *what);
+
+ public:
+  particle part;
+  tensor1 RU, RU_int, RU_new, RU_np1, RHS_RU;
+  tensor0 P, RHS_Pois, C, Rho, Rho_int, Rho_new, Rho_np1, divergence, RHS_Part_Temp;
+  double P0 = 0, P0_int = 0, P0_new = 0, P0_np1 = 0, dP0_dt = 0;
+  double T_cur = 0;
+  int num_timestep = 0;
+  int RK4_count = 0;
+
+  grid(gridsize *size, params *param, proc *pc, communicator *com);
+  ~grid();
+  grid(const grid &) = delete;
+  grid &operator=(const grid &) = delete;
+  tpx_ctx *context() { return ctx_; }
+
+  void Initialize();          // grid.cpp:117-486 (Initial = 1..5)
+  void Store();               // grid.cpp:489-593
+  void TimeAdvance();         // grid.cpp:595-606
+  void Update_Rho();          // grid.cpp:608-614
+  void Update_RU_WOP();       // grid.cpp:776-814
+  void Update_P0();           // grid.cpp:816-830
+  void Compute_Div_U_new();   // grid.cpp:832-839
+  void Compute_RHS_Pois();    // grid.cpp:841-849
+  void Solve_Poisson();       // grid.cpp:851-855
+  void Update_RU_WP();        // grid.cpp:857-863
+  void Update_Particle();     // grid.cpp:865-888
+  void TimeAdvance_RK4();     // grid.cpp:890-900
+  void Statistics();          // grid.cpp:902-982 (P0, iterations, density)
+  bool Touch();               // grid.cpp:94-108
+  int num_iteration() const { return PS_.num_iteration(); }
+
+  // passive scalar / vector, elongated-box copy: outside this build
+  void Update_Passive_Scalar() { unsupported("Update_Passive_Scalar"); }
+  void Update_RV_WOQ() { unsupported("Update_RV_WOQ"); }
+  void Compute_RHS_Pois_Q() { unsupported("Compute_RHS_Pois_Q"); }
+  void Solve_Poisson_Q() { unsupported("Solve_Poisson_Q"); }
+  void Update_RV_WQ() { unsupported("Update_RV_WQ"); }
+  void CopyBox() { unsupported("CopyBox"); }
+};
