@@ -35,11 +35,11 @@ def log(*a):
 
 
 def grid_rhs_bytes_per_launch(ncell, rk, has_s=True):
-    """Algorithmic bytes of one density + momentum RHS launch.  Per cell it
-    reads RU_int(3), Rho_int, RU(3), RU_np1(3), Rho, Rho_np1, S(3) and writes
-    RU_new(3), RU_np1(3), Rho_new, Rho_np1 (23 FP64 values); at the first
-    substep RU_np1/Rho_np1 are the base state (read once), at the last one
-    RU_new/Rho_new are RU_np1/Rho_np1 and RU/Rho are not read."""
+    """Algorithmic bytes of one density + momentum RHS launch (k_rhs_ru).
+    Per cell it reads RU_int(3), Rho_int, RU(3), RU_np1(3), Rho, Rho_np1, S(3)
+    and writes RU_new(3), RU_np1(3), Rho_new, Rho_np1 (23 FP64 values); at
+    the first substep RU_np1/Rho_np1 are the base state (read once), at the
+    last one RU_new/Rho_new are RU_np1/Rho_np1 and RU/Rho are not read."""
     s = 3 if has_s else 0
     if rk == 0:
         n = 3 + 1 + 3 + 1 + s + 8
@@ -50,6 +50,24 @@ def grid_rhs_bytes_per_launch(ncell, rk, has_s=True):
     return 8.0 * ncell * n
 
 
+def advance_bytes_per_launch(ncell, npart, rk, per_eps, per_tau, twc, counted):
+    """Algorithmic bytes of one particle advance launch (k_advance).  Per
+    particle it reads the int stage (7 FP64), the base stage (7; x, y, z only
+    at the last substep), the np1 stage (7; it is the base stage at the first
+    substep), RHS_T and the per-particle eps / tau, and writes the np1 and new
+    stages (14), the 6 drag shares with their sorted position (4 B read), the
+    cell key and the counting slot (4 + 4 B, plus the 8 B atomic update of the
+    cell count).  The gas it gathers is RU_int (3 fields) and Rho_int, read
+    once per cell."""
+    rd = 7 + (3 if rk == 3 else 7) + (0 if rk == 0 else 7) + 1 + per_eps + per_tau
+    wr = 14
+    b = 8.0 * (rd + wr)
+    if twc:
+        b += 6 * 8 + 4
+    b += 4 + (16 if counted else 0)
+    return npart * b + 8.0 * ncell * 4
+
+
 def step_bytes(ncell, npart, n_it):
     """Algorithmic bytes of one RK4 step: 4 substeps of 46 grid field passes,
     26 per Poisson iteration plus 4 per extra iteration, and 65 FP64 per
@@ -58,12 +76,14 @@ def step_bytes(ncell, npart, n_it):
     return 4 * per_sub
 
 
-def load_traffic(name):
-    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+def load_traffic(workload):
+    """Per-kernel HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (scaled by a plain copy of known size) for this workload."""
+    path = os.path.join(ROOT, "profiles", f"rk4_step_{workload}_traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as fh:
-        return json.load(fh)
+        return json.load(fh).get("kernels", {})
 
 
 def cpu_baseline(case, threads):
@@ -167,15 +187,40 @@ def main():
     per_step = {k: ms[k] / args.timing_steps for k in ms}
     log("[bench] device ms per step by kernel class: " +
         ", ".join(f"{k}={v:.3f}" for k, v in per_step.items()))
-    avg_launch_ms = ms["grid_rhs"] / max(launches["grid_rhs"], 1)
-    bytes_launch = sum(grid_rhs_bytes_per_launch(ncell, rk, p.TWC and npart > 0) for rk in range(4)) / 4.0
-    achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
-    traffic = load_traffic("k_rhs_ru")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic["bytes_per_launch"] if traffic else None,
-                "kernel": "k_rhs_ru (density + momentum right-hand side, fused RK4 update)",
-                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_launch_ms, 4)}
+    traffic = load_traffic(args.workload)
+    counted = bool(p.TWC or p.Nu > 0) and npart > 0
+    per_eps, per_tau = int(case.eps is not None), int(case.tau is not None)
+    kernels = {
+        "advance": ("k_advance", "particle RK4 advance with its velocity gather and cell count",
+                    sum(advance_bytes_per_launch(ncell, npart, rk, per_eps, per_tau, bool(p.TWC), counted)
+                        for rk in range(4)) / 4.0),
+        "grid_rhs": ("k_rhs_ru", "density + momentum right-hand side, fused RK4 update",
+                     sum(grid_rhs_bytes_per_launch(ncell, rk, p.TWC and npart > 0) for rk in range(4)) / 4.0),
+    }
+    lines = {}
+    for cls, (kname, what, bytes_launch) in kernels.items():
+        if launches.get(cls, 0) == 0:
+            continue
+        avg_ms = ms[cls] / launches[cls]
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+        t = traffic.get(kname)
+        lines[cls] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(achieved / HBM_PEAK_GBS, 4),
+                      "traffic": round(t["traffic_bytes"]) if t else None,
+                      "kernel": f"{kname} ({what})", "bytes_per_launch": bytes_launch,
+                      "avg_launch_ms": round(avg_ms, 4), "ms_per_step": round(per_step[cls], 3)}
+    dominant = max(lines, key=lambda k: lines[k]["ms_per_step"])
+    roofline = lines[dominant]
+    others = [v for k, v in lines.items() if k != dominant]
+
+    # what a caller pays to take the full state back to the host (the step
+    # itself never crosses PCIe; state is uploaded once and stays resident)
+    t0 = time.perf_counter()
+    for name in ("RU", "Rho", "P"):
+        s.get(name)
+    s.particles()
+    download_ms = (time.perf_counter() - t0) * 1e3
+    log(f"[bench] full state download {download_ms:.1f} ms")
 
     # whole-step algorithmic bandwidth with the iteration count of the run
     its = s.scalars()["num_iteration"]
@@ -198,9 +243,11 @@ def main():
                    "grid": p.Nx_tot, "particles": npart, "dt": p.dt,
                    "poisson_iterations_last_substep": n_it_last},
         "roofline": roofline,
+        "roofline_next": others,
         "cpu_baseline": cpu,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
         "step_algorithmic_GBps": round(whole, 1),
+        "state_download_ms": round(download_ms, 1),
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = round(value / cpu["value"], 1)

@@ -19,6 +19,32 @@
  * The one deliberate deviation: the reference's 3-D FFT (Sandia fft_3d over
  * FFTW 2.1.5, absent from the reference tree) is replaced by the radix-2 FFT
  * in or_fft.c, pinned against a brute-force DFT and numpy.fft in tests.
+ *
+ * How each part is pinned (tests/test_oracle_operators.py,
+ * tests/test_oracle_physics.py), since the reference itself cannot be built
+ * here and ships no golden vectors:
+ *   stencils S1-S9, fold    single-Fourier-mode closed forms, summation by
+ *                           parts, brute-force fold
+ *   or_fft3d, ccp_fft       brute-force DFT, numpy fixture, mode / zero-mode
+ *                           rule, exact inverse of the 7-point Laplacian
+ *   or_solve_poisson        uniform-coefficient exactness, residual decrease
+ *   or_update_rho (G1)      mass conservation
+ *   or_update_p0 (G2)       exact RK4 of a linear ODE
+ *   or_update_ru_wop (G4)   momentum/energy conservation of convection,
+ *                           viscous single modes, forcing, body force, drag
+ *   or_compute_div_u_new, or_compute_rhs_pois, or_update_ru_wp (G5, G6, G10)
+ *                           single-mode closed forms
+ *   or_make_mean_u0 (G12)   exact mean
+ *   or_initialize (G14)     known temperature gap -> known heat source
+ *   particle gathers and deposits (P1, P5, P6, P8, P9): exact transposes,
+ *                           partition of unity, separable exactness
+ *   particle RK4 (P2-P4)    frozen uniform gas = classical RK4 matrix map
+ *   or_p_send_recv (P7)     ids kept, one-period shifts into [0, L]
+ * Parity unpinned: the order in which or_p_send_recv re-appends wrapped
+ * particles (every comparison matches particles by flag, so it cannot change
+ * a result), and the composition of the methods in or_step beyond what the
+ * tests above fix (which stage value each term reads is restated from the
+ * reference's source and is not checked against a run of the reference).
  */
 #ifndef TPX_ORACLE_H
 #define TPX_ORACLE_H

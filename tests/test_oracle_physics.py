@@ -7,7 +7,12 @@ against what the mathematics fixes, not against a retyped formula:
       work on a discretely divergence-free field at uniform density; the
       viscous, forcing, body-force and drag terms are separated by linearity
       and checked against single-mode closed forms.
+* G5  divergence: (k/Cp) lap(1/Rho_new) - R/(Cp P0_new Vcell) Q on a mode.
+* G6  Poisson RHS: single-mode face divergence, the divergence subtracted,
+      the 1/(dt pre) scaling and the mean removed.
+* G10 projection: single-mode gradient with the RK4 post/pre coefficients.
 * G12 the shifted mean equals rho0 U0 exactly.
+* G14 Initialize's heat source from a known gas/particle temperature gap.
 * P1  velocity gather: 1-D linear along its own axis, piecewise constant
       across (grid.cpp / particle.cpp:340-360), checked on separable fields.
 * P6  the velocity deposit is the exact transpose of the P1 gather.
@@ -443,3 +448,107 @@ def test_b1_spectral_initial_condition():
     # the oracle's own divergence operator agrees
     o = new_oracle(base_params(), 0)
     assert np.max(np.abs(o.apply("div_f2c", u[0], u[1], u[2]))) < 1e-11
+
+
+# ---------------------------------------------------------------- G5 / G6 / G10 / G14
+def prepared(p, npart=0, parts=None, Rho=None):
+    o = new_oracle(p, npart)
+    load_grid(o, np.zeros((3, N, N, N)), np.ones((N, N, N)) if Rho is None else Rho)
+    if parts is not None:
+        o.set_particles(parts["x"], parts["y"], parts["z"], parts["u"], parts["v"], parts["w"], parts["T"],
+                        parts["flag"])
+    o.set_restart_numbers(p.P0)
+    o.initialize()
+    return o
+
+
+@pytest.mark.parametrize("cooling", [0, 1])
+def test_g5_divergence_closed_form(cooling):
+    """divergence = (k/Cp) lap(1/Rho_new) - R/(Cp P0_new Vcell) Q, with Q made
+    zero-mean when cooling: checked on 1/Rho_new = 1 + e cos(k x)."""
+    p = base_params(k=0.3, cooling=cooling)
+    o = prepared(p)
+    o.set_rk4_count(0)
+    o.update_p0()
+    P0n = o.scalars()["P0_new"]
+    assert P0n == p.P0
+    xc, yc, zc, _, _, _ = grid_coords()
+    kv = (2.0, 1.0, 3.0)
+    ph = kv[0] * xc + kv[1] * yc + kv[2] * zc
+    e = 0.2
+    o.set("Rho_new", 1.0 / (1.0 + e * np.cos(ph)))
+    Q = 0.4 + 0.1 * np.sin(ph)
+    o.set("RHS_Part_Temp", Q)
+    o.compute_div_u_new()
+    dx = L / N
+    lam = sum((2 / dx ** 2) * (1 - math.cos(k * dx)) for k in kv)
+    vcell = dx ** 3
+    Qe = Q - Q.mean() if cooling else Q
+    expect = (p.k / p.Cp) * (-lam * e * np.cos(ph)) - p.R / (p.Cp * P0n * vcell) * Qe
+    assert np.max(np.abs(o.get("divergence") - expect)) < 1e-11 * np.max(np.abs(expect))
+
+
+def test_g6_poisson_rhs_closed_form():
+    """RHS_Pois = (div_F2C(RU_new / Rho_face) - divergence) / (dt pre) made
+    zero-mean, for a single face mode at uniform density."""
+    p = base_params(dt=0.01)
+    o = prepared(p)
+    o.set_rk4_count(1)     # pre = 1/2
+    _, _, _, xf, _, _ = grid_coords()
+    xc, yc, zc, _, _, _ = grid_coords()
+    kx = 3.0
+    RU = np.zeros((3, N, N, N))
+    RU[0] = np.sin(kx * xf)
+    o.set("RU_new", RU)
+    o.set("Rho_new", np.ones((N, N, N)))
+    d = 0.3 * np.cos(yc) + 0.05
+    o.set("divergence", d)
+    o.compute_rhs_pois()
+    dx = L / N
+    div = (2 / dx) * math.sin(kx * dx / 2) * np.cos(kx * xc)
+    raw = (div - d) / (p.dt * 0.5)
+    assert np.max(np.abs(o.get("RHS_Pois") - (raw - raw.mean()))) < 1e-10
+
+
+@pytest.mark.parametrize("rk", [1, 3])
+def test_g10_projection_closed_form(rk):
+    """RU_np1 -= dt post grad P, RU_new -= dt pre grad P (RU_new = RU_np1 at
+    the last substep), with the single-mode gradient in closed form."""
+    p = base_params(dt=0.02)
+    o = prepared(p)
+    o.set_rk4_count(rk)
+    xc, yc, zc, xf, yf, zf = grid_coords()
+    kv = (1.0, 2.0, 2.0)
+    o.set("P", np.cos(kv[0] * xc + kv[1] * yc + kv[2] * zc))
+    rng = np.random.default_rng(20)
+    np1, new = rng.standard_normal((3, N, N, N)), rng.standard_normal((3, N, N, N))
+    o.set("RU_np1", np1)
+    o.set("RU_new", new)
+    o.update_ru_wp()
+    dx = L / N
+    pos = [(xf, yc, zc), (xc, yf, zc), (xc, yc, zf)]
+    G = np.stack([-(2 / dx) * math.sin(kv[a] * dx / 2) *
+                  np.sin(kv[0] * pos[a][0] + kv[1] * pos[a][1] + kv[2] * pos[a][2]) for a in range(3)])
+    post = [1 / 6, 1 / 3, 1 / 3, 1 / 6][rk]
+    pre = [0.5, 0.5, 1.0][rk] if rk < 3 else None
+    e_np1 = np1 - p.dt * post * G
+    assert np.max(np.abs(o.get("RU_np1") - e_np1)) < 1e-12
+    e_new = e_np1 if rk == 3 else new - p.dt * pre * G
+    assert np.max(np.abs(o.get("RU_new") - e_new)) < 1e-12
+
+
+def test_g14_initial_heat_source():
+    """Initialize's tail: T = P0/(R Rho) gathered at x_int, RHS_T =
+    (Tg - T_p) Nu pi Dp k, deposited: particles 1 degree colder than a uniform
+    gas give sum(RHS_Part_Temp) = Np Nu pi Dp k; particles at the gas
+    temperature give zero."""
+    p = base_params(Nu=2.0, k=0.05, Dp=0.01)
+    parts = random_particles(600, 21)
+    T_gas = p.P0 / (p.R * p.Rho0)
+    parts["T"] = np.full(600, T_gas - 1.0)
+    o = prepared(p, 600, parts)
+    expect = 600 * p.Nu * PI * p.Dp * p.k
+    assert abs(o.get("RHS_Part_Temp").sum() - expect) < 1e-12 * expect
+    parts["T"] = np.full(600, T_gas)
+    o = prepared(p, 600, parts)
+    assert np.max(np.abs(o.get("RHS_Part_Temp"))) < 1e-15
