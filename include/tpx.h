@@ -73,6 +73,7 @@ typedef enum tpx_scalar {
   TPX_S_P0 = 0, TPX_S_P0_INT, TPX_S_P0_NEW, TPX_S_P0_NP1, TPX_S_T_CUR,
   TPX_S_NUM_TIMESTEP, TPX_S_NUM_ITERATION, TPX_S_MEAN_ENERGY, TPX_S_A,
   TPX_S_I0, TPX_S_ITERATION, TPX_S_GX, TPX_S_GY, TPX_S_GZ, TPX_S_DP0_DT,
+  TPX_S_TOTAL_ITERATIONS,   /* Poisson iterations summed over all solves */
   TPX_S_COUNT
 } tpx_scalar;
 

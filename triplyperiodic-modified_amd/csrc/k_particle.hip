@@ -3,9 +3,9 @@
 // Particles are SoA arrays kept physically ordered by the cell of their base
 // position (re-sorted once per step).  Every substep they are re-binned by
 // the floor cell of x_new with a counting sort: the advance kernel counts
-// each particle into its cell with an integer atomic and keeps the returned
-// slot, an exclusive scan turns the counts into CSR offsets, a placement
-// pass writes every particle to its offset plus slot, and a per-cell
+// each particle into its cell with an integer atomic (no return value, so
+// no wave waits on it), an exclusive scan turns the counts into CSR offsets,
+// a placement pass takes slots from per-cell cursors, and a per-cell
 // insertion sort puts each cell's particles in increasing physical index.
 // Integer counts do not depend on the order the atomics took, and the final
 // sort removes the one thing that does, so the bins are identical on every
@@ -68,9 +68,9 @@ struct AdvArgs {
   long cstride;
   const int32_t *inv_int;    // particle -> sorted position of x_int
   int32_t *cnt;              // per-cell counts of x_new (null: no binning this substep)
-  int32_t *rank;             // slot of the particle among its cell's count
   int32_t *key;
   long n;
+  int int_is_base, np1_is_base;  // the int / np1 inputs are the base arrays
   double c_post, c_pre;     // dt*post, dt*pre
   double iTp, g3[3];        // 1/Tp, ParticleGravity*g
   double iS, Dp, I0;        // 1/(mp Cvp), particle diameter, lamp intensity
@@ -82,11 +82,20 @@ struct AdvArgs {
 // P1-P4, P6 shares and P7 wrap for one particle (particle.cpp:314-399,
 // 477-492, 494-520, 58-104)
 __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
   const Geo &g = a.g;
   Ix ix = make_ix(g);
-  double xi = a.in_int[0][p], yi = a.in_int[1][p], zi = a.in_int[2][p];
+  // stage values: base (always x, y, z; the rest unless the last substep),
+  // int (x, y, z, u, v, w) and np1, each loaded once when stages coincide
+  double b[7], vi[6], v1[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) b[q] = (q < 3 || !a.rk3) ? a.base[q][p] : 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) vi[q] = a.int_is_base ? b[q] : a.in_int[q][p];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) v1[q] = a.np1_is_base ? b[q] : a.np1_in[q][p];
+  double xi = vi[0], yi = vi[1], zi = vi[2];
   // gas2part_velocity (particle.cpp:340-360): x2i and the fractional part
   int ii = (int)(xi * g.idx), jj = (int)(yi * g.idy), kk = (int)(zi * g.idz);
   double xl = (xi * g.idx - ((int)(xi * g.idx)));
@@ -111,17 +120,17 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a) {
   // update_position (particle.cpp:314-338)
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double ui = a.in_int[3 + d][p];
-    n1[d] = a.np1_in[d][p] + a.c_post * ui;
-    nv[d] = a.rk3 ? n1[d] : a.base[d][p] + a.c_pre * ui;
+    double ui = vi[3 + d];
+    n1[d] = v1[d] + a.c_post * ui;
+    nv[d] = a.rk3 ? n1[d] : b[d] + a.c_pre * ui;
   }
   // update_velocity (particle.cpp:362-399) and the deposit shares
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double ui = a.in_int[3 + d][p];
+    double ui = vi[3 + d];
     double rhs = (gas[d] - ui) * itp;
-    n1[3 + d] = a.np1_in[3 + d][p] + a.c_post * (rhs + a.g3[d]);
-    nv[3 + d] = a.rk3 ? n1[3 + d] : a.base[3 + d][p] + a.c_pre * (rhs + a.g3[d]);
+    n1[3 + d] = v1[3 + d] + a.c_post * (rhs + a.g3[d]);
+    nv[3 + d] = a.rk3 ? n1[3 + d] : b[3 + d] + a.c_pre * (rhs + a.g3[d]);
     if (a.contrib) {  // component 2d: share of face ii+1, 2d+1: of face ii
       long q = a.inv_int[p];
       a.contrib[(2 * d) * a.cstride + q] = frac[d] * rhs;
@@ -133,11 +142,12 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a) {
     double epsp = a.eps ? a.eps[p] : a.epsp;
     double alpha = 0.25 * epsp * TPX_PI * a.Dp * a.Dp * a.I0;
     double rt = a.rhs_t[p];
-    n1[6] = a.np1_in[6][p] + a.c_post * a.iS * (rt + alpha);
-    nv[6] = a.rk3 ? n1[6] : a.base[6][p] + a.c_pre * a.iS * (rt + alpha);
+    n1[6] = v1[6] + a.c_post * a.iS * (rt + alpha);
+    nv[6] = a.rk3 ? n1[6] : b[6] + a.c_pre * a.iS * (rt + alpha);
   }
-  // periodic wrap of x_new, x and x_np1 (particle.cpp:58-104); x_int keeps its value
-  double bx[3] = {a.base[0][p], a.base[1][p], a.base[2][p]};
+  // periodic wrap of x_new, x and x_np1 (particle.cpp:58-104); x_int keeps
+  // its value unless it is the base buffer itself (first substep)
+  double bx[3] = {b[0], b[1], b[2]};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (nv[d] > a.L[d]) {
@@ -156,18 +166,18 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a) {
   int ki = (int)(nv[0] * g.idx), kj = (int)(nv[1] * g.idy), kz = (int)(nv[2] * g.idz);
   long nk = ix(ki, kj, kz);
   a.key[p] = (int32_t)nk;
-  if (a.cnt) a.rank[p] = atomicAdd(&a.cnt[nk], 1);
+  if (a.cnt) atomicAdd(&a.cnt[nk], 1);  // no return value: the wave does not wait
 }
 
 // floor-cell key of one stage of positions, counted into its cell
 __global__ void k_keys(Geo g, const double *x, const double *y, const double *z, int32_t *key,
-                       int32_t *cnt, int32_t *rank, long n) {
+                       int32_t *cnt, long n) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   Ix ix = make_ix(g);
   int32_t k = (int32_t)ix((int)(x[p] * g.idx), (int)(y[p] * g.idy), (int)(z[p] * g.idz));
   key[p] = k;
-  rank[p] = atomicAdd(&cnt[k], 1);
+  atomicAdd(&cnt[k], 1);
 }
 
 __global__ void k_set_i(int32_t *a, int32_t v, long n) {
@@ -175,17 +185,22 @@ __global__ void k_set_i(int32_t *a, int32_t v, long n) {
   if (p < n) a[p] = v;
 }
 
-// placement: CSR offset of the cell plus the slot the counting atomic gave
-__global__ void k_place(const int32_t *__restrict__ key, const int32_t *__restrict__ rank,
-                        const int32_t *__restrict__ start, int32_t *perm, long n) {
+__global__ void k_copy_i(int32_t *__restrict__ d, const int32_t *__restrict__ s, long n) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) perm[start[key[p]] + rank[p]] = (int32_t)p;
+  if (p < n) d[p] = s[p];
+}
+
+// placement: each particle takes the next free slot of its cell's segment
+// (cursor starts as a copy of the CSR offsets)
+__global__ void k_place(const int32_t *__restrict__ key, int32_t *cursor, int32_t *perm, long n) {
+  long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (p < n) perm[atomicAdd(&cursor[key[p]], 1)] = (int32_t)p;
 }
 
 // per-cell insertion sort of the placed indices: the result is the stable
 // order (by physical index) whatever order the counting atomics took
 __global__ void k_fixup(const int32_t *__restrict__ start, int32_t *perm, int32_t *inv, long ncell) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   int32_t b = start[c], e = start[c + 1];
   for (int32_t q = b + 1; q < e; ++q) {
@@ -252,20 +267,20 @@ struct TempArgs {
   int p0_slot;
   double R;
   double iTth;
-  const int32_t *perm;       // sorted position -> particle
+  const int32_t *inv;        // particle -> sorted position
   double *rhs_t;             // by particle
   double4 *rec;              // by sorted position
   int32_t *rkey;
   long n;
 };
 
-// P8/P10 gather in sorted order: T = (P0/R)/Rho (grid.cpp:882), cloud-in-cell
+// P8/P10 gather in physical order: T = (P0/R)/Rho (grid.cpp:882), cloud-in-cell
 // at the particle (particle.cpp:402-456), RHS_T = (Tg - T_p) Nu pi Dp k
-// (particle.cpp:460-474); also emits the deposit record of the particle
+// (particle.cpp:460-474); also emits the deposit record of the particle at
+// its sorted position
 __global__ __launch_bounds__(kBlock) void k_temp_gather(TempArgs a) {
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= a.n) return;
-  long p = a.perm[q];
+  long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
   const Geo &g = a.g;
   Ix ix = make_ix(g);
   double x = a.x[p], y = a.y[p], z = a.z[p];
@@ -283,10 +298,10 @@ __global__ __launch_bounds__(kBlock) void k_temp_gather(TempArgs a) {
               (1 - xl) * (1 - yl) * (1 - zl) * T(ii - 1, jj - 1, kk - 1);
   double v = (Tg - a.Tp[p]) * a.iTth;
   a.rhs_t[p] = v;
+  long q = a.inv[p];
   a.rec[q] = make_double4(xl, yl, zl, v);
   a.rkey[q] = (ii & ix.mx) | ((jj & ix.my) << 10) | ((kk & ix.mz) << 20);
 }
-
 
 // ---- heat deposit by per-cell moments (P9/P10) -----------------------------
 // Stage 1: one block covers `rows` whole x-rows and marches over `zc` output
@@ -480,10 +495,9 @@ __global__ __launch_bounds__(kBlock) void k_concentration(CicDepArgs a) {
 }
 
 struct PermArgs {
-  const int32_t *perm;
+  const int32_t *perm;       // sorted position -> particle
   const double *src[7];
-  double *dst_a[7];
-  double *dst_b[7];
+  double *dst[7];
   const double *s1[3];
   double *d1[3];
   const int32_t *flag_s;
@@ -491,18 +505,15 @@ struct PermArgs {
   long n;
 };
 
-// physical re-sort: gather the base state into cell order; the int stage
-// (equal to the base at a step boundary) gets the same values
+// physical re-sort into cell order: sorted position q gathers the base state
+// of particle perm[q] (the writes stay contiguous; a scatter of the same data
+// measured slower)
 __global__ __launch_bounds__(kBlock) void k_permute(PermArgs a) {
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (q >= a.n) return;
   long s = a.perm[q];
 #pragma unroll
-  for (int d = 0; d < 7; ++d) {
-    double v = a.src[d][s];
-    a.dst_a[d][q] = v;
-    a.dst_b[d][q] = v;
-  }
+  for (int d = 0; d < 7; ++d) a.dst[d][q] = a.src[d][s];
 #pragma unroll
   for (int d = 0; d < 3; ++d)
     if (a.s1[d]) a.d1[d][q] = a.s1[d][s];
@@ -550,13 +561,13 @@ void launch_exclusive_scan(tpx_ctx *c, int32_t *data, int64_t n) {
 
 // Stable bins of the particles by key (one int per particle): perm (sorted
 // position -> particle), inv (particle -> sorted position), start (CSR).
-// On entry `start` holds the per-cell counts (start[n_cell] = 0) and
-// pt.rank each particle's slot among its cell's count.
+// On entry `start` holds the per-cell counts (start[n_cell] = 0).
 static void bin_particles(tpx_ctx *c, const int32_t *key, int32_t *perm, int32_t *inv, int32_t *start) {
   Particles &pt = c->pt;
   long n = pt.n, nc = c->g.n;
   launch_exclusive_scan(c, start, nc + 1);
-  k_place<<<grid1d(n, kBlock), kBlock, 0, c->stream>>>(key, pt.rank, start, perm, n);
+  k_copy_i<<<grid1d(nc + 1, kBlock), kBlock, 0, c->stream>>>(pt.cursor, start, nc + 1);
+  k_place<<<grid1d(n, kBlock), kBlock, 0, c->stream>>>(key, pt.cursor, perm, n);
   k_fixup<<<grid1d(nc, kBlock), kBlock, 0, c->stream>>>(start, perm, inv, nc);
   TPX_LAUNCH_CHECK();
 }
@@ -574,7 +585,7 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
   a.g = c->g;
   for (int d = 0; d < 3; ++d) a.ru[d] = ru_int[d];
   a.rho = rho_int;
-  int sb = pt.st_base, si = pt.st_int;
+  int sb = pt.st_base, si = pt.int_buf();
   int sn = pt.np1_is_base ? pt.st_base : pt.st_np1;
   for (int q = 0; q < 7; ++q) {
     a.base[q] = pt.q[q][sb];
@@ -591,9 +602,10 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
   a.cstride = pt.cap;
   a.inv_int = pt.inv_int;
   a.cnt = count_new ? pt.start_new : nullptr;
-  a.rank = pt.rank;
   a.key = pt.key;
   a.n = pt.n;
+  a.int_is_base = pt.int_is_base || si == sb;
+  a.np1_is_base = pt.np1_is_base;
   a.c_post = k.dt * k.post;
   a.c_pre = k.dt * k.pre;
   a.iTp = c->Tp != 0 ? 1. / c->Tp : 0.;
@@ -641,12 +653,12 @@ void launch_sort_new(tpx_ctx *c) {
 void launch_sort_stage(tpx_ctx *c, int stage, bool into_int) {
   Particles &pt = c->pt;
   if (pt.n == 0) return;
-  int s = stage == TPX_BASE ? pt.st_base : stage == TPX_INT ? pt.st_int : stage == TPX_NEW ? pt.st_new : pt.st_np1;
+  int s = stage == TPX_BASE ? pt.st_base : stage == TPX_INT ? pt.int_buf() : stage == TPX_NEW ? pt.st_new : pt.st_np1;
   int32_t *start = into_int ? pt.start_int : pt.start_new;
   timing_begin(c, TPX_K_SORT);
   zero_counts(c, start);
   k_keys<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(c->g, pt.q[0][s], pt.q[1][s], pt.q[2][s], pt.key, start,
-                                                         pt.rank, pt.n);
+                                                         pt.n);
   TPX_LAUNCH_CHECK();
   bin_particles(c, pt.key, into_int ? pt.perm_int : pt.perm_new, into_int ? pt.inv_int : pt.inv_new, start);
   timing_end(c, TPX_K_SORT);
@@ -656,7 +668,7 @@ void launch_sort_stage(tpx_ctx *c, int stage, bool into_int) {
 void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot) {
   Particles &pt = c->pt;
   if (pt.n == 0) return;
-  int s = stage == TPX_INT ? pt.st_int : pt.st_new;
+  int s = stage == TPX_INT ? pt.int_buf() : pt.st_new;
   TempArgs a{};
   a.g = c->g;
   a.x = pt.q[0][s]; a.y = pt.q[1][s]; a.z = pt.q[2][s];
@@ -666,7 +678,7 @@ void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot) {
   a.p0_slot = p0_slot;
   a.R = c->R;
   a.iTth = c->p.nu * TPX_PI * c->p.dp * c->p.k;
-  a.perm = stage == TPX_INT ? pt.perm_int : pt.perm_new;
+  a.inv = stage == TPX_INT ? pt.inv_int : pt.inv_new;
   a.rhs_t = pt.rhs_t;
   a.rec = pt.rec;
   a.rkey = pt.rkey;
@@ -720,7 +732,7 @@ void launch_temp_deposit(tpx_ctx *c, int stage) {
 
 void launch_concentration(tpx_ctx *c) {
   Particles &pt = c->pt;
-  int s = pt.st_int;
+  int s = pt.int_buf();
   CicDepArgs a{};
   a.g = c->g;
   a.x = pt.q[0][s]; a.y = pt.q[1][s]; a.z = pt.q[2][s];
@@ -745,13 +757,12 @@ void launch_particle_permute(tpx_ctx *c) {
   Particles &pt = c->pt;
   if (pt.n == 0) return;
   int sb = pt.st_base;
-  int free_a = pt.st_np1, free_b = pt.st_int;  // np1 is stale (np1_is_base), int == base
+  int dst = pt.st_np1;  // np1 is stale (np1_is_base)
   PermArgs a{};
   a.perm = pt.perm_int;
   for (int q = 0; q < 7; ++q) {
     a.src[q] = pt.q[q][sb];
-    a.dst_a[q] = pt.q[q][free_a];
-    a.dst_b[q] = pt.q[q][free_b];
+    a.dst[q] = pt.q[q][dst];
   }
   double **own[3] = {&pt.rhs_t, &pt.tau, &pt.eps};
   bool use[3] = {true, pt.has_tau, pt.has_eps};
@@ -768,10 +779,17 @@ void launch_particle_permute(tpx_ctx *c) {
   for (int d = 0; d < 3; ++d)
     if (use[d]) std::swap(*own[d], pt.spare[d]);
   std::swap(pt.flag, pt.itmp);
-  pt.st_base = free_a;
+  pt.st_base = dst;
   pt.st_np1 = sb;
-  pt.st_int = free_b;
   pt.np1_is_base = true;
+  // the int stage equals the base stage: read it from the base buffer and
+  // keep st_int as the spare the first TimeAdvance_RK4 gives to the new stage
+  if (pt.st_int == dst || pt.st_int == sb) {
+    int used[3] = {pt.st_base, pt.st_np1, pt.st_new};
+    for (int b = 0; b < 4; ++b)
+      if (b != used[0] && b != used[1] && b != used[2]) pt.st_int = b;
+  }
+  pt.int_is_base = true;
   launch_iota(c, pt.perm_int, pt.n);
   launch_iota(c, pt.inv_int, pt.n);
   timing_end(c, TPX_K_SORT);

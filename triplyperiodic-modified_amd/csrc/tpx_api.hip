@@ -223,6 +223,7 @@ void do_solve_poisson(tpx_ctx *c, int *iters) {
     i++;
   }
   c->num_iteration = i;
+  c->total_iterations += i;
   if (iters) *iters = i;
 }
 
@@ -241,7 +242,8 @@ void do_time_advance_rk4(tpx_ctx *c) {
   c->ru_new = -1;
   launch_scalar_copy(c, SC_P0_INT, SC_P0_NEW);
   Particles &pt = c->pt;
-  std::swap(pt.st_int, pt.st_new);
+  std::swap(pt.st_int, pt.st_new);  // with int_is_base, new receives the spare buffer
+  pt.int_is_base = false;
   std::swap(pt.perm_int, pt.perm_new);
   std::swap(pt.inv_int, pt.inv_new);
   std::swap(pt.start_int, pt.start_new);
@@ -339,7 +341,7 @@ void release(tpx_ctx *c) {
   Particles &pt = c->pt;
   for (auto &q : pt.q) for (auto &p : q) f(p);
   f(pt.rhs_t); f(pt.tau); f(pt.eps); f(pt.flag); f(pt.contrib); f(pt.key);
-  f(pt.perm_int); f(pt.perm_new); f(pt.start_int); f(pt.start_new); f(pt.rank);
+  f(pt.perm_int); f(pt.perm_new); f(pt.start_int); f(pt.start_new); f(pt.cursor);
   f(pt.inv_int); f(pt.inv_new); f(pt.rkey); f(pt.rec);
   for (auto &p : pt.spare) f(p);
   f(pt.itmp);
@@ -474,7 +476,7 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
     ialloc(&pt.key, cap);
     ialloc(&pt.perm_int, cap); ialloc(&pt.perm_new, cap);
     ialloc(&pt.inv_int, cap); ialloc(&pt.inv_new, cap);
-    ialloc(&pt.start_int, n + 1); ialloc(&pt.start_new, n + 1); ialloc(&pt.rank, cap);
+    ialloc(&pt.start_int, n + 1); ialloc(&pt.start_new, n + 1); ialloc(&pt.cursor, n + 1);
     ialloc(&pt.rkey, cap);
     if (hipMalloc(&pt.rec, sizeof(double4) * (cap ? cap : 1)) != hipSuccess) throw std::bad_alloc();
     hip_check(hipMemset(pt.start_int, 0, sizeof(int32_t) * (n + 1)), "hipMemset");
@@ -568,6 +570,7 @@ int tpx_upload_particles(tpx_ctx *ctx, int64_t n, const double *x, const double 
     const double *src[7] = {x, y, z, u, v, w, T};
     pt.st_base = 0; pt.st_int = 1; pt.st_new = 2; pt.st_np1 = 3;
     pt.np1_is_base = true;
+    pt.int_is_base = false;
     for (int q = 0; q < 7; ++q)
       if (n) hip_check(hipMemcpyAsync(pt.q[q][pt.st_base], src[q], sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
     if (n) hip_check(hipMemcpyAsync(pt.flag, flag, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
@@ -597,7 +600,7 @@ int tpx_download_particles(tpx_ctx *ctx, int stage, double *x, double *y, double
     int s;
     switch (stage) {
     case TPX_BASE: s = pt.st_base; break;
-    case TPX_INT: s = pt.st_int; break;
+    case TPX_INT: s = pt.int_buf(); break;
     case TPX_NEW: s = pt.st_new; break;
     case TPX_NP1: s = pt.np1_is_base ? pt.st_base : pt.st_np1; break;
     default: throw std::invalid_argument("bad stage");
@@ -643,6 +646,7 @@ int tpx_get_scalars(tpx_ctx *ctx, double *o) {
     o[TPX_S_I0] = ctx->I0; o[TPX_S_ITERATION] = ctx->Iteration;
     o[TPX_S_GX] = ctx->gv[0]; o[TPX_S_GY] = ctx->gv[1]; o[TPX_S_GZ] = ctx->gv[2];
     o[TPX_S_DP0_DT] = h[SC_DP0_DT];
+    o[TPX_S_TOTAL_ITERATIONS] = (double)ctx->total_iterations;
     return TPX_OK;
   });
 }

@@ -53,6 +53,11 @@ struct Particles {
   double *q[7][4] = {};
   int st_base = 0, st_int = 1, st_new = 2, st_np1 = 3;
   bool np1_is_base = false;        // np1 equals base (start of a step)
+  // At the start of a step the int stage equals the base stage; it is then
+  // read from the base buffer and st_int names a spare buffer, which the
+  // first TimeAdvance_RK4 swap hands to the new stage.
+  bool int_is_base = false;
+  int int_buf() const { return int_is_base ? st_base : st_int; }
   double *rhs_t = nullptr, *tau = nullptr, *eps = nullptr;
   int32_t *flag = nullptr;
   double *contrib = nullptr;      // 6 per sorted position: velocity deposit shares at x_int
@@ -60,7 +65,7 @@ struct Particles {
   int32_t *perm_int = nullptr, *perm_new = nullptr;     // sorted position -> particle
   int32_t *inv_int = nullptr, *inv_new = nullptr;       // particle -> sorted position
   int32_t *start_int = nullptr, *start_new = nullptr;   // CSR over cells (n_cell+1)
-  int32_t *rank = nullptr;        // slot of each particle among its cell's count
+  int32_t *cursor = nullptr;      // placement cursors over cells (n_cell+1)
   bool new_counted = false;       // the last advance counted x_new into start_new
   double4 *rec = nullptr;         // heat deposit records in sorted order: x_l, y_l, z_l, RHS_T
   int32_t *rkey = nullptr;        // packed rounded (cloud-in-cell) cell of each record
@@ -91,6 +96,7 @@ struct tpx_ctx {
   double T_cur = 0;
   int num_timestep = 0;
   int num_iteration = 0;
+  int64_t total_iterations = 0;    // summed over all Poisson solves
   hipStream_t stream = nullptr;
   int device = 0;
   // grid fields

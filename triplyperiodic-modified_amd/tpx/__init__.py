@@ -26,7 +26,7 @@ VECTOR = {"RU", "RU_int", "RU_new", "RU_np1", "RHS_RU"}
 STAGE = {"base": 0, "int": 1, "new": 2, "np1": 3}
 SCALARS = ["P0", "P0_int", "P0_new", "P0_np1", "T_cur", "num_timestep",
            "num_iteration", "mean_energy_transferred", "A", "I0", "Iteration",
-           "gx", "gy", "gz", "dP0_dt"]
+           "gx", "gy", "gz", "dP0_dt", "total_iterations"]
 KCLASS = ["grid_rhs", "div_rhs", "fft", "project", "particle", "deposit", "sort", "other", "advance"]
 
 # C entry points of include/tpx.h, in declaration order
