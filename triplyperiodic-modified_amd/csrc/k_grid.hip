@@ -83,45 +83,6 @@ __device__ __forceinline__ double divU(const Geo &g, const Ix &ix, const RU3 &ru
          (Uface<2>(ix, ru.a[2], rho, i, j, k + 1) - Uface<2>(ix, ru.a[2], rho, i, j, k)) * g.idz;
 }
 
-// The right-hand side of momentum component A on face A of cell (i,j,k):
-// grid.cpp:780-809 (viscous + convective + forcing + body force), with the
-// drag deposit S_A already known.
-template <int A>
-__device__ __forceinline__ double rhs_face(const Geo &g, const Ix &ix, const Coef &kc,
-                                           const RU3 &ru, const double *__restrict__ rho,
-                                           double S_A, int i, int j, int k) {
-  constexpr int di = (A == 0), dj = (A == 1), dk = (A == 2);
-  const double id[3] = {g.idx, g.idy, g.idz};
-  const double *__restrict__ ra = ru.a[A];
-  double Up = Uface<A>(ix, ra, rho, i, j, k);
-  double Uxm = Uface<A>(ix, ra, rho, i - 1, j, k), Uxp = Uface<A>(ix, ra, rho, i + 1, j, k);
-  double Uym = Uface<A>(ix, ra, rho, i, j - 1, k), Uyp = Uface<A>(ix, ra, rho, i, j + 1, k);
-  double Uzm = Uface<A>(ix, ra, rho, i, j, k - 1), Uzp = Uface<A>(ix, ra, rho, i, j, k + 1);
-  // mu/3 grad(div U) and mu del2 U (grid.cpp:780-783)
-  double gdiv = (divU(g, ix, ru, rho, i, j, k) - divU(g, ix, ru, rho, i - di, j - dj, k - dk)) * id[A];
-  double lap = (Uxp - 2 * Up + Uxm) * g.idx2 + (Uyp - 2 * Up + Uym) * g.idy2 +
-               (Uzp - 2 * Up + Uzm) * g.idz2;
-  double r = kc.a_drag * S_A + kc.b_visc * gdiv + kc.c_visc * lap;
-  // convection (grid.cpp:784-801): div_F2C( I_d^{C2F}(U_A) * I_A^{C2F}(RU_d) )
-  const double *__restrict__ r0 = ru.a[0];
-  const double *__restrict__ r1 = ru.a[1];
-  const double *__restrict__ r2 = ru.a[2];
-  double fxp = ((Uxp + Up) * 0.5) * ((r0[ix(i + 1, j, k)] + r0[ix(i + 1 - di, j - dj, k - dk)]) * 0.5);
-  double fxm = ((Up + Uxm) * 0.5) * ((r0[ix(i, j, k)] + r0[ix(i - di, j - dj, k - dk)]) * 0.5);
-  double fyp = ((Uyp + Up) * 0.5) * ((r1[ix(i, j + 1, k)] + r1[ix(i - di, j + 1 - dj, k - dk)]) * 0.5);
-  double fym = ((Up + Uym) * 0.5) * ((r1[ix(i, j, k)] + r1[ix(i - di, j - dj, k - dk)]) * 0.5);
-  double fzp = ((Uzp + Up) * 0.5) * ((r2[ix(i, j, k + 1)] + r2[ix(i - di, j - dj, k + 1 - dk)]) * 0.5);
-  double fzm = ((Up + Uzm) * 0.5) * ((r2[ix(i, j, k)] + r2[ix(i - di, j - dj, k - dk)]) * 0.5);
-  double conv = (fxp - fxm) * g.idx + (fyp - fym) * g.idy + (fzp - fzm) * g.idz;
-  r = r - conv;
-  // linear forcing A*RU_int (grid.cpp:803)
-  r = r + kc.A * ra[ix(i, j, k)];
-  // body force (Rho_face - Rho0) g (grid.cpp:805-809)
-  double rf = (rho[ix(i, j, k)] + rho[ix(i - di, j - dj, k - dk)]) * 0.5;
-  r = r + (rf - kc.rho0) * kc.g[A];
-  return r;
-}
-
 struct RhsArgs {
   RU3 ru_int;
   const double *rho_int;
@@ -137,34 +98,198 @@ struct RhsArgs {
   double c_np1, c_new, c_rho_np1, c_rho_new;
 };
 
-// K1: Update_RU_WOP (grid.cpp:776-814), optionally fused with Update_Rho
-// (grid.cpp:608-614), which reads the same RU_int.
+// ---------------------------------------------------------------------------
+// K1: Update_RU_WOP (grid.cpp:776-814) fused with Update_Rho
+// (grid.cpp:608-614), which reads the same RU_int, as a z-marching tile.
+// The right-hand side of momentum component A on face A of cell (i,j,k) is
+// (grid.cpp:780-809)
+//   -(mp/Vcell) S_A + (mu/3) d_A(div U) + mu lap U_A
+//   - sum_d d_d^{F2C}[ I_d^{C2F}(U_A) I_A^{C2F}(RU_d) ] + A RU_A + (Rho_face - Rho0) g_A
+// with U = RU_int / Rho_face.  A block owns a TX x TY column of cells and
+// walks KC planes along z.  Per plane it stages RU_int and Rho_int (planes
+// k and k+1, with a halo of 2 below and 1 above in x and y) in LDS, forms
+// every face velocity of the plane once (U_z also for plane k+1) and the
+// cell divergences of U once, and evaluates the stencil from those values
+// in the reference's operand order: each U costs one division instead of
+// the ~19 per face component of a per-cell evaluation, and every value read
+// is the same double, so the results are bit-identical.  Values of plane
+// k-1 that the stencil still needs are carried in registers; the next plane
+// is prefetched into registers while the current one is computed.
+// ---------------------------------------------------------------------------
+struct ZmGeom { int TX, TY, KC; };
+
 template <bool WITH_RHO, bool HAS_S, bool RK3>
-__global__ __launch_bounds__(kBlock) void k_rhs_ru(Geo g, Coef kc, RhsArgs a) {
-  int i, j, k;
-  long c;
-  if (!cell_of(g, i, j, k, c)) return;
-  Ix ix = make_ix(g);
-  const double *__restrict__ rho = a.rho_int;
-  double r0 = rhs_face<0>(g, ix, kc, a.ru_int, rho, HAS_S ? a.S[0][c] : 0.0, i, j, k);
-  double r1 = rhs_face<1>(g, ix, kc, a.ru_int, rho, HAS_S ? a.S[1][c] : 0.0, i, j, k);
-  double r2 = rhs_face<2>(g, ix, kc, a.ru_int, rho, HAS_S ? a.S[2][c] : 0.0, i, j, k);
-  double rr[3] = {r0, r1, r2};
+__global__ __launch_bounds__(256) void k_rhs_ru(Geo g, Coef kc, RhsArgs a, ZmGeom zg) {
+  extern __shared__ double sm[];
+  const int TX = zg.TX, TY = zg.TY, KC = zg.KC;
+  const int RW = TX + 3, RN = RW * (TY + 3);   // raw tile: x from i0-2 to i0+TX
+  const int UW = TX + 2, UN = UW * (TY + 2);   // U tile: x from i0-1 to i0+TX
+  const int DW = TX + 1, DN = DW * (TY + 1);   // divU tile: x from i0-1 to i0+TX-1
+  double *raw = sm;              // [slot 2][field 4][RN]: RU_int x, y, z, Rho_int
+  double *u0t = raw + 8 * RN;    // U_x of plane k
+  double *u1t = u0t + UN;        // U_y of plane k
+  double *u2t = u1t + UN;        // U_z of planes k and k+1 (two buffers)
+  double *dvt = u2t + 2 * UN;    // div U of plane k
+  (void)DN;
+  const Ix ix = make_ix(g);
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int ntx = g.nx / TX, nty = g.ny / TY;
+  const int i0 = (int)(b % ntx) * TX, j0 = (int)((b / ntx) % nty) * TY, k0 = (int)(b / (ntx * nty)) * KC;
+  const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+  const int i = i0 + tx, j = j0 + ty;
+  const double *__restrict__ f4[4] = {a.ru_int.a[0], a.ru_int.a[1], a.ru_int.a[2], a.rho_int};
+  auto load_plane = [&](int kz, int slot) {
+    const long zoff = (long)(kz & ix.mz) * g.nxy;
+    for (int e = tid; e < RN; e += blockDim.x) {
+      const int li = e % RW, lj = e / RW;
+      const long off = zoff + (long)((j0 + lj - 2) & ix.my) * g.nx + ((i0 + li - 2) & ix.mx);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double np1 = a.ru_np1_in[d][c] + a.c_np1 * rr[d];     // RU_np1 += dt*post*RHS
-    a.ru_np1[d][c] = np1;
-    if (!RK3) a.ru_new[d][c] = 1 * a.ru_base[d][c] + a.c_new * rr[d];  // RU_new = RU + dt*pre*RHS
+      for (int q = 0; q < 4; ++q) raw[(slot * 4 + q) * RN + e] = f4[q][off];
+    }
+  };
+  // raw value of field q in slot s at cell (i0 + x, j0 + y)
+  auto RV = [&](int s, int q, int x, int y) -> double { return raw[(s * 4 + q) * RN + (y + 2) * RW + (x + 2)]; };
+  auto UT = [&](const double *t, int x, int y) -> double { return t[(y + 1) * UW + (x + 1)]; };
+  auto DT = [&](int x, int y) -> double { return dvt[(y + 1) * DW + (x + 1)]; };
+
+  // ---- prologue: plane k0-1 in slot 1, plane k0 in slot 0
+  load_plane(k0 - 1, 1);
+  load_plane(k0, 0);
+  __syncthreads();
+  double *u2c = u2t, *u2n = u2t + UN;
+  for (int e = tid; e < UN; e += blockDim.x) {
+    const int x = e % UW - 1, y = e / UW - 1;
+    u2c[e] = RV(0, 2, x, y) / ((RV(0, 3, x, y) + RV(1, 3, x, y)) * 0.5);
   }
-  if (WITH_RHO) {
-    const double *__restrict__ u0 = a.ru_int.a[0];
-    const double *__restrict__ u1 = a.ru_int.a[1];
-    const double *__restrict__ u2 = a.ru_int.a[2];
-    double R = (u0[ix(i + 1, j, k)] - u0[c]) * g.idx + (u1[ix(i, j + 1, k)] - u1[c]) * g.idy +
-               (u2[ix(i, j, k + 1)] - u2[c]) * g.idz;
-    double np1 = a.rho_np1_in[c] + a.c_rho_np1 * R;
-    a.rho_np1[c] = np1;
-    if (!RK3) a.rho_new[c] = 1 * a.rho_base[c] + a.c_rho_new * R;
+  // own-column values of plane k0-1
+  double Up[3], dvp, r0p, r0p1, r1p, r1p1, r2p, rhop;
+  Up[0] = RV(1, 0, tx, ty) / ((RV(1, 3, tx, ty) + RV(1, 3, tx - 1, ty)) * 0.5);
+  Up[1] = RV(1, 1, tx, ty) / ((RV(1, 3, tx, ty) + RV(1, 3, tx, ty - 1)) * 0.5);
+  Up[2] = Uface<2>(ix, a.ru_int.a[2], a.rho_int, i, j, k0 - 1);
+  dvp = divU(g, ix, a.ru_int, a.rho_int, i, j, k0 - 1);
+  r0p = RV(1, 0, tx, ty); r0p1 = RV(1, 0, tx + 1, ty);
+  r1p = RV(1, 1, tx, ty); r1p1 = RV(1, 1, tx, ty + 1);
+  r2p = RV(1, 2, tx, ty); rhop = RV(1, 3, tx, ty);
+  __syncthreads();
+  load_plane(k0 + 1, 1);
+  __syncthreads();
+
+  int cur = 0;
+  const double id[3] = {g.idx, g.idy, g.idz};
+  for (int kk = k0; kk < k0 + KC; ++kk) {
+    const int nxt = cur ^ 1;
+    // prefetch plane k+2 into registers; it is stored after this plane's work
+    // (a tile has at most 2 points per thread: (TX+3)(TY+3) <= 2 TX TY)
+    const bool more = kk + 1 < k0 + KC;
+    double pf[2][4];
+    if (more) {
+      const long zoff = (long)((kk + 2) & ix.mz) * g.nxy;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = tid + u * (int)blockDim.x;
+        if (e < RN) {
+          const int li = e % RW, lj = e / RW;
+          const long off = zoff + (long)((j0 + lj - 2) & ix.my) * g.nx + ((i0 + li - 2) & ix.mx);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) pf[u][q] = f4[q][off];
+        }
+      }
+    }
+    // ---- face velocities of plane k (x, y) and U_z of plane k+1
+    for (int e = tid; e < UN; e += blockDim.x) {
+      const int x = e % UW - 1, y = e / UW - 1;
+      u0t[e] = RV(cur, 0, x, y) / ((RV(cur, 3, x, y) + RV(cur, 3, x - 1, y)) * 0.5);
+      u1t[e] = RV(cur, 1, x, y) / ((RV(cur, 3, x, y) + RV(cur, 3, x, y - 1)) * 0.5);
+      u2n[e] = RV(nxt, 2, x, y) / ((RV(nxt, 3, x, y) + RV(cur, 3, x, y)) * 0.5);
+    }
+    __syncthreads();
+    // ---- div U of plane k (tensor0.cpp:249-256)
+    for (int e = tid; e < (TX + 1) * (TY + 1); e += blockDim.x) {
+      const int x = e % DW - 1, y = e / DW - 1;
+      dvt[e] = (UT(u0t, x + 1, y) - UT(u0t, x, y)) * g.idx + (UT(u1t, x, y + 1) - UT(u1t, x, y)) * g.idy +
+               (UT(u2n, x, y) - UT(u2c, x, y)) * g.idz;
+    }
+    __syncthreads();
+    const long c = (long)kk * g.nxy + (long)j * g.nx + i;
+    double rr[3];
+#pragma unroll
+    for (int A = 0; A < 3; ++A) {
+      const int di = (A == 0), dj = (A == 1), dk = (A == 2);
+      const double *ut = A == 0 ? u0t : (A == 1 ? u1t : u2c);
+      const double Upc = UT(ut, tx, ty);
+      const double Uxm = UT(ut, tx - 1, ty), Uxp = UT(ut, tx + 1, ty);
+      const double Uym = UT(ut, tx, ty - 1), Uyp = UT(ut, tx, ty + 1);
+      const double Uzm = Up[A];
+      const double Uzp = A == 0 ? RV(nxt, 0, tx, ty) / ((RV(nxt, 3, tx, ty) + RV(nxt, 3, tx - 1, ty)) * 0.5)
+                       : A == 1 ? RV(nxt, 1, tx, ty) / ((RV(nxt, 3, tx, ty) + RV(nxt, 3, tx, ty - 1)) * 0.5)
+                                : UT(u2n, tx, ty);
+      const double dv_own = DT(tx, ty);
+      const double dv_m = dk ? dvp : DT(tx - di, ty - dj);
+      double gdiv = (dv_own - dv_m) * id[A];
+      double lap = (Uxp - 2 * Upc + Uxm) * g.idx2 + (Uyp - 2 * Upc + Uym) * g.idy2 +
+                   (Uzp - 2 * Upc + Uzm) * g.idz2;
+      double r = kc.a_drag * (HAS_S ? a.S[A][c] : 0.0) + kc.b_visc * gdiv + kc.c_visc * lap;
+      // convection: RU_d at the positions the flux interpolants read
+      const double x0a = RV(cur, 0, tx + 1, ty);
+      const double x0b = dk ? r0p1 : RV(cur, 0, tx + 1 - di, ty - dj);
+      const double x0c = RV(cur, 0, tx, ty);
+      const double x0d = dk ? r0p : RV(cur, 0, tx - di, ty - dj);
+      const double y1a = RV(cur, 1, tx, ty + 1);
+      const double y1b = dk ? r1p1 : RV(cur, 1, tx - di, ty + 1 - dj);
+      const double y1c = RV(cur, 1, tx, ty);
+      const double y1d = dk ? r1p : RV(cur, 1, tx - di, ty - dj);
+      const double z2a = RV(nxt, 2, tx, ty);
+      const double z2b = dk ? RV(cur, 2, tx, ty) : RV(nxt, 2, tx - di, ty - dj);
+      const double z2c = RV(cur, 2, tx, ty);
+      const double z2d = dk ? r2p : RV(cur, 2, tx - di, ty - dj);
+      double fxp = ((Uxp + Upc) * 0.5) * ((x0a + x0b) * 0.5);
+      double fxm = ((Upc + Uxm) * 0.5) * ((x0c + x0d) * 0.5);
+      double fyp = ((Uyp + Upc) * 0.5) * ((y1a + y1b) * 0.5);
+      double fym = ((Upc + Uym) * 0.5) * ((y1c + y1d) * 0.5);
+      double fzp = ((Uzp + Upc) * 0.5) * ((z2a + z2b) * 0.5);
+      double fzm = ((Upc + Uzm) * 0.5) * ((z2c + z2d) * 0.5);
+      double conv = (fxp - fxm) * g.idx + (fyp - fym) * g.idy + (fzp - fzm) * g.idz;
+      r = r - conv;
+      r = r + kc.A * RV(cur, A, tx, ty);
+      const double rho_m = dk ? rhop : RV(cur, 3, tx - di, ty - dj);
+      double rf = (RV(cur, 3, tx, ty) + rho_m) * 0.5;
+      r = r + (rf - kc.rho0) * kc.g[A];
+      rr[A] = r;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double np1 = a.ru_np1_in[d][c] + a.c_np1 * rr[d];
+      a.ru_np1[d][c] = np1;
+      if (!RK3) a.ru_new[d][c] = 1 * a.ru_base[d][c] + a.c_new * rr[d];
+    }
+    if (WITH_RHO) {
+      double R = (RV(cur, 0, tx + 1, ty) - RV(cur, 0, tx, ty)) * g.idx +
+                 (RV(cur, 1, tx, ty + 1) - RV(cur, 1, tx, ty)) * g.idy +
+                 (RV(nxt, 2, tx, ty) - RV(cur, 2, tx, ty)) * g.idz;
+      double np1 = a.rho_np1_in[c] + a.c_rho_np1 * R;
+      a.rho_np1[c] = np1;
+      if (!RK3) a.rho_new[c] = 1 * a.rho_base[c] + a.c_rho_new * R;
+    }
+    // ---- carry plane k to the next step
+    Up[0] = UT(u0t, tx, ty); Up[1] = UT(u1t, tx, ty); Up[2] = UT(u2c, tx, ty);
+    dvp = DT(tx, ty);
+    r0p = RV(cur, 0, tx, ty); r0p1 = RV(cur, 0, tx + 1, ty);
+    r1p = RV(cur, 1, tx, ty); r1p1 = RV(cur, 1, tx, ty + 1);
+    r2p = RV(cur, 2, tx, ty); rhop = RV(cur, 3, tx, ty);
+    __syncthreads();
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = tid + u * (int)blockDim.x;
+        if (e < RN) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) raw[(cur * 4 + q) * RN + e] = pf[u][q];
+        }
+      }
+    }
+    double *t = u2c; u2c = u2n; u2n = t;
+    cur = nxt;
+    __syncthreads();
   }
 }
 
@@ -389,9 +514,18 @@ void launch_rhs_ru(tpx_ctx *c, const Coef &k, bool with_rho, const double *const
   a.c_rho_np1 = -(k.dt * k.post);
   a.c_rho_new = -k.dt * k.pre;
   bool hs = S != nullptr, r3 = k.rk == 3;
-  dim3 gr(cells_grid(c)), bl(kBlock);
+  const Geo &g = c->g;
+  ZmGeom zg;
+  zg.TX = g.nx < 32 ? g.nx : 32;
+  zg.TY = 256 / zg.TX < g.ny ? 256 / zg.TX : g.ny;
+  zg.KC = g.nz < 32 ? g.nz : 32;
+  unsigned nb = (unsigned)((g.nx / zg.TX) * (g.ny / zg.TY) * (g.nz / zg.KC));
+  int threads = zg.TX * zg.TY;
+  if ((zg.TX + 3) * (zg.TY + 3) > 2 * threads) throw std::logic_error("z-march tile exceeds two points per thread");
+  size_t sh = sizeof(double) * (size_t)(8 * (zg.TX + 3) * (zg.TY + 3) + 4 * (zg.TX + 2) * (zg.TY + 2) +
+                                        (zg.TX + 1) * (zg.TY + 1));
   timing_begin(c, TPX_K_GRID_RHS);
-#define TPX_K1(WR, HS, R3) k_rhs_ru<WR, HS, R3><<<gr, bl, 0, c->stream>>>(c->g, k, a)
+#define TPX_K1(WR, HS, R3) k_rhs_ru<WR, HS, R3><<<nb, threads, sh, c->stream>>>(g, k, a, zg)
   if (with_rho) {
     if (hs) { if (r3) TPX_K1(true, true, true); else TPX_K1(true, true, false); }
     else { if (r3) TPX_K1(true, false, true); else TPX_K1(true, false, false); }
