@@ -131,12 +131,59 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
 
 
+def ensemble_bench(args):
+    """Ensemble throughput: 64 independent 64^3 boxes (nu ~ U[0.01, 0.03],
+    St ~ logU[0.1, 10], particles ~ logU[1e4, 2e5] per box, mass loading 0.3)
+    stepped together, against the same boxes stepped one after another."""
+    import tpx
+    from tpx import ic
+    t0 = time.perf_counter()
+    cases = ic.ensemble_cases(64, 64, seed=0)
+    solvers = []
+    for c in cases:
+        s = tpx.Solver(c.params, np_capacity=len(c.particles["x"]))
+        s.load_case(c)
+        solvers.append(s)
+    ncells = sum(c.params.Nx_tot * c.params.Ny_tot * c.params.Nz_tot for c in cases)
+    npart = sum(len(c.particles["x"]) for c in cases)
+    log(f"[bench] ensemble of {len(cases)} boxes, {npart} particles, built in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    tpx.ensemble_step(solvers, args.warmup)
+    log(f"[bench] warmup {args.warmup} steps in {time.perf_counter() - t0:.2f} s")
+    best = tpx.tune_concurrency(solvers, candidates=(2, 4, 8, 16), steps=1)
+    log(f"[bench] tuned concurrency: {best}")
+    for s in solvers:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for b, s in enumerate(solvers):
+        s.step(args.steps)
+        if b % 16 == 15:
+            log(f"[bench] sequential: {b + 1} boxes in {time.perf_counter() - t0:.2f} s")
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tpx.ensemble_step(solvers, args.steps, best)
+    t_ens = time.perf_counter() - t0
+    value = ncells * args.steps / t_ens
+    seq = ncells * args.steps / t_seq
+    log(f"[bench] sequential {t_seq:.3f} s, ensemble {t_ens:.3f} s")
+    print(json.dumps({
+        "metric": "ensemble cells/s (64 boxes of 64^3)", "value": round(value, 1), "unit": "cells/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * t_ens / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "ensemble: 64 x 64^3 boxes, particles ~ logU[1e4, 2e5], nu ~ U[0.01, 0.03], "
+                               "St ~ logU[0.1, 10], mass loading 0.3, heat off",
+                   "boxes": len(cases), "particles": npart, "concurrency": best},
+        "sequential_cells_per_s": round(seq, 1), "speedup_vs_sequential": round(value / seq, 3),
+        "roofline": None, "cpu_baseline": None}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="heat256", choices=["heat256", "plain128", "plain32"])
+    ap.add_argument("--workload", default="heat256", choices=["heat256", "plain128", "plain32", "ensemble"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--timing-steps", type=int, default=2)
@@ -154,6 +201,10 @@ def main():
 
     import tpx
     from tpx import ic
+
+    if args.workload == "ensemble":
+        ensemble_bench(args)
+        return
 
     t0 = time.perf_counter()
     case = ic.config_case(args.workload, seed=0)

@@ -132,6 +132,14 @@ int tpx_time_advance(tpx_ctx *ctx);                  /* grid.cpp:595-606 */
  * nsteps times, with the substeps fused where no result changes. */
 int tpx_step(tpx_ctx *ctx, int nsteps);
 
+/* ---- a batch of independent boxes (an ensemble; no counterpart in the
+ * reference, which runs one box per job): every context advances nsteps
+ * whole steps exactly as tpx_step would, with up to `concurrency` contexts
+ * in flight at once, each driven by its own host thread on its own stream
+ * (<= 0: 16).  Contexts must be distinct.  Returns the first error any
+ * context reported; tpx_last_error of that context has the message. */
+int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency);
+
 /* ---- reductions of tensor0 (tensor0.cpp:364-396,466-478) on a field */
 int tpx_field_sum(tpx_ctx *ctx, int field, int comp, double *out);
 int tpx_field_max_abs(tpx_ctx *ctx, int field, int comp, double *out);

@@ -1,0 +1,41 @@
+"""The ensemble entry point (a batch of independent boxes stepped
+concurrently): every box must equal its own plain run bit for bit."""
+import numpy as np
+import pytest
+
+import tpx
+from helpers import solver_from_case
+from tpx import ic
+
+pytestmark = pytest.mark.gpu
+
+
+def test_ensemble_boxes_equal_their_own_runs(gpu_available):
+    cases = ic.ensemble_cases(n_boxes=5, n=16, seed=3, np_range=(200, 3000))
+    assert len({len(c.particles["x"]) for c in cases}) > 1
+    ens = [solver_from_case(c) for c in cases]
+    solo = [solver_from_case(c) for c in cases]
+    tpx.ensemble_step(ens, 3, concurrency=3)
+    for s in solo:
+        s.step(3)
+    for b, (a, s) in enumerate(zip(ens, solo)):
+        for f in ("RU", "Rho", "P"):
+            assert np.array_equal(a.get(f), s.get(f)), (b, f)
+        pa, ps = a.particles(), s.particles()
+        for q in ("x", "u", "T", "flag"):
+            assert np.array_equal(pa[q], ps[q]), (b, q)
+        assert a.scalars()["num_timestep"] == 3
+
+
+def test_ensemble_rejects_repeated_contexts(gpu_available):
+    cases = ic.ensemble_cases(n_boxes=2, n=8, seed=4, np_range=(100, 200))
+    s = [solver_from_case(c) for c in cases]
+    with pytest.raises(tpx.TpxError):
+        tpx.ensemble_step([s[0], s[1], s[0]], 1)
+
+
+def test_tuner_returns_a_candidate(gpu_available):
+    cases = ic.ensemble_cases(n_boxes=4, n=16, seed=5, np_range=(200, 1000))
+    s = [solver_from_case(c) for c in cases]
+    c = tpx.tune_concurrency(s, candidates=(1, 2, 4), steps=1)
+    assert c in (1, 2, 4)

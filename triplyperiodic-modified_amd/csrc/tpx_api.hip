@@ -5,8 +5,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <initializer_list>
 #include <stdexcept>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -417,7 +419,7 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
   try {
     c->p = *p;
     hip_check(hipGetDevice(&c->device), "hipGetDevice");
-    hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     Geo &g = c->g;
     g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
     g.nxy = (long)g.nx * g.ny;
@@ -782,6 +784,32 @@ int tpx_step(tpx_ctx *ctx, int nsteps) {
     sync(ctx);
     return TPX_OK;
   });
+}
+
+int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency) {
+  if (!ctxs || n < 0 || nsteps < 0) return TPX_E_ARG;
+  std::vector<tpx_ctx *> seen(ctxs, ctxs + n);
+  std::sort(seen.begin(), seen.end());
+  for (int i = 0; i < n; ++i)
+    if (!seen[i] || (i > 0 && seen[i] == seen[i - 1])) return TPX_E_ARG;  // null or repeated
+  const int workers = std::max(1, std::min(n, concurrency > 0 ? concurrency : 16));
+  std::atomic<int> next{0};
+  std::atomic<int> first_err{TPX_OK};
+  auto work = [&]() {
+    for (int idx = next.fetch_add(1); idx < n; idx = next.fetch_add(1)) {
+      tpx_ctx *c = ctxs[idx];
+      int rc = hipSetDevice(c->device) == hipSuccess ? tpx_step(c, nsteps) : TPX_E_HIP;
+      if (rc != TPX_OK) {
+        int expected = TPX_OK;
+        first_err.compare_exchange_strong(expected, rc);
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int w = 1; w < workers; ++w) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  return first_err.load();
 }
 
 int tpx_field_sum(tpx_ctx *ctx, int field, int comp, double *out) {

@@ -37,7 +37,8 @@ EXPORTS = [
     "tpx_get_scalars", "tpx_initialize", "tpx_params_update", "tpx_update_rho",
     "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop", "tpx_compute_div_u_new",
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
-    "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_field_sum", "tpx_field_max_abs",
+    "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step", "tpx_field_sum",
+    "tpx_field_max_abs",
     "tpx_deposit_concentration", "tpx_get_stream", "tpx_timing_enable", "tpx_timing_read",
     "tpx_synchronize",
 ]
@@ -86,6 +87,7 @@ def load(path: str = LIB_PATH):
                "tpx_compute_div_u_new", "tpx_compute_rhs_pois", "tpx_update_ru_wp", "tpx_step"):
         getattr(L, fn).argtypes = [P, I]
     L.tpx_solve_poisson.argtypes = [P, I, ctypes.POINTER(I)]
+    L.tpx_ensemble_step.argtypes = [ctypes.POINTER(P), I, I, I]
     L.tpx_make_mean_u0.argtypes = [P, DP]
     L.tpx_field_sum.argtypes = [P, I, I, DP]
     L.tpx_field_max_abs.argtypes = [P, I, I, DP]
@@ -288,3 +290,36 @@ class Solver:
                     fn(rk)
         self.make_mean_u0([p.U0 * p.Rho0, p.V0 * p.Rho0, p.W0 * p.Rho0])
         self.time_advance()
+
+
+# ---------------------------------------------------------------------------
+# ensembles of independent boxes
+# ---------------------------------------------------------------------------
+def ensemble_step(solvers, nsteps: int = 1, concurrency: int = 0) -> None:
+    """Advance every solver nsteps whole steps (each exactly as Solver.step),
+    with up to `concurrency` boxes in flight at once (0: the library's
+    default)."""
+    L = load()
+    arr = (ctypes.c_void_p * len(solvers))(*[s.h for s in solvers])
+    rc = L.tpx_ensemble_step(arr, len(solvers), int(nsteps), int(concurrency))
+    if rc != 0:
+        msgs = [m for m in ((L.tpx_last_error(s.h) or b"").decode() for s in solvers) if m]
+        raise TpxError(f"ensemble_step failed ({rc}): {msgs[0] if msgs else 'invalid arguments'}")
+
+
+def tune_concurrency(solvers, candidates=(2, 4, 8, 16, 32), steps: int = 1) -> int:
+    """Pick the number of boxes in flight that gives the highest throughput:
+    each candidate runs `steps` whole steps of the ensemble (advancing it)."""
+    import time
+    best, best_t = None, None
+    for c in candidates:
+        if c > len(solvers) and best is not None:
+            break
+        for s in solvers:
+            s.synchronize()
+        t0 = time.perf_counter()
+        ensemble_step(solvers, steps, c)
+        t = time.perf_counter() - t0
+        if best_t is None or t < best_t:
+            best, best_t = c, t
+    return best

@@ -223,7 +223,7 @@ class Case:
 def make_case(n: int, n_particles: int, nu: float, seed: int = 0, St=1.0, phi: float = 0.2,
               heat: bool = False, twc: bool = True, k0: float = 4.0, cfl: float = 0.3,
               eps_range: Optional[Tuple[float, float]] = None, forcing: bool = True,
-              L: float = 2 * PI, nsteps: int = 20) -> Case:
+              L: float = 2 * PI, nsteps: int = 20, drag_limit: bool = False) -> Case:
     """Synthetic case following the benchmark recipe: spectral IC (k0, u'=1),
     CFL-0.3 dt, Lundgren forcing A = eps0/(3 u'^2), particles at seeded
     uniform positions with the gas velocity interpolated to them, Stokes
@@ -264,6 +264,16 @@ def make_case(n: int, n_particles: int, nu: float, seed: int = 0, St=1.0, phi: f
     else:
         p.np0 = 0.0
     p.Cvp = p.Cv
+    if drag_limit and npart > 0:
+        # Two-way coupled drag relaxes a particle and the gas of its cell
+        # together at rate (1 + mp / (rho0 Vcell)) / tau; keep dt times that
+        # rate at 1, well inside RK4's real-axis stability limit (2.78) to
+        # leave room for cells holding several particles, so boxes whose
+        # particles outweigh a cell's gas do not blow up.
+        mcell = rho0 * vol / (n ** 3)
+        tau_min = float(np.min(st_arr)) * tau_eta if st_arr.ndim else tau_ref
+        p.dt = min(p.dt, tau_min / (1.0 + (phi * rho0 * vol / npart) / mcell))
+        p.T_final = p.dt * nsteps
     info = dict(eps0=eps0, tau_eta=tau_eta, uprime=1.0, seed=seed, St=st_ref, phi=phi)
     if heat:
         p.Nu = 2.0
@@ -308,3 +318,19 @@ def config_case(name: str, seed: int = 0, scale: int = 1) -> Case:
         return make_case(n, npart, nu=0.0025 * scale, seed=seed, St=1.0, phi=1.0, heat=True,
                          eps_range=(0.5, 1.0), nsteps=200)
     raise KeyError(name)
+
+
+def ensemble_cases(n_boxes: int = 64, n: int = 64, seed: int = 0, np_range=(1e4, 2e5),
+                   nu_range=(0.01, 0.03), st_range=(0.1, 10.0), phi: float = 0.3,
+                   nsteps: int = 50):
+    """The ensemble configuration: n_boxes independent n^3 boxes, box b with
+    its own seed, viscosity nu ~ U[nu_range], Stokes number St ~
+    logU[st_range] and particle count ~ logU[np_range]; mass loading phi,
+    heat off.  With few particles each one outweighs the gas of a cell, so
+    dt also obeys the coupled-drag limit of make_case(drag_limit=True)."""
+    rng = np.random.default_rng(seed + 4242)
+    nus = rng.uniform(nu_range[0], nu_range[1], n_boxes)
+    sts = loguniform(rng, st_range[0], st_range[1], n_boxes)
+    nps = np.round(loguniform(rng, np_range[0], np_range[1], n_boxes)).astype(int)
+    return [make_case(n, int(nps[b]), nu=float(nus[b]), seed=seed + 1000 * b, St=float(sts[b]), phi=phi,
+                      nsteps=nsteps, drag_limit=True) for b in range(n_boxes)]
