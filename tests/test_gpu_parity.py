@@ -230,3 +230,35 @@ def test_fft_lengths_noncubic(gpu_available, shape):
          "P": rel_err(s.get("P"), o.get("P"))}
     print(shape, {k: f"{v:.2e}" for k, v in e.items()}, "iterations", o.scalars()["num_iteration"])
     assert max(e.values()) <= TOL_1, e
+
+
+def test_long_run_diagnostics_within_2_percent(gpu_available):
+    """Beyond pointwise parity: after 200 steps of the 32^3 configuration the
+    energy spectrum (every shell holding at least 1e-4 of the energy), the
+    dissipation 2 nu <S_ij S_ij> and the particle rms velocity of the device
+    run agree with the oracle's within 2%."""
+    case = ic.config_case("plain32", seed=0)
+    o, s = oracle_from_case(case), solver_from_case(case)
+    o.step(200)
+    s.step(200)
+    p = case.params
+    L3_ = (p.Lx, p.Ly, p.Lz)
+    uo = o.get("RU") / 1.0
+    us = s.get("RU") / 1.0
+    ro, rs = o.get("Rho"), s.get("Rho")
+    # U = RU / Rho_face on each component's faces
+    def vel(ru, rho):
+        return np.stack([ru[0] / (0.5 * (rho + np.roll(rho, 1, axis=2))),
+                         ru[1] / (0.5 * (rho + np.roll(rho, 1, axis=1))),
+                         ru[2] / (0.5 * (rho + np.roll(rho, 1, axis=0)))])
+    vo, vs = vel(uo, ro), vel(us, rs)
+    eo, es = ic.energy_spectrum(vo, L3_), ic.energy_spectrum(vs, L3_)
+    shells = eo > 1e-4 * eo.sum()
+    spec_err = float(np.max(np.abs(es[shells] - eo[shells]) / eo[shells]))
+    nu = p.Mu0 / p.Rho0
+    d_err = abs(ic.dissipation(vs, nu, L3_) - ic.dissipation(vo, nu, L3_)) / ic.dissipation(vo, nu, L3_)
+    r_err = abs(ic.particle_rms(s.particles()) - ic.particle_rms(o.particles())) / ic.particle_rms(o.particles())
+    point = rel_err(us, uo)
+    print(f"200 steps: spectrum {spec_err:.2e} dissipation {d_err:.2e} particle rms {r_err:.2e} "
+          f"(pointwise RU {point:.2e})")
+    assert spec_err <= 0.02 and d_err <= 0.02 and r_err <= 0.02

@@ -552,3 +552,18 @@ def test_g14_initial_heat_source():
     parts["T"] = np.full(600, T_gas)
     o = prepared(p, 600, parts)
     assert np.max(np.abs(o.get("RHS_Part_Temp"))) < 1e-15
+
+
+def test_diagnostics_spectrum_parseval_and_shells():
+    """E(k) sums to <u.u>/2, and a single Fourier mode lands in its shell."""
+    rng = np.random.default_rng(30)
+    u = rng.standard_normal((3, N, N, N))
+    e = ic.energy_spectrum(u, (L, L, L))
+    assert abs(e.sum() - 0.5 * np.mean(np.sum(u * u, axis=0))) < 1e-12 * e.sum()
+    xc, yc, zc, _, _, _ = grid_coords()
+    m = np.zeros((3, N, N, N))
+    m[0] = np.cos(3 * xc + 4 * yc)        # |k| = 5
+    e = ic.energy_spectrum(m, (L, L, L))
+    assert np.argmax(e) == 5 and abs(e[5] - 0.25) < 1e-12 and e.sum() - e[5] < 1e-20
+    parts = {"u": np.full(10, 1.0), "v": np.full(10, 2.0), "w": np.full(10, 2.0)}
+    assert abs(ic.particle_rms(parts) - math.sqrt(3.0)) < 1e-15

@@ -128,6 +128,30 @@ def dissipation(u: np.ndarray, nu: float, L) -> float:
     return 2.0 * nu * sq
 
 
+def energy_spectrum(u: np.ndarray, L) -> np.ndarray:
+    """Shell-binned kinetic energy spectrum E(k) of a velocity field (each
+    staggered component transformed on its own grid), shells round(|k|)
+    with k in units of 2 pi / L (cubic box).  Normalised so that
+    sum_k E(k) = <u.u>/2 (Parseval)."""
+    nz, ny, nx = u.shape[1:]
+    kx = np.fft.fftfreq(nx, 1.0 / nx)
+    ky = np.fft.fftfreq(ny, 1.0 / ny)
+    kz = np.fft.fftfreq(nz, 1.0 / nz)
+    K = np.sqrt(kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    shell = np.rint(K).astype(int)
+    e = np.zeros(shell.max() + 1)
+    n = nx * ny * nz
+    for c in range(3):
+        uh = np.fft.fftn(u[c]) / n
+        np.add.at(e, shell.ravel(), 0.5 * (np.abs(uh) ** 2).ravel())
+    return e
+
+
+def particle_rms(parts) -> float:
+    """sqrt(<u_p^2 + v_p^2 + w_p^2> / 3) over the particles."""
+    return float(np.sqrt(np.mean(parts["u"] ** 2 + parts["v"] ** 2 + parts["w"] ** 2) / 3.0))
+
+
 def gather_face_velocity(u: np.ndarray, x, y, z, L) -> np.ndarray:
     """P1 interpolation (particle.cpp:340-360): linear along the component's
     own axis from the floor face, piecewise constant across."""
