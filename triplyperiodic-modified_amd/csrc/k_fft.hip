@@ -145,11 +145,23 @@ struct XArgs {
   double ncell;
 };
 
+// 1/x from the hardware reciprocal estimate and two Newton steps,
+// r <- r + r (1 - x r): the relative error squares each step, so the result
+// is within an ulp or so of the correctly rounded quotient, for a fraction
+// of the instructions of an IEEE division.  Used only inside the Poisson
+// iteration, whose result is compared with a tolerance (it is an FFT).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
 // C on face A of cell (i,j,k): 1./Rho_face (grid.cpp:853)
 template <int A>
 __device__ __forceinline__ double Cface(const Ix &ix, const double *__restrict__ rho, int i, int j, int k) {
   long p = ix(i, j, k), q = ix(i - (A == 0), j - (A == 1), k - (A == 2));
-  return 1. / ((rho[p] + rho[q]) * 0.5);
+  return rcp_nr((rho[p] + rho[q]) * 0.5);
 }
 
 // residual r = (div_F2C(C grad_C2F P) - RHS) * cc at cell (i,j,k)
@@ -161,7 +173,7 @@ __device__ __forceinline__ double residual(const XArgs &a, const Ix &ix, int i, 
   double cy0 = Cface<1>(ix, rho, i, j, k), cy1 = Cface<1>(ix, rho, i, j + 1, k);
   double cz0 = Cface<2>(ix, rho, i, j, k), cz1 = Cface<2>(ix, rho, i, j, k + 1);
   double third = 1. / 3.;
-  double cc = -1. / (third * ((cx0 + cx1) * 0.5 + (cy0 + cy1) * 0.5 + (cz0 + cz1) * 0.5));
+  double cc = -rcp_nr(third * ((cx0 + cx1) * 0.5 + (cy0 + cy1) * 0.5 + (cz0 + cz1) * 0.5));
   double pc = P[ix(i, j, k)];
   double gx0 = ((pc - P[ix(i - 1, j, k)]) * g.idx) * cx0, gx1 = ((P[ix(i + 1, j, k)] - pc) * g.idx) * cx1;
   double gy0 = ((pc - P[ix(i, j - 1, k)]) * g.idy) * cy0, gy1 = ((P[ix(i, j + 1, k)] - pc) * g.idy) * cy1;
@@ -385,7 +397,9 @@ __device__ __forceinline__ void step_a(double2 (&v)[R2], int t, const double2 *_
 // X forward, four-step: C lines per block, R1 work items per line in step A
 template <int R1, int R2>
 __global__ __launch_bounds__(256) void k_xfwd4(XArgs a, int C) {
-  extern __shared__ double2 lds[];
+  // The exchange moves real and imaginary parts in two rounds through an
+  // LDS buffer of doubles: half the LDS per block, more blocks per CU.
+  extern __shared__ double ldsd[];
   constexpr int n = R1 * R2, P = R1 + 1;  // padded row of the exchange
   const Geo &g = a.g;
   const int mxh = n / 2 + 1;
@@ -394,25 +408,34 @@ __global__ __launch_bounds__(256) void k_xfwd4(XArgs a, int C) {
   const double mean = a.scal[SC_SUM_RHS] / a.ncell;
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const long line0 = (long)blk * C;
+  const int ta = threadIdx.x % R1, ca = threadIdx.x / R1;
+  double2 v[R2];
   {
-    const int t = threadIdx.x % R1, c = threadIdx.x / R1;
-    const long line = line0 + c;
-    double2 v[R2];
+    const long line = line0 + ca;
     const bool live = line < nlines;
     const int j = (int)(line % g.ny), k = (int)(line / g.ny);
 #pragma unroll
-    for (int m = 0; m < R2; ++m) v[m] = make_double2(live ? residual(a, ix, t + R1 * m, j, k, mean) : 0.0, 0.0);
-    step_a<R2, false>(v, t, a.tw);
+    for (int m = 0; m < R2; ++m) v[m] = make_double2(live ? residual(a, ix, ta + R1 * m, j, k, mean) : 0.0, 0.0);
+    step_a<R2, false>(v, ta, a.tw);
+  }
+  const bool b_item = (int)threadIdx.x < C * R2;
+  const int k2 = threadIdx.x % R2, cb = threadIdx.x / R2;
+  double2 u[R1];
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) lds[(c * R2 + k2) * P + t] = v[k2];
+  for (int q = 0; q < R2; ++q) ldsd[(ca * R2 + q) * P + ta] = v[q].x;
+  __syncthreads();
+  if (b_item) {
+#pragma unroll
+    for (int t = 0; t < R1; ++t) u[t].x = ldsd[(cb * R2 + k2) * P + t];
   }
   __syncthreads();
-  if ((int)threadIdx.x < C * R2) {
-    const int k2 = threadIdx.x % R2, c = threadIdx.x / R2;
-    const long line = line0 + c;
-    double2 u[R1];
 #pragma unroll
-    for (int t = 0; t < R1; ++t) u[t] = lds[(c * R2 + k2) * P + t];
+  for (int q = 0; q < R2; ++q) ldsd[(ca * R2 + q) * P + ta] = v[q].y;
+  __syncthreads();
+  if (b_item) {
+    const long line = line0 + cb;
+#pragma unroll
+    for (int t = 0; t < R1; ++t) u[t].y = ldsd[(cb * R2 + k2) * P + t];
     RegDft<R1, false>::run(u);
     if (line < nlines) {
 #pragma unroll
@@ -691,7 +714,7 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
   const size_t shx4 = x4 ? sizeof(double2) * (size_t)cx * r2 * (r1 + 1) : 0;
   const size_t shx = sizeof(double2) * (size_t)g.nx * cx;
   if (x4) {
-#define L_XF(A, B) k_xfwd4<A, B><<<nbx, cx * A, shx4, c->stream>>>(xa, cx)
+#define L_XF(A, B) k_xfwd4<A, B><<<nbx, cx * A, shx4 / 2, c->stream>>>(xa, cx)
     TPX_FOUR_STEP_SWITCH(g.nx, L_XF)
 #undef L_XF
   } else {
