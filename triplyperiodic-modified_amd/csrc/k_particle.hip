@@ -5,8 +5,8 @@
 // the floor cell of x_new with a counting sort: the advance kernel counts
 // each particle into its cell with an integer atomic (no return value, so
 // no wave waits on it), an exclusive scan turns the counts into CSR offsets,
-// a placement pass takes slots from per-cell cursors, and a per-cell
-// insertion sort puts each cell's particles in increasing physical index.
+// a placement pass takes slots from per-cell cursors, and a per-cell sort
+// puts each cell's particles in increasing physical index.
 // Integer counts do not depend on the order the atomics took, and the final
 // sort removes the one thing that does, so the bins are identical on every
 // run.
@@ -25,6 +25,7 @@
 //     LDS, the z shifts in a register window while the block marches along
 //     z, and a last pass adds the y shifts.
 #include <math.h>
+#include <stdint.h>
 
 #include <algorithm>
 #include <stdexcept>
@@ -192,17 +193,48 @@ __global__ void k_copy_i(int32_t *__restrict__ d, const int32_t *__restrict__ s,
 
 // placement: each particle takes the next free slot of its cell's segment
 // (cursor starts as a copy of the CSR offsets)
-__global__ void k_place(const int32_t *__restrict__ key, int32_t *cursor, int32_t *perm, long n) {
+// placement: each particle takes the next free slot of its cell's segment
+// (cursor starts as a copy of the CSR offsets)
+__global__ __launch_bounds__(kBlock) void k_place(const int32_t *__restrict__ key, int32_t *__restrict__ cursor,
+                                                  int32_t *__restrict__ perm, long n) {
   long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p < n) perm[atomicAdd(&cursor[key[p]], 1)] = (int32_t)p;
 }
 
-// per-cell insertion sort of the placed indices: the result is the stable
-// order (by physical index) whatever order the counting atomics took
+__device__ __forceinline__ void cswap(int32_t &a, int32_t &b) {
+  int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
+
+// per-cell sort of the placed indices: the result is the stable order (by
+// physical index) whatever order the placement atomics took.  Segments of
+// up to 8 are sorted in registers by a fixed compare-exchange network
+// (padded with INT32_MAX); longer ones by insertion in memory.
 __global__ void k_fixup(const int32_t *__restrict__ start, int32_t *perm, int32_t *inv, long ncell) {
   long c = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   int32_t b = start[c], e = start[c + 1];
+  const int cnt = e - b;
+  if (cnt <= 8) {
+    int32_t v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = q < cnt ? perm[b + q] : INT32_MAX;
+    // Batcher odd-even merge sort of 8 (19 compare-exchanges)
+    cswap(v[0], v[1]); cswap(v[2], v[3]); cswap(v[4], v[5]); cswap(v[6], v[7]);
+    cswap(v[0], v[2]); cswap(v[1], v[3]); cswap(v[4], v[6]); cswap(v[5], v[7]);
+    cswap(v[1], v[2]); cswap(v[5], v[6]);
+    cswap(v[0], v[4]); cswap(v[1], v[5]); cswap(v[2], v[6]); cswap(v[3], v[7]);
+    cswap(v[2], v[4]); cswap(v[3], v[5]);
+    cswap(v[1], v[2]); cswap(v[3], v[4]); cswap(v[5], v[6]);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < cnt) {
+        perm[b + q] = v[q];
+        inv[v[q]] = b + q;
+      }
+    return;
+  }
   for (int32_t q = b + 1; q < e; ++q) {
     int32_t v = perm[q];
     int32_t r = q - 1;
@@ -278,9 +310,7 @@ struct TempArgs {
 // at the particle (particle.cpp:402-456), RHS_T = (Tg - T_p) Nu pi Dp k
 // (particle.cpp:460-474); also emits the deposit record of the particle at
 // its sorted position
-__global__ __launch_bounds__(kBlock) void k_temp_gather(TempArgs a) {
-  long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
+__device__ __forceinline__ void temp_gather_one(const TempArgs &a, long p) {
   const Geo &g = a.g;
   Ix ix = make_ix(g);
   double x = a.x[p], y = a.y[p], z = a.z[p];
@@ -301,6 +331,11 @@ __global__ __launch_bounds__(kBlock) void k_temp_gather(TempArgs a) {
   long q = a.inv[p];
   a.rec[q] = make_double4(xl, yl, zl, v);
   a.rkey[q] = (ii & ix.mx) | ((jj & ix.my) << 10) | ((kk & ix.mz) << 20);
+}
+
+__global__ __launch_bounds__(kBlock) void k_temp_gather(TempArgs a) {
+  const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (p < a.n) temp_gather_one(a, p);
 }
 
 // ---- heat deposit by per-cell moments (P9/P10) -----------------------------
