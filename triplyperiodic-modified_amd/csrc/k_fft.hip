@@ -625,10 +625,8 @@ __global__ __launch_bounds__(256) void k_zpass4(double2 *spec, int mxh, int ny, 
 __global__ __launch_bounds__(1024) void k_finish_max(const double *__restrict__ partials, long nparts,
                                                      double *scal, int slot) {
   __shared__ double sh[1024];
-  long per = (nparts + 1023) / 1024;
-  long b = (long)threadIdx.x * per, e = b + per < nparts ? b + per : nparts;
   double v = 0.0;
-  for (long q = b; q < e; ++q) v = fmax(v, partials[q]);
+  for (long q = threadIdx.x; q < nparts; q += 1024) v = fmax(v, partials[q]);  // coalesced
   sh[threadIdx.x] = v;
   __syncthreads();
   for (int s = 512; s > 0; s >>= 1) {

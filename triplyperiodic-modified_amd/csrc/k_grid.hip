@@ -432,15 +432,24 @@ __global__ __launch_bounds__(kBlock) void k_reduce(long n, const double *__restr
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// sum (OP 0) or max (OP 1) of nparts partials into scal[slot], fixed order
+// sum (OP 0) or max (OP 1) of nparts partials into scal[slot], fixed order:
+// thread t combines partials t, t + 1024, t + 2048, ... (coalesced reads,
+// four independent accumulators), then the block tree
 template <int OP>
 __global__ __launch_bounds__(1024) void k_finish(const double *__restrict__ partials, long nparts,
                                                  double *scal, int slot) {
   __shared__ double sh[1024];
-  long per = (nparts + 1023) / 1024;
-  long b = (long)threadIdx.x * per, e = b + per < nparts ? b + per : nparts;
-  double v = 0.0;
-  for (long q = b; q < e; ++q) v = OP == 0 ? v + partials[q] : fmax(v, partials[q]);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  long q = threadIdx.x;
+  for (; q + 3 * 1024 < nparts; q += 4 * 1024) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double x = partials[q + u * 1024];
+      acc[u] = OP == 0 ? acc[u] + x : fmax(acc[u], x);
+    }
+  }
+  for (int u = 0; q < nparts; q += 1024, ++u) acc[u] = OP == 0 ? acc[u] + partials[q] : fmax(acc[u], partials[q]);
+  double v = OP == 0 ? (acc[0] + acc[1]) + (acc[2] + acc[3]) : fmax(fmax(acc[0], acc[1]), fmax(acc[2], acc[3]));
   double s = OP == 0 ? block_sum(v, sh) : block_max(v, sh);
   if (threadIdx.x == 0) scal[slot] = s;
 }
