@@ -232,6 +232,38 @@ def test_fft_lengths_noncubic(gpu_available, shape):
     assert max(e.values()) <= TOL_1, e
 
 
+@pytest.mark.parametrize("shape", [(16, 512, 8), (8, 16, 512), (512, 8, 16)])
+def test_noncubic_particles_with_heat(gpu_available, shape):
+    """Particle binning, the fused temperature/heat deposit and the velocity
+    deposit take their launch shape from the x-row length (8 rows of 32 up to
+    one row of 512 threads): two heated, two-way coupled steps on non-cubic
+    boxes must match the oracle."""
+    case = _noncubic_case(shape, seed=3 + sum(shape))
+    p, (nz, ny, nx) = case.params, shape
+    rng = np.random.default_rng(11)
+    npart = 4000
+    vol = p.Lx * p.Ly * p.Lz
+    mp = 0.2 * vol / (nx * ny * nz)  # a fifth of a cell's gas
+    p.TWC, p.Nu = 1, 2.0
+    p.Dp = mp / (3 * np.pi * p.Mu0 * 0.05)
+    p.Rhop = 6 * mp / (np.pi * p.Dp ** 3)
+    p.np0 = (npart + 0.5) / vol
+    p.Cvp = p.Cv
+    p.I01 = p.I02 = 4 * 0.5 * p.T0 * p.Nu * np.pi * p.Dp * p.k / (0.75 * np.pi * p.Dp ** 2)
+    parts = {"x": rng.uniform(0, p.Lx, npart), "y": rng.uniform(0, p.Ly, npart), "z": rng.uniform(0, p.Lz, npart)}
+    for q in "uvw":
+        parts[q] = 0.3 * rng.standard_normal(npart)
+    parts["T"] = np.full(npart, p.T0)
+    parts["flag"] = np.arange(npart, dtype=np.int32)
+    case = ic.Case(p, 0.2 * case.RU, case.Rho, case.P, parts)  # gentler flow on the thin boxes
+    o, s = oracle_from_case(case), solver_from_case(case)
+    o.step(2)
+    s.step(2)
+    e = state_errors(o, s, case)
+    print(shape, {k: f"{v:.2e}" for k, v in e.items()})
+    assert max(e.values()) <= TOL_1, e
+
+
 def test_long_run_diagnostics_within_2_percent(gpu_available):
     """Beyond pointwise parity: after 200 steps of the 32^3 configuration the
     energy spectrum (every shell holding at least 1e-4 of the energy), the

@@ -421,6 +421,168 @@ __global__ __launch_bounds__(kHeatMaxBlock) void k_heat_planes(HeatArgs a) {
   }
 }
 
+// ---- fused temperature gather + heat deposit (P8/P10 gather, P9 deposit) ----
+// The heat moment kernel above, with the gather folded in: for each particle
+// of its base cell (in the cell's sorted order) the thread computes the
+// cloud-in-cell fractions at the particle, the gas temperature T = (P0/R)/Rho
+// from three planes of T staged in LDS (one division per cell instead of 8
+// per particle), RHS_T = (Tg - T_p) Nu pi Dp k (particle.cpp:460-474), stores
+// RHS_T by particle and accumulates the 27 moments.  No per-particle record
+// goes through memory.
+struct HeatFusedArgs {
+  Geo g;
+  const int32_t *start;      // CSR of the stage's cells
+  const int32_t *perm;       // sorted position -> particle
+  const double *x, *y, *z, *Tp;
+  const double *rho;
+  const double *scal;
+  int p0_slot;
+  double R, iTth;
+  double *rhs_t;             // by particle
+  double *t3;                // 3 x n: contribution of row j to row j - 1, j, j + 1
+  int rows, zc;
+};
+
+// The block works on whole x-rows of one base plane at a time; the cells of
+// those rows are consecutive, so their particles fill one consecutive range
+// of sorted positions.  That range is taken in chunks of kHeatChunk x
+// blockDim particles: every thread gathers T for its particles of the chunk
+// (one or two, loads issued together) and leaves a record (cloud-in-cell
+// fractions, RHS_T, upper-cell bits) in LDS, then every thread sums the
+// moments of its own cell's records.
+constexpr int kHeatChunk = 2;
+
+__global__ __launch_bounds__(kHeatMaxBlock) void k_heat_fused(HeatFusedArgs a) {
+  // LDS: 9 x blockDim doubles shared by the records of a chunk and the x
+  // exchange, then T: 3 planes x (rows + 2) x nx
+  extern __shared__ double xs[];
+  const Geo &g = a.g;
+  Ix ix = make_ix(g);
+  const int nx = g.nx, bd = (int)blockDim.x, t = (int)threadIdx.x;
+  const int i = t & ix.mx, r = t / nx;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int nyb = g.ny / a.rows;
+  const int j0 = (int)(b % (unsigned)nyb) * a.rows, j = j0 + r;
+  const int z0 = (int)(b / (unsigned)nyb) * a.zc;
+  const int il = r * nx + ((i - 1) & ix.mx), ir = r * nx + ((i + 1) & ix.mx);
+  double4 *rec = reinterpret_cast<double4 *>(xs);                        // kHeatChunk x blockDim
+  int32_t *rbit = reinterpret_cast<int32_t *>(xs + 4 * kHeatChunk * bd);  // kHeatChunk x blockDim
+  double *tt = xs + 9 * bd;
+  const int TPL = (a.rows + 2) * nx;  // doubles per staged plane
+  const double at = a.scal[a.p0_slot] / a.R;
+  // T of plane kz into ring slot s: rows j0-1 .. j0+rows
+  auto stage_T = [&](int kz, int s) {
+    const long zoff = (long)(kz & ix.mz) * g.nxy;
+    for (int e = t; e < TPL; e += bd) {
+      const int row = e / nx, col = e - row * nx;
+      tt[s * TPL + e] = at / a.rho[zoff + (long)((j0 + row - 1) & ix.my) * nx + col];
+    }
+  };
+  auto slot_of = [&](int kz) { return (kz - z0 + 3) % 3; };  // kz >= z0 - 2
+  stage_T(z0 - 2, slot_of(z0 - 2));
+  stage_T(z0 - 1, slot_of(z0 - 1));
+  stage_T(z0, slot_of(z0));
+  __syncthreads();
+  double sa[3] = {0., 0., 0.}, sb[3] = {0., 0., 0.};
+  for (int zz = -1; zz <= a.zc; ++zz) {
+    const int kabs = z0 + zz, k = kabs & ix.mz;
+    const int sm = slot_of(kabs - 1), s0 = slot_of(kabs), sp = slot_of(kabs + 1);
+    const long cb = ((long)k * g.ny + j0) * nx;  // first cell of the block's rows
+    const int32_t Qa = a.start[cb], Qb = a.start[cb + bd];
+    const int32_t q0 = a.start[cb + t], q1 = a.start[cb + t + 1];
+    // gather of one particle at sorted position q into record slot e
+    auto gather = [&](int32_t p, double x, double y, double z, double tp, int e) {
+      const double fx = x * g.idx, fy = y * g.idy, fz = z * g.idz;
+      const int ii = (int)(round(fx)), jj = (int)(round(fy)), kk = (int)(round(fz));
+      const double xl = fx - ii + 0.5, yl = fy - jj + 0.5, zl = fz - kk + 0.5;
+      // base (floor) cell of the particle: the cell it is binned in
+      const int bi = (int)fx, bj = (int)fy, bk = (int)fz;
+      const int ux = ii - bi, uy = jj - bj, uz = kk - bk;
+      const int ci = (bi & ix.mx) + ux, crow = (bj & ix.my) - j0 + 1 + uy;
+      const int s1 = uz ? sp : s0, s2 = uz ? s0 : sm;  // planes kk and kk - 1
+      auto T = [&](int s, int row, int col) { return tt[s * TPL + row * nx + (col & ix.mx)]; };
+      // gas2part_Temp (particle.cpp:402-456): T at (ii - a, jj - b, kk - c)
+      const double Tg = xl * yl * zl * T(s1, crow, ci) +
+                        (1 - xl) * yl * zl * T(s1, crow, ci - 1) + xl * (1 - yl) * zl * T(s1, crow - 1, ci) +
+                        xl * yl * (1 - zl) * T(s2, crow, ci) +
+                        (1 - xl) * (1 - yl) * zl * T(s1, crow - 1, ci - 1) +
+                        (1 - xl) * yl * (1 - zl) * T(s2, crow, ci - 1) +
+                        xl * (yl - 1) * (zl - 1) * T(s2, crow - 1, ci) +
+                        (1 - xl) * (1 - yl) * (1 - zl) * T(s2, crow - 1, ci - 1);
+      const double v = (Tg - tp) * a.iTth;
+      a.rhs_t[p] = v;
+      rec[e] = make_double4(xl, yl, zl, v);
+      rbit[e] = (ux & 1) | ((uy & 1) << 1) | ((uz & 1) << 2);
+    };
+    double m[3][3][3];
+#pragma unroll
+    for (int u = 0; u < 27; ++u) m[u / 9][(u / 3) % 3][u % 3] = 0.0;
+    for (int32_t c0 = Qa; c0 < Qb; c0 += kHeatChunk * bd) {
+      const int32_t ce = min(c0 + kHeatChunk * bd, Qb);
+      int32_t pp[kHeatChunk];
+      double px[kHeatChunk], py[kHeatChunk], pz[kHeatChunk], pt[kHeatChunk];
+#pragma unroll
+      for (int e = 0; e < kHeatChunk; ++e) {
+        const int32_t q = c0 + e * bd + t;
+        pp[e] = q < ce ? a.perm[q] : 0;
+      }
+#pragma unroll
+      for (int e = 0; e < kHeatChunk; ++e)
+        if (c0 + e * bd + t < ce) {
+          px[e] = a.x[pp[e]]; py[e] = a.y[pp[e]]; pz[e] = a.z[pp[e]]; pt[e] = a.Tp[pp[e]];
+        }
+#pragma unroll
+      for (int e = 0; e < kHeatChunk; ++e)
+        if (c0 + e * bd + t < ce) gather(pp[e], px[e], py[e], pz[e], pt[e], e * bd + t);
+      __syncthreads();
+      // the records of cell (i, j, k) inside this chunk, in sorted order
+      const int32_t qe = min(q1, ce);
+      for (int32_t q = max(q0, c0); q < qe; ++q) {
+        const double4 v = rec[q - c0];
+        const int ub = rbit[q - c0];
+        const bool ux = ub & 1, uy = ub & 2, uz = ub & 4;
+        const double wx[3] = {ux ? 0.0 : 1 - v.x, ux ? 1 - v.x : v.x, ux ? v.x : 0.0};
+        const double wy[3] = {uy ? 0.0 : 1 - v.y, uy ? 1 - v.y : v.y, uy ? v.y : 0.0};
+        const double wz[3] = {uz ? 0.0 : 1 - v.z, uz ? 1 - v.z : v.z, uz ? v.z : 0.0};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+          for (int ay = 0; ay < 3; ++ay)
+#pragma unroll
+            for (int az = 0; az < 3; ++az) m[ax][ay][az] += wx[ax] * wy[ay] * wz[az] * v.w;
+      }
+      __syncthreads();  // the records are read before the next chunk or the exchange
+    }
+    // x exchange in two rounds of 9: the right neighbour's offset -1 moments,
+    // then the left neighbour's offset +1 moments
+    double R[3][3];
+#pragma unroll
+    for (int s = 0; s < 9; ++s) xs[s * bd + t] = m[0][s / 3][s % 3];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 9; ++s) R[s / 3][s % 3] = xs[s * bd + ir];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 9; ++s) xs[s * bd + t] = m[2][s / 3][s % 3];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 9; ++s) R[s / 3][s % 3] = (xs[s * bd + il] + m[1][s / 3][s % 3]) + R[s / 3][s % 3];
+    if (zz >= 1) {
+      const long co = ((long)(z0 + zz - 1) * g.ny + j) * nx + i;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) a.t3[e * g.n + co] = sa[e] + R[e][0];
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      sa[e] = sb[e] + R[e][1];
+      sb[e] = R[e][2];
+    }
+    // plane k - 1 is no longer needed: stage plane k + 2 into its slot
+    if (zz < a.zc) stage_T(kabs + 2, sm);
+    __syncthreads();
+  }
+}
+
 // Stage 2: Q(i, j, k) = t3[2](i, j-1, k) + t3[1](i, j, k) + t3[0](i, j+1, k),
 // with block partial sums for the mean of Q
 __global__ __launch_bounds__(kBlock) void k_heat_combine(Geo g, const double *__restrict__ t3, double *out,
@@ -479,6 +641,93 @@ __global__ __launch_bounds__(kBlock) void k_vel_cells(Geo g, const int32_t *__re
   Sx[c] = o0 + n0;
   Sy[c] = o1 + n1;
   Sz[c] = o2 + n2;
+}
+
+// Sums of NV per-particle arrays over the particles of each of `ncell`
+// consecutive cells starting at cell `cb`, one cell per thread (threads past
+// `ncell` get zeros), each sum taken in sorted order from 0.  The cells'
+// particles occupy one consecutive range of sorted positions; it is read in
+// chunks of kVelChunk x blockDim positions, every thread loading its one or
+// two positions of every array (coalesced) into LDS, then every thread adding
+// up its own cell's positions from LDS.  All threads of the block call it.
+constexpr int kVelChunk = 2;
+
+template <int NV>
+__device__ __forceinline__ void cell_sums(const int32_t *__restrict__ start, long cb, int ncell,
+                                          const double *const (&arr)[NV], double *lds, double (&out)[NV]) {
+  const int t = (int)threadIdx.x, bd = (int)blockDim.x, cap = kVelChunk * bd;
+  const int32_t Qa = start[cb], Qb = start[cb + ncell];
+  int32_t q0 = 0, q1 = 0;
+  if (t < ncell) { q0 = start[cb + t]; q1 = start[cb + t + 1]; }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) out[v] = 0.0;
+  for (int32_t c0 = Qa; c0 < Qb; c0 += cap) {
+    const int32_t ce = min(c0 + cap, Qb);
+    double val[kVelChunk][NV];
+#pragma unroll
+    for (int e = 0; e < kVelChunk; ++e) {
+      const int32_t q = c0 + e * bd + t;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) val[e][v] = q < ce ? arr[v][q] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < kVelChunk; ++e)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) lds[v * cap + e * bd + t] = val[e][v];
+    __syncthreads();
+    const int32_t qe = min(q1, ce);
+    for (int32_t q = max(q0, c0); q < qe; ++q)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) out[v] += lds[v * cap + (q - c0)];
+    __syncthreads();  // LDS is free again
+  }
+}
+
+// P6 reduction by planes (particle.cpp:494-520): the block covers `rows`
+// whole x-rows and `zc` planes.  For every cell the sums of the six share
+// arrays over its own particles give the shares of its low faces (c1, c3,
+// c5); the shares of the high faces of the cell below along x, y, z (c0, c2,
+// c4 of cells (i-1, j, k), (i, j-1, k), (i, j, k-1)) come from the
+// neighbouring thread through LDS, from the halo row j0 - 1, and from the
+// previous plane of the march.
+__global__ __launch_bounds__(kHeatMaxBlock) void k_vel_planes(Geo g, const int32_t *__restrict__ start,
+                                                             const double *__restrict__ contrib, long cs,
+                                                             double *Sx, double *Sy, double *Sz, int rows,
+                                                             int zc) {
+  extern __shared__ double ls[];  // 6 x kVelChunk x blockDim
+  Ix ix = make_ix(g);
+  const int nx = g.nx, bd = (int)blockDim.x, t = (int)threadIdx.x;
+  const int i = t & ix.mx, r = t / nx;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int nyb = g.ny / rows;
+  const int j0 = (int)(b % (unsigned)nyb) * rows, j = j0 + r;
+  const int z0 = (int)(b / (unsigned)nyb) * zc;
+  const double *c[6];
+#pragma unroll
+  for (int v = 0; v < 6; ++v) c[v] = contrib + v * cs;
+  const double *const c4[1] = {c[4]};
+  const double *const c2[1] = {c[2]};
+  // c4 sums of plane z0 - 1
+  double below[1];
+  cell_sums<1>(start, ((long)((z0 - 1) & ix.mz) * g.ny + j0) * nx, bd, c4, ls, below);
+  for (int zz = 0; zz < zc; ++zz) {
+    const int k = z0 + zz;
+    double own[6], halo[1];
+    cell_sums<6>(start, ((long)k * g.ny + j0) * nx, bd, c, ls, own);
+    cell_sums<1>(start, ((long)k * g.ny + ((j0 - 1) & ix.my)) * nx, nx, c2, ls, halo);
+    ls[t] = own[0];
+    ls[bd + t] = own[2];
+    if (t < nx) ls[2 * bd + t] = halo[0];
+    __syncthreads();
+    const double nxs = ls[r * nx + ((i - 1) & ix.mx)];
+    const double nys = r > 0 ? ls[bd + (r - 1) * nx + i] : ls[2 * bd + i];
+    const long co = ((long)k * g.ny + j) * nx + i;
+    Sx[co] = own[1] + nxs;
+    Sy[co] = own[3] + nys;
+    Sz[co] = own[5] + below[0];
+    below[0] = own[4];
+    __syncthreads();
+  }
 }
 
 struct CicDepArgs {
@@ -664,11 +913,31 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
   pt.new_counted = count_new;
 }
 
+// Launch shape of the heat moment and velocity deposit kernels: whole x-rows per block (256
+// threads, or nx threads when nx > 256) and up to 32 output planes.
+struct HeatShape { int threads, rows, zc; unsigned blocks; };
+static HeatShape heat_shape(const Geo &g) {
+  HeatShape h;
+  h.rows = g.nx >= 256 ? 1 : std::min(256 / g.nx, g.ny);
+  h.threads = g.nx * h.rows;
+  h.zc = std::min(32, g.nz);
+  h.blocks = (unsigned)((g.ny / h.rows) * (g.nz / h.zc));
+  return h;
+}
+
 void launch_velocity_deposit(tpx_ctx *c) {
   Particles &pt = c->pt;
   timing_begin(c, TPX_K_DEPOSIT);
-  k_vel_cells<<<grid1d(c->g.n, kBlock), kBlock, 0, c->stream>>>(c->g, pt.start_int, pt.contrib, pt.cap, c->S[0],
-                                                                 c->S[1], c->S[2]);
+  const Geo &g = c->g;
+  HeatShape h = heat_shape(g);
+  const int zc = std::min(8, g.nz);
+  if (h.threads > kHeatMaxBlock || g.ny % h.rows || g.nz % zc)
+    throw std::logic_error("velocity deposit launch shape does not tile the grid");
+  const size_t lds = sizeof(double) * 6 * kVelChunk * (size_t)h.threads;
+  if (lds > c->max_lds) throw std::logic_error("velocity deposit needs more LDS than the device offers");
+  const unsigned blocks = (unsigned)((g.ny / h.rows) * (g.nz / zc));
+  k_vel_planes<<<blocks, h.threads, lds, c->stream>>>(g, pt.start_int, pt.contrib, pt.cap, c->S[0], c->S[1],
+                                                        c->S[2], h.rows, zc);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_DEPOSIT);
 }
@@ -724,18 +993,6 @@ void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot) {
   timing_end(c, TPX_K_PARTICLE);
 }
 
-// Launch shape of the heat moment kernel: whole x-rows per block (256
-// threads, or nx threads when nx > 256) and up to 32 output planes.
-struct HeatShape { int threads, rows, zc; unsigned blocks; };
-static HeatShape heat_shape(const Geo &g) {
-  HeatShape h;
-  h.rows = g.nx >= 256 ? 1 : std::min(256 / g.nx, g.ny);
-  h.threads = g.nx * h.rows;
-  h.zc = std::min(32, g.nz);
-  h.blocks = (unsigned)((g.ny / h.rows) * (g.nz / h.zc));
-  return h;
-}
-
 // heat deposit from the records of the last temperature gather
 void launch_temp_deposit(tpx_ctx *c, int stage) {
   Particles &pt = c->pt;
@@ -756,6 +1013,42 @@ void launch_temp_deposit(tpx_ctx *c, int stage) {
     a.zc = h.zc;
     size_t lds = sizeof(double) * 18 * (size_t)h.threads;
     k_heat_planes<<<h.blocks, h.threads, lds, c->stream>>>(a);
+    TPX_LAUNCH_CHECK();
+    unsigned nb = grid1d(g.n, kBlock);
+    k_heat_combine<<<nb, kBlock, 0, c->stream>>>(g, c->dep_tmp, c->Q, c->partials);
+    TPX_LAUNCH_CHECK();
+    launch_finish_sum(c, (int)nb, 0, SC_SUM_Q);
+  }
+  timing_end(c, TPX_K_DEPOSIT);
+}
+
+// temperature gather + heat deposit in one pass over the stage's bins
+void launch_heat_exchange(tpx_ctx *c, int stage, const double *rho, int p0_slot) {
+  Particles &pt = c->pt;
+  timing_begin(c, TPX_K_DEPOSIT);
+  if (pt.n > 0) {
+    const Geo &g = c->g;
+    HeatShape h = heat_shape(g);
+    if (h.threads > kHeatMaxBlock || g.ny % h.rows || g.nz % h.zc)
+      throw std::logic_error("heat deposit launch shape does not tile the grid");
+    const int s = stage == TPX_INT ? pt.int_buf() : pt.st_new;
+    HeatFusedArgs a{};
+    a.g = g;
+    a.start = stage == TPX_INT ? pt.start_int : pt.start_new;
+    a.perm = stage == TPX_INT ? pt.perm_int : pt.perm_new;
+    a.x = pt.q[0][s]; a.y = pt.q[1][s]; a.z = pt.q[2][s]; a.Tp = pt.q[6][s];
+    a.rho = rho;
+    a.scal = c->scal;
+    a.p0_slot = p0_slot;
+    a.R = c->R;
+    a.iTth = c->p.nu * TPX_PI * c->p.dp * c->p.k;
+    a.rhs_t = pt.rhs_t;
+    a.t3 = c->dep_tmp;
+    a.rows = h.rows;
+    a.zc = h.zc;
+    size_t lds = sizeof(double) * (9 * (size_t)h.threads + 3 * (size_t)(h.rows + 2) * g.nx);
+    if (lds > c->max_lds) throw std::logic_error("heat deposit needs more LDS than the device offers");
+    k_heat_fused<<<h.blocks, h.threads, lds, c->stream>>>(a);
     TPX_LAUNCH_CHECK();
     unsigned nb = grid1d(g.n, kBlock);
     k_heat_combine<<<nb, kBlock, 0, c->stream>>>(g, c->dep_tmp, c->Q, c->partials);

@@ -183,8 +183,7 @@ void do_update_particle(tpx_ctx *c, int rk) {
   launch_sort_new(c);
   c->binned_this_step = true;
   if (heat_active(c)) {
-    launch_temp_gather(c, TPX_NEW, c->rho[c->rho_new], SC_P0_NEW);
-    launch_temp_deposit(c, TPX_NEW);
+    launch_heat_exchange(c, TPX_NEW, c->rho[c->rho_new], SC_P0_NEW);
     c->q_zero = false;
   } else {
     launch_fill(c, pt.rhs_t, 0.0, pt.n);
@@ -311,8 +310,7 @@ void fused_substep(tpx_ctx *c, int rk) {
     c->binned_this_step = true;
   }
   if (parts && heat_active(c)) {
-    launch_temp_gather(c, TPX_NEW, c->rho[c->rho_new], SC_P0_NEW);
-    launch_temp_deposit(c, TPX_NEW);
+    launch_heat_exchange(c, TPX_NEW, c->rho[c->rho_new], SC_P0_NEW);
     c->q_zero = false;
   } else if (!c->q_zero) {
     if (parts) launch_fill(c, pt.rhs_t, 0.0, pt.n);
@@ -419,6 +417,10 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
   try {
     c->p = *p;
     hip_check(hipGetDevice(&c->device), "hipGetDevice");
+    int lds = 0;
+    hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device),
+              "hipDeviceGetAttribute");
+    c->max_lds = (size_t)lds;
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     Geo &g = c->g;
     g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
@@ -570,6 +572,13 @@ int tpx_upload_particles(tpx_ctx *ctx, int64_t n, const double *x, const double 
     if (n < 0 || n > pt.cap) throw std::invalid_argument("particle count exceeds the capacity");
     if (n > 0 && (!x || !y || !z || !u || !v || !w || !T || !flag)) throw std::invalid_argument("null particle array");
     const double *src[7] = {x, y, z, u, v, w, T};
+    // the binning and the deposits assume every particle inside the box, as
+    // the periodic wrap keeps it after every substep (particle.cpp:58-104)
+    const double Lb[3] = {ctx->p.lx, ctx->p.ly, ctx->p.lz};
+    for (int d = 0; d < 3; ++d)
+      for (int64_t e = 0; e < n; ++e)
+        if (!(src[d][e] >= 0.0 && src[d][e] <= Lb[d]))
+          throw std::invalid_argument("particle position outside the box [0, L]");
     pt.st_base = 0; pt.st_int = 1; pt.st_new = 2; pt.st_np1 = 3;
     pt.np1_is_base = true;
     pt.int_is_base = false;
@@ -669,8 +678,7 @@ int tpx_initialize(tpx_ctx *ctx) {
       launch_particle_permute(c);
     }
     if (pt.n > 0 && heat_active(c)) {
-      launch_temp_gather(c, TPX_INT, c->rho[c->rho_int], SC_P0_INT);
-      launch_temp_deposit(c, TPX_INT);
+      launch_heat_exchange(c, TPX_INT, c->rho[c->rho_int], SC_P0_INT);
       c->q_zero = false;
     } else {
       if (pt.n > 0) launch_fill(c, pt.rhs_t, 0.0, pt.n);

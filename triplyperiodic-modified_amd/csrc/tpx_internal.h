@@ -99,6 +99,7 @@ struct tpx_ctx {
   int64_t total_iterations = 0;    // summed over all Poisson solves
   hipStream_t stream = nullptr;
   int device = 0;
+  size_t max_lds = 65536;          // dynamic LDS bytes one workgroup may use on this device
   // grid fields
   double *ru[5][3] = {};          // buffer sets for RU
   int ru_base = 0, ru_int = 1, ru_new = 2, ru_np1 = 3;
@@ -165,6 +166,8 @@ void launch_velocity_deposit(tpx_ctx *c);
 void launch_sort_new(tpx_ctx *c);
 void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot);
 void launch_temp_deposit(tpx_ctx *c, int stage);
+// the two above in one pass (writes RHS_T by particle and Q)
+void launch_heat_exchange(tpx_ctx *c, int stage, const double *rho, int p0_slot);
 void launch_concentration(tpx_ctx *c);
 void launch_particle_permute(tpx_ctx *c);
 void launch_sort_stage(tpx_ctx *c, int stage, bool into_int);
