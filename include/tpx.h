@@ -154,8 +154,8 @@ typedef enum tpx_kclass {
   TPX_K_DIV_RHS,        /* low-Mach divergence + Poisson RHS (G5+G6) */
   TPX_K_FFT,            /* Poisson iteration passes (G8/G9) */
   TPX_K_PROJECT,        /* pressure-gradient projection (G10) */
-  TPX_K_PARTICLE,       /* particle temperature gather (P8) */
-  TPX_K_DEPOSIT,        /* deposits (P5, P6, P9) */
+  TPX_K_PARTICLE,       /* kept for numbering: the temperature gather (P8) runs inside TPX_K_DEPOSIT */
+  TPX_K_DEPOSIT,        /* deposits (P5, P6, P9) with the temperature gather (P8) */
   TPX_K_SORT,           /* cell sort of the particles */
   TPX_K_OTHER,          /* reductions, scalars, rotations */
   TPX_K_ADVANCE,        /* particle RK4 advance with its velocity gather (P1-P4, P7) */

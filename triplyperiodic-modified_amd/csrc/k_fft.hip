@@ -1,21 +1,27 @@
 // k_fft.hip -- the FFT-preconditioned Poisson iteration (poisson.cpp:391-462)
-// as five batched LDS radix-2 FFT passes per iteration (gfx950).
+// as three (or five) batched FFT passes per iteration (gfx950).
 //
 // One iteration of the reference is
 //   r  = (div_F2C(C grad_C2F P) - RHS) * cc,   C = 1/Rho_face, cc = -1/I_F2C(C)
 //   dp = -Re IFFT( FFT(r) / lambda(k) ) / N^3   (zero mode passed through)
 //   P += dp,  err = max|dp|
-// with r real.  The passes here are
+// with r real.  When every axis length is 16..256 the iteration is three
+// passes over the half spectrum (k_xy_fwd, k_yz_pass, k_xy_inv below: the y
+// transform is split between them along its four-step factorisation).
+// Otherwise it is five:
 //   X  (forward): residual computed in registers, real-to-half-complex FFT
 //                 along x; only kx = 0..nx/2 is stored ((nx/2+1) ny nz).
-//   Y  (forward): complex FFT along y on tiles of 8 kx columns.
+//   Y  (forward): complex FFT along y on tiles of kx columns.
 //   Z  (both):    forward FFT along z, divide by lambda, inverse FFT along z.
 //   Y  (inverse).
 //   X  (inverse): Hermitian extension, inverse FFT along x, P += dp and the
 //                 block maxima of |dp|.
-// C and cc are recomputed from Rho_new in the X pass instead of being stored.
+// Axis lengths 16..256 use four-step register kernels (n = R1 R2, one LDS
+// exchange); 8 and 512 use LDS Stockham kernels.  C and cc are recomputed
+// from Rho_new in the X pass instead of being stored.
 #include <math.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <vector>
 
@@ -622,6 +628,289 @@ __global__ __launch_bounds__(256) void k_zpass4(double2 *spec, int mxh, int ny, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Three-pass iteration (every axis length 16..256).  The y transform is split
+// between the passes along its four-step factorisation ny = Y1 Y2, with
+// y = t + Y1 m:
+//   pass 1, block (z, t):   residual and x transform of the Y2 rows
+//                           y = t + Y1 m, then y step A for every kx
+//                           (DFT over m, twiddle W_ny^{t k2});
+//   pass 2, block (kx, k2): y step B (DFT over t, ky = k2 + Y2 k1), z
+//                           transform, divide by lambda, inverse z, inverse
+//                           y step B (over k1 back to t);
+//   pass 3, block (z, t):   inverse y step A (twiddle W_ny^{-t k2}, DFT over
+//                           k2 back to m) for every kx, inverse x transform
+//                           of the Y2 rows, P += dp, block max |dp|.
+// Between the passes the spectrum is stored as
+//   S[(k2 mxh + kx) CS + z Y1 + t],   CS = nz Y1 + 8,
+// so a pass-2 block reads and writes one contiguous nz Y1 run, and the Y1
+// blocks (z, 0..Y1-1) of passes 1 and 3, which run next to each other on one
+// XCD, share every 256-byte run of it.  In passes 1 and 3 the lanes of a wave
+// take consecutive kx, 16 nz Y1 + 128 bytes apart: the 128 spare bytes per
+// run spread those accesses over the L2 channels (at a stride of a power of
+// two they would all queue on one).  Two passes over the spectrum fewer than
+// the five-pass form.
+constexpr int kRunPad = 8;  // spare complex values after each nz Y1 run
+// ---------------------------------------------------------------------------
+template <int R1, int R2, int Y2>
+__global__ __launch_bounds__(256) void k_xy_fwd(XArgs a, int Y1, const double2 *__restrict__ twy) {
+  extern __shared__ double ldsd[];
+  constexpr int n = R1 * R2, P = R1 + 1, SP = Y2 + 1;  // padded rows of the x exchange and of the kx columns
+  const Geo &g = a.g;
+  const int mxh = n / 2 + 1;
+  Ix ix{g.nx - 1, g.ny - 1, g.nz - 1, g.nx, g.nxy};
+  const double mean = a.scal[SC_SUM_RHS] / a.ncell;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int t = (int)(blk % (unsigned)Y1), z = (int)(blk / (unsigned)Y1);
+  // x step A: R1 work items per row, row ca is y = t + Y1 ca
+  const int ta = threadIdx.x % R1, ca = threadIdx.x / R1;
+  double2 v[R2];
+  {
+    const int j = t + Y1 * ca;
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = make_double2(residual(a, ix, ta + R1 * m, j, z, mean), 0.0);
+    step_a<R2, false>(v, ta, a.tw);
+  }
+  const bool b_item = (int)threadIdx.x < Y2 * R2;
+  const int k2x = threadIdx.x % R2, cb = threadIdx.x / R2;
+  double2 u[R1];
+#pragma unroll
+  for (int q = 0; q < R2; ++q) ldsd[(ca * R2 + q) * P + ta] = v[q].x;
+  __syncthreads();
+  if (b_item) {
+#pragma unroll
+    for (int q = 0; q < R1; ++q) u[q].x = ldsd[(cb * R2 + k2x) * P + q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < R2; ++q) ldsd[(ca * R2 + q) * P + ta] = v[q].y;
+  __syncthreads();
+  if (b_item) {
+#pragma unroll
+    for (int q = 0; q < R1; ++q) u[q].y = ldsd[(cb * R2 + k2x) * P + q];
+    RegDft<R1, false>::run(u);
+  }
+  __syncthreads();  // the exchange buffer now holds the rows' half spectra
+  double2 *S = reinterpret_cast<double2 *>(ldsd);  // S[kx SP + row]
+  if (b_item) {
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int kx = k2x + R2 * k1;
+      if (kx < mxh) S[kx * SP + cb] = u[k1];
+    }
+  }
+  __syncthreads();
+  // y step A for every kx
+  const long cs = (long)g.nz * Y1 + kRunPad;
+  const long ks = cs * mxh;  // distance between consecutive k2
+  for (int kx = threadIdx.x; kx < mxh; kx += blockDim.x) {
+    double2 w[Y2];
+#pragma unroll
+    for (int m = 0; m < Y2; ++m) w[m] = S[kx * SP + m];
+    step_a<Y2, false>(w, t, twy);
+    double2 *dst = a.spec + kx * cs + z * Y1 + t;
+#pragma unroll
+    for (int k2 = 0; k2 < Y2; ++k2) dst[k2 * ks] = w[k2];
+  }
+}
+
+template <int Y1, int Z1, int Z2>
+__global__ __launch_bounds__(256) void k_yz_pass(double2 *spec, int mxh, int Y2, const double2 *__restrict__ twz,
+                                                 const double *__restrict__ lamx,
+                                                 const double *__restrict__ lamy,
+                                                 const double *__restrict__ lamz) {
+  extern __shared__ double2 L[];
+  constexpr int nz = Z1 * Z2, NE = nz * Y1, RS = nz + nz / 16;
+  auto pad = [](int e) { return e + (e >> 4); };  // one spare slot every 16: fewer bank conflicts
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int k2 = (int)(blk / (unsigned)mxh), kx = (int)(blk % (unsigned)mxh);
+  double2 *base = spec + (long)blk * (NE + kRunPad);  // element (z, t) at z Y1 + t
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NE; e += blockDim.x) L[pad(e)] = base[e];
+  __syncthreads();
+  // y step B: item z, DFT over t
+  const bool zi = tid < nz;
+  double2 u[Y1];
+  if (zi) {
+#pragma unroll
+    for (int q = 0; q < Y1; ++q) u[q] = L[pad(tid * Y1 + q)];
+    RegDft<Y1, false>::run(u);  // u[k1] at ky = k2 + Y2 k1
+  }
+  __syncthreads();
+  if (zi) {
+#pragma unroll
+    for (int k1 = 0; k1 < Y1; ++k1) L[k1 * RS + pad(tid)] = u[k1];
+  }
+  __syncthreads();
+  // z step A: item (k1, tz), over z = tz + Z1 mz
+  const bool ai = tid < Y1 * Z1;
+  const int tz = tid % Z1, ka = tid / Z1;
+  {
+    double2 v[Z2];
+    if (ai) {
+#pragma unroll
+      for (int m = 0; m < Z2; ++m) v[m] = L[ka * RS + pad(tz + Z1 * m)];
+      step_a<Z2, false>(v, tz, twz);
+    }
+    __syncthreads();
+    if (ai) {
+#pragma unroll
+      for (int q = 0; q < Z2; ++q) L[ka * RS + pad(q * Z1 + tz)] = v[q];
+    }
+    __syncthreads();
+  }
+  // z step B: item (k1, kz2), DFT over tz; divide by lambda; inverse step A
+  // with the transposed factorisation
+  const bool bi = tid < Y1 * Z2;
+  const int kz2 = tid % Z2, kb = tid / Z2;
+  {
+    double2 w[Z1];
+    if (bi) {
+#pragma unroll
+      for (int q = 0; q < Z1; ++q) w[q] = L[kb * RS + pad(kz2 * Z1 + q)];
+      RegDft<Z1, false>::run(w);  // w[kz1] at kz = kz2 + Z2 kz1
+      const int ky = k2 + Y2 * kb;
+      const double lxy = lamx[kx] + lamy[ky];
+#pragma unroll
+      for (int kz1 = 0; kz1 < Z1; ++kz1) {
+        const int kz = kz2 + Z2 * kz1;
+        if (kx + ky + kz > 0) {
+          const double lam = lxy + lamz[kz];
+          w[kz1].y /= lam;
+          w[kz1].x /= lam;
+        }
+      }
+      RegDft<Z1, true>::run(w);
+#pragma unroll
+      for (int q = 1; q < Z1; ++q) {
+        double2 tw = twz[kz2 * q];
+        tw.y = -tw.y;
+        w[q] = cmul(w[q], tw);
+      }
+    }
+    __syncthreads();
+    if (bi) {
+#pragma unroll
+      for (int q = 0; q < Z1; ++q) L[kb * RS + pad(q * Z2 + kz2)] = w[q];
+    }
+    __syncthreads();
+  }
+  // inverse z step B: item (k1, q), DFT over kz2, z = q + Z1 k
+  {
+    const int q = tid % Z1, kc = tid / Z1;
+    double2 x[Z2];
+    if (ai) {
+#pragma unroll
+      for (int s = 0; s < Z2; ++s) x[s] = L[kc * RS + pad(q * Z2 + s)];
+      RegDft<Z2, true>::run(x);
+    }
+    __syncthreads();
+    if (ai) {
+#pragma unroll
+      for (int s = 0; s < Z2; ++s) L[kc * RS + pad(q + Z1 * s)] = x[s];
+    }
+    __syncthreads();
+  }
+  // inverse y step B: item z, DFT over k1 back to t
+  if (zi) {
+#pragma unroll
+    for (int k1 = 0; k1 < Y1; ++k1) u[k1] = L[k1 * RS + pad(tid)];
+    RegDft<Y1, true>::run(u);
+  }
+  __syncthreads();
+  if (zi) {
+#pragma unroll
+    for (int q = 0; q < Y1; ++q) L[pad(tid * Y1 + q)] = u[q];
+  }
+  __syncthreads();
+  for (int e = tid; e < NE; e += blockDim.x) base[e] = L[pad(e)];
+}
+
+template <int R1, int R2, int Y2>
+__global__ __launch_bounds__(256) void k_xy_inv(XInvArgs a, int Y1, const double2 *__restrict__ twy) {
+  extern __shared__ double ldsd[];
+  __shared__ double red[256];
+  constexpr int n = R1 * R2, P = R1 + 1, SP = Y2 + 1;
+  const Geo &g = a.g;
+  const int mxh = n / 2 + 1;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const int t = (int)(blk % (unsigned)Y1), z = (int)(blk / (unsigned)Y1);
+  double2 *S = reinterpret_cast<double2 *>(ldsd);  // S[kx SP + row]
+  // inverse y step A for every kx: row m is y = t + Y1 m
+  const long cs = (long)g.nz * Y1 + kRunPad;
+  const long ks = cs * mxh;
+  for (int kx = threadIdx.x; kx < mxh; kx += blockDim.x) {
+    const double2 *src = a.spec + kx * cs + z * Y1 + t;
+    double2 w[Y2];
+#pragma unroll
+    for (int k2 = 0; k2 < Y2; ++k2) {
+      w[k2] = src[k2 * ks];
+      if (k2 > 0) {
+        double2 tw = twy[t * k2];
+        tw.y = -tw.y;
+        w[k2] = cmul(w[k2], tw);
+      }
+    }
+    RegDft<Y2, true>::run(w);
+#pragma unroll
+    for (int m = 0; m < Y2; ++m) S[kx * SP + m] = w[m];
+  }
+  __syncthreads();
+  // inverse x: Hermitian extension of row c, step A over i = tx + R1 mm
+  const int tx = threadIdx.x % R1, c = threadIdx.x / R1;
+  double2 v[R2];
+#pragma unroll
+  for (int m = 0; m < R2; ++m) {
+    const int i = tx + R1 * m;
+    double2 s;
+    if (i < mxh) {
+      s = S[i * SP + c];
+      if (i == 0 || i == n / 2) s.y = 0.0;
+    } else {
+      s = S[(n - i) * SP + c];
+      s.y = -s.y;
+    }
+    v[m] = s;
+  }
+  step_a<R2, true>(v, tx, a.tw);
+  __syncthreads();  // S is reused by the exchange
+  const bool b_item = (int)threadIdx.x < Y2 * R2;
+  const int k2x = threadIdx.x % R2, cb = threadIdx.x / R2;
+  double2 u[R1];
+#pragma unroll
+  for (int q = 0; q < R2; ++q) ldsd[(c * R2 + q) * P + tx] = v[q].x;
+  __syncthreads();
+  if (b_item) {
+#pragma unroll
+    for (int q = 0; q < R1; ++q) u[q].x = ldsd[(cb * R2 + k2x) * P + q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < R2; ++q) ldsd[(c * R2 + q) * P + tx] = v[q].y;
+  __syncthreads();
+  double mx = 0.0;
+  if (b_item) {
+#pragma unroll
+    for (int q = 0; q < R1; ++q) u[q].y = ldsd[(cb * R2 + k2x) * P + q];
+    RegDft<R1, true>::run(u);
+    double *Pl = a.P + ((long)z * g.ny + t + Y1 * cb) * n;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      double dp = (-1) * u[k1].x / (a.ncell);
+      double *Pp = Pl + k2x + R2 * k1;
+      *Pp = *Pp + dp;
+      mx = fmax(mx, fabs(dp));
+    }
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.partials[blk] = red[0];
+}
+
 __global__ __launch_bounds__(1024) void k_finish_max(const double *__restrict__ partials, long nparts,
                                                      double *scal, int slot) {
   __shared__ double sh[1024];
@@ -672,7 +961,9 @@ int fft_setup(tpx_ctx *c) {
     if (hipMemcpy(c->lam[a], lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice) != hipSuccess) return TPX_E_HIP;
   }
   long mxh = g.nx / 2 + 1;
-  if (hipMalloc(&c->spec[0], sizeof(double2) * mxh * g.ny * g.nz) != hipSuccess) return TPX_E_OOM;
+  // the three-pass layout leaves kRunPad spare values after each of its mxh Y2 runs
+  if (hipMalloc(&c->spec[0], sizeof(double2) * mxh * ((long)g.ny * g.nz + 16 * kRunPad)) != hipSuccess)
+    return TPX_E_OOM;
   return TPX_OK;
 }
 
@@ -698,6 +989,33 @@ static bool four_step(int n, int &r1, int &r2) {
     default: LAUNCH(16, 16); break;      \
   }
 
+// launch helpers of the three-pass iteration: template arguments from the
+// run-time factorisations
+template <int R1, int R2>
+void xy_fwd_launch(unsigned nb, size_t sh, hipStream_t st, const XArgs &xa, int Y1, int Y2, const double2 *twy) {
+  switch (Y2) {
+    case 4: k_xy_fwd<R1, R2, 4><<<nb, 4 * R1, sh, st>>>(xa, Y1, twy); break;
+    case 8: k_xy_fwd<R1, R2, 8><<<nb, 8 * R1, sh, st>>>(xa, Y1, twy); break;
+    default: k_xy_fwd<R1, R2, 16><<<nb, 16 * R1, sh, st>>>(xa, Y1, twy); break;
+  }
+}
+template <int R1, int R2>
+void xy_inv_launch(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, int Y1, int Y2, const double2 *twy) {
+  switch (Y2) {
+    case 4: k_xy_inv<R1, R2, 4><<<nb, 4 * R1, sh, st>>>(xi, Y1, twy); break;
+    case 8: k_xy_inv<R1, R2, 8><<<nb, 8 * R1, sh, st>>>(xi, Y1, twy); break;
+    default: k_xy_inv<R1, R2, 16><<<nb, 16 * R1, sh, st>>>(xi, Y1, twy); break;
+  }
+}
+template <int Z1, int Z2>
+void yz_launch(unsigned nb, size_t sh, hipStream_t st, double2 *spec, int mxh, int Y1, int Y2, const tpx_ctx *c) {
+  switch (Y1) {
+    case 4: k_yz_pass<4, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2]); break;
+    case 8: k_yz_pass<8, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2]); break;
+    default: k_yz_pass<16, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2]); break;
+  }
+}
+
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
   const Geo &g = c->g;
   int mxh = g.nx / 2 + 1;
@@ -707,6 +1025,29 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
   XArgs xa{g, c->P, rho_new, c->rhs, c->scal, c->spec[0], c->tw[0], (double)g.n};
   XInvArgs xi{g, c->P, c->spec[0], c->tw[0], c->partials, (double)g.n};
   const bool x4 = four_step(g.nx, r1, r2);
+  int y1, y2, z1, z2;
+  if (c->fft_passes == 3 && x4 && four_step(g.ny, y1, y2) && four_step(g.nz, z1, z2)) {
+    const unsigned nb13 = (unsigned)(g.nz * y1);
+    const size_t sh13 = std::max(sizeof(double) * (size_t)y2 * r2 * (r1 + 1), sizeof(double2) * (size_t)mxh * (y2 + 1));
+#define L_XYF(A, B) xy_fwd_launch<A, B>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1])
+    TPX_FOUR_STEP_SWITCH(g.nx, L_XYF)
+#undef L_XYF
+    TPX_LAUNCH_CHECK();
+    const unsigned nb2 = (unsigned)(mxh * y2);
+    const size_t sh2 = sizeof(double2) * (size_t)y1 * (g.nz + g.nz / 16);
+#define L_YZ(A, B) yz_launch<A, B>(nb2, sh2, c->stream, c->spec[0], mxh, y1, y2, c)
+    TPX_FOUR_STEP_SWITCH(g.nz, L_YZ)
+#undef L_YZ
+    TPX_LAUNCH_CHECK();
+#define L_XYI(A, B) xy_inv_launch<A, B>(nb13, sh13, c->stream, xi, y1, y2, c->tw[1])
+    TPX_FOUR_STEP_SWITCH(g.nx, L_XYI)
+#undef L_XYI
+    TPX_LAUNCH_CHECK();
+    k_finish_max<<<1, 1024, 0, c->stream>>>(c->partials, nb13, c->scal, SC_MAX_DP);
+    TPX_LAUNCH_CHECK();
+    timing_end(c, TPX_K_FFT);
+    return;
+  }
   const int cx = x4 ? 128 / r1 : lines_per_block(g.nx);
   const unsigned nbx = (unsigned)((nlines + cx - 1) / cx);
   const size_t shx4 = x4 ? sizeof(double2) * (size_t)cx * r2 * (r1 + 1) : 0;

@@ -13,17 +13,19 @@
 //
 // Deposits are pull-form segment reductions, never floating-point atomics:
 //   velocity (particle.cpp:494-520): face i of cell c sums the (1 - x_l)
-//     shares of the particles in cell c and the x_l shares of cell c - e_x,
-//     read contiguously because the advance kernel writes the shares at the
-//     particles' sorted positions;
-//   heat (particle.cpp:620-654): the gather kernel runs in sorted order and
-//     writes one record per particle (cloud-in-cell fractions, RHS_T and the
-//     rounded cell).  The rounded cell of a particle in floor cell B is B or
-//     B + 1 along each axis, so its 8 targets lie in the 3x3x3 block around
-//     B.  Each thread sums the 27 weighted moments of its own cell's
-//     particles (every record is read once), the x shifts are added through
-//     LDS, the z shifts in a register window while the block marches along
-//     z, and a last pass adds the y shifts.
+//     shares of the particles in cell c and the x_l shares of cell c - e_x;
+//     the advance kernel writes the shares at the particles' sorted
+//     positions, so a block marching along z reads each plane's shares
+//     contiguously into LDS and takes the neighbour-cell sums through LDS, a
+//     halo row and the previous plane;
+//   heat (particle.cpp:620-654): the same march gathers T at each plane's
+//     particles (T of three planes staged in LDS) and keeps the per-particle
+//     record (cloud-in-cell fractions, RHS_T, rounded cell) in LDS.  The
+//     rounded cell of a particle in floor cell B is B or B + 1 along each
+//     axis, so its 8 targets lie in the 3x3x3 block around B.  Each thread
+//     sums the 27 weighted moments of its own cell's particles, the x shifts
+//     are added through LDS, the z shifts in a register window, and a last
+//     pass adds the y shifts.
 #include <math.h>
 #include <stdint.h>
 
@@ -290,145 +292,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(int32_t *data, long n, cons
 }
 
 // ---- deposits ----
-struct TempArgs {
-  Geo g;
-  const double *x, *y, *z;
-  const double *Tp;          // T_new or T_int
-  const double *rho;
-  const double *scal;
-  int p0_slot;
-  double R;
-  double iTth;
-  const int32_t *inv;        // particle -> sorted position
-  double *rhs_t;             // by particle
-  double4 *rec;              // by sorted position
-  int32_t *rkey;
-  long n;
-};
-
-// P8/P10 gather in physical order: T = (P0/R)/Rho (grid.cpp:882), cloud-in-cell
-// at the particle (particle.cpp:402-456), RHS_T = (Tg - T_p) Nu pi Dp k
-// (particle.cpp:460-474); also emits the deposit record of the particle at
-// its sorted position
-__device__ __forceinline__ void temp_gather_one(const TempArgs &a, long p) {
-  const Geo &g = a.g;
-  Ix ix = make_ix(g);
-  double x = a.x[p], y = a.y[p], z = a.z[p];
-  int ii = (int)(round(x * g.idx)), jj = (int)(round(y * g.idy)), kk = (int)(round(z * g.idz));
-  double xl = x * g.idx - ii + 0.5, yl = y * g.idy - jj + 0.5, zl = z * g.idz - kk + 0.5;
-  double at = a.scal[a.p0_slot] / a.R;
-  const double *__restrict__ rho = a.rho;
-  auto T = [&](int i, int j, int k) { return at / rho[ix(i, j, k)]; };
-  double Tg = xl * yl * zl * T(ii, jj, kk) +
-              (1 - xl) * yl * zl * T(ii - 1, jj, kk) + xl * (1 - yl) * zl * T(ii, jj - 1, kk) +
-              xl * yl * (1 - zl) * T(ii, jj, kk - 1) +
-              (1 - xl) * (1 - yl) * zl * T(ii - 1, jj - 1, kk) +
-              (1 - xl) * yl * (1 - zl) * T(ii - 1, jj, kk - 1) +
-              xl * (yl - 1) * (zl - 1) * T(ii, jj - 1, kk - 1) +
-              (1 - xl) * (1 - yl) * (1 - zl) * T(ii - 1, jj - 1, kk - 1);
-  double v = (Tg - a.Tp[p]) * a.iTth;
-  a.rhs_t[p] = v;
-  long q = a.inv[p];
-  a.rec[q] = make_double4(xl, yl, zl, v);
-  a.rkey[q] = (ii & ix.mx) | ((jj & ix.my) << 10) | ((kk & ix.mz) << 20);
-}
-
-__global__ __launch_bounds__(kBlock) void k_temp_gather(TempArgs a) {
-  const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p < a.n) temp_gather_one(a, p);
-}
-
-// ---- heat deposit by per-cell moments (P9/P10) -----------------------------
-// Stage 1: one block covers `rows` whole x-rows and marches over `zc` output
-// planes (plus one halo plane on each side).  For base plane z every thread
-// sums, over the records of its cell (i, j, z) in sorted order, the 27
-// products w_x(a) w_y(b) w_z(c) RHS_T for the target offsets a, b, c in
-// {-1, 0, 1}.  The x offsets are exchanged through LDS (left, own, right), the
-// z offsets accumulate in a three-plane register window, and the result for
-// output cell (i, j + b, z) from this row is stored as t3[b + 1].
-struct HeatArgs {
-  Geo g;
-  const int32_t *start;
-  const double4 *rec;
-  const int32_t *rkey;
-  double *t3;                // 3 x n: contribution of row j to row j - 1, j, j + 1
-  int rows, zc;
-};
-
 constexpr int kHeatMaxBlock = 512;
 
-__global__ __launch_bounds__(kHeatMaxBlock) void k_heat_planes(HeatArgs a) {
-  extern __shared__ double xs[];  // 18 x blockDim: offsets -1 (0..8) and +1 (9..17)
-  const Geo &g = a.g;
-  Ix ix = make_ix(g);
-  const int nx = g.nx, bd = (int)blockDim.x, t = (int)threadIdx.x;
-  const int i = t & ix.mx, r = t / nx;
-  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  const int nyb = g.ny / a.rows;
-  const int j = (int)(b % (unsigned)nyb) * a.rows + r;
-  const int z0 = (int)(b / (unsigned)nyb) * a.zc;
-  const int il = r * nx + ((i - 1) & ix.mx), ir = r * nx + ((i + 1) & ix.mx);
-  double sa[3] = {0., 0., 0.}, sb[3] = {0., 0., 0.};
-  long c = ((long)((z0 - 1) & ix.mz) * g.ny + j) * nx + i;
-  int32_t q0 = a.start[c], q1 = a.start[c + 1];
-  for (int zz = -1; zz <= a.zc; ++zz) {
-    const int k = (z0 + zz) & ix.mz;
-    // prefetch the next plane's segment bounds
-    const long cn = ((long)((k + 1) & ix.mz) * g.ny + j) * nx + i;
-    const int32_t n0 = a.start[cn], n1 = a.start[cn + 1];
-    double m[3][3][3];
-#pragma unroll
-    for (int u = 0; u < 27; ++u) m[u / 9][(u / 3) % 3][u % 3] = 0.0;
-    for (int32_t q = q0; q < q1; ++q) {
-      const int32_t rk = a.rkey[q];
-      const double4 v = a.rec[q];
-      // the rounded cell is B + 1 (upper) or B along each axis
-      const bool ux = (((rk & 1023) - i) & ix.mx) != 0;
-      const bool uy = ((((rk >> 10) & 1023) - j) & ix.my) != 0;
-      const bool uz = ((((rk >> 20) & 1023) - k) & ix.mz) != 0;
-      const double wx[3] = {ux ? 0.0 : 1 - v.x, ux ? 1 - v.x : v.x, ux ? v.x : 0.0};
-      const double wy[3] = {uy ? 0.0 : 1 - v.y, uy ? 1 - v.y : v.y, uy ? v.y : 0.0};
-      const double wz[3] = {uz ? 0.0 : 1 - v.z, uz ? 1 - v.z : v.z, uz ? v.z : 0.0};
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax)
-#pragma unroll
-        for (int ay = 0; ay < 3; ++ay)
-#pragma unroll
-          for (int az = 0; az < 3; ++az) m[ax][ay][az] += wx[ax] * wy[ay] * wz[az] * v.w;
-    }
-    __syncthreads();  // the previous plane's exchange has been read
-#pragma unroll
-    for (int s = 0; s < 9; ++s) {
-      xs[s * bd + t] = m[0][s / 3][s % 3];
-      xs[(9 + s) * bd + t] = m[2][s / 3][s % 3];
-    }
-    __syncthreads();
-    double R[3][3];
-#pragma unroll
-    for (int s = 0; s < 9; ++s) R[s / 3][s % 3] = (xs[(9 + s) * bd + il] + m[1][s / 3][s % 3]) + xs[s * bd + ir];
-    if (zz >= 1) {
-      const long co = ((long)(z0 + zz - 1) * g.ny + j) * nx + i;
-#pragma unroll
-      for (int e = 0; e < 3; ++e) a.t3[e * g.n + co] = sa[e] + R[e][0];
-    }
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-      sa[e] = sb[e] + R[e][1];
-      sb[e] = R[e][2];
-    }
-    q0 = n0;
-    q1 = n1;
-  }
-}
-
-// ---- fused temperature gather + heat deposit (P8/P10 gather, P9 deposit) ----
-// The heat moment kernel above, with the gather folded in: for each particle
-// of its base cell (in the cell's sorted order) the thread computes the
-// cloud-in-cell fractions at the particle, the gas temperature T = (P0/R)/Rho
-// from three planes of T staged in LDS (one division per cell instead of 8
-// per particle), RHS_T = (Tg - T_p) Nu pi Dp k (particle.cpp:460-474), stores
-// RHS_T by particle and accumulates the 27 moments.  No per-particle record
-// goes through memory.
+// ---- temperature gather + heat deposit by per-cell moments (P8-P10) --------
+// One block covers `rows` whole x-rows and marches over `zc` output planes
+// (plus one halo plane on each side).  For every particle of a base plane it
+// computes the cloud-in-cell fractions, the gas temperature T = (P0/R)/Rho
+// (grid.cpp:882) at the particle from three planes of T staged in LDS (one
+// division per cell instead of 8 per particle; particle.cpp:402-456) and
+// RHS_T = (Tg - T_p) Nu pi Dp k (particle.cpp:460-474), which it stores by
+// particle.  Every thread then sums, over its cell's particles in sorted
+// order, the 27 products w_x(a) w_y(b) w_z(c) RHS_T for the target offsets
+// a, b, c in {-1, 0, 1}.  The x offsets are exchanged through LDS, the z
+// offsets accumulate in a three-plane register window, and the result for
+// output cell (i, j + b, z) from this row is stored as t3[b + 1].
 struct HeatFusedArgs {
   Geo g;
   const int32_t *start;      // CSR of the stage's cells
@@ -612,37 +489,6 @@ __global__ __launch_bounds__(kBlock) void k_heat_combine(Geo g, const double *__
 // (1 - frac_a) RHS_a] + [sum over cell c - e_a of frac_a RHS_a], each part in
 // increasing sorted position.  Component 2a of contrib holds frac_a RHS_a,
 // component 2a+1 holds (1 - frac_a) RHS_a.
-__global__ __launch_bounds__(kBlock) void k_vel_cells(Geo g, const int32_t *__restrict__ start,
-                                                      const double *__restrict__ contrib, long cs,
-                                                      double *Sx, double *Sy, double *Sz) {
-  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  const long c = (long)b * blockDim.x + threadIdx.x;
-  if (c >= g.n) return;
-  Ix ix = make_ix(g);
-  const int i = (int)(c & ix.mx);
-  const long row = c / g.nx;
-  const int j = (int)(row % g.ny), k = (int)(row / g.ny);
-  const double *__restrict__ c0 = contrib;
-  const double *__restrict__ c1 = contrib + cs;
-  const double *__restrict__ c2 = contrib + 2 * cs;
-  const double *__restrict__ c3 = contrib + 3 * cs;
-  const double *__restrict__ c4 = contrib + 4 * cs;
-  const double *__restrict__ c5 = contrib + 5 * cs;
-  double o0 = 0., o1 = 0., o2 = 0., n0 = 0., n1 = 0., n2 = 0.;
-  for (int32_t q = start[c], e = start[c + 1]; q < e; ++q) {
-    o0 += c1[q];
-    o1 += c3[q];
-    o2 += c5[q];
-  }
-  const long fx = ix(i - 1, j, k), fy = ix(i, j - 1, k), fz = ix(i, j, k - 1);
-  for (int32_t q = start[fx], e = start[fx + 1]; q < e; ++q) n0 += c0[q];
-  for (int32_t q = start[fy], e = start[fy + 1]; q < e; ++q) n1 += c2[q];
-  for (int32_t q = start[fz], e = start[fz + 1]; q < e; ++q) n2 += c4[q];
-  Sx[c] = o0 + n0;
-  Sy[c] = o1 + n1;
-  Sz[c] = o2 + n2;
-}
-
 // Sums of NV per-particle arrays over the particles of each of `ncell`
 // consecutive cells starting at cell `cb`, one cell per thread (threads past
 // `ncell` get zeros), each sum taken in sorted order from 0.  The cells'
@@ -969,59 +815,6 @@ void launch_sort_stage(tpx_ctx *c, int stage, bool into_int) {
 }
 
 // temperature gather + heat records, in the sorted order of `stage`
-void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot) {
-  Particles &pt = c->pt;
-  if (pt.n == 0) return;
-  int s = stage == TPX_INT ? pt.int_buf() : pt.st_new;
-  TempArgs a{};
-  a.g = c->g;
-  a.x = pt.q[0][s]; a.y = pt.q[1][s]; a.z = pt.q[2][s];
-  a.Tp = pt.q[6][s];
-  a.rho = rho;
-  a.scal = c->scal;
-  a.p0_slot = p0_slot;
-  a.R = c->R;
-  a.iTth = c->p.nu * TPX_PI * c->p.dp * c->p.k;
-  a.inv = stage == TPX_INT ? pt.inv_int : pt.inv_new;
-  a.rhs_t = pt.rhs_t;
-  a.rec = pt.rec;
-  a.rkey = pt.rkey;
-  a.n = pt.n;
-  timing_begin(c, TPX_K_PARTICLE);
-  k_temp_gather<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(a);
-  TPX_LAUNCH_CHECK();
-  timing_end(c, TPX_K_PARTICLE);
-}
-
-// heat deposit from the records of the last temperature gather
-void launch_temp_deposit(tpx_ctx *c, int stage) {
-  Particles &pt = c->pt;
-  const int32_t *start = stage == TPX_INT ? pt.start_int : pt.start_new;
-  timing_begin(c, TPX_K_DEPOSIT);
-  if (pt.n > 0) {
-    const Geo &g = c->g;
-    HeatShape h = heat_shape(g);
-    if (h.threads > kHeatMaxBlock || g.ny % h.rows || g.nz % h.zc)
-      throw std::logic_error("heat deposit launch shape does not tile the grid");
-    HeatArgs a{};
-    a.g = g;
-    a.start = start;
-    a.rec = pt.rec;
-    a.rkey = pt.rkey;
-    a.t3 = c->dep_tmp;
-    a.rows = h.rows;
-    a.zc = h.zc;
-    size_t lds = sizeof(double) * 18 * (size_t)h.threads;
-    k_heat_planes<<<h.blocks, h.threads, lds, c->stream>>>(a);
-    TPX_LAUNCH_CHECK();
-    unsigned nb = grid1d(g.n, kBlock);
-    k_heat_combine<<<nb, kBlock, 0, c->stream>>>(g, c->dep_tmp, c->Q, c->partials);
-    TPX_LAUNCH_CHECK();
-    launch_finish_sum(c, (int)nb, 0, SC_SUM_Q);
-  }
-  timing_end(c, TPX_K_DEPOSIT);
-}
-
 // temperature gather + heat deposit in one pass over the stage's bins
 void launch_heat_exchange(tpx_ctx *c, int stage, const double *rho, int p0_slot) {
   Particles &pt = c->pt;

@@ -2,6 +2,7 @@
 // and the order of kernel launches for each reference method and for the
 // fused step.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -342,7 +343,7 @@ void release(tpx_ctx *c) {
   for (auto &q : pt.q) for (auto &p : q) f(p);
   f(pt.rhs_t); f(pt.tau); f(pt.eps); f(pt.flag); f(pt.contrib); f(pt.key);
   f(pt.perm_int); f(pt.perm_new); f(pt.start_int); f(pt.start_new); f(pt.cursor);
-  f(pt.inv_int); f(pt.inv_new); f(pt.rkey); f(pt.rec);
+  f(pt.inv_int); f(pt.inv_new);
   for (auto &p : pt.spare) f(p);
   f(pt.itmp);
   for (auto &e : c->tm.pool) (void)hipEventDestroy(e);
@@ -421,6 +422,9 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
     hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device),
               "hipDeviceGetAttribute");
     c->max_lds = (size_t)lds;
+    // TPX_FFT_PASSES=5 selects the five-pass Poisson iteration everywhere
+    // (the three-pass form is the default where the axis lengths allow it)
+    if (const char *e = getenv("TPX_FFT_PASSES")) c->fft_passes = atoi(e) == 5 ? 5 : 3;
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     Geo &g = c->g;
     g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
@@ -481,8 +485,6 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
     ialloc(&pt.perm_int, cap); ialloc(&pt.perm_new, cap);
     ialloc(&pt.inv_int, cap); ialloc(&pt.inv_new, cap);
     ialloc(&pt.start_int, n + 1); ialloc(&pt.start_new, n + 1); ialloc(&pt.cursor, n + 1);
-    ialloc(&pt.rkey, cap);
-    if (hipMalloc(&pt.rec, sizeof(double4) * (cap ? cap : 1)) != hipSuccess) throw std::bad_alloc();
     hip_check(hipMemset(pt.start_int, 0, sizeof(int32_t) * (n + 1)), "hipMemset");
     hip_check(hipMemset(pt.rhs_t, 0, sizeof(double) * (cap ? cap : 1)), "hipMemset");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");

@@ -67,8 +67,6 @@ struct Particles {
   int32_t *start_int = nullptr, *start_new = nullptr;   // CSR over cells (n_cell+1)
   int32_t *cursor = nullptr;      // placement cursors over cells (n_cell+1)
   bool new_counted = false;       // the last advance counted x_new into start_new
-  double4 *rec = nullptr;         // heat deposit records in sorted order: x_l, y_l, z_l, RHS_T
-  int32_t *rkey = nullptr;        // packed rounded (cloud-in-cell) cell of each record
   bool has_tau = false, has_eps = false;
   // spare arrays the step-end permutation writes into (rhs_t, tau, eps, flag)
   double *spare[3] = {};
@@ -100,6 +98,7 @@ struct tpx_ctx {
   hipStream_t stream = nullptr;
   int device = 0;
   size_t max_lds = 65536;          // dynamic LDS bytes one workgroup may use on this device
+  int fft_passes = 3;              // Poisson iteration: 3 passes where every axis allows, else 5
   // grid fields
   double *ru[5][3] = {};          // buffer sets for RU
   int ru_base = 0, ru_int = 1, ru_new = 2, ru_np1 = 3;
@@ -164,9 +163,8 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
                              const double *rho_int, bool count_new);
 void launch_velocity_deposit(tpx_ctx *c);
 void launch_sort_new(tpx_ctx *c);
-void launch_temp_gather(tpx_ctx *c, int stage, const double *rho, int p0_slot);
-void launch_temp_deposit(tpx_ctx *c, int stage);
-// the two above in one pass (writes RHS_T by particle and Q)
+// temperature gather at the stage's particles and heat deposit (writes RHS_T
+// by particle and Q)
 void launch_heat_exchange(tpx_ctx *c, int stage, const double *rho, int p0_slot);
 void launch_concentration(tpx_ctx *c);
 void launch_particle_permute(tpx_ctx *c);
