@@ -149,6 +149,7 @@ struct XArgs {
   double2 *spec;
   const double2 *tw;     // x twiddles
   double ncell;
+  const double *done;    // nonzero: the solve has converged, the iteration is skipped
 };
 
 // 1/x from the hardware reciprocal estimate and two Newton steps,
@@ -192,6 +193,7 @@ __device__ __forceinline__ double residual(const XArgs &a, const Ix &ix, int i, 
 
 // X forward pass: lpb consecutive x lines per block
 __global__ __launch_bounds__(kThreads) void k_xfwd(XArgs a, int lpb) {
+  if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   const Geo &g = a.g;
   int nx = g.nx, mxh = nx / 2 + 1;
@@ -223,10 +225,12 @@ struct XInvArgs {
   const double2 *tw;
   double *partials;
   double ncell;
+  const double *done;
 };
 
 // X inverse pass: Hermitian extension, inverse FFT, P += dp, block max|dp|
 __global__ __launch_bounds__(kThreads) void k_xinv(XInvArgs a, int lpb) {
+  if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   __shared__ double red[kThreads];
   const Geo &g = a.g;
@@ -267,7 +271,8 @@ __global__ __launch_bounds__(kThreads) void k_xinv(XInvArgs a, int lpb) {
 // Y pass (forward or inverse) on a tile of kTile kx columns at fixed z
 template <bool INV>
 __global__ __launch_bounds__(kThreads) void k_ypass(double2 *spec, int mxh, int ny, int nz,
-                                                    const double2 *__restrict__ tw, int kTile) {
+                                                    const double2 *__restrict__ tw, int kTile, const double *__restrict__ done) {
+  if (*done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   int ntile = (mxh + kTile - 1) / kTile;
   unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -293,7 +298,8 @@ __global__ __launch_bounds__(kThreads) void k_zpass(double2 *spec, int mxh, int 
                                                     const double2 *__restrict__ tw,
                                                     const double *__restrict__ lamx,
                                                     const double *__restrict__ lamy,
-                                                    const double *__restrict__ lamz, int kTile) {
+                                                    const double *__restrict__ lamz, int kTile, const double *__restrict__ done) {
+  if (*done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   int ntile = (mxh + kTile - 1) / kTile;
   unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -403,6 +409,7 @@ __device__ __forceinline__ void step_a(double2 (&v)[R2], int t, const double2 *_
 // X forward, four-step: C lines per block, R1 work items per line in step A
 template <int R1, int R2>
 __global__ __launch_bounds__(256) void k_xfwd4(XArgs a, int C) {
+  if (*a.done != 0.0) return;  // the solve has converged
   // The exchange moves real and imaginary parts in two rounds through an
   // LDS buffer of doubles: half the LDS per block, more blocks per CU.
   extern __shared__ double ldsd[];
@@ -456,6 +463,7 @@ __global__ __launch_bounds__(256) void k_xfwd4(XArgs a, int C) {
 // X inverse, four-step: Hermitian extension, P += dp, block max |dp|
 template <int R1, int R2>
 __global__ __launch_bounds__(256) void k_xinv4(XInvArgs a, int C) {
+  if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   __shared__ double red[256];
   constexpr int n = R1 * R2, P = R1 + 1;
@@ -532,7 +540,9 @@ __device__ __forceinline__ void strided_fwd_half(double2 (&v)[R2], double2 *lds,
 
 template <int R1, int R2, bool INV>
 __global__ __launch_bounds__(256) void k_ypass4(double2 *spec, int mxh, int ny, int nz,
-                                                const double2 *__restrict__ tw, int C) {
+                                                const double2 *__restrict__ tw, int C,
+                                                const double *__restrict__ done) {
+  if (*done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   const int ntile = (mxh + C - 1) / C;
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -564,7 +574,9 @@ __global__ __launch_bounds__(256) void k_zpass4(double2 *spec, int mxh, int ny, 
                                                 const double2 *__restrict__ tw,
                                                 const double *__restrict__ lamx,
                                                 const double *__restrict__ lamy,
-                                                const double *__restrict__ lamz, int C) {
+                                                const double *__restrict__ lamz, int C,
+                                                const double *__restrict__ done) {
+  if (*done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   const int ntile = (mxh + C - 1) / C;
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -654,6 +666,7 @@ constexpr int kRunPad = 8;  // spare complex values after each nz Y1 run
 // ---------------------------------------------------------------------------
 template <int R1, int R2, int Y2>
 __global__ __launch_bounds__(256) void k_xy_fwd(XArgs a, int Y1, const double2 *__restrict__ twy) {
+  if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double ldsd[];
   constexpr int n = R1 * R2, P = R1 + 1, SP = Y2 + 1;  // padded rows of the x exchange and of the kx columns
   const Geo &g = a.g;
@@ -718,7 +731,9 @@ template <int Y1, int Z1, int Z2>
 __global__ __launch_bounds__(256) void k_yz_pass(double2 *spec, int mxh, int Y2, const double2 *__restrict__ twz,
                                                  const double *__restrict__ lamx,
                                                  const double *__restrict__ lamy,
-                                                 const double *__restrict__ lamz) {
+                                                 const double *__restrict__ lamz,
+                                                 const double *__restrict__ done) {
+  if (*done != 0.0) return;  // the solve has converged
   extern __shared__ double2 L[];
   constexpr int nz = Z1 * Z2, NE = nz * Y1, RS = nz + nz / 16;
   auto pad = [](int e) { return e + (e >> 4); };  // one spare slot every 16: fewer bank conflicts
@@ -828,6 +843,7 @@ __global__ __launch_bounds__(256) void k_yz_pass(double2 *spec, int mxh, int Y2,
 
 template <int R1, int R2, int Y2>
 __global__ __launch_bounds__(256) void k_xy_inv(XInvArgs a, int Y1, const double2 *__restrict__ twy) {
+  if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double ldsd[];
   __shared__ double red[256];
   constexpr int n = R1 * R2, P = R1 + 1, SP = Y2 + 1;
@@ -911,9 +927,14 @@ __global__ __launch_bounds__(256) void k_xy_inv(XInvArgs a, int Y1, const double
   if (threadIdx.x == 0) a.partials[blk] = red[0];
 }
 
-__global__ __launch_bounds__(1024) void k_finish_max(const double *__restrict__ partials, long nparts,
-                                                     double *scal, int slot) {
+// End of one iteration: max |dp| over the block maxima into SC_MAX_DP, the
+// iteration count, and the loop test of poisson.cpp:391-462 (continue while
+// count < Iteration and max |dp| > epsilon) into SC_DONE.  A skipped
+// iteration leaves every slot alone.
+__global__ __launch_bounds__(1024) void k_finish_iter(const double *__restrict__ partials, long nparts,
+                                                      double *scal, int iteration, double epsilon) {
   __shared__ double sh[1024];
+  if (scal[SC_DONE] != 0.0) return;
   double v = 0.0;
   for (long q = threadIdx.x; q < nparts; q += 1024) v = fmax(v, partials[q]);  // coalesced
   sh[threadIdx.x] = v;
@@ -922,7 +943,18 @@ __global__ __launch_bounds__(1024) void k_finish_max(const double *__restrict__ 
     if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) scal[slot] = sh[0];
+  if (threadIdx.x == 0) {
+    const double err = sh[0], it = scal[SC_ITERS] + 1.0;
+    scal[SC_MAX_DP] = err;
+    scal[SC_ITERS] = it;
+    scal[SC_DONE] = (it < iteration && err > epsilon) ? 0.0 : 1.0;
+  }
+}
+
+// Start of a solve: no iteration yet, err = 100 (poisson.cpp:391-396)
+__global__ void k_solve_begin(double *scal, int iteration, double epsilon) {
+  scal[SC_ITERS] = 0.0;
+  scal[SC_DONE] = (0 < iteration && 100.0 > epsilon) ? 0.0 : 1.0;
 }
 
 // lines per block of the x passes and kx columns per block of the y/z
@@ -1010,10 +1042,15 @@ void xy_inv_launch(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, i
 template <int Z1, int Z2>
 void yz_launch(unsigned nb, size_t sh, hipStream_t st, double2 *spec, int mxh, int Y1, int Y2, const tpx_ctx *c) {
   switch (Y1) {
-    case 4: k_yz_pass<4, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2]); break;
-    case 8: k_yz_pass<8, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2]); break;
-    default: k_yz_pass<16, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2]); break;
+    case 4: k_yz_pass<4, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
+    case 8: k_yz_pass<8, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
+    default: k_yz_pass<16, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
   }
+}
+
+void launch_solve_begin(tpx_ctx *c) {
+  k_solve_begin<<<1, 1, 0, c->stream>>>(c->scal, c->Iteration, c->p.epsilon);
+  TPX_LAUNCH_CHECK();
 }
 
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
@@ -1022,8 +1059,9 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
   long nlines = (long)g.ny * g.nz;
   int r1, r2;
   timing_begin(c, TPX_K_FFT);
-  XArgs xa{g, c->P, rho_new, c->rhs, c->scal, c->spec[0], c->tw[0], (double)g.n};
-  XInvArgs xi{g, c->P, c->spec[0], c->tw[0], c->partials, (double)g.n};
+  const double *done = c->scal + SC_DONE;
+  XArgs xa{g, c->P, rho_new, c->rhs, c->scal, c->spec[0], c->tw[0], (double)g.n, done};
+  XInvArgs xi{g, c->P, c->spec[0], c->tw[0], c->partials, (double)g.n, done};
   const bool x4 = four_step(g.nx, r1, r2);
   int y1, y2, z1, z2;
   if (c->fft_passes == 3 && x4 && four_step(g.ny, y1, y2) && four_step(g.nz, z1, z2)) {
@@ -1043,7 +1081,7 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     TPX_FOUR_STEP_SWITCH(g.nx, L_XYI)
 #undef L_XYI
     TPX_LAUNCH_CHECK();
-    k_finish_max<<<1, 1024, 0, c->stream>>>(c->partials, nb13, c->scal, SC_MAX_DP);
+    k_finish_iter<<<1, 1024, 0, c->stream>>>(c->partials, nb13, c->scal, c->Iteration, c->p.epsilon);
     TPX_LAUNCH_CHECK();
     timing_end(c, TPX_K_FFT);
     return;
@@ -1065,14 +1103,14 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     const int C = 128 / r1;
     const unsigned nb = (unsigned)(((mxh + C - 1) / C) * g.nz);
     const size_t sh = sizeof(double2) * (size_t)C * g.ny;
-#define L_YF(A, B) k_ypass4<A, B, false><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], C)
+#define L_YF(A, B) k_ypass4<A, B, false><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], C, done)
     TPX_FOUR_STEP_SWITCH(g.ny, L_YF)
 #undef L_YF
   } else {
     int ty = lines_per_block(g.ny);
     unsigned nby = (unsigned)(((mxh + ty - 1) / ty) * g.nz);
     k_ypass<false><<<nby, kThreads, sizeof(double2) * (size_t)ty * g.ny, c->stream>>>(c->spec[0], mxh, g.ny, g.nz,
-                                                                                      c->tw[1], ty);
+                                                                                      c->tw[1], ty, done);
   }
   TPX_LAUNCH_CHECK();
   if (four_step(g.nz, r1, r2)) {
@@ -1081,14 +1119,14 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     const size_t sh = sizeof(double2) * (size_t)C * g.nz;
 #define L_Z(A, B)                                                                                          \
   k_zpass4<A, B><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[2], c->lam[0], c->lam[1], \
-                                               c->lam[2], C)
+                                               c->lam[2], C, done)
     TPX_FOUR_STEP_SWITCH(g.nz, L_Z)
 #undef L_Z
   } else {
     int tz = lines_per_block(g.nz);
     unsigned nbz = (unsigned)(((mxh + tz - 1) / tz) * g.ny);
     k_zpass<<<nbz, kThreads, sizeof(double2) * (size_t)tz * g.nz, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[2],
-                                                                               c->lam[0], c->lam[1], c->lam[2], tz);
+                                                                               c->lam[0], c->lam[1], c->lam[2], tz, done);
   }
   TPX_LAUNCH_CHECK();
   if (y4) {
@@ -1096,14 +1134,14 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     const int C = 128 / r1;
     const unsigned nb = (unsigned)(((mxh + C - 1) / C) * g.nz);
     const size_t sh = sizeof(double2) * (size_t)C * g.ny;
-#define L_YI(A, B) k_ypass4<A, B, true><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], C)
+#define L_YI(A, B) k_ypass4<A, B, true><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], C, done)
     TPX_FOUR_STEP_SWITCH(g.ny, L_YI)
 #undef L_YI
   } else {
     int ty = lines_per_block(g.ny);
     unsigned nby = (unsigned)(((mxh + ty - 1) / ty) * g.nz);
     k_ypass<true><<<nby, kThreads, sizeof(double2) * (size_t)ty * g.ny, c->stream>>>(c->spec[0], mxh, g.ny, g.nz,
-                                                                                     c->tw[1], ty);
+                                                                                     c->tw[1], ty, done);
   }
   TPX_LAUNCH_CHECK();
   if (x4) {
@@ -1114,7 +1152,7 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     k_xinv<<<nbx, kThreads, shx, c->stream>>>(xi, cx);
   }
   TPX_LAUNCH_CHECK();
-  k_finish_max<<<1, 1024, 0, c->stream>>>(c->partials, nbx, c->scal, SC_MAX_DP);
+  k_finish_iter<<<1, 1024, 0, c->stream>>>(c->partials, nbx, c->scal, c->Iteration, c->p.epsilon);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_FFT);
 }

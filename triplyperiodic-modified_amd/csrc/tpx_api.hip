@@ -216,14 +216,30 @@ void do_update_ru_wop(tpx_ctx *c, int rk, bool with_rho) {
                 with_rho ? c->rho[c->rho_np1] : nullptr, with_rho ? c->rho[c->rho_new] : nullptr);
 }
 
-void do_solve_poisson(tpx_ctx *c, int *iters) {
-  int i = 0;
-  double err = 100;
-  while ((i < c->Iteration) && (err > c->p.epsilon)) {
-    launch_poisson_iteration(c, c->rho[c->rho_new]);
-    err = read_scalar(c, SC_MAX_DP);
-    i++;
+// Solve_Poisson's loop (poisson.cpp:391-462).  The loop test runs on the
+// device after every iteration, and iterations queued after it fails skip
+// themselves, so the host queues iterations in batches and waits once per
+// batch instead of once per iteration: first as many as the last solve of
+// this substep took, then two at a time.
+void do_solve_poisson(tpx_ctx *c, int rk, int *iters) {
+  launch_solve_begin(c);
+  int queued = 0, batch = std::max(1, c->iter_guess[rk & 3]);
+  double st[2] = {0.0, 1.0};  // SC_ITERS, SC_DONE
+  while (queued < c->Iteration) {
+    const int nb = std::min(batch, c->Iteration - queued);
+    for (int b = 0; b < nb; ++b) launch_poisson_iteration(c, c->rho[c->rho_new]);
+    queued += nb;
+    hip_check(hipMemcpyAsync(c->h_scal + SC_ITERS, c->scal + SC_ITERS, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream),
+              "hipMemcpyAsync");
+    sync(c);
+    st[0] = c->h_scal[SC_ITERS];
+    st[1] = c->h_scal[SC_DONE];
+    if (st[1] != 0.0) break;
+    batch = 2;
   }
+  const int i = (int)st[0];
+  c->iter_guess[rk & 3] = i;
   c->num_iteration = i;
   c->total_iterations += i;
   if (iters) *iters = i;
@@ -322,7 +338,7 @@ void fused_substep(tpx_ctx *c, int rk) {
   const double *rn[3];
   ru_ptrs(c, c->ru_new, rn);
   launch_div_rhs(c, k, 0, rn, c->rho[c->rho_new], c->q_zero ? nullptr : c->Q, nullptr, c->rhs);
-  do_solve_poisson(c, nullptr);
+  do_solve_poisson(c, rk, nullptr);
   do_update_ru_wp(c, rk, rk == 3);
   do_time_advance_rk4(c);
 }
@@ -747,7 +763,7 @@ int tpx_solve_poisson(tpx_ctx *ctx, int rk, int *iterations) {
   return guarded(ctx, [&]() {
     TPX_RK_CHECK(rk);
     if (ctx->rho_new < 0) throw std::logic_error("Solve_Poisson needs Rho_new");
-    do_solve_poisson(ctx, iterations);
+    do_solve_poisson(ctx, rk, iterations);
     return TPX_OK;
   });
 }

@@ -28,6 +28,8 @@ enum Scal {
   SC_MEAN_RU0, SC_MEAN_RU1, SC_MEAN_RU2,
   SC_RED,          // generic reduction result
   SC_SUM_C,
+  SC_ITERS,        // iterations run by the current Poisson solve
+  SC_DONE,         // 1: the current solve has stopped (its loop test failed)
   SC_COUNT = 32
 };
 
@@ -95,6 +97,7 @@ struct tpx_ctx {
   int num_timestep = 0;
   int num_iteration = 0;
   int64_t total_iterations = 0;    // summed over all Poisson solves
+  int iter_guess[4] = {4, 4, 4, 4}; // iterations queued up front, per substep (last solve's count)
   hipStream_t stream = nullptr;
   int device = 0;
   size_t max_lds = 65536;          // dynamic LDS bytes one workgroup may use on this device
@@ -155,7 +158,10 @@ void launch_copy(tpx_ctx *c, double *dst, const double *src, long n);
 
 // ---- launchers (k_fft.hip) ----
 int fft_setup(tpx_ctx *c);
+// one Poisson iteration; skipped on the device once the solve has stopped
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
+// resets the device-side iteration count and loop test of a solve
+void launch_solve_begin(tpx_ctx *c);
 
 // ---- launchers (k_particle.hip) ----
 // count_new: zero start_new and count x_new into it for launch_sort_new
