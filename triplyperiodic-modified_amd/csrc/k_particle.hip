@@ -345,29 +345,77 @@ __global__ __launch_bounds__(kHeatMaxBlock) void k_heat_fused(HeatFusedArgs a) {
   double4 *rec = reinterpret_cast<double4 *>(xs);                        // kHeatChunk x blockDim
   int32_t *rbit = reinterpret_cast<int32_t *>(xs + 4 * kHeatChunk * bd);  // kHeatChunk x blockDim
   double *tt = xs + 9 * bd;
-  const int TPL = (a.rows + 2) * nx;  // doubles per staged plane
+  const int TPL = (a.rows + 2) * nx;  // doubles per staged plane, at most 3 per thread
   const double at = a.scal[a.p0_slot] / a.R;
-  // T of plane kz into ring slot s: rows j0-1 .. j0+rows
-  auto stage_T = [&](int kz, int s) {
+  // Rho of plane kz, rows j0-1 .. j0+rows, this thread's share
+  auto load_rho = [&](int kz, double (&rr)[3]) {
     const long zoff = (long)(kz & ix.mz) * g.nxy;
-    for (int e = t; e < TPL; e += bd) {
-      const int row = e / nx, col = e - row * nx;
-      tt[s * TPL + e] = at / a.rho[zoff + (long)((j0 + row - 1) & ix.my) * nx + col];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int e = t + u * bd;
+      if (e < TPL) {
+        const int row = e / nx, col = e - row * nx;
+        rr[u] = a.rho[zoff + (long)((j0 + row - 1) & ix.my) * nx + col];
+      }
+    }
+  };
+  // T = (P0/R)/Rho into ring slot s
+  auto store_T = [&](const double (&rr)[3], int s) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int e = t + u * bd;
+      if (e < TPL) tt[s * TPL + e] = at / rr[u];
     }
   };
   auto slot_of = [&](int kz) { return (kz - z0 + 3) % 3; };  // kz >= z0 - 2
-  stage_T(z0 - 2, slot_of(z0 - 2));
-  stage_T(z0 - 1, slot_of(z0 - 1));
-  stage_T(z0, slot_of(z0));
+  auto cell0 = [&](int kz) { return ((long)(kz & ix.mz) * g.ny + j0) * nx; };  // first cell of the rows
+  {
+    double r0[3], r1[3], r2[3];
+    load_rho(z0 - 2, r0);
+    load_rho(z0 - 1, r1);
+    load_rho(z0, r2);
+    store_T(r0, slot_of(z0 - 2));
+    store_T(r1, slot_of(z0 - 1));
+    store_T(r2, slot_of(z0));
+  }
+  // this plane's sorted ranges and the particles of its first chunk are
+  // loaded one plane ahead
+  int32_t Qa, Qb, q0, q1;
+  {
+    const long cb = cell0(z0 - 1);
+    Qa = a.start[cb]; Qb = a.start[cb + bd];
+    q0 = a.start[cb + t]; q1 = a.start[cb + t + 1];
+  }
+  int32_t pp[kHeatChunk];
+  double px[kHeatChunk], py[kHeatChunk], pz[kHeatChunk], pq[kHeatChunk];
+  auto load_chunk = [&](int32_t c0, int32_t ce) {
+#pragma unroll
+    for (int e = 0; e < kHeatChunk; ++e) {
+      const int32_t q = c0 + e * bd + t;
+      pp[e] = q < ce ? a.perm[q] : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < kHeatChunk; ++e)
+      if (c0 + e * bd + t < ce) {
+        px[e] = a.x[pp[e]]; py[e] = a.y[pp[e]]; pz[e] = a.z[pp[e]]; pq[e] = a.Tp[pp[e]];
+      }
+  };
+  load_chunk(Qa, min(Qa + kHeatChunk * bd, Qb));
   __syncthreads();
   double sa[3] = {0., 0., 0.}, sb[3] = {0., 0., 0.};
   for (int zz = -1; zz <= a.zc; ++zz) {
     const int kabs = z0 + zz, k = kabs & ix.mz;
     const int sm = slot_of(kabs - 1), s0 = slot_of(kabs), sp = slot_of(kabs + 1);
-    const long cb = ((long)k * g.ny + j0) * nx;  // first cell of the block's rows
-    const int32_t Qa = a.start[cb], Qb = a.start[cb + bd];
-    const int32_t q0 = a.start[cb + t], q1 = a.start[cb + t + 1];
-    // gather of one particle at sorted position q into record slot e
+    // next plane's ranges, and Rho of the plane staged after this one
+    int32_t nQa = 0, nQb = 0, nq0 = 0, nq1 = 0;
+    if (zz < a.zc) {
+      const long cb = cell0(kabs + 1);
+      nQa = a.start[cb]; nQb = a.start[cb + bd];
+      nq0 = a.start[cb + t]; nq1 = a.start[cb + t + 1];
+    }
+    double rr[3];
+    if (zz < a.zc) load_rho(kabs + 2, rr);
+    // gather of one particle into record slot e
     auto gather = [&](int32_t p, double x, double y, double z, double tp, int e) {
       const double fx = x * g.idx, fy = y * g.idy, fz = z * g.idz;
       const int ii = (int)(round(fx)), jj = (int)(round(fy)), kk = (int)(round(fz));
@@ -377,15 +425,17 @@ __global__ __launch_bounds__(kHeatMaxBlock) void k_heat_fused(HeatFusedArgs a) {
       const int ux = ii - bi, uy = jj - bj, uz = kk - bk;
       const int ci = (bi & ix.mx) + ux, crow = (bj & ix.my) - j0 + 1 + uy;
       const int s1 = uz ? sp : s0, s2 = uz ? s0 : sm;  // planes kk and kk - 1
-      auto T = [&](int s, int row, int col) { return tt[s * TPL + row * nx + (col & ix.mx)]; };
-      // gas2part_Temp (particle.cpp:402-456): T at (ii - a, jj - b, kk - c)
-      const double Tg = xl * yl * zl * T(s1, crow, ci) +
-                        (1 - xl) * yl * zl * T(s1, crow, ci - 1) + xl * (1 - yl) * zl * T(s1, crow - 1, ci) +
-                        xl * yl * (1 - zl) * T(s2, crow, ci) +
-                        (1 - xl) * (1 - yl) * zl * T(s1, crow - 1, ci - 1) +
-                        (1 - xl) * yl * (1 - zl) * T(s2, crow, ci - 1) +
-                        xl * (yl - 1) * (zl - 1) * T(s2, crow - 1, ci) +
-                        (1 - xl) * (1 - yl) * (1 - zl) * T(s2, crow - 1, ci - 1);
+      // gas2part_Temp (particle.cpp:402-456): the trilinear weights of T at
+      // (ii - a, jj - b, kk - c), a, b, c in {0, 1}, taken as nested
+      // interpolations along x, y, z (the same weights; rounding differs
+      // from the eight-term sum at the last bit)
+      const int ch = ci & ix.mx, cl = (ci - 1) & ix.mx;
+      const double *r11 = tt + s1 * TPL + crow * nx, *r10 = r11 - nx;  // plane kk, rows jj, jj - 1
+      const double *r01 = tt + s2 * TPL + crow * nx, *r00 = r01 - nx;  // plane kk - 1
+      auto lx = [&](const double *row) { return fma(xl, row[ch] - row[cl], row[cl]); };
+      const double t1 = fma(yl, lx(r11) - lx(r10), lx(r10));
+      const double t0 = fma(yl, lx(r01) - lx(r00), lx(r00));
+      const double Tg = fma(zl, t1 - t0, t0);
       const double v = (Tg - tp) * a.iTth;
       a.rhs_t[p] = v;
       rec[e] = make_double4(xl, yl, zl, v);
@@ -396,21 +446,10 @@ __global__ __launch_bounds__(kHeatMaxBlock) void k_heat_fused(HeatFusedArgs a) {
     for (int u = 0; u < 27; ++u) m[u / 9][(u / 3) % 3][u % 3] = 0.0;
     for (int32_t c0 = Qa; c0 < Qb; c0 += kHeatChunk * bd) {
       const int32_t ce = min(c0 + kHeatChunk * bd, Qb);
-      int32_t pp[kHeatChunk];
-      double px[kHeatChunk], py[kHeatChunk], pz[kHeatChunk], pt[kHeatChunk];
-#pragma unroll
-      for (int e = 0; e < kHeatChunk; ++e) {
-        const int32_t q = c0 + e * bd + t;
-        pp[e] = q < ce ? a.perm[q] : 0;
-      }
+      if (c0 != Qa) load_chunk(c0, ce);  // planes holding more than one chunk
 #pragma unroll
       for (int e = 0; e < kHeatChunk; ++e)
-        if (c0 + e * bd + t < ce) {
-          px[e] = a.x[pp[e]]; py[e] = a.y[pp[e]]; pz[e] = a.z[pp[e]]; pt[e] = a.Tp[pp[e]];
-        }
-#pragma unroll
-      for (int e = 0; e < kHeatChunk; ++e)
-        if (c0 + e * bd + t < ce) gather(pp[e], px[e], py[e], pz[e], pt[e], e * bd + t);
+        if (c0 + e * bd + t < ce) gather(pp[e], px[e], py[e], pz[e], pq[e], e * bd + t);
       __syncthreads();
       // the records of cell (i, j, k) inside this chunk, in sorted order
       const int32_t qe = min(q1, ce);
@@ -421,15 +460,21 @@ __global__ __launch_bounds__(kHeatMaxBlock) void k_heat_fused(HeatFusedArgs a) {
         const double wx[3] = {ux ? 0.0 : 1 - v.x, ux ? 1 - v.x : v.x, ux ? v.x : 0.0};
         const double wy[3] = {uy ? 0.0 : 1 - v.y, uy ? 1 - v.y : v.y, uy ? v.y : 0.0};
         const double wz[3] = {uz ? 0.0 : 1 - v.z, uz ? 1 - v.z : v.z, uz ? v.z : 0.0};
+        // (w_x w_y RHS_T) w_z with fused multiply-adds: the moments are
+        // deposit sums, compared with a tolerance, not bit for bit
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-          for (int ay = 0; ay < 3; ++ay)
+          for (int ay = 0; ay < 3; ++ay) {
+            const double wxyv = wx[ax] * wy[ay] * v.w;
 #pragma unroll
-            for (int az = 0; az < 3; ++az) m[ax][ay][az] += wx[ax] * wy[ay] * wz[az] * v.w;
+            for (int az = 0; az < 3; ++az) m[ax][ay][az] = fma(wxyv, wz[az], m[ax][ay][az]);
+          }
       }
       __syncthreads();  // the records are read before the next chunk or the exchange
     }
+    // the next plane's first chunk, while this plane's moments are exchanged
+    if (zz < a.zc) load_chunk(nQa, min(nQa + kHeatChunk * bd, nQb));
     // x exchange in two rounds of 9: the right neighbour's offset -1 moments,
     // then the left neighbour's offset +1 moments
     double R[3][3];
@@ -455,8 +500,9 @@ __global__ __launch_bounds__(kHeatMaxBlock) void k_heat_fused(HeatFusedArgs a) {
       sb[e] = R[e][2];
     }
     // plane k - 1 is no longer needed: stage plane k + 2 into its slot
-    if (zz < a.zc) stage_T(kabs + 2, sm);
+    if (zz < a.zc) store_T(rr, sm);
     __syncthreads();
+    Qa = nQa; Qb = nQb; q0 = nq0; q1 = nq1;
   }
 }
 
