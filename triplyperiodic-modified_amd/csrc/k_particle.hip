@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kHeatMaxBlock) void k_heat_fused(HeatFusedArgs a) {
   __syncthreads();
   double sa[3] = {0., 0., 0.}, sb[3] = {0., 0., 0.};
   for (int zz = -1; zz <= a.zc; ++zz) {
-    const int kabs = z0 + zz, k = kabs & ix.mz;
+    const int kabs = z0 + zz;
     const int sm = slot_of(kabs - 1), s0 = slot_of(kabs), sp = slot_of(kabs + 1);
     // next plane's ranges, and Rho of the plane staged after this one
     int32_t nQa = 0, nQb = 0, nq0 = 0, nq1 = 0;
