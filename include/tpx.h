@@ -140,6 +140,24 @@ int tpx_step(tpx_ctx *ctx, int nsteps);
  * context reported; tpx_last_error of that context has the message. */
 int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency);
 
+/* ---- a batch of boxes stepped by batched kernels (no counterpart in the
+ * reference): every kernel of a step is launched once for all boxes, the box
+ * in the grid's third dimension and each box's arguments in a table.  The
+ * contexts keep their own state, which the other entries read and write as
+ * usual between batch steps; each box advances exactly as tpx_step would
+ * advance it alone.  They must be distinct, on one device, with equal grid
+ * sizes and the same switches that select kernels (TWC, heat exchange on or
+ * off, particles present, per-particle tau / eps, Poisson pass count);
+ * particle counts, box lengths and all coefficients may differ.  Kernel timing
+ * must be off.  tpx_batch_create reports why it failed through
+ * tpx_batch_error(); tpx_batch_step through tpx_batch_last_error(). */
+typedef struct tpx_batch tpx_batch;
+int tpx_batch_create(tpx_ctx *const *ctxs, int n, tpx_batch **out);
+int tpx_batch_step(tpx_batch *b, int nsteps);
+int tpx_batch_destroy(tpx_batch *b);
+const char *tpx_batch_error(void);
+const char *tpx_batch_last_error(tpx_batch *b);
+
 /* ---- reductions of tensor0 (tensor0.cpp:364-396,466-478) on a field */
 int tpx_field_sum(tpx_ctx *ctx, int field, int comp, double *out);
 int tpx_field_max_abs(tpx_ctx *ctx, int field, int comp, double *out);

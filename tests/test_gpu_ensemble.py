@@ -1,5 +1,5 @@
-"""The ensemble entry point (a batch of independent boxes stepped
-concurrently): every box must equal its own plain run bit for bit."""
+"""The ensemble entry points (independent boxes stepped concurrently, or by
+batched kernels): every box must equal its own plain run bit for bit."""
 import numpy as np
 import pytest
 
@@ -39,3 +39,39 @@ def test_tuner_returns_a_candidate(gpu_available):
     s = [solver_from_case(c) for c in cases]
     c = tpx.tune_concurrency(s, candidates=(1, 2, 4), steps=1)
     assert c in (1, 2, 4)
+
+
+@pytest.mark.parametrize("heat", [False, True])
+def test_batched_boxes_equal_their_own_runs(gpu_available, heat):
+    """One launch per kernel for all boxes (box in the grid's third dimension,
+    arguments from a table): boxes with different particle counts, viscosities,
+    Stokes numbers and time steps each equal their own solo run bit for bit,
+    Poisson iteration counts included."""
+    cases = ic.ensemble_cases(n_boxes=5, n=16, seed=7, np_range=(200, 3000))
+    if heat:
+        cases = [ic.make_case(16, npart, nu=0.01 + 0.004 * b, seed=20 + b, St=0.5 + b, heat=True, nsteps=4)
+                 for b, npart in enumerate((300, 1200, 700, 2500))]
+    bat = [solver_from_case(c) for c in cases]
+    solo = [solver_from_case(c) for c in cases]
+    batch = tpx.Batch(bat)
+    batch.step(2)
+    batch.step(2)
+    batch.close()
+    for s in solo:
+        s.step(4)
+    for b, (a, s) in enumerate(zip(bat, solo)):
+        for f in ("RU", "Rho", "P"):
+            assert np.array_equal(a.get(f), s.get(f)), (b, f)
+        pa, ps = a.particles(), s.particles()
+        for q in ("x", "y", "z", "u", "v", "w", "T", "flag"):
+            assert np.array_equal(pa[q], ps[q]), (b, q)
+        sa, ss = a.scalars(), s.scalars()
+        assert sa["num_timestep"] == ss["num_timestep"] == 4
+        assert sa["total_iterations"] == ss["total_iterations"], b
+
+
+def test_batch_rejects_mismatched_boxes(gpu_available):
+    a = ic.make_case(16, 200, nu=0.01, seed=1, heat=False)
+    b = ic.make_case(16, 200, nu=0.01, seed=2, heat=True)
+    with pytest.raises(tpx.TpxError, match="heat"):
+        tpx.Batch([solver_from_case(a), solver_from_case(b)])

@@ -192,7 +192,9 @@ __device__ __forceinline__ double residual(const XArgs &a, const Ix &ix, int i, 
 }
 
 // X forward pass: lpb consecutive x lines per block
-__global__ __launch_bounds__(kThreads) void k_xfwd(XArgs a, int lpb) {
+struct k_xfwd {
+  static constexpr int kBounds = kThreads;
+  static __device__ __forceinline__ void run(XArgs a, int lpb) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   const Geo &g = a.g;
@@ -217,6 +219,7 @@ __global__ __launch_bounds__(kThreads) void k_xfwd(XArgs a, int lpb) {
     if (line < nlines) a.spec[line * mxh + m] = lds[l * nx + m];
   }
 }
+};
 
 struct XInvArgs {
   Geo g;
@@ -229,7 +232,9 @@ struct XInvArgs {
 };
 
 // X inverse pass: Hermitian extension, inverse FFT, P += dp, block max|dp|
-__global__ __launch_bounds__(kThreads) void k_xinv(XInvArgs a, int lpb) {
+struct k_xinv {
+  static constexpr int kBounds = kThreads;
+  static __device__ __forceinline__ void run(XInvArgs a, int lpb) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   __shared__ double red[kThreads];
@@ -267,10 +272,13 @@ __global__ __launch_bounds__(kThreads) void k_xinv(XInvArgs a, int lpb) {
   }
   if (threadIdx.x == 0) a.partials[blk] = red[0];
 }
+};
 
 // Y pass (forward or inverse) on a tile of kTile kx columns at fixed z
 template <bool INV>
-__global__ __launch_bounds__(kThreads) void k_ypass(double2 *spec, int mxh, int ny, int nz,
+struct k_ypass {
+  static constexpr int kBounds = kThreads;
+  static __device__ __forceinline__ void run(double2 *spec, int mxh, int ny, int nz,
                                                     const double2 *__restrict__ tw, int kTile, const double *__restrict__ done) {
   if (*done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
@@ -292,9 +300,12 @@ __global__ __launch_bounds__(kThreads) void k_ypass(double2 *spec, int mxh, int 
     if (col < cols) base[(long)j * mxh + col] = lds[e];
   }
 }
+};
 
 // Z pass: forward FFT along z, divide by lambda (poisson.cpp:433-451), inverse
-__global__ __launch_bounds__(kThreads) void k_zpass(double2 *spec, int mxh, int ny, int nz,
+struct k_zpass {
+  static constexpr int kBounds = kThreads;
+  static __device__ __forceinline__ void run(double2 *spec, int mxh, int ny, int nz,
                                                     const double2 *__restrict__ tw,
                                                     const double *__restrict__ lamx,
                                                     const double *__restrict__ lamy,
@@ -333,6 +344,7 @@ __global__ __launch_bounds__(kThreads) void k_zpass(double2 *spec, int mxh, int 
     if (col < cols) base[(long)k * stride + col] = lds[e];
   }
 }
+};
 
 // ---------------------------------------------------------------------------
 // Four-step register FFT for lines of n = R1 * R2 points (R1 >= R2, both in
@@ -408,7 +420,9 @@ __device__ __forceinline__ void step_a(double2 (&v)[R2], int t, const double2 *_
 
 // X forward, four-step: C lines per block, R1 work items per line in step A
 template <int R1, int R2>
-__global__ __launch_bounds__(256) void k_xfwd4(XArgs a, int C) {
+struct k_xfwd4 {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(XArgs a, int C) {
   if (*a.done != 0.0) return;  // the solve has converged
   // The exchange moves real and imaginary parts in two rounds through an
   // LDS buffer of doubles: half the LDS per block, more blocks per CU.
@@ -459,10 +473,13 @@ __global__ __launch_bounds__(256) void k_xfwd4(XArgs a, int C) {
     }
   }
 }
+};
 
 // X inverse, four-step: Hermitian extension, P += dp, block max |dp|
 template <int R1, int R2>
-__global__ __launch_bounds__(256) void k_xinv4(XInvArgs a, int C) {
+struct k_xinv4 {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(XInvArgs a, int C) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   __shared__ double red[256];
@@ -522,6 +539,7 @@ __global__ __launch_bounds__(256) void k_xinv4(XInvArgs a, int C) {
   }
   if (threadIdx.x == 0) a.partials[blk] = red[0];
 }
+};
 
 // Strided lines (the y and z passes): C kx columns per block, consecutive
 // threads on consecutive columns.  `base` points at element 0 of column 0,
@@ -539,7 +557,9 @@ __device__ __forceinline__ void strided_fwd_half(double2 (&v)[R2], double2 *lds,
 }
 
 template <int R1, int R2, bool INV>
-__global__ __launch_bounds__(256) void k_ypass4(double2 *spec, int mxh, int ny, int nz,
+struct k_ypass4 {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(double2 *spec, int mxh, int ny, int nz,
                                                 const double2 *__restrict__ tw, int C,
                                                 const double *__restrict__ done) {
   if (*done != 0.0) return;  // the solve has converged
@@ -566,11 +586,14 @@ __global__ __launch_bounds__(256) void k_ypass4(double2 *spec, int mxh, int ny, 
     }
   }
 }
+};
 
 // Z pass: forward FFT along z, divide by lambda (poisson.cpp:433-451), inverse
 // FFT along z with the transposed factorisation, all on one load and store.
 template <int R1, int R2>
-__global__ __launch_bounds__(256) void k_zpass4(double2 *spec, int mxh, int ny, int nz,
+struct k_zpass4 {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(double2 *spec, int mxh, int ny, int nz,
                                                 const double2 *__restrict__ tw,
                                                 const double *__restrict__ lamx,
                                                 const double *__restrict__ lamy,
@@ -639,6 +662,7 @@ __global__ __launch_bounds__(256) void k_zpass4(double2 *spec, int mxh, int ny, 
     }
   }
 }
+};
 
 // ---------------------------------------------------------------------------
 // Three-pass iteration (every axis length 16..256).  The y transform is split
@@ -665,7 +689,9 @@ __global__ __launch_bounds__(256) void k_zpass4(double2 *spec, int mxh, int ny, 
 constexpr int kRunPad = 8;  // spare complex values after each nz Y1 run
 // ---------------------------------------------------------------------------
 template <int R1, int R2, int Y2>
-__global__ __launch_bounds__(256) void k_xy_fwd(XArgs a, int Y1, const double2 *__restrict__ twy) {
+struct k_xy_fwd {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(XArgs a, int Y1, const double2 *__restrict__ twy) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double ldsd[];
   constexpr int n = R1 * R2, P = R1 + 1, SP = Y2 + 1;  // padded rows of the x exchange and of the kx columns
@@ -726,9 +752,12 @@ __global__ __launch_bounds__(256) void k_xy_fwd(XArgs a, int Y1, const double2 *
     for (int k2 = 0; k2 < Y2; ++k2) dst[k2 * ks] = w[k2];
   }
 }
+};
 
 template <int Y1, int Z1, int Z2>
-__global__ __launch_bounds__(256) void k_yz_pass(double2 *spec, int mxh, int Y2, const double2 *__restrict__ twz,
+struct k_yz_pass {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(double2 *spec, int mxh, int Y2, const double2 *__restrict__ twz,
                                                  const double *__restrict__ lamx,
                                                  const double *__restrict__ lamy,
                                                  const double *__restrict__ lamz,
@@ -840,9 +869,12 @@ __global__ __launch_bounds__(256) void k_yz_pass(double2 *spec, int mxh, int Y2,
   __syncthreads();
   for (int e = tid; e < NE; e += blockDim.x) base[e] = L[pad(e)];
 }
+};
 
 template <int R1, int R2, int Y2>
-__global__ __launch_bounds__(256) void k_xy_inv(XInvArgs a, int Y1, const double2 *__restrict__ twy) {
+struct k_xy_inv {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(XInvArgs a, int Y1, const double2 *__restrict__ twy) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double ldsd[];
   __shared__ double red[256];
@@ -926,12 +958,15 @@ __global__ __launch_bounds__(256) void k_xy_inv(XInvArgs a, int Y1, const double
   }
   if (threadIdx.x == 0) a.partials[blk] = red[0];
 }
+};
 
 // End of one iteration: max |dp| over the block maxima into SC_MAX_DP, the
 // iteration count, and the loop test of poisson.cpp:391-462 (continue while
 // count < Iteration and max |dp| > epsilon) into SC_DONE.  A skipped
 // iteration leaves every slot alone.
-__global__ __launch_bounds__(1024) void k_finish_iter(const double *__restrict__ partials, long nparts,
+struct k_finish_iter {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(const double *__restrict__ partials, long nparts,
                                                       double *scal, int iteration, double epsilon) {
   __shared__ double sh[1024];
   if (scal[SC_DONE] != 0.0) return;
@@ -950,12 +985,27 @@ __global__ __launch_bounds__(1024) void k_finish_iter(const double *__restrict__
     scal[SC_DONE] = (it < iteration && err > epsilon) ? 0.0 : 1.0;
   }
 }
+};
 
 // Start of a solve: no iteration yet, err = 100 (poisson.cpp:391-396)
-__global__ void k_solve_begin(double *scal, int iteration, double epsilon) {
+struct k_solve_begin {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(double *scal, int iteration, double epsilon) {
   scal[SC_ITERS] = 0.0;
   scal[SC_DONE] = (0 < iteration && 100.0 > epsilon) ? 0.0 : 1.0;
 }
+};
+
+// the loop state of a solve (iterations run, stopped) into dst[0..1]
+struct k_solve_state {
+  static constexpr int kBounds = 64;
+  static __device__ __forceinline__ void run(const double *scal, double *dst) {
+    if (threadIdx.x == 0) {
+      dst[0] = scal[SC_ITERS];
+      dst[1] = scal[SC_DONE];
+    }
+  }
+};
 
 // lines per block of the x passes and kx columns per block of the y/z
 // passes: 2048 points per block keeps every FFT pass within kMaxBf
@@ -1026,30 +1076,35 @@ static bool four_step(int n, int &r1, int &r2) {
 template <int R1, int R2>
 void xy_fwd_launch(unsigned nb, size_t sh, hipStream_t st, const XArgs &xa, int Y1, int Y2, const double2 *twy) {
   switch (Y2) {
-    case 4: k_xy_fwd<R1, R2, 4><<<nb, 4 * R1, sh, st>>>(xa, Y1, twy); break;
-    case 8: k_xy_fwd<R1, R2, 8><<<nb, 8 * R1, sh, st>>>(xa, Y1, twy); break;
-    default: k_xy_fwd<R1, R2, 16><<<nb, 16 * R1, sh, st>>>(xa, Y1, twy); break;
+    case 4: tpx_launch<k_xy_fwd<R1, R2, 4>>(nb, 4 * R1, sh, st, xa, Y1, twy); break;
+    case 8: tpx_launch<k_xy_fwd<R1, R2, 8>>(nb, 8 * R1, sh, st, xa, Y1, twy); break;
+    default: tpx_launch<k_xy_fwd<R1, R2, 16>>(nb, 16 * R1, sh, st, xa, Y1, twy); break;
   }
 }
 template <int R1, int R2>
 void xy_inv_launch(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, int Y1, int Y2, const double2 *twy) {
   switch (Y2) {
-    case 4: k_xy_inv<R1, R2, 4><<<nb, 4 * R1, sh, st>>>(xi, Y1, twy); break;
-    case 8: k_xy_inv<R1, R2, 8><<<nb, 8 * R1, sh, st>>>(xi, Y1, twy); break;
-    default: k_xy_inv<R1, R2, 16><<<nb, 16 * R1, sh, st>>>(xi, Y1, twy); break;
+    case 4: tpx_launch<k_xy_inv<R1, R2, 4>>(nb, 4 * R1, sh, st, xi, Y1, twy); break;
+    case 8: tpx_launch<k_xy_inv<R1, R2, 8>>(nb, 8 * R1, sh, st, xi, Y1, twy); break;
+    default: tpx_launch<k_xy_inv<R1, R2, 16>>(nb, 16 * R1, sh, st, xi, Y1, twy); break;
   }
 }
 template <int Z1, int Z2>
 void yz_launch(unsigned nb, size_t sh, hipStream_t st, double2 *spec, int mxh, int Y1, int Y2, const tpx_ctx *c) {
   switch (Y1) {
-    case 4: k_yz_pass<4, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
-    case 8: k_yz_pass<8, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
-    default: k_yz_pass<16, Z1, Z2><<<nb, 256, sh, st>>>(spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
+    case 4: tpx_launch<k_yz_pass<4, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
+    case 8: tpx_launch<k_yz_pass<8, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
+    default: tpx_launch<k_yz_pass<16, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
   }
 }
 
+void launch_solve_state(tpx_ctx *c, double *dst) {
+  tpx_launch<k_solve_state>(1, 64, 0, c->stream, (const double *)c->scal, dst);
+  TPX_LAUNCH_CHECK();
+}
+
 void launch_solve_begin(tpx_ctx *c) {
-  k_solve_begin<<<1, 1, 0, c->stream>>>(c->scal, c->Iteration, c->p.epsilon);
+  tpx_launch<k_solve_begin>(1, 1, 0, c->stream, c->scal, c->Iteration, c->p.epsilon);
   TPX_LAUNCH_CHECK();
 }
 
@@ -1081,7 +1136,7 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     TPX_FOUR_STEP_SWITCH(g.nx, L_XYI)
 #undef L_XYI
     TPX_LAUNCH_CHECK();
-    k_finish_iter<<<1, 1024, 0, c->stream>>>(c->partials, nb13, c->scal, c->Iteration, c->p.epsilon);
+    tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nb13, c->scal, c->Iteration, c->p.epsilon);
     TPX_LAUNCH_CHECK();
     timing_end(c, TPX_K_FFT);
     return;
@@ -1091,11 +1146,11 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
   const size_t shx4 = x4 ? sizeof(double2) * (size_t)cx * r2 * (r1 + 1) : 0;
   const size_t shx = sizeof(double2) * (size_t)g.nx * cx;
   if (x4) {
-#define L_XF(A, B) k_xfwd4<A, B><<<nbx, cx * A, shx4 / 2, c->stream>>>(xa, cx)
+#define L_XF(A, B) tpx_launch<k_xfwd4<A, B>>(nbx, cx * A, shx4 / 2, c->stream, xa, cx)
     TPX_FOUR_STEP_SWITCH(g.nx, L_XF)
 #undef L_XF
   } else {
-    k_xfwd<<<nbx, kThreads, shx, c->stream>>>(xa, cx);
+    tpx_launch<k_xfwd>(nbx, kThreads, shx, c->stream, xa, cx);
   }
   TPX_LAUNCH_CHECK();
   const bool y4 = four_step(g.ny, r1, r2);
@@ -1103,13 +1158,13 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     const int C = 128 / r1;
     const unsigned nb = (unsigned)(((mxh + C - 1) / C) * g.nz);
     const size_t sh = sizeof(double2) * (size_t)C * g.ny;
-#define L_YF(A, B) k_ypass4<A, B, false><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], C, done)
+#define L_YF(A, B) tpx_launch<k_ypass4<A, B, false>>(nb, C * A, sh, c->stream, c->spec[0], mxh, g.ny, g.nz, c->tw[1], C, done)
     TPX_FOUR_STEP_SWITCH(g.ny, L_YF)
 #undef L_YF
   } else {
     int ty = lines_per_block(g.ny);
     unsigned nby = (unsigned)(((mxh + ty - 1) / ty) * g.nz);
-    k_ypass<false><<<nby, kThreads, sizeof(double2) * (size_t)ty * g.ny, c->stream>>>(c->spec[0], mxh, g.ny, g.nz,
+    tpx_launch<k_ypass<false>>(nby, kThreads, sizeof(double2) * (size_t)ty * g.ny, c->stream, c->spec[0], mxh, g.ny, g.nz,
                                                                                       c->tw[1], ty, done);
   }
   TPX_LAUNCH_CHECK();
@@ -1118,14 +1173,14 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     const unsigned nb = (unsigned)(((mxh + C - 1) / C) * g.ny);
     const size_t sh = sizeof(double2) * (size_t)C * g.nz;
 #define L_Z(A, B)                                                                                          \
-  k_zpass4<A, B><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[2], c->lam[0], c->lam[1], \
+  tpx_launch<k_zpass4<A, B>>(nb, C * A, sh, c->stream, c->spec[0], mxh, g.ny, g.nz, c->tw[2], c->lam[0], c->lam[1], \
                                                c->lam[2], C, done)
     TPX_FOUR_STEP_SWITCH(g.nz, L_Z)
 #undef L_Z
   } else {
     int tz = lines_per_block(g.nz);
     unsigned nbz = (unsigned)(((mxh + tz - 1) / tz) * g.ny);
-    k_zpass<<<nbz, kThreads, sizeof(double2) * (size_t)tz * g.nz, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[2],
+    tpx_launch<k_zpass>(nbz, kThreads, sizeof(double2) * (size_t)tz * g.nz, c->stream, c->spec[0], mxh, g.ny, g.nz, c->tw[2],
                                                                                c->lam[0], c->lam[1], c->lam[2], tz, done);
   }
   TPX_LAUNCH_CHECK();
@@ -1134,25 +1189,25 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     const int C = 128 / r1;
     const unsigned nb = (unsigned)(((mxh + C - 1) / C) * g.nz);
     const size_t sh = sizeof(double2) * (size_t)C * g.ny;
-#define L_YI(A, B) k_ypass4<A, B, true><<<nb, C * A, sh, c->stream>>>(c->spec[0], mxh, g.ny, g.nz, c->tw[1], C, done)
+#define L_YI(A, B) tpx_launch<k_ypass4<A, B, true>>(nb, C * A, sh, c->stream, c->spec[0], mxh, g.ny, g.nz, c->tw[1], C, done)
     TPX_FOUR_STEP_SWITCH(g.ny, L_YI)
 #undef L_YI
   } else {
     int ty = lines_per_block(g.ny);
     unsigned nby = (unsigned)(((mxh + ty - 1) / ty) * g.nz);
-    k_ypass<true><<<nby, kThreads, sizeof(double2) * (size_t)ty * g.ny, c->stream>>>(c->spec[0], mxh, g.ny, g.nz,
+    tpx_launch<k_ypass<true>>(nby, kThreads, sizeof(double2) * (size_t)ty * g.ny, c->stream, c->spec[0], mxh, g.ny, g.nz,
                                                                                      c->tw[1], ty, done);
   }
   TPX_LAUNCH_CHECK();
   if (x4) {
-#define L_XI(A, B) k_xinv4<A, B><<<nbx, cx * A, shx4, c->stream>>>(xi, cx)
+#define L_XI(A, B) tpx_launch<k_xinv4<A, B>>(nbx, cx * A, shx4, c->stream, xi, cx)
     TPX_FOUR_STEP_SWITCH(g.nx, L_XI)
 #undef L_XI
   } else {
-    k_xinv<<<nbx, kThreads, shx, c->stream>>>(xi, cx);
+    tpx_launch<k_xinv>(nbx, kThreads, shx, c->stream, xi, cx);
   }
   TPX_LAUNCH_CHECK();
-  k_finish_iter<<<1, 1024, 0, c->stream>>>(c->partials, nbx, c->scal, c->Iteration, c->p.epsilon);
+  tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nbx, c->scal, c->Iteration, c->p.epsilon);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_FFT);
 }

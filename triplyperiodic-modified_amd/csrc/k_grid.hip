@@ -119,7 +119,9 @@ struct RhsArgs {
 struct ZmGeom { int TX, TY, KC; };
 
 template <bool WITH_RHO, bool HAS_S, bool RK3>
-__global__ __launch_bounds__(256) void k_rhs_ru(Geo g, Coef kc, RhsArgs a, ZmGeom zg) {
+struct k_rhs_ru {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(Geo g, Coef kc, RhsArgs a, ZmGeom zg) {
   extern __shared__ double sm[];
   const int TX = zg.TX, TY = zg.TY, KC = zg.KC;
   const int RW = TX + 3, RN = RW * (TY + 3);   // raw tile: x from i0-2 to i0+TX
@@ -292,10 +294,13 @@ __global__ __launch_bounds__(256) void k_rhs_ru(Geo g, Coef kc, RhsArgs a, ZmGeo
     __syncthreads();
   }
 }
+};
 
 // G1 alone: Update_Rho (grid.cpp:608-614)
 template <bool RK3>
-__global__ __launch_bounds__(kBlock) void k_update_rho(Geo g, RhsArgs a) {
+struct k_update_rho {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(Geo g, RhsArgs a) {
   int i, j, k;
   long c;
   if (!cell_of(g, i, j, k, c)) return;
@@ -309,6 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_update_rho(Geo g, RhsArgs a) {
   a.rho_np1[c] = np1;
   if (!RK3) a.rho_new[c] = 1 * a.rho_base[c] + a.c_rho_new * R;
 }
+};
 
 struct DivArgs {
   RU3 ru_new;
@@ -340,7 +346,9 @@ __device__ __forceinline__ double low_mach_div(const Geo &g, const Ix &ix, const
 // fuses both, MODE 1 stores the divergence field, MODE 2 reads it.  The raw
 // right-hand side is stored; its mean is removed where it is consumed.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_div_rhs(Geo g, Coef kc, DivArgs a) {
+struct k_div_rhs {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(Geo g, Coef kc, DivArgs a) {
   __shared__ double sh[kBlock];
   int i, j, k;
   long c;
@@ -367,6 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_div_rhs(Geo g, Coef kc, DivArgs a) {
     if (threadIdx.x == 0) a.partials[xcd_block(blockIdx.x, gridDim.x)] = s;
   }
 }
+};
 
 struct ProjArgs {
   const double *P;
@@ -380,7 +389,9 @@ struct ProjArgs {
 // G10: Update_RU_WP (grid.cpp:857-863); at RK4_count==3 RU_new aliases
 // RU_np1 and the block sums of RU_np1 feed make_mean_U0.
 template <bool RK3>
-__global__ __launch_bounds__(kBlock) void k_project(Geo g, ProjArgs a) {
+struct k_project {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(Geo g, ProjArgs a) {
   __shared__ double sh[kBlock];
   int i, j, k;
   long c;
@@ -409,9 +420,12 @@ __global__ __launch_bounds__(kBlock) void k_project(Geo g, ProjArgs a) {
     }
   }
 }
+};
 
 // tensor0::make_mean_U0 (tensor0.cpp:386-396): out = (in - mean) + a
-__global__ __launch_bounds__(kBlock) void k_shift_mean(long n, const double *__restrict__ in,
+struct k_shift_mean {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(long n, const double *__restrict__ in,
                                                        double *__restrict__ out, const double *scal,
                                                        int slot, double ncell, double a) {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -419,10 +433,13 @@ __global__ __launch_bounds__(kBlock) void k_shift_mean(long n, const double *__r
   double mean = scal[slot] / ncell;  // mean() = sum()/size_tot()
   out[c] = (in[c] - mean) + a;
 }
+};
 
 // generic reduction of one field into per-block partials
 template <int OP>
-__global__ __launch_bounds__(kBlock) void k_reduce(long n, const double *__restrict__ f,
+struct k_reduce {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(long n, const double *__restrict__ f,
                                                    double *partials) {
   __shared__ double sh[kBlock];
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -431,12 +448,15 @@ __global__ __launch_bounds__(kBlock) void k_reduce(long n, const double *__restr
   double s = OP == 0 ? block_sum(v, sh) : block_max(v, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
+};
 
 // sum (OP 0) or max (OP 1) of nparts partials into scal[slot], fixed order:
 // thread t combines partials t, t + 1024, t + 2048, ... (coalesced reads,
 // four independent accumulators), then the block tree
 template <int OP>
-__global__ __launch_bounds__(1024) void k_finish(const double *__restrict__ partials, long nparts,
+struct k_finish {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(const double *__restrict__ partials, long nparts,
                                                  double *scal, int slot) {
   __shared__ double sh[1024];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -453,9 +473,12 @@ __global__ __launch_bounds__(1024) void k_finish(const double *__restrict__ part
   double s = OP == 0 ? block_sum(v, sh) : block_max(v, sh);
   if (threadIdx.x == 0) scal[slot] = s;
 }
+};
 
 // grid::Update_P0 (grid.cpp:816-830) on the device scalars
-__global__ void k_update_p0(Coef kc, double *scal, double ncell, double c_np1, double c_new) {
+struct k_update_p0 {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(Coef kc, double *scal, double ncell, double c_np1, double c_new) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double met = scal[SC_SUM_Q] / ncell;
   scal[SC_MET] = met;
@@ -465,20 +488,30 @@ __global__ void k_update_p0(Coef kc, double *scal, double ncell, double c_np1, d
   if (kc.rk != 3) scal[SC_P0_NEW] = scal[SC_P0] + c_new * dP0;
   else scal[SC_P0_NEW] = scal[SC_P0_NP1];
 }
+};
 
-__global__ void k_scalar_copy(double *scal, int dst, int src) {
+struct k_scalar_copy {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(double *scal, int dst, int src) {
   if (threadIdx.x == 0 && blockIdx.x == 0) scal[dst] = scal[src];
 }
+};
 
-__global__ void k_fill(double *f, double v, long n) {
+struct k_fill {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(double *f, double v, long n) {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n) f[c] = v;
 }
+};
 
-__global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, long n) {
+struct k_copy {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(double *__restrict__ d, const double *__restrict__ s, long n) {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n) d[c] = s[c];
 }
+};
 
 }  // namespace
 
@@ -496,8 +529,8 @@ void launch_update_rho(tpx_ctx *c, const Coef &k, const double *const ru_int[3],
   a.c_rho_np1 = -(k.dt * k.post);
   a.c_rho_new = -k.dt * k.pre;
   timing_begin(c, TPX_K_GRID_RHS);
-  if (k.rk == 3) k_update_rho<true><<<cells_grid(c), kBlock, 0, c->stream>>>(c->g, a);
-  else k_update_rho<false><<<cells_grid(c), kBlock, 0, c->stream>>>(c->g, a);
+  if (k.rk == 3) tpx_launch<k_update_rho<true>>(cells_grid(c), kBlock, 0, c->stream, c->g, a);
+  else tpx_launch<k_update_rho<false>>(cells_grid(c), kBlock, 0, c->stream, c->g, a);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_GRID_RHS);
 }
@@ -534,7 +567,7 @@ void launch_rhs_ru(tpx_ctx *c, const Coef &k, bool with_rho, const double *const
   size_t sh = sizeof(double) * (size_t)(8 * (zg.TX + 3) * (zg.TY + 3) + 4 * (zg.TX + 2) * (zg.TY + 2) +
                                         (zg.TX + 1) * (zg.TY + 1));
   timing_begin(c, TPX_K_GRID_RHS);
-#define TPX_K1(WR, HS, R3) k_rhs_ru<WR, HS, R3><<<nb, threads, sh, c->stream>>>(g, k, a, zg)
+#define TPX_K1(WR, HS, R3) tpx_launch<k_rhs_ru<WR, HS, R3>>(nb, threads, sh, c->stream, g, k, a, zg)
   if (with_rho) {
     if (hs) { if (r3) TPX_K1(true, true, true); else TPX_K1(true, true, false); }
     else { if (r3) TPX_K1(true, false, true); else TPX_K1(true, false, false); }
@@ -557,14 +590,14 @@ void launch_div_rhs(tpx_ctx *c, const Coef &k, int mode, const double *const ru_
   a.ncell = (double)c->g.n;
   unsigned nb = cells_grid(c);
   timing_begin(c, TPX_K_DIV_RHS);
-  if (mode == 0) k_div_rhs<0><<<nb, kBlock, 0, c->stream>>>(c->g, k, a);
-  else if (mode == 1) k_div_rhs<1><<<nb, kBlock, 0, c->stream>>>(c->g, k, a);
-  else k_div_rhs<2><<<nb, kBlock, 0, c->stream>>>(c->g, k, a);
+  if (mode == 0) tpx_launch<k_div_rhs<0>>(nb, kBlock, 0, c->stream, c->g, k, a);
+  else if (mode == 1) tpx_launch<k_div_rhs<1>>(nb, kBlock, 0, c->stream, c->g, k, a);
+  else tpx_launch<k_div_rhs<2>>(nb, kBlock, 0, c->stream, c->g, k, a);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_DIV_RHS);
   if (mode != 1) {
     timing_begin(c, TPX_K_OTHER);
-    k_finish<0><<<1, 1024, 0, c->stream>>>(c->partials, nb, c->scal, SC_SUM_RHS);
+    tpx_launch<k_finish<0>>(1, 1024, 0, c->stream, c->partials, nb, c->scal, SC_SUM_RHS);
     TPX_LAUNCH_CHECK();
     timing_end(c, TPX_K_OTHER);
   }
@@ -581,14 +614,14 @@ void launch_project(tpx_ctx *c, const Coef &k, const double *P, double *const ru
   a.c_np1 = -(k.dt * k.post);
   a.c_new = -(k.dt * k.pre);
   timing_begin(c, TPX_K_PROJECT);
-  if (copy_np1_to_new) k_project<true><<<nb, kBlock, 0, c->stream>>>(c->g, a);
-  else k_project<false><<<nb, kBlock, 0, c->stream>>>(c->g, a);
+  if (copy_np1_to_new) tpx_launch<k_project<true>>(nb, kBlock, 0, c->stream, c->g, a);
+  else tpx_launch<k_project<false>>(nb, kBlock, 0, c->stream, c->g, a);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_PROJECT);
   if (copy_np1_to_new && want_sums) {
     timing_begin(c, TPX_K_OTHER);
     for (int d = 0; d < 3; ++d)
-      k_finish<0><<<1, 1024, 0, c->stream>>>(c->partials + (long)d * nb, nb, c->scal, SC_MEAN_RU0 + d);
+      tpx_launch<k_finish<0>>(1, 1024, 0, c->stream, c->partials + (long)d * nb, nb, c->scal, SC_MEAN_RU0 + d);
     TPX_LAUNCH_CHECK();
     timing_end(c, TPX_K_OTHER);
   }
@@ -598,7 +631,7 @@ void launch_shift_mean(tpx_ctx *c, const double *const in[3], double *const out[
                        const double a[3]) {
   timing_begin(c, TPX_K_OTHER);
   for (int d = 0; d < 3; ++d)
-    k_shift_mean<<<cells_grid(c), kBlock, 0, c->stream>>>(c->g.n, in[d], out[d], c->scal,
+    tpx_launch<k_shift_mean>(cells_grid(c), kBlock, 0, c->stream, c->g.n, in[d], out[d], c->scal,
                                                          SC_MEAN_RU0 + d, (double)c->g.n, a[d]);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_OTHER);
@@ -608,40 +641,40 @@ void launch_reduce(tpx_ctx *c, const double *f, int op, int slot) {
   unsigned nb = cells_grid(c);
   timing_begin(c, TPX_K_OTHER);
   if (op == 0) {
-    k_reduce<0><<<nb, kBlock, 0, c->stream>>>(c->g.n, f, c->partials);
-    k_finish<0><<<1, 1024, 0, c->stream>>>(c->partials, nb, c->scal, slot);
+    tpx_launch<k_reduce<0>>(nb, kBlock, 0, c->stream, c->g.n, f, c->partials);
+    tpx_launch<k_finish<0>>(1, 1024, 0, c->stream, c->partials, nb, c->scal, slot);
   } else {
-    k_reduce<1><<<nb, kBlock, 0, c->stream>>>(c->g.n, f, c->partials);
-    k_finish<1><<<1, 1024, 0, c->stream>>>(c->partials, nb, c->scal, slot);
+    tpx_launch<k_reduce<1>>(nb, kBlock, 0, c->stream, c->g.n, f, c->partials);
+    tpx_launch<k_finish<1>>(1, 1024, 0, c->stream, c->partials, nb, c->scal, slot);
   }
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_OTHER);
 }
 
 void launch_finish_sum(tpx_ctx *c, int nparts, int part_offset, int slot) {
-  k_finish<0><<<1, 1024, 0, c->stream>>>(c->partials + part_offset, nparts, c->scal, slot);
+  tpx_launch<k_finish<0>>(1, 1024, 0, c->stream, c->partials + part_offset, nparts, c->scal, slot);
   TPX_LAUNCH_CHECK();
 }
 
 void launch_update_p0(tpx_ctx *c, const Coef &k) {
   timing_begin(c, TPX_K_OTHER);
-  k_update_p0<<<1, 64, 0, c->stream>>>(k, c->scal, (double)c->g.n, k.dt * k.post, k.dt * k.pre);
+  tpx_launch<k_update_p0>(1, 64, 0, c->stream, k, c->scal, (double)c->g.n, k.dt * k.post, k.dt * k.pre);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_OTHER);
 }
 
 void launch_scalar_copy(tpx_ctx *c, int dst, int src) {
-  k_scalar_copy<<<1, 64, 0, c->stream>>>(c->scal, dst, src);
+  tpx_launch<k_scalar_copy>(1, 64, 0, c->stream, c->scal, dst, src);
   TPX_LAUNCH_CHECK();
 }
 
 void launch_fill(tpx_ctx *c, double *f, double v, long n) {
-  k_fill<<<grid1d(n, kBlock), kBlock, 0, c->stream>>>(f, v, n);
+  tpx_launch<k_fill>(grid1d(n, kBlock), kBlock, 0, c->stream, f, v, n);
   TPX_LAUNCH_CHECK();
 }
 
 void launch_copy(tpx_ctx *c, double *dst, const double *src, long n) {
-  k_copy<<<grid1d(n, kBlock), kBlock, 0, c->stream>>>(dst, src, n);
+  tpx_launch<k_copy>(grid1d(n, kBlock), kBlock, 0, c->stream, dst, src, n);
   TPX_LAUNCH_CHECK();
 }
 

@@ -84,7 +84,9 @@ struct AdvArgs {
 
 // P1-P4, P6 shares and P7 wrap for one particle (particle.cpp:314-399,
 // 477-492, 494-520, 58-104)
-__global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a) {
+struct k_advance {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(AdvArgs a) {
   long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
   const Geo &g = a.g;
@@ -171,9 +173,12 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a) {
   a.key[p] = (int32_t)nk;
   if (a.cnt) atomicAdd(&a.cnt[nk], 1);  // no return value: the wave does not wait
 }
+};
 
 // floor-cell key of one stage of positions, counted into its cell
-__global__ void k_keys(Geo g, const double *x, const double *y, const double *z, int32_t *key,
+struct k_keys {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(Geo g, const double *x, const double *y, const double *z, int32_t *key,
                        int32_t *cnt, long n) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -182,26 +187,36 @@ __global__ void k_keys(Geo g, const double *x, const double *y, const double *z,
   key[p] = k;
   atomicAdd(&cnt[k], 1);
 }
+};
 
-__global__ void k_set_i(int32_t *a, int32_t v, long n) {
+struct k_set_i {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(int32_t *a, int32_t v, long n) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) a[p] = v;
 }
+};
 
-__global__ void k_copy_i(int32_t *__restrict__ d, const int32_t *__restrict__ s, long n) {
+struct k_copy_i {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(int32_t *__restrict__ d, const int32_t *__restrict__ s, long n) {
   long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) d[p] = s[p];
 }
+};
 
 // placement: each particle takes the next free slot of its cell's segment
 // (cursor starts as a copy of the CSR offsets)
 // placement: each particle takes the next free slot of its cell's segment
 // (cursor starts as a copy of the CSR offsets)
-__global__ __launch_bounds__(kBlock) void k_place(const int32_t *__restrict__ key, int32_t *__restrict__ cursor,
+struct k_place {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const int32_t *__restrict__ key, int32_t *__restrict__ cursor,
                                                   int32_t *__restrict__ perm, long n) {
   long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p < n) perm[atomicAdd(&cursor[key[p]], 1)] = (int32_t)p;
 }
+};
 
 __device__ __forceinline__ void cswap(int32_t &a, int32_t &b) {
   int32_t lo = a < b ? a : b, hi = a < b ? b : a;
@@ -213,7 +228,9 @@ __device__ __forceinline__ void cswap(int32_t &a, int32_t &b) {
 // physical index) whatever order the placement atomics took.  Segments of
 // up to 8 are sorted in registers by a fixed compare-exchange network
 // (padded with INT32_MAX); longer ones by insertion in memory.
-__global__ void k_fixup(const int32_t *__restrict__ start, int32_t *perm, int32_t *inv, long ncell) {
+struct k_fixup {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(const int32_t *__restrict__ start, int32_t *perm, int32_t *inv, long ncell) {
   long c = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   int32_t b = start[c], e = start[c + 1];
@@ -245,17 +262,23 @@ __global__ void k_fixup(const int32_t *__restrict__ start, int32_t *perm, int32_
   }
   for (int32_t q = b; q < e; ++q) inv[perm[q]] = q;
 }
+};
 
-__global__ void k_iota(int32_t *p, long n) {
+struct k_iota {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(int32_t *p, long n) {
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) p[q] = (int32_t)q;
 }
+};
 
 // ---- exclusive scan of int32 (three phases, block size 256 x 8) ----
 constexpr int kScanPer = 8;
 constexpr int kScanChunk = kBlock * kScanPer;
 
-__global__ __launch_bounds__(kBlock) void k_scan_blocks(int32_t *data, long n, int32_t *block_sums) {
+struct k_scan_blocks {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(int32_t *data, long n, int32_t *block_sums) {
   __shared__ int32_t sh[kBlock];
   long base = (long)blockIdx.x * kScanChunk + (long)threadIdx.x * kScanPer;
   int32_t v[kScanPer];
@@ -283,13 +306,17 @@ __global__ __launch_bounds__(kBlock) void k_scan_blocks(int32_t *data, long n, i
   }
   if (threadIdx.x == kBlock - 1) block_sums[blockIdx.x] = sh[kBlock - 1];
 }
+};
 
-__global__ __launch_bounds__(kBlock) void k_scan_add(int32_t *data, long n, const int32_t *offs) {
+struct k_scan_add {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(int32_t *data, long n, const int32_t *offs) {
   long q = (long)blockIdx.x * kScanChunk + threadIdx.x;
   int32_t o = offs[blockIdx.x];
   for (int e = 0; e < kScanPer; ++e, q += kBlock)
     if (q < n) data[q] += o;
 }
+};
 
 // ---- deposits ----
 constexpr int kHeatMaxBlock = 512;
@@ -329,7 +356,9 @@ struct HeatFusedArgs {
 // moments of its own cell's records.
 constexpr int kHeatChunk = 2;
 
-__global__ __launch_bounds__(kHeatMaxBlock) void k_heat_fused(HeatFusedArgs a) {
+struct k_heat_fused {
+  static constexpr int kBounds = kHeatMaxBlock;
+  static __device__ __forceinline__ void run(HeatFusedArgs a) {
   // LDS: 9 x blockDim doubles shared by the records of a chunk and the x
   // exchange, then T: 3 planes x (rows + 2) x nx
   extern __shared__ double xs[];
@@ -505,10 +534,13 @@ __global__ __launch_bounds__(kHeatMaxBlock) void k_heat_fused(HeatFusedArgs a) {
     Qa = nQa; Qb = nQb; q0 = nq0; q1 = nq1;
   }
 }
+};
 
 // Stage 2: Q(i, j, k) = t3[2](i, j-1, k) + t3[1](i, j, k) + t3[0](i, j+1, k),
 // with block partial sums for the mean of Q
-__global__ __launch_bounds__(kBlock) void k_heat_combine(Geo g, const double *__restrict__ t3, double *out,
+struct k_heat_combine {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(Geo g, const double *__restrict__ t3, double *out,
                                                          double *partials) {
   __shared__ double red[kBlock];
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
@@ -530,6 +562,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_combine(Geo g, const double *__
   }
   if (threadIdx.x == 0) partials[b] = red[0];
 }
+};
 
 // velocity deposit (P6): S_a(c) = [sum over the particles of cell c of their
 // (1 - frac_a) RHS_a] + [sum over cell c - e_a of frac_a RHS_a], each part in
@@ -582,7 +615,9 @@ __device__ __forceinline__ void cell_sums(const int32_t *__restrict__ start, lon
 // c4 of cells (i-1, j, k), (i, j-1, k), (i, j, k-1)) come from the
 // neighbouring thread through LDS, from the halo row j0 - 1, and from the
 // previous plane of the march.
-__global__ __launch_bounds__(kHeatMaxBlock) void k_vel_planes(Geo g, const int32_t *__restrict__ start,
+struct k_vel_planes {
+  static constexpr int kBounds = kHeatMaxBlock;
+  static __device__ __forceinline__ void run(Geo g, const int32_t *__restrict__ start,
                                                              const double *__restrict__ contrib, long cs,
                                                              double *Sx, double *Sy, double *Sz, int rows,
                                                              int zc) {
@@ -621,6 +656,7 @@ __global__ __launch_bounds__(kHeatMaxBlock) void k_vel_planes(Geo g, const int32
     __syncthreads();
   }
 }
+};
 
 struct CicDepArgs {
   Geo g;
@@ -633,7 +669,9 @@ struct CicDepArgs {
 
 // P5 pull (statistics only): concentration at x_int, weight one per particle
 // (particle.cpp:522-555)
-__global__ __launch_bounds__(kBlock) void k_concentration(CicDepArgs a) {
+struct k_concentration {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(CicDepArgs a) {
   __shared__ double sh[kBlock];
   const Geo &g = a.g;
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
@@ -669,6 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_concentration(CicDepArgs a) {
   }
   if (threadIdx.x == 0 && a.partials) a.partials[b] = sh[0];
 }
+};
 
 struct PermArgs {
   const int32_t *perm;       // sorted position -> particle
@@ -684,7 +723,9 @@ struct PermArgs {
 // physical re-sort into cell order: sorted position q gathers the base state
 // of particle perm[q] (the writes stay contiguous; a scatter of the same data
 // measured slower)
-__global__ __launch_bounds__(kBlock) void k_permute(PermArgs a) {
+struct k_permute {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(PermArgs a) {
   long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (q >= a.n) return;
   long s = a.perm[q];
@@ -695,6 +736,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermArgs a) {
     if (a.s1[d]) a.d1[d][q] = a.s1[d][s];
   a.flag_d[q] = a.flag_s[s];
 }
+};
 
 }  // namespace
 
@@ -707,7 +749,7 @@ static void check(hipError_t e) {
 
 void launch_iota(tpx_ctx *c, int32_t *p, int64_t n) {
   if (n <= 0) return;
-  k_iota<<<grid1d(n, kBlock), kBlock, 0, c->stream>>>(p, n);
+  tpx_launch<k_iota>(grid1d(n, kBlock), kBlock, 0, c->stream, p, n);
   TPX_LAUNCH_CHECK();
 }
 
@@ -719,7 +761,7 @@ void launch_exclusive_scan(tpx_ctx *c, int32_t *data, int64_t n) {
   int32_t *tmp = c->scan_tmp;
   while (true) {
     long nb = (len + kScanChunk - 1) / kScanChunk;
-    k_scan_blocks<<<(unsigned)nb, kBlock, 0, c->stream>>>(cur, len, tmp);
+    tpx_launch<k_scan_blocks>((unsigned)nb, kBlock, 0, c->stream, cur, len, tmp);
     TPX_LAUNCH_CHECK();
     levels.push_back({cur, len});
     if (nb == 1) break;
@@ -730,7 +772,7 @@ void launch_exclusive_scan(tpx_ctx *c, int32_t *data, int64_t n) {
   for (int l = (int)levels.size() - 2; l >= 0; --l) {
     long len_l = levels[l].second;
     long nb = (len_l + kScanChunk - 1) / kScanChunk;
-    k_scan_add<<<(unsigned)nb, kBlock, 0, c->stream>>>(levels[l].first, len_l, levels[l + 1].first);
+    tpx_launch<k_scan_add>((unsigned)nb, kBlock, 0, c->stream, levels[l].first, len_l, levels[l + 1].first);
     TPX_LAUNCH_CHECK();
   }
 }
@@ -742,14 +784,14 @@ static void bin_particles(tpx_ctx *c, const int32_t *key, int32_t *perm, int32_t
   Particles &pt = c->pt;
   long n = pt.n, nc = c->g.n;
   launch_exclusive_scan(c, start, nc + 1);
-  k_copy_i<<<grid1d(nc + 1, kBlock), kBlock, 0, c->stream>>>(pt.cursor, start, nc + 1);
-  k_place<<<grid1d(n, kBlock), kBlock, 0, c->stream>>>(key, pt.cursor, perm, n);
-  k_fixup<<<grid1d(nc, kBlock), kBlock, 0, c->stream>>>(start, perm, inv, nc);
+  tpx_launch<k_copy_i>(grid1d(nc + 1, kBlock), kBlock, 0, c->stream, pt.cursor, start, nc + 1);
+  tpx_launch<k_place>(grid1d(n, kBlock), kBlock, 0, c->stream, key, pt.cursor, perm, n);
+  tpx_launch<k_fixup>(grid1d(nc, kBlock), kBlock, 0, c->stream, start, perm, inv, nc);
   TPX_LAUNCH_CHECK();
 }
 
 static void zero_counts(tpx_ctx *c, int32_t *cnt) {
-  k_set_i<<<grid1d(c->g.n + 1, kBlock), kBlock, 0, c->stream>>>(cnt, 0, c->g.n + 1);
+  tpx_launch<k_set_i>(grid1d(c->g.n + 1, kBlock), kBlock, 0, c->stream, cnt, 0, c->g.n + 1);
   TPX_LAUNCH_CHECK();
 }
 
@@ -798,7 +840,7 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
     timing_end(c, TPX_K_SORT);
   }
   timing_begin(c, TPX_K_ADVANCE);
-  k_advance<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(a);
+  tpx_launch<k_advance>(grid1d(pt.n, kBlock), kBlock, 0, c->stream, a);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_ADVANCE);
   pt.np1_is_base = false;  // the np1 arrays now hold x_np1 = x + dt*post*u_int
@@ -828,7 +870,7 @@ void launch_velocity_deposit(tpx_ctx *c) {
   const size_t lds = sizeof(double) * 6 * kVelChunk * (size_t)h.threads;
   if (lds > c->max_lds) throw std::logic_error("velocity deposit needs more LDS than the device offers");
   const unsigned blocks = (unsigned)((g.ny / h.rows) * (g.nz / zc));
-  k_vel_planes<<<blocks, h.threads, lds, c->stream>>>(g, pt.start_int, pt.contrib, pt.cap, c->S[0], c->S[1],
+  tpx_launch<k_vel_planes>(blocks, h.threads, lds, c->stream, g, pt.start_int, pt.contrib, pt.cap, c->S[0], c->S[1],
                                                         c->S[2], h.rows, zc);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_DEPOSIT);
@@ -853,7 +895,7 @@ void launch_sort_stage(tpx_ctx *c, int stage, bool into_int) {
   int32_t *start = into_int ? pt.start_int : pt.start_new;
   timing_begin(c, TPX_K_SORT);
   zero_counts(c, start);
-  k_keys<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(c->g, pt.q[0][s], pt.q[1][s], pt.q[2][s], pt.key, start,
+  tpx_launch<k_keys>(grid1d(pt.n, kBlock), kBlock, 0, c->stream, c->g, pt.q[0][s], pt.q[1][s], pt.q[2][s], pt.key, start,
                                                          pt.n);
   TPX_LAUNCH_CHECK();
   bin_particles(c, pt.key, into_int ? pt.perm_int : pt.perm_new, into_int ? pt.inv_int : pt.inv_new, start);
@@ -887,10 +929,10 @@ void launch_heat_exchange(tpx_ctx *c, int stage, const double *rho, int p0_slot)
     a.zc = h.zc;
     size_t lds = sizeof(double) * (9 * (size_t)h.threads + 3 * (size_t)(h.rows + 2) * g.nx);
     if (lds > c->max_lds) throw std::logic_error("heat deposit needs more LDS than the device offers");
-    k_heat_fused<<<h.blocks, h.threads, lds, c->stream>>>(a);
+    tpx_launch<k_heat_fused>(h.blocks, h.threads, lds, c->stream, a);
     TPX_LAUNCH_CHECK();
     unsigned nb = grid1d(g.n, kBlock);
-    k_heat_combine<<<nb, kBlock, 0, c->stream>>>(g, c->dep_tmp, c->Q, c->partials);
+    tpx_launch<k_heat_combine>(nb, kBlock, 0, c->stream, g, c->dep_tmp, c->Q, c->partials);
     TPX_LAUNCH_CHECK();
     launch_finish_sum(c, (int)nb, 0, SC_SUM_Q);
   }
@@ -909,7 +951,7 @@ void launch_concentration(tpx_ctx *c) {
   a.partials = c->partials;
   unsigned nb = grid1d(c->g.n, kBlock);
   timing_begin(c, TPX_K_DEPOSIT);
-  k_concentration<<<nb, kBlock, 0, c->stream>>>(a);
+  tpx_launch<k_concentration>(nb, kBlock, 0, c->stream, a);
   TPX_LAUNCH_CHECK();
   launch_finish_sum(c, (int)nb, 0, SC_SUM_C);
   timing_end(c, TPX_K_DEPOSIT);
@@ -941,7 +983,7 @@ void launch_particle_permute(tpx_ctx *c) {
   a.flag_d = pt.itmp;
   a.n = pt.n;
   timing_begin(c, TPX_K_SORT);
-  k_permute<<<grid1d(pt.n, kBlock), kBlock, 0, c->stream>>>(a);
+  tpx_launch<k_permute>(grid1d(pt.n, kBlock), kBlock, 0, c->stream, a);
   TPX_LAUNCH_CHECK();
   for (int d = 0; d < 3; ++d)
     if (use[d]) std::swap(*own[d], pt.spare[d]);

@@ -17,6 +17,10 @@
 
 using namespace tpx;
 
+namespace tpx {
+thread_local Recorder *g_rec = nullptr;
+}
+
 namespace {
 
 thread_local std::string g_create_error;
@@ -308,8 +312,9 @@ void do_params_update(tpx_ctx *c) {
 // The fused substep: the reference's method sequence with Update_Rho moved
 // after the particle advance (it reads nothing the particles write) and
 // merged into the momentum kernel, and Compute_Div_U_new merged into
-// Compute_RHS_Pois.  Results are identical to the method sequence.
-void fused_substep(tpx_ctx *c, int rk) {
+// Compute_RHS_Pois.  Results are identical to the method sequence.  The
+// part before the Poisson solve:
+void fused_substep_pre(tpx_ctx *c, int rk) {
   Coef k = coef(c, rk);
   launch_update_p0(c, k);
   Particles &pt = c->pt;
@@ -338,9 +343,24 @@ void fused_substep(tpx_ctx *c, int rk) {
   const double *rn[3];
   ru_ptrs(c, c->ru_new, rn);
   launch_div_rhs(c, k, 0, rn, c->rho[c->rho_new], c->q_zero ? nullptr : c->Q, nullptr, c->rhs);
-  do_solve_poisson(c, rk, nullptr);
+}
+
+void fused_substep_post(tpx_ctx *c, int rk) {
   do_update_ru_wp(c, rk, rk == 3);
   do_time_advance_rk4(c);
+}
+
+void fused_substep(tpx_ctx *c, int rk) {
+  fused_substep_pre(c, rk);
+  do_solve_poisson(c, rk, nullptr);
+  fused_substep_post(c, rk);
+}
+
+// the end of a step after its four substeps (simulation.cpp:80-82)
+void step_end(tpx_ctx *c) {
+  const double a[3] = {c->p.u0 * c->p.rho0, c->p.v0 * c->p.rho0, c->p.w0 * c->p.rho0};
+  do_make_mean_u0(c, a, true);
+  do_time_advance(c);
 }
 
 void release(tpx_ctx *c) {
@@ -386,7 +406,7 @@ std::string validate(const tpx_params *p) {
 namespace tpx {
 
 void timing_begin(tpx_ctx *c, int cls) {
-  if (!c->tm.on) return;
+  if (!c->tm.on || g_rec) return;
   Timing &t = c->tm;
   hipEvent_t a, b;
   if (t.pool.size() >= 2) {
@@ -401,7 +421,7 @@ void timing_begin(tpx_ctx *c, int cls) {
 }
 
 void timing_end(tpx_ctx *c, int cls) {
-  if (!c->tm.on) return;
+  if (!c->tm.on || g_rec) return;
   Timing &t = c->tm;
   for (auto it = t.recs.rbegin(); it != t.recs.rend(); ++it)
     if (it->cls == cls) {
@@ -800,12 +820,10 @@ int tpx_time_advance(tpx_ctx *ctx) {
 int tpx_step(tpx_ctx *ctx, int nsteps) {
   return guarded(ctx, [&]() {
     if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
-    double a[3] = {ctx->p.u0 * ctx->p.rho0, ctx->p.v0 * ctx->p.rho0, ctx->p.w0 * ctx->p.rho0};
     for (int s = 0; s < nsteps; ++s) {
       do_params_update(ctx);
       for (int rk = 0; rk < 4; ++rk) fused_substep(ctx, rk);
-      do_make_mean_u0(ctx, a, true);
-      do_time_advance(ctx);
+      step_end(ctx);
     }
     sync(ctx);
     return TPX_OK;
@@ -836,6 +854,247 @@ int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency) 
   work();
   for (auto &t : pool) t.join();
   return first_err.load();
+}
+
+}  // extern "C"
+
+// ---- batches: one launch per kernel for many boxes ------------------------
+// The host runs each box's step logic with its launches recorded (buffer
+// roles and flags advance per box exactly as in tpx_step), then starts every
+// recorded kernel once for all boxes on the batch's stream, with the boxes'
+// arguments packed into a table in device memory.  The Poisson loop test is
+// per box on the device; the host queues iterations for all boxes until every
+// box has stopped.
+struct tpx_batch {
+  std::vector<tpx_ctx *> ctx;
+  std::vector<Recorder> rec;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  static constexpr int kRing = 4;  // argument tables in flight
+  char *h_tab[kRing] = {};
+  char *d_tab[kRing] = {};
+  size_t cap[kRing] = {};
+  hipEvent_t ev[kRing] = {};
+  int slot = 0;
+  double *d_state = nullptr, *h_state = nullptr;  // per box: iterations run, stopped
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_batch_error;
+
+void batch_release(tpx_batch *b) {
+  if (!b) return;
+  for (int i = 0; i < tpx_batch::kRing; ++i) {
+    if (b->h_tab[i]) (void)hipHostFree(b->h_tab[i]);
+    if (b->d_tab[i]) (void)hipFree(b->d_tab[i]);
+    if (b->ev[i]) (void)hipEventDestroy(b->ev[i]);
+  }
+  if (b->d_state) (void)hipFree(b->d_state);
+  if (b->h_state) (void)hipHostFree(b->h_state);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// record fn(ctx) for every box, then launch each recorded kernel once for all
+template <class Fn>
+void batch_run(tpx_batch *b, Fn &&fn) {
+  const int nb = (int)b->ctx.size();
+  for (int i = 0; i < nb; ++i) {
+    b->rec[i].clear();
+    g_rec = &b->rec[i];
+    try {
+      fn(b->ctx[i], i);
+    } catch (...) {
+      g_rec = nullptr;
+      throw;
+    }
+    g_rec = nullptr;
+  }
+  const Recorder &r0 = b->rec[0];
+  const size_t nk = r0.recs.size();
+  if (nk == 0) return;
+  std::vector<size_t> off(nk), stride(nk);
+  size_t total = 0;
+  for (size_t j = 0; j < nk; ++j) {
+    for (int i = 1; i < nb; ++i) {
+      const Recorder::Rec &a = r0.recs[j];
+      if (b->rec[i].recs.size() != nk) throw std::logic_error("boxes of a batch launch different kernel sequences");
+      const Recorder::Rec &o = b->rec[i].recs[j];
+      if (o.launch != a.launch || o.block.x != a.block.x || o.block.y != a.block.y || o.block.z != a.block.z ||
+          o.shmem != a.shmem || o.grid.z != a.grid.z)
+        throw std::logic_error("boxes of a batch launch different kernel sequences");
+    }
+    stride[j] = (r0.recs[j].size + 15) / 16 * 16;
+    off[j] = total;
+    total += (stride[j] * nb + 255) / 256 * 256;
+  }
+  const int s = b->slot;
+  b->slot = (s + 1) % tpx_batch::kRing;
+  hip_check(hipEventSynchronize(b->ev[s]), "hipEventSynchronize");  // the slot's last copy has been read
+  if (total > b->cap[s]) {
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    if (b->h_tab[s]) hip_check(hipHostFree(b->h_tab[s]), "hipHostFree");
+    if (b->d_tab[s]) hip_check(hipFree(b->d_tab[s]), "hipFree");
+    b->h_tab[s] = nullptr;
+    b->d_tab[s] = nullptr;
+    b->cap[s] = 0;
+    const size_t want = std::max<size_t>(total * 2, 1 << 16);
+    hip_check(hipHostMalloc((void **)&b->h_tab[s], want), "hipHostMalloc");
+    if (hipMalloc((void **)&b->d_tab[s], want) != hipSuccess) throw std::bad_alloc();
+    b->cap[s] = want;
+  }
+  char *h = b->h_tab[s];
+  for (size_t j = 0; j < nk; ++j)
+    for (int i = 0; i < nb; ++i) {
+      const Recorder::Rec &o = b->rec[i].recs[j];
+      memcpy(h + off[j] + (size_t)i * stride[j], b->rec[i].bytes.data() + o.off, o.size);
+    }
+  hip_check(hipMemcpyAsync(b->d_tab[s], h, total, hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
+  hip_check(hipEventRecord(b->ev[s], b->stream), "hipEventRecord");
+  for (size_t j = 0; j < nk; ++j) {
+    dim3 grid = r0.recs[j].grid;
+    for (int i = 1; i < nb; ++i) {  // particle kernels: the largest box's grid, the others' extra blocks exit
+      grid.x = std::max(grid.x, b->rec[i].recs[j].grid.x);
+      grid.y = std::max(grid.y, b->rec[i].recs[j].grid.y);
+    }
+    r0.recs[j].launch(grid, r0.recs[j].block, r0.recs[j].shmem, b->stream, b->d_tab[s] + off[j],
+                      (unsigned)stride[j], (unsigned)nb);
+    TPX_LAUNCH_CHECK();
+  }
+}
+
+void batch_solve(tpx_batch *b, int rk) {
+  const int nb = (int)b->ctx.size();
+  int itmax = 0, guess = 1;
+  for (tpx_ctx *c : b->ctx) {
+    itmax = std::max(itmax, c->Iteration);
+    guess = std::max(guess, c->iter_guess[rk & 3]);
+  }
+  batch_run(b, [](tpx_ctx *c, int) { launch_solve_begin(c); });
+  int queued = 0, batch = guess;
+  while (queued < itmax) {
+    const int n = std::min(batch, itmax - queued);
+    for (int q = 0; q < n; ++q)
+      batch_run(b, [](tpx_ctx *c, int) { launch_poisson_iteration(c, c->rho[c->rho_new]); });
+    queued += n;
+    batch_run(b, [&](tpx_ctx *c, int i) { launch_solve_state(c, b->d_state + 2 * i); });
+    hip_check(hipMemcpyAsync(b->h_state, b->d_state, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, b->stream),
+              "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    bool all = true;
+    for (int i = 0; i < nb; ++i) all = all && b->h_state[2 * i + 1] != 0.0;
+    if (all) break;
+    batch = 2;
+  }
+  for (int i = 0; i < nb; ++i) {
+    tpx_ctx *c = b->ctx[i];
+    const int it = (int)b->h_state[2 * i];
+    c->iter_guess[rk & 3] = it;
+    c->num_iteration = it;
+    c->total_iterations += it;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tpx_batch_error(void) { return g_batch_error.c_str(); }
+
+int tpx_batch_create(tpx_ctx *const *ctxs, int n, tpx_batch **out) {
+  if (!out) return TPX_E_ARG;
+  *out = nullptr;
+  if (!ctxs || n < 1) {
+    g_batch_error = "a batch needs at least one context";
+    return TPX_E_ARG;
+  }
+  std::vector<tpx_ctx *> seen(ctxs, ctxs + n);
+  std::sort(seen.begin(), seen.end());
+  for (int i = 0; i < n; ++i)
+    if (!seen[i] || (i > 0 && seen[i] == seen[i - 1])) {
+      g_batch_error = "null or repeated context";
+      return TPX_E_ARG;
+    }
+  const tpx_ctx *a = ctxs[0];
+  for (int i = 1; i < n; ++i) {  // the switches that decide which kernels a step launches
+    const tpx_ctx *c = ctxs[i];
+    const char *why = nullptr;
+    if (c->g.nx != a->g.nx || c->g.ny != a->g.ny || c->g.nz != a->g.nz) why = "grid sizes differ";
+    else if (c->device != a->device) why = "contexts on different devices";
+    else if ((c->p.twc != 0) != (a->p.twc != 0)) why = "TWC differs";
+    else if (heat_active(c) != heat_active(a)) why = "heat exchange on in some boxes only";
+    else if ((c->pt.n > 0) != (a->pt.n > 0)) why = "particles in some boxes only";
+    else if (c->pt.has_tau != a->pt.has_tau || c->pt.has_eps != a->pt.has_eps) why = "per-particle tau/eps differ";
+    else if (c->fft_passes != a->fft_passes) why = "Poisson pass counts differ";
+    if (why) {
+      g_batch_error = why;
+      return TPX_E_ARG;
+    }
+  }
+  tpx_batch *b = new tpx_batch();
+  try {
+    b->ctx.assign(ctxs, ctxs + n);
+    b->rec.resize(n);
+    b->device = a->device;
+    hip_check(hipSetDevice(b->device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto &e : b->ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    if (hipMalloc((void **)&b->d_state, sizeof(double) * 2 * n) != hipSuccess) throw std::bad_alloc();
+    hip_check(hipHostMalloc((void **)&b->h_state, sizeof(double) * 2 * n), "hipHostMalloc");
+  } catch (const std::exception &e) {
+    g_batch_error = e.what();
+    batch_release(b);
+    return TPX_E_HIP;
+  }
+  *out = b;
+  return TPX_OK;
+}
+
+int tpx_batch_step(tpx_batch *b, int nsteps) {
+  if (!b) return TPX_E_ARG;
+  if (nsteps < 0) {
+    b->err = "nsteps < 0";
+    return TPX_E_ARG;
+  }
+  try {
+    hip_check(hipSetDevice(b->device), "hipSetDevice");
+    for (tpx_ctx *c : b->ctx) {  // work queued on the boxes' own streams comes first
+      if (c->tm.on) throw std::logic_error("kernel timing is not recorded in a batch");
+      sync(c);
+    }
+    for (int s = 0; s < nsteps; ++s) {
+      for (tpx_ctx *c : b->ctx) do_params_update(c);
+      for (int rk = 0; rk < 4; ++rk) {
+        batch_run(b, [rk](tpx_ctx *c, int) { fused_substep_pre(c, rk); });
+        batch_solve(b, rk);
+        batch_run(b, [rk](tpx_ctx *c, int) { fused_substep_post(c, rk); });
+      }
+      batch_run(b, [](tpx_ctx *c, int) { step_end(c); });
+    }
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+  } catch (const std::logic_error &e) {
+    b->err = e.what();
+    return TPX_E_STATE;
+  } catch (const std::bad_alloc &) {
+    b->err = "out of memory";
+    return TPX_E_OOM;
+  } catch (const std::exception &e) {
+    b->err = e.what();
+    return TPX_E_HIP;
+  }
+  return TPX_OK;
+}
+
+const char *tpx_batch_last_error(tpx_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int tpx_batch_destroy(tpx_batch *b) {
+  if (!b) return TPX_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_release(b);
+  return TPX_OK;
 }
 
 int tpx_field_sum(tpx_ctx *ctx, int field, int comp, double *out) {

@@ -10,7 +10,12 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
+
 #include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/tpx.h"
@@ -162,6 +167,8 @@ int fft_setup(tpx_ctx *c);
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
 // resets the device-side iteration count and loop test of a solve
 void launch_solve_begin(tpx_ctx *c);
+// copies the solve's iteration count and stopped flag to dst[0..1] (device)
+void launch_solve_state(tpx_ctx *c, double *dst);
 
 // ---- launchers (k_particle.hip) ----
 // count_new: zero start_new and count x_new into it for launch_sort_new
@@ -192,6 +199,73 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
   unsigned per = nb / nxcd;
   if (per == 0 || b >= per * nxcd) return b;
   return (b % nxcd) * per + (b / nxcd);
+}
+
+
+// ---- kernel launches: one box, or recorded for a batch of boxes ----------
+// Every kernel is a struct F with `static constexpr int kBounds` (its launch
+// bound) and `static __device__ void run(...)` (its body).  tpx_launch() starts
+// it on a stream; while the launches of a box are being recorded for a batch
+// (g_rec set), it stores the launch shape and the arguments instead, and the
+// batch later starts one launch per recorded kernel for all of its boxes:
+// box = blockIdx.z, arguments from a table in device memory.
+template <class F, class... A>
+__global__ __launch_bounds__(F::kBounds) void k_single(A... a) {
+  F::run(a...);
+}
+
+template <class F, class T, size_t... I>
+__device__ __forceinline__ void run_args(const T &t, std::index_sequence<I...>) {
+  F::run(std::get<I>(t)...);
+}
+
+template <class F, class... A>
+__global__ __launch_bounds__(F::kBounds) void k_batched(const char *__restrict__ tbl, unsigned stride) {
+  const auto &t = *reinterpret_cast<const std::tuple<A...> *>(tbl + (size_t)blockIdx.z * stride);
+  run_args<F>(t, std::index_sequence_for<A...>{});
+}
+
+struct Recorder {
+  struct Rec {
+    void (*launch)(dim3 grid, dim3 block, size_t shmem, hipStream_t st, const char *tbl, unsigned stride,
+                   unsigned nbox);
+    dim3 grid, block;
+    size_t shmem;
+    unsigned size, align;
+    size_t off;  // of the argument tuple in bytes
+  };
+  std::vector<Rec> recs;
+  std::vector<char> bytes;
+  void clear() { recs.clear(); bytes.clear(); }
+};
+extern thread_local Recorder *g_rec;
+
+template <class F, class... A>
+void launch_batched(dim3 grid, dim3 block, size_t shmem, hipStream_t st, const char *tbl, unsigned stride,
+                    unsigned nbox) {
+  hipLaunchKernelGGL((k_batched<F, A...>), dim3(grid.x, grid.y, nbox), block, shmem, st, tbl, stride);
+}
+
+template <class F, class... A>
+inline void tpx_launch(dim3 grid, dim3 block, size_t shmem, hipStream_t st, A... a) {
+  static_assert((std::is_trivially_copyable<A>::value && ...), "kernel arguments are copied as bytes");
+  if (Recorder *r = g_rec) {
+    using T = std::tuple<A...>;
+    Recorder::Rec rec;
+    rec.launch = &launch_batched<F, A...>;
+    rec.grid = grid;
+    rec.block = block;
+    rec.shmem = shmem;
+    rec.size = sizeof(T);
+    rec.align = alignof(T);
+    rec.off = (r->bytes.size() + alignof(T) - 1) / alignof(T) * alignof(T);
+    r->bytes.resize(rec.off + sizeof(T));
+    const T t(a...);
+    memcpy(r->bytes.data() + rec.off, static_cast<const void *>(&t), sizeof(T));
+    r->recs.push_back(rec);
+    return;
+  }
+  hipLaunchKernelGGL((k_single<F, A...>), grid, block, shmem, st, a...);
 }
 
 }  // namespace tpx

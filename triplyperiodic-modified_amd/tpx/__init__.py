@@ -37,7 +37,9 @@ EXPORTS = [
     "tpx_get_scalars", "tpx_initialize", "tpx_params_update", "tpx_update_rho",
     "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop", "tpx_compute_div_u_new",
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
-    "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step", "tpx_field_sum",
+    "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
+    "tpx_batch_create", "tpx_batch_step", "tpx_batch_destroy", "tpx_batch_error", "tpx_batch_last_error",
+    "tpx_field_sum",
     "tpx_field_max_abs",
     "tpx_deposit_concentration", "tpx_get_stream", "tpx_timing_enable", "tpx_timing_read",
     "tpx_synchronize",
@@ -88,6 +90,12 @@ def load(path: str = LIB_PATH):
         getattr(L, fn).argtypes = [P, I]
     L.tpx_solve_poisson.argtypes = [P, I, ctypes.POINTER(I)]
     L.tpx_ensemble_step.argtypes = [ctypes.POINTER(P), I, I, I]
+    L.tpx_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
+    L.tpx_batch_step.argtypes = [P, I]
+    L.tpx_batch_destroy.argtypes = [P]
+    L.tpx_batch_error.restype = ctypes.c_char_p
+    L.tpx_batch_last_error.argtypes = [P]
+    L.tpx_batch_last_error.restype = ctypes.c_char_p
     L.tpx_make_mean_u0.argtypes = [P, DP]
     L.tpx_field_sum.argtypes = [P, I, I, DP]
     L.tpx_field_max_abs.argtypes = [P, I, I, DP]
@@ -305,6 +313,40 @@ def ensemble_step(solvers, nsteps: int = 1, concurrency: int = 0) -> None:
     if rc != 0:
         msgs = [m for m in ((L.tpx_last_error(s.h) or b"").decode() for s in solvers) if m]
         raise TpxError(f"ensemble_step failed ({rc}): {msgs[0] if msgs else 'invalid arguments'}")
+
+
+class Batch:
+    """Boxes stepped by batched kernels: every kernel of a step is launched
+    once for all boxes (tpx_batch_*).  The solvers keep their own state and
+    stay usable between batch steps; each box advances exactly as its own
+    Solver.step would advance it."""
+
+    def __init__(self, solvers):
+        L = load()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s.h for s in self.solvers])
+        h = ctypes.c_void_p()
+        rc = L.tpx_batch_create(arr, len(self.solvers), ctypes.byref(h))
+        if rc != 0:
+            raise TpxError(f"batch_create failed ({rc}): {(L.tpx_batch_error() or b'').decode()}")
+        self.h = h
+
+    def step(self, nsteps: int = 1) -> None:
+        L = load()
+        rc = L.tpx_batch_step(self.h, int(nsteps))
+        if rc != 0:
+            raise TpxError(f"batch_step failed ({rc}): {(L.tpx_batch_last_error(self.h) or b'').decode()}")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            load().tpx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def tune_concurrency(solvers, candidates=(2, 4, 8, 16, 32), steps: int = 1) -> int:
