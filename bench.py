@@ -18,6 +18,8 @@ reference step) on a bounded sample of the same workload.
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -185,12 +187,52 @@ def ensemble_bench(args):
         "roofline": None, "cpu_baseline": None}), flush=True)
 
 
+def adjoint_bench(args):
+    """Adjoint cost: 128^3, 2e5 particles, St = 2, mass loading 0.5, one-way
+    coupling, 50 steps with a particle checkpoint every 10.  The reverse sweep
+    (recomputing each checkpoint segment's particle steps from the taped gas,
+    then the transposed RK4 steps) is timed against the plain forward steps."""
+    import tpx
+    from tpx import ic
+    nsteps, every = 50, 10
+    case = ic.make_case(128, 200_000, nu=0.01, seed=0, St=2.0, phi=0.5, heat=False, twc=False, nsteps=nsteps)
+    plain = tpx.Solver(case.params, np_capacity=200_000)
+    plain.load_case(case)
+    plain.step(1)
+    plain.synchronize()
+    t0 = time.perf_counter()
+    plain.step(nsteps)
+    plain.synchronize()
+    t_fwd = time.perf_counter() - t0
+    log(f"[bench] forward {nsteps} steps {t_fwd:.3f} s")
+    s = tpx.Solver(case.params, np_capacity=200_000)
+    s.load_case(case)
+    s.step(1)
+    t0 = time.perf_counter()
+    J = s.adjoint_record(nsteps, every)
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = s.adjoint_gradient()
+    t_rev = time.perf_counter() - t0
+    s.adjoint_release()
+    log(f"[bench] recorded forward {t_rec:.3f} s, reverse sweep {t_rev:.3f} s, J = {J:.6e}, |g| = {np.linalg.norm(g):.3e}")
+    print(json.dumps({
+        "metric": "adjoint/forward cost ratio (128^3, 2e5 particles, 50 steps)", "value": round(t_rev / t_fwd, 4),
+        "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": 1, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "adjoint: 128^3, 2e5 particles, St=2, mass loading 0.5, one-way coupling, "
+                               "J = Var(C) of the final positions, gradient w.r.t. every particle's tau, "
+                               "checkpoint every 10 steps"},
+        "forward_s": round(t_fwd, 4), "recorded_forward_s": round(t_rec, 4), "reverse_s": round(t_rev, 4),
+        "recorded_over_plain": round(t_rec / t_fwd, 4), "roofline": None, "cpu_baseline": None}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="heat256", choices=["heat256", "plain128", "plain32", "ensemble"])
+    ap.add_argument("--workload", default="heat256", choices=["heat256", "plain128", "plain32", "ensemble", "adjoint"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--timing-steps", type=int, default=2)
@@ -209,6 +251,8 @@ def main():
     import tpx
     from tpx import ic
 
+    if args.workload == "adjoint":
+        return adjoint_bench(args)
     if args.workload == "ensemble":
         ensemble_bench(args)
         return

@@ -151,6 +151,20 @@ int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency);
  * particle counts, box lengths and all coefficients may differ.  Kernel timing
  * must be off.  tpx_batch_create reports why it failed through
  * tpx_batch_error(); tpx_batch_step through tpx_batch_last_error(). */
+/* ---- adjoint of the particle path (no counterpart in the reference).
+ * Objective J = Var(C) over the cells, C the cloud-in-cell concentration of
+ * the particles' positions after nsteps steps (particle.cpp:522-555 weights).
+ * tpx_adjoint_record advances the context nsteps whole steps exactly as
+ * tpx_step would, taping the gas stage fields each particle advance reads
+ * and checkpointing the particles every `every` steps, and returns J.
+ * tpx_adjoint_gradient then writes dJ/dtau_p for every particle (in the
+ * context's particle order, the order tpx_download_particles uses).  Only for
+ * one-way coupling (TWC off, no heat exchange), where the gas does not depend
+ * on tau_p; tpx_adjoint_release frees the tape (nsteps x 16 grid fields). */
+int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J);
+int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau);
+int tpx_adjoint_release(tpx_ctx *ctx);
+
 typedef struct tpx_batch tpx_batch;
 int tpx_batch_create(tpx_ctx *const *ctxs, int n, tpx_batch **out);
 int tpx_batch_step(tpx_batch *b, int nsteps);

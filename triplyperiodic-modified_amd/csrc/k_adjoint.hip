@@ -1,0 +1,364 @@
+// k_adjoint.hip -- hand-written adjoint of the particle path (gfx950).
+//
+// Objective J = Var(C) over the cells, C the cloud-in-cell concentration of
+// the particles' final positions (particle.cpp:522-555 weights).  For runs
+// with one-way coupling (no drag deposit, no heat exchange) the gas does not
+// depend on the particles, so dJ/dtau_p is the particle path's adjoint alone:
+// the forward run tapes the gas stage fields every particle advance reads
+// (RU_int, Rho_int of each substep) and checkpoints the particles' base state;
+// the reverse sweep restores a checkpoint, recomputes the steps up to the one
+// being reversed with the same arithmetic as the advance kernel (P1-P3, P7 of
+// particle.cpp:314-399, 58-104), keeps its four stage states, and runs the
+// transposed RK4 stage recursion
+//   lam_k(s) = dt post_s lam_end + dt pre_s lam_Y(s+1),  lam_Y(s) = J_f(Y_s)^T lam_k(s),
+//   lam_y0   = lam_end + sum_s lam_Y(s)
+// with f(x, u) = (u, (u_g(x) - u)/tau + g): J_f^T (a, b) = (b du_g/dx / tau, a - b/tau)
+// (u_g of each axis is linear along its own axis between two faces and piecewise
+// constant across), and dJ/dtau_p = -sum_s b(s).(u_g(Y_s) - u_Y(s)) / tau^2.
+// The periodic wraps shift positions by constants and drop out of the adjoint.
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <stdexcept>
+
+#include "tpx_internal.h"
+
+namespace tpx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Ix {
+  int mx, my, mz, nx;
+  long nxy;
+  __device__ __forceinline__ long operator()(int i, int j, int k) const {
+    return (long)(k & mz) * nxy + (long)(j & my) * nx + (i & mx);
+  }
+};
+
+__device__ __forceinline__ Ix make_ix(const Geo &g) {
+  Ix r;
+  r.mx = g.nx - 1; r.my = g.ny - 1; r.mz = g.nz - 1; r.nx = g.nx; r.nxy = g.nxy;
+  return r;
+}
+
+// One substep's taped gas: RU_int (3 components) then Rho_int, n values each.
+struct Gas {
+  const double *f;
+  long n;
+};
+
+// u_g at x (gas2part_velocity, particle.cpp:340-360, as in the advance
+// kernel) and its derivative along each component's own axis
+__device__ __forceinline__ void gather(const Geo &g, const Ix &ix, Gas gs, const double x[3], double ug[3],
+                                       double dug[3]) {
+  const double *r0 = gs.f, *r1 = gs.f + gs.n, *r2 = gs.f + 2 * gs.n, *rho = gs.f + 3 * gs.n;
+  int ii = (int)(x[0] * g.idx), jj = (int)(x[1] * g.idy), kk = (int)(x[2] * g.idz);
+  double xl = (x[0] * g.idx - ((int)(x[0] * g.idx)));
+  double yl = (x[1] * g.idy - ((int)(x[1] * g.idy)));
+  double zl = (x[2] * g.idz - ((int)(x[2] * g.idz)));
+  long c000 = ix(ii, jj, kk);
+  double ux1 = r0[ix(ii + 1, jj, kk)] / ((rho[ix(ii + 1, jj, kk)] + rho[c000]) * 0.5);
+  double ux0 = r0[c000] / ((rho[c000] + rho[ix(ii - 1, jj, kk)]) * 0.5);
+  double uy1 = r1[ix(ii, jj + 1, kk)] / ((rho[ix(ii, jj + 1, kk)] + rho[c000]) * 0.5);
+  double uy0 = r1[c000] / ((rho[c000] + rho[ix(ii, jj - 1, kk)]) * 0.5);
+  double uz1 = r2[ix(ii, jj, kk + 1)] / ((rho[ix(ii, jj, kk + 1)] + rho[c000]) * 0.5);
+  double uz0 = r2[c000] / ((rho[c000] + rho[ix(ii, jj, kk - 1)]) * 0.5);
+  ug[0] = xl * ux1 + (1 - xl) * ux0;
+  ug[1] = yl * uy1 + (1 - yl) * uy0;
+  ug[2] = zl * uz1 + (1 - zl) * uz0;
+  dug[0] = (ux1 - ux0) * g.idx;
+  dug[1] = (uy1 - uy0) * g.idy;
+  dug[2] = (uz1 - uz0) * g.idz;
+}
+
+struct StepArgs {
+  Geo g;
+  const double *gas;   // 4 substeps x 4 fields x n
+  double *y[6];        // base x, y, z, u, v, w: read, then overwritten by the step's result
+  double *stage;       // non-null: the 4 stage states, 24 arrays of np
+  const double *tau;   // per particle, or null
+  double tp;           // scalar tau when tau is null
+  double c_pre[4], c_post[4];
+  double g3[3], L[3];
+  long np, ncell;
+};
+
+// One whole RK4 step of the particles' positions and velocities with the
+// advance kernel's arithmetic (update_position, update_velocity, the wrap),
+// one particle per thread; optionally stores the int state of each substep.
+struct k_adj_step_fwd {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(StepArgs a) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.np) return;
+    const Geo &g = a.g;
+    Ix ix = make_ix(g);
+    double b[6], vi[6], v1[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) b[q] = vi[q] = v1[q] = a.y[q][p];
+    const double itp = a.tau ? 1. / a.tau[p] : 1. / a.tp;
+    for (int rk = 0; rk < 4; ++rk) {
+      if (a.stage)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a.stage[(rk * 6 + q) * a.np + p] = vi[q];
+      double ug[3], dug[3];
+      gather(g, ix, Gas{a.gas + (long)rk * 4 * a.ncell, a.ncell}, vi, ug, dug);
+      double nv[6], n1[6];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double ui = vi[3 + d];
+        n1[d] = v1[d] + a.c_post[rk] * ui;
+        nv[d] = rk == 3 ? n1[d] : b[d] + a.c_pre[rk] * ui;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double ui = vi[3 + d];
+        double rhs = (ug[d] - ui) * itp;
+        n1[3 + d] = v1[3 + d] + a.c_post[rk] * (rhs + a.g3[d]);
+        nv[3 + d] = rk == 3 ? n1[3 + d] : b[3 + d] + a.c_pre[rk] * (rhs + a.g3[d]);
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (nv[d] > a.L[d]) {
+          nv[d] -= a.L[d]; b[d] -= a.L[d]; n1[d] -= a.L[d];
+        } else if (nv[d] < 0) {
+          nv[d] += a.L[d]; b[d] += a.L[d]; n1[d] += a.L[d];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        v1[q] = n1[q];
+        vi[q] = nv[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.y[q][p] = v1[q];
+  }
+};
+
+struct AdjArgs {
+  Geo g;
+  const double *gas;    // the step's 4 substeps
+  const double *stage;  // 24 arrays of np: the int state of each substep
+  double *lam[6];       // adjoint of the step's result in, of its start out
+  double *dtau;         // dJ/dtau accumulated
+  const double *tau;
+  double tp;
+  double c_pre[4], c_post[4];
+  long np, ncell;
+};
+
+// The transposed RK4 step for one particle per thread.
+struct k_adj_step_rev {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(AdjArgs a) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.np) return;
+    const Geo &g = a.g;
+    Ix ix = make_ix(g);
+    const double tau = a.tau ? a.tau[p] : a.tp;
+    const double itp = 1. / tau;
+    double lend[6], lb[6], lnext[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      lend[q] = a.lam[q][p];
+      lb[q] = lend[q];  // the np1 accumulator starts as the base state
+      lnext[q] = 0.0;
+    }
+    double dtau = 0.0;
+    for (int rk = 3; rk >= 0; --rk) {
+      double Y[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Y[q] = a.stage[(rk * 6 + q) * a.np + p];
+      double lk[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) lk[q] = a.c_post[rk] * lend[q] + (rk < 3 ? a.c_pre[rk] * lnext[q] : 0.0);
+      if (rk < 3)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) lb[q] += lnext[q];  // new = base + dt pre k
+      double ug[3], dug[3];
+      gather(g, ix, Gas{a.gas + (long)rk * 4 * a.ncell, a.ncell}, Y, ug, dug);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        lnext[d] = lk[3 + d] * dug[d] * itp;       // through u_g(x)
+        lnext[3 + d] = lk[d] - lk[3 + d] * itp;     // k_x = u, and -u/tau
+        dtau -= lk[3 + d] * (ug[d] - Y[3 + d]) * itp * itp;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.lam[q][p] = lb[q] + lnext[q];  // the first int state is the base
+    a.dtau[p] += dtau;
+  }
+};
+
+// dJ/dC = 2 (C - mean C) / ncell, and the block partials of C^2
+struct k_var_terms {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const double *C, long n, double *partials) {
+    __shared__ double sh[kBlock];
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double v = c < n ? C[c] : 0.0;
+    sh[threadIdx.x] = v * v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+  }
+};
+
+struct k_dj_dc {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const double *C, long n, const double *scal, double *out) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double mean = scal[SC_SUM_C] / (double)n;
+    out[c] = 2.0 * (C[c] - mean) / (double)n;
+  }
+};
+
+// lam_x of each particle: the concentration weights' position derivatives
+// (particle.cpp:522-555 weights) against dJ/dC; lam_u = 0
+struct k_conc_adj {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(Geo g, const double *x, const double *y, const double *z,
+                                             const double *djdc, double *l0, double *l1, double *l2, double *l3,
+                                             double *l4, double *l5, long np) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    Ix ix = make_ix(g);
+    const double xp = x[p], yp = y[p], zp = z[p];
+    int ii = (int)(round(xp * g.idx)), jj = (int)(round(yp * g.idy)), kk = (int)(round(zp * g.idz));
+    double xl = xp * g.idx - ii + 0.5, yl = yp * g.idy - jj + 0.5, zl = zp * g.idz - kk + 0.5;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int oz = 0; oz < 2; ++oz)
+      for (int oy = 0; oy < 2; ++oy)
+        for (int ox = 0; ox < 2; ++ox) {
+          // corner (ii - 1 + ox, jj - 1 + oy, kk - 1 + oz) has weight wx wy wz
+          const double wx = ox ? xl : 1 - xl, wy = oy ? yl : 1 - yl, wz = oz ? zl : 1 - zl;
+          const double sx = ox ? g.idx : -g.idx, sy = oy ? g.idy : -g.idy, sz = oz ? g.idz : -g.idz;
+          const double d = djdc[ix(ii - 1 + ox, jj - 1 + oy, kk - 1 + oz)];
+          gx += d * sx * wy * wz;
+          gy += d * wx * sy * wz;
+          gz += d * wx * wy * sz;
+        }
+    l0[p] = gx; l1[p] = gy; l2[p] = gz;
+    l3[p] = 0.0; l4[p] = 0.0; l5[p] = 0.0;
+  }
+};
+
+unsigned grid1(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+void hip_check_adj(hipError_t e) {
+  if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+}
+
+StepArgs step_args(tpx_ctx *c, int step, double *const y[6], double *stage) {
+  Particles &pt = c->pt;
+  AdjointTape &t = c->adj;
+  StepArgs a{};
+  a.g = c->g;
+  a.gas = t.gas + (long)step * 16 * c->g.n;
+  for (int q = 0; q < 6; ++q) a.y[q] = y[q];
+  a.stage = stage;
+  a.tau = pt.has_tau ? pt.tau : nullptr;
+  a.tp = c->Tp;
+  for (int r = 0; r < 4; ++r) {
+    a.c_pre[r] = c->p.dt * c->pre[r];
+    a.c_post[r] = c->p.dt * c->post[r];
+  }
+  for (int d = 0; d < 3; ++d) a.g3[d] = t.g3[(size_t)step * 3 + d];
+  a.L[0] = c->p.lx; a.L[1] = c->p.ly; a.L[2] = c->p.lz;
+  a.np = pt.n;
+  a.ncell = c->g.n;
+  return a;
+}
+
+}  // namespace
+
+// copies this substep's RU_int and Rho_int into the tape
+void adjoint_tape_substep(tpx_ctx *c, int rk, const double *const ru_int[3], const double *rho_int) {
+  AdjointTape &t = c->adj;
+  if (t.step < 0) return;
+  if (t.step >= t.nsteps) throw std::logic_error("adjoint tape overrun");
+  const long n = c->g.n;
+  double *dst = t.gas + ((long)t.step * 4 + rk) * 4 * n;
+  for (int d = 0; d < 3; ++d) launch_copy(c, dst + d * n, ru_int[d], n);
+  launch_copy(c, dst + 3 * n, rho_int, n);
+  if (rk == 0)
+    for (int d = 0; d < 3; ++d) t.g3[(size_t)t.step * 3 + d] = (double)c->p.particle_gravity * c->gv[d];
+}
+
+void adjoint_checkpoint(tpx_ctx *c, int slot) {
+  Particles &pt = c->pt;
+  for (int q = 0; q < 6; ++q) launch_copy(c, c->adj.ckpt + ((long)slot * 6 + q) * pt.n, pt.q[q][pt.st_base], pt.n);
+}
+
+double adjoint_objective(tpx_ctx *c) {
+  const long n = c->g.n;
+  // bins of the final base positions (also the int state at a step boundary)
+  launch_sort_stage(c, TPX_BASE, true);
+  launch_concentration(c);
+  const unsigned nb = grid1(n);
+  tpx_launch<k_var_terms>(nb, kBlock, 0, c->stream, (const double *)c->C, n, c->partials);
+  TPX_LAUNCH_CHECK();
+  launch_finish_sum(c, (int)nb, 0, SC_RED);
+  tpx_launch<k_dj_dc>(nb, kBlock, 0, c->stream, (const double *)c->C, n, (const double *)c->scal, c->adj.djdc);
+  TPX_LAUNCH_CHECK();
+  hip_check_adj(hipStreamSynchronize(c->stream));
+  double s[2];
+  hip_check_adj(hipMemcpy(&s[0], c->scal + SC_SUM_C, sizeof(double), hipMemcpyDeviceToHost));
+  hip_check_adj(hipMemcpy(&s[1], c->scal + SC_RED, sizeof(double), hipMemcpyDeviceToHost));
+  const double mean = s[0] / (double)n;
+  return s[1] / (double)n - mean * mean;
+}
+
+void adjoint_reverse(tpx_ctx *c) {
+  Particles &pt = c->pt;
+  AdjointTape &t = c->adj;
+  const long np = pt.n;
+  double *lam[6], *y[6];
+  for (int q = 0; q < 6; ++q) {
+    lam[q] = t.lam + q * np;
+    y[q] = t.y + q * np;
+  }
+  // objective: lam_x from the concentration weights at the final positions
+  tpx_launch<k_conc_adj>(grid1(np), kBlock, 0, c->stream, c->g, (const double *)pt.q[0][pt.st_base],
+                         (const double *)pt.q[1][pt.st_base], (const double *)pt.q[2][pt.st_base],
+                         (const double *)t.djdc, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5], np);
+  TPX_LAUNCH_CHECK();
+  launch_fill(c, t.dtau, 0.0, np);
+  for (int s = t.nsteps - 1; s >= 0; --s) {
+    // the base state of step s: its checkpoint, then the steps after it
+    const int k0 = s / t.every;
+    for (int q = 0; q < 6; ++q) launch_copy(c, y[q], t.ckpt + ((long)k0 * 6 + q) * np, np);
+    for (int r = k0 * t.every; r < s; ++r) {
+      tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, r, y, nullptr));
+      TPX_LAUNCH_CHECK();
+    }
+    tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, s, y, t.stage));
+    TPX_LAUNCH_CHECK();
+    AdjArgs a{};
+    a.g = c->g;
+    a.gas = t.gas + (long)s * 16 * c->g.n;
+    a.stage = t.stage;
+    for (int q = 0; q < 6; ++q) a.lam[q] = lam[q];
+    a.dtau = t.dtau;
+    a.tau = pt.has_tau ? pt.tau : nullptr;
+    a.tp = c->Tp;
+    for (int r = 0; r < 4; ++r) {
+      a.c_pre[r] = c->p.dt * c->pre[r];
+      a.c_post[r] = c->p.dt * c->post[r];
+    }
+    a.np = np;
+    a.ncell = c->g.n;
+    tpx_launch<k_adj_step_rev>(grid1(np), kBlock, 0, c->stream, a);
+    TPX_LAUNCH_CHECK();
+  }
+}
+
+}  // namespace tpx

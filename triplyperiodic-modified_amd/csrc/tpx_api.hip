@@ -323,6 +323,7 @@ void fused_substep_pre(tpx_ctx *c, int rk) {
   if (parts) {
     const double *ri[3];
     ru_ptrs(c, c->ru_int, ri);
+    if (c->adj.step >= 0) adjoint_tape_substep(c, rk, ri, c->rho[c->rho_int]);
     launch_particle_advance(c, k, ri, c->rho[c->rho_int], rebin);
     if (c->p.twc) launch_velocity_deposit(c);
   }
@@ -374,6 +375,7 @@ void release(tpx_ctx *c) {
   for (auto &p : c->lam) f(p);
   for (auto &p : c->tw) f(p);
   f(c->scal); f(c->partials); f(c->scan_tmp);
+  f(c->adj.gas); f(c->adj.ckpt); f(c->adj.y); f(c->adj.stage); f(c->adj.lam); f(c->adj.dtau); f(c->adj.djdc);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   Particles &pt = c->pt;
   for (auto &q : pt.q) for (auto &p : q) f(p);
@@ -1095,6 +1097,78 @@ int tpx_batch_destroy(tpx_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   batch_release(b);
   return TPX_OK;
+}
+
+// ---- adjoint of the particle path ----------------------------------------
+int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
+  return guarded(ctx, [&]() {
+    tpx_ctx *c = ctx;
+    Particles &pt = c->pt;
+    if (nsteps < 1 || every < 1) throw std::invalid_argument("nsteps and every must be positive");
+    if (pt.n == 0) throw std::invalid_argument("the adjoint needs particles");
+    if (c->p.twc || heat_active(c))
+      throw std::logic_error("the particle adjoint needs one-way coupling: TWC off and no heat exchange");
+    AdjointTape &t = c->adj;
+    auto f = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
+    f(t.gas); f(t.ckpt); f(t.y); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
+    t.ready = false;
+    t.nsteps = nsteps;
+    t.every = every;
+    const long n = c->g.n, np = pt.n;
+    auto alloc = [](double **p, size_t count) {
+      if (hipMalloc((void **)p, sizeof(double) * count) != hipSuccess) throw std::bad_alloc();
+    };
+    alloc(&t.gas, (size_t)nsteps * 16 * n);
+    alloc(&t.ckpt, (size_t)(nsteps / every + 1) * 6 * np);
+    alloc(&t.y, 6 * (size_t)np);
+    alloc(&t.stage, 24 * (size_t)np);
+    alloc(&t.lam, 6 * (size_t)np);
+    alloc(&t.dtau, (size_t)np);
+    alloc(&t.djdc, (size_t)n);
+    t.g3.assign((size_t)nsteps * 3, 0.0);
+    adjoint_checkpoint(c, 0);
+    try {
+      for (int s = 0; s < nsteps; ++s) {
+        t.step = s;
+        do_params_update(c);
+        for (int rk = 0; rk < 4; ++rk) fused_substep(c, rk);
+        step_end(c);
+        t.step = -1;
+        if ((s + 1) % every == 0 && s + 1 < nsteps) adjoint_checkpoint(c, (s + 1) / every);
+      }
+    } catch (...) {
+      t.step = -1;
+      throw;
+    }
+    const double j = adjoint_objective(c);
+    if (J) *J = j;
+    t.ready = true;
+    return TPX_OK;
+  });
+}
+
+int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau) {
+  return guarded(ctx, [&]() {
+    AdjointTape &t = ctx->adj;
+    if (!t.ready) throw std::logic_error("no recorded run: call tpx_adjoint_record first");
+    if (!dJ_dtau) throw std::invalid_argument("null output");
+    adjoint_reverse(ctx);
+    hip_check(hipMemcpyAsync(dJ_dtau, t.dtau, sizeof(double) * ctx->pt.n, hipMemcpyDeviceToHost, ctx->stream),
+              "download");
+    sync(ctx);
+    return TPX_OK;
+  });
+}
+
+int tpx_adjoint_release(tpx_ctx *ctx) {
+  return guarded(ctx, [&]() {
+    AdjointTape &t = ctx->adj;
+    sync(ctx);
+    auto f = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
+    f(t.gas); f(t.ckpt); f(t.y); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
+    t.ready = false;
+    return TPX_OK;
+  });
 }
 
 int tpx_field_sum(tpx_ctx *ctx, int field, int comp, double *out) {

@@ -80,6 +80,23 @@ struct Particles {
   int32_t *itmp = nullptr;
 };
 
+// The adjoint of the particle path (one-way coupling): the gas stage fields
+// of every substep of a recorded run and the particle base state every
+// `every` steps, plus the reverse sweep's work arrays.
+struct AdjointTape {
+  int nsteps = 0, every = 1;
+  int step = -1;                 // step being recorded (-1: not recording)
+  bool ready = false;            // a whole run is recorded and its objective computed
+  double *gas = nullptr;         // nsteps x 4 substeps x (RU_int 3, Rho_int) x n
+  std::vector<double> g3;        // particle gravity of each step (3 per step)
+  double *ckpt = nullptr;        // (nsteps / every + 1) x 6 x np
+  double *y = nullptr;           // 6 x np
+  double *stage = nullptr;       // 4 x 6 x np
+  double *lam = nullptr;         // 6 x np
+  double *dtau = nullptr;        // np
+  double *djdc = nullptr;        // n
+};
+
 struct Timing {
   bool on = false;
   struct Rec { int cls; hipEvent_t a, b; };
@@ -107,6 +124,7 @@ struct tpx_ctx {
   int device = 0;
   size_t max_lds = 65536;          // dynamic LDS bytes one workgroup may use on this device
   int fft_passes = 3;              // Poisson iteration: 3 passes where every axis allows, else 5
+  tpx::AdjointTape adj;
   // grid fields
   double *ru[5][3] = {};          // buffer sets for RU
   int ru_base = 0, ru_int = 1, ru_new = 2, ru_np1 = 3;
@@ -184,6 +202,12 @@ void launch_particle_permute(tpx_ctx *c);
 void launch_sort_stage(tpx_ctx *c, int stage, bool into_int);
 void launch_iota(tpx_ctx *c, int32_t *p, int64_t n);
 void launch_exclusive_scan(tpx_ctx *c, int32_t *data, int64_t n);  // in place, n+1 output
+
+// k_adjoint.hip
+void adjoint_tape_substep(tpx_ctx *c, int rk, const double *const ru_int[3], const double *rho_int);
+void adjoint_checkpoint(tpx_ctx *c, int slot);
+double adjoint_objective(tpx_ctx *c);  // J = Var(C) of the base positions; leaves dJ/dC in adj.djdc
+void adjoint_reverse(tpx_ctx *c);      // dJ/dtau into adj.dtau
 
 // ---- timing helpers (tpx_api.hip) ----
 void timing_begin(tpx_ctx *c, int cls);

@@ -38,6 +38,7 @@ EXPORTS = [
     "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop", "tpx_compute_div_u_new",
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
     "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
+    "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_release",
     "tpx_batch_create", "tpx_batch_step", "tpx_batch_destroy", "tpx_batch_error", "tpx_batch_last_error",
     "tpx_field_sum",
     "tpx_field_max_abs",
@@ -90,6 +91,9 @@ def load(path: str = LIB_PATH):
         getattr(L, fn).argtypes = [P, I]
     L.tpx_solve_poisson.argtypes = [P, I, ctypes.POINTER(I)]
     L.tpx_ensemble_step.argtypes = [ctypes.POINTER(P), I, I, I]
+    L.tpx_adjoint_record.argtypes = [P, I, I, DP]
+    L.tpx_adjoint_gradient.argtypes = [P, DP]
+    L.tpx_adjoint_release.argtypes = [P]
     L.tpx_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.tpx_batch_step.argtypes = [P, I]
     L.tpx_batch_destroy.argtypes = [P]
@@ -236,6 +240,25 @@ class Solver:
 
     def step(self, n: int = 1) -> None:
         self._ok(self.L.tpx_step(self.h, int(n)), "step")
+
+    # ---- adjoint of the particle path (one-way coupling) ----
+    def adjoint_record(self, nsteps: int, every: int = 10) -> float:
+        """Advance nsteps whole steps (as step()) while taping what the
+        particle adjoint needs; returns J = Var(C) of the final positions."""
+        out = ctypes.c_double()
+        self._ok(self.L.tpx_adjoint_record(self.h, int(nsteps), int(every), ctypes.byref(out)), "adjoint_record")
+        return out.value
+
+    def adjoint_gradient(self) -> np.ndarray:
+        """dJ/dtau_p of the recorded run, in particle order."""
+        n = ctypes.c_int64()
+        self._ok(self.L.tpx_num_particles(self.h, ctypes.byref(n)), "num_particles")
+        g = np.zeros(n.value)
+        self._ok(self.L.tpx_adjoint_gradient(self.h, _dp(g)), "adjoint_gradient")
+        return g
+
+    def adjoint_release(self) -> None:
+        self._ok(self.L.tpx_adjoint_release(self.h), "adjoint_release")
 
     def deposit_concentration(self) -> None:
         self._ok(self.L.tpx_deposit_concentration(self.h), "deposit_concentration")
