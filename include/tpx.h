@@ -157,12 +157,14 @@ int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency);
  * tpx_adjoint_record advances the context nsteps whole steps exactly as
  * tpx_step would, taping the gas stage fields each particle advance reads
  * and checkpointing the particles every `every` steps, and returns J.
- * tpx_adjoint_gradient then writes dJ/dtau_p for every particle (in the
- * context's particle order, the order tpx_download_particles uses).  Only for
+ * tpx_adjoint_gradient then writes dJ/dtau_p for every particle and, if
+ * dJ_dstate is not null, dJ/d of the particles' state when recording began
+ * (6 arrays of np: x, y, z, u, v, w), both in the context's particle order
+ * (the order tpx_download_particles uses; match by flag).  Only for
  * one-way coupling (TWC off, no heat exchange), where the gas does not depend
  * on tau_p; tpx_adjoint_release frees the tape (nsteps x 16 grid fields). */
 int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J);
-int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau);
+int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate);
 int tpx_adjoint_release(tpx_ctx *ctx);
 
 typedef struct tpx_batch tpx_batch;

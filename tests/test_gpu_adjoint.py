@@ -77,6 +77,33 @@ def test_adjoint_matches_central_differences(gpu_available, nsteps, every):
     assert worst <= FD_REL
 
 
+def test_initial_velocity_gradient_matches_central_differences(gpu_available):
+    """dJ/du_p(0): the adjoint of the recorded run's initial particle
+    velocities against central differences of the full solver."""
+    case, _ = _case()
+    s = solver_from_case(case)
+    s.adjoint_record(4, 2)
+    _, gs = s.adjoint_gradient(state=True)
+    gu = _by_flag(s, gs[3])
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    dirs = [rng.standard_normal(len(gu)) for _ in range(4)]
+    ads = [float(gu @ d) for d in dirs]
+    scale = float(np.median(np.abs(ads)))
+    for k, (d, ad) in enumerate(zip(dirs, ads)):
+        Js = []
+        for sign in (1, -1):
+            c, _ = _case()
+            c.particles["u"] = c.particles["u"] + sign * h * d
+            t = solver_from_case(c)
+            Js.append(t.adjoint_record(4, 2))
+            t.adjoint_release()
+        fd = (Js[0] - Js[1]) / (2 * h)
+        rel = abs(fd - ad) / max(abs(fd), abs(ad), scale)
+        print(f"u direction {k}: FD {fd:.10e} adjoint {ad:.10e} rel {rel:.2e}")
+        assert rel <= FD_REL
+
+
 def test_scalar_tau_gives_the_same_gradient(gpu_available):
     """With one response time for all particles (no per-particle array) the
     per-particle gradient is the same as with an equal per-particle array."""

@@ -1147,14 +1147,18 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
   });
 }
 
-int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau) {
+int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate) {
   return guarded(ctx, [&]() {
     AdjointTape &t = ctx->adj;
     if (!t.ready) throw std::logic_error("no recorded run: call tpx_adjoint_record first");
-    if (!dJ_dtau) throw std::invalid_argument("null output");
+    if (!dJ_dtau && !dJ_dstate) throw std::invalid_argument("null outputs");
     adjoint_reverse(ctx);
-    hip_check(hipMemcpyAsync(dJ_dtau, t.dtau, sizeof(double) * ctx->pt.n, hipMemcpyDeviceToHost, ctx->stream),
-              "download");
+    const size_t np = (size_t)ctx->pt.n;
+    if (dJ_dtau)
+      hip_check(hipMemcpyAsync(dJ_dtau, t.dtau, sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream), "download");
+    if (dJ_dstate)  // adjoint of the recorded run's initial x, y, z, u, v, w
+      hip_check(hipMemcpyAsync(dJ_dstate, t.lam, sizeof(double) * 6 * np, hipMemcpyDeviceToHost, ctx->stream),
+                "download");
     sync(ctx);
     return TPX_OK;
   });

@@ -92,7 +92,7 @@ def load(path: str = LIB_PATH):
     L.tpx_solve_poisson.argtypes = [P, I, ctypes.POINTER(I)]
     L.tpx_ensemble_step.argtypes = [ctypes.POINTER(P), I, I, I]
     L.tpx_adjoint_record.argtypes = [P, I, I, DP]
-    L.tpx_adjoint_gradient.argtypes = [P, DP]
+    L.tpx_adjoint_gradient.argtypes = [P, DP, DP]
     L.tpx_adjoint_release.argtypes = [P]
     L.tpx_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.tpx_batch_step.argtypes = [P, I]
@@ -249,13 +249,16 @@ class Solver:
         self._ok(self.L.tpx_adjoint_record(self.h, int(nsteps), int(every), ctypes.byref(out)), "adjoint_record")
         return out.value
 
-    def adjoint_gradient(self) -> np.ndarray:
-        """dJ/dtau_p of the recorded run, in particle order."""
+    def adjoint_gradient(self, state: bool = False):
+        """dJ/dtau_p of the recorded run, in particle order; with state=True
+        also dJ/d(x, y, z, u, v, w) of the particles when recording began,
+        shape (6, np)."""
         n = ctypes.c_int64()
         self._ok(self.L.tpx_num_particles(self.h, ctypes.byref(n)), "num_particles")
         g = np.zeros(n.value)
-        self._ok(self.L.tpx_adjoint_gradient(self.h, _dp(g)), "adjoint_gradient")
-        return g
+        gs = np.zeros((6, n.value)) if state else None
+        self._ok(self.L.tpx_adjoint_gradient(self.h, _dp(g), _dp(gs) if state else None), "adjoint_gradient")
+        return (g, gs) if state else g
 
     def adjoint_release(self) -> None:
         self._ok(self.L.tpx_adjoint_release(self.h), "adjoint_release")
