@@ -91,8 +91,8 @@ struct StepArgs {
 // one particle per thread; optionally stores the int state of each substep.
 struct k_adj_step_fwd {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(StepArgs a) {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  static __device__ __forceinline__ void run(const StepArgs &a) {
+    const long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (p >= a.np) return;
     const Geo &g = a.g;
     Ix ix = make_ix(g);
@@ -154,8 +154,8 @@ struct AdjArgs {
 // The transposed RK4 step for one particle per thread.
 struct k_adj_step_rev {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(AdjArgs a) {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  static __device__ __forceinline__ void run(const AdjArgs &a) {
+    const long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (p >= a.np) return;
     const Geo &g = a.g;
     Ix ix = make_ix(g);
@@ -199,11 +199,11 @@ struct k_var_terms {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const double *C, long n, double *partials) {
     __shared__ double sh[kBlock];
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     const double v = c < n ? C[c] : 0.0;
     sh[threadIdx.x] = v * v;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    for (int s = block_dim_x() / 2; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
       __syncthreads();
     }
@@ -214,7 +214,7 @@ struct k_var_terms {
 struct k_dj_dc {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const double *C, long n, const double *scal, double *out) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (c >= n) return;
     const double mean = scal[SC_SUM_C] / (double)n;
     out[c] = 2.0 * (C[c] - mean) / (double)n;
@@ -225,10 +225,10 @@ struct k_dj_dc {
 // (particle.cpp:522-555 weights) against dJ/dC; lam_u = 0
 struct k_conc_adj {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(Geo g, const double *x, const double *y, const double *z,
+  static __device__ __forceinline__ void run(const Geo &g, const double *x, const double *y, const double *z,
                                              const double *djdc, double *l0, double *l1, double *l2, double *l3,
                                              double *l4, double *l5, long np) {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (p >= np) return;
     Ix ix = make_ix(g);
     const double xp = x[p], yp = y[p], zp = z[p];

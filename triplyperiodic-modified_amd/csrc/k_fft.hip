@@ -72,7 +72,7 @@ __device__ void lds_fft(double2 *buf, int n, int lines, const double2 *__restric
     int dst[kMaxBf];
 #pragma unroll
     for (int q = 0; q < kMaxBf; ++q) {
-      int b = threadIdx.x + q * blockDim.x;
+      int b = threadIdx.x + q * block_dim_x();
       if (b < total) {
         int l = ILV ? b % lines : b / per, j = ILV ? b / lines : b - (b / per) * per;
         double2 a0 = at(l, j), a1 = at(l, j + per);
@@ -84,7 +84,7 @@ __device__ void lds_fft(double2 *buf, int n, int lines, const double2 *__restric
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kMaxBf; ++q) {
-      int b = threadIdx.x + q * blockDim.x;
+      int b = threadIdx.x + q * block_dim_x();
       if (b < total) {
         int l = dst[q] & 0xffff, d = dst[q] >> 16;
         at(l, d) = y0[q];
@@ -101,7 +101,7 @@ __device__ void lds_fft(double2 *buf, int n, int lines, const double2 *__restric
     int tstep = n / (4 * Ns);
 #pragma unroll
     for (int q = 0; q < kMaxBf; ++q) {
-      int b = threadIdx.x + q * blockDim.x;
+      int b = threadIdx.x + q * block_dim_x();
       if (b < total) {
         int l = ILV ? b % lines : b / per, j = ILV ? b / lines : b - (b / per) * per;
         int k = j & (Ns - 1);
@@ -127,7 +127,7 @@ __device__ void lds_fft(double2 *buf, int n, int lines, const double2 *__restric
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kMaxBf; ++q) {
-      int b = threadIdx.x + q * blockDim.x;
+      int b = threadIdx.x + q * block_dim_x();
       if (b < total) {
         int l = dst[q] & 0xffff, d = dst[q] >> 16;
         at(l, d) = y[q][0];
@@ -194,7 +194,7 @@ __device__ __forceinline__ double residual(const XArgs &a, const Ix &ix, int i, 
 // X forward pass: lpb consecutive x lines per block
 struct k_xfwd {
   static constexpr int kBounds = kThreads;
-  static __device__ __forceinline__ void run(XArgs a, int lpb) {
+  static __device__ __forceinline__ void run(const XArgs &a, int lpb) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   const Geo &g = a.g;
@@ -204,7 +204,7 @@ struct k_xfwd {
   double mean = a.scal[SC_SUM_RHS] / a.ncell;
   unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   long line0 = (long)blk * lpb;
-  for (int e = threadIdx.x; e < lpb * nx; e += blockDim.x) {
+  for (int e = threadIdx.x; e < lpb * nx; e += block_dim_x()) {
     int l = e / nx, i = e - l * nx;
     long line = line0 + l;
     double r = 0.0;
@@ -213,7 +213,7 @@ struct k_xfwd {
   }
   __syncthreads();
   lds_fft<false, false>(lds, nx, lpb, a.tw);
-  for (int e = threadIdx.x; e < lpb * mxh; e += blockDim.x) {
+  for (int e = threadIdx.x; e < lpb * mxh; e += block_dim_x()) {
     int l = e / mxh, m = e - l * mxh;
     long line = line0 + l;
     if (line < nlines) a.spec[line * mxh + m] = lds[l * nx + m];
@@ -234,7 +234,7 @@ struct XInvArgs {
 // X inverse pass: Hermitian extension, inverse FFT, P += dp, block max|dp|
 struct k_xinv {
   static constexpr int kBounds = kThreads;
-  static __device__ __forceinline__ void run(XInvArgs a, int lpb) {
+  static __device__ __forceinline__ void run(const XInvArgs &a, int lpb) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   __shared__ double red[kThreads];
@@ -243,7 +243,7 @@ struct k_xinv {
   long nlines = (long)g.ny * g.nz;
   unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   long line0 = (long)blk * lpb;
-  for (int e = threadIdx.x; e < lpb * mxh; e += blockDim.x) {
+  for (int e = threadIdx.x; e < lpb * mxh; e += block_dim_x()) {
     int l = e / mxh, m = e - l * mxh;
     long line = line0 + l;
     double2 v = line < nlines ? a.spec[line * mxh + m] : make_double2(0.0, 0.0);
@@ -254,7 +254,7 @@ struct k_xinv {
   __syncthreads();
   lds_fft<true, false>(lds, nx, lpb, a.tw);
   double mx = 0.0;
-  for (int e = threadIdx.x; e < lpb * nx; e += blockDim.x) {
+  for (int e = threadIdx.x; e < lpb * nx; e += block_dim_x()) {
     int l = e / nx, i = e - l * nx;
     long line = line0 + l;
     if (line < nlines) {
@@ -266,7 +266,7 @@ struct k_xinv {
   }
   red[threadIdx.x] = mx;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = block_dim_x() / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
@@ -288,14 +288,14 @@ struct k_ypass {
   int m0 = tile * kTile;
   int cols = min(kTile, mxh - m0);
   double2 *base = spec + (long)k * ny * mxh + m0;
-  for (int e = threadIdx.x; e < ny * kTile; e += blockDim.x) {
+  for (int e = threadIdx.x; e < ny * kTile; e += block_dim_x()) {
     int j = e / kTile, col = e - j * kTile;
     double2 v = col < cols ? base[(long)j * mxh + col] : make_double2(0.0, 0.0);
     lds[e] = v;  // element j of column col at j*kTile + col
   }
   __syncthreads();
   lds_fft<INV, true>(lds, ny, kTile, tw);
-  for (int e = threadIdx.x; e < ny * kTile; e += blockDim.x) {
+  for (int e = threadIdx.x; e < ny * kTile; e += block_dim_x()) {
     int j = e / kTile, col = e - j * kTile;
     if (col < cols) base[(long)j * mxh + col] = lds[e];
   }
@@ -319,14 +319,14 @@ struct k_zpass {
   int cols = min(kTile, mxh - m0);
   long stride = (long)ny * mxh;
   double2 *base = spec + (long)j * mxh + m0;
-  for (int e = threadIdx.x; e < nz * kTile; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nz * kTile; e += block_dim_x()) {
     int k = e / kTile, col = e - k * kTile;
     double2 v = col < cols ? base[(long)k * stride + col] : make_double2(0.0, 0.0);
     lds[e] = v;  // element k of column col at k*kTile + col
   }
   __syncthreads();
   lds_fft<false, true>(lds, nz, kTile, tw);
-  for (int e = threadIdx.x; e < nz * kTile; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nz * kTile; e += block_dim_x()) {
     int k = e / kTile, col = e - k * kTile;
     int m = m0 + col;
     if (col < cols && (m + j + k > 0)) {
@@ -339,7 +339,7 @@ struct k_zpass {
   }
   __syncthreads();
   lds_fft<true, true>(lds, nz, kTile, tw);
-  for (int e = threadIdx.x; e < nz * kTile; e += blockDim.x) {
+  for (int e = threadIdx.x; e < nz * kTile; e += block_dim_x()) {
     int k = e / kTile, col = e - k * kTile;
     if (col < cols) base[(long)k * stride + col] = lds[e];
   }
@@ -422,7 +422,7 @@ __device__ __forceinline__ void step_a(double2 (&v)[R2], int t, const double2 *_
 template <int R1, int R2>
 struct k_xfwd4 {
   static constexpr int kBounds = 256;
-  static __device__ __forceinline__ void run(XArgs a, int C) {
+  static __device__ __forceinline__ void run(const XArgs &a, int C) {
   if (*a.done != 0.0) return;  // the solve has converged
   // The exchange moves real and imaginary parts in two rounds through an
   // LDS buffer of doubles: half the LDS per block, more blocks per CU.
@@ -479,7 +479,7 @@ struct k_xfwd4 {
 template <int R1, int R2>
 struct k_xinv4 {
   static constexpr int kBounds = 256;
-  static __device__ __forceinline__ void run(XInvArgs a, int C) {
+  static __device__ __forceinline__ void run(const XInvArgs &a, int C) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double2 lds[];
   __shared__ double red[256];
@@ -533,7 +533,7 @@ struct k_xinv4 {
   }
   red[threadIdx.x] = mx;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = block_dim_x() / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
@@ -691,7 +691,7 @@ constexpr int kRunPad = 8;  // spare complex values after each nz Y1 run
 template <int R1, int R2, int Y2>
 struct k_xy_fwd {
   static constexpr int kBounds = 256;
-  static __device__ __forceinline__ void run(XArgs a, int Y1, const double2 *__restrict__ twy) {
+  static __device__ __forceinline__ void run(const XArgs &a, int Y1, const double2 *__restrict__ twy) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double ldsd[];
   constexpr int n = R1 * R2, P = R1 + 1, SP = Y2 + 1;  // padded rows of the x exchange and of the kx columns
@@ -742,7 +742,7 @@ struct k_xy_fwd {
   // y step A for every kx
   const long cs = (long)g.nz * Y1 + kRunPad;
   const long ks = cs * mxh;  // distance between consecutive k2
-  for (int kx = threadIdx.x; kx < mxh; kx += blockDim.x) {
+  for (int kx = threadIdx.x; kx < mxh; kx += block_dim_x()) {
     double2 w[Y2];
 #pragma unroll
     for (int m = 0; m < Y2; ++m) w[m] = S[kx * SP + m];
@@ -770,7 +770,7 @@ struct k_yz_pass {
   const int k2 = (int)(blk / (unsigned)mxh), kx = (int)(blk % (unsigned)mxh);
   double2 *base = spec + (long)blk * (NE + kRunPad);  // element (z, t) at z Y1 + t
   const int tid = threadIdx.x;
-  for (int e = tid; e < NE; e += blockDim.x) L[pad(e)] = base[e];
+  for (int e = tid; e < NE; e += block_dim_x()) L[pad(e)] = base[e];
   __syncthreads();
   // y step B: item z, DFT over t
   const bool zi = tid < nz;
@@ -867,14 +867,14 @@ struct k_yz_pass {
     for (int q = 0; q < Y1; ++q) L[pad(tid * Y1 + q)] = u[q];
   }
   __syncthreads();
-  for (int e = tid; e < NE; e += blockDim.x) base[e] = L[pad(e)];
+  for (int e = tid; e < NE; e += block_dim_x()) base[e] = L[pad(e)];
 }
 };
 
 template <int R1, int R2, int Y2>
 struct k_xy_inv {
   static constexpr int kBounds = 256;
-  static __device__ __forceinline__ void run(XInvArgs a, int Y1, const double2 *__restrict__ twy) {
+  static __device__ __forceinline__ void run(const XInvArgs &a, int Y1, const double2 *__restrict__ twy) {
   if (*a.done != 0.0) return;  // the solve has converged
   extern __shared__ double ldsd[];
   __shared__ double red[256];
@@ -887,7 +887,7 @@ struct k_xy_inv {
   // inverse y step A for every kx: row m is y = t + Y1 m
   const long cs = (long)g.nz * Y1 + kRunPad;
   const long ks = cs * mxh;
-  for (int kx = threadIdx.x; kx < mxh; kx += blockDim.x) {
+  for (int kx = threadIdx.x; kx < mxh; kx += block_dim_x()) {
     const double2 *src = a.spec + kx * cs + z * Y1 + t;
     double2 w[Y2];
 #pragma unroll
@@ -952,7 +952,7 @@ struct k_xy_inv {
   }
   red[threadIdx.x] = mx;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = block_dim_x() / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }

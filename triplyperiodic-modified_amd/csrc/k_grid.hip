@@ -33,7 +33,7 @@ __device__ __forceinline__ Ix make_ix(const Geo &g) {
 // cell (i,j,k) handled by this thread, with the XCD-contiguous block order
 __device__ __forceinline__ bool cell_of(const Geo &g, int &i, int &j, int &k, long &c) {
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  c = (long)b * blockDim.x + threadIdx.x;
+  c = (long)b * block_dim_x() + threadIdx.x;
   if (c >= g.n) return false;
   i = (int)(c % g.nx);
   long r = c / g.nx;
@@ -46,7 +46,7 @@ __device__ __forceinline__ bool cell_of(const Geo &g, int &i, int &j, int &k, lo
 __device__ __forceinline__ double block_sum(double v, double *sh) {
   sh[threadIdx.x] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = block_dim_x() / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
@@ -56,7 +56,7 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 __device__ __forceinline__ double block_max(double v, double *sh) {
   sh[threadIdx.x] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = block_dim_x() / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
@@ -121,186 +121,23 @@ struct ZmGeom { int TX, TY, KC; };
 template <bool WITH_RHO, bool HAS_S, bool RK3>
 struct k_rhs_ru {
   static constexpr int kBounds = 256;
-  static __device__ __forceinline__ void run(Geo g, Coef kc, RhsArgs a, ZmGeom zg) {
-  extern __shared__ double sm[];
-  const int TX = zg.TX, TY = zg.TY, KC = zg.KC;
-  const int RW = TX + 3, RN = RW * (TY + 3);   // raw tile: x from i0-2 to i0+TX
-  const int UW = TX + 2, UN = UW * (TY + 2);   // U tile: x from i0-1 to i0+TX
-  const int DW = TX + 1, DN = DW * (TY + 1);   // divU tile: x from i0-1 to i0+TX-1
-  double *raw = sm;              // [slot 2][field 4][RN]: RU_int x, y, z, Rho_int
-  double *u0t = raw + 8 * RN;    // U_x of plane k
-  double *u1t = u0t + UN;        // U_y of plane k
-  double *u2t = u1t + UN;        // U_z of planes k and k+1 (two buffers)
-  double *dvt = u2t + 2 * UN;    // div U of plane k
-  (void)DN;
-  const Ix ix = make_ix(g);
-  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  const int ntx = g.nx / TX, nty = g.ny / TY;
-  const int i0 = (int)(b % ntx) * TX, j0 = (int)((b / ntx) % nty) * TY, k0 = (int)(b / (ntx * nty)) * KC;
-  const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-  const int i = i0 + tx, j = j0 + ty;
-  const double *__restrict__ f4[4] = {a.ru_int.a[0], a.ru_int.a[1], a.ru_int.a[2], a.rho_int};
-  auto load_plane = [&](int kz, int slot) {
-    const long zoff = (long)(kz & ix.mz) * g.nxy;
-    for (int e = tid; e < RN; e += blockDim.x) {
-      const int li = e % RW, lj = e / RW;
-      const long off = zoff + (long)((j0 + lj - 2) & ix.my) * g.nx + ((i0 + li - 2) & ix.mx);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) raw[(slot * 4 + q) * RN + e] = f4[q][off];
-    }
-  };
-  // raw value of field q in slot s at cell (i0 + x, j0 + y)
-  auto RV = [&](int s, int q, int x, int y) -> double { return raw[(s * 4 + q) * RN + (y + 2) * RW + (x + 2)]; };
-  auto UT = [&](const double *t, int x, int y) -> double { return t[(y + 1) * UW + (x + 1)]; };
-  auto DT = [&](int x, int y) -> double { return dvt[(y + 1) * DW + (x + 1)]; };
-
-  // ---- prologue: plane k0-1 in slot 1, plane k0 in slot 0
-  load_plane(k0 - 1, 1);
-  load_plane(k0, 0);
-  __syncthreads();
-  double *u2c = u2t, *u2n = u2t + UN;
-  for (int e = tid; e < UN; e += blockDim.x) {
-    const int x = e % UW - 1, y = e / UW - 1;
-    u2c[e] = RV(0, 2, x, y) / ((RV(0, 3, x, y) + RV(1, 3, x, y)) * 0.5);
+  static __device__ __forceinline__ void run(const Geo &g, const Coef &kc, const RhsArgs &a, const ZmGeom &zg) {
+#include "k_rhs_ru_body.inc"
   }
-  // own-column values of plane k0-1
-  double Up[3], dvp, r0p, r0p1, r1p, r1p1, r2p, rhop;
-  Up[0] = RV(1, 0, tx, ty) / ((RV(1, 3, tx, ty) + RV(1, 3, tx - 1, ty)) * 0.5);
-  Up[1] = RV(1, 1, tx, ty) / ((RV(1, 3, tx, ty) + RV(1, 3, tx, ty - 1)) * 0.5);
-  Up[2] = Uface<2>(ix, a.ru_int.a[2], a.rho_int, i, j, k0 - 1);
-  dvp = divU(g, ix, a.ru_int, a.rho_int, i, j, k0 - 1);
-  r0p = RV(1, 0, tx, ty); r0p1 = RV(1, 0, tx + 1, ty);
-  r1p = RV(1, 1, tx, ty); r1p1 = RV(1, 1, tx, ty + 1);
-  r2p = RV(1, 2, tx, ty); rhop = RV(1, 3, tx, ty);
-  __syncthreads();
-  load_plane(k0 + 1, 1);
-  __syncthreads();
-
-  int cur = 0;
-  const double id[3] = {g.idx, g.idy, g.idz};
-  for (int kk = k0; kk < k0 + KC; ++kk) {
-    const int nxt = cur ^ 1;
-    // prefetch plane k+2 into registers; it is stored after this plane's work
-    // (a tile has at most 2 points per thread: (TX+3)(TY+3) <= 2 TX TY)
-    const bool more = kk + 1 < k0 + KC;
-    double pf[2][4];
-    if (more) {
-      const long zoff = (long)((kk + 2) & ix.mz) * g.nxy;
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int e = tid + u * (int)blockDim.x;
-        if (e < RN) {
-          const int li = e % RW, lj = e / RW;
-          const long off = zoff + (long)((j0 + lj - 2) & ix.my) * g.nx + ((i0 + li - 2) & ix.mx);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) pf[u][q] = f4[q][off];
-        }
-      }
-    }
-    // ---- face velocities of plane k (x, y) and U_z of plane k+1
-    for (int e = tid; e < UN; e += blockDim.x) {
-      const int x = e % UW - 1, y = e / UW - 1;
-      u0t[e] = RV(cur, 0, x, y) / ((RV(cur, 3, x, y) + RV(cur, 3, x - 1, y)) * 0.5);
-      u1t[e] = RV(cur, 1, x, y) / ((RV(cur, 3, x, y) + RV(cur, 3, x, y - 1)) * 0.5);
-      u2n[e] = RV(nxt, 2, x, y) / ((RV(nxt, 3, x, y) + RV(cur, 3, x, y)) * 0.5);
-    }
-    __syncthreads();
-    // ---- div U of plane k (tensor0.cpp:249-256)
-    for (int e = tid; e < (TX + 1) * (TY + 1); e += blockDim.x) {
-      const int x = e % DW - 1, y = e / DW - 1;
-      dvt[e] = (UT(u0t, x + 1, y) - UT(u0t, x, y)) * g.idx + (UT(u1t, x, y + 1) - UT(u1t, x, y)) * g.idy +
-               (UT(u2n, x, y) - UT(u2c, x, y)) * g.idz;
-    }
-    __syncthreads();
-    const long c = (long)kk * g.nxy + (long)j * g.nx + i;
-    double rr[3];
-#pragma unroll
-    for (int A = 0; A < 3; ++A) {
-      const int di = (A == 0), dj = (A == 1), dk = (A == 2);
-      const double *ut = A == 0 ? u0t : (A == 1 ? u1t : u2c);
-      const double Upc = UT(ut, tx, ty);
-      const double Uxm = UT(ut, tx - 1, ty), Uxp = UT(ut, tx + 1, ty);
-      const double Uym = UT(ut, tx, ty - 1), Uyp = UT(ut, tx, ty + 1);
-      const double Uzm = Up[A];
-      const double Uzp = A == 0 ? RV(nxt, 0, tx, ty) / ((RV(nxt, 3, tx, ty) + RV(nxt, 3, tx - 1, ty)) * 0.5)
-                       : A == 1 ? RV(nxt, 1, tx, ty) / ((RV(nxt, 3, tx, ty) + RV(nxt, 3, tx, ty - 1)) * 0.5)
-                                : UT(u2n, tx, ty);
-      const double dv_own = DT(tx, ty);
-      const double dv_m = dk ? dvp : DT(tx - di, ty - dj);
-      double gdiv = (dv_own - dv_m) * id[A];
-      double lap = (Uxp - 2 * Upc + Uxm) * g.idx2 + (Uyp - 2 * Upc + Uym) * g.idy2 +
-                   (Uzp - 2 * Upc + Uzm) * g.idz2;
-      double r = kc.a_drag * (HAS_S ? a.S[A][c] : 0.0) + kc.b_visc * gdiv + kc.c_visc * lap;
-      // convection: RU_d at the positions the flux interpolants read
-      const double x0a = RV(cur, 0, tx + 1, ty);
-      const double x0b = dk ? r0p1 : RV(cur, 0, tx + 1 - di, ty - dj);
-      const double x0c = RV(cur, 0, tx, ty);
-      const double x0d = dk ? r0p : RV(cur, 0, tx - di, ty - dj);
-      const double y1a = RV(cur, 1, tx, ty + 1);
-      const double y1b = dk ? r1p1 : RV(cur, 1, tx - di, ty + 1 - dj);
-      const double y1c = RV(cur, 1, tx, ty);
-      const double y1d = dk ? r1p : RV(cur, 1, tx - di, ty - dj);
-      const double z2a = RV(nxt, 2, tx, ty);
-      const double z2b = dk ? RV(cur, 2, tx, ty) : RV(nxt, 2, tx - di, ty - dj);
-      const double z2c = RV(cur, 2, tx, ty);
-      const double z2d = dk ? r2p : RV(cur, 2, tx - di, ty - dj);
-      double fxp = ((Uxp + Upc) * 0.5) * ((x0a + x0b) * 0.5);
-      double fxm = ((Upc + Uxm) * 0.5) * ((x0c + x0d) * 0.5);
-      double fyp = ((Uyp + Upc) * 0.5) * ((y1a + y1b) * 0.5);
-      double fym = ((Upc + Uym) * 0.5) * ((y1c + y1d) * 0.5);
-      double fzp = ((Uzp + Upc) * 0.5) * ((z2a + z2b) * 0.5);
-      double fzm = ((Upc + Uzm) * 0.5) * ((z2c + z2d) * 0.5);
-      double conv = (fxp - fxm) * g.idx + (fyp - fym) * g.idy + (fzp - fzm) * g.idz;
-      r = r - conv;
-      r = r + kc.A * RV(cur, A, tx, ty);
-      const double rho_m = dk ? rhop : RV(cur, 3, tx - di, ty - dj);
-      double rf = (RV(cur, 3, tx, ty) + rho_m) * 0.5;
-      r = r + (rf - kc.rho0) * kc.g[A];
-      rr[A] = r;
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      double np1 = a.ru_np1_in[d][c] + a.c_np1 * rr[d];
-      a.ru_np1[d][c] = np1;
-      if (!RK3) a.ru_new[d][c] = 1 * a.ru_base[d][c] + a.c_new * rr[d];
-    }
-    if (WITH_RHO) {
-      double R = (RV(cur, 0, tx + 1, ty) - RV(cur, 0, tx, ty)) * g.idx +
-                 (RV(cur, 1, tx, ty + 1) - RV(cur, 1, tx, ty)) * g.idy +
-                 (RV(nxt, 2, tx, ty) - RV(cur, 2, tx, ty)) * g.idz;
-      double np1 = a.rho_np1_in[c] + a.c_rho_np1 * R;
-      a.rho_np1[c] = np1;
-      if (!RK3) a.rho_new[c] = 1 * a.rho_base[c] + a.c_rho_new * R;
-    }
-    // ---- carry plane k to the next step
-    Up[0] = UT(u0t, tx, ty); Up[1] = UT(u1t, tx, ty); Up[2] = UT(u2c, tx, ty);
-    dvp = DT(tx, ty);
-    r0p = RV(cur, 0, tx, ty); r0p1 = RV(cur, 0, tx + 1, ty);
-    r1p = RV(cur, 1, tx, ty); r1p1 = RV(cur, 1, tx, ty + 1);
-    r2p = RV(cur, 2, tx, ty); rhop = RV(cur, 3, tx, ty);
-    __syncthreads();
-    if (more) {
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int e = tid + u * (int)blockDim.x;
-        if (e < RN) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) raw[(cur * 4 + q) * RN + e] = pf[u][q];
-        }
-      }
-    }
-    double *t = u2c; u2c = u2n; u2n = t;
-    cur = nxt;
-    __syncthreads();
-  }
-}
 };
+
+// The single-box launch keeps the body in the kernel itself: inlined from a
+// device function it allocates scalar registers worse (measured 10 % slower).
+template <bool WITH_RHO, bool HAS_S, bool RK3>
+__global__ __launch_bounds__(256) void k_rhs_ru_kernel(Geo g, Coef kc, RhsArgs a, ZmGeom zg) {
+#include "k_rhs_ru_body.inc"
+}
 
 // G1 alone: Update_Rho (grid.cpp:608-614)
 template <bool RK3>
 struct k_update_rho {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(Geo g, RhsArgs a) {
+  static __device__ __forceinline__ void run(const Geo &g, const RhsArgs &a) {
   int i, j, k;
   long c;
   if (!cell_of(g, i, j, k, c)) return;
@@ -348,7 +185,7 @@ __device__ __forceinline__ double low_mach_div(const Geo &g, const Ix &ix, const
 template <int MODE>
 struct k_div_rhs {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(Geo g, Coef kc, DivArgs a) {
+  static __device__ __forceinline__ void run(const Geo &g, const Coef &kc, const DivArgs &a) {
   __shared__ double sh[kBlock];
   int i, j, k;
   long c;
@@ -391,7 +228,7 @@ struct ProjArgs {
 template <bool RK3>
 struct k_project {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(Geo g, ProjArgs a) {
+  static __device__ __forceinline__ void run(const Geo &g, const ProjArgs &a) {
   __shared__ double sh[kBlock];
   int i, j, k;
   long c;
@@ -428,7 +265,7 @@ struct k_shift_mean {
   static __device__ __forceinline__ void run(long n, const double *__restrict__ in,
                                                        double *__restrict__ out, const double *scal,
                                                        int slot, double ncell, double a) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
   if (c >= n) return;
   double mean = scal[slot] / ncell;  // mean() = sum()/size_tot()
   out[c] = (in[c] - mean) + a;
@@ -442,7 +279,7 @@ struct k_reduce {
   static __device__ __forceinline__ void run(long n, const double *__restrict__ f,
                                                    double *partials) {
   __shared__ double sh[kBlock];
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
   double v = 0.0;
   if (c < n) v = OP == 0 ? f[c] : fabs(f[c]);
   double s = OP == 0 ? block_sum(v, sh) : block_max(v, sh);
@@ -478,7 +315,7 @@ struct k_finish {
 // grid::Update_P0 (grid.cpp:816-830) on the device scalars
 struct k_update_p0 {
   static constexpr int kBounds = 1024;
-  static __device__ __forceinline__ void run(Coef kc, double *scal, double ncell, double c_np1, double c_new) {
+  static __device__ __forceinline__ void run(const Coef &kc, double *scal, double ncell, double c_np1, double c_new) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double met = scal[SC_SUM_Q] / ncell;
   scal[SC_MET] = met;
@@ -500,7 +337,7 @@ struct k_scalar_copy {
 struct k_fill {
   static constexpr int kBounds = 1024;
   static __device__ __forceinline__ void run(double *f, double v, long n) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
   if (c < n) f[c] = v;
 }
 };
@@ -508,7 +345,7 @@ struct k_fill {
 struct k_copy {
   static constexpr int kBounds = 1024;
   static __device__ __forceinline__ void run(double *__restrict__ d, const double *__restrict__ s, long n) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
   if (c < n) d[c] = s[c];
 }
 };
@@ -567,7 +404,11 @@ void launch_rhs_ru(tpx_ctx *c, const Coef &k, bool with_rho, const double *const
   size_t sh = sizeof(double) * (size_t)(8 * (zg.TX + 3) * (zg.TY + 3) + 4 * (zg.TX + 2) * (zg.TY + 2) +
                                         (zg.TX + 1) * (zg.TY + 1));
   timing_begin(c, TPX_K_GRID_RHS);
-#define TPX_K1(WR, HS, R3) tpx_launch<k_rhs_ru<WR, HS, R3>>(nb, threads, sh, c->stream, g, k, a, zg)
+#define TPX_K1(WR, HS, R3)                                                            \
+  do {                                                                                \
+    if (g_rec) tpx_launch<k_rhs_ru<WR, HS, R3>>(nb, threads, sh, c->stream, g, k, a, zg); \
+    else k_rhs_ru_kernel<WR, HS, R3><<<nb, threads, sh, c->stream>>>(g, k, a, zg);         \
+  } while (0)
   if (with_rho) {
     if (hs) { if (r3) TPX_K1(true, true, true); else TPX_K1(true, true, false); }
     else { if (r3) TPX_K1(true, false, true); else TPX_K1(true, false, false); }

@@ -86,8 +86,8 @@ struct AdvArgs {
 // 477-492, 494-520, 58-104)
 struct k_advance {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(AdvArgs a) {
-  long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  static __device__ __forceinline__ void run(const AdvArgs &a) {
+  long p = (long)xcd_block(blockIdx.x, gridDim.x) * block_dim_x() + threadIdx.x;
   if (p >= a.n) return;
   const Geo &g = a.g;
   Ix ix = make_ix(g);
@@ -178,9 +178,9 @@ struct k_advance {
 // floor-cell key of one stage of positions, counted into its cell
 struct k_keys {
   static constexpr int kBounds = 1024;
-  static __device__ __forceinline__ void run(Geo g, const double *x, const double *y, const double *z, int32_t *key,
+  static __device__ __forceinline__ void run(const Geo &g, const double *x, const double *y, const double *z, int32_t *key,
                        int32_t *cnt, long n) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
   if (p >= n) return;
   Ix ix = make_ix(g);
   int32_t k = (int32_t)ix((int)(x[p] * g.idx), (int)(y[p] * g.idy), (int)(z[p] * g.idz));
@@ -192,7 +192,7 @@ struct k_keys {
 struct k_set_i {
   static constexpr int kBounds = 1024;
   static __device__ __forceinline__ void run(int32_t *a, int32_t v, long n) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
   if (p < n) a[p] = v;
 }
 };
@@ -200,7 +200,7 @@ struct k_set_i {
 struct k_copy_i {
   static constexpr int kBounds = 1024;
   static __device__ __forceinline__ void run(int32_t *__restrict__ d, const int32_t *__restrict__ s, long n) {
-  long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
   if (p < n) d[p] = s[p];
 }
 };
@@ -213,7 +213,7 @@ struct k_place {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const int32_t *__restrict__ key, int32_t *__restrict__ cursor,
                                                   int32_t *__restrict__ perm, long n) {
-  long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  long p = (long)xcd_block(blockIdx.x, gridDim.x) * block_dim_x() + threadIdx.x;
   if (p < n) perm[atomicAdd(&cursor[key[p]], 1)] = (int32_t)p;
 }
 };
@@ -231,7 +231,7 @@ __device__ __forceinline__ void cswap(int32_t &a, int32_t &b) {
 struct k_fixup {
   static constexpr int kBounds = 1024;
   static __device__ __forceinline__ void run(const int32_t *__restrict__ start, int32_t *perm, int32_t *inv, long ncell) {
-  long c = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  long c = (long)xcd_block(blockIdx.x, gridDim.x) * block_dim_x() + threadIdx.x;
   if (c >= ncell) return;
   int32_t b = start[c], e = start[c + 1];
   const int cnt = e - b;
@@ -267,7 +267,7 @@ struct k_fixup {
 struct k_iota {
   static constexpr int kBounds = 1024;
   static __device__ __forceinline__ void run(int32_t *p, long n) {
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long q = (long)blockIdx.x * block_dim_x() + threadIdx.x;
   if (q < n) p[q] = (int32_t)q;
 }
 };
@@ -358,13 +358,13 @@ constexpr int kHeatChunk = 2;
 
 struct k_heat_fused {
   static constexpr int kBounds = kHeatMaxBlock;
-  static __device__ __forceinline__ void run(HeatFusedArgs a) {
+  static __device__ __forceinline__ void run(const HeatFusedArgs &a) {
   // LDS: 9 x blockDim doubles shared by the records of a chunk and the x
   // exchange, then T: 3 planes x (rows + 2) x nx
   extern __shared__ double xs[];
   const Geo &g = a.g;
   Ix ix = make_ix(g);
-  const int nx = g.nx, bd = (int)blockDim.x, t = (int)threadIdx.x;
+  const int nx = g.nx, bd = (int)block_dim_x(), t = (int)threadIdx.x;
   const int i = t & ix.mx, r = t / nx;
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
   const int nyb = g.ny / a.rows;
@@ -540,11 +540,11 @@ struct k_heat_fused {
 // with block partial sums for the mean of Q
 struct k_heat_combine {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(Geo g, const double *__restrict__ t3, double *out,
+  static __device__ __forceinline__ void run(const Geo &g, const double *__restrict__ t3, double *out,
                                                          double *partials) {
   __shared__ double red[kBlock];
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  const long c = (long)b * blockDim.x + threadIdx.x;
+  const long c = (long)b * block_dim_x() + threadIdx.x;
   double v = 0.0;
   if (c < g.n) {
     const int my = g.ny - 1;
@@ -556,7 +556,7 @@ struct k_heat_combine {
   }
   red[threadIdx.x] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = block_dim_x() / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
@@ -580,7 +580,7 @@ constexpr int kVelChunk = 2;
 template <int NV>
 __device__ __forceinline__ void cell_sums(const int32_t *__restrict__ start, long cb, int ncell,
                                           const double *const (&arr)[NV], double *lds, double (&out)[NV]) {
-  const int t = (int)threadIdx.x, bd = (int)blockDim.x, cap = kVelChunk * bd;
+  const int t = (int)threadIdx.x, bd = (int)block_dim_x(), cap = kVelChunk * bd;
   const int32_t Qa = start[cb], Qb = start[cb + ncell];
   int32_t q0 = 0, q1 = 0;
   if (t < ncell) { q0 = start[cb + t]; q1 = start[cb + t + 1]; }
@@ -617,13 +617,13 @@ __device__ __forceinline__ void cell_sums(const int32_t *__restrict__ start, lon
 // previous plane of the march.
 struct k_vel_planes {
   static constexpr int kBounds = kHeatMaxBlock;
-  static __device__ __forceinline__ void run(Geo g, const int32_t *__restrict__ start,
+  static __device__ __forceinline__ void run(const Geo &g, const int32_t *__restrict__ start,
                                                              const double *__restrict__ contrib, long cs,
                                                              double *Sx, double *Sy, double *Sz, int rows,
                                                              int zc) {
   extern __shared__ double ls[];  // 6 x kVelChunk x blockDim
   Ix ix = make_ix(g);
-  const int nx = g.nx, bd = (int)blockDim.x, t = (int)threadIdx.x;
+  const int nx = g.nx, bd = (int)block_dim_x(), t = (int)threadIdx.x;
   const int i = t & ix.mx, r = t / nx;
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
   const int nyb = g.ny / rows;
@@ -671,11 +671,11 @@ struct CicDepArgs {
 // (particle.cpp:522-555)
 struct k_concentration {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(CicDepArgs a) {
+  static __device__ __forceinline__ void run(const CicDepArgs &a) {
   __shared__ double sh[kBlock];
   const Geo &g = a.g;
   unsigned b = xcd_block(blockIdx.x, gridDim.x);
-  long c = (long)b * blockDim.x + threadIdx.x;
+  long c = (long)b * block_dim_x() + threadIdx.x;
   Ix ix = make_ix(g);
   double acc = 0.0;
   if (c < g.n) {
@@ -701,7 +701,7 @@ struct k_concentration {
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = block_dim_x() / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
@@ -725,8 +725,8 @@ struct PermArgs {
 // measured slower)
 struct k_permute {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(PermArgs a) {
-  long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  static __device__ __forceinline__ void run(const PermArgs &a) {
+  long q = (long)xcd_block(blockIdx.x, gridDim.x) * block_dim_x() + threadIdx.x;
   if (q >= a.n) return;
   long s = a.perm[q];
 #pragma unroll

@@ -233,14 +233,26 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
 // (g_rec set), it stores the launch shape and the arguments instead, and the
 // batch later starts one launch per recorded kernel for all of its boxes:
 // box = blockIdx.z, arguments from a table in device memory.
-template <class F, class... A>
-__global__ __launch_bounds__(F::kBounds) void k_single(A... a) {
-  F::run(a...);
+// The work-group width (the hidden group-size argument, 12 bytes into the
+// implicit kernel arguments).  Kernel bodies are device functions inlined into
+// their launch wrappers, where blockDim.x would be read with a run-time test
+// for a partial last work group (the compiler drops that test only for reads
+// written in the kernel itself); every launch here uses whole work groups.
+__device__ __forceinline__ unsigned block_dim_x() {
+  return *(const uint16_t *)((const char *)__builtin_amdgcn_implicitarg_ptr() + 12);
 }
 
 template <class F, class T, size_t... I>
 __device__ __forceinline__ void run_args(const T &t, std::index_sequence<I...>) {
   F::run(std::get<I>(t)...);
+}
+
+// Struct arguments reach the body by reference to the kernel argument itself
+// (constant memory), so fields the body needs again are reloaded from there
+// rather than held in scalar registers.
+template <class F, class... A>
+__global__ __launch_bounds__(F::kBounds) void k_single(A... a) {
+  F::run(a...);
 }
 
 template <class F, class... A>
