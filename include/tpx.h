@@ -151,20 +151,32 @@ int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency);
  * particle counts, box lengths and all coefficients may differ.  Kernel timing
  * must be off.  tpx_batch_create reports why it failed through
  * tpx_batch_error(); tpx_batch_step through tpx_batch_last_error(). */
-/* ---- adjoint of the particle path (no counterpart in the reference).
+/* ---- adjoint of the step (no counterpart in the reference).
  * Objective J = Var(C) over the cells, C the cloud-in-cell concentration of
  * the particles' positions after nsteps steps (particle.cpp:522-555 weights).
  * tpx_adjoint_record advances the context nsteps whole steps exactly as
- * tpx_step would, taping the gas stage fields each particle advance reads
- * and checkpointing the particles every `every` steps, and returns J.
+ * tpx_step would, taping the gas stage fields each particle advance reads and
+ * the gas at each step's start (RU, P), checkpointing the particles every
+ * `every` steps, and returns J.  Only for one-way coupling (TWC off, no heat
+ * exchange), where the gas does not depend on the particles.
  * tpx_adjoint_gradient then writes dJ/dtau_p for every particle and, if
  * dJ_dstate is not null, dJ/d of the particles' state when recording began
  * (6 arrays of np: x, y, z, u, v, w), both in the context's particle order
- * (the order tpx_download_particles uses; match by flag).  Only for
- * one-way coupling (TWC off, no heat exchange), where the gas does not depend
- * on tau_p; tpx_adjoint_release frees the tape (nsteps x 16 grid fields). */
+ * (the order tpx_download_particles uses; match by flag).
+ * tpx_adjoint_gradient_full adds the gas: dJ/dA (the forcing coefficient in
+ * effect, one value) and dJ/d of the gas state when recording began, RU
+ * (3 x n), Rho (n) and P (n), in tpx_download_field's order with bs = 0; any
+ * output may be null, and the gas sweep runs only if a gas output is asked
+ * for.  It recomputes each step of the gas from the tape and reverses it
+ * (the momentum, density, projection and Poisson-iteration transposes), and
+ * leaves the context's gas where the recorded run ended.  The Poisson solves
+ * are differentiated with their recorded iteration counts.
+ * tpx_adjoint_release frees the tape (nsteps x 20 grid fields) and the sweep's
+ * work arrays. */
 int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J);
 int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate);
+int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, double *dJ_dA, double *dJ_dRU,
+                              double *dJ_dRho, double *dJ_dP);
 int tpx_adjoint_release(tpx_ctx *ctx);
 
 typedef struct tpx_batch tpx_batch;

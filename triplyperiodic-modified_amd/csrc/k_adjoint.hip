@@ -145,11 +145,29 @@ struct AdjArgs {
   const double *stage;  // 24 arrays of np: the int state of each substep
   double *lam[6];       // adjoint of the step's result in, of its start out
   double *dtau;         // dJ/dtau accumulated
+  double *lup;          // null, or 4 substeps x 3 face fields: dJ/du_g deposited (the gas adjoint's input)
   const double *tau;
   double tp;
   double c_pre[4], c_post[4];
   long np, ncell;
 };
+
+// The transpose of the gather: lug, the adjoint of u_g at x, goes to the two
+// faces each component was interpolated from, with the gather's weights.
+__device__ __forceinline__ void gather_transpose(const Geo &g, const Ix &ix, const double x[3], const double lug[3],
+                                                 double *lup, long n) {
+  int ii = (int)(x[0] * g.idx), jj = (int)(x[1] * g.idy), kk = (int)(x[2] * g.idz);
+  double xl = (x[0] * g.idx - ((int)(x[0] * g.idx)));
+  double yl = (x[1] * g.idy - ((int)(x[1] * g.idy)));
+  double zl = (x[2] * g.idz - ((int)(x[2] * g.idz)));
+  long c000 = ix(ii, jj, kk);
+  atomicAdd(lup + c000, (1 - xl) * lug[0]);
+  atomicAdd(lup + ix(ii + 1, jj, kk), xl * lug[0]);
+  atomicAdd(lup + n + c000, (1 - yl) * lug[1]);
+  atomicAdd(lup + n + ix(ii, jj + 1, kk), yl * lug[1]);
+  atomicAdd(lup + 2 * n + c000, (1 - zl) * lug[2]);
+  atomicAdd(lup + 2 * n + ix(ii, jj, kk + 1), zl * lug[2]);
+}
 
 // The transposed RK4 step for one particle per thread.
 struct k_adj_step_rev {
@@ -186,6 +204,10 @@ struct k_adj_step_rev {
         lnext[d] = lk[3 + d] * dug[d] * itp;       // through u_g(x)
         lnext[3 + d] = lk[d] - lk[3 + d] * itp;     // k_x = u, and -u/tau
         dtau -= lk[3 + d] * (ug[d] - Y[3 + d]) * itp * itp;
+      }
+      if (a.lup) {
+        const double lug[3] = {lk[3] * itp, lk[4] * itp, lk[5] * itp};
+        gather_transpose(g, ix, Y, lug, a.lup + (long)rk * 3 * a.ncell, a.ncell);
       }
     }
 #pragma unroll
@@ -317,7 +339,7 @@ double adjoint_objective(tpx_ctx *c) {
   return s[1] / (double)n - mean * mean;
 }
 
-void adjoint_reverse(tpx_ctx *c) {
+void adjoint_reverse(tpx_ctx *c, const std::function<void(int)> &gas_step) {
   Particles &pt = c->pt;
   AdjointTape &t = c->adj;
   const long np = pt.n;
@@ -348,6 +370,8 @@ void adjoint_reverse(tpx_ctx *c) {
     a.stage = t.stage;
     for (int q = 0; q < 6; ++q) a.lam[q] = lam[q];
     a.dtau = t.dtau;
+    a.lup = gas_step ? t.lup : nullptr;
+    if (a.lup) launch_fill(c, t.lup, 0.0, 12 * c->g.n);
     a.tau = pt.has_tau ? pt.tau : nullptr;
     a.tp = c->Tp;
     for (int r = 0; r < 4; ++r) {
@@ -358,6 +382,7 @@ void adjoint_reverse(tpx_ctx *c) {
     a.ncell = c->g.n;
     tpx_launch<k_adj_step_rev>(grid1(np), kBlock, 0, c->stream, a);
     TPX_LAUNCH_CHECK();
+    if (gas_step) gas_step(s);
   }
 }
 

@@ -1,0 +1,569 @@
+// k_adjoint_gas.hip -- hand-written adjoint of the gas substep (gfx950).
+//
+// The reverse sweep of one RK4 substep of the one-way coupled gas
+// (grid.cpp:608-614, 776-863 with RHS_RU = 0 and RHS_Part_Temp = 0), run
+// after the substep has been recomputed from its step's checkpoint.  With
+//   F  = I_C2F(Rho_int),  U = RU_int / F                       (G3)
+//   RHS = mu/3 grad div U + mu del2 U - conv(U, RU_int)
+//         + A RU_int + g (F - rho0)                            (G4)
+//   D_rho = div RU_int                                         (G1)
+//   F' = I_C2F(Rho_new),  C = 1/F',  cc = -1/I_F2C(C)
+//   RHS_P = Z((div(RU_new/F') - k/cp del2(1/Rho_new)) / (dt pre))   (G5, G6)
+//   P_{i+1} = P_i + M(cc (div(C grad P_i) - RHS_P)),  i < n     (G7)
+//   RU_np1 -= dt post grad P,  RU_new -= dt pre grad P          (G10)
+// (Z the mean-zero projection, M the spectral solve of poisson.cpp:413-462,
+// which is real and symmetric), the transposes used here are
+//   div^T = -grad, I_C2F^T = I_F2C along the same axis, del2 and grad div
+//   symmetric, M^T = M, div(C grad .) symmetric for fixed C,
+// and the Poisson iterations are reversed with their recorded count:
+//   nu = M lam_P,  mu = cc nu,  lam_P += div(C grad mu),  lam_RHS_P -= mu,
+//   lam_C -= grad mu . grad P_i,  lam_cc += nu (div(C grad P_i) - RHS_P).
+// Face fields follow the forward layout: component a at index x is the face
+// between cells x - e_a and x.  Every kernel is one thread per cell (and the
+// three faces at its index) over the periodic box.
+#include <math.h>
+
+#include <stdexcept>
+
+#include "tpx_internal.h"
+
+namespace tpx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// a cell and its periodic neighbours
+struct Cell {
+  int i, j, k, mx, my, mz, nx;
+  long nxy;
+  __device__ __forceinline__ Cell(const Geo &g, long c) {
+    mx = g.nx - 1; my = g.ny - 1; mz = g.nz - 1; nx = g.nx; nxy = g.nxy;
+    const int lx = __builtin_ctz(g.nx), ly = __builtin_ctz(g.ny);
+    i = (int)(c & mx);
+    j = (int)((c >> lx) & my);
+    k = (int)(c >> (lx + ly));
+  }
+  __device__ __forceinline__ long off(int di, int dj, int dk) const {
+    return (long)((k + dk) & mz) * nxy + (long)((j + dj) & my) * nx + ((i + di) & mx);
+  }
+  // shifted by da along axis a and db along axis b
+  __device__ __forceinline__ long at(int a, int da, int b = 0, int db = 0) const {
+    return off((a == 0) * da + (b == 0) * db, (a == 1) * da + (b == 1) * db, (a == 2) * da + (b == 2) * db);
+  }
+};
+
+__device__ __forceinline__ double inv_d(const Geo &g, int a) { return a == 0 ? g.idx : (a == 1 ? g.idy : g.idz); }
+__device__ __forceinline__ double inv_d2(const Geo &g, int a) { return a == 0 ? g.idx2 : (a == 1 ? g.idy2 : g.idz2); }
+
+// 1/x as the forward Poisson iteration forms it (reciprocal estimate and two
+// Newton steps), so that the spectral solve below sees the residual it is given
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
+// C = 1/Rho_face on face a at index p (between p - e_a and p)
+__device__ __forceinline__ double cface(const double *__restrict__ rho, long p, long q) {
+  return rcp_nr((rho[p] + rho[q]) * 0.5);
+}
+
+// cc = -1/I_F2C(C) of the cell at offset (di, dj, dk)
+__device__ __forceinline__ double cc_at(const Cell &x, const double *__restrict__ rho, int di, int dj, int dk) {
+  const long c = x.off(di, dj, dk);
+  const double cx0 = cface(rho, c, x.off(di - 1, dj, dk)), cx1 = cface(rho, x.off(di + 1, dj, dk), c);
+  const double cy0 = cface(rho, c, x.off(di, dj - 1, dk)), cy1 = cface(rho, x.off(di, dj + 1, dk), c);
+  const double cz0 = cface(rho, c, x.off(di, dj, dk - 1)), cz1 = cface(rho, x.off(di, dj, dk + 1), c);
+  const double third = 1. / 3.;
+  return -rcp_nr(third * ((cx0 + cx1) * 0.5 + (cy0 + cy1) * 0.5 + (cz0 + cz1) * 0.5));
+}
+
+// div(C grad f) at the cell
+__device__ __forceinline__ double div_c_grad(const Geo &g, const Cell &x, const double *__restrict__ rho,
+                                             const double *__restrict__ f, long c) {
+  const double fc = f[c];
+  double s = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const long m = x.at(a, -1), p = x.at(a, 1);
+    const double id = inv_d(g, a);
+    const double c0 = cface(rho, c, m), c1 = cface(rho, p, c);
+    s += (c1 * ((f[p] - fc) * id) - c0 * ((fc - f[m]) * id)) * id;
+  }
+  return s;
+}
+
+// ---- projection (G10) ------------------------------------------------------
+struct GradPArgs {
+  Geo g;
+  const double *np1[3], *nw[3];
+  double *lp;
+  double c_np1, c_new;  // dt post, and dt pre (dt post at the last substep)
+};
+
+struct k_adj_grad_p {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const GradPArgs &a) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= a.g.n) return;
+    const Cell x(a.g, c);
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long p = x.at(d, 1);
+      const double q0 = a.c_np1 * a.np1[d][c] + a.c_new * a.nw[d][c];
+      const double q1 = a.c_np1 * a.np1[d][p] + a.c_new * a.nw[d][p];
+      s += (q1 - q0) * inv_d(a.g, d);
+    }
+    a.lp[c] += s;
+  }
+};
+
+// ---- Poisson solve (G7) ----------------------------------------------------
+struct PoisArgs {
+  Geo g;
+  const double *rho;       // Rho_new
+  const double *Pi;        // P before the iteration being reversed
+  const double *rhs;       // raw RHS_Pois
+  const double *rhs_sum;   // its sum (one value)
+  double *lp;              // lam_P
+  double *tmp_rhs, *tmp_p; // the spectral solve's input and output
+  double *mu;
+  double *lrhs, *lcc, *lC[3], *lF[3];
+  double *scal;            // slots of the spectral solve
+  int first;               // the first reversed iteration writes the accumulators
+};
+
+// the spectral solve of lam_P: a forward iteration from P = 0 with the
+// right-hand side -lam_P/cc has residual lam_P
+struct k_adj_pois_prep {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const PoisArgs &a) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c == 0) {
+      a.scal[SC_SUM_RHS] = 0.0;
+      a.scal[SC_ITERS] = 0.0;
+      a.scal[SC_DONE] = 0.0;
+    }
+    if (c >= a.g.n) return;
+    const Cell x(a.g, c);
+    a.tmp_rhs[c] = -a.lp[c] / cc_at(x, a.rho, 0, 0, 0);
+    a.tmp_p[c] = 0.0;
+  }
+};
+
+// mu = cc nu, lam_RHS -= mu, lam_cc += nu (div(C grad P_i) - RHS_P)
+struct k_adj_pois_a {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const PoisArgs &a) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= a.g.n) return;
+    const Cell x(a.g, c);
+    const double nu = a.tmp_p[c];
+    const double mu = cc_at(x, a.rho, 0, 0, 0) * nu;
+    a.mu[c] = mu;
+    const double mean = a.rhs_sum[0] / (double)a.g.n;
+    const double rr = div_c_grad(a.g, x, a.rho, a.Pi, c) - (a.rhs[c] - mean);
+    a.lrhs[c] = (a.first ? 0.0 : a.lrhs[c]) - mu;
+    a.lcc[c] = (a.first ? 0.0 : a.lcc[c]) + nu * rr;
+  }
+};
+
+// lam_P += div(C grad mu), lam_C -= grad mu . grad P_i
+struct k_adj_pois_b {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const PoisArgs &a) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= a.g.n) return;
+    const Cell x(a.g, c);
+    a.lp[c] += div_c_grad(a.g, x, a.rho, a.mu, c);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long m = x.at(d, -1);
+      const double id = inv_d(a.g, d);
+      const double v = -((a.mu[c] - a.mu[m]) * id) * ((a.Pi[c] - a.Pi[m]) * id);
+      a.lC[d][c] = (a.first ? 0.0 : a.lC[d][c]) + v;
+    }
+  }
+};
+
+// cc's dependence on C, then C = 1/F': lam_F' = -lam_C C^2 (first write of lam_F')
+struct k_adj_pois_c {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const PoisArgs &a) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= a.g.n) return;
+    const Cell x(a.g, c);
+    const double cc0 = cc_at(x, a.rho, 0, 0, 0);
+    const double w0 = cc0 * cc0 * a.lcc[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long m = x.at(d, -1);
+      const double ccm = cc_at(x, a.rho, -(d == 0), -(d == 1), -(d == 2));
+      const double lc = a.lC[d][c] + (w0 + ccm * ccm * a.lcc[m]) * (1. / 6.);
+      const double C = 1.0 / ((a.rho[c] + a.rho[m]) * 0.5);
+      a.lF[d][c] = -lc * C * C;
+    }
+  }
+};
+
+// ---- Poisson right-hand side and conduction term (G5, G6) -------------------
+struct RhsPArgs {
+  Geo g;
+  const double *rho;      // Rho_new
+  const double *ru_new[3];
+  const double *lrhs;
+  const double *scal;     // SC_ADJ_SUM: sum of lam_RHS
+  double s, kappa;        // 1/(dt pre), k/cp
+  double *lnew[3], *lF[3], *lrho;
+};
+
+struct k_adj_rhs_pois {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const RhsPArgs &a) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= a.g.n) return;
+    const Cell x(a.g, c);
+    const double mean = a.scal[SC_ADJ_SUM] / (double)a.g.n;
+    auto nu = [&](long p) { return a.s * (a.lrhs[p] - mean); };
+    const double nc = nu(c), rc = a.rho[c];
+    double lap = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long m = x.at(d, -1), p = x.at(d, 1);
+      const double nm = nu(m);
+      lap += (nu(p) - 2 * nc + nm) * inv_d2(a.g, d);
+      const double F = (rc + a.rho[m]) * 0.5;
+      const double lu = -((nc - nm) * inv_d(a.g, d));
+      a.lnew[d][c] += lu / F;
+      a.lF[d][c] += -lu * a.ru_new[d][c] / (F * F);
+    }
+    a.lrho[c] += a.kappa * lap / (rc * rc);
+  }
+};
+
+// lam_rho += I_F2C-transpose of a face adjoint: each face feeds its two cells
+struct k_adj_fold {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const Geo &g, const double *lF0, const double *lF1, const double *lF2,
+                                             double *out, int accumulate) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= g.n) return;
+    const Cell x(g, c);
+    const double v = 0.5 * ((lF0[c] + lF0[x.at(0, 1)]) + (lF1[c] + lF1[x.at(1, 1)]) + (lF2[c] + lF2[x.at(2, 1)]));
+    out[c] = (accumulate ? out[c] : 0.0) + v;
+  }
+};
+
+// ---- RK4 bookkeeping of the momentum and density updates (G4, G1) ---------
+struct RkArgs {
+  long n;
+  double *lnp1[3], *lnew[3], *lru[3], *lrhs[3];
+  double *lr, *lrnp1, *lrnew, *ld;
+  double c_post, c_pre;
+  int last;  // the fourth substep: new = np1
+};
+
+struct k_adj_rk {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const RkArgs &a) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= a.n) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double ln = a.lnew[d][c];
+      if (a.last) {
+        const double l1 = a.lnp1[d][c] + ln;
+        a.lnp1[d][c] = l1;
+        a.lrhs[d][c] = a.c_post * l1;
+      } else {
+        a.lrhs[d][c] = a.c_post * a.lnp1[d][c] + a.c_pre * ln;
+        a.lru[d][c] += ln;
+      }
+    }
+    const double lrn = a.lrnew[c];
+    if (a.last) {
+      const double l1 = a.lrnp1[c] + lrn;
+      a.lrnp1[c] = l1;
+      a.ld[c] = -a.c_post * l1;
+    } else {
+      a.ld[c] = -a.c_post * a.lrnp1[c] - a.c_pre * lrn;
+      a.lr[c] += lrn;
+    }
+  }
+};
+
+// ---- momentum right-hand side (G4) -----------------------------------------
+struct MomArgs {
+  Geo g;
+  const double *lrhs[3];
+  const double *rui[3];
+  const double *rho;        // Rho_int
+  const double *lup[3];     // the particles' gather adjoint on the faces, or null
+  const double *ld;         // adjoint of div RU_int (G1)
+  double *lu[3];            // adjoint of U
+  double *lint[3], *lF[3];  // adjoint of RU_int, of F (first writes)
+  double *partials;         // per block: sum lam_RHS . RU_int (the forcing coefficient's adjoint)
+  double b_visc, c_visc, A, gv[3];
+};
+
+// adjoint of U = RU_int/F: viscous terms, the convective term's U factor
+// and the particles' gather
+struct k_adj_mom_u {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const MomArgs &a) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= a.g.n) return;
+    const Cell x(a.g, c);
+    const Geo &g = a.g;
+    // div lam_RHS at the cell and at its lower neighbour along each axis
+    auto divl = [&](int di, int dj, int dk) {
+      const long q = x.off(di, dj, dk);
+      return (a.lrhs[0][x.off(di + 1, dj, dk)] - a.lrhs[0][q]) * g.idx +
+             (a.lrhs[1][x.off(di, dj + 1, dk)] - a.lrhs[1][q]) * g.idy +
+             (a.lrhs[2][x.off(di, dj, dk + 1)] - a.lrhs[2][q]) * g.idz;
+    };
+    const double dv0 = divl(0, 0, 0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {  // component d of U
+      const double *L = a.lrhs[d];
+      const double lc = L[c];
+      double v = a.b_visc * (dv0 - divl(-(d == 0), -(d == 1), -(d == 2))) * inv_d(g, d);
+      double lap = 0.0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) lap += (L[x.at(b, 1)] - 2 * lc + L[x.at(b, -1)]) * inv_d2(g, b);
+      v += a.c_visc * lap;
+      // -div_F2C(I_C2F(U_d) * I_d(RU_int)): the U_d factor, through every axis b
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const double idb = inv_d(g, b);
+        const double *R = a.rui[b];
+        const long pb = x.at(b, 1);
+        const double w0 = (lc - L[x.at(b, -1)]) * idb;
+        const double w1 = (L[pb] - lc) * idb;
+        const double d0 = (R[c] + R[x.at(d, -1)]) * 0.5;
+        const double d1 = (R[pb] + R[x.at(b, 1, d, -1)]) * 0.5;
+        v += 0.5 * (w0 * d0 + w1 * d1);
+      }
+      if (a.lup[0]) v += a.lup[d][c];
+      a.lu[d][c] = v;
+    }
+  }
+};
+
+// adjoint of RU_int (forcing, convection's RU factor, U = RU/F, div RU_int of
+// the density update) and of F (body force, U = RU/F); the forcing
+// coefficient's adjoint as block partial sums
+struct k_adj_mom_r {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const MomArgs &a) {
+    __shared__ double sh[kBlock];
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    double part = 0.0;
+    if (c < a.g.n) {
+      const Cell x(a.g, c);
+      const Geo &g = a.g;
+      auto U = [&](int d, long p, long q) { return a.rui[d][p] / ((a.rho[p] + a.rho[q]) * 0.5); };
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {  // component b of RU_int, face b at c
+        const double idb = inv_d(g, b);
+        const long mb = x.at(b, -1);
+        const double Fb = (a.rho[c] + a.rho[mb]) * 0.5;
+        const double Ub = a.rui[b][c] / Fb;
+        const double lrb = a.lrhs[b][c];
+        double v = a.lu[b][c] / Fb + a.A * lrb - (a.ld[c] - a.ld[mb]) * idb;
+        // the RU_b factor of every component d's convective flux along b
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double *L = a.lrhs[d];
+          const long pd = x.at(d, 1), pdmb = x.at(d, 1, b, -1);
+          const double w0 = (L[c] - L[mb]) * idb;
+          const double w1 = (L[pd] - L[pdmb]) * idb;
+          // U_d on its faces at c, c - e_b, c + e_d, c + e_d - e_b
+          const double u00 = U(d, c, x.at(d, -1)), u0m = U(d, mb, x.at(b, -1, d, -1));
+          const double u10 = U(d, pd, c), u1m = U(d, pdmb, mb);
+          v += 0.5 * (w0 * (u00 + u0m) * 0.5 + w1 * (u10 + u1m) * 0.5);
+        }
+        a.lint[b][c] = v;
+        a.lF[b][c] = a.gv[b] * lrb - a.lu[b][c] * Ub / Fb;
+        part += lrb * a.rui[b][c];
+      }
+    }
+    sh[threadIdx.x] = part;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = sh[0];
+  }
+};
+
+// acc[0] += sum of partials (one block)
+struct k_adj_acc {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(const double *partials, long nparts, double *acc) {
+    __shared__ double sh[1024];
+    double v = 0.0;
+    for (long q = threadIdx.x; q < nparts; q += 1024) v += partials[q];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += sh[0];
+  }
+};
+
+// ---- step boundary (TimeAdvance, make_mean_U0) -----------------------------
+// RU = RU_np1 and Rho = Rho_np1 at the start of the next step: their adjoints
+// join the np1 accumulators (and the base accumulators restart at zero)
+struct k_adj_boundary {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(long n, double *l0, double *l1, double *l2, double *b0, double *b1,
+                                             double *b2, double *lr, double *br) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= n) return;
+    l0[c] += b0[c]; b0[c] = 0.0;
+    l1[c] += b1[c]; b1[c] = 0.0;
+    l2[c] += b2[c]; b2[c] = 0.0;
+    lr[c] += br[c]; br[c] = 0.0;
+  }
+};
+
+// f -= scal[slot] / n: the transpose of make_mean_U0's mean removal
+struct k_adj_sub_mean {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(long n, double *f, const double *scal, int slot) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= n) return;
+    f[c] -= scal[slot] / (double)n;
+  }
+};
+
+unsigned cells(const tpx_ctx *c) { return grid1d(c->g.n, kBlock); }
+
+}  // namespace
+
+void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga) {
+  const long n = c->g.n;
+  tpx_launch<k_adj_boundary>(cells(c), kBlock, 0, c->stream, n, ga.lnp1[0], ga.lnp1[1], ga.lnp1[2], ga.lru[0],
+                             ga.lru[1], ga.lru[2], ga.lrnp1, ga.lr);
+  TPX_LAUNCH_CHECK();
+  for (int d = 0; d < 3; ++d) {
+    launch_reduce(c, ga.lnp1[d], 0, SC_ADJ_M0 + d);
+    tpx_launch<k_adj_sub_mean>(cells(c), kBlock, 0, c->stream, n, ga.lnp1[d], (const double *)c->scal, SC_ADJ_M0 + d);
+    TPX_LAUNCH_CHECK();
+  }
+}
+
+void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k) {
+  const long n = c->g.n;
+  const unsigned nb = cells(c);
+  // G11: the int state after this substep is its new state
+  for (int d = 0; d < 3; ++d) std::swap(ga.lnew[d], ga.lint[d]);
+  std::swap(ga.lrnew, ga.lrint);
+  const bool last = k.rk == 3;
+  // G10
+  {
+    GradPArgs a{};
+    a.g = c->g;
+    for (int d = 0; d < 3; ++d) { a.np1[d] = ga.lnp1[d]; a.nw[d] = ga.lnew[d]; }
+    a.lp = ga.lp;
+    a.c_np1 = k.dt * k.post;
+    a.c_new = last ? k.dt * k.post : k.dt * k.pre;
+    tpx_launch<k_adj_grad_p>(nb, kBlock, 0, c->stream, a);
+    TPX_LAUNCH_CHECK();
+  }
+  // G7, iterations in reverse
+  PoisArgs pa{};
+  pa.g = c->g;
+  pa.rho = s.rho_new;
+  pa.rhs = s.rhs;
+  pa.rhs_sum = s.rhs_sum;
+  pa.lp = ga.lp;
+  pa.tmp_rhs = ga.tmp_rhs;
+  pa.tmp_p = ga.tmp_p;
+  pa.mu = ga.mu;
+  pa.lrhs = ga.lrhs_p;
+  pa.lcc = ga.lcc;
+  for (int d = 0; d < 3; ++d) { pa.lC[d] = ga.lC[d]; pa.lF[d] = ga.lF[d]; }
+  pa.scal = ga.scal;
+  if (s.iters < 1) throw std::logic_error("adjoint: a Poisson solve without iterations");
+  for (int i = s.iters - 1; i >= 0; --i) {
+    pa.Pi = s.P[i];
+    pa.first = i == s.iters - 1;
+    tpx_launch<k_adj_pois_prep>(nb, kBlock, 0, c->stream, pa);
+    TPX_LAUNCH_CHECK();
+    launch_poisson_iteration_on(c, s.rho_new, ga.tmp_p, ga.tmp_rhs, ga.scal);
+    tpx_launch<k_adj_pois_a>(nb, kBlock, 0, c->stream, pa);
+    TPX_LAUNCH_CHECK();
+    tpx_launch<k_adj_pois_b>(nb, kBlock, 0, c->stream, pa);
+    TPX_LAUNCH_CHECK();
+  }
+  tpx_launch<k_adj_pois_c>(nb, kBlock, 0, c->stream, pa);
+  TPX_LAUNCH_CHECK();
+  // G6 and G5
+  launch_reduce(c, ga.lrhs_p, 0, SC_ADJ_SUM);
+  {
+    RhsPArgs a{};
+    a.g = c->g;
+    a.rho = s.rho_new;
+    for (int d = 0; d < 3; ++d) { a.ru_new[d] = s.ru_new[d]; a.lnew[d] = ga.lnew[d]; a.lF[d] = ga.lF[d]; }
+    a.lrhs = ga.lrhs_p;
+    a.scal = c->scal;
+    a.s = 1. / (k.dt * k.pre);
+    a.kappa = k.k_over_cp;
+    a.lrho = ga.lrnew;
+    tpx_launch<k_adj_rhs_pois>(nb, kBlock, 0, c->stream, a);
+    TPX_LAUNCH_CHECK();
+  }
+  tpx_launch<k_adj_fold>(nb, kBlock, 0, c->stream, c->g, (const double *)ga.lF[0], (const double *)ga.lF[1],
+                         (const double *)ga.lF[2], ga.lrnew, 1);
+  TPX_LAUNCH_CHECK();
+  // G4 and G1: the RK4 updates, then the right-hand sides
+  {
+    RkArgs a{};
+    a.n = n;
+    for (int d = 0; d < 3; ++d) {
+      a.lnp1[d] = ga.lnp1[d]; a.lnew[d] = ga.lnew[d]; a.lru[d] = ga.lru[d]; a.lrhs[d] = ga.lrhs[d];
+    }
+    a.lr = ga.lr; a.lrnp1 = ga.lrnp1; a.lrnew = ga.lrnew; a.ld = ga.ld;
+    a.c_post = k.dt * k.post;
+    a.c_pre = k.dt * k.pre;
+    a.last = last;
+    tpx_launch<k_adj_rk>(nb, kBlock, 0, c->stream, a);
+    TPX_LAUNCH_CHECK();
+  }
+  MomArgs m{};
+  m.g = c->g;
+  for (int d = 0; d < 3; ++d) {
+    m.lrhs[d] = ga.lrhs[d];
+    m.rui[d] = s.ru_int[d];
+    m.lup[d] = s.lup ? s.lup + (long)d * n : nullptr;
+    m.lu[d] = ga.lu[d];
+    m.lint[d] = ga.lint[d];
+    m.lF[d] = ga.lF[d];
+    m.gv[d] = k.g[d];
+  }
+  m.rho = s.rho_int;
+  m.ld = ga.ld;
+  m.partials = c->partials;
+  m.b_visc = k.b_visc;
+  m.c_visc = k.c_visc;
+  m.A = k.A;
+  tpx_launch<k_adj_mom_u>(nb, kBlock, 0, c->stream, m);
+  TPX_LAUNCH_CHECK();
+  tpx_launch<k_adj_mom_r>(nb, kBlock, 0, c->stream, m);
+  TPX_LAUNCH_CHECK();
+  tpx_launch<k_adj_acc>(1, 1024, 0, c->stream, (const double *)c->partials, (long)nb, c->scal + SC_ADJ_A);
+  TPX_LAUNCH_CHECK();
+  tpx_launch<k_adj_fold>(nb, kBlock, 0, c->stream, c->g, (const double *)ga.lF[0], (const double *)ga.lF[1],
+                         (const double *)ga.lF[2], ga.lrint, 0);
+  TPX_LAUNCH_CHECK();
+}
+
+}  // namespace tpx

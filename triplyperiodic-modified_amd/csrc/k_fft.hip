@@ -1090,11 +1090,12 @@ void xy_inv_launch(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, i
   }
 }
 template <int Z1, int Z2>
-void yz_launch(unsigned nb, size_t sh, hipStream_t st, double2 *spec, int mxh, int Y1, int Y2, const tpx_ctx *c) {
+void yz_launch(unsigned nb, size_t sh, hipStream_t st, double2 *spec, int mxh, int Y1, int Y2, const tpx_ctx *c,
+               const double *done) {
   switch (Y1) {
-    case 4: tpx_launch<k_yz_pass<4, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
-    case 8: tpx_launch<k_yz_pass<8, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
-    default: tpx_launch<k_yz_pass<16, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], c->scal + SC_DONE); break;
+    case 4: tpx_launch<k_yz_pass<4, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], done); break;
+    case 8: tpx_launch<k_yz_pass<8, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], done); break;
+    default: tpx_launch<k_yz_pass<16, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], done); break;
   }
 }
 
@@ -1109,14 +1110,18 @@ void launch_solve_begin(tpx_ctx *c) {
 }
 
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
+  launch_poisson_iteration_on(c, rho_new, c->P, c->rhs, c->scal);
+}
+
+void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal) {
   const Geo &g = c->g;
   int mxh = g.nx / 2 + 1;
   long nlines = (long)g.ny * g.nz;
   int r1, r2;
   timing_begin(c, TPX_K_FFT);
-  const double *done = c->scal + SC_DONE;
-  XArgs xa{g, c->P, rho_new, c->rhs, c->scal, c->spec[0], c->tw[0], (double)g.n, done};
-  XInvArgs xi{g, c->P, c->spec[0], c->tw[0], c->partials, (double)g.n, done};
+  const double *done = scal + SC_DONE;
+  XArgs xa{g, P, rho_new, rhs, scal, c->spec[0], c->tw[0], (double)g.n, done};
+  XInvArgs xi{g, P, c->spec[0], c->tw[0], c->partials, (double)g.n, done};
   const bool x4 = four_step(g.nx, r1, r2);
   int y1, y2, z1, z2;
   if (c->fft_passes == 3 && x4 && four_step(g.ny, y1, y2) && four_step(g.nz, z1, z2)) {
@@ -1128,7 +1133,7 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     TPX_LAUNCH_CHECK();
     const unsigned nb2 = (unsigned)(mxh * y2);
     const size_t sh2 = sizeof(double2) * (size_t)y1 * (g.nz + g.nz / 16);
-#define L_YZ(A, B) yz_launch<A, B>(nb2, sh2, c->stream, c->spec[0], mxh, y1, y2, c)
+#define L_YZ(A, B) yz_launch<A, B>(nb2, sh2, c->stream, c->spec[0], mxh, y1, y2, c, done)
     TPX_FOUR_STEP_SWITCH(g.nz, L_YZ)
 #undef L_YZ
     TPX_LAUNCH_CHECK();
@@ -1136,7 +1141,7 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     TPX_FOUR_STEP_SWITCH(g.nx, L_XYI)
 #undef L_XYI
     TPX_LAUNCH_CHECK();
-    tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nb13, c->scal, c->Iteration, c->p.epsilon);
+    tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nb13, scal, c->Iteration, c->p.epsilon);
     TPX_LAUNCH_CHECK();
     timing_end(c, TPX_K_FFT);
     return;
@@ -1207,7 +1212,7 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
     tpx_launch<k_xinv>(nbx, kThreads, shx, c->stream, xi, cx);
   }
   TPX_LAUNCH_CHECK();
-  tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nbx, c->scal, c->Iteration, c->p.epsilon);
+  tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nbx, scal, c->Iteration, c->p.epsilon);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_FFT);
 }

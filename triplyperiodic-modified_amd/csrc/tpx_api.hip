@@ -364,6 +364,17 @@ void step_end(tpx_ctx *c) {
   do_time_advance(c);
 }
 
+// frees every array of a recorded adjoint run
+void adjoint_free(AdjointTape &t) {
+  auto f = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
+  f(t.gas); f(t.ckpt); f(t.y); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
+  f(t.start); f(t.final_state); f(t.work); f(t.cap); f(t.lup);
+  for (auto &p : t.pcap) f(p);
+  t.pcap.clear();
+  t.ga = GasAdjoint{};
+  t.ready = false;
+}
+
 void release(tpx_ctx *c) {
   if (!c) return;
   auto f = [](void *p) { if (p) (void)hipFree(p); };
@@ -375,7 +386,7 @@ void release(tpx_ctx *c) {
   for (auto &p : c->lam) f(p);
   for (auto &p : c->tw) f(p);
   f(c->scal); f(c->partials); f(c->scan_tmp);
-  f(c->adj.gas); f(c->adj.ckpt); f(c->adj.y); f(c->adj.stage); f(c->adj.lam); f(c->adj.dtau); f(c->adj.djdc);
+  adjoint_free(c->adj);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   Particles &pt = c->pt;
   for (auto &q : pt.q) for (auto &p : q) f(p);
@@ -999,6 +1010,166 @@ void batch_solve(tpx_batch *b, int rk) {
   }
 }
 
+// ---- gas adjoint: recomputation and the reverse sweep -----------------------
+// The gas state of a step boundary, RU, RU_int, Rho (= Rho_int) and P, saved
+// to or restored from dst (8 n values).
+void gas_state_copy(tpx_ctx *c, double *dst, bool save) {
+  const long n = c->g.n;
+  if (c->ru_int == c->ru_base || !c->ru_np1_is_base || !c->rho_np1_is_base)
+    throw std::logic_error("adjoint: the gas is not at a step boundary");
+  double *f[8] = {c->ru[c->ru_base][0], c->ru[c->ru_base][1], c->ru[c->ru_base][2], c->ru[c->ru_int][0],
+                  c->ru[c->ru_int][1],  c->ru[c->ru_int][2],  c->rho[c->rho_base], c->P};
+  for (int q = 0; q < 8; ++q) {
+    if (save) launch_copy(c, dst + q * n, f[q], n);
+    else launch_copy(c, f[q], dst + q * n, n);
+  }
+  if (!save && c->rho_int != c->rho_base) launch_copy(c, c->rho[c->rho_int], dst + 6 * n, n);
+}
+
+// Recomputes step s of the recorded run, gas only (the gas does not depend on
+// the particles under one-way coupling), from its start on the tape, keeping
+// what the reverse sweep reads: RU_new and Rho_new after the momentum and
+// density updates, the raw Poisson right-hand side and its sum, and P before
+// every Poisson iteration.  The arithmetic is the fused step's.
+void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
+  AdjointTape &t = c->adj;
+  const long n = c->g.n;
+  if (c->ru_int == c->ru_base) throw std::logic_error("adjoint: RU_int aliases RU at a step boundary");
+  const double *st = t.start + (long)s * 4 * n;
+  const double *g0 = t.gas + (long)s * 16 * n;
+  for (int d = 0; d < 3; ++d) {
+    launch_copy(c, c->ru[c->ru_base][d], st + d * n, n);
+    launch_copy(c, c->ru[c->ru_int][d], g0 + d * n, n);
+  }
+  launch_copy(c, c->rho[c->rho_base], g0 + 3 * n, n);
+  if (c->rho_int != c->rho_base) launch_copy(c, c->rho[c->rho_int], g0 + 3 * n, n);
+  launch_copy(c, c->P, st + 3 * n, n);
+  c->A = t.A[s];
+  c->Iteration = t.iteration[s];
+  for (int d = 0; d < 3; ++d) c->gv[d] = t.gv[(size_t)s * 3 + d];
+  size_t used = 0;
+  for (int rk = 0; rk < 4; ++rk) {
+    Coef k = coef(c, rk);
+    GasSubstep &g = sub[rk];
+    const double *gs = t.gas + ((long)s * 4 + rk) * 4 * n;
+    for (int d = 0; d < 3; ++d) g.ru_int[d] = gs + d * n;
+    g.rho_int = gs + 3 * n;
+    launch_update_p0(c, k);
+    do_update_ru_wop(c, rk, true);
+    double *cp = t.cap + (long)rk * 5 * n;
+    for (int d = 0; d < 3; ++d) {
+      launch_copy(c, cp + d * n, c->ru[c->ru_new][d], n);
+      g.ru_new[d] = cp + d * n;
+    }
+    launch_copy(c, cp + 3 * n, c->rho[c->rho_new], n);
+    g.rho_new = cp + 3 * n;
+    const double *rn[3];
+    ru_ptrs(c, c->ru_new, rn);
+    launch_div_rhs(c, k, 0, rn, c->rho[c->rho_new], c->q_zero ? nullptr : c->Q, nullptr, c->rhs);
+    launch_copy(c, cp + 4 * n, c->rhs, n);
+    g.rhs = cp + 4 * n;
+    double *sum = t.cap + 20 * n + rk;
+    launch_copy(c, sum, c->scal + SC_SUM_RHS, 1);
+    g.rhs_sum = sum;
+    // the solve one iteration at a time, P kept before each
+    launch_solve_begin(c);
+    g.P.clear();
+    for (int it = 0; it < c->Iteration; ++it) {
+      if (used == t.pcap.size()) {
+        double *p = nullptr;
+        if (hipMalloc((void **)&p, sizeof(double) * n) != hipSuccess) throw std::bad_alloc();
+        t.pcap.push_back(p);
+      }
+      launch_copy(c, t.pcap[used], c->P, n);
+      g.P.push_back(t.pcap[used++]);
+      launch_poisson_iteration(c, c->rho[c->rho_new]);
+      hip_check(hipMemcpyAsync(c->h_scal + SC_ITERS, c->scal + SC_ITERS, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream),
+                "hipMemcpyAsync");
+      sync(c);
+      if (c->h_scal[SC_DONE] != 0.0) break;
+    }
+    g.iters = (int)c->h_scal[SC_ITERS];
+    if (g.iters != (int)g.P.size()) throw std::logic_error("adjoint: recomputed Poisson solve out of step");
+    do_update_ru_wp(c, rk, rk == 3);
+    c->rho_int = c->rho_new;  // the gas half of TimeAdvance_RK4
+    c->ru_int = c->ru_new;
+    c->rho_new = -1;
+    c->ru_new = -1;
+    launch_scalar_copy(c, SC_P0_INT, SC_P0_NEW);
+  }
+  const double a[3] = {c->p.u0 * c->p.rho0, c->p.v0 * c->p.rho0, c->p.w0 * c->p.rho0};
+  do_make_mean_u0(c, a, true);
+  c->rho_base = rho_np1_read(c);  // the gas half of TimeAdvance
+  c->rho_np1_is_base = true;
+  c->ru_base = ru_np1_read(c);
+  c->ru_np1_is_base = true;
+  launch_scalar_copy(c, SC_P0, SC_P0_NP1);
+}
+
+// The reverse sweep of the whole recorded run, particles and gas together:
+// after the particles' step s has deposited dJ/du_g of its four substeps,
+// the gas step s is recomputed and reversed.  The solver's gas ends where the
+// recorded run ended.
+void gas_adjoint_run(tpx_ctx *c) {
+  AdjointTape &t = c->adj;
+  const long n = c->g.n;
+  auto alloc = [](double **p, size_t count) {
+    if (!*p && hipMalloc((void **)p, sizeof(double) * count) != hipSuccess) throw std::bad_alloc();
+  };
+  alloc(&t.work, (size_t)GasAdjoint::kFields * n + SC_COUNT);
+  alloc(&t.cap, 20 * (size_t)n + 8);
+  alloc(&t.lup, 12 * (size_t)n);
+  GasAdjoint &ga = t.ga;
+  double *w = t.work;
+  auto take = [&]() { double *p = w; w += n; return p; };
+  for (auto *grp : {ga.lru, ga.lnp1, ga.lint, ga.lnew, ga.lrhs, ga.lu, ga.lC, ga.lF})
+    for (int d = 0; d < 3; ++d) grp[d] = take();
+  for (double **p : {&ga.lr, &ga.lrnp1, &ga.lrint, &ga.lrnew, &ga.lp, &ga.lrhs_p, &ga.lcc, &ga.ld, &ga.mu,
+                     &ga.tmp_rhs, &ga.tmp_p})
+    *p = take();
+  ga.scal = w;
+  launch_fill(c, t.work, 0.0, (long)GasAdjoint::kFields * n + SC_COUNT);
+  launch_fill(c, c->scal + SC_ADJ_A, 0.0, 1);
+  const double A0 = c->A, g0[3] = {c->gv[0], c->gv[1], c->gv[2]};
+  const int it0 = c->Iteration;
+  GasSubstep sub[4];
+  auto step = [&](int s) {
+    gas_recompute_step(c, s, sub);
+    if (s < t.nsteps - 1) adjoint_gas_boundary(c, ga);
+    for (int rk = 3; rk >= 0; --rk) {
+      sub[rk].lup = t.lup + (long)rk * 3 * n;
+      adjoint_gas_substep(c, ga, sub[rk], coef(c, rk));
+    }
+  };
+  try {
+    adjoint_reverse(c, step);
+  } catch (...) {
+    c->A = A0; c->Iteration = it0;
+    for (int d = 0; d < 3; ++d) c->gv[d] = g0[d];
+    gas_state_copy(c, t.final_state, false);
+    throw;
+  }
+  c->A = A0; c->Iteration = it0;
+  for (int d = 0; d < 3; ++d) c->gv[d] = g0[d];
+  gas_state_copy(c, t.final_state, false);
+}
+
+// host copy of the sum of device arrays (n values each)
+void download_sum(tpx_ctx *c, std::initializer_list<const double *> src, double *dst) {
+  const size_t n = (size_t)c->g.n;
+  std::vector<double> tmp(n);
+  bool first = true;
+  for (const double *p : src) {
+    hip_check(hipMemcpyAsync(first ? dst : tmp.data(), p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream),
+              "download");
+    sync(c);
+    if (!first)
+      for (size_t q = 0; q < n; ++q) dst[q] += tmp[q];
+    first = false;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1109,9 +1280,8 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
     if (c->p.twc || heat_active(c))
       throw std::logic_error("the particle adjoint needs one-way coupling: TWC off and no heat exchange");
     AdjointTape &t = c->adj;
-    auto f = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
-    f(t.gas); f(t.ckpt); f(t.y); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
-    t.ready = false;
+    sync(c);
+    adjoint_free(t);
     t.nsteps = nsteps;
     t.every = every;
     const long n = c->g.n, np = pt.n;
@@ -1125,12 +1295,23 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
     alloc(&t.lam, 6 * (size_t)np);
     alloc(&t.dtau, (size_t)np);
     alloc(&t.djdc, (size_t)n);
+    alloc(&t.start, (size_t)nsteps * 4 * n);
+    alloc(&t.final_state, 8 * (size_t)n);
     t.g3.assign((size_t)nsteps * 3, 0.0);
+    t.A.assign((size_t)nsteps, 0.0);
+    t.gv.assign((size_t)nsteps * 3, 0.0);
+    t.iteration.assign((size_t)nsteps, 0);
     adjoint_checkpoint(c, 0);
     try {
       for (int s = 0; s < nsteps; ++s) {
         t.step = s;
         do_params_update(c);
+        // the gas at the step's start (RU_int and Rho are the first substep's tape)
+        for (int d = 0; d < 3; ++d) launch_copy(c, t.start + ((long)s * 4 + d) * n, c->ru[c->ru_base][d], n);
+        launch_copy(c, t.start + ((long)s * 4 + 3) * n, c->P, n);
+        t.A[s] = c->A;
+        t.iteration[s] = c->Iteration;
+        for (int d = 0; d < 3; ++d) t.gv[(size_t)s * 3 + d] = c->gv[d];
         for (int rk = 0; rk < 4; ++rk) fused_substep(c, rk);
         step_end(c);
         t.step = -1;
@@ -1140,6 +1321,7 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
       t.step = -1;
       throw;
     }
+    gas_state_copy(c, t.final_state, true);
     const double j = adjoint_objective(c);
     if (J) *J = j;
     t.ready = true;
@@ -1148,29 +1330,42 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
 }
 
 int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate) {
+  return tpx_adjoint_gradient_full(ctx, dJ_dtau, dJ_dstate, nullptr, nullptr, nullptr, nullptr);
+}
+
+int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, double *dJ_dA, double *dJ_dRU,
+                              double *dJ_dRho, double *dJ_dP) {
   return guarded(ctx, [&]() {
     AdjointTape &t = ctx->adj;
     if (!t.ready) throw std::logic_error("no recorded run: call tpx_adjoint_record first");
-    if (!dJ_dtau && !dJ_dstate) throw std::invalid_argument("null outputs");
-    adjoint_reverse(ctx);
-    const size_t np = (size_t)ctx->pt.n;
+    const bool gas = dJ_dA || dJ_dRU || dJ_dRho || dJ_dP;
+    if (!gas && !dJ_dtau && !dJ_dstate) throw std::invalid_argument("null outputs");
+    if (gas) gas_adjoint_run(ctx);
+    else adjoint_reverse(ctx, nullptr);
+    const size_t np = (size_t)ctx->pt.n, n = (size_t)ctx->g.n;
     if (dJ_dtau)
       hip_check(hipMemcpyAsync(dJ_dtau, t.dtau, sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream), "download");
     if (dJ_dstate)  // adjoint of the recorded run's initial x, y, z, u, v, w
       hip_check(hipMemcpyAsync(dJ_dstate, t.lam, sizeof(double) * 6 * np, hipMemcpyDeviceToHost, ctx->stream),
                 "download");
+    if (dJ_dA)
+      hip_check(hipMemcpyAsync(dJ_dA, ctx->scal + SC_ADJ_A, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                "download");
     sync(ctx);
+    const GasAdjoint &ga = t.ga;
+    // the recorded run starts with RU_int = RU_np1 = RU and Rho_int = Rho_np1 = Rho
+    if (dJ_dRU)
+      for (int d = 0; d < 3; ++d) download_sum(ctx, {ga.lru[d], ga.lnp1[d], ga.lint[d]}, dJ_dRU + d * n);
+    if (dJ_dRho) download_sum(ctx, {ga.lr, ga.lrnp1, ga.lrint}, dJ_dRho);
+    if (dJ_dP) download_sum(ctx, {ga.lp}, dJ_dP);
     return TPX_OK;
   });
 }
 
 int tpx_adjoint_release(tpx_ctx *ctx) {
   return guarded(ctx, [&]() {
-    AdjointTape &t = ctx->adj;
     sync(ctx);
-    auto f = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
-    f(t.gas); f(t.ckpt); f(t.y); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
-    t.ready = false;
+    adjoint_free(ctx->adj);
     return TPX_OK;
   });
 }

@@ -12,6 +12,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <functional>
+
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -35,6 +37,9 @@ enum Scal {
   SC_SUM_C,
   SC_ITERS,        // iterations run by the current Poisson solve
   SC_DONE,         // 1: the current solve has stopped (its loop test failed)
+  SC_ADJ_SUM,      // adjoint sweep: sum of the Poisson right-hand side's adjoint
+  SC_ADJ_M0, SC_ADJ_M1, SC_ADJ_M2,  // adjoint sweep: sums for make_mean_U0's transpose
+  SC_ADJ_A,        // adjoint sweep: dJ/dA accumulated
   SC_COUNT = 32
 };
 
@@ -80,9 +85,30 @@ struct Particles {
   int32_t *itmp = nullptr;
 };
 
-// The adjoint of the particle path (one-way coupling): the gas stage fields
-// of every substep of a recorded run and the particle base state every
-// `every` steps, plus the reverse sweep's work arrays.
+// Work arrays of the gas adjoint sweep (n values each): adjoints of the
+// base, np1, int and new momentum and density, of P, of the Poisson
+// right-hand side, of the momentum right-hand side, of U and of the face
+// fields F (Rho_face) and C (1/Rho_face), plus the spectral solve's buffers.
+struct GasAdjoint {
+  double *lru[3], *lnp1[3], *lint[3], *lnew[3], *lrhs[3], *lu[3], *lC[3], *lF[3];
+  double *lr, *lrnp1, *lrint, *lrnew, *lp, *lrhs_p, *lcc, *ld, *mu, *tmp_rhs, *tmp_p;
+  double *scal;  // SC_COUNT slots of the spectral solve
+  static constexpr int kFields = 35;
+};
+
+// One recomputed substep as the gas adjoint reads it.
+struct GasSubstep {
+  const double *ru_int[3], *rho_int;   // its start (the tape)
+  const double *ru_new[3], *rho_new;   // after the momentum and density updates
+  const double *rhs, *rhs_sum;         // raw Poisson right-hand side and its sum (device)
+  std::vector<const double *> P;       // P before each Poisson iteration
+  int iters = 0;
+  const double *lup = nullptr;         // the particles' gather adjoint on the faces (3 n), or null
+};
+
+// The adjoint of a recorded run (one-way coupling): the gas stage fields of
+// every substep, each step's gas start (RU, P), the particle base state
+// every `every` steps, and the reverse sweeps' work arrays.
 struct AdjointTape {
   int nsteps = 0, every = 1;
   int step = -1;                 // step being recorded (-1: not recording)
@@ -95,6 +121,16 @@ struct AdjointTape {
   double *lam = nullptr;         // 6 x np
   double *dtau = nullptr;        // np
   double *djdc = nullptr;        // n
+  // gas adjoint
+  double *start = nullptr;       // nsteps x (RU 3, P) x n at each step's start
+  std::vector<double> A, gv;     // forcing coefficient and gravity of each step (gv: 3 per step)
+  std::vector<int> iteration;    // Poisson iteration cap of each step
+  double *final_state = nullptr; // RU 3, RU_int 3, Rho, P of the recorded run's end
+  double *work = nullptr;        // GasAdjoint::kFields x n + SC_COUNT
+  double *cap = nullptr;         // 4 substeps x (RU_new 3, Rho_new, rhs) x n + 4 sums
+  double *lup = nullptr;         // 4 substeps x 3 x n
+  std::vector<double *> pcap;    // P before each Poisson iteration (grown on demand)
+  GasAdjoint ga{};
 };
 
 struct Timing {
@@ -183,6 +219,8 @@ void launch_copy(tpx_ctx *c, double *dst, const double *src, long n);
 int fft_setup(tpx_ctx *c);
 // one Poisson iteration; skipped on the device once the solve has stopped
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
+// the same iteration on caller-given P, raw right-hand side and scalar slots
+void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal);
 // resets the device-side iteration count and loop test of a solve
 void launch_solve_begin(tpx_ctx *c);
 // copies the solve's iteration count and stopped flag to dst[0..1] (device)
@@ -207,7 +245,13 @@ void launch_exclusive_scan(tpx_ctx *c, int32_t *data, int64_t n);  // in place, 
 void adjoint_tape_substep(tpx_ctx *c, int rk, const double *const ru_int[3], const double *rho_int);
 void adjoint_checkpoint(tpx_ctx *c, int slot);
 double adjoint_objective(tpx_ctx *c);  // J = Var(C) of the base positions; leaves dJ/dC in adj.djdc
-void adjoint_reverse(tpx_ctx *c);      // dJ/dtau into adj.dtau
+// dJ/dtau into adj.dtau; with gas_step set, the particles' gather adjoint of
+// each step goes to adj.lup and gas_step(s) runs after the particles' step s
+void adjoint_reverse(tpx_ctx *c, const std::function<void(int)> &gas_step);
+
+// k_adjoint_gas.hip
+void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
+void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga);
 
 // ---- timing helpers (tpx_api.hip) ----
 void timing_begin(tpx_ctx *c, int cls);

@@ -38,7 +38,7 @@ EXPORTS = [
     "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop", "tpx_compute_div_u_new",
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
     "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
-    "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_release",
+    "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_release",
     "tpx_batch_create", "tpx_batch_step", "tpx_batch_destroy", "tpx_batch_error", "tpx_batch_last_error",
     "tpx_field_sum",
     "tpx_field_max_abs",
@@ -93,6 +93,7 @@ def load(path: str = LIB_PATH):
     L.tpx_ensemble_step.argtypes = [ctypes.POINTER(P), I, I, I]
     L.tpx_adjoint_record.argtypes = [P, I, I, DP]
     L.tpx_adjoint_gradient.argtypes = [P, DP, DP]
+    L.tpx_adjoint_gradient_full.argtypes = [P, DP, DP, DP, DP, DP, DP]
     L.tpx_adjoint_release.argtypes = [P]
     L.tpx_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.tpx_batch_step.argtypes = [P, I]
@@ -241,10 +242,10 @@ class Solver:
     def step(self, n: int = 1) -> None:
         self._ok(self.L.tpx_step(self.h, int(n)), "step")
 
-    # ---- adjoint of the particle path (one-way coupling) ----
+    # ---- adjoint of the step (one-way coupling) ----
     def adjoint_record(self, nsteps: int, every: int = 10) -> float:
         """Advance nsteps whole steps (as step()) while taping what the
-        particle adjoint needs; returns J = Var(C) of the final positions."""
+        adjoint needs; returns J = Var(C) of the final positions."""
         out = ctypes.c_double()
         self._ok(self.L.tpx_adjoint_record(self.h, int(nsteps), int(every), ctypes.byref(out)), "adjoint_record")
         return out.value
@@ -259,6 +260,21 @@ class Solver:
         gs = np.zeros((6, n.value)) if state else None
         self._ok(self.L.tpx_adjoint_gradient(self.h, _dp(g), _dp(gs) if state else None), "adjoint_gradient")
         return (g, gs) if state else g
+
+    def adjoint_gradient_full(self) -> dict:
+        """Every gradient of the recorded run: "tau" (np), "state" (6, np)
+        in particle order, "A" (the forcing coefficient, a float), and the
+        initial gas "RU" (3, nz, ny, nx), "Rho" and "P" (nz, ny, nx).  Leaves
+        the gas where the recorded run ended."""
+        n = ctypes.c_int64()
+        self._ok(self.L.tpx_num_particles(self.h, ctypes.byref(n)), "num_particles")
+        out = dict(tau=np.zeros(n.value), state=np.zeros((6, n.value)), A=np.zeros(1),
+                   RU=np.zeros((3,) + self.shape), Rho=np.zeros(self.shape), P=np.zeros(self.shape))
+        self._ok(self.L.tpx_adjoint_gradient_full(self.h, _dp(out["tau"]), _dp(out["state"]), _dp(out["A"]),
+                                                  _dp(out["RU"]), _dp(out["Rho"]), _dp(out["P"])),
+                 "adjoint_gradient_full")
+        out["A"] = float(out["A"][0])
+        return out
 
     def adjoint_release(self) -> None:
         self._ok(self.L.tpx_adjoint_release(self.h), "adjoint_release")
