@@ -177,6 +177,11 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J);
 int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate);
 int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, double *dJ_dA, double *dJ_dRU,
                               double *dJ_dRho, double *dJ_dP);
+/* How tpx_adjoint_record keeps the gas for the gas sweep: 0 (default) keeps
+ * every substep's momentum, density and Poisson iterates when they fit in
+ * half the free device memory, else recomputes each step from its start;
+ * 1 always keeps them; 2 always recomputes.  Applies to the next recording. */
+int tpx_adjoint_set_gas_tape(tpx_ctx *ctx, int mode);
 int tpx_adjoint_release(tpx_ctx *ctx);
 
 typedef struct tpx_batch tpx_batch;

@@ -164,6 +164,34 @@ def test_full_gradient_keeps_the_particle_gradient_and_the_run(gpu_available):
         assert np.array_equal(pa[q], pb[q]), q
 
 
+@pytest.mark.parametrize("iterations", [6, 2])
+def test_kept_and_recomputed_tapes_agree(gpu_available, iterations):
+    """The sweep from a tape that keeps every substep and the sweep that
+    recomputes each step from its start give the same gradients (to the
+    rounding of the particles' atomic deposits), and both leave the run as
+    it ended."""
+    case = _variable_density_case(iterations)
+    out = {}
+    for mode in ("keep", "recompute"):
+        s = solver_from_case(case)
+        s.adjoint_set_gas_tape(mode)
+        J = s.adjoint_record(NSTEPS, EVERY)
+        out[mode] = (J, s.adjoint_gradient_full())
+        s.adjoint_release()
+        s.step(1)
+        out[mode] += (s.get("RU"), s.get("P"))
+        s.close()
+    (Ja, ga, rua, pa), (Jb, gb, rub, pb) = out["keep"], out["recompute"]
+    assert Ja == Jb
+    assert np.array_equal(rua, rub) and np.array_equal(pa, pb)
+    assert np.array_equal(ga["tau"], gb["tau"])
+    assert abs(ga["A"] - gb["A"]) <= 1e-10 * abs(ga["A"])
+    for q in ("RU", "Rho", "P"):
+        scale = float(np.max(np.abs(ga[q])))
+        assert scale > 0.0, q
+        np.testing.assert_allclose(ga[q], gb[q], rtol=0, atol=1e-10 * scale, err_msg=q)
+
+
 def test_torch_shim_flow_backward(gpu_available):
     torch = pytest.importorskip("torch")
     from tpx.adjoint import var_c_flow

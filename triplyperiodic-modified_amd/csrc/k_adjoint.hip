@@ -6,10 +6,11 @@
 // depend on the particles, so dJ/dtau_p is the particle path's adjoint alone:
 // the forward run tapes the gas stage fields every particle advance reads
 // (RU_int, Rho_int of each substep) and checkpoints the particles' base state;
-// the reverse sweep restores a checkpoint, recomputes the steps up to the one
-// being reversed with the same arithmetic as the advance kernel (P1-P3, P7 of
-// particle.cpp:314-399, 58-104), keeps its four stage states, and runs the
-// transposed RK4 stage recursion
+// the reverse sweep recomputes each checkpoint segment once from its
+// checkpoint, keeping every step's start, with the same arithmetic as the
+// advance kernel (P1-P3, P7 of particle.cpp:314-399, 58-104), then for each
+// step backwards recomputes its four stage states and runs the transposed
+// RK4 stage recursion
 //   lam_k(s) = dt post_s lam_end + dt pre_s lam_Y(s+1),  lam_Y(s) = J_f(Y_s)^T lam_k(s),
 //   lam_y0   = lam_end + sum_s lam_Y(s)
 // with f(x, u) = (u, (u_g(x) - u)/tau + g): J_f^T (a, b) = (b du_g/dx / tau, a - b/tau)
@@ -77,7 +78,8 @@ __device__ __forceinline__ void gather(const Geo &g, const Ix &ix, Gas gs, const
 struct StepArgs {
   Geo g;
   const double *gas;   // 4 substeps x 4 fields x n
-  double *y[6];        // base x, y, z, u, v, w: read, then overwritten by the step's result
+  const double *yin[6];  // base x, y, z, u, v, w at the step's start
+  double *y[6];          // the step's result (may be yin)
   double *stage;       // non-null: the 4 stage states, 24 arrays of np
   const double *tau;   // per particle, or null
   double tp;           // scalar tau when tau is null
@@ -98,7 +100,7 @@ struct k_adj_step_fwd {
     Ix ix = make_ix(g);
     double b[6], vi[6], v1[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) b[q] = vi[q] = v1[q] = a.y[q][p];
+    for (int q = 0; q < 6; ++q) b[q] = vi[q] = v1[q] = a.yin[q][p];
     const double itp = a.tau ? 1. / a.tau[p] : 1. / a.tp;
     for (int rk = 0; rk < 4; ++rk) {
       if (a.stage)
@@ -279,13 +281,16 @@ void hip_check_adj(hipError_t e) {
   if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
 }
 
-StepArgs step_args(tpx_ctx *c, int step, double *const y[6], double *stage) {
+StepArgs step_args(tpx_ctx *c, int step, const double *const yin[6], double *const y[6], double *stage) {
   Particles &pt = c->pt;
   AdjointTape &t = c->adj;
   StepArgs a{};
   a.g = c->g;
   a.gas = t.gas + (long)step * 16 * c->g.n;
-  for (int q = 0; q < 6; ++q) a.y[q] = y[q];
+  for (int q = 0; q < 6; ++q) {
+    a.yin[q] = yin[q];
+    a.y[q] = y[q];
+  }
   a.stage = stage;
   a.tau = pt.has_tau ? pt.tau : nullptr;
   a.tp = c->Tp;
@@ -354,15 +359,32 @@ void adjoint_reverse(tpx_ctx *c, const std::function<void(int)> &gas_step) {
                          (const double *)t.djdc, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5], np);
   TPX_LAUNCH_CHECK();
   launch_fill(c, t.dtau, 0.0, np);
+  // the start of every step of the checkpoint segment being reversed,
+  // recomputed once per segment from its checkpoint
+  const int seg_len = std::min(t.every, t.nsteps);
+  if (!t.seg && hipMalloc((void **)&t.seg, sizeof(double) * 6 * (size_t)np * seg_len) != hipSuccess)
+    throw std::bad_alloc();
+  const double *in[6];
+  double *out[6];
+  auto start_of = [&](int j, int q) { return t.seg + ((long)j * 6 + q) * np; };
+  int seg_k = -1;
   for (int s = t.nsteps - 1; s >= 0; --s) {
-    // the base state of step s: its checkpoint, then the steps after it
-    const int k0 = s / t.every;
-    for (int q = 0; q < 6; ++q) launch_copy(c, y[q], t.ckpt + ((long)k0 * 6 + q) * np, np);
-    for (int r = k0 * t.every; r < s; ++r) {
-      tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, r, y, nullptr));
-      TPX_LAUNCH_CHECK();
+    const int k0 = s / t.every, base = k0 * t.every;
+    if (k0 != seg_k) {
+      for (int q = 0; q < 6; ++q) launch_copy(c, start_of(0, q), t.ckpt + ((long)k0 * 6 + q) * np, np);
+      for (int r = base; r < s; ++r) {
+        for (int q = 0; q < 6; ++q) {
+          in[q] = start_of(r - base, q);
+          out[q] = start_of(r - base + 1, q);
+        }
+        tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, r, in, out, nullptr));
+        TPX_LAUNCH_CHECK();
+      }
+      seg_k = k0;
     }
-    tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, s, y, t.stage));
+    // the stage states of step s
+    for (int q = 0; q < 6; ++q) in[q] = start_of(s - base, q);
+    tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, s, in, y, t.stage));
     TPX_LAUNCH_CHECK();
     AdjArgs a{};
     a.g = c->g;

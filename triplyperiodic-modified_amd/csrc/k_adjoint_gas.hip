@@ -57,7 +57,8 @@ __device__ __forceinline__ double inv_d(const Geo &g, int a) { return a == 0 ? g
 __device__ __forceinline__ double inv_d2(const Geo &g, int a) { return a == 0 ? g.idx2 : (a == 1 ? g.idy2 : g.idz2); }
 
 // 1/x as the forward Poisson iteration forms it (reciprocal estimate and two
-// Newton steps), so that the spectral solve below sees the residual it is given
+// Newton steps): C and cc here are the iteration's own, so that the spectral
+// solve below sees the residual it is given
 __device__ __forceinline__ double rcp_nr(double x) {
   double r = __builtin_amdgcn_rcp(x);
   r = fma(r, fma(-x, r, 1.0), r);
@@ -65,23 +66,8 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return r;
 }
 
-// C = 1/Rho_face on face a at index p (between p - e_a and p)
-__device__ __forceinline__ double cface(const double *__restrict__ rho, long p, long q) {
-  return rcp_nr((rho[p] + rho[q]) * 0.5);
-}
-
-// cc = -1/I_F2C(C) of the cell at offset (di, dj, dk)
-__device__ __forceinline__ double cc_at(const Cell &x, const double *__restrict__ rho, int di, int dj, int dk) {
-  const long c = x.off(di, dj, dk);
-  const double cx0 = cface(rho, c, x.off(di - 1, dj, dk)), cx1 = cface(rho, x.off(di + 1, dj, dk), c);
-  const double cy0 = cface(rho, c, x.off(di, dj - 1, dk)), cy1 = cface(rho, x.off(di, dj + 1, dk), c);
-  const double cz0 = cface(rho, c, x.off(di, dj, dk - 1)), cz1 = cface(rho, x.off(di, dj, dk + 1), c);
-  const double third = 1. / 3.;
-  return -rcp_nr(third * ((cx0 + cx1) * 0.5 + (cy0 + cy1) * 0.5 + (cz0 + cz1) * 0.5));
-}
-
-// div(C grad f) at the cell
-__device__ __forceinline__ double div_c_grad(const Geo &g, const Cell &x, const double *__restrict__ rho,
+// div(C grad f) at the cell, C on the faces at the cell (Cf) and above it
+__device__ __forceinline__ double div_c_grad(const Geo &g, const Cell &x, const double *const Cf[3],
                                              const double *__restrict__ f, long c) {
   const double fc = f[c];
   double s = 0.0;
@@ -89,35 +75,73 @@ __device__ __forceinline__ double div_c_grad(const Geo &g, const Cell &x, const 
   for (int a = 0; a < 3; ++a) {
     const long m = x.at(a, -1), p = x.at(a, 1);
     const double id = inv_d(g, a);
-    const double c0 = cface(rho, c, m), c1 = cface(rho, p, c);
+    const double c0 = Cf[a][c], c1 = Cf[a][p];
     s += (c1 * ((f[p] - fc) * id) - c0 * ((fc - f[m]) * id)) * id;
   }
   return s;
+}
+
+// block sum of v into partials[blockIdx.x]
+__device__ __forceinline__ void block_sum(double v, double *partials) {
+  __shared__ double sh[kBlock];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// the spectral solve of lam_P: a forward iteration from P = 0 with the
+// right-hand side -lam_P/cc has residual lam_P
+__device__ __forceinline__ void spectral_input(long c, double lp, double cc, double *tmp_rhs, double *tmp_p,
+                                               double *scal) {
+  if (c == 0) {
+    scal[SC_SUM_RHS] = 0.0;
+    scal[SC_ITERS] = 0.0;
+    scal[SC_DONE] = 0.0;
+  }
+  tmp_rhs[c] = -lp / cc;
+  tmp_p[c] = 0.0;
 }
 
 // ---- projection (G10) ------------------------------------------------------
 struct GradPArgs {
   Geo g;
   const double *np1[3], *nw[3];
+  const double *rho;      // Rho_new
   double *lp;
-  double c_np1, c_new;  // dt post, and dt pre (dt post at the last substep)
+  double *Cf[3], *cc;     // out: C on the faces at each cell, and cc, for the Poisson kernels
+  double *tmp_rhs, *tmp_p, *scal;
+  double c_np1, c_new;    // dt post, and dt pre (dt post at the last substep)
 };
 
+// lam_P += div(dt post lam_np1 + dt pre lam_new); also C and cc of the
+// substep's Poisson coefficient, and the first spectral solve's input
 struct k_adj_grad_p {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const GradPArgs &a) {
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (c >= a.g.n) return;
     const Cell x(a.g, c);
-    double s = 0.0;
+    double s = 0.0, csum = 0.0;
+    const double rc = a.rho[c];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const long p = x.at(d, 1);
+      const long p = x.at(d, 1), m = x.at(d, -1);
       const double q0 = a.c_np1 * a.np1[d][c] + a.c_new * a.nw[d][c];
       const double q1 = a.c_np1 * a.np1[d][p] + a.c_new * a.nw[d][p];
       s += (q1 - q0) * inv_d(a.g, d);
+      const double c0 = rcp_nr((rc + a.rho[m]) * 0.5), c1 = rcp_nr((a.rho[p] + rc) * 0.5);
+      a.Cf[d][c] = c0;
+      csum += (c0 + c1) * 0.5;
     }
-    a.lp[c] += s;
+    const double cc = -rcp_nr((1. / 3.) * csum);
+    a.cc[c] = cc;
+    const double lp = a.lp[c] + s;
+    a.lp[c] = lp;
+    spectral_input(c, lp, cc, a.tmp_rhs, a.tmp_p, a.scal);
   }
 };
 
@@ -125,6 +149,7 @@ struct k_adj_grad_p {
 struct PoisArgs {
   Geo g;
   const double *rho;       // Rho_new
+  const double *Cf[3], *cc;
   const double *Pi;        // P before the iteration being reversed
   const double *rhs;       // raw RHS_Pois
   const double *rhs_sum;   // its sum (one value)
@@ -133,25 +158,9 @@ struct PoisArgs {
   double *mu;
   double *lrhs, *lcc, *lC[3], *lF[3];
   double *scal;            // slots of the spectral solve
+  double *partials;        // block sums of lam_RHS after the last reversed iteration
   int first;               // the first reversed iteration writes the accumulators
-};
-
-// the spectral solve of lam_P: a forward iteration from P = 0 with the
-// right-hand side -lam_P/cc has residual lam_P
-struct k_adj_pois_prep {
-  static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(const PoisArgs &a) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
-    if (c == 0) {
-      a.scal[SC_SUM_RHS] = 0.0;
-      a.scal[SC_ITERS] = 0.0;
-      a.scal[SC_DONE] = 0.0;
-    }
-    if (c >= a.g.n) return;
-    const Cell x(a.g, c);
-    a.tmp_rhs[c] = -a.lp[c] / cc_at(x, a.rho, 0, 0, 0);
-    a.tmp_p[c] = 0.0;
-  }
+  int last;                // the last one (the solve's first iteration)
 };
 
 // mu = cc nu, lam_RHS -= mu, lam_cc += nu (div(C grad P_i) - RHS_P)
@@ -159,33 +168,41 @@ struct k_adj_pois_a {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const PoisArgs &a) {
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
-    if (c >= a.g.n) return;
-    const Cell x(a.g, c);
-    const double nu = a.tmp_p[c];
-    const double mu = cc_at(x, a.rho, 0, 0, 0) * nu;
-    a.mu[c] = mu;
-    const double mean = a.rhs_sum[0] / (double)a.g.n;
-    const double rr = div_c_grad(a.g, x, a.rho, a.Pi, c) - (a.rhs[c] - mean);
-    a.lrhs[c] = (a.first ? 0.0 : a.lrhs[c]) - mu;
-    a.lcc[c] = (a.first ? 0.0 : a.lcc[c]) + nu * rr;
+    double lr = 0.0;
+    if (c < a.g.n) {
+      const Cell x(a.g, c);
+      const double nu = a.tmp_p[c];
+      const double mu = a.cc[c] * nu;
+      a.mu[c] = mu;
+      const double mean = a.rhs_sum[0] / (double)a.g.n;
+      const double rr = div_c_grad(a.g, x, a.Cf, a.Pi, c) - (a.rhs[c] - mean);
+      lr = (a.first ? 0.0 : a.lrhs[c]) - mu;
+      a.lrhs[c] = lr;
+      a.lcc[c] = (a.first ? 0.0 : a.lcc[c]) + nu * rr;
+    }
+    if (a.last) block_sum(lr, a.partials);
   }
 };
 
-// lam_P += div(C grad mu), lam_C -= grad mu . grad P_i
+// lam_P += div(C grad mu), lam_C -= grad mu . grad P_i; then the next
+// reversed iteration's spectral solve input
 struct k_adj_pois_b {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const PoisArgs &a) {
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (c >= a.g.n) return;
     const Cell x(a.g, c);
-    a.lp[c] += div_c_grad(a.g, x, a.rho, a.mu, c);
+    const double lp = a.lp[c] + div_c_grad(a.g, x, a.Cf, a.mu, c);
+    a.lp[c] = lp;
+    const double muc = a.mu[c], pc = a.Pi[c];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const long m = x.at(d, -1);
       const double id = inv_d(a.g, d);
-      const double v = -((a.mu[c] - a.mu[m]) * id) * ((a.Pi[c] - a.Pi[m]) * id);
+      const double v = -((muc - a.mu[m]) * id) * ((pc - a.Pi[m]) * id);
       a.lC[d][c] = (a.first ? 0.0 : a.lC[d][c]) + v;
     }
+    if (!a.last) spectral_input(c, lp, a.cc[c], a.tmp_rhs, a.tmp_p, a.scal);
   }
 };
 
@@ -196,14 +213,14 @@ struct k_adj_pois_c {
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (c >= a.g.n) return;
     const Cell x(a.g, c);
-    const double cc0 = cc_at(x, a.rho, 0, 0, 0);
+    const double cc0 = a.cc[c];
     const double w0 = cc0 * cc0 * a.lcc[c];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const long m = x.at(d, -1);
-      const double ccm = cc_at(x, a.rho, -(d == 0), -(d == 1), -(d == 2));
+      const double ccm = a.cc[m];
       const double lc = a.lC[d][c] + (w0 + ccm * ccm * a.lcc[m]) * (1. / 6.);
-      const double C = 1.0 / ((a.rho[c] + a.rho[m]) * 0.5);
+      const double C = a.Cf[d][c];
       a.lF[d][c] = -lc * C * C;
     }
   }
@@ -245,23 +262,30 @@ struct k_adj_rhs_pois {
 };
 
 // lam_rho += I_F2C-transpose of a face adjoint: each face feeds its two cells
+__device__ __forceinline__ double fold_at(const Cell &x, long c, const double *const lF[3]) {
+  return 0.5 * ((lF[0][c] + lF[0][x.at(0, 1)]) + (lF[1][c] + lF[1][x.at(1, 1)]) + (lF[2][c] + lF[2][x.at(2, 1)]));
+}
+
 struct k_adj_fold {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const Geo &g, const double *lF0, const double *lF1, const double *lF2,
-                                             double *out, int accumulate) {
+                                             double *out) {
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (c >= g.n) return;
     const Cell x(g, c);
-    const double v = 0.5 * ((lF0[c] + lF0[x.at(0, 1)]) + (lF1[c] + lF1[x.at(1, 1)]) + (lF2[c] + lF2[x.at(2, 1)]));
-    out[c] = (accumulate ? out[c] : 0.0) + v;
+    const double *lF[3] = {lF0, lF1, lF2};
+    out[c] = fold_at(x, c, lF);
   }
 };
 
 // ---- RK4 bookkeeping of the momentum and density updates (G4, G1) ---------
 struct RkArgs {
-  long n;
+  Geo g;
   double *lnp1[3], *lnew[3], *lru[3], *lrhs[3];
   double *lr, *lrnp1, *lrnew, *ld;
+  const double *lF[3];       // the Poisson and right-hand side's face adjoint, folded into lam_Rho_new first
+  const double *rui[3], *rho_int;
+  double *U[3];              // out: U = RU_int / I_C2F(Rho_int) for the momentum kernels
   double c_post, c_pre;
   int last;  // the fourth substep: new = np1
 };
@@ -270,7 +294,9 @@ struct k_adj_rk {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const RkArgs &a) {
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
-    if (c >= a.n) return;
+    if (c >= a.g.n) return;
+    const Cell x(a.g, c);
+    const double rc = a.rho_int[c];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double ln = a.lnew[d][c];
@@ -282,8 +308,9 @@ struct k_adj_rk {
         a.lrhs[d][c] = a.c_post * a.lnp1[d][c] + a.c_pre * ln;
         a.lru[d][c] += ln;
       }
+      a.U[d][c] = a.rui[d][c] / ((rc + a.rho_int[x.at(d, -1)]) * 0.5);
     }
-    const double lrn = a.lrnew[c];
+    const double lrn = a.lrnew[c] + fold_at(x, c, a.lF);
     if (a.last) {
       const double l1 = a.lrnp1[c] + lrn;
       a.lrnp1[c] = l1;
@@ -301,6 +328,7 @@ struct MomArgs {
   const double *lrhs[3];
   const double *rui[3];
   const double *rho;        // Rho_int
+  const double *U[3];       // RU_int / I_C2F(Rho_int)
   const double *lup[3];     // the particles' gather adjoint on the faces, or null
   const double *ld;         // adjoint of div RU_int (G1)
   double *lu[3];            // adjoint of U
@@ -359,45 +387,34 @@ struct k_adj_mom_u {
 struct k_adj_mom_r {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const MomArgs &a) {
-    __shared__ double sh[kBlock];
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     double part = 0.0;
     if (c < a.g.n) {
       const Cell x(a.g, c);
       const Geo &g = a.g;
-      auto U = [&](int d, long p, long q) { return a.rui[d][p] / ((a.rho[p] + a.rho[q]) * 0.5); };
 #pragma unroll
       for (int b = 0; b < 3; ++b) {  // component b of RU_int, face b at c
         const double idb = inv_d(g, b);
         const long mb = x.at(b, -1);
-        const double Fb = (a.rho[c] + a.rho[mb]) * 0.5;
-        const double Ub = a.rui[b][c] / Fb;
-        const double lrb = a.lrhs[b][c];
-        double v = a.lu[b][c] / Fb + a.A * lrb - (a.ld[c] - a.ld[mb]) * idb;
+        const double iF = 1.0 / ((a.rho[c] + a.rho[mb]) * 0.5);
+        const double lrb = a.lrhs[b][c], lub = a.lu[b][c];
+        double v = lub * iF + a.A * lrb - (a.ld[c] - a.ld[mb]) * idb;
         // the RU_b factor of every component d's convective flux along b
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const double *L = a.lrhs[d];
+          const double *L = a.lrhs[d], *Ud = a.U[d];
           const long pd = x.at(d, 1), pdmb = x.at(d, 1, b, -1);
           const double w0 = (L[c] - L[mb]) * idb;
           const double w1 = (L[pd] - L[pdmb]) * idb;
           // U_d on its faces at c, c - e_b, c + e_d, c + e_d - e_b
-          const double u00 = U(d, c, x.at(d, -1)), u0m = U(d, mb, x.at(b, -1, d, -1));
-          const double u10 = U(d, pd, c), u1m = U(d, pdmb, mb);
-          v += 0.5 * (w0 * (u00 + u0m) * 0.5 + w1 * (u10 + u1m) * 0.5);
+          v += 0.5 * (w0 * (Ud[c] + Ud[mb]) * 0.5 + w1 * (Ud[pd] + Ud[pdmb]) * 0.5);
         }
         a.lint[b][c] = v;
-        a.lF[b][c] = a.gv[b] * lrb - a.lu[b][c] * Ub / Fb;
+        a.lF[b][c] = a.gv[b] * lrb - lub * a.U[b][c] * iF;
         part += lrb * a.rui[b][c];
       }
     }
-    sh[threadIdx.x] = part;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) a.partials[blockIdx.x] = sh[0];
+    block_sum(part, a.partials);
   }
 };
 
@@ -463,16 +480,22 @@ void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga) {
 void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k) {
   const long n = c->g.n;
   const unsigned nb = cells(c);
+  if (s.iters < 1) throw std::logic_error("adjoint: a Poisson solve without iterations");
   // G11: the int state after this substep is its new state
   for (int d = 0; d < 3; ++d) std::swap(ga.lnew[d], ga.lint[d]);
   std::swap(ga.lrnew, ga.lrint);
   const bool last = k.rk == 3;
-  // G10
+  // G10 (and the Poisson coefficient, and the first spectral solve's input)
   {
     GradPArgs a{};
     a.g = c->g;
-    for (int d = 0; d < 3; ++d) { a.np1[d] = ga.lnp1[d]; a.nw[d] = ga.lnew[d]; }
+    for (int d = 0; d < 3; ++d) { a.np1[d] = ga.lnp1[d]; a.nw[d] = ga.lnew[d]; a.Cf[d] = ga.Cf[d]; }
+    a.rho = s.rho_new;
     a.lp = ga.lp;
+    a.cc = ga.cc;
+    a.tmp_rhs = ga.tmp_rhs;
+    a.tmp_p = ga.tmp_p;
+    a.scal = ga.scal;
     a.c_np1 = k.dt * k.post;
     a.c_new = last ? k.dt * k.post : k.dt * k.pre;
     tpx_launch<k_adj_grad_p>(nb, kBlock, 0, c->stream, a);
@@ -482,6 +505,8 @@ void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const 
   PoisArgs pa{};
   pa.g = c->g;
   pa.rho = s.rho_new;
+  for (int d = 0; d < 3; ++d) { pa.Cf[d] = ga.Cf[d]; pa.lC[d] = ga.lC[d]; pa.lF[d] = ga.lF[d]; }
+  pa.cc = ga.cc;
   pa.rhs = s.rhs;
   pa.rhs_sum = s.rhs_sum;
   pa.lp = ga.lp;
@@ -490,24 +515,22 @@ void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const 
   pa.mu = ga.mu;
   pa.lrhs = ga.lrhs_p;
   pa.lcc = ga.lcc;
-  for (int d = 0; d < 3; ++d) { pa.lC[d] = ga.lC[d]; pa.lF[d] = ga.lF[d]; }
   pa.scal = ga.scal;
-  if (s.iters < 1) throw std::logic_error("adjoint: a Poisson solve without iterations");
+  pa.partials = c->partials;
   for (int i = s.iters - 1; i >= 0; --i) {
     pa.Pi = s.P[i];
     pa.first = i == s.iters - 1;
-    tpx_launch<k_adj_pois_prep>(nb, kBlock, 0, c->stream, pa);
-    TPX_LAUNCH_CHECK();
+    pa.last = i == 0;
     launch_poisson_iteration_on(c, s.rho_new, ga.tmp_p, ga.tmp_rhs, ga.scal);
     tpx_launch<k_adj_pois_a>(nb, kBlock, 0, c->stream, pa);
     TPX_LAUNCH_CHECK();
     tpx_launch<k_adj_pois_b>(nb, kBlock, 0, c->stream, pa);
     TPX_LAUNCH_CHECK();
   }
+  launch_finish_sum(c, (int)nb, 0, SC_ADJ_SUM);  // sum of lam_RHS_Pois from the last k_adj_pois_a
   tpx_launch<k_adj_pois_c>(nb, kBlock, 0, c->stream, pa);
   TPX_LAUNCH_CHECK();
   // G6 and G5
-  launch_reduce(c, ga.lrhs_p, 0, SC_ADJ_SUM);
   {
     RhsPArgs a{};
     a.g = c->g;
@@ -521,17 +544,17 @@ void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const 
     tpx_launch<k_adj_rhs_pois>(nb, kBlock, 0, c->stream, a);
     TPX_LAUNCH_CHECK();
   }
-  tpx_launch<k_adj_fold>(nb, kBlock, 0, c->stream, c->g, (const double *)ga.lF[0], (const double *)ga.lF[1],
-                         (const double *)ga.lF[2], ga.lrnew, 1);
-  TPX_LAUNCH_CHECK();
-  // G4 and G1: the RK4 updates, then the right-hand sides
+  // G4 and G1: lam_Rho_new complete (fold of the face adjoint), the RK4
+  // updates, then the right-hand sides
   {
     RkArgs a{};
-    a.n = n;
+    a.g = c->g;
     for (int d = 0; d < 3; ++d) {
       a.lnp1[d] = ga.lnp1[d]; a.lnew[d] = ga.lnew[d]; a.lru[d] = ga.lru[d]; a.lrhs[d] = ga.lrhs[d];
+      a.lF[d] = ga.lF[d]; a.rui[d] = s.ru_int[d]; a.U[d] = ga.U[d];
     }
     a.lr = ga.lr; a.lrnp1 = ga.lrnp1; a.lrnew = ga.lrnew; a.ld = ga.ld;
+    a.rho_int = s.rho_int;
     a.c_post = k.dt * k.post;
     a.c_pre = k.dt * k.pre;
     a.last = last;
@@ -543,6 +566,7 @@ void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const 
   for (int d = 0; d < 3; ++d) {
     m.lrhs[d] = ga.lrhs[d];
     m.rui[d] = s.ru_int[d];
+    m.U[d] = ga.U[d];
     m.lup[d] = s.lup ? s.lup + (long)d * n : nullptr;
     m.lu[d] = ga.lu[d];
     m.lint[d] = ga.lint[d];
@@ -562,7 +586,7 @@ void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const 
   tpx_launch<k_adj_acc>(1, 1024, 0, c->stream, (const double *)c->partials, (long)nb, c->scal + SC_ADJ_A);
   TPX_LAUNCH_CHECK();
   tpx_launch<k_adj_fold>(nb, kBlock, 0, c->stream, c->g, (const double *)ga.lF[0], (const double *)ga.lF[1],
-                         (const double *)ga.lF[2], ga.lrint, 0);
+                         (const double *)ga.lF[2], ga.lrint);
   TPX_LAUNCH_CHECK();
 }
 

@@ -225,13 +225,16 @@ void do_update_ru_wop(tpx_ctx *c, int rk, bool with_rho) {
 // themselves, so the host queues iterations in batches and waits once per
 // batch instead of once per iteration: first as many as the last solve of
 // this substep took, then two at a time.
-void do_solve_poisson(tpx_ctx *c, int rk, int *iters) {
+void do_solve_poisson(tpx_ctx *c, int rk, int *iters, const std::function<void(int)> &before_iteration = {}) {
   launch_solve_begin(c);
   int queued = 0, batch = std::max(1, c->iter_guess[rk & 3]);
   double st[2] = {0.0, 1.0};  // SC_ITERS, SC_DONE
   while (queued < c->Iteration) {
     const int nb = std::min(batch, c->Iteration - queued);
-    for (int b = 0; b < nb; ++b) launch_poisson_iteration(c, c->rho[c->rho_new]);
+    for (int b = 0; b < nb; ++b) {
+      if (before_iteration) before_iteration(queued + b);
+      launch_poisson_iteration(c, c->rho[c->rho_new]);
+    }
     queued += nb;
     hip_check(hipMemcpyAsync(c->h_scal + SC_ITERS, c->scal + SC_ITERS, 2 * sizeof(double), hipMemcpyDeviceToHost,
                              c->stream),
@@ -367,8 +370,9 @@ void step_end(tpx_ctx *c) {
 // frees every array of a recorded adjoint run
 void adjoint_free(AdjointTape &t) {
   auto f = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
-  f(t.gas); f(t.ckpt); f(t.y); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
-  f(t.start); f(t.final_state); f(t.work); f(t.cap); f(t.lup);
+  f(t.gas); f(t.ckpt); f(t.y); f(t.seg); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
+  f(t.start); f(t.final_state); f(t.work); f(t.cap); f(t.lup); f(t.iters_seen); f(t.sub);
+  t.store = false;
   for (auto &p : t.pcap) f(p);
   t.pcap.clear();
   t.ga = GasAdjoint{};
@@ -1071,10 +1075,12 @@ void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
     double *sum = t.cap + 20 * n + rk;
     launch_copy(c, sum, c->scal + SC_SUM_RHS, 1);
     g.rhs_sum = sum;
-    // the solve one iteration at a time, P kept before each
+    // the recorded number of iterations, P kept before each; the count the
+    // device's loop test reaches is kept for a check after the sweep
     launch_solve_begin(c);
     g.P.clear();
-    for (int it = 0; it < c->Iteration; ++it) {
+    g.iters = t.iters[(size_t)s * 4 + rk];
+    for (int it = 0; it < g.iters; ++it) {
       if (used == t.pcap.size()) {
         double *p = nullptr;
         if (hipMalloc((void **)&p, sizeof(double) * n) != hipSuccess) throw std::bad_alloc();
@@ -1083,14 +1089,8 @@ void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
       launch_copy(c, t.pcap[used], c->P, n);
       g.P.push_back(t.pcap[used++]);
       launch_poisson_iteration(c, c->rho[c->rho_new]);
-      hip_check(hipMemcpyAsync(c->h_scal + SC_ITERS, c->scal + SC_ITERS, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                               c->stream),
-                "hipMemcpyAsync");
-      sync(c);
-      if (c->h_scal[SC_DONE] != 0.0) break;
     }
-    g.iters = (int)c->h_scal[SC_ITERS];
-    if (g.iters != (int)g.P.size()) throw std::logic_error("adjoint: recomputed Poisson solve out of step");
+    launch_copy(c, t.iters_seen + (long)s * 4 + rk, c->scal + SC_ITERS, 1);
     do_update_ru_wp(c, rk, rk == 3);
     c->rho_int = c->rho_new;  // the gas half of TimeAdvance_RK4
     c->ru_int = c->ru_new;
@@ -1107,6 +1107,45 @@ void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
   launch_scalar_copy(c, SC_P0, SC_P0_NP1);
 }
 
+// One pool buffer of n values (grown on demand); null when out of memory.
+double *pcap_slot(AdjointTape &t, size_t i, long n) {
+  while (t.pcap.size() <= i) {
+    double *p = nullptr;
+    if (hipMalloc((void **)&p, sizeof(double) * n) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    t.pcap.push_back(p);
+  }
+  return t.pcap[i];
+}
+
+// Step s of the recorded run as the gas sweep reads it, from the tape kept
+// while recording.
+void taped_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
+  AdjointTape &t = c->adj;
+  const long n = c->g.n;
+  c->A = t.A[s];
+  c->Iteration = t.iteration[s];
+  for (int d = 0; d < 3; ++d) c->gv[d] = t.gv[(size_t)s * 3 + d];
+  for (int rk = 0; rk < 4; ++rk) {
+    GasSubstep &g = sub[rk];
+    const size_t q = (size_t)s * 4 + rk;
+    const double *gs = t.gas + (long)q * 4 * n, *sb = t.sub + (long)q * 5 * n;
+    for (int d = 0; d < 3; ++d) {
+      g.ru_int[d] = gs + d * n;
+      g.ru_new[d] = sb + d * n;
+    }
+    g.rho_int = gs + 3 * n;
+    g.rho_new = sb + 3 * n;
+    g.rhs = sb + 4 * n;
+    g.rhs_sum = t.sub + (long)t.nsteps * 20 * n + q;
+    g.iters = t.iters[q];
+    g.P.clear();
+    for (int i = 0; i < g.iters; ++i) g.P.push_back(t.pcap[t.p_first[q] + i]);
+  }
+}
+
 // The reverse sweep of the whole recorded run, particles and gas together:
 // after the particles' step s has deposited dJ/du_g of its four substeps,
 // the gas step s is recomputed and reversed.  The solver's gas ends where the
@@ -1118,15 +1157,18 @@ void gas_adjoint_run(tpx_ctx *c) {
     if (!*p && hipMalloc((void **)p, sizeof(double) * count) != hipSuccess) throw std::bad_alloc();
   };
   alloc(&t.work, (size_t)GasAdjoint::kFields * n + SC_COUNT);
-  alloc(&t.cap, 20 * (size_t)n + 8);
   alloc(&t.lup, 12 * (size_t)n);
+  if (!t.store) {
+    alloc(&t.cap, 20 * (size_t)n + 8);
+    alloc(&t.iters_seen, 4 * (size_t)t.nsteps);
+  }
   GasAdjoint &ga = t.ga;
   double *w = t.work;
   auto take = [&]() { double *p = w; w += n; return p; };
-  for (auto *grp : {ga.lru, ga.lnp1, ga.lint, ga.lnew, ga.lrhs, ga.lu, ga.lC, ga.lF})
+  for (auto *grp : {ga.lru, ga.lnp1, ga.lint, ga.lnew, ga.lrhs, ga.lu, ga.lC, ga.lF, ga.Cf, ga.U})
     for (int d = 0; d < 3; ++d) grp[d] = take();
   for (double **p : {&ga.lr, &ga.lrnp1, &ga.lrint, &ga.lrnew, &ga.lp, &ga.lrhs_p, &ga.lcc, &ga.ld, &ga.mu,
-                     &ga.tmp_rhs, &ga.tmp_p})
+                     &ga.tmp_rhs, &ga.tmp_p, &ga.cc})
     *p = take();
   ga.scal = w;
   launch_fill(c, t.work, 0.0, (long)GasAdjoint::kFields * n + SC_COUNT);
@@ -1134,8 +1176,10 @@ void gas_adjoint_run(tpx_ctx *c) {
   const double A0 = c->A, g0[3] = {c->gv[0], c->gv[1], c->gv[2]};
   const int it0 = c->Iteration;
   GasSubstep sub[4];
+  const bool stored = t.store;
   auto step = [&](int s) {
-    gas_recompute_step(c, s, sub);
+    if (stored) taped_step(c, s, sub);
+    else gas_recompute_step(c, s, sub);
     if (s < t.nsteps - 1) adjoint_gas_boundary(c, ga);
     for (int rk = 3; rk >= 0; --rk) {
       sub[rk].lup = t.lup + (long)rk * 3 * n;
@@ -1152,7 +1196,15 @@ void gas_adjoint_run(tpx_ctx *c) {
   }
   c->A = A0; c->Iteration = it0;
   for (int d = 0; d < 3; ++d) c->gv[d] = g0[d];
+  if (stored) return;
   gas_state_copy(c, t.final_state, false);
+  // every recomputed solve stopped where the recorded one did
+  std::vector<double> seen((size_t)t.nsteps * 4);
+  hip_check(hipMemcpyAsync(seen.data(), t.iters_seen, sizeof(double) * seen.size(), hipMemcpyDeviceToHost, c->stream),
+            "download");
+  sync(c);
+  for (size_t q = 0; q < seen.size(); ++q)
+    if ((int)seen[q] != t.iters[q]) throw std::logic_error("adjoint: a recomputed Poisson solve differs from the recorded one");
 }
 
 // host copy of the sum of device arrays (n values each)
@@ -1297,10 +1349,30 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
     alloc(&t.djdc, (size_t)n);
     alloc(&t.start, (size_t)nsteps * 4 * n);
     alloc(&t.final_state, 8 * (size_t)n);
+    // keep every substep's momentum, density and Poisson data when the
+    // estimate (three Poisson iterations a solve) fits in half the free HBM
+    t.p_first.assign((size_t)nsteps * 4, 0);
+    t.pnext = 0;
+    t.store = false;
+    if (t.mode != 2) {
+      size_t free_b = 0, total_b = 0;
+      hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+      const double need = (double)nsteps * 4 * (5 + 3) * n * sizeof(double);
+      if (t.mode == 1 || need < 0.5 * (double)free_b) {
+        t.store = hipMalloc((void **)&t.sub, sizeof(double) * ((size_t)nsteps * 20 * n + (size_t)nsteps * 4)) ==
+                  hipSuccess;
+        if (!t.store) {
+          (void)hipGetLastError();
+          t.sub = nullptr;
+          if (t.mode == 1) throw std::bad_alloc();
+        }
+      }
+    }
     t.g3.assign((size_t)nsteps * 3, 0.0);
     t.A.assign((size_t)nsteps, 0.0);
     t.gv.assign((size_t)nsteps * 3, 0.0);
     t.iteration.assign((size_t)nsteps, 0);
+    t.iters.assign((size_t)nsteps * 4, 0);
     adjoint_checkpoint(c, 0);
     try {
       for (int s = 0; s < nsteps; ++s) {
@@ -1312,7 +1384,31 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
         t.A[s] = c->A;
         t.iteration[s] = c->Iteration;
         for (int d = 0; d < 3; ++d) t.gv[(size_t)s * 3 + d] = c->gv[d];
-        for (int rk = 0; rk < 4; ++rk) fused_substep(c, rk);
+        for (int rk = 0; rk < 4; ++rk) {  // fused_substep, keeping each solve's iteration count
+          fused_substep_pre(c, rk);
+          const size_t q = (size_t)s * 4 + rk;
+          if (t.store) {  // what the gas sweep reads, so that it need not recompute the step
+            double *dst = t.sub + (long)q * 5 * n;
+            for (int d = 0; d < 3; ++d) launch_copy(c, dst + d * n, c->ru[c->ru_new][d], n);
+            launch_copy(c, dst + 3 * n, c->rho[c->rho_new], n);
+            launch_copy(c, dst + 4 * n, c->rhs, n);
+            launch_copy(c, t.sub + (long)nsteps * 20 * n + q, c->scal + SC_SUM_RHS, 1);
+          }
+          const size_t p0 = t.pnext;
+          t.p_first[q] = p0;
+          auto keep_p = [&](int i) {
+            if (!t.store) return;
+            double *buf = pcap_slot(t, p0 + (size_t)i, n);
+            if (!buf) {  // out of memory: the sweep will recompute instead
+              t.store = false;
+              return;
+            }
+            launch_copy(c, buf, c->P, n);
+          };
+          do_solve_poisson(c, rk, &t.iters[q], keep_p);
+          t.pnext = p0 + (size_t)t.iters[q];
+          fused_substep_post(c, rk);
+        }
         step_end(c);
         t.step = -1;
         if ((s + 1) % every == 0 && s + 1 < nsteps) adjoint_checkpoint(c, (s + 1) / every);
@@ -1358,6 +1454,14 @@ int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, 
       for (int d = 0; d < 3; ++d) download_sum(ctx, {ga.lru[d], ga.lnp1[d], ga.lint[d]}, dJ_dRU + d * n);
     if (dJ_dRho) download_sum(ctx, {ga.lr, ga.lrnp1, ga.lrint}, dJ_dRho);
     if (dJ_dP) download_sum(ctx, {ga.lp}, dJ_dP);
+    return TPX_OK;
+  });
+}
+
+int tpx_adjoint_set_gas_tape(tpx_ctx *ctx, int mode) {
+  return guarded(ctx, [&]() {
+    if (mode < 0 || mode > 2) throw std::invalid_argument("gas tape mode must be 0 (by memory), 1 (keep) or 2 (recompute)");
+    ctx->adj.mode = mode;
     return TPX_OK;
   });
 }

@@ -88,12 +88,13 @@ struct Particles {
 // Work arrays of the gas adjoint sweep (n values each): adjoints of the
 // base, np1, int and new momentum and density, of P, of the Poisson
 // right-hand side, of the momentum right-hand side, of U and of the face
-// fields F (Rho_face) and C (1/Rho_face), plus the spectral solve's buffers.
+// fields F (Rho_face) and C (1/Rho_face); the substep's C, cc and U; the
+// spectral solve's buffers.
 struct GasAdjoint {
-  double *lru[3], *lnp1[3], *lint[3], *lnew[3], *lrhs[3], *lu[3], *lC[3], *lF[3];
-  double *lr, *lrnp1, *lrint, *lrnew, *lp, *lrhs_p, *lcc, *ld, *mu, *tmp_rhs, *tmp_p;
+  double *lru[3], *lnp1[3], *lint[3], *lnew[3], *lrhs[3], *lu[3], *lC[3], *lF[3], *Cf[3], *U[3];
+  double *lr, *lrnp1, *lrint, *lrnew, *lp, *lrhs_p, *lcc, *ld, *mu, *tmp_rhs, *tmp_p, *cc;
   double *scal;  // SC_COUNT slots of the spectral solve
-  static constexpr int kFields = 35;
+  static constexpr int kFields = 42;
 };
 
 // One recomputed substep as the gas adjoint reads it.
@@ -117,6 +118,7 @@ struct AdjointTape {
   std::vector<double> g3;        // particle gravity of each step (3 per step)
   double *ckpt = nullptr;        // (nsteps / every + 1) x 6 x np
   double *y = nullptr;           // 6 x np
+  double *seg = nullptr;         // min(every, nsteps) x 6 x np: step starts of the segment being reversed
   double *stage = nullptr;       // 4 x 6 x np
   double *lam = nullptr;         // 6 x np
   double *dtau = nullptr;        // np
@@ -125,11 +127,18 @@ struct AdjointTape {
   double *start = nullptr;       // nsteps x (RU 3, P) x n at each step's start
   std::vector<double> A, gv;     // forcing coefficient and gravity of each step (gv: 3 per step)
   std::vector<int> iteration;    // Poisson iteration cap of each step
+  std::vector<int> iters;        // iterations each substep's solve ran (4 per step)
+  double *iters_seen = nullptr;  // the recomputed solves' counts (4 per step, device)
   double *final_state = nullptr; // RU 3, RU_int 3, Rho, P of the recorded run's end
   double *work = nullptr;        // GasAdjoint::kFields x n + SC_COUNT
   double *cap = nullptr;         // 4 substeps x (RU_new 3, Rho_new, rhs) x n + 4 sums
   double *lup = nullptr;         // 4 substeps x 3 x n
   std::vector<double *> pcap;    // P before each Poisson iteration (grown on demand)
+  int mode = 0;                  // gas tape: 0 keep when it fits in memory, 1 keep, 2 recompute each step
+  bool store = false;            // the recorded run kept every substep (sub, pcap): no recomputation
+  double *sub = nullptr;         // nsteps x 4 x (RU_new 3, Rho_new, raw RHS_Pois) x n, then nsteps x 4 sums
+  std::vector<size_t> p_first;   // first pcap buffer of each substep's solve
+  size_t pnext = 0;
   GasAdjoint ga{};
 };
 

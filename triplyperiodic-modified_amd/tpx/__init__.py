@@ -38,7 +38,8 @@ EXPORTS = [
     "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop", "tpx_compute_div_u_new",
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
     "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
-    "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_release",
+    "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_set_gas_tape",
+    "tpx_adjoint_release",
     "tpx_batch_create", "tpx_batch_step", "tpx_batch_destroy", "tpx_batch_error", "tpx_batch_last_error",
     "tpx_field_sum",
     "tpx_field_max_abs",
@@ -94,6 +95,7 @@ def load(path: str = LIB_PATH):
     L.tpx_adjoint_record.argtypes = [P, I, I, DP]
     L.tpx_adjoint_gradient.argtypes = [P, DP, DP]
     L.tpx_adjoint_gradient_full.argtypes = [P, DP, DP, DP, DP, DP, DP]
+    L.tpx_adjoint_set_gas_tape.argtypes = [P, I]
     L.tpx_adjoint_release.argtypes = [P]
     L.tpx_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.tpx_batch_step.argtypes = [P, I]
@@ -275,6 +277,13 @@ class Solver:
                  "adjoint_gradient_full")
         out["A"] = float(out["A"][0])
         return out
+
+    def adjoint_set_gas_tape(self, mode: str = "auto") -> None:
+        """How the next recording keeps the gas for the gas sweep: "auto"
+        (keep every substep when it fits in half the free device memory),
+        "keep", or "recompute" (each step from its start)."""
+        code = {"auto": 0, "keep": 1, "recompute": 2}[mode]
+        self._ok(self.L.tpx_adjoint_set_gas_tape(self.h, code), "adjoint_set_gas_tape")
 
     def adjoint_release(self) -> None:
         self._ok(self.L.tpx_adjoint_release(self.h), "adjoint_release")
