@@ -11,9 +11,10 @@ projection, followed by make_mean_U0 and TimeAdvance.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 
 Prints one JSON line.  `value` is steps/s with all state resident in HBM.
-`roofline` reports the dominant kernel (the density + momentum right-hand
-side) with its algorithmic bytes per launch over its event-timed launch
-duration; `cpu_baseline` times the oracle (the CPU restatement of the
+`roofline` reports the dominant kernel (the one with the most device time
+per step among the particle advance, the momentum right-hand side and the
+three Poisson-iteration passes) with its algorithmic bytes per launch over
+its event-timed launch duration; `cpu_baseline` times the oracle (the CPU restatement of the
 reference step) on a bounded sample of the same workload.
 """
 import argparse
@@ -317,9 +318,11 @@ def main():
 
     # per-kernel-class device times with HIP events on the library's stream
     s.timing(True)
+    it_t0 = s.scalars()["total_iterations"]
     s.step(args.timing_steps)
     ms, launches = s.timing_read()
     s.timing(False)
+    iters_timed = s.scalars()["total_iterations"] - it_t0
     per_step = {k: ms[k] / args.timing_steps for k in ms}
     log("[bench] device ms per step by kernel class: " +
         ", ".join(f"{k}={v:.3f}" for k, v in per_step.items()))
@@ -333,11 +336,21 @@ def main():
         "grid_rhs": ("k_rhs_ru", "density + momentum right-hand side, fused RK4 update",
                      sum(grid_rhs_bytes_per_launch(ncell, rk, p.TWC and npart > 0) for rk in range(4)) / 4.0),
     }
+    # the three passes of a Poisson iteration; their launches queued after a
+    # solve has stopped return at once, so a pass's time is divided by the
+    # iterations that ran
+    spec = (p.Nx_tot // 2 + 1) * p.Ny_tot * p.Nz_tot * 16.0
+    fft = {
+        "fft": ("k_xy_fwd", "Poisson iteration pass 1: residual, x transform, first y step", 3 * 8.0 * ncell + spec),
+        "fft_yz": ("k_yz_pass", "Poisson iteration pass 2: y and z transforms, division by lambda", 2 * spec),
+        "fft_inv": ("k_xy_inv", "Poisson iteration pass 3: inverse y and x transforms, P += dp", spec + 2 * 8.0 * ncell),
+    }
+    kernels.update(fft)
     lines = {}
     for cls, (kname, what, bytes_launch) in kernels.items():
         if launches.get(cls, 0) == 0:
             continue
-        avg_ms = ms[cls] / launches[cls]
+        avg_ms = ms[cls] / (iters_timed if cls in fft else launches[cls])
         achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
         t = traffic.get(kname)
         lines[cls] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -203,13 +203,15 @@ int tpx_deposit_concentration(tpx_ctx *ctx);
 typedef enum tpx_kclass {
   TPX_K_GRID_RHS = 0,   /* density + momentum right-hand side (G1+G4) */
   TPX_K_DIV_RHS,        /* low-Mach divergence + Poisson RHS (G5+G6) */
-  TPX_K_FFT,            /* Poisson iteration passes (G8/G9) */
+  TPX_K_FFT,            /* Poisson iteration, first pass: residual, x transform, first y step (G8/G9) */
   TPX_K_PROJECT,        /* pressure-gradient projection (G10) */
   TPX_K_PARTICLE,       /* kept for numbering: the temperature gather (P8) runs inside TPX_K_DEPOSIT */
   TPX_K_DEPOSIT,        /* deposits (P5, P6, P9) with the temperature gather (P8) */
   TPX_K_SORT,           /* cell sort of the particles */
   TPX_K_OTHER,          /* reductions, scalars, rotations */
   TPX_K_ADVANCE,        /* particle RK4 advance with its velocity gather (P1-P4, P7) */
+  TPX_K_FFT_YZ,         /* Poisson iteration, middle pass: y and z transforms, division by lambda */
+  TPX_K_FFT_INV,        /* Poisson iteration, last pass: inverse y and x transforms, P += dp, loop test */
   TPX_K_COUNT
 } tpx_kclass;
 int tpx_get_stream(tpx_ctx *ctx, void **stream);

@@ -186,9 +186,11 @@ def test_kept_and_recomputed_tapes_agree(gpu_available, iterations):
     assert np.array_equal(rua, rub) and np.array_equal(pa, pb)
     assert np.array_equal(ga["tau"], gb["tau"])
     assert abs(ga["A"] - gb["A"]) <= 1e-10 * abs(ga["A"])
+    # one scale for the three fields: after six iterations dJ/dP0 is itself
+    # rounding (~1e-18), so its own maximum is no measure of agreement
+    scale = max(float(np.max(np.abs(ga[q]))) for q in ("RU", "Rho", "P"))
+    assert scale > 0.0
     for q in ("RU", "Rho", "P"):
-        scale = float(np.max(np.abs(ga[q])))
-        assert scale > 0.0, q
         np.testing.assert_allclose(ga[q], gb[q], rtol=0, atol=1e-10 * scale, err_msg=q)
 
 

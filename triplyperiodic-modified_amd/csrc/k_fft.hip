@@ -1131,19 +1131,23 @@ void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, c
     TPX_FOUR_STEP_SWITCH(g.nx, L_XYF)
 #undef L_XYF
     TPX_LAUNCH_CHECK();
+    timing_end(c, TPX_K_FFT);
+    timing_begin(c, TPX_K_FFT_YZ);
     const unsigned nb2 = (unsigned)(mxh * y2);
     const size_t sh2 = sizeof(double2) * (size_t)y1 * (g.nz + g.nz / 16);
 #define L_YZ(A, B) yz_launch<A, B>(nb2, sh2, c->stream, c->spec[0], mxh, y1, y2, c, done)
     TPX_FOUR_STEP_SWITCH(g.nz, L_YZ)
 #undef L_YZ
     TPX_LAUNCH_CHECK();
+    timing_end(c, TPX_K_FFT_YZ);
+    timing_begin(c, TPX_K_FFT_INV);
 #define L_XYI(A, B) xy_inv_launch<A, B>(nb13, sh13, c->stream, xi, y1, y2, c->tw[1])
     TPX_FOUR_STEP_SWITCH(g.nx, L_XYI)
 #undef L_XYI
     TPX_LAUNCH_CHECK();
     tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nb13, scal, c->Iteration, c->p.epsilon);
     TPX_LAUNCH_CHECK();
-    timing_end(c, TPX_K_FFT);
+    timing_end(c, TPX_K_FFT_INV);
     return;
   }
   const int cx = x4 ? 128 / r1 : lines_per_block(g.nx);
@@ -1158,6 +1162,8 @@ void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, c
     tpx_launch<k_xfwd>(nbx, kThreads, shx, c->stream, xa, cx);
   }
   TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_FFT);
+  timing_begin(c, TPX_K_FFT_YZ);
   const bool y4 = four_step(g.ny, r1, r2);
   if (y4) {
     const int C = 128 / r1;
@@ -1204,6 +1210,8 @@ void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, c
                                                                                      c->tw[1], ty, done);
   }
   TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_FFT_YZ);
+  timing_begin(c, TPX_K_FFT_INV);
   if (x4) {
 #define L_XI(A, B) tpx_launch<k_xinv4<A, B>>(nbx, cx * A, shx4, c->stream, xi, cx)
     TPX_FOUR_STEP_SWITCH(g.nx, L_XI)
@@ -1214,7 +1222,7 @@ void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, c
   TPX_LAUNCH_CHECK();
   tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nbx, scal, c->Iteration, c->p.epsilon);
   TPX_LAUNCH_CHECK();
-  timing_end(c, TPX_K_FFT);
+  timing_end(c, TPX_K_FFT_INV);
 }
 
 }  // namespace tpx

@@ -27,7 +27,8 @@ STAGE = {"base": 0, "int": 1, "new": 2, "np1": 3}
 SCALARS = ["P0", "P0_int", "P0_new", "P0_np1", "T_cur", "num_timestep",
            "num_iteration", "mean_energy_transferred", "A", "I0", "Iteration",
            "gx", "gy", "gz", "dP0_dt", "total_iterations"]
-KCLASS = ["grid_rhs", "div_rhs", "fft", "project", "particle", "deposit", "sort", "other", "advance"]
+KCLASS = ["grid_rhs", "div_rhs", "fft", "project", "particle", "deposit", "sort", "other", "advance",
+          "fft_yz", "fft_inv"]
 
 # C entry points of include/tpx.h, in declaration order
 EXPORTS = [
