@@ -333,7 +333,7 @@ def main():
         "advance": ("k_advance", "particle RK4 advance with its velocity gather and cell count",
                     sum(advance_bytes_per_launch(ncell, npart, rk, per_eps, per_tau, bool(p.TWC), counted)
                         for rk in range(4)) / 4.0),
-        "grid_rhs": ("k_rhs_ru", "density + momentum right-hand side, fused RK4 update",
+        "grid_rhs": ("k_rhs_ru_kernel", "density + momentum right-hand side, fused RK4 update",
                      sum(grid_rhs_bytes_per_launch(ncell, rk, p.TWC and npart > 0) for rk in range(4)) / 4.0),
     }
     # the three passes of a Poisson iteration; their launches queued after a
