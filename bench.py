@@ -189,8 +189,8 @@ def ensemble_bench(args):
 
 
 def adjoint_bench(args):
-    """Adjoint cost: 128^3, 2e5 particles, St = 2, mass loading 0.5, one-way
-    coupling, 50 steps with a particle checkpoint every 10.  The whole reverse
+    """Adjoint cost: 128^3, 2e5 particles, St = 2, two-way coupling at mass
+    loading 0.5, 50 steps with a particle checkpoint every 10.  The whole reverse
     sweep (per step, backwards: the particles' transposed RK4 step from their
     checkpoint segment, then the gas step recomputed from its start and
     reversed through the projection, the Poisson iterations, the momentum and
@@ -199,7 +199,7 @@ def adjoint_bench(args):
     import tpx
     from tpx import ic
     nsteps, every = 50, 10
-    case = ic.make_case(128, 200_000, nu=0.01, seed=0, St=2.0, phi=0.5, heat=False, twc=False, nsteps=nsteps)
+    case = ic.make_case(128, 200_000, nu=0.01, seed=0, St=2.0, phi=0.5, heat=False, twc=True, nsteps=nsteps)
     plain = tpx.Solver(case.params, np_capacity=200_000)
     plain.load_case(case)
     plain.step(1)
@@ -248,7 +248,7 @@ def adjoint_bench(args):
         "metric": "adjoint/forward cost ratio (128^3, 2e5 particles, 50 steps)", "value": round(t_rev / t_fwd, 4),
         "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": 1, "higher_is_better": False,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "adjoint: 128^3, 2e5 particles, St=2, mass loading 0.5, one-way coupling, "
+        "config": {"workload": "adjoint: 128^3, 2e5 particles, St=2, two-way coupling at mass loading 0.5, "
                                "J = Var(C) of the final positions, gradients w.r.t. every particle's tau and "
                                "initial state, the forcing coefficient A and the initial RU, Rho, P; "
                                "particle checkpoint every 10 steps, gas tape kept per substep (it fits in "

@@ -137,8 +137,8 @@ def test_torch_shim_backward(gpu_available):
     assert np.array_equal(tau.grad.numpy(), g)
 
 
-def test_two_way_coupling_is_refused(gpu_available):
-    case = ic.make_case(16, 200, nu=0.02, seed=3, St=2.0, heat=False, twc=True)
+def test_heat_exchange_is_refused(gpu_available):
+    case = ic.make_case(16, 200, nu=0.02, seed=3, St=2.0, heat=True, twc=False)
     s = solver_from_case(case)
-    with pytest.raises(tpx.TpxError, match="one-way"):
+    with pytest.raises(tpx.TpxError, match="heat exchange"):
         s.adjoint_record(2, 1)

@@ -147,7 +147,6 @@ struct AdjArgs {
   const double *stage;  // 24 arrays of np: the int state of each substep
   double *lam[6];       // adjoint of the step's result in, of its start out
   double *dtau;         // dJ/dtau accumulated
-  double *lup;          // null, or 4 substeps x 3 face fields: dJ/du_g deposited (the gas adjoint's input)
   const double *tau;
   double tp;
   double c_pre[4], c_post[4];
@@ -207,14 +206,99 @@ struct k_adj_step_rev {
         lnext[3 + d] = lk[d] - lk[3 + d] * itp;     // k_x = u, and -u/tau
         dtau -= lk[3 + d] * (ug[d] - Y[3 + d]) * itp * itp;
       }
-      if (a.lup) {
-        const double lug[3] = {lk[3] * itp, lk[4] * itp, lk[5] * itp};
-        gather_transpose(g, ix, Y, lug, a.lup + (long)rk * 3 * a.ncell, a.ncell);
-      }
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) a.lam[q][p] = lb[q] + lnext[q];  // the first int state is the base
     a.dtau[p] += dtau;
+  }
+};
+
+// One stage of the transposed RK4 step, for the sweep that reverses the gas
+// between the stages.  Under two-way coupling the stage's drag deposit
+// S = D(RHS_u) (particle.cpp:494-520, the gather's weights) feeds the gas
+// momentum right-hand side as a_drag S, so the gas sweep's lam_RHS of this
+// substep returns to the particle through the gather of lam_S = a_drag lam_RHS:
+// it adds to the adjoint of RHS_u, and its derivative along each axis times
+// RHS_u to the adjoint of the position.
+struct StageArgs {
+  Geo g;
+  const double *gas;     // this substep's taped gas (RU_int 3, Rho_int)
+  const double *stage;   // the state entering this substep: 6 arrays of np
+  const double *lend;    // 6 x np: adjoint of the step's result
+  double *lb, *lnext;    // 6 x np each: base accumulator; adjoint of the next stage's state in, of this one's out
+  double *dtau;
+  double *lup;           // the gather's transpose of dJ/du_g: 3 face fields of this substep, or null
+  const double *ls[3];   // lam_RHS of this substep's momentum update (3 face fields), or null
+  double a_drag;         // -mp/Vcell
+  const double *tau;
+  double tp;
+  double c_pre, c_post;  // dt pre, dt post of this substep
+  int last;              // the fourth substep
+  long np, ncell;
+};
+
+struct k_adj_stage_rev {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const StageArgs &a) {
+    const long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (p >= a.np) return;
+    const Geo &g = a.g;
+    Ix ix = make_ix(g);
+    const double itp = 1. / (a.tau ? a.tau[p] : a.tp);
+    double Y[6], lk[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      Y[q] = a.stage[q * a.np + p];
+      const double le = a.lend[q * a.np + p];
+      if (a.last) {
+        lk[q] = a.c_post * le;
+        a.lb[q * a.np + p] = le;  // the np1 accumulator starts as the base state
+      } else {
+        const double ln = a.lnext[q * a.np + p];
+        lk[q] = a.c_post * le + a.c_pre * ln;
+        a.lb[q * a.np + p] += ln;  // new = base + dt pre k
+      }
+    }
+    double ug[3], dug[3];
+    gather(g, ix, Gas{a.gas, a.ncell}, Y, ug, dug);
+    double ls[3] = {0.0, 0.0, 0.0}, dls[3] = {0.0, 0.0, 0.0};
+    if (a.ls[0]) {  // lam_S at the particle with the gather's weights, and its derivative
+      int ii = (int)(Y[0] * g.idx), jj = (int)(Y[1] * g.idy), kk = (int)(Y[2] * g.idz);
+      const double fr[3] = {Y[0] * g.idx - ((int)(Y[0] * g.idx)), Y[1] * g.idy - ((int)(Y[1] * g.idy)),
+                            Y[2] * g.idz - ((int)(Y[2] * g.idz))};
+      const long c0 = ix(ii, jj, kk);
+      const long c1[3] = {ix(ii + 1, jj, kk), ix(ii, jj + 1, kk), ix(ii, jj, kk + 1)};
+      const double id[3] = {g.idx, g.idy, g.idz};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double l1 = a.a_drag * a.ls[d][c1[d]], l0 = a.a_drag * a.ls[d][c0];
+        ls[d] = fr[d] * l1 + (1 - fr[d]) * l0;
+        dls[d] = (l1 - l0) * id[d];
+      }
+    }
+    double ln[6], dtau = 0.0, lug[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double rhs = (ug[d] - Y[3 + d]) * itp;
+      const double lr = lk[3 + d] + ls[d];  // adjoint of RHS_u
+      ln[d] = lr * itp * dug[d] + rhs * dls[d];
+      ln[3 + d] = lk[d] - lr * itp;
+      dtau -= lr * (ug[d] - Y[3 + d]) * itp * itp;
+      lug[d] = lr * itp;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.lnext[q * a.np + p] = ln[q];
+    a.dtau[p] += dtau;
+    if (a.lup) gather_transpose(g, ix, Y, lug, a.lup, a.ncell);
+  }
+};
+
+// the adjoint of the step's start: base accumulator + the first stage's state
+struct k_adj_stage_end {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const double *lb, const double *lnext, double *lam, long n6) {
+    const long i = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (i < n6) lam[i] = lb[i] + lnext[i];
   }
 };
 
@@ -250,12 +334,13 @@ struct k_dj_dc {
 struct k_conc_adj {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const Geo &g, const double *x, const double *y, const double *z,
-                                             const double *djdc, double *l0, double *l1, double *l2, double *l3,
-                                             double *l4, double *l5, long np) {
-    const long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
-    if (p >= np) return;
+                                             const int32_t *cum, const double *djdc, double *l0, double *l1,
+                                             double *l2, double *l3, double *l4, double *l5, long np) {
+    const long i = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (i >= np) return;
+    const long p = cum[i];  // the particle at i, in the recording's order
     Ix ix = make_ix(g);
-    const double xp = x[p], yp = y[p], zp = z[p];
+    const double xp = x[i], yp = y[i], zp = z[i];
     int ii = (int)(round(xp * g.idx)), jj = (int)(round(yp * g.idy)), kk = (int)(round(zp * g.idz));
     double xl = xp * g.idx - ii + 0.5, yl = yp * g.idy - jj + 0.5, zl = zp * g.idz - kk + 0.5;
     double gx = 0.0, gy = 0.0, gz = 0.0;
@@ -272,6 +357,36 @@ struct k_conc_adj {
         }
     l0[p] = gx; l1[p] = gy; l2[p] = gz;
     l3[p] = 0.0; l4[p] = 0.0; l5[p] = 0.0;
+  }
+};
+
+// The particle order of a recorded run: cum[i] is the position at the start
+// of the recording of the particle now at i.  A re-sort of the particle arrays
+// (dst[q] = src[perm[q]]) is applied to cum too; checkpoints, tau and the
+// sweep's arrays are kept in the recording's order.
+struct k_adj_gather_i {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const int32_t *src, const int32_t *perm, int32_t *dst, long n) {
+    const long q = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (q < n) dst[q] = src[perm[q]];
+  }
+};
+
+// dst[q][cum[i]] = src[q][i] for the 6 state arrays
+struct k_adj_scatter6 {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const double *s0, const double *s1, const double *s2, const double *s3,
+                                             const double *s4, const double *s5, const int32_t *cum, double *dst,
+                                             long n) {
+    const long i = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (i >= n) return;
+    const long p = cum[i];
+    dst[p] = s0[i];
+    dst[n + p] = s1[i];
+    dst[2 * n + p] = s2[i];
+    dst[3 * n + p] = s3[i];
+    dst[4 * n + p] = s4[i];
+    dst[5 * n + p] = s5[i];
   }
 };
 
@@ -292,7 +407,7 @@ StepArgs step_args(tpx_ctx *c, int step, const double *const yin[6], double *con
     a.y[q] = y[q];
   }
   a.stage = stage;
-  a.tau = pt.has_tau ? pt.tau : nullptr;
+  a.tau = pt.has_tau ? t.tau0 : nullptr;
   a.tp = c->Tp;
   for (int r = 0; r < 4; ++r) {
     a.c_pre[r] = c->p.dt * c->pre[r];
@@ -322,7 +437,22 @@ void adjoint_tape_substep(tpx_ctx *c, int rk, const double *const ru_int[3], con
 
 void adjoint_checkpoint(tpx_ctx *c, int slot) {
   Particles &pt = c->pt;
-  for (int q = 0; q < 6; ++q) launch_copy(c, c->adj.ckpt + ((long)slot * 6 + q) * pt.n, pt.q[q][pt.st_base], pt.n);
+  const int b = pt.st_base;
+  tpx_launch<k_adj_scatter6>(grid1(pt.n), kBlock, 0, c->stream, (const double *)pt.q[0][b], (const double *)pt.q[1][b],
+                             (const double *)pt.q[2][b], (const double *)pt.q[3][b], (const double *)pt.q[4][b],
+                             (const double *)pt.q[5][b], (const int32_t *)c->adj.cum,
+                             c->adj.ckpt + (long)slot * 6 * pt.n, (long)pt.n);
+  TPX_LAUNCH_CHECK();
+}
+
+// a re-sort of the particle arrays by pt.perm_int is about to happen
+void adjoint_track_permutation(tpx_ctx *c) {
+  AdjointTape &t = c->adj;
+  const long np = c->pt.n;
+  tpx_launch<k_adj_gather_i>(grid1(np), kBlock, 0, c->stream, (const int32_t *)t.cum, (const int32_t *)c->pt.perm_int,
+                             t.cum_tmp, np);
+  TPX_LAUNCH_CHECK();
+  std::swap(t.cum, t.cum_tmp);
 }
 
 double adjoint_objective(tpx_ctx *c) {
@@ -344,7 +474,7 @@ double adjoint_objective(tpx_ctx *c) {
   return s[1] / (double)n - mean * mean;
 }
 
-void adjoint_reverse(tpx_ctx *c, const std::function<void(int)> &gas_step) {
+void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
   Particles &pt = c->pt;
   AdjointTape &t = c->adj;
   const long np = pt.n;
@@ -356,11 +486,17 @@ void adjoint_reverse(tpx_ctx *c, const std::function<void(int)> &gas_step) {
   // objective: lam_x from the concentration weights at the final positions
   tpx_launch<k_conc_adj>(grid1(np), kBlock, 0, c->stream, c->g, (const double *)pt.q[0][pt.st_base],
                          (const double *)pt.q[1][pt.st_base], (const double *)pt.q[2][pt.st_base],
-                         (const double *)t.djdc, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5], np);
+                         (const int32_t *)t.cum, (const double *)t.djdc, lam[0], lam[1], lam[2], lam[3], lam[4],
+                         lam[5], np);
   TPX_LAUNCH_CHECK();
   launch_fill(c, t.dtau, 0.0, np);
   // the start of every step of the checkpoint segment being reversed,
   // recomputed once per segment from its checkpoint
+  if (gas && !t.lb) {
+    if (hipMalloc((void **)&t.lb, sizeof(double) * 6 * (size_t)np) != hipSuccess ||
+        hipMalloc((void **)&t.lnext, sizeof(double) * 6 * (size_t)np) != hipSuccess)
+      throw std::bad_alloc();
+  }
   const int seg_len = std::min(t.every, t.nsteps);
   if (!t.seg && hipMalloc((void **)&t.seg, sizeof(double) * 6 * (size_t)np * seg_len) != hipSuccess)
     throw std::bad_alloc();
@@ -386,25 +522,55 @@ void adjoint_reverse(tpx_ctx *c, const std::function<void(int)> &gas_step) {
     for (int q = 0; q < 6; ++q) in[q] = start_of(s - base, q);
     tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, s, in, y, t.stage));
     TPX_LAUNCH_CHECK();
-    AdjArgs a{};
-    a.g = c->g;
-    a.gas = t.gas + (long)s * 16 * c->g.n;
-    a.stage = t.stage;
-    for (int q = 0; q < 6; ++q) a.lam[q] = lam[q];
-    a.dtau = t.dtau;
-    a.lup = gas_step ? t.lup : nullptr;
-    if (a.lup) launch_fill(c, t.lup, 0.0, 12 * c->g.n);
-    a.tau = pt.has_tau ? pt.tau : nullptr;
-    a.tp = c->Tp;
-    for (int r = 0; r < 4; ++r) {
-      a.c_pre[r] = c->p.dt * c->pre[r];
-      a.c_post[r] = c->p.dt * c->post[r];
+    if (!gas) {
+      AdjArgs a{};
+      a.g = c->g;
+      a.gas = t.gas + (long)s * 16 * c->g.n;
+      a.stage = t.stage;
+      for (int q = 0; q < 6; ++q) a.lam[q] = lam[q];
+      a.dtau = t.dtau;
+      a.tau = pt.has_tau ? t.tau0 : nullptr;
+      a.tp = c->Tp;
+      for (int r = 0; r < 4; ++r) {
+        a.c_pre[r] = c->p.dt * c->pre[r];
+        a.c_post[r] = c->p.dt * c->post[r];
+      }
+      a.np = np;
+      a.ncell = c->g.n;
+      tpx_launch<k_adj_step_rev>(grid1(np), kBlock, 0, c->stream, a);
+      TPX_LAUNCH_CHECK();
+      continue;
     }
-    a.np = np;
-    a.ncell = c->g.n;
-    tpx_launch<k_adj_step_rev>(grid1(np), kBlock, 0, c->stream, a);
+    // the particles' stages interleaved with the gas substeps, backwards
+    gas->begin(s);
+    for (int rk = 3; rk >= 0; --rk) {
+      const double *const *lrhs = gas->part_a(s, rk);
+      StageArgs a{};
+      a.g = c->g;
+      a.gas = t.gas + ((long)s * 4 + rk) * 4 * c->g.n;
+      a.stage = t.stage + (long)rk * 6 * np;
+      a.lend = t.lam;
+      a.lb = t.lb;
+      a.lnext = t.lnext;
+      a.dtau = t.dtau;
+      a.lup = t.lup;
+      launch_fill(c, t.lup, 0.0, 3 * c->g.n);
+      for (int d = 0; d < 3; ++d) a.ls[d] = gas->a_drag != 0.0 ? lrhs[d] : nullptr;
+      a.a_drag = gas->a_drag;
+      a.tau = pt.has_tau ? t.tau0 : nullptr;
+      a.tp = c->Tp;
+      a.c_pre = c->p.dt * c->pre[rk];
+      a.c_post = c->p.dt * c->post[rk];
+      a.last = rk == 3;
+      a.np = np;
+      a.ncell = c->g.n;
+      tpx_launch<k_adj_stage_rev>(grid1(np), kBlock, 0, c->stream, a);
+      TPX_LAUNCH_CHECK();
+      gas->part_b(s, rk, t.lup);
+    }
+    tpx_launch<k_adj_stage_end>(grid1(6 * np), kBlock, 0, c->stream, (const double *)t.lb, (const double *)t.lnext,
+                                t.lam, 6 * np);
     TPX_LAUNCH_CHECK();
-    if (gas_step) gas_step(s);
   }
 }
 

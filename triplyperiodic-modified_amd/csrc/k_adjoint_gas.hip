@@ -477,8 +477,7 @@ void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga) {
   }
 }
 
-void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k) {
-  const long n = c->g.n;
+const double *const *adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k) {
   const unsigned nb = cells(c);
   if (s.iters < 1) throw std::logic_error("adjoint: a Poisson solve without iterations");
   // G11: the int state after this substep is its new state
@@ -561,6 +560,12 @@ void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const 
     tpx_launch<k_adj_rk>(nb, kBlock, 0, c->stream, a);
     TPX_LAUNCH_CHECK();
   }
+  return ga.lrhs;
+}
+
+void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k) {
+  const long n = c->g.n;
+  const unsigned nb = cells(c);
   MomArgs m{};
   m.g = c->g;
   for (int d = 0; d < 3; ++d) {

@@ -299,7 +299,10 @@ void do_time_advance(tpx_ctx *c) {
   Particles &pt = c->pt;
   if (!pt.np1_is_base) std::swap(pt.st_base, pt.st_np1);
   pt.np1_is_base = true;
-  if (pt.n > 0 && c->binned_this_step) launch_particle_permute(c);
+  if (pt.n > 0 && c->binned_this_step) {
+    if (c->adj.step >= 0) adjoint_track_permutation(c);
+    launch_particle_permute(c);
+  }
   c->binned_this_step = false;
 }
 
@@ -370,8 +373,10 @@ void step_end(tpx_ctx *c) {
 // frees every array of a recorded adjoint run
 void adjoint_free(AdjointTape &t) {
   auto f = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
-  f(t.gas); f(t.ckpt); f(t.y); f(t.seg); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
-  f(t.start); f(t.final_state); f(t.work); f(t.cap); f(t.lup); f(t.iters_seen); f(t.sub);
+  f(t.gas); f(t.ckpt); f(t.y); f(t.seg); f(t.lb); f(t.lnext); f(t.S); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
+  f(t.start); f(t.final_state); f(t.work); f(t.cap); f(t.lup); f(t.iters_seen); f(t.sub); f(t.tau0);
+  auto fi = [](int32_t *&p) { if (p) (void)hipFree(p); p = nullptr; };
+  fi(t.cum); fi(t.cum_tmp);
   t.store = false;
   for (auto &p : t.pcap) f(p);
   t.pcap.clear();
@@ -1059,6 +1064,8 @@ void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
     for (int d = 0; d < 3; ++d) g.ru_int[d] = gs + d * n;
     g.rho_int = gs + 3 * n;
     launch_update_p0(c, k);
+    if (t.S)  // two-way coupling: the recorded drag deposit of this substep
+      for (int d = 0; d < 3; ++d) launch_copy(c, c->S[d], t.S + (((long)s * 4 + rk) * 3 + d) * n, n);
     do_update_ru_wop(c, rk, true);
     double *cp = t.cap + (long)rk * 5 * n;
     for (int d = 0; d < 3; ++d) {
@@ -1157,7 +1164,7 @@ void gas_adjoint_run(tpx_ctx *c) {
     if (!*p && hipMalloc((void **)p, sizeof(double) * count) != hipSuccess) throw std::bad_alloc();
   };
   alloc(&t.work, (size_t)GasAdjoint::kFields * n + SC_COUNT);
-  alloc(&t.lup, 12 * (size_t)n);
+  alloc(&t.lup, 3 * (size_t)n);
   if (!t.store) {
     alloc(&t.cap, 20 * (size_t)n + 8);
     alloc(&t.iters_seen, 4 * (size_t)t.nsteps);
@@ -1177,17 +1184,20 @@ void gas_adjoint_run(tpx_ctx *c) {
   const int it0 = c->Iteration;
   GasSubstep sub[4];
   const bool stored = t.store;
-  auto step = [&](int s) {
+  GasSweep sweep;
+  sweep.begin = [&](int s) {
     if (stored) taped_step(c, s, sub);
     else gas_recompute_step(c, s, sub);
     if (s < t.nsteps - 1) adjoint_gas_boundary(c, ga);
-    for (int rk = 3; rk >= 0; --rk) {
-      sub[rk].lup = t.lup + (long)rk * 3 * n;
-      adjoint_gas_substep(c, ga, sub[rk], coef(c, rk));
-    }
   };
+  sweep.part_a = [&](int, int rk) { return adjoint_gas_substep_a(c, ga, sub[rk], coef(c, rk)); };
+  sweep.part_b = [&](int, int rk, const double *lup) {
+    sub[rk].lup = lup;
+    adjoint_gas_substep_b(c, ga, sub[rk], coef(c, rk));
+  };
+  sweep.a_drag = c->p.twc ? -c->mp / c->Vcell : 0.0;
   try {
-    adjoint_reverse(c, step);
+    adjoint_reverse(c, &sweep);
   } catch (...) {
     c->A = A0; c->Iteration = it0;
     for (int d = 0; d < 3; ++d) c->gv[d] = g0[d];
@@ -1329,8 +1339,7 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
     Particles &pt = c->pt;
     if (nsteps < 1 || every < 1) throw std::invalid_argument("nsteps and every must be positive");
     if (pt.n == 0) throw std::invalid_argument("the adjoint needs particles");
-    if (c->p.twc || heat_active(c))
-      throw std::logic_error("the particle adjoint needs one-way coupling: TWC off and no heat exchange");
+    if (heat_active(c)) throw std::logic_error("the adjoint does not differentiate the heat exchange: Nu pi Dp k must be 0");
     AdjointTape &t = c->adj;
     sync(c);
     adjoint_free(t);
@@ -1347,7 +1356,17 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
     alloc(&t.lam, 6 * (size_t)np);
     alloc(&t.dtau, (size_t)np);
     alloc(&t.djdc, (size_t)n);
+    if (hipMalloc((void **)&t.cum, sizeof(int32_t) * np) != hipSuccess ||
+        hipMalloc((void **)&t.cum_tmp, sizeof(int32_t) * np) != hipSuccess)
+      throw std::bad_alloc();
+    launch_iota(c, t.cum, np);
+    if (pt.has_tau) {
+      alloc(&t.tau0, (size_t)np);
+      launch_copy(c, t.tau0, pt.tau, np);
+    }
     alloc(&t.start, (size_t)nsteps * 4 * n);
+    const bool twc = c->p.twc != 0;
+    if (twc) alloc(&t.S, (size_t)nsteps * 12 * n);  // the drag deposits, for a gas recomputation
     alloc(&t.final_state, 8 * (size_t)n);
     // keep every substep's momentum, density and Poisson data when the
     // estimate (three Poisson iterations a solve) fits in half the free HBM
@@ -1387,6 +1406,8 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
         for (int rk = 0; rk < 4; ++rk) {  // fused_substep, keeping each solve's iteration count
           fused_substep_pre(c, rk);
           const size_t q = (size_t)s * 4 + rk;
+          if (twc)
+            for (int d = 0; d < 3; ++d) launch_copy(c, t.S + ((long)q * 3 + d) * n, c->S[d], n);
           if (t.store) {  // what the gas sweep reads, so that it need not recompute the step
             double *dst = t.sub + (long)q * 5 * n;
             for (int d = 0; d < 3; ++d) launch_copy(c, dst + d * n, c->ru[c->ru_new][d], n);
@@ -1436,14 +1457,28 @@ int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, 
     if (!t.ready) throw std::logic_error("no recorded run: call tpx_adjoint_record first");
     const bool gas = dJ_dA || dJ_dRU || dJ_dRho || dJ_dP;
     if (!gas && !dJ_dtau && !dJ_dstate) throw std::invalid_argument("null outputs");
-    if (gas) gas_adjoint_run(ctx);
+    // under two-way coupling the gas depends on the particles, so every
+    // gradient needs the gas sweep
+    if (gas || ctx->p.twc) gas_adjoint_run(ctx);
     else adjoint_reverse(ctx, nullptr);
     const size_t np = (size_t)ctx->pt.n, n = (size_t)ctx->g.n;
+    // the sweep's particle arrays are in the recording's order: give them in
+    // the context's present order (cum maps the one to the other)
+    std::vector<int32_t> cum(np);
+    std::vector<double> tmp(6 * np);
+    hip_check(hipMemcpyAsync(cum.data(), t.cum, sizeof(int32_t) * np, hipMemcpyDeviceToHost, ctx->stream), "download");
     if (dJ_dtau)
-      hip_check(hipMemcpyAsync(dJ_dtau, t.dtau, sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream), "download");
-    if (dJ_dstate)  // adjoint of the recorded run's initial x, y, z, u, v, w
-      hip_check(hipMemcpyAsync(dJ_dstate, t.lam, sizeof(double) * 6 * np, hipMemcpyDeviceToHost, ctx->stream),
+      hip_check(hipMemcpyAsync(tmp.data(), t.dtau, sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream), "download");
+    sync(ctx);
+    if (dJ_dtau)
+      for (size_t i = 0; i < np; ++i) dJ_dtau[i] = tmp[(size_t)cum[i]];
+    if (dJ_dstate) {  // adjoint of the recorded run's initial x, y, z, u, v, w
+      hip_check(hipMemcpyAsync(tmp.data(), t.lam, sizeof(double) * 6 * np, hipMemcpyDeviceToHost, ctx->stream),
                 "download");
+      sync(ctx);
+      for (int q = 0; q < 6; ++q)
+        for (size_t i = 0; i < np; ++i) dJ_dstate[q * np + i] = tmp[q * np + (size_t)cum[i]];
+    }
     if (dJ_dA)
       hip_check(hipMemcpyAsync(dJ_dA, ctx->scal + SC_ADJ_A, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
                 "download");

@@ -119,6 +119,13 @@ struct AdjointTape {
   double *ckpt = nullptr;        // (nsteps / every + 1) x 6 x np
   double *y = nullptr;           // 6 x np
   double *seg = nullptr;         // min(every, nsteps) x 6 x np: step starts of the segment being reversed
+  double *lb = nullptr, *lnext = nullptr;  // 6 x np each: the staged sweep's base and next-stage adjoints
+  double *S = nullptr;           // two-way coupling: the drag deposit of every substep (nsteps x 4 x 3 n)
+  // particle order: with re-binning (two-way coupling) the particle arrays are
+  // re-sorted at every step's end; the tape and the sweep keep the order the
+  // recording started in.  cum[i]: that position of the particle now at i.
+  int32_t *cum = nullptr, *cum_tmp = nullptr;
+  double *tau0 = nullptr;        // per-particle tau in that order (null: scalar tau)
   double *stage = nullptr;       // 4 x 6 x np
   double *lam = nullptr;         // 6 x np
   double *dtau = nullptr;        // np
@@ -253,13 +260,28 @@ void launch_exclusive_scan(tpx_ctx *c, int32_t *data, int64_t n);  // in place, 
 // k_adjoint.hip
 void adjoint_tape_substep(tpx_ctx *c, int rk, const double *const ru_int[3], const double *rho_int);
 void adjoint_checkpoint(tpx_ctx *c, int slot);
+void adjoint_track_permutation(tpx_ctx *c);  // before a re-sort by pt.perm_int during a recording
 double adjoint_objective(tpx_ctx *c);  // J = Var(C) of the base positions; leaves dJ/dC in adj.djdc
-// dJ/dtau into adj.dtau; with gas_step set, the particles' gather adjoint of
-// each step goes to adj.lup and gas_step(s) runs after the particles' step s
-void adjoint_reverse(tpx_ctx *c, const std::function<void(int)> &gas_step);
+// The gas half of the reverse sweep as the particle sweep drives it: for each
+// step s backwards begin(s), then for each substep rk backwards part_a(s, rk)
+// (up to the adjoint of the momentum right-hand side, which it returns: three
+// face fields), the particles' stage rk, and part_b(s, rk, lup) with lup the
+// particles' dJ/du_g on the faces.  a_drag = -mp/Vcell under two-way
+// coupling (the drag deposit's weight in the momentum right-hand side), else 0.
+struct GasSweep {
+  std::function<void(int)> begin;
+  std::function<const double *const *(int, int)> part_a;
+  std::function<void(int, int, const double *)> part_b;
+  double a_drag = 0.0;
+};
+// dJ/dtau into adj.dtau and dJ/d(initial particle state) into adj.lam; with
+// gas set, the gas sweep runs between the particles' stages
+void adjoint_reverse(tpx_ctx *c, const GasSweep *gas);
 
-// k_adjoint_gas.hip
-void adjoint_gas_substep(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
+// k_adjoint_gas.hip: one substep of the gas sweep in two halves (around the
+// particles' stage) and the step boundary
+const double *const *adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
+void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
 void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga);
 
 // ---- timing helpers (tpx_api.hip) ----
