@@ -157,21 +157,26 @@ int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency);
  * tpx_adjoint_record advances the context nsteps whole steps exactly as
  * tpx_step would, taping the gas stage fields each particle advance reads and
  * the gas at each step's start (RU, P), checkpointing the particles every
- * `every` steps, and returns J.  Only for one-way coupling (TWC off, no heat
- * exchange), where the gas does not depend on the particles.
+ * `every` steps, and returns J.  For runs without heat exchange, with one-
+ * or two-way coupling (under two-way coupling the drag deposits are taped
+ * too, and the particles' stages are reversed between the gas substeps).
  * tpx_adjoint_gradient then writes dJ/dtau_p for every particle and, if
  * dJ_dstate is not null, dJ/d of the particles' state when recording began
- * (6 arrays of np: x, y, z, u, v, w), both in the context's particle order
- * (the order tpx_download_particles uses; match by flag).
+ * (6 arrays of np: x, y, z, u, v, w), both in the context's present particle
+ * order (the order tpx_download_particles uses; match by flag): each value
+ * belongs to the particle now at that position.
  * tpx_adjoint_gradient_full adds the gas: dJ/dA (the forcing coefficient in
  * effect, one value) and dJ/d of the gas state when recording began, RU
  * (3 x n), Rho (n) and P (n), in tpx_download_field's order with bs = 0; any
  * output may be null, and the gas sweep runs only if a gas output is asked
- * for.  It recomputes each step of the gas from the tape and reverses it
+ * for.  It takes each step of the gas from the tape (kept, or recomputed
+ * from the step's start) and reverses it
  * (the momentum, density, projection and Poisson-iteration transposes), and
  * leaves the context's gas where the recorded run ended.  The Poisson solves
  * are differentiated with their recorded iteration counts.
- * tpx_adjoint_release frees the tape (nsteps x 20 grid fields) and the sweep's
+ * Under two-way coupling tpx_adjoint_gradient runs the gas sweep as well
+ * (the gas depends on the particles).  tpx_adjoint_release frees the tape
+ * (nsteps x 20 grid fields, + 12 under two-way coupling) and the sweep's
  * work arrays. */
 int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J);
 int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate);

@@ -1,10 +1,12 @@
 // k_adjoint.hip -- hand-written adjoint of the particle path (gfx950).
 //
 // Objective J = Var(C) over the cells, C the cloud-in-cell concentration of
-// the particles' final positions (particle.cpp:522-555 weights).  For runs
-// with one-way coupling (no drag deposit, no heat exchange) the gas does not
-// depend on the particles, so dJ/dtau_p is the particle path's adjoint alone:
-// the forward run tapes the gas stage fields every particle advance reads
+// the particles' final positions (particle.cpp:522-555 weights).  Under
+// one-way coupling (no drag deposit, no heat exchange) the gas does not
+// depend on the particles and dJ/dtau_p is the particle path's adjoint alone
+// (k_adj_step_rev); under two-way coupling the stages are reversed one at a
+// time between the gas substeps' reversal (k_adj_stage_rev, the gas sweep in
+// k_adjoint_gas.hip).  The forward run tapes the gas stage fields every particle advance reads
 // (RU_int, Rho_int of each substep) and checkpoints the particles' base state;
 // the reverse sweep recomputes each checkpoint segment once from its
 // checkpoint, keeping every step's start, with the same arithmetic as the

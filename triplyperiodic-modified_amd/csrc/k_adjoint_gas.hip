@@ -1,11 +1,13 @@
 // k_adjoint_gas.hip -- hand-written adjoint of the gas substep (gfx950).
 //
-// The reverse sweep of one RK4 substep of the one-way coupled gas
-// (grid.cpp:608-614, 776-863 with RHS_RU = 0 and RHS_Part_Temp = 0), run
-// after the substep has been recomputed from its step's checkpoint.  With
+// The reverse sweep of one RK4 substep of the gas (grid.cpp:608-614,
+// 776-863 with RHS_Part_Temp = 0: no heat exchange), from the substep as the
+// recording kept it or as recomputed from its step's start.  With
 //   F  = I_C2F(Rho_int),  U = RU_int / F                       (G3)
-//   RHS = mu/3 grad div U + mu del2 U - conv(U, RU_int)
+//   RHS = a_drag S + mu/3 grad div U + mu del2 U - conv(U, RU_int)
 //         + A RU_int + g (F - rho0)                            (G4)
+// (S the particles' drag deposit under two-way coupling: its adjoint
+// a_drag lam_RHS goes back to the particles' stage, k_adjoint.hip)
 //   D_rho = div RU_int                                         (G1)
 //   F' = I_C2F(Rho_new),  C = 1/F',  cc = -1/I_F2C(C)
 //   RHS_P = Z((div(RU_new/F') - k/cp del2(1/Rho_new)) / (dt pre))   (G5, G6)

@@ -1036,7 +1036,8 @@ void gas_state_copy(tpx_ctx *c, double *dst, bool save) {
 }
 
 // Recomputes step s of the recorded run, gas only (the gas does not depend on
-// the particles under one-way coupling), from its start on the tape, keeping
+// the particles under one-way coupling; under two-way coupling the recorded
+// drag deposits stand in for them), from its start on the tape, keeping
 // what the reverse sweep reads: RU_new and Rho_new after the momentum and
 // density updates, the raw Poisson right-hand side and its sum, and P before
 // every Poisson iteration.  The arithmetic is the fused step's.

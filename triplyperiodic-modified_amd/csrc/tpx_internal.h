@@ -107,7 +107,7 @@ struct GasSubstep {
   const double *lup = nullptr;         // the particles' gather adjoint on the faces (3 n), or null
 };
 
-// The adjoint of a recorded run (one-way coupling): the gas stage fields of
+// The adjoint of a recorded run (no heat exchange): the gas stage fields of
 // every substep, each step's gas start (RU, P), the particle base state
 // every `every` steps, and the reverse sweeps' work arrays.
 struct AdjointTape {

@@ -245,7 +245,7 @@ class Solver:
     def step(self, n: int = 1) -> None:
         self._ok(self.L.tpx_step(self.h, int(n)), "step")
 
-    # ---- adjoint of the step (one-way coupling) ----
+    # ---- adjoint of the step (no heat exchange; one- or two-way coupling) ----
     def adjoint_record(self, nsteps: int, every: int = 10) -> float:
         """Advance nsteps whole steps (as step()) while taping what the
         adjoint needs; returns J = Var(C) of the final positions."""

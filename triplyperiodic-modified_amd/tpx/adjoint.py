@@ -3,7 +3,7 @@
 `var_c(tau, make_solver, nsteps, every)` is a differentiable function of the
 per-particle response times: it builds a solver whose particles have response
 times `tau` (make_solver(tau_numpy) -> tpx.Solver, loaded and initialised),
-advances it `nsteps` whole steps with one-way coupling and returns
+advances it `nsteps` whole steps (no heat exchange) and returns
 J = Var(C) of the final positions; J.backward() fills tau.grad with the
 hand-written adjoint's dJ/dtau (tpx_adjoint_gradient).  The solver's
 particle ids must be the indices of `tau` (as tpx.ic builds them): the
