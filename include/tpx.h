@@ -220,6 +220,39 @@ typedef enum tpx_kclass {
   TPX_K_COUNT
 } tpx_kclass;
 int tpx_get_stream(tpx_ctx *ctx, void **stream);
+/* ---- grid::Statistics (grid.cpp:902-982) on the device.
+ * tpx_statistics_enable(ctx, 1) makes every step keep what the statistics
+ * read from inside its last substep (the concentration C at that substep's
+ * particle positions, the mean square of its pre-projection velocity
+ * RU_new / I_C2F(Rho_new) — the reference's stale U — and the maximum of the
+ * gathered gas x velocity); tpx_statistics then reduces the rest over the
+ * state at the step boundary and writes TPX_ST_COUNT values (reductions on
+ * the device, scalars to the host).  The passive scalar and vector are not
+ * built: their entries are 0, as they are in the reference with them off. */
+enum {
+  TPX_ST_TG_MEAN = 0,   /* mean of P0 / (R Rho) */
+  TPX_ST_TP_MEAN,       /* mean particle temperature */
+  TPX_ST_HT_MEAN,       /* mean of RHS_Part_Temp (mean_energy_transferred) */
+  TPX_ST_TKE2,          /* mean_squares of the last substep's pre-projection U */
+  TPX_ST_TKE_U, TPX_ST_TKE_V, TPX_ST_TKE_W,  /* mean of U_c RU_c, U = RU / I_C2F(Rho) */
+  TPX_ST_PASSIVE_SCALAR_MEAN, TPX_ST_TKEV_V1, TPX_ST_TKEV_V2, TPX_ST_TKEV_V3, TPX_ST_VV,  /* 0 */
+  TPX_ST_P0,
+  TPX_ST_C_MAX, TPX_ST_C_MIN, TPX_ST_C_MEAN,
+  TPX_ST_RHO_MAX, TPX_ST_RHO_MIN, TPX_ST_RHO_MEAN,
+  TPX_ST_PARTICLE_MAX_CFL,  /* 0, as in the reference */
+  TPX_ST_GAS_MAX_CFL,       /* max over axes of max(U_c) dt / d_c */
+  TPX_ST_GAS_MAX_DIFF_CFL,  /* dt / (dx^2 Rho_min / Mu0) * 6 */
+  TPX_ST_NUM_ITERATION,
+  TPX_ST_BALANCE_INDEX,     /* Np / N0 on one process */
+  TPX_ST_U_MAX,             /* max over axes of max(U_c) */
+  TPX_ST_VP_MAX,            /* max particle x velocity */
+  TPX_ST_UG_MAX,            /* max gathered gas x velocity, last substep */
+  TPX_ST_TP_MAX,            /* max particle temperature */
+  TPX_ST_COUNT
+};
+int tpx_statistics_enable(tpx_ctx *ctx, int on);
+int tpx_statistics(tpx_ctx *ctx, double *out /* TPX_ST_COUNT */);
+
 int tpx_timing_enable(tpx_ctx *ctx, int on);
 int tpx_timing_read(tpx_ctx *ctx, double *ms /* TPX_K_COUNT */, int64_t *launches /* TPX_K_COUNT */);
 int tpx_synchronize(tpx_ctx *ctx);

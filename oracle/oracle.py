@@ -232,6 +232,46 @@ class Oracle:
                        "compute_div_u_new", "compute_rhs_pois", "solve_poisson",
                        "update_ru_wp", "time_advance_rk4")
 
+    # ---- grid::Statistics (grid.cpp:902-982) ----
+    def statistics(self, p) -> Dict[str, float]:
+        """The reference's statistics over the oracle's state at a step
+        boundary, keyed as tpx.STATS; p is the run's tpx.params.Params.  U is
+        the field the last Compute_RHS_Pois left (its mean square is TKE2),
+        then U = RU_np1 / Rho_face as the reference recomputes it; C and the
+        particles' ug are what the last Update_Particle left.  The passive
+        scalar and vector are off: their entries are 0."""
+        nx, ny, nz = p.Nx_tot, p.Ny_tot, p.Nz_tot
+        n = nx * ny * nz
+        dx, dy, dz = p.Lx / nx, p.Ly / ny, p.Lz / nz
+        rho, C = self.get("Rho"), self.get("C")
+        U_last = self.get("U")
+        RU, Rf = self.get("RU_np1"), self.get("Rho_face")
+        U = RU / Rf
+        parts = self.particles("base")
+        ug = self.aux("ug")
+        sc = self.scalars()
+        npart = self.np
+        st = dict.fromkeys(["Passive_Scalar_mean", "TKEV_V1", "TKEV_V2", "TKEV_V3", "VV",
+                            "Particle_CFL_max"], 0.0)
+        st["Rho_mean"], st["Rho_max"], st["Rho_min"] = rho.sum() / n, rho.max(), rho.min()
+        st["C_max"], st["C_min"], st["C_mean"] = C.max(), C.min(), C.sum() / n
+        st["TKE2"] = float((U_last ** 2).sum()) / n
+        st["u_max"] = max(U[0].max(), U[1].max(), U[2].max())
+        st["Gas_CFL_max"] = max(U[0].max() * p.dt / dx, U[1].max() * p.dt / dy, U[2].max() * p.dt / dz)
+        UR = U * RU
+        st["TKE_U"], st["TKE_V"], st["TKE_W"] = (UR[c].sum() / n for c in range(3))
+        st["Vp_max"] = parts["u"].max() if npart else -1e9
+        st["ug_max"] = ug.max() if npart else -1e9
+        st["Tp_max"] = parts["T"].max() if npart else -1e9
+        st["Tp_mean"] = parts["T"].sum() / npart if npart else 0.0
+        st["Gas_diff_CFL_max"] = p.dt / (dx * dx * st["Rho_min"] / p.Mu0) * 6.
+        st["balance_index"] = npart / float(p.N0) if p.N0 > 0 else 0.0
+        st["HT_mean"] = sc["mean_energy_transferred"]
+        st["Tg_mean"] = ((sc["P0"] / p.R) / rho).sum() / n
+        st["P0"] = sc["P0"]
+        st["num_iteration"] = sc["num_iteration"]
+        return {k: float(v) for k, v in st.items()}
+
     # ---- particle primitives ----
     def p_gather_velocity(self): self.L.or_p_gather_velocity(self.h)
     def p_deposit_velocity(self): self.L.or_p_deposit_velocity(self.h)

@@ -181,7 +181,14 @@ void do_update_particle(tpx_ctx *c, int rk) {
   }
   const double *ri[3];
   ru_ptrs(c, c->ru_int, ri);
-  launch_concentration(c);
+  if (c->stats_on && rk == 3) {
+    stats_capture_particles(c, ri, c->rho[c->rho_int]);
+  } else {
+    // the bins of x_int are current only when the particles are re-binned
+    // every substep (two-way coupling or heat exchange)
+    if (!(c->p.twc || heat_active(c))) launch_sort_stage(c, TPX_INT, true);
+    launch_concentration(c);
+  }
   launch_particle_advance(c, k, ri, c->rho[c->rho_int], true);
   if (c->p.twc) launch_velocity_deposit(c);
   else for (int d = 0; d < 3; ++d) launch_fill(c, c->S[d], 0.0, c->g.n);
@@ -326,6 +333,11 @@ void fused_substep_pre(tpx_ctx *c, int rk) {
   Particles &pt = c->pt;
   bool parts = pt.n > 0;
   bool rebin = parts && (c->p.twc || heat_active(c));
+  if (c->stats_on && rk == 3) {  // Update_Particle's concentration and gather at this stage
+    const double *ri[3];
+    ru_ptrs(c, c->ru_int, ri);
+    stats_capture_particles(c, ri, c->rho[c->rho_int]);
+  }
   if (parts) {
     const double *ri[3];
     ru_ptrs(c, c->ru_int, ri);
@@ -349,6 +361,7 @@ void fused_substep_pre(tpx_ctx *c, int rk) {
   }
   const double *rn[3];
   ru_ptrs(c, c->ru_new, rn);
+  if (c->stats_on && rk == 3) stats_capture_velocity(c, rn, c->rho[c->rho_new]);  // Compute_RHS_Pois's U
   launch_div_rhs(c, k, 0, rn, c->rho[c->rho_new], c->q_zero ? nullptr : c->Q, nullptr, c->rhs);
 }
 
@@ -394,7 +407,7 @@ void release(tpx_ctx *c) {
   for (auto &p : c->spec) f(p);
   for (auto &p : c->lam) f(p);
   for (auto &p : c->tw) f(p);
-  f(c->scal); f(c->partials); f(c->scan_tmp);
+  f(c->scal); f(c->partials); f(c->scan_tmp); f(c->st_part);
   adjoint_free(c->adj);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   Particles &pt = c->pt;
@@ -796,6 +809,7 @@ int tpx_compute_rhs_pois(tpx_ctx *ctx, int rk) {
     if (ctx->rho_new < 0 || ctx->ru_new < 0) throw std::logic_error("Compute_RHS_Pois needs RU_new and Rho_new");
     const double *rn[3];
     ru_ptrs(ctx, ctx->ru_new, rn);
+    if (ctx->stats_on && rk == 3) stats_capture_velocity(ctx, rn, ctx->rho[ctx->rho_new]);
     launch_div_rhs(ctx, coef(ctx, rk), 2, rn, ctx->rho[ctx->rho_new], nullptr, ctx->divergence, ctx->rhs);
     return TPX_OK;
   });
@@ -1543,6 +1557,66 @@ int tpx_get_stream(tpx_ctx *ctx, void **stream) {
   return guarded(ctx, [&]() {
     if (!stream) throw std::invalid_argument("null");
     *stream = (void *)ctx->stream;
+    return TPX_OK;
+  });
+}
+
+// grid::Statistics (grid.cpp:902-982): reductions on the device at a step
+// boundary; the fourth substep's quantities were captured during the step
+int tpx_statistics_enable(tpx_ctx *ctx, int on) {
+  return guarded(ctx, [&]() {
+    ctx->stats_on = on != 0;
+    return TPX_OK;
+  });
+}
+
+int tpx_statistics(tpx_ctx *ctx, double *out) {
+  return guarded(ctx, [&]() {
+    tpx_ctx *c = ctx;
+    if (!out) throw std::invalid_argument("null output");
+    if (!c->stats_on) throw std::logic_error("statistics are off: call tpx_statistics_enable first");
+    if (!c->ru_np1_is_base || !c->rho_np1_is_base) throw std::logic_error("statistics are taken at a step boundary");
+    const double *ru[3];
+    ru_ptrs(c, c->ru_base, ru);
+    double raw[16];
+    stats_reduce(c, c->rho[c->rho_base], ru, raw);
+    double sc[SC_COUNT];
+    hip_check(hipMemcpy(sc, c->scal, sizeof(sc), hipMemcpyDeviceToHost), "download");
+    const double n = (double)c->g.n, dt = c->p.dt;
+    for (int q = 0; q < TPX_ST_COUNT; ++q) out[q] = 0.0;
+    out[TPX_ST_RHO_MEAN] = raw[0] / n;
+    out[TPX_ST_TG_MEAN] = raw[1] / n;
+    out[TPX_ST_TKE_U] = raw[2] / n;
+    out[TPX_ST_TKE_V] = raw[3] / n;
+    out[TPX_ST_TKE_W] = raw[4] / n;
+    out[TPX_ST_C_MEAN] = raw[5] / n;
+    out[TPX_ST_RHO_MAX] = raw[6];
+    out[TPX_ST_RHO_MIN] = raw[7];
+    out[TPX_ST_C_MAX] = raw[8];
+    out[TPX_ST_C_MIN] = raw[9];
+    // U.max() and U.max_cfl(dt) (tensor1.cpp:327-335, 365-373)
+    const double d[3] = {c->g.dx, c->g.dy, c->g.dz};
+    double umax = raw[10], cfl = raw[10] * dt / d[0];
+    for (int a = 1; a < 3; ++a) {
+      umax = raw[10 + a] > umax ? raw[10 + a] : umax;
+      const double ca = raw[10 + a] * dt / d[a];
+      cfl = ca > cfl ? ca : cfl;
+    }
+    out[TPX_ST_U_MAX] = umax;
+    out[TPX_ST_GAS_MAX_CFL] = cfl;
+    out[TPX_ST_VP_MAX] = raw[13];
+    out[TPX_ST_TP_MAX] = raw[14];
+    const double np = (double)c->pt.n;
+    out[TPX_ST_TP_MEAN] = c->pt.n > 0 ? raw[15] / np : 0.0;
+    out[TPX_ST_HT_MEAN] = sc[SC_MET];
+    out[TPX_ST_TKE2] = sc[SC_ST_TKE2] / n;
+    out[TPX_ST_UG_MAX] = sc[SC_ST_UG];
+    out[TPX_ST_P0] = sc[SC_P0];
+    out[TPX_ST_PARTICLE_MAX_CFL] = 0.0;
+    out[TPX_ST_GAS_MAX_DIFF_CFL] = dt / (c->g.dx * c->g.dx * out[TPX_ST_RHO_MIN] / c->p.mu0) * 6.;
+    out[TPX_ST_NUM_ITERATION] = (double)c->num_iteration;
+    const double n0 = (double)(int)(c->p.np0 * c->p.lx * c->p.ly * c->p.lz);  // params.h:120
+    out[TPX_ST_BALANCE_INDEX] = n0 > 0 ? np / n0 : 0.0;
     return TPX_OK;
   });
 }

@@ -40,6 +40,8 @@ enum Scal {
   SC_ADJ_SUM,      // adjoint sweep: sum of the Poisson right-hand side's adjoint
   SC_ADJ_M0, SC_ADJ_M1, SC_ADJ_M2,  // adjoint sweep: sums for make_mean_U0's transpose
   SC_ADJ_A,        // adjoint sweep: dJ/dA accumulated
+  SC_ST_TKE2,      // statistics: sum of squares of the last substep's pre-projection U
+  SC_ST_UG,        // statistics: max gathered gas x velocity of the last substep
   SC_COUNT = 32
 };
 
@@ -195,6 +197,9 @@ struct tpx_ctx {
   double2 *tw[3] = {};            // twiddles per axis (forward)
   double *scal = nullptr;         // device scalars
   double *partials = nullptr;     // block partial results, 3 x n_partials
+  bool stats_on = false;          // capture the statistics' last-substep quantities
+  double *st_part = nullptr;      // statistics partials (16 x st_stride) and results
+  long st_stride = 0;
   long n_partials = 0;
   double *h_scal = nullptr;       // pinned host mirror of scal
   int32_t *scan_tmp = nullptr;    // scan block sums
@@ -277,6 +282,13 @@ struct GasSweep {
 // dJ/dtau into adj.dtau and dJ/d(initial particle state) into adj.lam; with
 // gas set, the gas sweep runs between the particles' stages
 void adjoint_reverse(tpx_ctx *c, const GasSweep *gas);
+
+// k_stats.hip: grid::Statistics.  The captures run in the fourth substep:
+// before the particle advance (C and the gathered gas velocity at the stage's
+// positions) and after the momentum update (the pre-projection velocity).
+void stats_capture_particles(tpx_ctx *c, const double *const ru_int[3], const double *rho_int);
+void stats_capture_velocity(tpx_ctx *c, const double *const ru_new[3], const double *rho_new);
+void stats_reduce(tpx_ctx *c, const double *rho, const double *const ru[3], double raw[16]);
 
 // k_adjoint_gas.hip: one substep of the gas sweep in two halves (around the
 // particles' stage) and the step boundary

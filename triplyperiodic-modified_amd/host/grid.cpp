@@ -25,15 +25,35 @@ grid::grid(gridsize *size, params *param, proc *pc, communicator *com)
   Rho_np1.bind(&ctx_, size_, TPX_RHO_NP1, 0);
   divergence.bind(&ctx_, size_, TPX_DIVERGENCE, 0);
   RHS_Part_Temp.bind(&ctx_, size_, TPX_RHS_PART_TEMP, 0);
-  if (param_->Statistics() && pc_->IsRoot()) {
-    auto open = [&](std::ofstream &f, const char *name) {
-      f.open(param_->stat_dir() + name, std::ios::out | std::ios::trunc);
-      f.precision(16);
-    };
-    open(stat_P0, "P0.dat");
-    open(stat_NumIteration, "NumIteration.dat");
-    open(stat_RhoMean, "RhoMean.dat");
-    open(stat_RhoMax, "RhoMax.dat");
+  if (param_->Statistics()) {
+    ok(tpx_statistics_enable(ctx_, 1), "tpx_statistics_enable");
+    if (pc_->IsRoot()) {  // grid.cpp:32-57 and open_stat_file (grid.cpp:1147-1154)
+      auto open = [&](const char *name, std::ofstream &f) {
+        f.open(param_->stat_dir() + name + ".dat", std::ios::out | std::ios::app);
+      };
+      open("Tg", stat_Tg);
+      open("Tp", stat_Tp);
+      open("HT", stat_HT);
+      open("RU", stat_TKE2);
+      open("TKE_U", stat_TKE_U);
+      open("TKE_V", stat_TKE_V);
+      open("TKE_W", stat_TKE_W);
+      open("Passive_Scalar_mean", stat_Passive_Scalar_mean);
+      open("TKE_V1", stat_TKEV_V1);
+      open("TKE_V2", stat_TKEV_V2);
+      open("TKE_V3", stat_TKEV_V3);
+      open("P0", stat_P0);
+      open("C_Max", stat_CMax);
+      open("C_Min", stat_CMin);
+      open("C_Mean", stat_CMean);
+      open("Rho_Max", stat_RhoMax);
+      open("Rho_Min", stat_RhoMin);
+      open("Rho_Mean", stat_RhoMean);
+      open("Max_CFL_Vp", stat_ParticleMaxCFL);
+      open("Max_CFL_U", stat_GasMaxCFL);
+      open("Num_Iteration", stat_NumIteration);
+      open("Balance_Index", stat_BalanceIndex);
+    }
   }
 }
 
@@ -143,13 +163,60 @@ void grid::TimeAdvance() {
   pull_scalars();
 }
 
+// grid::Statistics (grid.cpp:902-982): the reductions run on the device
+// (tpx_statistics); the particle trajectory file (part.trajectory) is not
+// written by this build.
 void grid::Statistics() {
-  if (!param_->Statistics() || !pc_->IsRoot()) return;
+  if (!param_->Statistics()) return;
+  double st[TPX_ST_COUNT];
+  ok(tpx_statistics(ctx_, st), "tpx_statistics");
   pull_scalars();
-  stat_P0 << T_cur << " " << P0 << std::endl;
-  stat_NumIteration << T_cur << " " << PS_.num_iteration() << std::endl;
-  stat_RhoMean << T_cur << " " << Rho.mean() << std::endl;
-  stat_RhoMax << T_cur << " " << Rho.max_abs() << std::endl;
+  if (!pc_->IsRoot()) return;
+  auto line = [&](std::ofstream &f, double v) { f << T_cur << " " << v << std::endl; };
+  line(stat_Tg, st[TPX_ST_TG_MEAN]);
+  line(stat_Tp, st[TPX_ST_TP_MEAN]);
+  line(stat_HT, st[TPX_ST_HT_MEAN]);
+  line(stat_TKE2, st[TPX_ST_TKE2]);
+  line(stat_TKE_U, st[TPX_ST_TKE_U]);
+  line(stat_TKE_V, st[TPX_ST_TKE_V]);
+  line(stat_TKE_W, st[TPX_ST_TKE_W]);
+  line(stat_Passive_Scalar_mean, st[TPX_ST_PASSIVE_SCALAR_MEAN]);
+  line(stat_TKEV_V1, st[TPX_ST_TKEV_V1]);
+  line(stat_TKEV_V2, st[TPX_ST_TKEV_V2]);
+  line(stat_TKEV_V3, st[TPX_ST_TKEV_V3]);
+  line(stat_P0, st[TPX_ST_P0]);
+  line(stat_CMax, st[TPX_ST_C_MAX]);
+  line(stat_CMin, st[TPX_ST_C_MIN]);
+  line(stat_CMean, st[TPX_ST_C_MEAN]);
+  line(stat_RhoMax, st[TPX_ST_RHO_MAX]);
+  line(stat_RhoMin, st[TPX_ST_RHO_MIN]);
+  line(stat_RhoMean, st[TPX_ST_RHO_MEAN]);
+  line(stat_ParticleMaxCFL, st[TPX_ST_PARTICLE_MAX_CFL]);
+  line(stat_GasMaxCFL, st[TPX_ST_GAS_MAX_CFL]);
+  line(stat_GasMaxDiffCFL, st[TPX_ST_GAS_MAX_DIFF_CFL]);
+  line(stat_NumIteration, st[TPX_ST_NUM_ITERATION]);
+  line(stat_BalanceIndex, st[TPX_ST_BALANCE_INDEX]);
+  if (!param_->Stat_print()) return;
+  const double tke = st[TPX_ST_TKE_U] + st[TPX_ST_TKE_V] + st[TPX_ST_TKE_W];
+  std::cout << std::endl << "::::::::::TIME=" << T_cur << "::::::::::STEP=" << num_timestep << "::::::::::" << std::endl;
+  std::cout << "*** Rho_min=" << st[TPX_ST_RHO_MIN] << "  ,  Rho_max=" << st[TPX_ST_RHO_MAX]
+            << "  ,  Rho_mean=" << st[TPX_ST_RHO_MEAN] << std::endl;
+  std::cout << "*** C_min=" << st[TPX_ST_C_MIN] << "  ,  C_max=" << st[TPX_ST_C_MAX] << "  ,  C_mean=" << st[TPX_ST_C_MEAN]
+            << std::endl;
+  std::cout << "*** Particle Maximum CFL=" << st[TPX_ST_PARTICLE_MAX_CFL] << "  ,  Gas Maximum CFL="
+            << st[TPX_ST_GAS_MAX_CFL] << "  ,  Gas Maximum diffusive CFL=" << st[TPX_ST_GAS_MAX_DIFF_CFL] << std::endl;
+  std::cout << "*** P0=" << st[TPX_ST_P0] << "   ,   Number of Poisson solve iterations=" << st[TPX_ST_NUM_ITERATION]
+            << std::endl;
+  std::cout << "*** Twice TKE_U =" << st[TPX_ST_TKE_U] << "  ,  TKE_V =" << st[TPX_ST_TKE_V] << "  ,  TKE_W ="
+            << st[TPX_ST_TKE_W] << "  , Twice TKE =" << tke << "  , Twice TKE2=" << st[TPX_ST_TKE2] << std::endl;
+  std::cout << "*** Passive_Scalar_mean =" << st[TPX_ST_PASSIVE_SCALAR_MEAN] << std::endl;
+  std::cout << "*** Twice TKEV_V1 =" << st[TPX_ST_TKEV_V1] << "  ,  TKEV_V2 =" << st[TPX_ST_TKEV_V2] << "  ,  TKEV_V3 ="
+            << st[TPX_ST_TKEV_V3] << "  , Twice TKEV =" << 0.0 << "  , Twice VV=" << st[TPX_ST_VV] << std::endl;
+  std::cout << "*** Particle u_max=" << st[TPX_ST_VP_MAX] << "  ,  Gas interpolated u_max=" << st[TPX_ST_UG_MAX]
+            << "  ,  Gas u_max=" << st[TPX_ST_U_MAX] << std::endl;
+  std::cout << "Mean energy transferred from particle to gas=" << -st[TPX_ST_HT_MEAN]
+            << "  ,  Balance_Index=" << st[TPX_ST_BALANCE_INDEX] << std::endl;
+  std::cout << "*** Particle Tmax =" << st[TPX_ST_TP_MAX] << std::endl;
 }
 
 bool grid::Touch() {

@@ -19,7 +19,12 @@ class grid {
   tpx_ctx *ctx_ = nullptr;
   poisson PS_;
   bool Is_touch_ = false;
-  std::ofstream stat_P0, stat_NumIteration, stat_RhoMean, stat_RhoMax;
+  // the statistics streams (grid.h:22-49); GasMaxDiffCFL is never opened
+  // there, so its lines go nowhere, as here
+  std::ofstream stat_Tg, stat_Tp, stat_HT, stat_TKE2, stat_TKE_U, stat_TKE_V, stat_TKE_W,
+      stat_Passive_Scalar_mean, stat_TKEV_V1, stat_TKEV_V2, stat_TKEV_V3, stat_P0, stat_CMax, stat_CMin,
+      stat_CMean, stat_RhoMax, stat_RhoMin, stat_RhoMean, stat_ParticleMaxCFL, stat_GasMaxCFL,
+      stat_GasMaxDiffCFL, stat_NumIteration, stat_BalanceIndex;
   void pull_scalars();
   void ok(int rc, const char *what) { tpx_check(ctx_, rc, what); }
   [[noreturn]] void unsupported(const char *what);

@@ -27,6 +27,11 @@ STAGE = {"base": 0, "int": 1, "new": 2, "np1": 3}
 SCALARS = ["P0", "P0_int", "P0_new", "P0_np1", "T_cur", "num_timestep",
            "num_iteration", "mean_energy_transferred", "A", "I0", "Iteration",
            "gx", "gy", "gz", "dP0_dt", "total_iterations"]
+# tpx_statistics outputs, in TPX_ST_* order (grid.cpp:902-982)
+STATS = ["Tg_mean", "Tp_mean", "HT_mean", "TKE2", "TKE_U", "TKE_V", "TKE_W", "Passive_Scalar_mean",
+         "TKEV_V1", "TKEV_V2", "TKEV_V3", "VV", "P0", "C_max", "C_min", "C_mean", "Rho_max", "Rho_min",
+         "Rho_mean", "Particle_CFL_max", "Gas_CFL_max", "Gas_diff_CFL_max", "num_iteration",
+         "balance_index", "u_max", "Vp_max", "ug_max", "Tp_max"]
 KCLASS = ["grid_rhs", "div_rhs", "fft", "project", "particle", "deposit", "sort", "other", "advance",
           "fft_yz", "fft_inv"]
 
@@ -39,7 +44,7 @@ EXPORTS = [
     "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop", "tpx_compute_div_u_new",
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
     "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
-    "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_set_gas_tape",
+    "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_set_gas_tape", "tpx_statistics_enable", "tpx_statistics",
     "tpx_adjoint_release",
     "tpx_batch_create", "tpx_batch_step", "tpx_batch_destroy", "tpx_batch_error", "tpx_batch_last_error",
     "tpx_field_sum",
@@ -97,6 +102,8 @@ def load(path: str = LIB_PATH):
     L.tpx_adjoint_gradient.argtypes = [P, DP, DP]
     L.tpx_adjoint_gradient_full.argtypes = [P, DP, DP, DP, DP, DP, DP]
     L.tpx_adjoint_set_gas_tape.argtypes = [P, I]
+    L.tpx_statistics_enable.argtypes = [P, I]
+    L.tpx_statistics.argtypes = [P, DP]
     L.tpx_adjoint_release.argtypes = [P]
     L.tpx_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.tpx_batch_step.argtypes = [P, I]
@@ -288,6 +295,18 @@ class Solver:
 
     def adjoint_release(self) -> None:
         self._ok(self.L.tpx_adjoint_release(self.h), "adjoint_release")
+
+    # ---- grid::Statistics on the device ----
+    def statistics_enable(self, on: bool = True) -> None:
+        """Keep what the statistics read from inside each step's last
+        substep (call before stepping)."""
+        self._ok(self.L.tpx_statistics_enable(self.h, int(bool(on))), "statistics_enable")
+
+    def statistics(self) -> dict:
+        """grid::Statistics at the current step boundary (tpx.STATS names)."""
+        out = np.zeros(len(STATS))
+        self._ok(self.L.tpx_statistics(self.h, _dp(out)), "statistics")
+        return dict(zip(STATS, out.tolist()))
 
     def deposit_concentration(self) -> None:
         self._ok(self.L.tpx_deposit_concentration(self.h), "deposit_concentration")
