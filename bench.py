@@ -152,6 +152,13 @@ def ensemble_bench(args):
     ncells = sum(c.params.Nx_tot * c.params.Ny_tot * c.params.Nz_tot for c in cases)
     npart = sum(len(c.particles["x"]) for c in cases)
     log(f"[bench] ensemble of {len(cases)} boxes, {npart} particles, built in {time.perf_counter() - t0:.1f} s")
+    # tile shapes tuned on the box with the median particle count, given to all
+    counts = [len(c.particles["x"]) for c in cases]
+    ref = int(np.argsort(counts)[len(counts) // 2])
+    tiles = solvers[ref].tune_tiles(max_steps=12)
+    for s in solvers:
+        s.set_tiles(heat=tiles["heat"], velocity=tiles["velocity"])
+    log(f"[bench] tile shapes {tiles} tuned on box {ref}")
     batch = tpx.Batch(solvers)
     t0 = time.perf_counter()
     batch.step(args.warmup)
@@ -181,8 +188,10 @@ def ensemble_bench(args):
         "ms_per_step": round(1000.0 * t_bat / args.steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "ensemble: 64 x 64^3 boxes, particles ~ logU[1e4, 2e5], nu ~ U[0.01, 0.03], "
-                               "St ~ logU[0.1, 10], mass loading 0.3, heat off; batched kernels",
-                   "boxes": len(cases), "particles": npart},
+                               "St ~ logU[0.1, 10], mass loading 0.3, heat off; batched kernels, "
+                               "deposit tile shapes from the online tuner",
+                   "boxes": len(cases), "particles": npart,
+                   "deposit_tiles": {"heat": list(tiles["heat"]), "velocity": list(tiles["velocity"])}},
         "sequential_cells_per_s": round(seq, 1), "speedup_vs_sequential": round(value / seq, 3),
         "four_streams_cells_per_s": round(ncells * args.steps / t_str, 1),
         "roofline": None, "cpu_baseline": None}), flush=True)
@@ -297,8 +306,18 @@ def main():
     ncell = p.Nx_tot * p.Ny_tot * p.Nz_tot
     log(f"[bench] case {args.workload}: {p.Nx_tot}^3, {npart} particles, dt={p.dt:.5g} "
         f"(built in {time.perf_counter() - t0:.1f} s)")
+    # the deposit kernels' tile shapes, tuned inside steps of a second solver
+    # on the same case (so the timed steps are the same steps of the run)
+    t0 = time.perf_counter()
+    tuner = tpx.Solver(p, np_capacity=npart)
+    tuner.load_case(case)
+    tiles = tuner.tune_tiles(max_steps=12)
+    tuner.close()
+    del tuner
+    log(f"[bench] tile shapes {tiles} tuned in {time.perf_counter() - t0:.2f} s")
     s = tpx.Solver(p, np_capacity=npart)
     s.load_case(case)
+    s.set_tiles(heat=tiles["heat"], velocity=tiles["velocity"])
     log("[bench] uploaded and initialised")
     s.step(args.warmup)
     s.synchronize()
@@ -389,6 +408,7 @@ def main():
                                f"St=1, mass loading {case.info.get('phi')}, heat "
                                f"{'on' if p.Nu > 0 else 'off'}, FFT Poisson (epsilon {p.epsilon:g})",
                    "grid": p.Nx_tot, "particles": npart, "dt": p.dt,
+                   "deposit_tiles": {"heat": list(tiles["heat"]), "velocity": list(tiles["velocity"])},
                    "poisson_iterations_last_substep": n_it_last,
                    "poisson_iterations_mean": round(mean_its, 3)},
         "roofline": roofline,

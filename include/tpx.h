@@ -253,6 +253,20 @@ enum {
 int tpx_statistics_enable(tpx_ctx *ctx, int on);
 int tpx_statistics(tpx_ctx *ctx, double *out /* TPX_ST_COUNT */);
 
+/* ---- deposit tile shapes (no counterpart in the reference).
+ * The heat and velocity deposit kernels march `zc` planes with `rows` whole
+ * x-rows per block.  tpx_tune_tiles starts an online tuner: the next
+ * tpx_step calls launch those kernels with each candidate shape in turn
+ * (rows 1, 2, 4, 8 x zc 2..32 that tile the grid and fit in LDS), timed with
+ * events, and keep the fastest once each has run twice (results do not
+ * depend on the shape).  tpx_get_tiles writes {heat rows, heat zc, velocity rows,
+ * velocity zc, tuning still running}; 0 means the built-in shape.
+ * tpx_set_tiles sets the four values (0, 0 for the built-in shape), e.g. to
+ * give every box of an ensemble the shape tuned on one. */
+int tpx_tune_tiles(tpx_ctx *ctx);
+int tpx_get_tiles(tpx_ctx *ctx, int *out /* 5 */);
+int tpx_set_tiles(tpx_ctx *ctx, const int *in /* 4 */);
+
 int tpx_timing_enable(tpx_ctx *ctx, int on);
 int tpx_timing_read(tpx_ctx *ctx, double *ms /* TPX_K_COUNT */, int64_t *launches /* TPX_K_COUNT */);
 int tpx_synchronize(tpx_ctx *ctx);

@@ -847,32 +847,116 @@ void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_i
   pt.new_counted = count_new;
 }
 
-// Launch shape of the heat moment and velocity deposit kernels: whole x-rows per block (256
-// threads, or nx threads when nx > 256) and up to 32 output planes.
+// Launch shape of the heat moment and velocity deposit kernels: `rows` whole
+// x-rows per block and `zc` planes per z-march.  The built-in choice is 256
+// threads (nx when nx > 256) and up to 32 planes (8 for the velocity
+// deposit); a tuned or set shape replaces it.
 struct HeatShape { int threads, rows, zc; unsigned blocks; };
-static HeatShape heat_shape(const Geo &g) {
+static HeatShape deposit_shape(const tpx_ctx *c, int kind) {
+  const Geo &g = c->g;
+  const TileTune::Shape &set = kind == 0 ? c->tile.heat : c->tile.vel;
   HeatShape h;
-  h.rows = g.nx >= 256 ? 1 : std::min(256 / g.nx, g.ny);
+  h.rows = set.rows > 0 ? set.rows : (g.nx >= 256 ? 1 : std::min(256 / g.nx, g.ny));
+  h.zc = set.zc > 0 ? set.zc : std::min(kind == 0 ? 32 : 8, g.nz);
   h.threads = g.nx * h.rows;
-  h.zc = std::min(32, g.nz);
   h.blocks = (unsigned)((g.ny / h.rows) * (g.nz / h.zc));
   return h;
+}
+
+static size_t deposit_lds(const Geo &g, int kind, int rows) {
+  const size_t threads = (size_t)g.nx * rows;
+  return kind == 0 ? sizeof(double) * (9 * threads + 3 * (size_t)(rows + 2) * g.nx)
+                   : sizeof(double) * 6 * kVelChunk * threads;
+}
+
+bool tile_shape_valid(const tpx_ctx *c, int kind, int rows, int zc) {
+  const Geo &g = c->g;
+  return rows >= 1 && zc >= 1 && g.nx * rows <= kHeatMaxBlock && g.ny % rows == 0 && g.nz % zc == 0 &&
+         deposit_lds(g, kind, rows) <= c->max_lds;
+}
+
+void tile_tune_begin(tpx_ctx *c) {
+  TileTune &t = c->tile;
+  for (auto &p : t.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  t.pending.clear();
+  for (int k = 0; k < 2; ++k) {
+    t.cand[k].clear();
+    for (int rows = 1; rows <= 8; rows *= 2)
+      for (int zc = 2; zc <= 32; zc *= 2)
+        if (tile_shape_valid(c, k, rows, zc)) t.cand[k].push_back({rows, zc});
+    t.best_ms[k].assign(t.cand[k].size(), -1.0);
+    t.runs[k].assign(t.cand[k].size(), 0);
+    t.next[k] = 0;
+  }
+  t.active = true;
+}
+
+// the shape for this launch: the next candidate while tuning, with its events
+static void tile_pick(tpx_ctx *c, int kind, hipEvent_t *ev) {
+  TileTune &t = c->tile;
+  ev[0] = ev[1] = nullptr;
+  if (!t.active || g_rec || t.cand[kind].empty()) return;
+  const size_t i = t.next[kind]++ % t.cand[kind].size();
+  (kind == 0 ? t.heat : t.vel) = t.cand[kind][i];
+  if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess)
+    throw std::runtime_error("hipEventCreate");
+  t.pending.push_back({kind, i, ev[0], ev[1]});
+}
+
+void tile_tune_collect(tpx_ctx *c) {
+  TileTune &t = c->tile;
+  if (!t.active) return;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) throw std::runtime_error("hipStreamSynchronize");
+  for (auto &p : t.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      double &b = t.best_ms[p.kind][p.cand];
+      if (b < 0 || ms < b) b = ms;
+      ++t.runs[p.kind][p.cand];
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  t.pending.clear();
+  // done once each kernel that ran has timed all its candidates kRuns times
+  // (the fastest run of each counts); a kernel that did not run keeps the
+  // built-in shape
+  bool done = true;
+  for (int k = 0; k < 2; ++k) {
+    if (t.next[k] == 0) continue;
+    for (int r : t.runs[k]) done = done && r >= TileTune::kRuns;
+  }
+  const bool any = t.next[0] > 0 || t.next[1] > 0;
+  if (!done || !any) return;
+  for (int k = 0; k < 2; ++k) {
+    TileTune::Shape &s = k == 0 ? t.heat : t.vel;
+    if (t.next[k] == 0) {
+      s = {0, 0};
+      continue;
+    }
+    size_t best = 0;
+    for (size_t i = 1; i < t.cand[k].size(); ++i)
+      if (t.best_ms[k][i] < t.best_ms[k][best]) best = i;
+    s = t.cand[k][best];
+  }
+  t.active = false;
 }
 
 void launch_velocity_deposit(tpx_ctx *c) {
   Particles &pt = c->pt;
   timing_begin(c, TPX_K_DEPOSIT);
   const Geo &g = c->g;
-  HeatShape h = heat_shape(g);
-  const int zc = std::min(8, g.nz);
-  if (h.threads > kHeatMaxBlock || g.ny % h.rows || g.nz % zc)
-    throw std::logic_error("velocity deposit launch shape does not tile the grid");
-  const size_t lds = sizeof(double) * 6 * kVelChunk * (size_t)h.threads;
-  if (lds > c->max_lds) throw std::logic_error("velocity deposit needs more LDS than the device offers");
-  const unsigned blocks = (unsigned)((g.ny / h.rows) * (g.nz / zc));
-  tpx_launch<k_vel_planes>(blocks, h.threads, lds, c->stream, g, pt.start_int, pt.contrib, pt.cap, c->S[0], c->S[1],
-                                                        c->S[2], h.rows, zc);
+  hipEvent_t ev[2];
+  tile_pick(c, 1, ev);
+  HeatShape h = deposit_shape(c, 1);
+  if (!tile_shape_valid(c, 1, h.rows, h.zc))
+    throw std::logic_error("velocity deposit launch shape does not tile the grid or fit in LDS");
+  const size_t lds = deposit_lds(g, 1, h.rows);
+  if (ev[0]) check(hipEventRecord(ev[0], c->stream));
+  tpx_launch<k_vel_planes>(h.blocks, h.threads, lds, c->stream, g, pt.start_int, pt.contrib, pt.cap, c->S[0], c->S[1],
+                                                        c->S[2], h.rows, h.zc);
   TPX_LAUNCH_CHECK();
+  if (ev[1]) check(hipEventRecord(ev[1], c->stream));
   timing_end(c, TPX_K_DEPOSIT);
 }
 
@@ -909,9 +993,11 @@ void launch_heat_exchange(tpx_ctx *c, int stage, const double *rho, int p0_slot)
   timing_begin(c, TPX_K_DEPOSIT);
   if (pt.n > 0) {
     const Geo &g = c->g;
-    HeatShape h = heat_shape(g);
-    if (h.threads > kHeatMaxBlock || g.ny % h.rows || g.nz % h.zc)
-      throw std::logic_error("heat deposit launch shape does not tile the grid");
+    hipEvent_t ev[2];
+    tile_pick(c, 0, ev);
+    HeatShape h = deposit_shape(c, 0);
+    if (!tile_shape_valid(c, 0, h.rows, h.zc))
+      throw std::logic_error("heat deposit launch shape does not tile the grid or fit in LDS");
     const int s = stage == TPX_INT ? pt.int_buf() : pt.st_new;
     HeatFusedArgs a{};
     a.g = g;
@@ -927,10 +1013,11 @@ void launch_heat_exchange(tpx_ctx *c, int stage, const double *rho, int p0_slot)
     a.t3 = c->dep_tmp;
     a.rows = h.rows;
     a.zc = h.zc;
-    size_t lds = sizeof(double) * (9 * (size_t)h.threads + 3 * (size_t)(h.rows + 2) * g.nx);
-    if (lds > c->max_lds) throw std::logic_error("heat deposit needs more LDS than the device offers");
+    const size_t lds = deposit_lds(g, 0, h.rows);
+    if (ev[0]) check(hipEventRecord(ev[0], c->stream));
     tpx_launch<k_heat_fused>(h.blocks, h.threads, lds, c->stream, a);
     TPX_LAUNCH_CHECK();
+    if (ev[1]) check(hipEventRecord(ev[1], c->stream));
     unsigned nb = grid1d(g.n, kBlock);
     tpx_launch<k_heat_combine>(nb, kBlock, 0, c->stream, g, c->dep_tmp, c->Q, c->partials);
     TPX_LAUNCH_CHECK();

@@ -417,6 +417,7 @@ void release(tpx_ctx *c) {
   f(pt.inv_int); f(pt.inv_new);
   for (auto &p : pt.spare) f(p);
   f(pt.itmp);
+  for (auto &e : c->tile.pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto &e : c->tm.pool) (void)hipEventDestroy(e);
   for (auto &r : c->tm.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -860,6 +861,7 @@ int tpx_step(tpx_ctx *ctx, int nsteps) {
       do_params_update(ctx);
       for (int rk = 0; rk < 4; ++rk) fused_substep(ctx, rk);
       step_end(ctx);
+      if (ctx->tile.active) tile_tune_collect(ctx);
     }
     sync(ctx);
     return TPX_OK;
@@ -1312,6 +1314,7 @@ int tpx_batch_step(tpx_batch *b, int nsteps) {
     hip_check(hipSetDevice(b->device), "hipSetDevice");
     for (tpx_ctx *c : b->ctx) {  // work queued on the boxes' own streams comes first
       if (c->tm.on) throw std::logic_error("kernel timing is not recorded in a batch");
+      if (c->tile.active) throw std::logic_error("a box is still tuning its tile shapes: step it alone first");
       sync(c);
     }
     for (int s = 0; s < nsteps; ++s) {
@@ -1557,6 +1560,40 @@ int tpx_get_stream(tpx_ctx *ctx, void **stream) {
   return guarded(ctx, [&]() {
     if (!stream) throw std::invalid_argument("null");
     *stream = (void *)ctx->stream;
+    return TPX_OK;
+  });
+}
+
+// Deposit tile shapes: tuned online inside the next steps, or set
+int tpx_tune_tiles(tpx_ctx *ctx) {
+  return guarded(ctx, [&]() {
+    tile_tune_begin(ctx);
+    return TPX_OK;
+  });
+}
+
+int tpx_get_tiles(tpx_ctx *ctx, int *out) {
+  return guarded(ctx, [&]() {
+    if (!out) throw std::invalid_argument("null output");
+    const TileTune &t = ctx->tile;
+    out[0] = t.heat.rows; out[1] = t.heat.zc; out[2] = t.vel.rows; out[3] = t.vel.zc; out[4] = t.active ? 1 : 0;
+    return TPX_OK;
+  });
+}
+
+int tpx_set_tiles(tpx_ctx *ctx, const int *in) {
+  return guarded(ctx, [&]() {
+    if (!in) throw std::invalid_argument("null input");
+    for (int k = 0; k < 2; ++k) {
+      const int rows = in[2 * k], zc = in[2 * k + 1];
+      const bool builtin = rows == 0 && zc == 0;
+      if (!builtin && !tile_shape_valid(ctx, k, rows, zc))
+        throw std::invalid_argument("tile shape does not tile the grid or fit in LDS");
+    }
+    TileTune &t = ctx->tile;
+    t.heat = {in[0], in[1]};
+    t.vel = {in[2], in[3]};
+    t.active = false;
     return TPX_OK;
   });
 }

@@ -151,6 +151,24 @@ struct AdjointTape {
   GasAdjoint ga{};
 };
 
+// Tile shapes of the two deposit kernels (k_heat_fused, k_vel_planes): x-rows
+// per block and planes per z-march (0: the built-in choice), and the online
+// tuner that picks them.  Both kernels sum each cell's own particles in bin
+// order, so the results do not depend on the shape and the tuner can time
+// the candidates inside real steps.
+struct TileTune {
+  struct Shape { int rows, zc; };
+  Shape heat{0, 0}, vel{0, 0};
+  bool active = false;
+  std::vector<Shape> cand[2];           // 0 heat, 1 velocity
+  std::vector<double> best_ms[2];       // fastest time seen per candidate (-1: none yet)
+  std::vector<int> runs[2];             // times each candidate was timed
+  static constexpr int kRuns = 2;       // each candidate is timed this often before a choice
+  size_t next[2] = {0, 0};
+  struct Pending { int kind; size_t cand; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+};
+
 struct Timing {
   bool on = false;
   struct Rec { int cls; hipEvent_t a, b; };
@@ -198,6 +216,7 @@ struct tpx_ctx {
   double *scal = nullptr;         // device scalars
   double *partials = nullptr;     // block partial results, 3 x n_partials
   bool stats_on = false;          // capture the statistics' last-substep quantities
+  tpx::TileTune tile;
   double *st_part = nullptr;      // statistics partials (16 x st_stride) and results
   long st_stride = 0;
   long n_partials = 0;
@@ -289,6 +308,11 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas);
 void stats_capture_particles(tpx_ctx *c, const double *const ru_int[3], const double *rho_int);
 void stats_capture_velocity(tpx_ctx *c, const double *const ru_new[3], const double *rho_new);
 void stats_reduce(tpx_ctx *c, const double *rho, const double *const ru[3], double raw[16]);
+
+// k_particle.hip: deposit tile shapes and their tuner
+bool tile_shape_valid(const tpx_ctx *c, int kind, int rows, int zc);  // kind 0 heat, 1 velocity
+void tile_tune_begin(tpx_ctx *c);
+void tile_tune_collect(tpx_ctx *c);  // after a step: read the timings, choose once every candidate ran
 
 // k_adjoint_gas.hip: one substep of the gas sweep in two halves (around the
 // particles' stage) and the step boundary

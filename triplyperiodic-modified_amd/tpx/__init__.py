@@ -45,6 +45,7 @@ EXPORTS = [
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
     "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
     "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_set_gas_tape", "tpx_statistics_enable", "tpx_statistics",
+    "tpx_tune_tiles", "tpx_get_tiles", "tpx_set_tiles",
     "tpx_adjoint_release",
     "tpx_batch_create", "tpx_batch_step", "tpx_batch_destroy", "tpx_batch_error", "tpx_batch_last_error",
     "tpx_field_sum",
@@ -103,6 +104,9 @@ def load(path: str = LIB_PATH):
     L.tpx_adjoint_gradient_full.argtypes = [P, DP, DP, DP, DP, DP, DP]
     L.tpx_adjoint_set_gas_tape.argtypes = [P, I]
     L.tpx_statistics_enable.argtypes = [P, I]
+    L.tpx_tune_tiles.argtypes = [P]
+    L.tpx_get_tiles.argtypes = [P, IP]
+    L.tpx_set_tiles.argtypes = [P, IP]
     L.tpx_statistics.argtypes = [P, DP]
     L.tpx_adjoint_release.argtypes = [P]
     L.tpx_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
@@ -295,6 +299,27 @@ class Solver:
 
     def adjoint_release(self) -> None:
         self._ok(self.L.tpx_adjoint_release(self.h), "adjoint_release")
+
+    # ---- deposit tile shapes ----
+    def tune_tiles(self, max_steps: int = 12) -> dict:
+        """Tune the deposit kernels' tile shapes inside real steps (they
+        advance the run; the results do not depend on the shape) and return
+        the chosen shapes."""
+        self._ok(self.L.tpx_tune_tiles(self.h), "tune_tiles")
+        for _ in range(max_steps):
+            self.step(1)
+            if not self.tiles()["tuning"]:
+                break
+        return self.tiles()
+
+    def tiles(self) -> dict:
+        out = np.zeros(5, dtype=np.int32)
+        self._ok(self.L.tpx_get_tiles(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "get_tiles")
+        return {"heat": (int(out[0]), int(out[1])), "velocity": (int(out[2]), int(out[3])), "tuning": bool(out[4])}
+
+    def set_tiles(self, heat=(0, 0), velocity=(0, 0)) -> None:
+        arr = np.array([heat[0], heat[1], velocity[0], velocity[1]], dtype=np.int32)
+        self._ok(self.L.tpx_set_tiles(self.h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "set_tiles")
 
     # ---- grid::Statistics on the device ----
     def statistics_enable(self, on: bool = True) -> None:
