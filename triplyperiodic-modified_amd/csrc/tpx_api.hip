@@ -915,6 +915,13 @@ struct tpx_batch {
   hipEvent_t ev[kRing] = {};
   int slot = 0;
   double *d_state = nullptr, *h_state = nullptr;  // per box: iterations run, stopped
+  // one Poisson iteration of every box as a captured graph, replayed for each
+  // iteration of each solve; its argument table lives at a fixed address and
+  // is refreshed (stream-ordered) at the start of each solve
+  char *h_iter = nullptr, *d_iter = nullptr;
+  size_t iter_cap = 0;
+  hipGraphExec_t iter_exec = nullptr;
+  std::vector<char> iter_sig;  // the launches the graph was captured from
   std::string err;
 };
 
@@ -931,13 +938,28 @@ void batch_release(tpx_batch *b) {
   }
   if (b->d_state) (void)hipFree(b->d_state);
   if (b->h_state) (void)hipHostFree(b->h_state);
+  if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
+  if (b->h_iter) (void)hipHostFree(b->h_iter);
+  if (b->d_iter) (void)hipFree(b->d_iter);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
-// record fn(ctx) for every box, then launch each recorded kernel once for all
+// A recorded segment: each kernel once for all boxes, with its place in the
+// packed argument table.
+struct BatchPlan {
+  struct K {
+    void (*launch)(dim3, dim3, size_t, hipStream_t, const char *, unsigned, unsigned);
+    dim3 grid, block;
+    size_t shmem, off, stride;
+  };
+  std::vector<K> k;
+  size_t total = 0;
+};
+
+// record fn(ctx) for every box and lay out the table
 template <class Fn>
-void batch_run(tpx_batch *b, Fn &&fn) {
+void batch_record(tpx_batch *b, Fn &&fn, BatchPlan &plan) {
   const int nb = (int)b->ctx.size();
   for (int i = 0; i < nb; ++i) {
     b->rec[i].clear();
@@ -952,55 +974,120 @@ void batch_run(tpx_batch *b, Fn &&fn) {
   }
   const Recorder &r0 = b->rec[0];
   const size_t nk = r0.recs.size();
-  if (nk == 0) return;
-  std::vector<size_t> off(nk), stride(nk);
-  size_t total = 0;
+  plan.k.clear();
+  plan.total = 0;
   for (size_t j = 0; j < nk; ++j) {
+    const Recorder::Rec &a = r0.recs[j];
+    BatchPlan::K k{a.launch, a.grid, a.block, a.shmem, 0, 0};
     for (int i = 1; i < nb; ++i) {
-      const Recorder::Rec &a = r0.recs[j];
       if (b->rec[i].recs.size() != nk) throw std::logic_error("boxes of a batch launch different kernel sequences");
       const Recorder::Rec &o = b->rec[i].recs[j];
       if (o.launch != a.launch || o.block.x != a.block.x || o.block.y != a.block.y || o.block.z != a.block.z ||
           o.shmem != a.shmem || o.grid.z != a.grid.z)
         throw std::logic_error("boxes of a batch launch different kernel sequences");
+      // particle kernels: the largest box's grid, the others' extra blocks exit
+      k.grid.x = std::max(k.grid.x, o.grid.x);
+      k.grid.y = std::max(k.grid.y, o.grid.y);
     }
-    stride[j] = (r0.recs[j].size + 15) / 16 * 16;
-    off[j] = total;
-    total += (stride[j] * nb + 255) / 256 * 256;
+    k.stride = (a.size + 15) / 16 * 16;
+    k.off = plan.total;
+    plan.total += (k.stride * nb + 255) / 256 * 256;
+    plan.k.push_back(k);
   }
+}
+
+void batch_pack(const tpx_batch *b, const BatchPlan &plan, char *h) {
+  const int nb = (int)b->ctx.size();
+  for (size_t j = 0; j < plan.k.size(); ++j)
+    for (int i = 0; i < nb; ++i) {
+      const Recorder::Rec &o = b->rec[i].recs[j];
+      memcpy(h + plan.k[j].off + (size_t)i * plan.k[j].stride, b->rec[i].bytes.data() + o.off, o.size);
+    }
+}
+
+void batch_launch(tpx_batch *b, const BatchPlan &plan, const char *d) {
+  const unsigned nb = (unsigned)b->ctx.size();
+  for (const BatchPlan::K &k : plan.k) {
+    k.launch(k.grid, k.block, k.shmem, b->stream, d + k.off, (unsigned)k.stride, nb);
+    TPX_LAUNCH_CHECK();
+  }
+}
+
+// record fn(ctx) for every box, then launch each recorded kernel once for all
+// boxes, the arguments from one table copied in one transfer
+template <class Fn>
+void batch_run(tpx_batch *b, Fn &&fn) {
+  BatchPlan plan;
+  batch_record(b, fn, plan);
+  if (plan.k.empty()) return;
   const int s = b->slot;
   b->slot = (s + 1) % tpx_batch::kRing;
   hip_check(hipEventSynchronize(b->ev[s]), "hipEventSynchronize");  // the slot's last copy has been read
-  if (total > b->cap[s]) {
+  if (plan.total > b->cap[s]) {
     hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
     if (b->h_tab[s]) hip_check(hipHostFree(b->h_tab[s]), "hipHostFree");
     if (b->d_tab[s]) hip_check(hipFree(b->d_tab[s]), "hipFree");
     b->h_tab[s] = nullptr;
     b->d_tab[s] = nullptr;
     b->cap[s] = 0;
-    const size_t want = std::max<size_t>(total * 2, 1 << 16);
+    const size_t want = std::max<size_t>(plan.total * 2, 1 << 16);
     hip_check(hipHostMalloc((void **)&b->h_tab[s], want), "hipHostMalloc");
     if (hipMalloc((void **)&b->d_tab[s], want) != hipSuccess) throw std::bad_alloc();
     b->cap[s] = want;
   }
-  char *h = b->h_tab[s];
-  for (size_t j = 0; j < nk; ++j)
-    for (int i = 0; i < nb; ++i) {
-      const Recorder::Rec &o = b->rec[i].recs[j];
-      memcpy(h + off[j] + (size_t)i * stride[j], b->rec[i].bytes.data() + o.off, o.size);
-    }
-  hip_check(hipMemcpyAsync(b->d_tab[s], h, total, hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
+  batch_pack(b, plan, b->h_tab[s]);
+  hip_check(hipMemcpyAsync(b->d_tab[s], b->h_tab[s], plan.total, hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
   hip_check(hipEventRecord(b->ev[s], b->stream), "hipEventRecord");
-  for (size_t j = 0; j < nk; ++j) {
-    dim3 grid = r0.recs[j].grid;
-    for (int i = 1; i < nb; ++i) {  // particle kernels: the largest box's grid, the others' extra blocks exit
-      grid.x = std::max(grid.x, b->rec[i].recs[j].grid.x);
-      grid.y = std::max(grid.y, b->rec[i].recs[j].grid.y);
-    }
-    r0.recs[j].launch(grid, r0.recs[j].block, r0.recs[j].shmem, b->stream, b->d_tab[s] + off[j],
-                      (unsigned)stride[j], (unsigned)nb);
-    TPX_LAUNCH_CHECK();
+  batch_launch(b, plan, b->d_tab[s]);
+}
+
+// The Poisson iteration of every box, its table refreshed for this solve and
+// the graph of its launches captured if the launches changed.
+void batch_iteration_graph(tpx_batch *b) {
+  BatchPlan plan;
+  batch_record(b, [](tpx_ctx *c, int) { launch_poisson_iteration(c, c->rho[c->rho_new]); }, plan);
+  if (plan.total > b->iter_cap) {
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
+    b->iter_exec = nullptr;
+    if (b->h_iter) hip_check(hipHostFree(b->h_iter), "hipHostFree");
+    if (b->d_iter) hip_check(hipFree(b->d_iter), "hipFree");
+    b->h_iter = nullptr;
+    b->d_iter = nullptr;
+    b->iter_cap = 0;
+    const size_t want = std::max<size_t>(plan.total * 2, 1 << 16);
+    hip_check(hipHostMalloc((void **)&b->h_iter, want), "hipHostMalloc");
+    if (hipMalloc((void **)&b->d_iter, want) != hipSuccess) throw std::bad_alloc();
+    b->iter_cap = want;
   }
+  // the previous solve's copy has completed: batch_solve waits on the stream
+  // before it returns
+  batch_pack(b, plan, b->h_iter);
+  hip_check(hipMemcpyAsync(b->d_iter, b->h_iter, plan.total, hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
+  std::vector<char> sig(sizeof(BatchPlan::K) * plan.k.size());
+  for (size_t j = 0; j < plan.k.size(); ++j) {
+    BatchPlan::K k = plan.k[j];
+    memcpy(sig.data() + j * sizeof(k), &k, sizeof(k));
+  }
+  if (b->iter_exec && sig == b->iter_sig) return;
+  if (b->iter_exec) {
+    (void)hipGraphExecDestroy(b->iter_exec);
+    b->iter_exec = nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  try {
+    batch_launch(b, plan, b->d_iter);
+  } catch (...) {
+    (void)hipStreamEndCapture(b->stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(b->stream, &graph), "hipStreamEndCapture");
+  const hipError_t e = hipGraphInstantiate(&b->iter_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  hip_check(e, "hipGraphInstantiate");
+  b->iter_sig = sig;
 }
 
 void batch_solve(tpx_batch *b, int rk) {
@@ -1011,11 +1098,11 @@ void batch_solve(tpx_batch *b, int rk) {
     guess = std::max(guess, c->iter_guess[rk & 3]);
   }
   batch_run(b, [](tpx_ctx *c, int) { launch_solve_begin(c); });
+  batch_iteration_graph(b);
   int queued = 0, batch = guess;
   while (queued < itmax) {
     const int n = std::min(batch, itmax - queued);
-    for (int q = 0; q < n; ++q)
-      batch_run(b, [](tpx_ctx *c, int) { launch_poisson_iteration(c, c->rho[c->rho_new]); });
+    for (int q = 0; q < n; ++q) hip_check(hipGraphLaunch(b->iter_exec, b->stream), "hipGraphLaunch");
     queued += n;
     batch_run(b, [&](tpx_ctx *c, int i) { launch_solve_state(c, b->d_state + 2 * i); });
     hip_check(hipMemcpyAsync(b->h_state, b->d_state, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, b->stream),
