@@ -157,9 +157,10 @@ int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency);
  * tpx_adjoint_record advances the context nsteps whole steps exactly as
  * tpx_step would, taping the gas stage fields each particle advance reads and
  * the gas at each step's start (RU, P), checkpointing the particles every
- * `every` steps, and returns J.  For runs without heat exchange, with one-
- * or two-way coupling (under two-way coupling the drag deposits are taped
- * too, and the particles' stages are reversed between the gas substeps).
+ * `every` steps, and returns J.  For one- or two-way coupling, with or
+ * without the heat exchange (under two-way coupling the drag deposits are
+ * taped too, with the heat exchange every substep's RHS_Part_Temp, its sum
+ * and P0, and the particles' stages are reversed between the gas substeps).
  * tpx_adjoint_gradient then writes dJ/dtau_p for every particle and, if
  * dJ_dstate is not null, dJ/d of the particles' state when recording began
  * (6 arrays of np: x, y, z, u, v, w), both in the context's present particle
@@ -174,14 +175,22 @@ int tpx_ensemble_step(tpx_ctx *const *ctxs, int n, int nsteps, int concurrency);
  * (the momentum, density, projection and Poisson-iteration transposes), and
  * leaves the context's gas where the recorded run ended.  The Poisson solves
  * are differentiated with their recorded iteration counts.
- * Under two-way coupling tpx_adjoint_gradient runs the gas sweep as well
- * (the gas depends on the particles).  tpx_adjoint_release frees the tape
- * (nsteps x 20 grid fields, + 12 under two-way coupling) and the sweep's
- * work arrays. */
+ * Under two-way coupling or with the heat exchange tpx_adjoint_gradient runs
+ * the gas sweep as well (the gas depends on the particles).
+ * tpx_adjoint_gradient_heat (runs with the heat exchange) writes, from the
+ * last sweep (running one if none has run since the recording), dJ/d of each
+ * particle's temperature when recording began and dJ/d of its absorption
+ * efficiency eps (np values each, in the present particle order), and dJ/d
+ * of the initial thermodynamic pressure P0 (one value); the recording's
+ * first RHS_T is taken as the exchange of its initial state, as
+ * tpx_initialize and every step's end form it.  Any output may be null.
+ * tpx_adjoint_release frees the tape (nsteps x 20 grid fields, + 12 under
+ * two-way coupling, + 4 with the heat exchange) and the sweep's work arrays. */
 int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J);
 int tpx_adjoint_gradient(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate);
 int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, double *dJ_dA, double *dJ_dRU,
                               double *dJ_dRho, double *dJ_dP);
+int tpx_adjoint_gradient_heat(tpx_ctx *ctx, double *dJ_dT, double *dJ_deps, double *dJ_dP0);
 /* How tpx_adjoint_record keeps the gas for the gas sweep: 0 (default) keeps
  * every substep's momentum, density and Poisson iterates when they fit in
  * half the free device memory, else recomputes each step from its start;

@@ -137,8 +137,10 @@ def test_torch_shim_backward(gpu_available):
     assert np.array_equal(tau.grad.numpy(), g)
 
 
-def test_heat_exchange_is_refused(gpu_available):
-    case = ic.make_case(16, 200, nu=0.02, seed=3, St=2.0, heat=True, twc=False)
+def test_heat_gradient_needs_a_heat_exchange_run(gpu_available):
+    case = ic.make_case(16, 200, nu=0.02, seed=3, St=2.0, heat=False, twc=False)
     s = solver_from_case(case)
-    with pytest.raises(tpx.TpxError, match="heat exchange"):
-        s.adjoint_record(2, 1)
+    s.adjoint_record(2, 1)
+    with pytest.raises(tpx.TpxError, match="no heat exchange"):
+        s.adjoint_gradient_heat()
+    s.adjoint_release()

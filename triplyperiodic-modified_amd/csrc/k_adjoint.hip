@@ -19,6 +19,20 @@
 // (u_g of each axis is linear along its own axis between two faces and piecewise
 // constant across), and dJ/dtau_p = -sum_s b(s).(u_g(Y_s) - u_Y(s)) / tau^2.
 // The periodic wraps shift positions by constants and drop out of the adjoint.
+//
+// With the heat exchange the state gains the particle temperature T: its
+// stage right-hand side is iS (h + alpha eps) (particle.cpp:477-492), h =
+// RHS_T = K (Tg(x; Rho, P0) - T) (particle.cpp:430-474, K = Nu pi Dp k, Tg
+// the cloud-in-cell gather of T = P0/(R Rho)) evaluated by the heat exchange
+// at the end of the previous substep, i.e. at this stage's state with the
+// gas of this substep's start.  The same h is deposited (Q = D(x) h, the
+// gather's weights) into the Q of the previous substep's divergence and its
+// sum into the next Update_P0, so the reverse of h(Y) (k_adj_heat) takes
+//   lam_h = iS lam_kT (the temperature update) + D(x)^T lam_Q + lam_(sum Q)
+// and gives lam_T -= K lam_h, lam_x += K lam_h dTg/dx + h (dD/dx)^T lam_Q,
+// lam_Rho -= K lam_h w T/Rho at the gather's corners and lam_P0 += K lam_h
+// Tg/P0.  It runs between the gas sweep's divergence and its RK bookkeeping,
+// where lam_Q of the substep and lam_Rho_new are at hand.
 #include <math.h>
 #include <stdint.h>
 
@@ -77,16 +91,70 @@ __device__ __forceinline__ void gather(const Geo &g, const Ix &ix, Gas gs, const
   dug[2] = (uz1 - uz0) * g.idz;
 }
 
+// The cloud-in-cell corners of x with the temperature/concentration weights
+// (particle.cpp:402-456: ii = round(x idx), xl = x idx - ii + 1/2); corner
+// (ii - 1 + ox, jj - 1 + oy, kk - 1 + oz) is index ox + 2 oy + 4 oz.
+struct Cic {
+  long c[8];
+  double xl, yl, zl;
+  __device__ __forceinline__ Cic(const Geo &g, const Ix &ix, double x, double y, double z) {
+    const double fx = x * g.idx, fy = y * g.idy, fz = z * g.idz;
+    const int ii = (int)(round(fx)), jj = (int)(round(fy)), kk = (int)(round(fz));
+    xl = fx - ii + 0.5; yl = fy - jj + 0.5; zl = fz - kk + 0.5;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) c[e] = ix(ii - 1 + (e & 1), jj - 1 + ((e >> 1) & 1), kk - 1 + (e >> 2));
+  }
+  __device__ __forceinline__ double w(int e) const {
+    return ((e & 1) ? xl : 1 - xl) * ((e & 2) ? yl : 1 - yl) * ((e & 4) ? zl : 1 - zl);
+  }
+  // the interpolation of corner values v as nested lerps along x, y, z (the
+  // heat exchange's order), and its derivatives along x, y, z (per cell width)
+  __device__ __forceinline__ double lerp(const double v[8], double d[3]) const {
+    double lx[4], dx[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      lx[r] = fma(xl, v[2 * r + 1] - v[2 * r], v[2 * r]);
+      dx[r] = v[2 * r + 1] - v[2 * r];
+    }
+    const double t0 = fma(yl, lx[1] - lx[0], lx[0]), t1 = fma(yl, lx[3] - lx[2], lx[2]);
+    const double e0 = fma(yl, dx[1] - dx[0], dx[0]), e1 = fma(yl, dx[3] - dx[2], dx[2]);
+    const double f0 = lx[1] - lx[0], f1 = lx[3] - lx[2];
+    d[0] = fma(zl, e1 - e0, e0);
+    d[1] = fma(zl, f1 - f0, f0);
+    d[2] = t1 - t0;
+    return fma(zl, t1 - t0, t0);
+  }
+};
+
+// RHS_T = K (Tg - T) at x, with T = (P0/R)/Rho on the cells, as the heat
+// exchange forms it (k_particle.hip)
+__device__ __forceinline__ double heat_rhs(const Cic &w, const double *rho, double at, double Tp, double K) {
+  double v[8], d[3];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = at / rho[w.c[e]];
+  return (w.lerp(v, d) - Tp) * K;
+}
+
+struct HeatParams {
+  const double *p0;       // P0_int of the step's four substeps (device)
+  const double *eps;      // per particle, or null
+  double epsp;            // scalar eps when eps is null
+  double iS, K, R, Dp, I0;
+  int on;
+};
+
 struct StepArgs {
   Geo g;
   const double *gas;   // 4 substeps x 4 fields x n
-  const double *yin[6];  // base x, y, z, u, v, w at the step's start
-  double *y[6];          // the step's result (may be yin)
-  double *stage;       // non-null: the 4 stage states, 24 arrays of np
+  const double *yin[7];  // base x, y, z, u, v, w (and T) at the step's start
+  double *y[7];          // the step's result (may be yin)
+  double *stage;       // non-null: the 4 stage states, 4 x nq arrays of np
   const double *tau;   // per particle, or null
   double tp;           // scalar tau when tau is null
   double c_pre[4], c_post[4];
   double g3[3], L[3];
+  HeatParams h;
+  int nq;
   long np, ncell;
 };
 
@@ -100,14 +168,27 @@ struct k_adj_step_fwd {
     if (p >= a.np) return;
     const Geo &g = a.g;
     Ix ix = make_ix(g);
-    double b[6], vi[6], v1[6];
+    double b[7], vi[7], v1[7];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) b[q] = vi[q] = v1[q] = a.yin[q][p];
+    for (int q = 0; q < 7; ++q) b[q] = vi[q] = v1[q] = q < a.nq ? a.yin[q][p] : 0.0;
     const double itp = a.tau ? 1. / a.tau[p] : 1. / a.tp;
+    double alpha = 0.0;
+    if (a.h.on) {
+      const double epsp = a.h.eps ? a.h.eps[p] : a.h.epsp;
+      alpha = 0.25 * epsp * TPX_PI * a.h.Dp * a.h.Dp * a.h.I0;
+    }
     for (int rk = 0; rk < 4; ++rk) {
       if (a.stage)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) a.stage[(rk * 6 + q) * a.np + p] = vi[q];
+        for (int q = 0; q < 7; ++q)
+          if (q < a.nq) a.stage[(rk * a.nq + q) * a.np + p] = vi[q];
+      if (a.h.on) {  // update_Temp with the RHS_T of this stage's state
+        const double *rho = a.gas + (long)rk * 4 * a.ncell + 3 * a.ncell;
+        const double rt = heat_rhs(Cic(g, ix, vi[0], vi[1], vi[2]), rho, a.h.p0[rk] / a.h.R, vi[6], a.h.K);
+        const double n1t = v1[6] + a.c_post[rk] * a.h.iS * (rt + alpha);
+        vi[6] = rk == 3 ? n1t : b[6] + a.c_pre[rk] * a.h.iS * (rt + alpha);
+        v1[6] = n1t;
+      }
       double ug[3], dug[3];
       gather(g, ix, Gas{a.gas + (long)rk * 4 * a.ncell, a.ncell}, vi, ug, dug);
       double nv[6], n1[6];
@@ -139,7 +220,8 @@ struct k_adj_step_fwd {
       }
     }
 #pragma unroll
-    for (int q = 0; q < 6; ++q) a.y[q][p] = v1[q];
+    for (int q = 0; q < 7; ++q)
+      if (q < a.nq) a.y[q][p] = v1[q];
   }
 };
 
@@ -225,9 +307,9 @@ struct k_adj_step_rev {
 struct StageArgs {
   Geo g;
   const double *gas;     // this substep's taped gas (RU_int 3, Rho_int)
-  const double *stage;   // the state entering this substep: 6 arrays of np
-  const double *lend;    // 6 x np: adjoint of the step's result
-  double *lb, *lnext;    // 6 x np each: base accumulator; adjoint of the next stage's state in, of this one's out
+  const double *stage;   // the state entering this substep: nq arrays of np
+  const double *lend;    // nq x np: adjoint of the step's result
+  double *lb, *lnext;    // nq x np each: base accumulator; adjoint of the next stage's state in, of this one's out
   double *dtau;
   double *lup;           // the gather's transpose of dJ/du_g: 3 face fields of this substep, or null
   const double *ls[3];   // lam_RHS of this substep's momentum update (3 face fields), or null
@@ -236,6 +318,12 @@ struct StageArgs {
   double tp;
   double c_pre, c_post;  // dt pre, dt post of this substep
   int last;              // the fourth substep
+  // heat exchange (nq = 7): the temperature row
+  double *lh;            // out: iS lam_kT, the adjoint of this stage's RHS_T through the update
+  double *deps;          // dJ/deps accumulated
+  double alpha_eps;      // d alpha / d eps = pi Dp^2 I0 / 4
+  double iS;
+  int nq;
   long np, ncell;
 };
 
@@ -247,10 +335,11 @@ struct k_adj_stage_rev {
     const Geo &g = a.g;
     Ix ix = make_ix(g);
     const double itp = 1. / (a.tau ? a.tau[p] : a.tp);
-    double Y[6], lk[6];
+    double Y[6], lk[7];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      Y[q] = a.stage[q * a.np + p];
+    for (int q = 0; q < 7; ++q) {
+      if (q == a.nq) break;
+      if (q < 6) Y[q] = a.stage[q * a.np + p];
       const double le = a.lend[q * a.np + p];
       if (a.last) {
         lk[q] = a.c_post * le;
@@ -291,7 +380,92 @@ struct k_adj_stage_rev {
 #pragma unroll
     for (int q = 0; q < 6; ++q) a.lnext[q * a.np + p] = ln[q];
     a.dtau[p] += dtau;
+    if (a.nq == 7) {  // k_T = iS (RHS_T + alpha eps): T enters only through RHS_T (k_adj_heat)
+      a.lnext[6 * a.np + p] = 0.0;
+      a.lh[p] = a.iS * lk[6];
+      a.deps[p] += a.iS * lk[6] * a.alpha_eps;
+    }
     if (a.lup) gather_transpose(g, ix, Y, lug, a.lup, a.ncell);
+  }
+};
+
+// The reverse of one heat exchange h(Y) = RHS_T at the state Y (with its
+// deposit Q = D(x) h and sum Q): see the header.  lam (nq x np, T row 6)
+// is the adjoint of Y, accumulated; lam_P0 of the exchange goes to the block
+// partials.
+struct HeatRevArgs {
+  Geo g;
+  const double *Y;       // nq x np: x, y, z at rows 0-2, T at row 6
+  const double *rho;     // Rho of the exchange
+  const double *p0;      // P0 of the exchange (device)
+  const double *lq;      // adjoint of the deposit Q, or null
+  const double *lsq;     // adjoint of sum Q (device scalar)
+  const double *lh;      // per particle: adjoint of RHS_T through the temperature update
+  double *lam;
+  double *lrho;          // adjoint of Rho, accumulated (atomics)
+  double *partials;
+  double K, R;
+  long np;
+};
+
+struct k_adj_heat {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const HeatRevArgs &a) {
+    const long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    double lp0 = 0.0;
+    if (p < a.np) {
+      const Geo &g = a.g;
+      Ix ix = make_ix(g);
+      const double x = a.Y[p], y = a.Y[a.np + p], z = a.Y[2 * a.np + p], Tp = a.Y[6 * a.np + p];
+      const Cic w(g, ix, x, y, z);
+      const double P0 = a.p0[0], at = P0 / a.R;
+      double T[8], dT[3];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) T[e] = at / a.rho[w.c[e]];
+      const double Tg = w.lerp(T, dT);
+      const double h = (Tg - Tp) * a.K;
+      double lqi = 0.0, dq[3] = {0.0, 0.0, 0.0};
+      if (a.lq) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = a.lq[w.c[e]];
+        lqi = w.lerp(v, dq);
+      }
+      const double lh = a.lh[p] + a.lsq[0] + lqi;
+      const double gT = a.K * lh;  // adjoint of Tg
+      const double id[3] = {g.idx, g.idy, g.idz};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) a.lam[d * a.np + p] += (gT * dT[d] + h * dq[d]) * id[d];
+      a.lam[6 * a.np + p] -= gT;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) atomicAdd(a.lrho + w.c[e], -gT * w.w(e) * T[e] / a.rho[w.c[e]]);
+      lp0 = gT * Tg / P0;
+    }
+    __shared__ double sh[kBlock];
+    sh[threadIdx.x] = lp0;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = sh[0];
+  }
+};
+
+// acc[0] += sum of partials (one block)
+struct k_adj_acc1 {
+  static constexpr int kBounds = 1024;
+  static __device__ __forceinline__ void run(const double *partials, long nparts, double *acc) {
+    __shared__ double sh[1024];
+    double v = 0.0;
+    for (long q = threadIdx.x; q < nparts; q += 1024) v += partials[q];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += sh[0];
   }
 };
 
@@ -337,7 +511,8 @@ struct k_conc_adj {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const Geo &g, const double *x, const double *y, const double *z,
                                              const int32_t *cum, const double *djdc, double *l0, double *l1,
-                                             double *l2, double *l3, double *l4, double *l5, long np) {
+                                             double *l2, double *l3, double *l4, double *l5, double *l6,
+                                             long np) {
     const long i = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (i >= np) return;
     const long p = cum[i];  // the particle at i, in the recording's order
@@ -359,6 +534,7 @@ struct k_conc_adj {
         }
     l0[p] = gx; l1[p] = gy; l2[p] = gz;
     l3[p] = 0.0; l4[p] = 0.0; l5[p] = 0.0;
+    if (l6) l6[p] = 0.0;
   }
 };
 
@@ -374,12 +550,12 @@ struct k_adj_gather_i {
   }
 };
 
-// dst[q][cum[i]] = src[q][i] for the 6 state arrays
+// dst[q][cum[i]] = src[q][i] for the 6 (or 7, s6 set) state arrays
 struct k_adj_scatter6 {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const double *s0, const double *s1, const double *s2, const double *s3,
-                                             const double *s4, const double *s5, const int32_t *cum, double *dst,
-                                             long n) {
+                                             const double *s4, const double *s5, const double *s6,
+                                             const int32_t *cum, double *dst, long n) {
     const long i = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (i >= n) return;
     const long p = cum[i];
@@ -389,6 +565,7 @@ struct k_adj_scatter6 {
     dst[3 * n + p] = s3[i];
     dst[4 * n + p] = s4[i];
     dst[5 * n + p] = s5[i];
+    if (s6) dst[6 * n + p] = s6[i];
   }
 };
 
@@ -398,16 +575,34 @@ void hip_check_adj(hipError_t e) {
   if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
 }
 
-StepArgs step_args(tpx_ctx *c, int step, const double *const yin[6], double *const y[6], double *stage) {
+HeatParams heat_params(const tpx_ctx *c, int step) {
+  const AdjointTape &t = c->adj;
+  HeatParams h{};
+  h.on = t.heat ? 1 : 0;
+  if (!t.heat) return h;
+  h.p0 = t.p0t + (long)step * 4;
+  h.eps = t.eps0;
+  h.epsp = c->p.epsilonp;
+  h.iS = 1. / (c->mp * c->p.cvp);
+  h.K = c->p.nu * TPX_PI * c->p.dp * c->p.k;
+  h.R = c->R;
+  h.Dp = c->p.dp;
+  h.I0 = t.I0[(size_t)step];
+  return h;
+}
+
+StepArgs step_args(tpx_ctx *c, int step, const double *const yin[7], double *const y[7], double *stage) {
   Particles &pt = c->pt;
   AdjointTape &t = c->adj;
   StepArgs a{};
   a.g = c->g;
   a.gas = t.gas + (long)step * 16 * c->g.n;
-  for (int q = 0; q < 6; ++q) {
+  a.nq = t.nq;
+  for (int q = 0; q < t.nq; ++q) {
     a.yin[q] = yin[q];
     a.y[q] = y[q];
   }
+  a.h = heat_params(c, step);
   a.stage = stage;
   a.tau = pt.has_tau ? t.tau0 : nullptr;
   a.tp = c->Tp;
@@ -433,6 +628,7 @@ void adjoint_tape_substep(tpx_ctx *c, int rk, const double *const ru_int[3], con
   double *dst = t.gas + ((long)t.step * 4 + rk) * 4 * n;
   for (int d = 0; d < 3; ++d) launch_copy(c, dst + d * n, ru_int[d], n);
   launch_copy(c, dst + 3 * n, rho_int, n);
+  if (t.heat) launch_copy(c, t.p0t + (long)t.step * 4 + rk, c->scal + SC_P0_INT, 1);
   if (rk == 0)
     for (int d = 0; d < 3; ++d) t.g3[(size_t)t.step * 3 + d] = (double)c->p.particle_gravity * c->gv[d];
 }
@@ -440,10 +636,11 @@ void adjoint_tape_substep(tpx_ctx *c, int rk, const double *const ru_int[3], con
 void adjoint_checkpoint(tpx_ctx *c, int slot) {
   Particles &pt = c->pt;
   const int b = pt.st_base;
+  const int nq = c->adj.nq;
   tpx_launch<k_adj_scatter6>(grid1(pt.n), kBlock, 0, c->stream, (const double *)pt.q[0][b], (const double *)pt.q[1][b],
                              (const double *)pt.q[2][b], (const double *)pt.q[3][b], (const double *)pt.q[4][b],
-                             (const double *)pt.q[5][b], (const int32_t *)c->adj.cum,
-                             c->adj.ckpt + (long)slot * 6 * pt.n, (long)pt.n);
+                             (const double *)pt.q[5][b], nq == 7 ? (const double *)pt.q[6][b] : nullptr,
+                             (const int32_t *)c->adj.cum, c->adj.ckpt + (long)slot * nq * pt.n, (long)pt.n);
   TPX_LAUNCH_CHECK();
 }
 
@@ -476,12 +673,39 @@ double adjoint_objective(tpx_ctx *c) {
   return s[1] / (double)n - mean * mean;
 }
 
+// The reverse of one heat exchange at the state Y (nq x np, the recording's
+// order) into lam; its lam_P0 is added to scal[slot].
+void heat_reverse(tpx_ctx *c, const double *Y, const GasSweep::HeatMid &m, const double *p0, double *lam, int slot) {
+  AdjointTape &t = c->adj;
+  HeatRevArgs a{};
+  a.g = c->g;
+  a.Y = Y;
+  a.rho = m.rho;
+  a.p0 = p0;
+  a.lq = m.lq;
+  a.lsq = c->scal + SC_ADJ_LSQ;
+  a.lh = t.lh;
+  a.lam = lam;
+  a.lrho = m.lrho;
+  a.partials = t.hpart;
+  a.K = c->p.nu * TPX_PI * c->p.dp * c->p.k;
+  a.R = c->R;
+  a.np = c->pt.n;
+  const unsigned nb = grid1(a.np);
+  tpx_launch<k_adj_heat>(nb, kBlock, 0, c->stream, a);
+  TPX_LAUNCH_CHECK();
+  tpx_launch<k_adj_acc1>(1, 1024, 0, c->stream, (const double *)t.hpart, (long)nb, c->scal + slot);
+  TPX_LAUNCH_CHECK();
+}
+
 void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
   Particles &pt = c->pt;
   AdjointTape &t = c->adj;
   const long np = pt.n;
-  double *lam[6], *y[6];
-  for (int q = 0; q < 6; ++q) {
+  const int nq = t.nq;
+  if (t.heat && !gas) throw std::logic_error("adjoint: the heat exchange needs the gas sweep");
+  double *lam[7] = {}, *y[7] = {};
+  for (int q = 0; q < nq; ++q) {
     lam[q] = t.lam + q * np;
     y[q] = t.y + q * np;
   }
@@ -489,29 +713,33 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
   tpx_launch<k_conc_adj>(grid1(np), kBlock, 0, c->stream, c->g, (const double *)pt.q[0][pt.st_base],
                          (const double *)pt.q[1][pt.st_base], (const double *)pt.q[2][pt.st_base],
                          (const int32_t *)t.cum, (const double *)t.djdc, lam[0], lam[1], lam[2], lam[3], lam[4],
-                         lam[5], np);
+                         lam[5], lam[6], np);
   TPX_LAUNCH_CHECK();
   launch_fill(c, t.dtau, 0.0, np);
+  if (t.heat) {
+    launch_fill(c, t.deps, 0.0, np);
+    launch_fill(c, t.lh, 0.0, np);
+  }
   // the start of every step of the checkpoint segment being reversed,
   // recomputed once per segment from its checkpoint
   if (gas && !t.lb) {
-    if (hipMalloc((void **)&t.lb, sizeof(double) * 6 * (size_t)np) != hipSuccess ||
-        hipMalloc((void **)&t.lnext, sizeof(double) * 6 * (size_t)np) != hipSuccess)
+    if (hipMalloc((void **)&t.lb, sizeof(double) * nq * (size_t)np) != hipSuccess ||
+        hipMalloc((void **)&t.lnext, sizeof(double) * nq * (size_t)np) != hipSuccess)
       throw std::bad_alloc();
   }
   const int seg_len = std::min(t.every, t.nsteps);
-  if (!t.seg && hipMalloc((void **)&t.seg, sizeof(double) * 6 * (size_t)np * seg_len) != hipSuccess)
+  if (!t.seg && hipMalloc((void **)&t.seg, sizeof(double) * nq * (size_t)np * seg_len) != hipSuccess)
     throw std::bad_alloc();
-  const double *in[6];
-  double *out[6];
-  auto start_of = [&](int j, int q) { return t.seg + ((long)j * 6 + q) * np; };
+  const double *in[7] = {};
+  double *out[7] = {};
+  auto start_of = [&](int j, int q) { return t.seg + ((long)j * nq + q) * np; };
   int seg_k = -1;
   for (int s = t.nsteps - 1; s >= 0; --s) {
     const int k0 = s / t.every, base = k0 * t.every;
     if (k0 != seg_k) {
-      for (int q = 0; q < 6; ++q) launch_copy(c, start_of(0, q), t.ckpt + ((long)k0 * 6 + q) * np, np);
+      for (int q = 0; q < nq; ++q) launch_copy(c, start_of(0, q), t.ckpt + ((long)k0 * nq + q) * np, np);
       for (int r = base; r < s; ++r) {
-        for (int q = 0; q < 6; ++q) {
+        for (int q = 0; q < nq; ++q) {
           in[q] = start_of(r - base, q);
           out[q] = start_of(r - base + 1, q);
         }
@@ -521,7 +749,7 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
       seg_k = k0;
     }
     // the stage states of step s
-    for (int q = 0; q < 6; ++q) in[q] = start_of(s - base, q);
+    for (int q = 0; q < nq; ++q) in[q] = start_of(s - base, q);
     tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, s, in, y, t.stage));
     TPX_LAUNCH_CHECK();
     if (!gas) {
@@ -546,11 +774,17 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
     // the particles' stages interleaved with the gas substeps, backwards
     gas->begin(s);
     for (int rk = 3; rk >= 0; --rk) {
-      const double *const *lrhs = gas->part_a(s, rk);
+      gas->part_a(s, rk);
+      if (t.heat) {  // the exchange at the end of this substep, at the next stage's state
+        const double *Y = rk < 3 ? t.stage + (long)(rk + 1) * nq * np : t.y;
+        heat_reverse(c, Y, gas->heat_mid(s, rk), t.p0t + (long)s * 4 + rk + 1, rk < 3 ? t.lnext : t.lam,
+                     SC_ADJ_P0NEW);
+      }
+      const double *const *lrhs = gas->part_rk(s, rk);
       StageArgs a{};
       a.g = c->g;
       a.gas = t.gas + ((long)s * 4 + rk) * 4 * c->g.n;
-      a.stage = t.stage + (long)rk * 6 * np;
+      a.stage = t.stage + (long)rk * nq * np;
       a.lend = t.lam;
       a.lb = t.lb;
       a.lnext = t.lnext;
@@ -564,16 +798,25 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
       a.c_pre = c->p.dt * c->pre[rk];
       a.c_post = c->p.dt * c->post[rk];
       a.last = rk == 3;
+      a.nq = nq;
+      if (t.heat) {
+        a.lh = t.lh;
+        a.deps = t.deps;
+        a.alpha_eps = 0.25 * TPX_PI * c->p.dp * c->p.dp * t.I0[(size_t)s];
+        a.iS = 1. / (c->mp * c->p.cvp);
+      }
       a.np = np;
       a.ncell = c->g.n;
       tpx_launch<k_adj_stage_rev>(grid1(np), kBlock, 0, c->stream, a);
       TPX_LAUNCH_CHECK();
       gas->part_b(s, rk, t.lup);
     }
-    tpx_launch<k_adj_stage_end>(grid1(6 * np), kBlock, 0, c->stream, (const double *)t.lb, (const double *)t.lnext,
-                                t.lam, 6 * np);
+    tpx_launch<k_adj_stage_end>(grid1(nq * np), kBlock, 0, c->stream, (const double *)t.lb, (const double *)t.lnext,
+                                t.lam, nq * np);
     TPX_LAUNCH_CHECK();
   }
+  // the exchange that made the recording's first RHS_T (at its initial state)
+  if (t.heat) heat_reverse(c, t.ckpt, gas->heat_init, t.p0t, t.lam, SC_ADJ_P0I);
 }
 
 }  // namespace tpx

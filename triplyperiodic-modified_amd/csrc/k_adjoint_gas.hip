@@ -1,8 +1,8 @@
 // k_adjoint_gas.hip -- hand-written adjoint of the gas substep (gfx950).
 //
 // The reverse sweep of one RK4 substep of the gas (grid.cpp:608-614,
-// 776-863 with RHS_Part_Temp = 0: no heat exchange), from the substep as the
-// recording kept it or as recomputed from its step's start.  With
+// 776-863), from the substep as the recording kept it or as recomputed from
+// its step's start.  With
 //   F  = I_C2F(Rho_int),  U = RU_int / F                       (G3)
 //   RHS = a_drag S + mu/3 grad div U + mu del2 U - conv(U, RU_int)
 //         + A RU_int + g (F - rho0)                            (G4)
@@ -10,7 +10,8 @@
 // a_drag lam_RHS goes back to the particles' stage, k_adjoint.hip)
 //   D_rho = div RU_int                                         (G1)
 //   F' = I_C2F(Rho_new),  C = 1/F',  cc = -1/I_F2C(C)
-//   RHS_P = Z((div(RU_new/F') - k/cp del2(1/Rho_new)) / (dt pre))   (G5, G6)
+//   div   = k/cp del2(1/Rho_new) + c0 Q',  c0 = -R/(cp P0_new Vcell)  (G5)
+//   RHS_P = Z((div(RU_new/F') - div) / (dt pre))               (G6)
 //   P_{i+1} = P_i + M(cc (div(C grad P_i) - RHS_P)),  i < n     (G7)
 //   RU_np1 -= dt post grad P,  RU_new -= dt pre grad P          (G10)
 // (Z the mean-zero projection, M the spectral solve of poisson.cpp:413-462,
@@ -20,6 +21,13 @@
 // and the Poisson iterations are reversed with their recorded count:
 //   nu = M lam_P,  mu = cc nu,  lam_P += div(C grad mu),  lam_RHS_P -= mu,
 //   lam_C -= grad mu . grad P_i,  lam_cc += nu (div(C grad P_i) - RHS_P).
+// With the heat exchange (Q = RHS_Part_Temp, Q' = Q - [cooling] mean Q) the
+// divergence's Q term gives lam_Q' = -c0 nu (nu the adjoint of -div) and
+// lam_P0_new += (c0/P0_new) sum nu Q', and Update_P0 (grid.cpp:816-830),
+//   dP0 = -(R/cv) (sum Q / ncell) / Vcell,  P0_np1 += dt post dP0,
+//   P0_new = P0 + dt pre dP0 (P0_np1 at the fourth substep),
+// is reversed on the device scalars; the particles' side of the exchange
+// (the deposit of RHS_T and the gather of T = P0/(R Rho)) is in k_adjoint.hip.
 // Face fields follow the forward layout: component a at index x is the face
 // between cells x - e_a and x.  Every kernel is one thread per cell (and the
 // three faces at its index) over the periodic box.
@@ -93,6 +101,25 @@ __device__ __forceinline__ void block_sum(double v, double *partials) {
     __syncthreads();
   }
   if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// block sums of u and v into pu[blockIdx.x] and pv[blockIdx.x]
+__device__ __forceinline__ void block_sum2(double u, double v, double *pu, double *pv) {
+  __shared__ double sh[2][kBlock];
+  sh[0][threadIdx.x] = u;
+  sh[1][threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pu[blockIdx.x] = sh[0][0];
+    pv[blockIdx.x] = sh[1][0];
+  }
 }
 
 // the spectral solve of lam_P: a forward iteration from P = 0 with the
@@ -237,29 +264,47 @@ struct RhsPArgs {
   const double *scal;     // SC_ADJ_SUM: sum of lam_RHS
   double s, kappa;        // 1/(dt pre), k/cp
   double *lnew[3], *lF[3], *lrho;
+  // heat exchange (Q null: none)
+  const double *Q, *q_sum, *p0n;
+  double *lq;
+  double *pa, *pb;        // per block: lam_P0_new and lam_(sum Q) terms
+  double R, cp, Vcell;
+  int cooling;
 };
 
 struct k_adj_rhs_pois {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const RhsPArgs &a) {
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
-    if (c >= a.g.n) return;
-    const Cell x(a.g, c);
-    const double mean = a.scal[SC_ADJ_SUM] / (double)a.g.n;
-    auto nu = [&](long p) { return a.s * (a.lrhs[p] - mean); };
-    const double nc = nu(c), rc = a.rho[c];
-    double lap = 0.0;
+    double ta = 0.0, tb = 0.0;
+    if (c < a.g.n) {
+      const Cell x(a.g, c);
+      const double mean = a.scal[SC_ADJ_SUM] / (double)a.g.n;
+      auto nu = [&](long p) { return a.s * (a.lrhs[p] - mean); };
+      const double nc = nu(c), rc = a.rho[c];
+      double lap = 0.0;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const long m = x.at(d, -1), p = x.at(d, 1);
-      const double nm = nu(m);
-      lap += (nu(p) - 2 * nc + nm) * inv_d2(a.g, d);
-      const double F = (rc + a.rho[m]) * 0.5;
-      const double lu = -((nc - nm) * inv_d(a.g, d));
-      a.lnew[d][c] += lu / F;
-      a.lF[d][c] += -lu * a.ru_new[d][c] / (F * F);
+      for (int d = 0; d < 3; ++d) {
+        const long m = x.at(d, -1), p = x.at(d, 1);
+        const double nm = nu(m);
+        lap += (nu(p) - 2 * nc + nm) * inv_d2(a.g, d);
+        const double F = (rc + a.rho[m]) * 0.5;
+        const double lu = -((nc - nm) * inv_d(a.g, d));
+        a.lnew[d][c] += lu / F;
+        a.lF[d][c] += -lu * a.ru_new[d][c] / (F * F);
+      }
+      a.lrho[c] += a.kappa * lap / (rc * rc);
+      if (a.Q) {
+        const double P0n = a.p0n[0];
+        const double c0 = -a.R / (a.cp * P0n * a.Vcell);
+        const double qp = a.Q[c] - (a.cooling ? a.q_sum[0] / (double)a.g.n : 0.0);
+        const double lqp = -c0 * nc;
+        a.lq[c] = lqp;
+        ta = (c0 / P0n) * nc * qp;
+        tb = a.cooling ? -lqp / (double)a.g.n : 0.0;
+      }
     }
-    a.lrho[c] += a.kappa * lap / (rc * rc);
+    if (a.Q) block_sum2(ta, tb, a.pa, a.pb);
   }
 };
 
@@ -463,12 +508,46 @@ struct k_adj_sub_mean {
   }
 };
 
+// Update_P0 reversed for one substep (step false), or the step boundary
+// (step true: P0 = P0_np1 at the next step's start, P0_np1 = P0 at this
+// one's): the adjoints of P0, P0_np1 and this substep's P0_new, and that of
+// the sum of RHS_Part_Temp the update read (written, not accumulated: the
+// heat exchange that produced it consumes it next)
+struct k_adj_p0 {
+  static constexpr int kBounds = 64;
+  static __device__ __forceinline__ void run(double *scal, bool step, int last, int cooling, double c_np1,
+                                             double c_new, double dsum) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (step) {
+      scal[SC_ADJ_P0NP1] += scal[SC_ADJ_P0];
+      scal[SC_ADJ_P0] = 0.0;
+      return;
+    }
+    const double ln = scal[SC_ADJ_P0NEW];
+    double ld;
+    if (last) {
+      const double l1 = scal[SC_ADJ_P0NP1] + ln;
+      scal[SC_ADJ_P0NP1] = l1;
+      ld = c_np1 * l1;
+    } else {
+      ld = c_np1 * scal[SC_ADJ_P0NP1] + c_new * ln;
+      scal[SC_ADJ_P0] += ln;
+    }
+    scal[SC_ADJ_LSQ] = cooling ? 0.0 : ld * dsum;
+    scal[SC_ADJ_P0NEW] = 0.0;
+  }
+};
+
 unsigned cells(const tpx_ctx *c) { return grid1d(c->g.n, kBlock); }
 
 }  // namespace
 
-void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga) {
+void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga, bool heat) {
   const long n = c->g.n;
+  if (heat) {
+    tpx_launch<k_adj_p0>(1, 64, 0, c->stream, c->scal, true, 0, 0, 0.0, 0.0, 0.0);
+    TPX_LAUNCH_CHECK();
+  }
   tpx_launch<k_adj_boundary>(cells(c), kBlock, 0, c->stream, n, ga.lnp1[0], ga.lnp1[1], ga.lnp1[2], ga.lru[0],
                              ga.lru[1], ga.lru[2], ga.lrnp1, ga.lr);
   TPX_LAUNCH_CHECK();
@@ -479,7 +558,7 @@ void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga) {
   }
 }
 
-const double *const *adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k) {
+void adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k) {
   const unsigned nb = cells(c);
   if (s.iters < 1) throw std::logic_error("adjoint: a Poisson solve without iterations");
   // G11: the int state after this substep is its new state
@@ -542,11 +621,34 @@ const double *const *adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const Gas
     a.s = 1. / (k.dt * k.pre);
     a.kappa = k.k_over_cp;
     a.lrho = ga.lrnew;
+    a.Q = s.Q;
+    if (s.Q) {
+      a.q_sum = s.q_sum;
+      a.p0n = s.p0n;
+      a.lq = ga.lq;
+      a.pa = c->partials;
+      a.pb = c->partials + c->n_partials;
+      a.R = k.R;
+      a.cp = k.cp;
+      a.Vcell = k.Vcell;
+      a.cooling = k.cooling;
+    }
     tpx_launch<k_adj_rhs_pois>(nb, kBlock, 0, c->stream, a);
     TPX_LAUNCH_CHECK();
+    if (s.Q) {
+      tpx_launch<k_adj_acc>(1, 1024, 0, c->stream, (const double *)a.pa, (long)nb, c->scal + SC_ADJ_P0NEW);
+      TPX_LAUNCH_CHECK();
+      tpx_launch<k_adj_acc>(1, 1024, 0, c->stream, (const double *)a.pb, (long)nb, c->scal + SC_ADJ_LSQ);
+      TPX_LAUNCH_CHECK();
+    }
   }
-  // G4 and G1: lam_Rho_new complete (fold of the face adjoint), the RK4
-  // updates, then the right-hand sides
+}
+
+const double *const *adjoint_gas_substep_rk(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k) {
+  const unsigned nb = cells(c);
+  const bool last = k.rk == 3;
+  // G4 and G1: lam_Rho_new complete (fold of the face adjoint, and the heat
+  // exchange's gather of T), the RK4 updates, then the right-hand sides
   {
     RkArgs a{};
     a.g = c->g;
@@ -595,6 +697,11 @@ void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
   tpx_launch<k_adj_fold>(nb, kBlock, 0, c->stream, c->g, (const double *)ga.lF[0], (const double *)ga.lF[1],
                          (const double *)ga.lF[2], ga.lrint);
   TPX_LAUNCH_CHECK();
+  if (s.Q) {  // G2
+    tpx_launch<k_adj_p0>(1, 64, 0, c->stream, c->scal, false, k.rk == 3 ? 1 : 0, k.cooling, k.dt * k.post,
+                         k.dt * k.pre, -k.R / k.cv / k.Vcell / (double)n);
+    TPX_LAUNCH_CHECK();
+  }
 }
 
 }  // namespace tpx

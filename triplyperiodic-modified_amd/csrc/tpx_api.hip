@@ -388,6 +388,10 @@ void adjoint_free(AdjointTape &t) {
   auto f = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
   f(t.gas); f(t.ckpt); f(t.y); f(t.seg); f(t.lb); f(t.lnext); f(t.S); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
   f(t.start); f(t.final_state); f(t.work); f(t.cap); f(t.lup); f(t.iters_seen); f(t.sub); f(t.tau0);
+  f(t.qt); f(t.qs); f(t.p0t); f(t.eps0); f(t.deps); f(t.lh); f(t.hpart); f(t.scal_save);
+  t.heat = false;
+  t.nq = 6;
+  t.swept = false;
   auto fi = [](int32_t *&p) { if (p) (void)hipFree(p); p = nullptr; };
   fi(t.cum); fi(t.cum_tmp);
   t.store = false;
@@ -1140,7 +1144,8 @@ void gas_state_copy(tpx_ctx *c, double *dst, bool save) {
 
 // Recomputes step s of the recorded run, gas only (the gas does not depend on
 // the particles under one-way coupling; under two-way coupling the recorded
-// drag deposits stand in for them), from its start on the tape, keeping
+// drag deposits stand in for them, with the heat exchange the recorded
+// RHS_Part_Temp, its sum and P0_new), from its start on the tape, keeping
 // what the reverse sweep reads: RU_new and Rho_new after the momentum and
 // density updates, the raw Poisson right-hand side and its sum, and P before
 // every Poisson iteration.  The arithmetic is the fused step's.
@@ -1167,7 +1172,18 @@ void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
     const double *gs = t.gas + ((long)s * 4 + rk) * 4 * n;
     for (int d = 0; d < 3; ++d) g.ru_int[d] = gs + d * n;
     g.rho_int = gs + 3 * n;
-    launch_update_p0(c, k);
+    const size_t q = (size_t)s * 4 + rk;
+    if (t.heat) {  // Update_P0's P0_new, and the substep's deposit for the divergence
+      launch_copy(c, c->scal + SC_P0_NEW, t.p0t + q + 1, 1);
+      launch_copy(c, c->Q, t.qt + (long)q * n, n);
+      launch_copy(c, c->scal + SC_SUM_Q, t.qs + q, 1);
+      c->q_zero = false;
+      g.Q = t.qt + (long)q * n;
+      g.q_sum = t.qs + q;
+      g.p0n = t.p0t + q + 1;
+    } else {
+      launch_update_p0(c, k);
+    }
     if (t.S)  // two-way coupling: the recorded drag deposit of this substep
       for (int d = 0; d < 3; ++d) launch_copy(c, c->S[d], t.S + (((long)s * 4 + rk) * 3 + d) * n, n);
     do_update_ru_wop(c, rk, true);
@@ -1251,6 +1267,11 @@ void taped_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
     g.rho_new = sb + 3 * n;
     g.rhs = sb + 4 * n;
     g.rhs_sum = t.sub + (long)t.nsteps * 20 * n + q;
+    if (t.heat) {
+      g.Q = t.qt + (long)q * n;
+      g.q_sum = t.qs + q;
+      g.p0n = t.p0t + q + 1;
+    }
     g.iters = t.iters[q];
     g.P.clear();
     for (int i = 0; i < g.iters; ++i) g.P.push_back(t.pcap[t.p_first[q] + i]);
@@ -1279,11 +1300,16 @@ void gas_adjoint_run(tpx_ctx *c) {
   for (auto *grp : {ga.lru, ga.lnp1, ga.lint, ga.lnew, ga.lrhs, ga.lu, ga.lC, ga.lF, ga.Cf, ga.U})
     for (int d = 0; d < 3; ++d) grp[d] = take();
   for (double **p : {&ga.lr, &ga.lrnp1, &ga.lrint, &ga.lrnew, &ga.lp, &ga.lrhs_p, &ga.lcc, &ga.ld, &ga.mu,
-                     &ga.tmp_rhs, &ga.tmp_p, &ga.cc})
+                     &ga.tmp_rhs, &ga.tmp_p, &ga.cc, &ga.lq})
     *p = take();
   ga.scal = w;
   launch_fill(c, t.work, 0.0, (long)GasAdjoint::kFields * n + SC_COUNT);
   launch_fill(c, c->scal + SC_ADJ_A, 0.0, 1);
+  if (t.heat) {
+    launch_fill(c, c->scal + SC_ADJ_P0, 0.0, SC_ADJ_P0I - SC_ADJ_P0 + 1);
+    // a recomputing sweep rewrites the Update_P0 slots and Q: keep the run's end
+    if (!t.store) launch_copy(c, t.scal_save, c->scal, SC_SUM_Q + 1);
+  }
   const double A0 = c->A, g0[3] = {c->gv[0], c->gv[1], c->gv[2]};
   const int it0 = c->Iteration;
   GasSubstep sub[4];
@@ -1292,26 +1318,37 @@ void gas_adjoint_run(tpx_ctx *c) {
   sweep.begin = [&](int s) {
     if (stored) taped_step(c, s, sub);
     else gas_recompute_step(c, s, sub);
-    if (s < t.nsteps - 1) adjoint_gas_boundary(c, ga);
+    if (s < t.nsteps - 1) adjoint_gas_boundary(c, ga, t.heat);
   };
-  sweep.part_a = [&](int, int rk) { return adjoint_gas_substep_a(c, ga, sub[rk], coef(c, rk)); };
+  sweep.part_a = [&](int, int rk) { adjoint_gas_substep_a(c, ga, sub[rk], coef(c, rk)); };
+  sweep.part_rk = [&](int, int rk) { return adjoint_gas_substep_rk(c, ga, sub[rk], coef(c, rk)); };
+  sweep.heat_mid = [&](int, int rk) { return GasSweep::HeatMid{sub[rk].rho_new, ga.lq, ga.lrnew}; };
+  sweep.heat_init = GasSweep::HeatMid{t.gas + 3 * n, nullptr, ga.lr};
   sweep.part_b = [&](int, int rk, const double *lup) {
     sub[rk].lup = lup;
     adjoint_gas_substep_b(c, ga, sub[rk], coef(c, rk));
   };
   sweep.a_drag = c->p.twc ? -c->mp / c->Vcell : 0.0;
+  // the run's end where a recomputing sweep rewrote it
+  auto restore = [&]() {
+    c->A = A0; c->Iteration = it0;
+    for (int d = 0; d < 3; ++d) c->gv[d] = g0[d];
+    if (stored) return;
+    gas_state_copy(c, t.final_state, false);
+    if (t.heat) {
+      launch_copy(c, c->scal, t.scal_save, SC_SUM_Q + 1);
+      launch_copy(c, c->Q, t.qt + ((long)t.nsteps * 4 - 1) * n, n);
+    }
+  };
   try {
     adjoint_reverse(c, &sweep);
   } catch (...) {
-    c->A = A0; c->Iteration = it0;
-    for (int d = 0; d < 3; ++d) c->gv[d] = g0[d];
-    gas_state_copy(c, t.final_state, false);
+    restore();
     throw;
   }
-  c->A = A0; c->Iteration = it0;
-  for (int d = 0; d < 3; ++d) c->gv[d] = g0[d];
+  restore();
+  t.swept = true;
   if (stored) return;
-  gas_state_copy(c, t.final_state, false);
   // every recomputed solve stopped where the recorded one did
   std::vector<double> seen((size_t)t.nsteps * 4);
   hip_check(hipMemcpyAsync(seen.data(), t.iters_seen, sizeof(double) * seen.size(), hipMemcpyDeviceToHost, c->stream),
@@ -1444,21 +1481,37 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
     Particles &pt = c->pt;
     if (nsteps < 1 || every < 1) throw std::invalid_argument("nsteps and every must be positive");
     if (pt.n == 0) throw std::invalid_argument("the adjoint needs particles");
-    if (heat_active(c)) throw std::logic_error("the adjoint does not differentiate the heat exchange: Nu pi Dp k must be 0");
     AdjointTape &t = c->adj;
     sync(c);
     adjoint_free(t);
     t.nsteps = nsteps;
     t.every = every;
+    t.heat = heat_active(c);
+    t.nq = t.heat ? 7 : 6;
     const long n = c->g.n, np = pt.n;
+    const size_t nq = (size_t)t.nq;
     auto alloc = [](double **p, size_t count) {
       if (hipMalloc((void **)p, sizeof(double) * count) != hipSuccess) throw std::bad_alloc();
     };
     alloc(&t.gas, (size_t)nsteps * 16 * n);
-    alloc(&t.ckpt, (size_t)(nsteps / every + 1) * 6 * np);
-    alloc(&t.y, 6 * (size_t)np);
-    alloc(&t.stage, 24 * (size_t)np);
-    alloc(&t.lam, 6 * (size_t)np);
+    alloc(&t.ckpt, (size_t)(nsteps / every + 1) * nq * np);
+    alloc(&t.y, nq * (size_t)np);
+    alloc(&t.stage, 4 * nq * (size_t)np);
+    alloc(&t.lam, nq * (size_t)np);
+    if (t.heat) {  // RHS_Part_Temp, its sum and P0 of every substep
+      alloc(&t.qt, (size_t)nsteps * 4 * n);
+      alloc(&t.qs, (size_t)nsteps * 4);
+      alloc(&t.p0t, (size_t)nsteps * 4 + 1);
+      alloc(&t.deps, (size_t)np);
+      alloc(&t.lh, (size_t)np);
+      alloc(&t.hpart, (size_t)grid1d(np, 256));
+      alloc(&t.scal_save, SC_COUNT);
+      if (pt.has_eps) {
+        alloc(&t.eps0, (size_t)np);
+        launch_copy(c, t.eps0, pt.eps, np);
+      }
+      t.I0.assign((size_t)nsteps, 0.0);
+    }
     alloc(&t.dtau, (size_t)np);
     alloc(&t.djdc, (size_t)n);
     if (hipMalloc((void **)&t.cum, sizeof(int32_t) * np) != hipSuccess ||
@@ -1506,6 +1559,7 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
         for (int d = 0; d < 3; ++d) launch_copy(c, t.start + ((long)s * 4 + d) * n, c->ru[c->ru_base][d], n);
         launch_copy(c, t.start + ((long)s * 4 + 3) * n, c->P, n);
         t.A[s] = c->A;
+        if (t.heat) t.I0[s] = c->I0;
         t.iteration[s] = c->Iteration;
         for (int d = 0; d < 3; ++d) t.gv[(size_t)s * 3 + d] = c->gv[d];
         for (int rk = 0; rk < 4; ++rk) {  // fused_substep, keeping each solve's iteration count
@@ -1513,6 +1567,10 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
           const size_t q = (size_t)s * 4 + rk;
           if (twc)
             for (int d = 0; d < 3; ++d) launch_copy(c, t.S + ((long)q * 3 + d) * n, c->S[d], n);
+          if (t.heat) {
+            launch_copy(c, t.qt + (long)q * n, c->Q, n);
+            launch_copy(c, t.qs + q, c->scal + SC_SUM_Q, 1);
+          }
           if (t.store) {  // what the gas sweep reads, so that it need not recompute the step
             double *dst = t.sub + (long)q * 5 * n;
             for (int d = 0; d < 3; ++d) launch_copy(c, dst + d * n, c->ru[c->ru_new][d], n);
@@ -1544,6 +1602,7 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
       throw;
     }
     gas_state_copy(c, t.final_state, true);
+    if (t.heat) launch_copy(c, t.p0t + (long)nsteps * 4, c->scal + SC_P0, 1);
     const double j = adjoint_objective(c);
     if (J) *J = j;
     t.ready = true;
@@ -1562,9 +1621,9 @@ int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, 
     if (!t.ready) throw std::logic_error("no recorded run: call tpx_adjoint_record first");
     const bool gas = dJ_dA || dJ_dRU || dJ_dRho || dJ_dP;
     if (!gas && !dJ_dtau && !dJ_dstate) throw std::invalid_argument("null outputs");
-    // under two-way coupling the gas depends on the particles, so every
-    // gradient needs the gas sweep
-    if (gas || ctx->p.twc) gas_adjoint_run(ctx);
+    // under two-way coupling or with the heat exchange the gas depends on
+    // the particles, so every gradient needs the gas sweep
+    if (gas || ctx->p.twc || t.heat) gas_adjoint_run(ctx);
     else adjoint_reverse(ctx, nullptr);
     const size_t np = (size_t)ctx->pt.n, n = (size_t)ctx->g.n;
     // the sweep's particle arrays are in the recording's order: give them in
@@ -1577,7 +1636,7 @@ int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, 
     sync(ctx);
     if (dJ_dtau)
       for (size_t i = 0; i < np; ++i) dJ_dtau[i] = tmp[(size_t)cum[i]];
-    if (dJ_dstate) {  // adjoint of the recorded run's initial x, y, z, u, v, w
+    if (dJ_dstate) {  // adjoint of the recorded run's initial x, y, z, u, v, w (the first 6 rows)
       hip_check(hipMemcpyAsync(tmp.data(), t.lam, sizeof(double) * 6 * np, hipMemcpyDeviceToHost, ctx->stream),
                 "download");
       sync(ctx);
@@ -1594,6 +1653,35 @@ int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, 
       for (int d = 0; d < 3; ++d) download_sum(ctx, {ga.lru[d], ga.lnp1[d], ga.lint[d]}, dJ_dRU + d * n);
     if (dJ_dRho) download_sum(ctx, {ga.lr, ga.lrnp1, ga.lrint}, dJ_dRho);
     if (dJ_dP) download_sum(ctx, {ga.lp}, dJ_dP);
+    return TPX_OK;
+  });
+}
+
+int tpx_adjoint_gradient_heat(tpx_ctx *ctx, double *dJ_dT, double *dJ_deps, double *dJ_dP0) {
+  return guarded(ctx, [&]() {
+    AdjointTape &t = ctx->adj;
+    if (!t.ready) throw std::logic_error("no recorded run: call tpx_adjoint_record first");
+    if (!t.heat) throw std::logic_error("the recorded run has no heat exchange");
+    if (!dJ_dT && !dJ_deps && !dJ_dP0) throw std::invalid_argument("null outputs");
+    if (!t.swept) gas_adjoint_run(ctx);
+    const size_t np = (size_t)ctx->pt.n;
+    std::vector<int32_t> cum(np);
+    std::vector<double> tmp(np);
+    hip_check(hipMemcpyAsync(cum.data(), t.cum, sizeof(int32_t) * np, hipMemcpyDeviceToHost, ctx->stream), "download");
+    sync(ctx);
+    auto give = [&](const double *src, double *dst) {  // in the context's present particle order
+      hip_check(hipMemcpyAsync(tmp.data(), src, sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream), "download");
+      sync(ctx);
+      for (size_t i = 0; i < np; ++i) dst[i] = tmp[(size_t)cum[i]];
+    };
+    if (dJ_dT) give(t.lam + 6 * np, dJ_dT);
+    if (dJ_deps) give(t.deps, dJ_deps);
+    if (dJ_dP0) {  // the initial P0 is both P0 and P0_np1 of the first step, and fed its first exchange
+      double sc[SC_ADJ_P0I - SC_ADJ_P0 + 1];
+      hip_check(hipMemcpyAsync(sc, ctx->scal + SC_ADJ_P0, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream), "download");
+      sync(ctx);
+      *dJ_dP0 = sc[0] + sc[SC_ADJ_P0NP1 - SC_ADJ_P0] + sc[SC_ADJ_P0I - SC_ADJ_P0];
+    }
     return TPX_OK;
   });
 }

@@ -42,6 +42,11 @@ enum Scal {
   SC_ADJ_A,        // adjoint sweep: dJ/dA accumulated
   SC_ST_TKE2,      // statistics: sum of squares of the last substep's pre-projection U
   SC_ST_UG,        // statistics: max gathered gas x velocity of the last substep
+  // adjoint sweep with heat exchange: adjoints of the thermodynamic pressure
+  // P0 (base, np1 and new of the substep being reversed), of the pending sum
+  // of RHS_Part_Temp (the one the next Update_P0 reads), and of the recorded
+  // run's initial P0
+  SC_ADJ_P0, SC_ADJ_P0NP1, SC_ADJ_P0NEW, SC_ADJ_LSQ, SC_ADJ_P0I,
   SC_COUNT = 32
 };
 
@@ -95,8 +100,9 @@ struct Particles {
 struct GasAdjoint {
   double *lru[3], *lnp1[3], *lint[3], *lnew[3], *lrhs[3], *lu[3], *lC[3], *lF[3], *Cf[3], *U[3];
   double *lr, *lrnp1, *lrint, *lrnew, *lp, *lrhs_p, *lcc, *ld, *mu, *tmp_rhs, *tmp_p, *cc;
+  double *lq;    // adjoint of the substep's heat deposit RHS_Part_Temp
   double *scal;  // SC_COUNT slots of the spectral solve
-  static constexpr int kFields = 42;
+  static constexpr int kFields = 43;
 };
 
 // One recomputed substep as the gas adjoint reads it.
@@ -107,29 +113,34 @@ struct GasSubstep {
   std::vector<const double *> P;       // P before each Poisson iteration
   int iters = 0;
   const double *lup = nullptr;         // the particles' gather adjoint on the faces (3 n), or null
+  // heat exchange (else null): the substep's deposit RHS_Part_Temp, its sum
+  // and P0_new (device)
+  const double *Q = nullptr, *q_sum = nullptr, *p0n = nullptr;
 };
 
-// The adjoint of a recorded run (no heat exchange): the gas stage fields of
-// every substep, each step's gas start (RU, P), the particle base state
-// every `every` steps, and the reverse sweeps' work arrays.
+// The adjoint of a recorded run: the gas stage fields of every substep, each
+// step's gas start (RU, P), the particle base state every `every` steps, and
+// the reverse sweeps' work arrays.  Particle arrays hold nq quantities: x, y,
+// z, u, v, w, and T when the heat exchange is on.
 struct AdjointTape {
   int nsteps = 0, every = 1;
+  int nq = 6;                    // particle quantities taped and reversed (7 with the heat exchange)
   int step = -1;                 // step being recorded (-1: not recording)
   bool ready = false;            // a whole run is recorded and its objective computed
   double *gas = nullptr;         // nsteps x 4 substeps x (RU_int 3, Rho_int) x n
   std::vector<double> g3;        // particle gravity of each step (3 per step)
-  double *ckpt = nullptr;        // (nsteps / every + 1) x 6 x np
-  double *y = nullptr;           // 6 x np
-  double *seg = nullptr;         // min(every, nsteps) x 6 x np: step starts of the segment being reversed
-  double *lb = nullptr, *lnext = nullptr;  // 6 x np each: the staged sweep's base and next-stage adjoints
+  double *ckpt = nullptr;        // (nsteps / every + 1) x nq x np
+  double *y = nullptr;           // nq x np
+  double *seg = nullptr;         // min(every, nsteps) x nq x np: step starts of the segment being reversed
+  double *lb = nullptr, *lnext = nullptr;  // nq x np each: the staged sweep's base and next-stage adjoints
   double *S = nullptr;           // two-way coupling: the drag deposit of every substep (nsteps x 4 x 3 n)
   // particle order: with re-binning (two-way coupling) the particle arrays are
   // re-sorted at every step's end; the tape and the sweep keep the order the
   // recording started in.  cum[i]: that position of the particle now at i.
   int32_t *cum = nullptr, *cum_tmp = nullptr;
   double *tau0 = nullptr;        // per-particle tau in that order (null: scalar tau)
-  double *stage = nullptr;       // 4 x 6 x np
-  double *lam = nullptr;         // 6 x np
+  double *stage = nullptr;       // 4 x nq x np
+  double *lam = nullptr;         // nq x np
   double *dtau = nullptr;        // np
   double *djdc = nullptr;        // n
   // gas adjoint
@@ -148,6 +159,18 @@ struct AdjointTape {
   double *sub = nullptr;         // nsteps x 4 x (RU_new 3, Rho_new, raw RHS_Pois) x n, then nsteps x 4 sums
   std::vector<size_t> p_first;   // first pcap buffer of each substep's solve
   size_t pnext = 0;
+  // heat exchange
+  bool heat = false;
+  double *qt = nullptr;          // nsteps x 4 x n: RHS_Part_Temp of every substep
+  double *qs = nullptr;          // nsteps x 4: its sum
+  double *p0t = nullptr;         // nsteps x 4 + 1: P0_int at every substep's start, then the final P0
+  double *eps0 = nullptr;        // per-particle absorption efficiency in the recording's order (or null)
+  double *deps = nullptr;        // np: dJ/deps
+  double *lh = nullptr;          // np: adjoint of RHS_T through the next particle temperature update
+  double *hpart = nullptr;       // block partials of the exchange's reversal (one per 256 particles)
+  double *scal_save = nullptr;   // the Update_P0 slots of the run's end, kept over a recomputing sweep
+  std::vector<double> I0;        // lamp intensity of each step
+  bool swept = false;            // a sweep has run since the recording
   GasAdjoint ga{};
 };
 
@@ -294,9 +317,19 @@ double adjoint_objective(tpx_ctx *c);  // J = Var(C) of the base positions; leav
 // coupling (the drag deposit's weight in the momentum right-hand side), else 0.
 struct GasSweep {
   std::function<void(int)> begin;
-  std::function<const double *const *(int, int)> part_a;
+  std::function<void(int, int)> part_a;                    // through the Poisson right-hand side (G5, G6)
+  std::function<const double *const *(int, int)> part_rk;  // lam_Rho_new complete, RK bookkeeping; returns lam_RHS
   std::function<void(int, int, const double *)> part_b;
   double a_drag = 0.0;
+  // heat exchange: where the reversal of the deposit at the end of substep
+  // (s, rk) reads and writes, between part_a and part_rk
+  struct HeatMid {
+    const double *rho;  // Rho_new of the substep
+    const double *lq;   // adjoint of its RHS_Part_Temp (null: none)
+    double *lrho;       // adjoint of Rho_new, accumulated
+  };
+  std::function<HeatMid(int, int)> heat_mid;
+  HeatMid heat_init{};  // the recorded run's initial exchange (Rho at the start)
 };
 // dJ/dtau into adj.dtau and dJ/d(initial particle state) into adj.lam; with
 // gas set, the gas sweep runs between the particles' stages
@@ -314,11 +347,14 @@ bool tile_shape_valid(const tpx_ctx *c, int kind, int rows, int zc);  // kind 0 
 void tile_tune_begin(tpx_ctx *c);
 void tile_tune_collect(tpx_ctx *c);  // after a step: read the timings, choose once every candidate ran
 
-// k_adjoint_gas.hip: one substep of the gas sweep in two halves (around the
-// particles' stage) and the step boundary
-const double *const *adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
+// k_adjoint_gas.hip: one substep of the gas sweep in three parts (through
+// the Poisson right-hand side; the RK bookkeeping once lam_Rho_new is
+// complete; after the particles' stage) and the step boundary.  With the
+// heat exchange, part b also reverses the substep's Update_P0.
+void adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
+const double *const *adjoint_gas_substep_rk(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
 void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
-void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga);
+void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga, bool heat);
 
 // ---- timing helpers (tpx_api.hip) ----
 void timing_begin(tpx_ctx *c, int cls);

@@ -44,7 +44,7 @@ EXPORTS = [
     "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop", "tpx_compute_div_u_new",
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
     "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
-    "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_set_gas_tape", "tpx_statistics_enable", "tpx_statistics",
+    "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_gradient_heat", "tpx_adjoint_set_gas_tape", "tpx_statistics_enable", "tpx_statistics",
     "tpx_tune_tiles", "tpx_get_tiles", "tpx_set_tiles",
     "tpx_adjoint_release",
     "tpx_batch_create", "tpx_batch_step", "tpx_batch_destroy", "tpx_batch_error", "tpx_batch_last_error",
@@ -102,6 +102,7 @@ def load(path: str = LIB_PATH):
     L.tpx_adjoint_record.argtypes = [P, I, I, DP]
     L.tpx_adjoint_gradient.argtypes = [P, DP, DP]
     L.tpx_adjoint_gradient_full.argtypes = [P, DP, DP, DP, DP, DP, DP]
+    L.tpx_adjoint_gradient_heat.argtypes = [P, DP, DP, DP]
     L.tpx_adjoint_set_gas_tape.argtypes = [P, I]
     L.tpx_statistics_enable.argtypes = [P, I]
     L.tpx_tune_tiles.argtypes = [P]
@@ -288,6 +289,28 @@ class Solver:
                                                   _dp(out["RU"]), _dp(out["Rho"]), _dp(out["P"])),
                  "adjoint_gradient_full")
         out["A"] = float(out["A"][0])
+        if self.heat:
+            out.update(self.adjoint_gradient_heat())
+        return out
+
+    @property
+    def heat(self) -> bool:
+        """The heat exchange is on (Nu pi Dp k != 0)."""
+        p = self.params
+        return p.Nu * p.Dp * p.k != 0.0
+
+    def adjoint_gradient_heat(self) -> dict:
+        """For a recorded run with the heat exchange: "T" and "eps" (np, in
+        particle order), dJ/d of each particle's initial temperature and of
+        its absorption efficiency, and "P0" (a float), dJ/d of the initial
+        thermodynamic pressure; from the last sweep (one runs if none has
+        since the recording)."""
+        n = ctypes.c_int64()
+        self._ok(self.L.tpx_num_particles(self.h, ctypes.byref(n)), "num_particles")
+        out = dict(T=np.zeros(n.value), eps=np.zeros(n.value), P0=np.zeros(1))
+        self._ok(self.L.tpx_adjoint_gradient_heat(self.h, _dp(out["T"]), _dp(out["eps"]), _dp(out["P0"])),
+                 "adjoint_gradient_heat")
+        out["P0"] = float(out["P0"][0])
         return out
 
     def adjoint_set_gas_tape(self, mode: str = "auto") -> None:
