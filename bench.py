@@ -199,16 +199,20 @@ def ensemble_bench(args):
 
 def adjoint_bench(args):
     """Adjoint cost: 128^3, 2e5 particles, St = 2, two-way coupling at mass
-    loading 0.5, 50 steps with a particle checkpoint every 10.  The whole reverse
-    sweep (per step, backwards: the particles' transposed RK4 step from their
-    checkpoint segment, then the gas step recomputed from its start and
-    reversed through the projection, the Poisson iterations, the momentum and
-    density updates) is timed against the plain forward steps; the particle
-    sweep alone is reported beside it."""
+    loading 0.5 and the heat exchange on, 50 steps with a particle checkpoint
+    every 10.  The whole reverse sweep (per step, backwards: the particles'
+    stage states recomputed from their checkpoint segment; per substep the gas
+    reversed through the projection, the Poisson iterations and the
+    divergence, the heat exchange's gather and deposit, the RK bookkeeping,
+    the particles' transposed stage, the momentum and density updates and
+    Update_P0) is timed against the plain forward steps.  With two-way
+    coupling every gradient needs the gas sweep, so the tau-only call costs
+    the same sweep."""
     import tpx
     from tpx import ic
     nsteps, every = 50, 10
-    case = ic.make_case(128, 200_000, nu=0.01, seed=0, St=2.0, phi=0.5, heat=False, twc=True, nsteps=nsteps)
+    case = ic.make_case(128, 200_000, nu=0.01, seed=0, St=2.0, phi=0.5, heat=True, twc=True, nsteps=nsteps,
+                        eps_range=(0.5, 1.0))
     plain = tpx.Solver(case.params, np_capacity=200_000)
     plain.load_case(case)
     plain.step(1)
@@ -234,7 +238,8 @@ def adjoint_bench(args):
     full = s.adjoint_gradient_full()
     t_rev = time.perf_counter() - t0
     s.adjoint_release()
-    log(f"[bench] full sweep {t_rev:.3f} s, dJ/dA = {full['A']:.6e}, |dJ/dRU| = {np.linalg.norm(full['RU']):.3e}")
+    log(f"[bench] full sweep {t_rev:.3f} s, dJ/dA = {full['A']:.6e}, |dJ/dRU| = {np.linalg.norm(full['RU']):.3e}, "
+        f"|dJ/deps| = {np.linalg.norm(full['eps']):.3e}, dJ/dP0 = {full['P0']:.6e}")
     s.close()
     # the same sweep with the gas recomputed step by step from its start
     # (what runs when the kept tape does not fit in device memory)
@@ -258,8 +263,9 @@ def adjoint_bench(args):
         "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": 1, "higher_is_better": False,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "adjoint: 128^3, 2e5 particles, St=2, two-way coupling at mass loading 0.5, "
-                               "J = Var(C) of the final positions, gradients w.r.t. every particle's tau and "
-                               "initial state, the forcing coefficient A and the initial RU, Rho, P; "
+                               "heat exchange on (per-particle eps in [0.5, 1]), "
+                               "J = Var(C) of the final positions, gradients w.r.t. every particle's tau, eps and "
+                               "initial state (with T), the forcing coefficient A and the initial RU, Rho, P, P0; "
                                "particle checkpoint every 10 steps, gas tape kept per substep (it fits in "
                                "HBM; recompute_* fields: the gas recomputed per step instead)"},
         "forward_s": round(t_fwd, 4), "recorded_forward_s": round(t_rec, 4), "reverse_s": round(t_rev, 4),
