@@ -216,7 +216,8 @@ def _noncubic_case(shape, seed):
     return ic.Case(p, RU, Rho, np.zeros((nz, ny, nx)), empty)
 
 
-@pytest.mark.parametrize("shape", [(16, 512, 8), (8, 16, 512), (512, 8, 16), (32, 64, 128), (256, 8, 16)])
+@pytest.mark.parametrize("shape", [(16, 512, 8), (8, 16, 512), (512, 8, 16), (32, 64, 128), (256, 8, 16),
+                                   (16, 16, 256), (16, 256, 16)])
 def test_fft_lengths_noncubic(gpu_available, shape):
     """Every axis length has its own FFT kernel shape (four-step register
     kernels for 16..256, LDS Stockham kernels for 8 and 512): one step on

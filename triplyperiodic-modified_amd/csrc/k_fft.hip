@@ -171,24 +171,94 @@ __device__ __forceinline__ double Cface(const Ix &ix, const double *__restrict__
   return rcp_nr((rho[p] + rho[q]) * 0.5);
 }
 
-// residual r = (div_F2C(C grad_C2F P) - RHS) * cc at cell (i,j,k)
+// residual r = (div_F2C(C grad_C2F P) - RHS) * cc from P at the cell and its
+// six neighbours, C on the cell's six faces and the raw RHS
+__device__ __forceinline__ double residual_of(const Geo &g, double pc, double pxm, double pxp, double pym,
+                                              double pyp, double pzm, double pzp, double cx0, double cx1,
+                                              double cy0, double cy1, double cz0, double cz1, double rhs,
+                                              double mean) {
+  double third = 1. / 3.;
+  double cc = -rcp_nr(third * ((cx0 + cx1) * 0.5 + (cy0 + cy1) * 0.5 + (cz0 + cz1) * 0.5));
+  double gx0 = ((pc - pxm) * g.idx) * cx0, gx1 = ((pxp - pc) * g.idx) * cx1;
+  double gy0 = ((pc - pym) * g.idy) * cy0, gy1 = ((pyp - pc) * g.idy) * cy1;
+  double gz0 = ((pc - pzm) * g.idz) * cz0, gz1 = ((pzp - pc) * g.idz) * cz1;
+  double rn = (gx1 - gx0) * g.idx + (gy1 - gy0) * g.idy + (gz1 - gz0) * g.idz;
+  double RHS = rhs - mean;  // RHS_Pois after make_mean_zero
+  rn = rn - RHS;
+  return rn * cc;
+}
+
+// the residual at cell (i,j,k)
 __device__ __forceinline__ double residual(const XArgs &a, const Ix &ix, int i, int j, int k, double mean) {
   const double *__restrict__ rho = a.rho;
   const double *__restrict__ P = a.P;
-  const Geo &g = a.g;
-  double cx0 = Cface<0>(ix, rho, i, j, k), cx1 = Cface<0>(ix, rho, i + 1, j, k);
-  double cy0 = Cface<1>(ix, rho, i, j, k), cy1 = Cface<1>(ix, rho, i, j + 1, k);
-  double cz0 = Cface<2>(ix, rho, i, j, k), cz1 = Cface<2>(ix, rho, i, j, k + 1);
-  double third = 1. / 3.;
-  double cc = -rcp_nr(third * ((cx0 + cx1) * 0.5 + (cy0 + cy1) * 0.5 + (cz0 + cz1) * 0.5));
-  double pc = P[ix(i, j, k)];
-  double gx0 = ((pc - P[ix(i - 1, j, k)]) * g.idx) * cx0, gx1 = ((P[ix(i + 1, j, k)] - pc) * g.idx) * cx1;
-  double gy0 = ((pc - P[ix(i, j - 1, k)]) * g.idy) * cy0, gy1 = ((P[ix(i, j + 1, k)] - pc) * g.idy) * cy1;
-  double gz0 = ((pc - P[ix(i, j, k - 1)]) * g.idz) * cz0, gz1 = ((P[ix(i, j, k + 1)] - pc) * g.idz) * cz1;
-  double rn = (gx1 - gx0) * g.idx + (gy1 - gy0) * g.idy + (gz1 - gz0) * g.idz;
-  double RHS = a.rhs[ix(i, j, k)] - mean;  // RHS_Pois after make_mean_zero
-  rn = rn - RHS;
-  return rn * cc;
+  return residual_of(a.g, P[ix(i, j, k)], P[ix(i - 1, j, k)], P[ix(i + 1, j, k)], P[ix(i, j - 1, k)],
+                     P[ix(i, j + 1, k)], P[ix(i, j, k - 1)], P[ix(i, j, k + 1)], Cface<0>(ix, rho, i, j, k),
+                     Cface<0>(ix, rho, i + 1, j, k), Cface<1>(ix, rho, i, j, k), Cface<1>(ix, rho, i, j + 1, k),
+                     Cface<2>(ix, rho, i, j, k), Cface<2>(ix, rho, i, j, k + 1), a.rhs[ix(i, j, k)], mean);
+}
+
+// v moved along the 16-lane rows of the wave: CTRL 0x121 (row_ror:1) gives
+// each lane the value of the lane before it in its row, 0x12F (row_ror:15)
+// that of the lane after it (the first and last lanes wrap within the row)
+template <int CTRL>
+__device__ __forceinline__ double row_rotate(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// The residuals of one whole x-row as the four-step x transform holds it:
+// the 16 lanes ta of a wave's row take x = ta + 16 m, m < R2 (nx = 16 R2).
+// The x neighbours of a point are then the lanes beside it (or, at the ends
+// of the lane row, the lane at the other end one m over), so P and Rho at
+// x - 1 and x + 1, and C on the face above x, come from the neighbouring
+// lanes instead of memory: 11 loads and 6 reciprocals a point instead of 15
+// and 7, the same arithmetic.  Every lane of the wave must be active.
+template <int R2>
+__device__ __forceinline__ void row_residuals(const XArgs &a, const Ix &ix, int ta, int j, int z, double mean,
+                                              double2 (&v)[R2]) {
+  const double *__restrict__ rho = a.rho;
+  const double *__restrict__ P = a.P;
+  const double *__restrict__ rhs = a.rhs;
+  const long rc = (long)(z & ix.mz) * ix.nxy + (long)(j & ix.my) * ix.nx;
+  const long ryl = (long)(z & ix.mz) * ix.nxy + (long)((j - 1) & ix.my) * ix.nx;
+  const long ryh = (long)(z & ix.mz) * ix.nxy + (long)((j + 1) & ix.my) * ix.nx;
+  const long rzl = (long)((z - 1) & ix.mz) * ix.nxy + (long)(j & ix.my) * ix.nx;
+  const long rzh = (long)((z + 1) & ix.mz) * ix.nxy + (long)(j & ix.my) * ix.nx;
+  // value at x - 1 / x + 1 of the point whose own value is cur, given the
+  // value of the previous / next point of the lane
+  auto lo_of = [&](double cur, double prev) {
+    const double u = row_rotate<0x121>(cur), w = row_rotate<0x121>(prev);
+    return ta == 0 ? w : u;
+  };
+  auto hi_of = [&](double cur, double next) {
+    const double u = row_rotate<0x12F>(cur), w = row_rotate<0x12F>(next);
+    return ta == 15 ? w : u;
+  };
+  double pp = P[rc + ta + 16 * (R2 - 1)], rp = rho[rc + ta + 16 * (R2 - 1)];  // the point before m = 0
+  double pcur = P[rc + ta], rcur = rho[rc + ta];
+  double cx0 = rcp_nr((rcur + lo_of(rcur, rp)) * 0.5);  // C on the x face below the point
+#pragma unroll
+  for (int m = 0; m < R2; ++m) {
+    const int x = ta + 16 * m, xn = ta + 16 * ((m + 1) % R2);
+    const double pn = P[rc + xn], rn = rho[rc + xn];
+    const double pym = P[ryl + x], pyp = P[ryh + x], pzm = P[rzl + x], pzp = P[rzh + x];
+    const double qym = rho[ryl + x], qyp = rho[ryh + x], qzm = rho[rzl + x], qzp = rho[rzh + x];
+    const double r0 = rhs[rc + x];
+    const double cx0n = rcp_nr((rn + lo_of(rn, rcur)) * 0.5);  // of the next point
+    const double cx1 = hi_of(cx0, cx0n);                       // = C on the face below x + 1
+    const double cy0 = rcp_nr((rcur + qym) * 0.5), cy1 = rcp_nr((qyp + rcur) * 0.5);
+    const double cz0 = rcp_nr((rcur + qzm) * 0.5), cz1 = rcp_nr((qzp + rcur) * 0.5);
+    v[m] = make_double2(residual_of(a.g, pcur, lo_of(pcur, pp), hi_of(pcur, pn), pym, pyp, pzm, pzp, cx0, cx1, cy0,
+                                    cy1, cz0, cz1, r0, mean),
+                        0.0);
+    pp = pcur;
+    pcur = pn;
+    rcur = rn;
+    cx0 = cx0n;
+  }
 }
 
 // X forward pass: lpb consecutive x lines per block
@@ -706,8 +776,12 @@ struct k_xy_fwd {
   double2 v[R2];
   {
     const int j = t + Y1 * ca;
+    if constexpr (R1 == 16) {
+      row_residuals<R2>(a, ix, ta, j, z, mean, v);
+    } else {
 #pragma unroll
-    for (int m = 0; m < R2; ++m) v[m] = make_double2(residual(a, ix, ta + R1 * m, j, z, mean), 0.0);
+      for (int m = 0; m < R2; ++m) v[m] = make_double2(residual(a, ix, ta + R1 * m, j, z, mean), 0.0);
+    }
     step_a<R2, false>(v, ta, a.tw);
   }
   const bool b_item = (int)threadIdx.x < Y2 * R2;
