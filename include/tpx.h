@@ -53,6 +53,9 @@ typedef struct tpx_params {
   double t0, mu0, k;
   double gx1, gy1, gz1, gx2, gy2, gz2;
   double cp, cv, np0, dp, rhop, cvp, epsilonp, nu, i01, i02;
+  int solve_for_scalar;           /* solve_for_scalar: advance the passive scalar */
+  int s1_type;                    /* S1_type != 0 (the source form of params.h:78) */
+  double d_m;                     /* D_M: its molecular diffusivity */
 } tpx_params;
 
 typedef struct tpx_ctx tpx_ctx;
@@ -62,7 +65,8 @@ typedef enum tpx_field {
   TPX_RU = 0, TPX_RU_INT, TPX_RU_NEW, TPX_RU_NP1,  /* tensor1, comp 0..2 */
   TPX_RHS_RU,                                       /* tensor1: deposited drag */
   TPX_P = 16, TPX_RHS_POIS, TPX_C, TPX_RHO, TPX_RHO_INT, TPX_RHO_NEW,
-  TPX_RHO_NP1, TPX_DIVERGENCE, TPX_RHS_PART_TEMP
+  TPX_RHO_NP1, TPX_DIVERGENCE, TPX_RHS_PART_TEMP,
+  TPX_PASSIVE_SCALAR, TPX_PASSIVE_SCALAR_INT, TPX_PASSIVE_SCALAR_NEW, TPX_PASSIVE_SCALAR_NP1
 } tpx_field;
 
 /* particle stage identifiers (particle.h:63-113) */
@@ -116,6 +120,9 @@ int tpx_params_update(tpx_ctx *ctx);                 /* params.cpp:75-86 at T_cu
 int tpx_update_rho(tpx_ctx *ctx, int rk);            /* grid.cpp:608-614 */
 int tpx_update_p0(tpx_ctx *ctx, int rk);             /* grid.cpp:816-830 */
 int tpx_update_particle(tpx_ctx *ctx, int rk);       /* grid.cpp:865-888 */
+/* grid.cpp:711-745 (C_Source :686-708); tpx_step runs it after
+ * Update_Particle when solve_for_scalar is set (simulation.cpp:65) */
+int tpx_update_passive_scalar(tpx_ctx *ctx, int rk);
 int tpx_update_ru_wop(tpx_ctx *ctx, int rk);         /* grid.cpp:776-814 */
 int tpx_compute_div_u_new(tpx_ctx *ctx, int rk);     /* grid.cpp:832-839 */
 int tpx_compute_rhs_pois(tpx_ctx *ctx, int rk);      /* grid.cpp:841-849 */
@@ -236,15 +243,16 @@ int tpx_get_stream(tpx_ctx *ctx, void **stream);
  * RU_new / I_C2F(Rho_new) — the reference's stale U — and the maximum of the
  * gathered gas x velocity); tpx_statistics then reduces the rest over the
  * state at the step boundary and writes TPX_ST_COUNT values (reductions on
- * the device, scalars to the host).  The passive scalar and vector are not
- * built: their entries are 0, as they are in the reference with them off. */
+ * the device, scalars to the host).  The passive vector is not built: its
+ * entries are 0, as they are in the reference with it off. */
 enum {
   TPX_ST_TG_MEAN = 0,   /* mean of P0 / (R Rho) */
   TPX_ST_TP_MEAN,       /* mean particle temperature */
   TPX_ST_HT_MEAN,       /* mean of RHS_Part_Temp (mean_energy_transferred) */
   TPX_ST_TKE2,          /* mean_squares of the last substep's pre-projection U */
   TPX_ST_TKE_U, TPX_ST_TKE_V, TPX_ST_TKE_W,  /* mean of U_c RU_c, U = RU / I_C2F(Rho) */
-  TPX_ST_PASSIVE_SCALAR_MEAN, TPX_ST_TKEV_V1, TPX_ST_TKEV_V2, TPX_ST_TKEV_V3, TPX_ST_VV,  /* 0 */
+  TPX_ST_PASSIVE_SCALAR_MEAN,  /* mean_squares of the passive scalar (0 if it has no buffers) */
+  TPX_ST_TKEV_V1, TPX_ST_TKEV_V2, TPX_ST_TKEV_V3, TPX_ST_VV,  /* 0: no passive vector */
   TPX_ST_P0,
   TPX_ST_C_MAX, TPX_ST_C_MIN, TPX_ST_C_MEAN,
   TPX_ST_RHO_MAX, TPX_ST_RHO_MIN, TPX_ST_RHO_MEAN,

@@ -14,8 +14,8 @@
 #include "oracle.h"
 
 #define OR_PI 3.141592653589793 /* params.h:5 */
-#define NT1 9
-#define NT0 11
+#define NT1 10
+#define NT0 17
 
 struct or_state {
   or_params p;
@@ -765,6 +765,9 @@ void or_initialize(or_state *s)
   }
   ew_copy(s, T0(s, OR_RHO_INT), T0(s, OR_RHO));
   ew_copy(s, T0(s, OR_RHO_NP1), T0(s, OR_RHO));
+  ew_copy(s, T0(s, OR_PS_INT), T0(s, OR_PS));  /* grid.cpp:465-466, 476 */
+  ew_copy(s, T0(s, OR_PS_NP1), T0(s, OR_PS));
+  i_c2f(s, T1(s, OR_PS_FACE), T0(s, OR_PS));
   s->P0_int = s->P0; s->P0_np1 = s->P0;
   i_c2f(s, T1(s, OR_RHO_FACE), T0(s, OR_RHO));
   for (int a = 0; a < 7; ++a) {
@@ -902,8 +905,57 @@ void or_time_advance_rk4(or_state *s)
 {
   ew_copy(s, T0(s, OR_RHO_INT), T0(s, OR_RHO_NEW));
   for (int c = 0; c < 3; ++c) ew_copy(s, T1(s, OR_RU_INT)[c], T1(s, OR_RU_NEW)[c]);
+  ew_copy(s, T0(s, OR_PS_INT), T0(s, OR_PS_NEW));  /* grid.cpp:897 */
   s->P0_int = s->P0_new;
   for (int a = 0; a < 7; ++a) memcpy(s->q[a][OR_INT], s->q[a][OR_NEW], sizeof(double) * (size_t)s->np);
+}
+
+/* grid::C_Source (grid.cpp:686-708) with scalar_source.h:4-8: A cos(k x),
+ * A = k = 1, at the cell centre X = dx/2 + i dx (the reference passes
+ * (X, Y, X, T); the function reads only its first argument) */
+static void c_source(or_state *s, double *o)
+{
+  IN_LOOP(s) {
+    long c = G(s, i, j, k);
+    double X = s->dx / 2. + (i - s->bs) * s->dx;
+    double A = 1., kk = 1.;
+    o[c] = A * cos(kk * X);
+  }
+  fill(s, o);
+}
+
+/* grid::Update_Passive_Scalar (grid.cpp:711-745): flux-form transport by
+ * U = RU_int/Rho_face of the scalar on the faces (I_C2F of the previous
+ * substep's new value), molecular diffusion D_M del2, a source (S1_type 0:
+ * C_Source minus its mean; else -Ix_F2C(U.x)), then the RK4 updates. */
+void or_update_passive_scalar(or_state *s)
+{
+  double dt = s->p.dt;
+  double *const *U = T1(s, OR_U), *const *D = T1(s, OR_DUMMY), *const *D2 = T1(s, OR_DUMMY2);
+  double *const *F = T1(s, OR_PS_FACE);
+  double *rhs = T0(s, OR_RHS_PS), *S1 = T0(s, OR_S1);
+  for (int c = 0; c < 3; ++c) ew_div(s, U[c], T1(s, OR_RU_INT)[c], T1(s, OR_RHO_FACE)[c]);
+  for (int c = 0; c < 3; ++c) {  /* tensor1::Equal_Mult(U, Passive_Scalar_face) */
+    double *o = D[c];
+    const double *a = U[c], *b = F[c];
+#pragma omp parallel for schedule(static)
+    EW_LOOP o[e] = a[e] * b[e];
+  }
+  div_f2c(s, D2[0], D);
+  del2(s, D2[1], T0(s, OR_PS_INT));
+  c_source(s, S1);
+  ew_lc2(s, rhs, s->p.d_m, D2[1], -1, D2[0]);
+  if (!s->p.s1_type) {
+    ew_sub_s(s, S1, red_mean(s, S1));
+    ew_add(s, rhs, S1);
+  } else {
+    i_f2c_axis(s, D[0], U[0], 0);
+    ew_pem(s, rhs, -1, D[0]);
+  }
+  ew_pem(s, T0(s, OR_PS_NP1), dt * s->post[s->rk], rhs);
+  if (s->rk != 3) ew_lc2(s, T0(s, OR_PS_NEW), 1, T0(s, OR_PS), dt * s->pre[s->rk], rhs);
+  else ew_copy(s, T0(s, OR_PS_NEW), T0(s, OR_PS_NP1));
+  i_c2f(s, F, T0(s, OR_PS_NEW));
 }
 
 /* G12: tensor1::make_mean_U0 on RU_np1 (tensor1.cpp:406-411, tensor0.cpp:386-396) */
@@ -922,14 +974,16 @@ void or_time_advance(or_state *s)
 {
   ew_copy(s, T0(s, OR_RHO), T0(s, OR_RHO_NP1));
   for (int c = 0; c < 3; ++c) ew_copy(s, T1(s, OR_RU)[c], T1(s, OR_RU_NP1)[c]);
+  ew_copy(s, T0(s, OR_PS), T0(s, OR_PS_NP1));  /* grid.cpp:598 */
   s->P0 = s->P0_np1;
   for (int a = 0; a < 7; ++a) memcpy(s->q[a][OR_BASE], s->q[a][OR_NP1], sizeof(double) * (size_t)s->np);
   s->T_cur += s->p.dt;
   s->num_timestep++;
 }
 
-/* simulation.cpp:57-93 with Update_Particle enabled at :64 (scalar, vector,
- * elongated box, statistics and touch switch off). */
+/* simulation.cpp:57-93 with Update_Particle enabled at :64 and the passive
+ * scalar when solve_for_scalar is set (vector, elongated box, statistics and
+ * touch switch off). */
 void or_step(or_state *s, int nsteps)
 {
   double a[3] = {s->p.u0 * s->p.rho0, s->p.v0 * s->p.rho0, s->p.w0 * s->p.rho0};
@@ -939,6 +993,7 @@ void or_step(or_state *s, int nsteps)
       or_update_rho(s);
       or_update_p0(s);
       or_update_particle(s);
+      if (s->p.solve_for_scalar) or_update_passive_scalar(s);
       or_update_ru_wop(s);
       or_compute_div_u_new(s);
       or_compute_rhs_pois(s);

@@ -40,6 +40,11 @@
  *                           partition of unity, separable exactness
  *   particle RK4 (P2-P4)    frozen uniform gas = classical RK4 matrix map
  *   or_p_send_recv (P7)     ids kept, one-period shifts into [0, L]
+ *   or_update_passive_scalar (grid.cpp:686-745)
+ *                           RK4 amplification of a diffusing single mode and of
+ *                           a mode carried by a uniform velocity (closed
+ *                           forms), exact growth under either source form,
+ *                           conservation of the scalar's sum
  * Parity unpinned: the order in which or_p_send_recv re-appends wrapped
  * particles (every comparison matches particles by flag, so it cannot change
  * a result), and the composition of the methods in or_step beyond what the
@@ -67,6 +72,8 @@ typedef struct or_params {
   double t0, mu0, k;
   double gx1, gy1, gz1, gx2, gy2, gz2;
   double cp, cv, np0, dp, rhop, cvp, epsilonp, nu, i01, i02;
+  int solve_for_scalar, s1_type;  /* passive scalar switch and source form */
+  double d_m;                     /* its molecular diffusivity */
 } or_params;
 
 typedef struct or_state or_state;
@@ -74,9 +81,10 @@ typedef struct or_state or_state;
 /* field identifiers */
 enum {
   OR_RU = 0, OR_RU_INT, OR_RU_NEW, OR_RU_NP1, OR_RHS_RU, OR_U, OR_RHO_FACE,
-  OR_DUMMY, OR_DUMMY2,                     /* tensor1 (component 0..2) */
+  OR_DUMMY, OR_DUMMY2, OR_PS_FACE,         /* tensor1 (component 0..2) */
   OR_P = 16, OR_RHS_POIS, OR_C, OR_RHO, OR_RHO_INT, OR_RHO_NEW, OR_RHO_NP1,
-  OR_T, OR_DIVERGENCE, OR_RHS_PART_TEMP, OR_RHS_RHO  /* tensor0 */
+  OR_T, OR_DIVERGENCE, OR_RHS_PART_TEMP, OR_RHS_RHO,
+  OR_PS, OR_PS_INT, OR_PS_NEW, OR_PS_NP1, OR_RHS_PS, OR_S1  /* tensor0; PS the passive scalar */
 };
 /* particle stage identifiers */
 enum { OR_BASE = 0, OR_INT = 1, OR_NEW = 2, OR_NP1 = 3 };
@@ -118,6 +126,7 @@ void or_params_update(or_state *s);           /* params.cpp:75-86 with T_cur */
 void or_set_rk4_count(or_state *s, int rk);
 void or_update_rho(or_state *s);              /* grid.cpp:608-614 */
 void or_update_p0(or_state *s);               /* grid.cpp:816-830 */
+void or_update_passive_scalar(or_state *s);   /* grid.cpp:711-745 */
 void or_update_particle(or_state *s);         /* grid.cpp:865-888 */
 void or_update_ru_wop(or_state *s);           /* grid.cpp:776-814 */
 void or_compute_div_u_new(or_state *s);       /* grid.cpp:832-839 */

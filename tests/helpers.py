@@ -21,6 +21,8 @@ def oracle_from_case(case: ic.Case) -> Oracle:
     pp = case.particles
     o.set_particles(pp["x"], pp["y"], pp["z"], pp["u"], pp["v"], pp["w"], pp["T"], pp["flag"],
                     tau=case.tau, eps=case.eps)
+    if getattr(case, "PS", None) is not None:
+        o.set("Passive_Scalar", case.PS)
     o.set_restart_numbers(case.params.P0, 0.0, 0)
     o.initialize()
     return o

@@ -281,8 +281,8 @@ struct k_reduce {
   __shared__ double sh[kBlock];
   long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
   double v = 0.0;
-  if (c < n) v = OP == 0 ? f[c] : fabs(f[c]);
-  double s = OP == 0 ? block_sum(v, sh) : block_max(v, sh);
+  if (c < n) v = OP == 0 ? f[c] : (OP == 2 ? f[c] * f[c] : fabs(f[c]));
+  double s = OP == 1 ? block_max(v, sh) : block_sum(v, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 };
@@ -421,6 +421,81 @@ void launch_rhs_ru(tpx_ctx *c, const Coef &k, bool with_rho, const double *const
   timing_end(c, TPX_K_GRID_RHS);
 }
 
+// grid::Update_Passive_Scalar (grid.cpp:711-745) in one pass: the scalar on
+// the faces is I_C2F of its int value (the reference keeps it from the
+// previous substep's new value, grid.cpp:744, 476), U = RU_int / I_C2F(Rho_int),
+// flux-form transport, molecular diffusion D_M del2, the source (form 0:
+// C_Source (grid.cpp:686-708, scalar_source.h:4-8: cos(X) at the cell
+// centre) minus its mean, held in SC_PS_S1 as a sum; form 1: -Ix_F2C(U.x)),
+// then the RK4 updates of np1 (in place) and new, operand for operand.
+struct PsArgs {
+  RU3 ru_int;
+  const double *rho_int, *cint, *cbase;
+  double *cnp1, *cnew;
+  const double *scal;
+  double d_m, c_post, c_pre, ncell;
+  int s1_type, rk3;
+};
+
+struct k_passive_scalar {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const Geo &g, const PsArgs &a) {
+  int i, j, k;
+  long c;
+  if (!cell_of(g, i, j, k, c)) return;
+  Ix ix = make_ix(g);
+  const double *__restrict__ rho = a.rho_int;
+  const double *__restrict__ C = a.cint;
+  const long m[3] = {ix(i - 1, j, k), ix(i, j - 1, k), ix(i, j, k - 1)};
+  const long p[3] = {ix(i + 1, j, k), ix(i, j + 1, k), ix(i, j, k + 1)};
+  const double id[3] = {g.idx, g.idy, g.idz}, id2[3] = {g.idx2, g.idy2, g.idz2};
+  const double rc = rho[c], cc = C[c];
+  double div = 0.0, lap = 0.0, ux0 = 0.0, ux1 = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double u0 = a.ru_int.a[d][c] / ((rc + rho[m[d]]) * 0.5);
+    const double u1 = a.ru_int.a[d][p[d]] / ((rho[p[d]] + rc) * 0.5);
+    const double f0 = u0 * ((cc + C[m[d]]) * 0.5), f1 = u1 * ((C[p[d]] + cc) * 0.5);
+    div = d == 0 ? (f1 - f0) * id[d] : div + (f1 - f0) * id[d];
+    const double l = (C[p[d]] - 2 * cc + C[m[d]]) * id2[d];
+    lap = d == 0 ? l : lap + l;
+    if (d == 0) { ux0 = u0; ux1 = u1; }
+  }
+  double rhs = a.d_m * lap + -1 * div;
+  if (a.s1_type == 0) {
+    const double X = g.dx / 2. + i * g.dx, A = 1., kk = 1.;
+    rhs += A * cos(kk * X) - a.scal[SC_PS_S1] / a.ncell;
+  } else {
+    rhs += -1 * ((ux0 + ux1) * 0.5);
+  }
+  const double np1 = a.cnp1[c] + a.c_post * rhs;
+  a.cnp1[c] = np1;
+  a.cnew[c] = a.rk3 ? np1 : 1 * a.cbase[c] + a.c_pre * rhs;
+}
+};
+
+// block sums of C_Source over the cells (its mean is removed in form 0)
+struct k_ps_source_sum {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const Geo &g, double *partials) {
+  __shared__ double sh[kBlock];
+  int i, j, k;
+  long c;
+  double v = 0.0;
+  if (cell_of(g, i, j, k, c)) {
+    const double X = g.dx / 2. + i * g.dx, A = 1., kk = 1.;
+    v = A * cos(kk * X);
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+};
+
 void launch_div_rhs(tpx_ctx *c, const Coef &k, int mode, const double *const ru_new[3],
                     const double *rho_new, const double *Q, double *divergence, double *rhs) {
   DivArgs a{};
@@ -484,6 +559,9 @@ void launch_reduce(tpx_ctx *c, const double *f, int op, int slot) {
   if (op == 0) {
     tpx_launch<k_reduce<0>>(nb, kBlock, 0, c->stream, c->g.n, f, c->partials);
     tpx_launch<k_finish<0>>(1, 1024, 0, c->stream, c->partials, nb, c->scal, slot);
+  } else if (op == 2) {  // sum of squares
+    tpx_launch<k_reduce<2>>(nb, kBlock, 0, c->stream, c->g.n, f, c->partials);
+    tpx_launch<k_finish<0>>(1, 1024, 0, c->stream, c->partials, nb, c->scal, slot);
   } else {
     tpx_launch<k_reduce<1>>(nb, kBlock, 0, c->stream, c->g.n, f, c->partials);
     tpx_launch<k_finish<1>>(1, 1024, 0, c->stream, c->partials, nb, c->scal, slot);
@@ -495,6 +573,34 @@ void launch_reduce(tpx_ctx *c, const double *f, int op, int slot) {
 void launch_finish_sum(tpx_ctx *c, int nparts, int part_offset, int slot) {
   tpx_launch<k_finish<0>>(1, 1024, 0, c->stream, c->partials + part_offset, nparts, c->scal, slot);
   TPX_LAUNCH_CHECK();
+}
+
+void launch_passive_scalar(tpx_ctx *c, const Coef &k, const double *const ru_int[3], const double *rho_int) {
+  PsArgs a{};
+  for (int d = 0; d < 3; ++d) a.ru_int.a[d] = ru_int[d];
+  a.rho_int = rho_int;
+  a.cint = c->ps[c->ps_int];
+  a.cbase = c->ps[c->ps_base];
+  a.cnp1 = c->ps[c->ps_np1];
+  a.cnew = c->ps[c->ps_new];
+  a.scal = c->scal;
+  a.d_m = c->p.d_m;
+  a.c_post = k.dt * k.post;
+  a.c_pre = k.dt * k.pre;
+  a.ncell = (double)c->g.n;
+  a.s1_type = c->p.s1_type;
+  a.rk3 = k.rk == 3;
+  timing_begin(c, TPX_K_OTHER);
+  tpx_launch<k_passive_scalar>(cells_grid(c), kBlock, 0, c->stream, c->g, a);
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_OTHER);
+}
+
+void launch_passive_scalar_source_sum(tpx_ctx *c) {
+  const unsigned nb = cells_grid(c);
+  tpx_launch<k_ps_source_sum>(nb, kBlock, 0, c->stream, c->g, c->partials);
+  TPX_LAUNCH_CHECK();
+  launch_finish_sum(c, (int)nb, 0, SC_PS_S1);
 }
 
 void launch_update_p0(tpx_ctx *c, const Coef &k) {

@@ -113,7 +113,17 @@ void ru_wptrs(tpx_ctx *c, int b, double *out[3]) {
   for (int d = 0; d < 3; ++d) out[d] = c->ru[b][d];
 }
 
+// the passive scalar's four buffers, allocated (zero) on first use
+void ensure_passive_scalar(tpx_ctx *c) {
+  if (c->ps[0]) return;
+  for (auto &f : c->ps) {
+    if (hipMalloc(&f, sizeof(double) * (size_t)c->g.n) != hipSuccess) throw std::bad_alloc();
+    hip_check(hipMemsetAsync(f, 0, sizeof(double) * (size_t)c->g.n, c->stream), "hipMemsetAsync");
+  }
+}
+
 double *field_buffer(tpx_ctx *c, int field, int comp) {
+  if (field >= TPX_PASSIVE_SCALAR && field <= TPX_PASSIVE_SCALAR_NP1) ensure_passive_scalar(c);
   switch (field) {
   case TPX_RU: return c->ru[c->ru_base][comp];
   case TPX_RU_INT: return c->ru[c->ru_int][comp];
@@ -129,6 +139,10 @@ double *field_buffer(tpx_ctx *c, int field, int comp) {
   case TPX_RHO_NP1: return c->rho[rho_np1_read(c)];
   case TPX_DIVERGENCE: return c->divergence;
   case TPX_RHS_PART_TEMP: return c->Q;
+  case TPX_PASSIVE_SCALAR: return c->ps[c->ps_base];
+  case TPX_PASSIVE_SCALAR_INT: return c->ps[c->ps_int];
+  case TPX_PASSIVE_SCALAR_NEW: return c->ps[c->ps_new];
+  case TPX_PASSIVE_SCALAR_NP1: return c->ps[c->ps_np1];
   default: return nullptr;
   }
 }
@@ -207,6 +221,15 @@ void do_update_particle(tpx_ctx *c, int rk) {
   }
 }
 
+// grid::Update_Passive_Scalar (grid.cpp:711-745)
+void do_update_passive_scalar(tpx_ctx *c, int rk) {
+  ensure_passive_scalar(c);
+  Coef k = coef(c, rk);
+  const double *ri[3];
+  ru_ptrs(c, c->ru_int, ri);
+  launch_passive_scalar(c, k, ri, c->rho[c->rho_int]);
+}
+
 void do_update_ru_wop(tpx_ctx *c, int rk, bool with_rho) {
   Coef k = coef(c, rk);
   int np1_in = ru_np1_read(c);
@@ -273,6 +296,7 @@ void do_time_advance_rk4(tpx_ctx *c) {
   c->rho_new = -1;
   c->ru_new = -1;
   launch_scalar_copy(c, SC_P0_INT, SC_P0_NEW);
+  if (c->ps[0]) std::swap(c->ps_int, c->ps_new);  // Passive_Scalar_int = Passive_Scalar_new (grid.cpp:897)
   Particles &pt = c->pt;
   std::swap(pt.st_int, pt.st_new);  // with int_is_base, new receives the spare buffer
   pt.int_is_base = false;
@@ -301,6 +325,7 @@ void do_time_advance(tpx_ctx *c) {
   c->ru_base = ru_np1_read(c);
   c->ru_np1_is_base = true;
   launch_scalar_copy(c, SC_P0, SC_P0_NP1);
+  if (c->ps[0]) launch_copy(c, c->ps[c->ps_base], c->ps[c->ps_np1], c->g.n);  // grid.cpp:598
   c->T_cur += c->p.dt;
   c->num_timestep++;
   Particles &pt = c->pt;
@@ -345,6 +370,7 @@ void fused_substep_pre(tpx_ctx *c, int rk) {
     launch_particle_advance(c, k, ri, c->rho[c->rho_int], rebin);
     if (c->p.twc) launch_velocity_deposit(c);
   }
+  if (c->p.solve_for_scalar) do_update_passive_scalar(c, rk);  // simulation.cpp:65
   do_update_ru_wop(c, rk, true);
   if (rebin) {
     launch_sort_new(c);
@@ -412,6 +438,7 @@ void release(tpx_ctx *c) {
   for (auto &p : c->lam) f(p);
   for (auto &p : c->tw) f(p);
   f(c->scal); f(c->partials); f(c->scan_tmp); f(c->st_part);
+  for (auto &p : c->ps) f(p);
   adjoint_free(c->adj);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   Particles &pt = c->pt;
@@ -593,8 +620,8 @@ int tpx_upload_field(tpx_ctx *ctx, int field, int comp, const double *host, int 
     if (!host || bs < 0) throw std::invalid_argument("bad host array or bs");
     if (is_vector(field) ? (comp < 0 || comp > 2) : comp != 0) throw std::invalid_argument("bad component");
     if (!(field == TPX_RU || field == TPX_RHO || field == TPX_P || field == TPX_RHS_PART_TEMP ||
-          field == TPX_RHS_RU))
-      throw std::invalid_argument("only RU, Rho, P, RHS_Part_Temp and RHS_RU can be uploaded");
+          field == TPX_RHS_RU || field == TPX_PASSIVE_SCALAR))
+      throw std::invalid_argument("only RU, Rho, P, RHS_Part_Temp, RHS_RU and Passive_Scalar can be uploaded");
     const Geo &g = ctx->g;
     // uploading the base state resets the aliasing of int/np1 to the base
     if (field == TPX_RU && ctx->ru_int != ctx->ru_base) throw std::logic_error("upload RU before the first substep only");
@@ -748,6 +775,12 @@ int tpx_initialize(tpx_ctx *ctx) {
     c->rho_int = c->rho_base; c->rho_new = -1; c->rho_np1 = -1; c->rho_np1_is_base = true;
     launch_scalar_copy(c, SC_P0_INT, SC_P0);
     launch_scalar_copy(c, SC_P0_NP1, SC_P0);
+    if (c->p.solve_for_scalar) ensure_passive_scalar(c);
+    if (c->ps[0]) {  // grid.cpp:465-466 (the face values are formed from int where used)
+      launch_copy(c, c->ps[c->ps_int], c->ps[c->ps_base], c->g.n);
+      launch_copy(c, c->ps[c->ps_np1], c->ps[c->ps_base], c->g.n);
+    }
+    if (c->p.solve_for_scalar) launch_passive_scalar_source_sum(c);
     Particles &pt = c->pt;
     pt.np1_is_base = true;
     if (pt.n > 0) {
@@ -789,6 +822,15 @@ int tpx_update_particle(tpx_ctx *ctx, int rk) {
     TPX_RK_CHECK(rk);
     if (ctx->rho_new < 0) throw std::logic_error("Update_Particle needs Rho_new (call Update_Rho first)");
     do_update_particle(ctx, rk);
+    return TPX_OK;
+  });
+}
+
+int tpx_update_passive_scalar(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (!ctx->p.solve_for_scalar) throw std::logic_error("Update_Passive_Scalar with solve_for_scalar = 0");
+    do_update_passive_scalar(ctx, rk);
     return TPX_OK;
   });
 }
@@ -1401,6 +1443,7 @@ int tpx_batch_create(tpx_ctx *const *ctxs, int n, tpx_batch **out) {
     else if (c->device != a->device) why = "contexts on different devices";
     else if ((c->p.twc != 0) != (a->p.twc != 0)) why = "TWC differs";
     else if (heat_active(c) != heat_active(a)) why = "heat exchange on in some boxes only";
+    else if (c->p.solve_for_scalar) why = "the passive scalar is not batched (use tpx_ensemble_step)";
     else if ((c->pt.n > 0) != (a->pt.n > 0)) why = "particles in some boxes only";
     else if (c->pt.has_tau != a->pt.has_tau || c->pt.has_eps != a->pt.has_eps) why = "per-particle tau/eps differ";
     else if (c->fft_passes != a->fft_passes) why = "Poisson pass counts differ";
@@ -1823,6 +1866,10 @@ int tpx_statistics(tpx_ctx *ctx, double *out) {
     out[TPX_ST_HT_MEAN] = sc[SC_MET];
     out[TPX_ST_TKE2] = sc[SC_ST_TKE2] / n;
     out[TPX_ST_UG_MAX] = sc[SC_ST_UG];
+    if (c->ps[0]) {  // Passive_Scalar.mean_squares() (grid.cpp:913)
+      launch_reduce(c, c->ps[c->ps_base], 2, SC_RED);
+      out[TPX_ST_PASSIVE_SCALAR_MEAN] = read_scalar(c, SC_RED) / n;
+    }
     out[TPX_ST_P0] = sc[SC_P0];
     out[TPX_ST_PARTICLE_MAX_CFL] = 0.0;
     out[TPX_ST_GAS_MAX_DIFF_CFL] = dt / (c->g.dx * c->g.dx * out[TPX_ST_RHO_MIN] / c->p.mu0) * 6.;

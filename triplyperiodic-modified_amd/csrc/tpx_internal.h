@@ -47,6 +47,7 @@ enum Scal {
   // of RHS_Part_Temp (the one the next Update_P0 reads), and of the recorded
   // run's initial P0
   SC_ADJ_P0, SC_ADJ_P0NP1, SC_ADJ_P0NEW, SC_ADJ_LSQ, SC_ADJ_P0I,
+  SC_PS_S1,        // passive scalar: sum over the cells of its source C_Source
   SC_COUNT = 32
 };
 
@@ -230,6 +231,11 @@ struct tpx_ctx {
   double *P = nullptr, *rhs = nullptr, *S[3] = {}, *Q = nullptr, *C = nullptr,
          *divergence = nullptr;
   double *dep_tmp = nullptr;      // 3 x n: row partials of the heat deposit
+  // passive scalar (grid.h:79-83): base, int, new and np1 buffers, allocated
+  // when it is solved or first uploaded; TimeAdvance_RK4 swaps int and new,
+  // TimeAdvance copies np1 into base
+  double *ps[4] = {};
+  int ps_base = 0, ps_int = 1, ps_new = 2, ps_np1 = 3;
   bool rhs_has_mean = false;      // rhs holds the raw RHS, SC_SUM_RHS its sum
   bool q_zero = false;            // Q is known to be identically zero
   bool binned_this_step = false;  // particles were re-binned during the current step
@@ -271,7 +277,7 @@ void launch_project(tpx_ctx *c, const Coef &k, const double *P, double *const ru
                     double *const ru_new[3], bool copy_np1_to_new, bool want_sums);
 void launch_shift_mean(tpx_ctx *c, const double *const in[3], double *const out[3],
                        const double a[3]);
-void launch_reduce(tpx_ctx *c, const double *f, int op, int slot);  // op 0 sum, 1 max|.|
+void launch_reduce(tpx_ctx *c, const double *f, int op, int slot);  // op 0 sum, 1 max|.|, 2 sum of squares
 void launch_finish_sum(tpx_ctx *c, int nparts, int part_offset, int slot);
 void launch_update_p0(tpx_ctx *c, const Coef &k);
 void launch_fill(tpx_ctx *c, double *f, double v, long n);
@@ -284,6 +290,9 @@ int fft_setup(tpx_ctx *c);
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
 // the same iteration on caller-given P, raw right-hand side and scalar slots
 void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal);
+// grid::Update_Passive_Scalar (k_grid.hip) and the sum of its source
+void launch_passive_scalar(tpx_ctx *c, const Coef &k, const double *const ru_int[3], const double *rho_int);
+void launch_passive_scalar_source_sum(tpx_ctx *c);
 // resets the device-side iteration count and loop test of a solve
 void launch_solve_begin(tpx_ctx *c);
 // copies the solve's iteration count and stopped flag to dst[0..1] (device)

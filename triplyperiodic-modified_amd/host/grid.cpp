@@ -25,6 +25,10 @@ grid::grid(gridsize *size, params *param, proc *pc, communicator *com)
   Rho_np1.bind(&ctx_, size_, TPX_RHO_NP1, 0);
   divergence.bind(&ctx_, size_, TPX_DIVERGENCE, 0);
   RHS_Part_Temp.bind(&ctx_, size_, TPX_RHS_PART_TEMP, 0);
+  Passive_Scalar.bind(&ctx_, size_, TPX_PASSIVE_SCALAR, 0);
+  Passive_Scalar_int.bind(&ctx_, size_, TPX_PASSIVE_SCALAR_INT, 0);
+  Passive_Scalar_new.bind(&ctx_, size_, TPX_PASSIVE_SCALAR_NEW, 0);
+  Passive_Scalar_np1.bind(&ctx_, size_, TPX_PASSIVE_SCALAR_NP1, 0);
   if (param_->Statistics()) {
     ok(tpx_statistics_enable(ctx_, 1), "tpx_statistics_enable");
     if (pc_->IsRoot()) {  // grid.cpp:32-57 and open_stat_file (grid.cpp:1147-1154)
@@ -127,6 +131,29 @@ void grid::Initialize() {
   } else {
     unsupported("Initial = 0 (initial turbulence from U/V/W files)");
   }
+  // the passive scalar's initial field (grid.cpp:385-457), read or set when
+  // it is solved: Initial_C 0 and 1 read Passive_Scalar.bin and
+  // Restart_Passive_Scalar.bin, 3-5 are products of two sines
+  int ic = param_->Initial_C();
+  if (param_->solve_for_scalar()) {
+    if (ic == 0) com_->read(Passive_Scalar, "Passive_Scalar.bin");
+    else if (ic == 1) com_->read(Passive_Scalar, "Restart_Passive_Scalar.bin");
+    else if (ic >= 3 && ic <= 5) {
+      std::vector<double> c(Passive_Scalar.ghosted_size(), 0.0);
+      int b = g.bs();
+      long gx = g.Nx() + 2 * b, gy = g.Ny() + 2 * b;
+      double kx = TWO_PI / g.Lx(), ky = TWO_PI / g.Ly(), kz = TWO_PI / g.Lz();
+      for (int k = 0; k < g.Nz(); ++k)
+        for (int j = 0; j < g.Ny(); ++j)
+          for (int i = 0; i < g.Nx(); ++i) {
+            double X = g.dx() / 2. + i * g.dx(), Y = g.dy() / 2. + j * g.dy(), Z = g.dz() / 2. + k * g.dz();
+            double v = ic == 5 ? std::sin(kx * X) * std::sin(ky * Y)
+                     : ic == 4 ? std::sin(kz * Z) * std::sin(kx * X) : std::sin(ky * Y) * std::sin(kz * Z);
+            c[((long)(k + b) * gy + (j + b)) * gx + (i + b)] = v;
+          }
+      Passive_Scalar.upload(c.data());
+    }
+  }
   ok(tpx_initialize(ctx_), "Initialize");
   pull_scalars();
 }
@@ -143,6 +170,7 @@ void grid::Update_P0() {
 }
 
 void grid::Update_Particle() { ok(tpx_update_particle(ctx_, RK4_count), "Update_Particle"); }
+void grid::Update_Passive_Scalar() { ok(tpx_update_passive_scalar(ctx_, RK4_count), "Update_Passive_Scalar"); }
 void grid::Update_RU_WOP() { ok(tpx_update_ru_wop(ctx_, RK4_count), "Update_RU_WOP"); }
 void grid::Compute_Div_U_new() { ok(tpx_compute_div_u_new(ctx_, RK4_count), "Compute_Div_U_new"); }
 void grid::Compute_RHS_Pois() { ok(tpx_compute_rhs_pois(ctx_, RK4_count), "Compute_RHS_Pois"); }
@@ -237,6 +265,8 @@ void grid::Store() {
   };
   if (num_timestep % param_->data_freq_fast() == 0 || Is_touch_) {
     com_->write(RU, name("RU", num_timestep, ".bin").c_str());
+    // the passive scalar as C_n.bin (grid.cpp:538-544; RV_n.bin: no passive vector here)
+    if (param_->solve_for_scalar()) com_->write(Passive_Scalar, name("C", num_timestep, ".bin").c_str());
     std::ofstream numbers(name("numbers", num_timestep, ".dat"));
     numbers.precision(16);
     numbers << P0 << std::endl << T_cur << std::endl << num_timestep << std::endl;

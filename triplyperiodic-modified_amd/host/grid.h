@@ -33,6 +33,7 @@ class grid {
   particle part;
   tensor1 RU, RU_int, RU_new, RU_np1, RHS_RU;
   tensor0 P, RHS_Pois, C, Rho, Rho_int, Rho_new, Rho_np1, divergence, RHS_Part_Temp;
+  tensor0 Passive_Scalar, Passive_Scalar_int, Passive_Scalar_new, Passive_Scalar_np1;
   double P0 = 0, P0_int = 0, P0_new = 0, P0_np1 = 0, dP0_dt = 0;
   double T_cur = 0;
   int num_timestep = 0;
@@ -55,13 +56,13 @@ class grid {
   void Solve_Poisson();       // grid.cpp:851-855
   void Update_RU_WP();        // grid.cpp:857-863
   void Update_Particle();     // grid.cpp:865-888
+  void Update_Passive_Scalar();  // grid.cpp:711-745
   void TimeAdvance_RK4();     // grid.cpp:890-900
   void Statistics();          // grid.cpp:902-982 (P0, iterations, density)
   bool Touch();               // grid.cpp:94-108
   int num_iteration() const { return PS_.num_iteration(); }
 
-  // passive scalar / vector, elongated-box copy: outside this build
-  void Update_Passive_Scalar() { unsupported("Update_Passive_Scalar"); }
+  // passive vector, elongated-box copy: outside this build
   void Update_RV_WOQ() { unsupported("Update_RV_WOQ"); }
   void Compute_RHS_Pois_Q() { unsupported("Compute_RHS_Pois_Q"); }
   void Solve_Poisson_Q() { unsupported("Solve_Poisson_Q"); }

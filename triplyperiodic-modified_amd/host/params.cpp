@@ -92,6 +92,7 @@ tpx_params params::to_abi() const {
   p.gx1 = gx1_; p.gy1 = gy1_; p.gz1 = gz1_; p.gx2 = gx2_; p.gy2 = gy2_; p.gz2 = gz2_;
   p.cp = Cp_; p.cv = Cv_; p.np0 = np0_; p.dp = Dp_; p.rhop = Rhop_; p.cvp = Cvp_;
   p.epsilonp = epsilonp_; p.nu = Nu_; p.i01 = I01_; p.i02 = I02_;
+  p.solve_for_scalar = solve_for_scalar_ ? 1 : 0; p.s1_type = s1t_ != 0 ? 1 : 0; p.d_m = D_M_;
   return p;
 }
 

@@ -21,7 +21,8 @@ TPX_OK, TPX_E_ARG, TPX_E_HIP, TPX_E_STATE, TPX_E_UNSUPP, TPX_E_OOM = 0, -1, -2, 
 
 FIELD = {"RU": 0, "RU_int": 1, "RU_new": 2, "RU_np1": 3, "RHS_RU": 4,
          "P": 16, "RHS_Pois": 17, "C": 18, "Rho": 19, "Rho_int": 20, "Rho_new": 21,
-         "Rho_np1": 22, "divergence": 23, "RHS_Part_Temp": 24}
+         "Rho_np1": 22, "divergence": 23, "RHS_Part_Temp": 24,
+         "Passive_Scalar": 25, "Passive_Scalar_int": 26, "Passive_Scalar_new": 27, "Passive_Scalar_np1": 28}
 VECTOR = {"RU", "RU_int", "RU_new", "RU_np1", "RHS_RU"}
 STAGE = {"base": 0, "int": 1, "new": 2, "np1": 3}
 SCALARS = ["P0", "P0_int", "P0_new", "P0_np1", "T_cur", "num_timestep",
@@ -41,7 +42,7 @@ EXPORTS = [
     "tpx_upload_field", "tpx_download_field", "tpx_upload_particles", "tpx_num_particles",
     "tpx_download_particles", "tpx_download_particle_rhs_t", "tpx_set_restart_numbers",
     "tpx_get_scalars", "tpx_initialize", "tpx_params_update", "tpx_update_rho",
-    "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop", "tpx_compute_div_u_new",
+    "tpx_update_p0", "tpx_update_particle", "tpx_update_passive_scalar", "tpx_update_ru_wop", "tpx_compute_div_u_new",
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_update_ru_wp", "tpx_time_advance_rk4",
     "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
     "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_gradient_heat", "tpx_adjoint_set_gas_tape", "tpx_statistics_enable", "tpx_statistics",
@@ -94,7 +95,7 @@ def load(path: str = LIB_PATH):
     for fn in ("tpx_initialize", "tpx_params_update", "tpx_time_advance_rk4", "tpx_time_advance",
                "tpx_deposit_concentration", "tpx_synchronize"):
         getattr(L, fn).argtypes = [P]
-    for fn in ("tpx_update_rho", "tpx_update_p0", "tpx_update_particle", "tpx_update_ru_wop",
+    for fn in ("tpx_update_rho", "tpx_update_p0", "tpx_update_particle", "tpx_update_passive_scalar", "tpx_update_ru_wop",
                "tpx_compute_div_u_new", "tpx_compute_rhs_pois", "tpx_update_ru_wp", "tpx_step"):
         getattr(L, fn).argtypes = [P, I]
     L.tpx_solve_poisson.argtypes = [P, I, ctypes.POINTER(I)]
@@ -237,6 +238,8 @@ class Solver:
     def update_rho(self, rk): self._ok(self.L.tpx_update_rho(self.h, rk), "update_rho")
     def update_p0(self, rk): self._ok(self.L.tpx_update_p0(self.h, rk), "update_p0")
     def update_particle(self, rk): self._ok(self.L.tpx_update_particle(self.h, rk), "update_particle")
+    def update_passive_scalar(self, rk):
+        self._ok(self.L.tpx_update_passive_scalar(self.h, rk), "update_passive_scalar")
     def update_ru_wop(self, rk): self._ok(self.L.tpx_update_ru_wop(self.h, rk), "update_ru_wop")
     def compute_div_u_new(self, rk): self._ok(self.L.tpx_compute_div_u_new(self.h, rk), "compute_div_u_new")
     def compute_rhs_pois(self, rk): self._ok(self.L.tpx_compute_rhs_pois(self.h, rk), "compute_rhs_pois")
@@ -397,10 +400,12 @@ class Solver:
         pp = case.particles
         self.set_particles(pp["x"], pp["y"], pp["z"], pp["u"], pp["v"], pp["w"], pp["T"], pp["flag"],
                            tau=case.tau, eps=case.eps)
+        if getattr(case, "PS", None) is not None:
+            self.set("Passive_Scalar", case.PS)
         self.set_restart_numbers(case.params.P0, 0.0, 0)
         self.initialize()
 
-    SUBSTEP_METHODS = ("update_rho", "update_p0", "update_particle", "update_ru_wop",
+    SUBSTEP_METHODS = ("update_rho", "update_p0", "update_particle", "update_passive_scalar", "update_ru_wop",
                        "compute_div_u_new", "compute_rhs_pois", "solve_poisson",
                        "update_ru_wp", "time_advance_rk4")
 
@@ -410,6 +415,8 @@ class Solver:
         self.params_update()
         for rk in range(4):
             for m in self.SUBSTEP_METHODS:
+                if m == "update_passive_scalar" and not p.solve_for_scalar:
+                    continue  # simulation.cpp:65
                 fn = getattr(self, m)
                 if m == "time_advance_rk4":
                     fn()

@@ -234,10 +234,12 @@ class Case:
 
     def __init__(self, params: Params, RU: np.ndarray, Rho: np.ndarray, P: np.ndarray,
                  particles: Dict[str, np.ndarray], tau: Optional[np.ndarray] = None,
-                 eps: Optional[np.ndarray] = None, info: Optional[Dict] = None):
+                 eps: Optional[np.ndarray] = None, info: Optional[Dict] = None,
+                 PS: Optional[np.ndarray] = None):
         self.params, self.RU, self.Rho, self.P = params, RU, Rho, P
         self.particles, self.tau, self.eps = particles, tau, eps
         self.info = info or {}
+        self.PS = PS  # the passive scalar's initial field (nz, ny, nx), or None
 
     @property
     def shape(self):

@@ -217,6 +217,7 @@ class Params:
             gx2=self.gx2, gy2=self.gy2, gz2=self.gz2,
             cp=self.Cp, cv=self.Cv, np0=self.np0, dp=self.Dp, rhop=self.Rhop,
             cvp=self.Cvp, epsilonp=self.epsilonp, nu=self.Nu, i01=self.I01, i02=self.I02,
+            solve_for_scalar=int(self.solve_for_scalar), s1_type=int(self.s1t != 0), d_m=self.D_M,
         )
 
 
@@ -232,6 +233,7 @@ STRUCT_FIELDS: List[Tuple[str, str]] = [
     ("gx1", "d"), ("gy1", "d"), ("gz1", "d"), ("gx2", "d"), ("gy2", "d"), ("gz2", "d"),
     ("cp", "d"), ("cv", "d"), ("np0", "d"), ("dp", "d"), ("rhop", "d"),
     ("cvp", "d"), ("epsilonp", "d"), ("nu", "d"), ("i01", "d"), ("i02", "d"),
+    ("solve_for_scalar", "i"), ("s1_type", "i"), ("d_m", "d"),
 ]
 
 
