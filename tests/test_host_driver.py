@@ -5,7 +5,8 @@ tpx_sim on top of the C ABI.
 CPU: the C++ input-file parser reads the same values as the Python one
 (params.cpp:6-86), including the derived Tp, mp, P0 and N0.
 GPU: tpx_sim run from restart files (Initial = 1) writes RU/Rho/P and the
-particle files that match the oracle after two steps.
+particle files that match the oracle after two steps, and with the passive
+scalar on (Initial_C = 5) its C_<n>.bin.
 """
 import os
 import subprocess
@@ -95,3 +96,37 @@ def test_driver_from_restart_matches_oracle(gpu_available, sim, tmp_path):
     assert not os.path.exists(tmp_path / "RU_3.bin")
     print({k: f"{v:.2e}" for k, v in errs.items()})
     assert max(errs.values()) <= 1e-10, errs
+
+
+@pytest.mark.gpu
+def test_driver_runs_the_passive_scalar(gpu_available, sim, tmp_path):
+    """solve_for_scalar = 1 with the analytic Initial_C = 5: the driver calls
+    Update_Passive_Scalar after Update_Particle and writes the scalar as
+    C_<n>.bin (grid.cpp:538-544), matching the oracle."""
+    case = small_case(n=16, npart=1000, heat=True, variable_rho=True, eps_range=None, seed=29)
+    p = case.params
+    npart = len(case.particles["x"])
+    p.np0 = (npart + 0.5) / p.Vol
+    p.Initial, p.Initial_C = 1, 5
+    p.solve_for_scalar, p.D_M, p.s1t = 1, 0.05, 0.0
+    p.Data_dir = str(tmp_path) + "/"
+    p.Stat_dir = str(tmp_path) + "/"
+    p.Data_freq_fast = 1
+    p.Data_freq_slow = 2
+    p.T_final = 2.5 * p.dt
+    p.write(str(tmp_path / "input.txt"))
+    ic.write_restart(str(tmp_path), case.RU, case.Rho, case.P, p.P0, 0.0, 0, case.particles)
+    print("tpx_sim 16^3 + 1000 particles with the passive scalar, 3 steps", flush=True)
+    r = subprocess.run([sim, "input.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    n = p.Nx_tot
+    d = p.Lx / n
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    X, Y = d / 2. + i * d, d / 2. + j * d
+    case.PS = np.sin(2 * np.pi / p.Lx * X) * np.sin(2 * np.pi / p.Ly * Y)  # grid.cpp:385-402
+    o = oracle_from_case(case)
+    o.step(2)
+    C = np.fromfile(tmp_path / "C_2.bin").reshape(n, n, n)
+    e = rel_err(C, o.get("Passive_Scalar"))
+    print("passive scalar after 2 steps", e)
+    assert e <= 1e-10
