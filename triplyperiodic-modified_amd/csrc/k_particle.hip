@@ -206,15 +206,35 @@ struct k_copy_i {
 };
 
 // placement: each particle takes the next free slot of its cell's segment
-// (cursor starts as a copy of the CSR offsets)
-// placement: each particle takes the next free slot of its cell's segment
-// (cursor starts as a copy of the CSR offsets)
+// (cursor starts as a copy of the CSR offsets).  The particles arrive nearly
+// in cell order, so a wave's keys come in runs of equal cells on a few cache
+// lines, and returning atomics on one line are served one after another: each
+// run of equal keys in a wave reserves its slots with one atomic (taken by
+// the run's first lane) and its lanes take consecutive slots.  The order of
+// the slots within a cell is undone by k_fixup.
 struct k_place {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const int32_t *__restrict__ key, int32_t *__restrict__ cursor,
                                                   int32_t *__restrict__ perm, long n) {
-  long p = (long)xcd_block(blockIdx.x, gridDim.x) * block_dim_x() + threadIdx.x;
-  if (p < n) perm[atomicAdd(&cursor[key[p]], 1)] = (int32_t)p;
+  const long p = (long)xcd_block(blockIdx.x, gridDim.x) * block_dim_x() + threadIdx.x;
+  const bool ok = p < n;  // the valid lanes of a wave are a prefix of it
+  const int k = ok ? key[p] : -1;
+  const int lane = (int)(threadIdx.x & 63);
+  const int kprev = __shfl_up(k, 1, 64);
+  const bool head = ok && (lane == 0 || kprev != k);
+  const unsigned long long heads = __ballot(head);
+  const int nvalid = __popcll(__ballot(ok));
+  // the head of this lane's run: the highest head at or below it (a valid
+  // lane's run starts at or before it: lane 0 is a head, and a non-head has
+  // the key of the valid lane before it)
+  const unsigned long long below = heads & ((2ull << lane) - 1);
+  const int hl = ok ? 63 - __clzll(below) : lane;
+  const unsigned long long above = heads & ~((2ull << hl) - 1);
+  const int end = min(above ? __ffsll((long long)above) - 1 : 64, nvalid);
+  int base = 0;
+  if (head) base = atomicAdd(&cursor[k], end - hl);
+  base = __shfl(base, hl, 64);
+  if (ok) perm[base + (lane - hl)] = (int32_t)p;
 }
 };
 
