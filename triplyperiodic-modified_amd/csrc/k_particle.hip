@@ -84,6 +84,23 @@ struct AdvArgs {
 
 // P1-P4, P6 shares and P7 wrap for one particle (particle.cpp:314-399,
 // 477-492, 494-520, 58-104)
+// cnt[k] += 1 for every active lane, with one atomic per run of equal keys
+// among consecutive lanes (the particles come nearly in cell order, and
+// atomics on one cache line are served one after another).  The active lanes
+// must be a prefix of the wave.
+__device__ __forceinline__ void count_runs(int32_t *cnt, int k) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int kprev = __shfl_up(k, 1, 64);
+  const bool head = lane == 0 || kprev != k;
+  const unsigned long long heads = __ballot(head);
+  const int nact = __popcll(__ballot(true));
+  if (head) {
+    const unsigned long long above = heads & ~((2ull << lane) - 1);
+    const int end = min(above ? __ffsll((long long)above) - 1 : 64, nact);
+    atomicAdd(&cnt[k], end - lane);
+  }
+}
+
 struct k_advance {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const AdvArgs &a) {
@@ -171,7 +188,7 @@ struct k_advance {
   int ki = (int)(nv[0] * g.idx), kj = (int)(nv[1] * g.idy), kz = (int)(nv[2] * g.idz);
   long nk = ix(ki, kj, kz);
   a.key[p] = (int32_t)nk;
-  if (a.cnt) atomicAdd(&a.cnt[nk], 1);  // no return value: the wave does not wait
+  if (a.cnt) count_runs(a.cnt, (int)nk);  // only lanes past n left early: the active lanes are a prefix
 }
 };
 
@@ -185,7 +202,7 @@ struct k_keys {
   Ix ix = make_ix(g);
   int32_t k = (int32_t)ix((int)(x[p] * g.idx), (int)(y[p] * g.idy), (int)(z[p] * g.idz));
   key[p] = k;
-  atomicAdd(&cnt[k], 1);
+  count_runs(cnt, k);  // only lanes past n left early
 }
 };
 
