@@ -2,7 +2,8 @@
 and one-way coupled: the particle temperature joins the reversed state, the
 exchange's gather of T = P0/(R Rho) and its deposit RHS_Part_Temp are
 reversed, and so are the divergence's deposit term and Update_P0 (the
-thermodynamic pressure P0 evolves: no cooling).  dJ/deps_p (per-particle
+thermodynamic pressure P0 evolves; one case with cooling, where it is held
+and the deposit term is made mean-free).  dJ/deps_p (per-particle
 absorption efficiency), dJ/dT_p(0), dJ/dP0(0), and the gradients the runs
 without heat exchange have (tau, A, RU, Rho, u_p), of J = Var(C), are checked
 against central finite differences of the full solver along seeded
@@ -219,3 +220,27 @@ def test_one_way_coupling_with_heat(gpu_available):
     J, g = _gradients(case, "auto")
     assert np.any(g["eps"] != 0.0)
     assert _particle_fd(case, g, "eps", "eps", 1e-4, 4, 67, "eps (one-way)") <= FD_REL
+
+
+def test_cooling_branch(gpu_available):
+    """cooling = 1: the divergence's deposit term is made mean-free and
+    Update_P0 holds P0 (grid.cpp:816-839), so the reversal takes the mean
+    removal's transpose and no P0 update; dJ/deps, dJ/dP0 and dJ/dRho
+    against central differences."""
+    case, _ = _case()
+    case.params.cooling = 1
+    J, g = _gradients(case, "auto")
+    worst = _particle_fd(case, g, "eps", "eps", 1e-4, 4, 71, "eps (cooling)")
+    P0 = case.params.P0
+    h = 1e-4 * P0
+    worst = max(worst, _worst([g["P0"]], [(_J(case, P0 + h) - _J(case, P0 - h)) / (2 * h)], "P0 (cooling)"))
+    rng = np.random.default_rng(73)
+    dirs = [rng.standard_normal(case.Rho.shape) for _ in range(2)]
+    fds = []
+    for d in dirs:
+        cp, cm = copy.deepcopy(case), copy.deepcopy(case)
+        cp.Rho = case.Rho + 1e-6 * d
+        cm.Rho = case.Rho - 1e-6 * d
+        fds.append((_J(cp) - _J(cm)) / 2e-6)
+    worst = max(worst, _worst([float(np.sum(g["Rho"] * d)) for d in dirs], fds, "Rho (cooling)"))
+    assert worst <= FD_REL
