@@ -97,3 +97,21 @@ def test_batch_refuses_the_passive_scalar(gpu_available):
     boxes = [solver_from_case(_case(0, seed=q)) for q in range(2)]
     with pytest.raises(tpx.TpxError, match="passive scalar"):
         tpx.Batch(boxes)
+
+
+@pytest.mark.parametrize("mode", ["keep", "recompute"])
+def test_adjoint_sweep_leaves_the_scalar_where_the_run_ended(gpu_available, mode):
+    """The recording advances the scalar with the step; the gas sweep (which
+    recomputes the gas only) leaves it alone: one more step equals a plain run
+    of three steps, bit for bit."""
+    case = _case(0)
+    a = solver_from_case(case)
+    a.adjoint_set_gas_tape(mode)
+    a.adjoint_record(2, 1)
+    a.adjoint_gradient_full()
+    a.adjoint_release()
+    a.step(1)
+    b = solver_from_case(case)
+    b.step(3)
+    assert np.array_equal(a.get("Passive_Scalar"), b.get("Passive_Scalar"))
+    assert np.array_equal(a.get("RU"), b.get("RU"))
