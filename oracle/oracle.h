@@ -45,11 +45,24 @@
  *                           a mode carried by a uniform velocity (closed
  *                           forms), exact growth under either source form,
  *                           conservation of the scalar's sum
+ *   or_step (the composition of simulation.cpp:57-93, params::update)
+ *                           whole steps of cases that collapse to linear
+ *                           systems with closed-form RK4 maps
+ *                           (tests/test_oracle_step_composition.py): shear-
+ *                           mode decay with forcing, the particle-temperature
+ *                           / P0 pair with the lagged heat source, the drag
+ *                           pair with make_mean_U0 once per step, the body
+ *                           force on a z-varying density, the threshold
+ *                           switch of A, g, I0 and Iteration, and a
+ *                           divergence-free start staying divergence-free;
+ *                           copies of this file with a planted composition
+ *                           mistake (RU_new from RU_int, Update_P0 after
+ *                           Update_Particle, make_mean_U0 per substep,
+ *                           Rho_face not rebuilt, no threshold switch) each
+ *                           fail at least one of them.
  * Parity unpinned: the order in which or_p_send_recv re-appends wrapped
  * particles (every comparison matches particles by flag, so it cannot change
- * a result), and the composition of the methods in or_step beyond what the
- * tests above fix (which stage value each term reads is restated from the
- * reference's source and is not checked against a run of the reference).
+ * a result).
  */
 #ifndef TPX_ORACLE_H
 #define TPX_ORACLE_H

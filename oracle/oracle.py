@@ -52,15 +52,18 @@ OPS = {"div_f2c": 0, "grad_c2f": 1, "del2": 2, "i_c2f": 3, "i_f2c": 4,
        "ix_c2f": 5, "iy_c2f": 6, "iz_c2f": 7, "ix_f2c": 8, "iy_f2c": 9, "iz_f2c": 10,
        "fold": 11, "ccp_fft": 12}
 
-_lib = None
+_libs: Dict[str, ctypes.CDLL] = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
-        L = ctypes.CDLL(LIB_PATH)
+def lib(path: Optional[str] = None):
+    """The oracle library at `path` (default: the in-tree build), loaded once
+    per path.  Other paths serve the mutation tests, which build copies of the
+    oracle with a planted mistake and check that the closed-form pins fail."""
+    path = os.path.abspath(path or LIB_PATH)
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        L = ctypes.CDLL(path)
         P, D, I = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int
         L.or_create.restype = P
         L.or_create.argtypes = [ctypes.POINTER(OrParams), I]
@@ -96,8 +99,8 @@ def lib():
         L.or_p_gather_temp.argtypes = [P, I]
         L.or_p_deposit_temp.argtypes = [P, I]
         L.or_fft3d.argtypes = [D, I, I, I, I]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _dp(a: Optional[np.ndarray]):
@@ -121,8 +124,8 @@ def fft3d(data: np.ndarray, sign: int) -> np.ndarray:
 class Oracle:
     """One single-rank instance of the reference step, restated on the CPU."""
 
-    def __init__(self, fields: Dict, np_capacity: int = 1):
-        self.L = lib()
+    def __init__(self, fields: Dict, np_capacity: int = 1, lib_path: Optional[str] = None):
+        self.L = lib(lib_path)
         self.par = OrParams(**fields)
         self.shape = (fields["nz"], fields["ny"], fields["nx"])
         self.bs = fields["bs"]
