@@ -89,12 +89,16 @@ def load_traffic(workload):
         return json.load(fh).get("kernels", {})
 
 
-def cpu_baseline(case, threads):
-    """Time the oracle on a bounded sample: the first RK4 substep of the same
-    workload (initialisation excluded), scaled to steps/s."""
+def cpu_baseline(case, threads, full=False):
+    """The oracle (the CPU restatement of the reference step) on this host's
+    cores, on the same workload from the same initial state, initialisation
+    excluded.  All threads (OpenMP over z-slabs, standing in for the
+    reference's MPI ranks): one warm-up step, then one timed step (three with
+    full=True); its Poisson iterations per step are reported beside the
+    device's.  One thread (the reference's per-rank code path): the next
+    substep (a whole step with full=True), timed alone."""
     from oracle.oracle import Oracle, threads as set_threads
-    from tpx import ic  # noqa: F401
-    n_used = set_threads(threads)
+    n_all = set_threads(threads)
     p = case.params
     o = Oracle(p.struct_fields(), np_capacity=len(case.particles["x"]))
     o.set("RU", case.RU)
@@ -105,18 +109,42 @@ def cpu_baseline(case, threads):
                     tau=case.tau, eps=case.eps)
     o.set_restart_numbers(p.P0, 0.0, 0)
     o.initialize()
-    o.params_update()
-    o.set_rk4_count(0)
     t0 = time.perf_counter()
-    for m in o.SUBSTEP_METHODS:
-        getattr(o, m)()
-    dt = time.perf_counter() - t0
-    n_it = int(o.scalars()["num_iteration"])
+    o.step(1)
+    t_warm = time.perf_counter() - t0
+    log(f"[bench] CPU warm-up step {t_warm:.2f} s on {n_all} threads")
+    k = 3 if full else 1
+    it0 = o.scalars()["total_iterations"]
+    t0 = time.perf_counter()
+    o.step(k)
+    t_all = (time.perf_counter() - t0) / k
+    its_all = (o.scalars()["total_iterations"] - it0) / k
+    log(f"[bench] CPU {t_all:.2f} s a step on {n_all} threads ({its_all:g} Poisson iterations a step)")
+    set_threads(1)
+    it0 = o.scalars()["total_iterations"]
+    t0 = time.perf_counter()
+    if full:
+        o.step(1)
+        what, scale = "one whole step", 1.0
+    else:
+        o.params_update()
+        o.set_rk4_count(0)
+        for m in o.SUBSTEP_METHODS:
+            getattr(o, m)()
+        what, scale = "the first substep of the next step, value = 1/(4 x its time)", 4.0
+    t_one = time.perf_counter() - t0
+    its_one = o.scalars()["total_iterations"] - it0
+    log(f"[bench] CPU {t_one:.2f} s for {what.split(',')[0]} on 1 thread ({its_one:g} Poisson iterations)")
+    set_threads(n_all)
     o.close()
-    return {"value": 1.0 / (4.0 * dt), "unit": "steps/s", "cores": n_used, "kind": "port",
-            "sample": f"oracle (C restatement of the reference step, OpenMP over {n_used} host threads) "
-                      f"timed over the first RK4 substep of the same 256^3 + 1.6e7-particle workload "
-                      f"({dt:.2f} s, {n_it} Poisson iterations), value = 1/(4 x substep time)"}
+    return {"value": 1.0 / t_all, "unit": "steps/s", "cores": n_all, "kind": "port",
+            "sample": f"oracle (C restatement of the reference step, OpenMP over {n_all} host threads) on the "
+                      f"same workload from the same initial state: 1 warm-up step ({t_warm:.1f} s), then {k} timed "
+                      f"whole step(s) ({t_all:.2f} s a step, {its_all:g} Poisson iterations a step)",
+            "poisson_iterations_per_step": its_all, "warmup_steps": 1, "timed_steps": k,
+            "single_thread": {"value": 1.0 / (scale * t_one), "unit": "steps/s", "cores": 1,
+                              "sample": f"the same oracle on 1 thread after those steps: {what} "
+                                        f"({t_one:.2f} s, {its_one:g} Poisson iterations)"}}
 
 
 def copy_probe(mib):
@@ -282,6 +310,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="heat256", choices=["heat256", "plain128", "plain32", "ensemble", "adjoint"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline over 3 timed steps on all threads and a whole step on 1 thread")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -402,7 +432,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
-        cpu = cpu_baseline(case, args.cpu_threads)
+        cpu = cpu_baseline(case, args.cpu_threads, full=args.cpu_full)
         log(f"[bench] CPU baseline {cpu['value']:.4f} steps/s on {cpu['cores']} threads")
 
     out = {
@@ -416,7 +446,8 @@ def main():
                    "grid": p.Nx_tot, "particles": npart, "dt": p.dt,
                    "deposit_tiles": {"heat": list(tiles["heat"]), "velocity": list(tiles["velocity"])},
                    "poisson_iterations_last_substep": n_it_last,
-                   "poisson_iterations_mean": round(mean_its, 3)},
+                   "poisson_iterations_mean": round(mean_its, 3),
+                   "poisson_iterations_per_step": round(4 * mean_its, 2)},
         "roofline": roofline,
         "roofline_next": others,
         "cpu_baseline": cpu,

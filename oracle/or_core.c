@@ -30,6 +30,7 @@ struct or_state {
   double *t0[NT0];
   double *cc, *rhs_new, *pd[3], *fft;
   int num_iteration;
+  long total_iterations;  /* Poisson iterations summed over every solve */
   double P0, P0_int, P0_new, P0_np1, dP0_dt, met, T_cur;
   int num_timestep, rk;
   int np, cap, has_tau, has_eps;
@@ -37,6 +38,8 @@ struct or_state {
   double *aux[10];
   int *flag;
   double *scratch[4];
+  int *bk_plane, *bk_idx, *bk_start, *bk_cnt;  /* deposit buckets by z-plane */
+  int bk_cap, bk_threads;
 };
 
 #define G(s, i, j, k) ((long)(k) * (s)->NXNY + (long)(j) * (s)->NX + (long)(i))
@@ -95,6 +98,7 @@ void or_destroy(or_state *s)
   for (int a = 0; a < 7; ++a) for (int b = 0; b < 4; ++b) free(s->q[a][b]);
   for (int a = 0; a < 10; ++a) free(s->aux[a]);
   free(s->flag);
+  free(s->bk_plane); free(s->bk_idx); free(s->bk_start); free(s->bk_cnt);
   free(s);
 }
 
@@ -459,9 +463,12 @@ static void update_position(or_state *s)
   for (int a = QX; a <= QZ; ++a) {
     double *np1 = s->q[a][OR_NP1], *nw = s->q[a][OR_NEW], *b = s->q[a][OR_BASE];
     const double *ui = s->q[a + 3][OR_INT];
+#pragma omp parallel for schedule(static)
     for (int p = 0; p < s->np; ++p) np1[p] += dt * post * ui[p];
-    if (s->rk != 3) for (int p = 0; p < s->np; ++p) nw[p] = b[p] + dt * pre * ui[p];
-    else for (int p = 0; p < s->np; ++p) nw[p] = np1[p];
+    if (s->rk != 3) {
+#pragma omp parallel for schedule(static)
+      for (int p = 0; p < s->np; ++p) nw[p] = b[p] + dt * pre * ui[p];
+    } else memcpy(nw, np1, sizeof(double) * (size_t)s->np);
   }
 }
 
@@ -477,6 +484,7 @@ static void update_velocity(or_state *s)
   double *rhs[3] = {s->aux[OR_AUX_RHS_U], s->aux[OR_AUX_RHS_V], s->aux[OR_AUX_RHS_W]};
   for (int a = 0; a < 3; ++a) {
     const double *ui = s->q[QU + a][OR_INT];
+#pragma omp parallel for schedule(static)
     for (int p = 0; p < s->np; ++p) {
       double itp = s->has_tau ? 1. / s->aux[OR_AUX_TAU][p] : iTp;
       rhs[a][p] = (gas[a][p] - ui[p]) * itp;
@@ -484,9 +492,12 @@ static void update_velocity(or_state *s)
   }
   for (int a = 0; a < 3; ++a) {
     double *np1 = s->q[QU + a][OR_NP1], *nw = s->q[QU + a][OR_NEW], *b = s->q[QU + a][OR_BASE];
+#pragma omp parallel for schedule(static)
     for (int p = 0; p < s->np; ++p) np1[p] += dt * post * (rhs[a][p] + g[a]);
-    if (s->rk != 3) for (int p = 0; p < s->np; ++p) nw[p] = b[p] + dt * pre * (rhs[a][p] + g[a]);
-    else for (int p = 0; p < s->np; ++p) nw[p] = np1[p];
+    if (s->rk != 3) {
+#pragma omp parallel for schedule(static)
+      for (int p = 0; p < s->np; ++p) nw[p] = b[p] + dt * pre * (rhs[a][p] + g[a]);
+    } else memcpy(nw, np1, sizeof(double) * (size_t)s->np);
   }
 }
 
@@ -498,19 +509,20 @@ static void update_temp(or_state *s)
   double iS = 1. / (s->mp * s->p.cvp);
   double *np1 = s->q[QT][OR_NP1], *nw = s->q[QT][OR_NEW], *b = s->q[QT][OR_BASE];
   const double *rt = s->aux[OR_AUX_RHS_T];
+#pragma omp parallel for schedule(static)
   for (int p = 0; p < s->np; ++p) {
     double epsp = s->has_eps ? s->aux[OR_AUX_EPS][p] : s->p.epsilonp;
     double alpha = 0.25 * epsp * OR_PI * s->p.dp * s->p.dp * s->I0;
     np1[p] += dt * post * iS * (rt[p] + alpha);
   }
-  if (s->rk != 3)
+  if (s->rk != 3) {
+#pragma omp parallel for schedule(static)
     for (int p = 0; p < s->np; ++p) {
       double epsp = s->has_eps ? s->aux[OR_AUX_EPS][p] : s->p.epsilonp;
       double alpha = 0.25 * epsp * OR_PI * s->p.dp * s->p.dp * s->I0;
       nw[p] = b[p] + dt * pre * iS * (rt[p] + alpha);
     }
-  else
-    for (int p = 0; p < s->np; ++p) nw[p] = np1[p];
+  } else memcpy(nw, np1, sizeof(double) * (size_t)s->np);
 }
 
 /* CIC index/weight helper (particle.cpp:409-420, 437-448): ii=round(x/dx),
@@ -523,25 +535,170 @@ static void cic(const or_state *s, double x, double y, double z, int *ii, int *j
   *ii = i + s->bs; *jj = j + s->bs; *kk = k + s->bs;
 }
 
+/* Deposits run in parallel without changing the order in which any cell
+ * receives its contributions.  The particles are bucketed by the z-plane of
+ * their stencil's base cell (a stable counting sort: each thread counts and
+ * places one contiguous index range), and each thread then owns whole target
+ * planes: for target plane t it merges, in increasing particle index, the
+ * buckets of the two base planes whose stencils reach t and adds only the
+ * contributions that land in t.  Every cell therefore sums its particles in
+ * the order of the reference's serial loop (particle.cpp:494-654), so the
+ * result is the same bit for bit at any thread count; on P ranks the
+ * reference itself splits the loop the same way, one subdomain per rank. */
+static int nthreads_now(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* plane[p] in [-1, NZ] is stored shifted by one; buckets live in bk_idx
+ * between bk_start[b] and bk_start[b + 1], b = plane + 1 */
+static void bucket_planes(or_state *s)
+{
+  int np = s->np, nb = s->NZ + 2, T = nthreads_now();
+  if (T > np / 4096 + 1) T = np / 4096 + 1;
+  if (T < 1) T = 1;
+  if (s->bk_threads < T) {
+    free(s->bk_cnt);
+    s->bk_cnt = (int *)calloc((size_t)T * (nb + 1), sizeof(int));
+    s->bk_threads = T;
+  }
+  if (!s->bk_start) s->bk_start = (int *)calloc((size_t)nb + 1, sizeof(int));
+  int *cnt = s->bk_cnt, *pl = s->bk_plane, *idx = s->bk_idx, *st = s->bk_start;
+#pragma omp parallel num_threads(T)
+  {
+#ifdef _OPENMP
+    int t = omp_get_thread_num();
+#else
+    int t = 0;
+#endif
+    long lo = (long)np * t / T, hi = (long)np * (t + 1) / T;
+    int *c = cnt + (size_t)t * (nb + 1);
+    for (int b = 0; b <= nb; ++b) c[b] = 0;
+    for (long p = lo; p < hi; ++p) c[pl[p] + 1]++;
+#pragma omp barrier
+#pragma omp single
+    {
+      int run = 0;
+      for (int b = 0; b < nb; ++b) {
+        st[b] = run;
+        for (int u = 0; u < T; ++u) {
+          int k = cnt[(size_t)u * (nb + 1) + b];
+          cnt[(size_t)u * (nb + 1) + b] = run;
+          run += k;
+        }
+      }
+      st[nb] = run;
+    }
+    for (long p = lo; p < hi; ++p) idx[c[pl[p] + 1]++] = (int)p;
+  }
+}
+
+static void bucket_alloc(or_state *s)
+{
+  if (s->bk_cap < s->np) {
+    free(s->bk_plane); free(s->bk_idx);
+    s->bk_cap = s->cap > s->np ? s->cap : s->np;
+    s->bk_plane = (int *)malloc(sizeof(int) * (size_t)s->bk_cap);
+    s->bk_idx = (int *)malloc(sizeof(int) * (size_t)s->bk_cap);
+  }
+}
+
+/* next particle of the merge of buckets b0 and b1 (increasing index), -1 at the end */
+#define MERGE_NEXT(ia, ea, ib, eb, idx) \
+  ((ia) < (ea) && ((ib) >= (eb) || (idx)[ia] < (idx)[ib]) ? (idx)[(ia)++] : ((ib) < (eb) ? (idx)[(ib)++] : -1))
+
+/* the contributions of particle p's cloud-in-cell stencil (base cell
+ * (ii, jj, kk), planes kk and kk - 1) that land in plane t (t < 0: all), in
+ * the reference's expression order (particle.cpp:522-555, 620-654) */
+static inline void cic_one(or_state *s, double *A, const double *x, const double *y, const double *z,
+                           const double *w, int p, int t)
+{
+  int ii, jj, kk; double xl, yl, zl;
+  cic(s, x[p], y[p], z[p], &ii, &jj, &kk, &xl, &yl, &zl);
+  int lo = t < 0 || kk == t, hi = t < 0 || kk - 1 == t;
+  if (w) {
+    double r = w[p];
+    if (lo) {
+      A[G(s, ii, jj, kk)] += xl * yl * zl * r;
+      A[G(s, ii - 1, jj, kk)] += (1 - xl) * yl * zl * r;
+      A[G(s, ii, jj - 1, kk)] += xl * (1 - yl) * zl * r;
+      A[G(s, ii - 1, jj - 1, kk)] += (1 - xl) * (1 - yl) * zl * r;
+    }
+    if (hi) {
+      A[G(s, ii, jj, kk - 1)] += xl * yl * (1 - zl) * r;
+      A[G(s, ii - 1, jj, kk - 1)] += (1 - xl) * yl * (1 - zl) * r;
+      A[G(s, ii, jj - 1, kk - 1)] += xl * (1 - yl) * (1 - zl) * r;
+      A[G(s, ii - 1, jj - 1, kk - 1)] += (1 - xl) * (1 - yl) * (1 - zl) * r;
+    }
+  } else {
+    if (lo) {
+      A[G(s, ii, jj, kk)] += (xl * yl * zl);
+      A[G(s, ii - 1, jj, kk)] += (1 - xl) * yl * zl;
+      A[G(s, ii, jj - 1, kk)] += xl * (1 - yl) * zl;
+      A[G(s, ii - 1, jj - 1, kk)] += (1 - xl) * (1 - yl) * zl;
+    }
+    if (hi) {
+      A[G(s, ii, jj, kk - 1)] += xl * yl * (1 - zl);
+      A[G(s, ii - 1, jj, kk - 1)] += (1 - xl) * yl * (1 - zl);
+      A[G(s, ii, jj - 1, kk - 1)] += xl * (1 - yl) * (1 - zl);
+      A[G(s, ii - 1, jj - 1, kk - 1)] += (1 - xl) * (1 - yl) * (1 - zl);
+    }
+  }
+}
+
+/* cloud-in-cell deposit of w (NULL: weight 1) at positions x,y,z into A */
+static void deposit_cic(or_state *s, double *A, const double *x, const double *y, const double *z,
+                        const double *w)
+{
+  if (nthreads_now() == 1) {
+    for (int p = 0; p < s->np; ++p) cic_one(s, A, x, y, z, w, p, -1);
+    return;
+  }
+  bucket_alloc(s);
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < s->np; ++p) s->bk_plane[p] = (int)(round(z[p] * s->idz)) + s->bs;
+  bucket_planes(s);
+  const int *idx = s->bk_idx, *st = s->bk_start;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int t = 0; t < s->NZ; ++t) {
+    int ia = st[t + 1], ea = st[t + 2], ib = st[t + 2], eb = st[t + 3];  /* base t, base t + 1 */
+    for (int p; (p = MERGE_NEXT(ia, ea, ib, eb, idx)) >= 0;) cic_one(s, A, x, y, z, w, p, t);
+  }
+}
+
 /* P5: particle::part2gas_concentration (particle.cpp:522-555) */
 void or_p_deposit_concentration(or_state *s)
 {
   double *A = T0(s, OR_C);
   ew_set(s, A, 0.);
-  double *xi = s->q[QX][OR_INT], *yi = s->q[QY][OR_INT], *zi = s->q[QZ][OR_INT];
-  for (int p = 0; p < s->np; ++p) {
-    int ii, jj, kk; double xl, yl, zl;
-    cic(s, xi[p], yi[p], zi[p], &ii, &jj, &kk, &xl, &yl, &zl);
-    A[G(s, ii, jj, kk)] += (xl * yl * zl);
-    A[G(s, ii - 1, jj, kk)] += (1 - xl) * yl * zl;
-    A[G(s, ii, jj - 1, kk)] += xl * (1 - yl) * zl;
-    A[G(s, ii, jj, kk - 1)] += xl * yl * (1 - zl);
-    A[G(s, ii - 1, jj - 1, kk)] += (1 - xl) * (1 - yl) * zl;
-    A[G(s, ii - 1, jj, kk - 1)] += (1 - xl) * yl * (1 - zl);
-    A[G(s, ii, jj - 1, kk - 1)] += xl * (1 - yl) * (1 - zl);
-    A[G(s, ii - 1, jj - 1, kk - 1)] += (1 - xl) * (1 - yl) * (1 - zl);
-  }
+  deposit_cic(s, A, s->q[QX][OR_INT], s->q[QY][OR_INT], s->q[QZ][OR_INT], NULL);
   fold(s, A);
+}
+
+/* the contributions of particle p's face deposit (base cell (ii, jj, kk):
+ * plane kk for the x, y and z faces, plane kk + 1 for the z faces) that land
+ * in plane t (t < 0: all), in the order of particle.cpp:494-520 */
+static inline void face_one(or_state *s, double **A, int p, int t)
+{
+  const double *xi = s->q[QX][OR_INT], *yi = s->q[QY][OR_INT], *zi = s->q[QZ][OR_INT];
+  const double *ru = s->aux[OR_AUX_RHS_U], *rv = s->aux[OR_AUX_RHS_V], *rw = s->aux[OR_AUX_RHS_W];
+  int ii = x2i(s, xi[p]), jj = y2j(s, yi[p]), kk = z2k(s, zi[p]);
+  double xl = xi[p] * s->idx - (int)(xi[p] * s->idx);
+  double yl = yi[p] * s->idy - (int)(yi[p] * s->idy);
+  double zl = zi[p] * s->idz - (int)(zi[p] * s->idz);
+  int lo = t < 0 || kk == t, hi = t < 0 || kk + 1 == t;
+  if (lo) {
+    A[0][G(s, ii + 1, jj, kk)] += xl * ru[p];
+    A[0][G(s, ii, jj, kk)] += (1 - xl) * ru[p];
+    A[1][G(s, ii, jj + 1, kk)] += yl * rv[p];
+    A[1][G(s, ii, jj, kk)] += (1 - yl) * rv[p];
+  }
+  if (hi) A[2][G(s, ii, jj, kk + 1)] += zl * rw[p];
+  if (lo) A[2][G(s, ii, jj, kk)] += (1 - zl) * rw[p];
 }
 
 /* P6: particle::part2gas_velocity (particle.cpp:494-520) into RHS_RU */
@@ -549,19 +706,20 @@ void or_p_deposit_velocity(or_state *s)
 {
   double **A = T1(s, OR_RHS_RU);
   for (int c = 0; c < 3; ++c) ew_set(s, A[c], 0.);
-  double *xi = s->q[QX][OR_INT], *yi = s->q[QY][OR_INT], *zi = s->q[QZ][OR_INT];
-  const double *ru = s->aux[OR_AUX_RHS_U], *rv = s->aux[OR_AUX_RHS_V], *rw = s->aux[OR_AUX_RHS_W];
-  for (int p = 0; p < s->np; ++p) {
-    int ii = x2i(s, xi[p]), jj = y2j(s, yi[p]), kk = z2k(s, zi[p]);
-    double xl = xi[p] * s->idx - (int)(xi[p] * s->idx);
-    double yl = yi[p] * s->idy - (int)(yi[p] * s->idy);
-    double zl = zi[p] * s->idz - (int)(zi[p] * s->idz);
-    A[0][G(s, ii + 1, jj, kk)] += xl * ru[p];
-    A[0][G(s, ii, jj, kk)] += (1 - xl) * ru[p];
-    A[1][G(s, ii, jj + 1, kk)] += yl * rv[p];
-    A[1][G(s, ii, jj, kk)] += (1 - yl) * rv[p];
-    A[2][G(s, ii, jj, kk + 1)] += zl * rw[p];
-    A[2][G(s, ii, jj, kk)] += (1 - zl) * rw[p];
+  if (nthreads_now() == 1) {
+    for (int p = 0; p < s->np; ++p) face_one(s, A, p, -1);
+  } else {
+    const double *zi = s->q[QZ][OR_INT];
+    bucket_alloc(s);
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < s->np; ++p) s->bk_plane[p] = z2k(s, zi[p]);
+    bucket_planes(s);
+    const int *idx = s->bk_idx, *st = s->bk_start;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < s->NZ; ++t) {
+      int ia = st[t], ea = st[t + 1], ib = st[t + 1], eb = st[t + 2];  /* base t - 1, base t */
+      for (int p; (p = MERGE_NEXT(ia, ea, ib, eb, idx)) >= 0;) face_one(s, A, p, t);
+    }
   }
   for (int c = 0; c < 3; ++c) { fold(s, A[c]); fill(s, A[c]); }
 }
@@ -591,6 +749,7 @@ static void compute_rhs_temp(or_state *s, int stage)
 {
   double iTth = s->p.nu * OR_PI * s->p.dp * s->p.k;
   const double *Tp = s->q[QT][stage];
+#pragma omp parallel for schedule(static)
   for (int p = 0; p < s->np; ++p) s->aux[OR_AUX_RHS_T][p] = (s->aux[OR_AUX_TG][p] - Tp[p]) * iTth;
 }
 
@@ -599,20 +758,7 @@ void or_p_deposit_temp(or_state *s, int stage)
 {
   double *A = T0(s, OR_RHS_PART_TEMP);
   ew_set(s, A, 0.);
-  double *x = s->q[QX][stage], *y = s->q[QY][stage], *z = s->q[QZ][stage];
-  const double *rt = s->aux[OR_AUX_RHS_T];
-  for (int p = 0; p < s->np; ++p) {
-    int ii, jj, kk; double xl, yl, zl;
-    cic(s, x[p], y[p], z[p], &ii, &jj, &kk, &xl, &yl, &zl);
-    A[G(s, ii, jj, kk)] += xl * yl * zl * rt[p];
-    A[G(s, ii - 1, jj, kk)] += (1 - xl) * yl * zl * rt[p];
-    A[G(s, ii, jj - 1, kk)] += xl * (1 - yl) * zl * rt[p];
-    A[G(s, ii, jj, kk - 1)] += xl * yl * (1 - zl) * rt[p];
-    A[G(s, ii - 1, jj - 1, kk)] += (1 - xl) * (1 - yl) * zl * rt[p];
-    A[G(s, ii - 1, jj, kk - 1)] += (1 - xl) * yl * (1 - zl) * rt[p];
-    A[G(s, ii, jj - 1, kk - 1)] += xl * (1 - yl) * (1 - zl) * rt[p];
-    A[G(s, ii - 1, jj - 1, kk - 1)] += (1 - xl) * (1 - yl) * (1 - zl) * rt[p];
-  }
+  deposit_cic(s, A, s->q[QX][stage], s->q[QY][stage], s->q[QZ][stage], s->aux[OR_AUX_RHS_T]);
   fold(s, A);
   fill(s, A);
 }
@@ -740,6 +886,7 @@ static void poisson_fft(or_state *s, double *const *C, double *P, const double *
     it++;
   }
   s->num_iteration = it;
+  s->total_iterations += it;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1012,7 +1159,7 @@ void or_get_scalars(const or_state *s, double *o)
   o[OR_S_T_CUR] = s->T_cur; o[OR_S_NUM_TIMESTEP] = s->num_timestep;
   o[OR_S_NUM_ITERATION] = s->num_iteration; o[OR_S_MEAN_ENERGY] = s->met; o[OR_S_A] = s->A;
   o[OR_S_I0] = s->I0; o[OR_S_ITERATION] = s->Iteration; o[OR_S_GX] = s->gx; o[OR_S_GY] = s->gy;
-  o[OR_S_GZ] = s->gz; o[OR_S_DP0_DT] = s->dP0_dt;
+  o[OR_S_GZ] = s->gz; o[OR_S_DP0_DT] = s->dP0_dt; o[OR_S_TOTAL_ITERATIONS] = (double)s->total_iterations;
 }
 
 /* ------------------------------------------------------------------ */

@@ -108,7 +108,7 @@ enum { OR_AUX_RHS_U = 0, OR_AUX_RHS_V, OR_AUX_RHS_W, OR_AUX_UG, OR_AUX_VG,
 enum { OR_S_P0 = 0, OR_S_P0_INT, OR_S_P0_NEW, OR_S_P0_NP1, OR_S_T_CUR,
        OR_S_NUM_TIMESTEP, OR_S_NUM_ITERATION, OR_S_MEAN_ENERGY, OR_S_A,
        OR_S_I0, OR_S_ITERATION, OR_S_GX, OR_S_GY, OR_S_GZ, OR_S_DP0_DT,
-       OR_S_COUNT };
+       OR_S_TOTAL_ITERATIONS, OR_S_COUNT };
 
 or_state *or_create(const or_params *p, int np_capacity);
 void or_destroy(or_state *s);

@@ -47,7 +47,7 @@ AUX = {"RHS_u": 0, "RHS_v": 1, "RHS_w": 2, "ug": 3, "vg": 4, "wg": 5, "Tg": 6,
        "RHS_T": 7, "tau": 8, "eps": 9}
 SCALARS = ["P0", "P0_int", "P0_new", "P0_np1", "T_cur", "num_timestep",
            "num_iteration", "mean_energy_transferred", "A", "I0", "Iteration",
-           "gx", "gy", "gz", "dP0_dt"]
+           "gx", "gy", "gz", "dP0_dt", "total_iterations"]
 OPS = {"div_f2c": 0, "grad_c2f": 1, "del2": 2, "i_c2f": 3, "i_f2c": 4,
        "ix_c2f": 5, "iy_c2f": 6, "iz_c2f": 7, "ix_f2c": 8, "iy_f2c": 9, "iz_f2c": 10,
        "fold": 11, "ccp_fft": 12}

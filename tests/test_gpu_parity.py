@@ -171,6 +171,30 @@ def div_f2c(F, d):
     return sum((np.roll(F[c], -1, axis=2 - c) - F[c]) / d[c] for c in range(3))
 
 
+def del2(f, d):
+    """7-point Laplacian (tensor0.cpp:234-247, 258-266), periodic."""
+    return sum((np.roll(f, -1, axis=2 - c) - 2 * f + np.roll(f, 1, axis=2 - c)) / (d[c] * d[c]) for c in range(3))
+
+
+def step_end_constraint_residual(s, p):
+    """max|div(RU_int / I_C2F(Rho_int)) - (div - mean(div))| / max|div| after a
+    whole step.  At the last substep TimeAdvance_RK4 leaves RU_int = RU_new
+    (projected, before make_mean_U0 shifts RU_np1) and Rho_int = Rho_new, and
+    the velocity divergence of Compute_Div_U_new (grid.cpp:832-839) is
+    div = (k/Cp) del2(1/Rho_new) - R/(Cp P0_new V_cell) Q with Q the heat
+    deposit of that substep (kept for the next Update_P0) and P0_new = P0_int;
+    Compute_RHS_Pois removes the mean of its right-hand side (:841-849)."""
+    d = (p.Lx / p.Nx_tot, p.Ly / p.Ny_tot, p.Lz / p.Nz_tot)
+    rho = s.get("Rho_int")
+    Q = s.get("RHS_Part_Temp")
+    if p.cooling:
+        Q = Q - Q.mean()
+    vcell = d[0] * d[1] * d[2]
+    div = (p.k / p.Cp) * del2(1.0 / rho, d) - p.R / (p.Cp * s.scalars()["P0_int"] * vcell) * Q
+    r = div_f2c(s.get("RU_int") / face_average(rho), d) - (div - div.mean())
+    return float(np.max(np.abs(r))) / float(np.max(np.abs(div)))
+
+
 def test_mass_and_divergence_at_scale(gpu_available):
     """Size-independent checks at 128^3 with 2e5 heated, two-way coupled
     particles: the density update conserves mass; the particle ids survive
@@ -193,11 +217,8 @@ def test_mass_and_divergence_at_scale(gpu_available):
     P = case.params
     for q, Lq in (("x", P.Lx), ("y", P.Ly), ("z", P.Lz)):
         assert np.all((p[q] >= 0) & (p[q] <= Lq)), q
-    d = (P.Lx / P.Nx_tot, P.Ly / P.Ny_tot, P.Lz / P.Nz_tot)
-    div = s.get("divergence")
-    resid = div_f2c(s.get("RU_int") / face_average(s.get("Rho_int")), d) - (div - div.mean())
-    rel = float(np.max(np.abs(resid))) / float(np.max(np.abs(div)))
-    print(f"low-Mach constraint residual {rel:.2e} of max|divergence| {np.max(np.abs(div)):.3e}")
+    rel = step_end_constraint_residual(s, P)
+    print(f"low-Mach constraint residual {rel:.2e} of max|divergence|")
     assert rel <= 1e-6, rel
 
 

@@ -567,3 +567,26 @@ def test_diagnostics_spectrum_parseval_and_shells():
     assert np.argmax(e) == 5 and abs(e[5] - 0.25) < 1e-12 and e.sum() - e[5] < 1e-20
     parts = {"u": np.full(10, 1.0), "v": np.full(10, 2.0), "w": np.full(10, 2.0)}
     assert abs(ic.particle_rms(parts) - math.sqrt(3.0)) < 1e-15
+
+
+def test_oracle_is_bitwise_independent_of_its_thread_count():
+    """The oracle's deposits run in parallel over z-planes of the target with
+    each cell's contributions added in particle order, so a whole heated,
+    two-way coupled step gives the same bits on 1 and on several threads."""
+    from helpers import oracle_from_case, small_case
+    from oracle.oracle import threads
+    case = small_case(n=16, npart=6000, heat=True, variable_rho=True, eps_range=(0.5, 1.0), seed=12)
+    out = []
+    n0 = threads(0)
+    try:
+        for nt in (1, 4):
+            threads(nt)
+            o = oracle_from_case(case)
+            o.step(2)
+            out.append({k: o.get(k) for k in ("RU", "Rho", "P", "RHS_Part_Temp", "C", "RHS_RU")})
+            out[-1].update({"p_" + k: v for k, v in o.particles().items()})
+            o.close()
+    finally:
+        threads(n0)
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k

@@ -137,7 +137,7 @@ double *field_buffer(tpx_ctx *c, int field, int comp) {
   case TPX_RHO_INT: return c->rho[c->rho_int];
   case TPX_RHO_NEW: return c->rho_new >= 0 ? c->rho[c->rho_new] : nullptr;
   case TPX_RHO_NP1: return c->rho[rho_np1_read(c)];
-  case TPX_DIVERGENCE: return c->divergence;
+  case TPX_DIVERGENCE: return c->div_valid ? c->divergence : nullptr;
   case TPX_RHS_PART_TEMP: return c->Q;
   case TPX_PASSIVE_SCALAR: return c->ps[c->ps_base];
   case TPX_PASSIVE_SCALAR_INT: return c->ps[c->ps_int];
@@ -389,6 +389,7 @@ void fused_substep_pre(tpx_ctx *c, int rk) {
   ru_ptrs(c, c->ru_new, rn);
   if (c->stats_on && rk == 3) stats_capture_velocity(c, rn, c->rho[c->rho_new]);  // Compute_RHS_Pois's U
   launch_div_rhs(c, k, 0, rn, c->rho[c->rho_new], c->q_zero ? nullptr : c->Q, nullptr, c->rhs);
+  c->div_valid = false;
 }
 
 void fused_substep_post(tpx_ctx *c, int rk) {
@@ -836,7 +837,12 @@ int tpx_update_passive_scalar(tpx_ctx *ctx, int rk) {
 }
 
 int tpx_update_ru_wop(tpx_ctx *ctx, int rk) {
-  return guarded(ctx, [&]() { TPX_RK_CHECK(rk); do_update_ru_wop(ctx, rk, false); return TPX_OK; });
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    do_update_ru_wop(ctx, rk, false);
+    ctx->div_valid = false;  // the reference's Update_RU_WOP overwrites divergence with div(U)
+    return TPX_OK;
+  });
 }
 
 int tpx_compute_div_u_new(tpx_ctx *ctx, int rk) {
@@ -846,6 +852,7 @@ int tpx_compute_div_u_new(tpx_ctx *ctx, int rk) {
     const double *rn[3] = {nullptr, nullptr, nullptr};
     launch_div_rhs(ctx, coef(ctx, rk), 1, rn, ctx->rho[ctx->rho_new], ctx->q_zero ? nullptr : ctx->Q,
                    ctx->divergence, nullptr);
+    ctx->div_valid = true;
     return TPX_OK;
   });
 }

@@ -238,6 +238,8 @@ struct tpx_ctx {
   int ps_base = 0, ps_int = 1, ps_new = 2, ps_np1 = 3;
   bool rhs_has_mean = false;      // rhs holds the raw RHS, SC_SUM_RHS its sum
   bool q_zero = false;            // Q is known to be identically zero
+  bool div_valid = false;         // divergence holds Compute_Div_U_new's field (the fused
+                                  // step forms it inside k_div_rhs and never stores it)
   bool binned_this_step = false;  // particles were re-binned during the current step
   double2 *spec[2] = {};          // half-spectrum ping-pong (nx/2+1)*ny*nz
   double *lam[3] = {};            // modified-wavenumber tables per axis
