@@ -130,6 +130,19 @@ int tpx_compute_rhs_pois(tpx_ctx *ctx, int rk);      /* grid.cpp:841-849 */
  * *iterations receives poisson::num_iteration() (may be NULL) */
 int tpx_solve_poisson(tpx_ctx *ctx, int rk, int *iterations);
 int tpx_update_ru_wp(tpx_ctx *ctx, int rk);          /* grid.cpp:857-863 */
+/* poisson::Solve(tensor1& C, tensor0& P, tensor0& RHS) with the caller's face
+ * coefficient C (poisson.h:79, poisson.cpp:115-117 -> FFT :391-462): iterates
+ *   r = (div_F2C(C grad_C2F P) - RHS) cc,  cc = -1 / I_F2C(C),
+ *   P += -Re IFFT(FFT(r) / lambda) / N^3 (zero mode passed through)
+ * until Iteration iterations have run or max|dp| <= epsilon (the context's
+ * params and params::update state).  cx, cy, cz, P and rhs are host arrays in
+ * the ghosted layout with border bs (ghosts ignored on input, filled
+ * periodically in P on output); P is the initial guess and receives the
+ * result; RHS is used as given (the reference's callers remove its mean).
+ * *iterations receives poisson::num_iteration() (may be NULL).  The context's
+ * grid state is not touched. */
+int tpx_poisson_solve(tpx_ctx *ctx, const double *cx, const double *cy, const double *cz, double *P,
+                      const double *rhs, int bs, int *iterations);
 int tpx_time_advance_rk4(tpx_ctx *ctx);              /* grid.cpp:890-900 */
 /* tensor1::make_mean_U0 on RU_np1 (tensor1.cpp:406-411), a = rho0*(U0,V0,W0) */
 int tpx_make_mean_u0(tpx_ctx *ctx, const double a[3]);

@@ -44,6 +44,9 @@ struct or_state {
 
 #define G(s, i, j, k) ((long)(k) * (s)->NXNY + (long)(j) * (s)->NX + (long)(i))
 
+static void load_int(or_state *s, double *f, const double *in);
+static void store_int(const or_state *s, const double *f, double *out);
+
 /* ------------------------------------------------------------------ */
 /* allocation                                                          */
 /* ------------------------------------------------------------------ */
@@ -1033,6 +1036,24 @@ void or_solve_poisson(or_state *s)
   double **d = T1(s, OR_DUMMY);
   for (int c = 0; c < 3; ++c) ew_sdiv(s, d[c], 1., T1(s, OR_RHO_FACE)[c]);
   poisson_fft(s, d, T0(s, OR_P), T0(s, OR_RHS_POIS));
+}
+
+/* poisson::Solve(tensor1& C, tensor0& P, tensor0& RHS) (poisson.cpp:115-117)
+ * with the caller's coefficient: C, P and RHS in the interior layout, ghosts
+ * filled here; P is the initial guess and receives the result.  The
+ * coefficient is held in the grid's dummy2 tensor, as grid::Solve_Poisson_Q
+ * holds its coefficient in dummy (grid.cpp:666-670). */
+void or_poisson_solve(or_state *s, const double *cx, const double *cy, const double *cz, double *P,
+                      const double *rhs, int *iters)
+{
+  double **C = T1(s, OR_DUMMY2);
+  const double *cin[3] = {cx, cy, cz};
+  for (int c = 0; c < 3; ++c) load_int(s, C[c], cin[c]);
+  load_int(s, s->scratch[0], P);
+  load_int(s, s->scratch[1], rhs);
+  poisson_fft(s, C, s->scratch[0], s->scratch[1]);
+  store_int(s, s->scratch[0], P);
+  if (iters) *iters = s->num_iteration;
 }
 
 /* G10: grid::Update_RU_WP (grid.cpp:857-863) */

@@ -146,6 +146,10 @@ void or_compute_div_u_new(or_state *s);       /* grid.cpp:832-839 */
 void or_compute_rhs_pois(or_state *s);        /* grid.cpp:841-849 */
 void or_solve_poisson(or_state *s);           /* grid.cpp:851-855 */
 void or_update_ru_wp(or_state *s);            /* grid.cpp:857-863 */
+/* poisson::Solve with the caller's face coefficient (poisson.cpp:115-117,
+ * 391-462); interior layout, P in/out */
+void or_poisson_solve(or_state *s, const double *cx, const double *cy, const double *cz, double *P,
+                      const double *rhs, int *iters);
 void or_time_advance_rk4(or_state *s);        /* grid.cpp:890-900 */
 void or_make_mean_u0(or_state *s, const double a[3]); /* tensor1.cpp:406-411 on RU_np1 */
 void or_time_advance(or_state *s);            /* grid.cpp:595-606 */

@@ -99,6 +99,7 @@ def lib(path: Optional[str] = None):
         L.or_p_gather_temp.argtypes = [P, I]
         L.or_p_deposit_temp.argtypes = [P, I]
         L.or_fft3d.argtypes = [D, I, I, I, I]
+        L.or_poisson_solve.argtypes = [P, D, D, D, D, D, ctypes.POINTER(I)]
         _libs[path] = L
     return _libs[path]
 
@@ -227,6 +228,17 @@ class Oracle:
     def update_ru_wp(self): self.L.or_update_ru_wp(self.h)
     def time_advance_rk4(self): self.L.or_time_advance_rk4(self.h)
     def time_advance(self): self.L.or_time_advance(self.h)
+
+    def poisson_solve(self, C, P, RHS):
+        """poisson::Solve(C, P, RHS) with the caller's face coefficient C
+        (3, nz, ny, nx); returns the new P and the iteration count."""
+        C = _c(C).reshape((3,) + self.shape)
+        cs = [np.ascontiguousarray(C[c]) for c in range(3)]
+        Pn = _c(P).reshape(self.shape).copy()
+        R = _c(RHS).reshape(self.shape)
+        it = ctypes.c_int(0)
+        self.L.or_poisson_solve(self.h, *[_dp(a) for a in cs], _dp(Pn), _dp(R), ctypes.byref(it))
+        return Pn, it.value
 
     def make_mean_u0(self, a: Sequence[float]):
         arr = _c(a)

@@ -150,6 +150,8 @@ struct XArgs {
   const double2 *tw;     // x twiddles
   double ncell;
   const double *done;    // nonzero: the solve has converged, the iteration is skipped
+  const double *res;     // the residual formed by k_residual_coef (a solve with the
+                         // caller's face coefficient), or null: formed here from rho
 };
 
 // 1/x from the hardware reciprocal estimate and two Newton steps,
@@ -190,6 +192,7 @@ __device__ __forceinline__ double residual_of(const Geo &g, double pc, double px
 
 // the residual at cell (i,j,k)
 __device__ __forceinline__ double residual(const XArgs &a, const Ix &ix, int i, int j, int k, double mean) {
+  if (a.res) return a.res[ix(i, j, k)];
   const double *__restrict__ rho = a.rho;
   const double *__restrict__ P = a.P;
   return residual_of(a.g, P[ix(i, j, k)], P[ix(i - 1, j, k)], P[ix(i + 1, j, k)], P[ix(i, j - 1, k)],
@@ -1038,6 +1041,27 @@ struct k_xy_inv {
 // iteration count, and the loop test of poisson.cpp:391-462 (continue while
 // count < Iteration and max |dp| > epsilon) into SC_DONE.  A skipped
 // iteration leaves every slot alone.
+// The residual of poisson::FFT with the caller's face coefficient C
+// (poisson.cpp:391-411): r = (div_F2C(C grad_C2F P) - RHS) cc with
+// cc = -1/I_F2C(C), the mean of the cell's six face values; RHS is used as
+// given.  One thread per cell; C.x at the low x face of its cell.
+struct k_residual_coef {
+  static constexpr int kBounds = kThreads;
+  static __device__ __forceinline__ void run(Geo g, const double *__restrict__ cx, const double *__restrict__ cy,
+                                             const double *__restrict__ cz, const double *__restrict__ P,
+                                             const double *__restrict__ rhs, double *__restrict__ res,
+                                             const double *done) {
+    if (*done != 0.0) return;
+    const long e = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (e >= g.n) return;
+    const int i = (int)(e % g.nx), j = (int)((e / g.nx) % g.ny), k = (int)(e / g.nxy);
+    Ix ix{g.nx - 1, g.ny - 1, g.nz - 1, g.nx, g.nxy};
+    res[e] = residual_of(g, P[e], P[ix(i - 1, j, k)], P[ix(i + 1, j, k)], P[ix(i, j - 1, k)], P[ix(i, j + 1, k)],
+                         P[ix(i, j, k - 1)], P[ix(i, j, k + 1)], cx[e], cx[ix(i + 1, j, k)], cy[e],
+                         cy[ix(i, j + 1, k)], cz[e], cz[ix(i, j, k + 1)], rhs[e], 0.0);
+  }
+};
+
 struct k_finish_iter {
   static constexpr int kBounds = 1024;
   static __device__ __forceinline__ void run(const double *__restrict__ partials, long nparts,
@@ -1187,18 +1211,28 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new) {
   launch_poisson_iteration_on(c, rho_new, c->P, c->rhs, c->scal);
 }
 
-void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal) {
+void launch_residual_coef(tpx_ctx *c, const double *const cf[3], const double *P, const double *rhs, double *res) {
+  const Geo &g = c->g;
+  tpx_launch<k_residual_coef>((unsigned)((g.n + kThreads - 1) / kThreads), kThreads, 0, c->stream, g, cf[0], cf[1],
+                              cf[2], P, rhs, res, (const double *)(c->scal + SC_DONE));
+  TPX_LAUNCH_CHECK();
+}
+
+void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal,
+                                 const double *res) {
   const Geo &g = c->g;
   int mxh = g.nx / 2 + 1;
   long nlines = (long)g.ny * g.nz;
   int r1, r2;
   timing_begin(c, TPX_K_FFT);
   const double *done = scal + SC_DONE;
-  XArgs xa{g, P, rho_new, rhs, scal, c->spec[0], c->tw[0], (double)g.n, done};
+  XArgs xa{g, P, rho_new, rhs, scal, c->spec[0], c->tw[0], (double)g.n, done, res};
   XInvArgs xi{g, P, c->spec[0], c->tw[0], c->partials, (double)g.n, done};
   const bool x4 = four_step(g.nx, r1, r2);
   int y1, y2, z1, z2;
-  if (c->fft_passes == 3 && x4 && four_step(g.ny, y1, y2) && four_step(g.nz, z1, z2)) {
+  // the three-pass form forms the residual itself from rho; a precomputed
+  // residual goes through the five-pass kernels
+  if (!res && c->fft_passes == 3 && x4 && four_step(g.ny, y1, y2) && four_step(g.nz, z1, z2)) {
     const unsigned nb13 = (unsigned)(g.nz * y1);
     const size_t sh13 = std::max(sizeof(double) * (size_t)y2 * r2 * (r1 + 1), sizeof(double2) * (size_t)mxh * (y2 + 1));
 #define L_XYF(A, B) xy_fwd_launch<A, B>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1])

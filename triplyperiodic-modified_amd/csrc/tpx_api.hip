@@ -255,16 +255,17 @@ void do_update_ru_wop(tpx_ctx *c, int rk, bool with_rho) {
 // themselves, so the host queues iterations in batches and waits once per
 // batch instead of once per iteration: first as many as the last solve of
 // this substep took, then two at a time.
-void do_solve_poisson(tpx_ctx *c, int rk, int *iters, const std::function<void(int)> &before_iteration = {}) {
+// The host side of every Poisson solve: the device counts the iterations and
+// evaluates the stopping rule (poisson.cpp:397) itself, so iterations are
+// queued in batches (the first as long as `guess`, then two at a time) and
+// the host waits once per batch.  Returns the iterations that ran.
+int solve_loop(tpx_ctx *c, int guess, const std::function<void(int)> &iterate) {
   launch_solve_begin(c);
-  int queued = 0, batch = std::max(1, c->iter_guess[rk & 3]);
+  int queued = 0, batch = std::max(1, guess);
   double st[2] = {0.0, 1.0};  // SC_ITERS, SC_DONE
   while (queued < c->Iteration) {
     const int nb = std::min(batch, c->Iteration - queued);
-    for (int b = 0; b < nb; ++b) {
-      if (before_iteration) before_iteration(queued + b);
-      launch_poisson_iteration(c, c->rho[c->rho_new]);
-    }
+    for (int b = 0; b < nb; ++b) iterate(queued + b);
     queued += nb;
     hip_check(hipMemcpyAsync(c->h_scal + SC_ITERS, c->scal + SC_ITERS, 2 * sizeof(double), hipMemcpyDeviceToHost,
                              c->stream),
@@ -275,10 +276,29 @@ void do_solve_poisson(tpx_ctx *c, int rk, int *iters, const std::function<void(i
     if (st[1] != 0.0) break;
     batch = 2;
   }
-  const int i = (int)st[0];
+  return (int)st[0];
+}
+
+void do_solve_poisson(tpx_ctx *c, int rk, int *iters, const std::function<void(int)> &before_iteration = {}) {
+  const int i = solve_loop(c, c->iter_guess[rk & 3], [&](int it) {
+    if (before_iteration) before_iteration(it);
+    launch_poisson_iteration(c, c->rho[c->rho_new]);
+  });
   c->iter_guess[rk & 3] = i;
   c->num_iteration = i;
   c->total_iterations += i;
+  if (iters) *iters = i;
+}
+
+// poisson::FFT with the caller's face coefficient (poisson.cpp:391-411): each
+// iteration forms the residual in its own kernel into `res` and runs the
+// five-pass transform on it.  P, RHS, C and res are dense device arrays.
+void do_solve_coef(tpx_ctx *c, const double *const cf[3], double *P, const double *rhs, double *res, int *iters) {
+  const int i = solve_loop(c, 2, [&](int) {
+    launch_residual_coef(c, cf, P, rhs, res);
+    launch_poisson_iteration_on(c, nullptr, P, rhs, c->scal, res);
+  });
+  c->num_iteration = i;  // the reference's poisson object keeps the last solve's count
   if (iters) *iters = i;
 }
 
@@ -433,7 +453,7 @@ void release(tpx_ctx *c) {
   auto f = [](void *p) { if (p) (void)hipFree(p); };
   for (auto &b : c->ru) for (auto &p : b) f(p);
   for (auto &p : c->rho) f(p);
-  f(c->P); f(c->rhs); f(c->Q); f(c->C); f(c->divergence); f(c->dep_tmp);
+  f(c->P); f(c->rhs); f(c->Q); f(c->C); f(c->divergence); f(c->dep_tmp); f(c->coef_stage);
   for (auto &p : c->S) f(p);
   for (auto &p : c->spec) f(p);
   for (auto &p : c->lam) f(p);
@@ -874,6 +894,41 @@ int tpx_solve_poisson(tpx_ctx *ctx, int rk, int *iterations) {
     TPX_RK_CHECK(rk);
     if (ctx->rho_new < 0) throw std::logic_error("Solve_Poisson needs Rho_new");
     do_solve_poisson(ctx, rk, iterations);
+    return TPX_OK;
+  });
+}
+
+int tpx_poisson_solve(tpx_ctx *ctx, const double *cx, const double *cy, const double *cz, double *P,
+                      const double *rhs, int bs, int *iterations) {
+  return guarded(ctx, [&]() {
+    if (!cx || !cy || !cz || !P || !rhs || bs < 0) throw std::invalid_argument("bad host array or bs");
+    const tpx::Geo &g = ctx->g;
+    const size_t n = (size_t)g.n;
+    if (!ctx->coef_stage && hipMalloc((void **)&ctx->coef_stage, sizeof(double) * 6 * n) != hipSuccess)
+      throw std::bad_alloc();
+    double *d = ctx->coef_stage;
+    const int GX = g.nx + 2 * bs, GY = g.ny + 2 * bs;
+    std::vector<double> tmp(5 * n);
+    const double *src[5] = {cx, cy, cz, P, rhs};
+    for (int f = 0; f < 5; ++f)
+      for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j)
+          for (int i = 0; i < g.nx; ++i)
+            tmp[f * n + ((size_t)k * g.ny + j) * g.nx + i] = src[f][((size_t)(k + bs) * GY + (j + bs)) * GX + (i + bs)];
+    hip_check(hipMemcpyAsync(d, tmp.data(), sizeof(double) * 5 * n, hipMemcpyHostToDevice, ctx->stream), "upload");
+    const double *cf[3] = {d, d + n, d + 2 * n};
+    do_solve_coef(ctx, cf, d + 3 * n, d + 4 * n, d + 5 * n, iterations);
+    hip_check(hipMemcpyAsync(tmp.data(), d + 3 * n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream),
+              "download");
+    sync(ctx);
+    const int GZ = g.nz + 2 * bs;
+    for (int k = 0; k < GZ; ++k)
+      for (int j = 0; j < GY; ++j)
+        for (int i = 0; i < GX; ++i) {
+          const int si = ((i - bs) % g.nx + g.nx) % g.nx, sj = ((j - bs) % g.ny + g.ny) % g.ny,
+                    sk = ((k - bs) % g.nz + g.nz) % g.nz;
+          P[((size_t)k * GY + j) * GX + i] = tmp[((size_t)sk * g.ny + sj) * g.nx + si];
+        }
     return TPX_OK;
   });
 }

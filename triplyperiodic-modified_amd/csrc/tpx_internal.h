@@ -231,6 +231,8 @@ struct tpx_ctx {
   double *P = nullptr, *rhs = nullptr, *S[3] = {}, *Q = nullptr, *C = nullptr,
          *divergence = nullptr;
   double *dep_tmp = nullptr;      // 3 x n: row partials of the heat deposit
+  double *coef_stage = nullptr;   // 6 x n: a coefficient solve's C (3 faces), P, RHS and
+                                  // residual, allocated on its first call
   // passive scalar (grid.h:79-83): base, int, new and np1 buffers, allocated
   // when it is solved or first uploaded; TimeAdvance_RK4 swaps int and new,
   // TimeAdvance copies np1 into base
@@ -291,7 +293,10 @@ int fft_setup(tpx_ctx *c);
 // one Poisson iteration; skipped on the device once the solve has stopped
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
 // the same iteration on caller-given P, raw right-hand side and scalar slots
-void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal);
+void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal,
+                                 const double *res = nullptr);
+// r = (div_F2C(C grad_C2F P) - RHS) cc for a caller's face coefficient C (cf[0..2])
+void launch_residual_coef(tpx_ctx *c, const double *const cf[3], const double *P, const double *rhs, double *res);
 // grid::Update_Passive_Scalar (k_grid.hip) and the sum of its source
 void launch_passive_scalar(tpx_ctx *c, const Coef &k, const double *const ru_int[3], const double *rho_int);
 void launch_passive_scalar_source_sum(tpx_ctx *c);

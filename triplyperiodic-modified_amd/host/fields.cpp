@@ -210,8 +210,19 @@ void particle::Store_All(int num_ts) {
 }
 
 // ---------------------------------------------------------------- poisson
-void poisson::Solve(tensor1 &, tensor0 &P, tensor0 &RHS) {
-  if (P.field() != TPX_P || RHS.field() != TPX_RHS_POIS)
-    throw std::runtime_error("poisson::Solve runs on the grid's own P and RHS_Pois");
-  tpx_check(P.ctx(), tpx_solve_poisson(P.ctx(), 0, &num_iteration_), "poisson::Solve");
+void poisson::Solve(tensor1 &C, tensor0 &P, tensor0 &RHS) {
+  // the grid's own solve (grid.cpp:851-855): C = 1/Rho_face is formed on the
+  // device from Rho_new, so grid::Solve_Poisson passes an unbound C
+  if (!C.x.ctx()) {
+    if (P.field() != TPX_P || RHS.field() != TPX_RHS_POIS)
+      throw std::runtime_error("poisson::Solve without a coefficient runs on the grid's own P and RHS_Pois");
+    tpx_check(P.ctx(), tpx_solve_poisson(P.ctx(), 0, &num_iteration_), "poisson::Solve");
+    return;
+  }
+  // any other coefficient, pressure and right-hand side (poisson.cpp:115-117)
+  std::vector<double> cx = C.x.host(), cy = C.y.host(), cz = C.z.host(), p = P.host(), r = RHS.host();
+  tpx_check(P.ctx(), tpx_poisson_solve(P.ctx(), cx.data(), cy.data(), cz.data(), p.data(), r.data(),
+                                       P.size()->bs(), &num_iteration_),
+            "poisson::Solve");
+  P.upload(p.data());
 }

@@ -97,9 +97,9 @@ class poisson {
   int num_iteration_ = 0;
 
  public:
-  // Solves div(C grad P) = RHS for the grid's own P and RHS_Pois, with C the
-  // inverse face density the device derives from Rho_new
-  // (poisson.cpp:115-117, 391-462).
+  // Solves div(C grad P) = RHS (poisson.cpp:115-117, 391-462).  With an
+  // unbound C it is the grid's own solve (P and RHS_Pois, C = 1/Rho_face from
+  // Rho_new on the device); otherwise C, P and RHS are taken as they are.
   void Solve(tensor1 &C, tensor0 &P, tensor0 &RHS);
   int num_iteration() const { return num_iteration_; }
 };

@@ -99,6 +99,7 @@ def load(path: str = LIB_PATH):
                "tpx_compute_div_u_new", "tpx_compute_rhs_pois", "tpx_update_ru_wp", "tpx_step"):
         getattr(L, fn).argtypes = [P, I]
     L.tpx_solve_poisson.argtypes = [P, I, ctypes.POINTER(I)]
+    L.tpx_poisson_solve.argtypes = [P, DP, DP, DP, DP, DP, I, ctypes.POINTER(I)]
     L.tpx_ensemble_step.argtypes = [ctypes.POINTER(P), I, I, I]
     L.tpx_adjoint_record.argtypes = [P, I, I, DP]
     L.tpx_adjoint_gradient.argtypes = [P, DP, DP]
@@ -248,6 +249,18 @@ class Solver:
         it = ctypes.c_int(0)
         self._ok(self.L.tpx_solve_poisson(self.h, rk, ctypes.byref(it)), "solve_poisson")
         return it.value
+
+    def poisson_solve(self, C, P, RHS):
+        """poisson::Solve(C, P, RHS) with the caller's face coefficient C
+        (3, nz, ny, nx): returns the new P and the iteration count."""
+        C = _c(C).reshape((3,) + self.shape)
+        cs = [np.ascontiguousarray(C[c]) for c in range(3)]
+        Pn = _c(P).reshape(self.shape).copy()
+        R = _c(RHS).reshape(self.shape)
+        it = ctypes.c_int(0)
+        self._ok(self.L.tpx_poisson_solve(self.h, *[_dp(a) for a in cs], _dp(Pn), _dp(R), 0, ctypes.byref(it)),
+                 "poisson_solve")
+        return Pn, it.value
 
     def update_ru_wp(self, rk): self._ok(self.L.tpx_update_ru_wp(self.h, rk), "update_ru_wp")
     def time_advance_rk4(self): self._ok(self.L.tpx_time_advance_rk4(self.h), "time_advance_rk4")
