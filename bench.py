@@ -148,18 +148,22 @@ def cpu_baseline(case, threads, full=False):
 
 
 def copy_probe(mib):
-    """A plain device-to-device copy of known size (read `mib` MiB, write
-    `mib` MiB), five times: the calibration kernel for the FETCH_SIZE and
-    WRITE_SIZE counters that the traffic figures are scaled by."""
-    import torch
-    n = mib * (1 << 20) // 8
-    a = torch.rand(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
+    """Streaming-bandwidth probe (libtpx_probe.so, a hand-written HIP kernel):
+    the best of five copies, reads and writes of `mib` MiB with 16-byte
+    accesses.  The copy is the practical ceiling of a streaming kernel on this
+    device and, its bytes being known, the calibration kernel of the
+    FETCH_SIZE / WRITE_SIZE counters the traffic figures are scaled by."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "triplyperiodic-modified_amd", "libtpx_probe.so"))
+    lib.tpx_probe_bandwidth.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    out = (ctypes.c_double * 3)()
+    nbytes = mib * (1 << 20)
+    rc = lib.tpx_probe_bandwidth(nbytes, 5, 0, out)
+    if rc != 0:
+        raise RuntimeError(f"bandwidth probe failed ({rc})")
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": 5,
+                      "copy_GBps": round(out[0], 1), "read_GBps": round(out[1], 1), "write_GBps": round(out[2], 1),
+                      "copy_frac_of_8TBps": round(out[0] / HBM_PEAK_GBS, 4)}))
 
 
 def ensemble_bench(args):
