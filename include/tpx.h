@@ -143,6 +143,18 @@ int tpx_update_ru_wp(tpx_ctx *ctx, int rk);          /* grid.cpp:857-863 */
  * grid state is not touched. */
 int tpx_poisson_solve(tpx_ctx *ctx, const double *cx, const double *cy, const double *cz, double *P,
                       const double *rhs, int bs, int *iterations);
+/* The same solve on dense device arrays (nz*ny*nx doubles each, x fastest, no
+ * ghosts: the caller's inputs cx, cy, cz, rhs and its P, read as the initial
+ * guess and overwritten), allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The residual it iterates on lives in a file-scope static device array of
+ * 2^21 doubles (16 MiB) per GPU, held until the process ends, so grids of
+ * more than 2^21 cells (128^3) are rejected with TPX_E_ARG.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (two contexts on
+ * two host threads) race.  The call returns when the solve has ended. */
+int tpx_poisson_solve_device(tpx_ctx *ctx, const double *cx, const double *cy, const double *cz, double *P,
+                             const double *rhs, int *iterations);
 int tpx_time_advance_rk4(tpx_ctx *ctx);              /* grid.cpp:890-900 */
 /* tensor1::make_mean_U0 on RU_np1 (tensor1.cpp:406-411), a = rho0*(U0,V0,W0) */
 int tpx_make_mean_u0(tpx_ctx *ctx, const double a[3]);

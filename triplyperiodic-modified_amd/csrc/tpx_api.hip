@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <atomic>
 #include <initializer_list>
+#include <mutex>
 #include <stdexcept>
 #include <thread>
 #include <string>
@@ -21,9 +22,39 @@ namespace tpx {
 thread_local Recorder *g_rec = nullptr;
 }
 
+// the static scratch of tpx_poisson_solve_device (see static_scratch below)
+constexpr long kStaticScratchDoubles = 1L << 21;
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+
 namespace {
 
 thread_local std::string g_create_error;
+
+// The scratch of tpx_poisson_solve_device: one file-scope device array per
+// GPU, holding the residual of one coefficient solve (2^21 doubles, 16 MiB:
+// grids of up to 128^3 cells).  Host code never names the array itself (its
+// name is a host-side stand-in, not a device address): tpx_create looks up
+// its address on the current device once with hipGetSymbolAddress and every
+// use goes through that pointer.  The array lives until the process ends.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+void hip_check_symbol(hipError_t e) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+}
+
+// the array's address on device `dev` (current), looked up on first use
+double *static_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index beyond the static scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    hip_check_symbol(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
 
 void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -542,6 +573,7 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
   try {
     c->p = *p;
     hip_check(hipGetDevice(&c->device), "hipGetDevice");
+    c->static_scratch = static_scratch(c->device);
     int lds = 0;
     hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device),
               "hipDeviceGetAttribute");
@@ -929,6 +961,21 @@ int tpx_poisson_solve(tpx_ctx *ctx, const double *cx, const double *cy, const do
                     sk = ((k - bs) % g.nz + g.nz) % g.nz;
           P[((size_t)k * GY + j) * GX + i] = tmp[((size_t)sk * g.ny + sj) * g.nx + si];
         }
+    return TPX_OK;
+  });
+}
+
+int tpx_poisson_solve_device(tpx_ctx *ctx, const double *cx, const double *cy, const double *cz, double *P,
+                             const double *rhs, int *iterations) {
+  return guarded(ctx, [&]() {
+    if (!cx || !cy || !cz || !P || !rhs) throw std::invalid_argument("null device array");
+    if (ctx->g.n > kStaticScratchDoubles)
+      throw std::invalid_argument("tpx_poisson_solve_device: the grid has " + std::to_string(ctx->g.n) +
+                                  " cells, more than its static scratch holds (" +
+                                  std::to_string(kStaticScratchDoubles) + " cells, 128^3)");
+    if (!ctx->static_scratch) throw std::logic_error("static scratch not set up");
+    const double *cf[3] = {cx, cy, cz};
+    do_solve_coef(ctx, cf, P, rhs, ctx->static_scratch, iterations);
     return TPX_OK;
   });
 }

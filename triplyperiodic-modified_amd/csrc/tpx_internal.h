@@ -231,6 +231,7 @@ struct tpx_ctx {
   double *P = nullptr, *rhs = nullptr, *S[3] = {}, *Q = nullptr, *C = nullptr,
          *divergence = nullptr;
   double *dep_tmp = nullptr;      // 3 x n: row partials of the heat deposit
+  double *static_scratch = nullptr;  // this device's address of tpx_api.hip's static scratch
   double *coef_stage = nullptr;   // 6 x n: a coefficient solve's C (3 faces), P, RHS and
                                   // residual, allocated on its first call
   // passive scalar (grid.h:79-83): base, int, new and np1 buffers, allocated

@@ -43,7 +43,7 @@ EXPORTS = [
     "tpx_download_particles", "tpx_download_particle_rhs_t", "tpx_set_restart_numbers",
     "tpx_get_scalars", "tpx_initialize", "tpx_params_update", "tpx_update_rho",
     "tpx_update_p0", "tpx_update_particle", "tpx_update_passive_scalar", "tpx_update_ru_wop", "tpx_compute_div_u_new",
-    "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_poisson_solve", "tpx_update_ru_wp", "tpx_time_advance_rk4",
+    "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_poisson_solve", "tpx_poisson_solve_device", "tpx_update_ru_wp", "tpx_time_advance_rk4",
     "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
     "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_gradient_heat", "tpx_adjoint_set_gas_tape", "tpx_statistics_enable", "tpx_statistics",
     "tpx_tune_tiles", "tpx_get_tiles", "tpx_set_tiles",
