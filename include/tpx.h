@@ -56,6 +56,10 @@ typedef struct tpx_params {
   int solve_for_scalar;           /* solve_for_scalar: advance the passive scalar */
   int s1_type;                    /* S1_type != 0 (the source form of params.h:78) */
   double d_m;                     /* D_M: its molecular diffusivity */
+  int solve_for_vector;           /* solve_for_vector: advance the passive vector */
+  int s2_type;                    /* S2_type != 0 (the source form of params.h:80) */
+  double eta;                     /* eta0: the vector's diffusivity (params.h:79) */
+  double rho_forv;                /* Rho_forV: its constant density (params.h:100) */
 } tpx_params;
 
 typedef struct tpx_ctx tpx_ctx;
@@ -64,9 +68,11 @@ typedef struct tpx_ctx tpx_ctx;
 typedef enum tpx_field {
   TPX_RU = 0, TPX_RU_INT, TPX_RU_NEW, TPX_RU_NP1,  /* tensor1, comp 0..2 */
   TPX_RHS_RU,                                       /* tensor1: deposited drag */
+  TPX_RV, TPX_RV_INT, TPX_RV_NEW, TPX_RV_NP1,      /* tensor1: the passive vector (grid.h:65-68) */
   TPX_P = 16, TPX_RHS_POIS, TPX_C, TPX_RHO, TPX_RHO_INT, TPX_RHO_NEW,
   TPX_RHO_NP1, TPX_DIVERGENCE, TPX_RHS_PART_TEMP,
-  TPX_PASSIVE_SCALAR, TPX_PASSIVE_SCALAR_INT, TPX_PASSIVE_SCALAR_NEW, TPX_PASSIVE_SCALAR_NP1
+  TPX_PASSIVE_SCALAR, TPX_PASSIVE_SCALAR_INT, TPX_PASSIVE_SCALAR_NEW, TPX_PASSIVE_SCALAR_NP1,
+  TPX_Q, TPX_RHS_POIS_Q                             /* the passive vector's pressure and its RHS */
 } tpx_field;
 
 /* particle stage identifiers (particle.h:63-113) */
@@ -123,6 +129,12 @@ int tpx_update_particle(tpx_ctx *ctx, int rk);       /* grid.cpp:865-888 */
 /* grid.cpp:711-745 (C_Source :686-708); tpx_step runs it after
  * Update_Particle when solve_for_scalar is set (simulation.cpp:65) */
 int tpx_update_passive_scalar(tpx_ctx *ctx, int rk);
+/* the passive vector (simulation.cpp:66-71; tpx_step runs the four after
+ * Update_Passive_Scalar when solve_for_vector is set) */
+int tpx_update_rv_woq(tpx_ctx *ctx, int rk);         /* grid.cpp:646-692, V_Source :616-641 */
+int tpx_compute_rhs_pois_q(tpx_ctx *ctx, int rk);    /* grid.cpp:694-701 */
+int tpx_solve_poisson_q(tpx_ctx *ctx, int rk, int *iterations); /* grid.cpp:703-707 */
+int tpx_update_rv_wq(tpx_ctx *ctx, int rk);          /* grid.cpp:709-715 */
 int tpx_update_ru_wop(tpx_ctx *ctx, int rk);         /* grid.cpp:776-814 */
 int tpx_compute_div_u_new(tpx_ctx *ctx, int rk);     /* grid.cpp:832-839 */
 int tpx_compute_rhs_pois(tpx_ctx *ctx, int rk);      /* grid.cpp:841-849 */
@@ -294,6 +306,11 @@ enum {
 };
 int tpx_statistics_enable(tpx_ctx *ctx, int on);
 int tpx_statistics(tpx_ctx *ctx, double *out /* TPX_ST_COUNT */);
+/* particle::trajectory (particle.cpp:717-745): the positions x, y, z of the
+ * particles whose flag is m*stride, m < count, into xyz[3m..3m+2] (NaN where no
+ * particle has that flag); the reference tracks flags 0, K, 2K, ... below N0
+ * with K = ceil(N0 / Np_track) */
+int tpx_track_particles(tpx_ctx *ctx, int stride, int count, double *xyz);
 
 /* ---- deposit tile shapes (no counterpart in the reference).
  * The heat and velocity deposit kernels march `zc` planes with `rows` whole

@@ -14,8 +14,8 @@
 #include "oracle.h"
 
 #define OR_PI 3.141592653589793 /* params.h:5 */
-#define NT1 10
-#define NT0 17
+#define NT1 17
+#define NT0 19
 
 struct or_state {
   or_params p;
@@ -889,7 +889,6 @@ static void poisson_fft(or_state *s, double *const *C, double *P, const double *
     it++;
   }
   s->num_iteration = it;
-  s->total_iterations += it;
 }
 
 /* ------------------------------------------------------------------ */
@@ -915,6 +914,10 @@ void or_initialize(or_state *s)
   }
   ew_copy(s, T0(s, OR_RHO_INT), T0(s, OR_RHO));
   ew_copy(s, T0(s, OR_RHO_NP1), T0(s, OR_RHO));
+  for (int c = 0; c < 3; ++c) {  /* grid.cpp:463-464 */
+    ew_copy(s, T1(s, OR_RV_INT)[c], T1(s, OR_RV)[c]);
+    ew_copy(s, T1(s, OR_RV_NP1)[c], T1(s, OR_RV)[c]);
+  }
   ew_copy(s, T0(s, OR_PS_INT), T0(s, OR_PS));  /* grid.cpp:465-466, 476 */
   ew_copy(s, T0(s, OR_PS_NP1), T0(s, OR_PS));
   i_c2f(s, T1(s, OR_PS_FACE), T0(s, OR_PS));
@@ -1036,6 +1039,7 @@ void or_solve_poisson(or_state *s)
   double **d = T1(s, OR_DUMMY);
   for (int c = 0; c < 3; ++c) ew_sdiv(s, d[c], 1., T1(s, OR_RHO_FACE)[c]);
   poisson_fft(s, d, T0(s, OR_P), T0(s, OR_RHS_POIS));
+  s->total_iterations += s->num_iteration;
 }
 
 /* poisson::Solve(tensor1& C, tensor0& P, tensor0& RHS) (poisson.cpp:115-117)
@@ -1073,6 +1077,7 @@ void or_time_advance_rk4(or_state *s)
 {
   ew_copy(s, T0(s, OR_RHO_INT), T0(s, OR_RHO_NEW));
   for (int c = 0; c < 3; ++c) ew_copy(s, T1(s, OR_RU_INT)[c], T1(s, OR_RU_NEW)[c]);
+  for (int c = 0; c < 3; ++c) ew_copy(s, T1(s, OR_RV_INT)[c], T1(s, OR_RV_NEW)[c]);  /* grid.cpp:894 */
   ew_copy(s, T0(s, OR_PS_INT), T0(s, OR_PS_NEW));  /* grid.cpp:897 */
   s->P0_int = s->P0_new;
   for (int a = 0; a < 7; ++a) memcpy(s->q[a][OR_INT], s->q[a][OR_NEW], sizeof(double) * (size_t)s->np);
@@ -1126,6 +1131,119 @@ void or_update_passive_scalar(or_state *s)
   i_c2f(s, F, T0(s, OR_PS_NEW));
 }
 
+/* ------------------------------------------------------------------ */
+/* passive vector (grid.cpp:616-715)                                    */
+/* ------------------------------------------------------------------ */
+/* Equal_Divide(tensor0 a, double b): o = a/b (tensor0.cpp:143-147) */
+static void ew_divs(const or_state *s, double *o, const double *a, double b) {
+#pragma omp parallel for schedule(static)
+  EW_LOOP o[e] = a[e] / b; }
+
+/* grid::V_Source (grid.cpp:616-641) with vector_source.h:4-15: S2.x =
+ * Vector_Source_x(X - dx/2, Y, Z) = 0, S2.y = Vector_Source_y(X, Y - dy/2, Z)
+ * = A cos(K X) (A = K = 1, X the cell-centre x), S2.z = 0 */
+static void v_source(or_state *s, double *const *S2)
+{
+  IN_LOOP(s) {
+    long c = G(s, i, j, k);
+    double X = (i - s->bs) * s->dx + s->dx / 2.;
+    double A = 1., K = 1.;
+    S2[0][c] = 0.;
+    S2[1][c] = A * cos(K * X);
+    S2[2][c] = 0.;
+  }
+  fill(s, S2[0]); fill(s, S2[1]); fill(s, S2[2]);
+}
+
+/* tensor0::Equal_Iy_C2F(T) called with T the tensor itself
+ * (grid.cpp:684: dummy.x.Equal_Iy_C2F(dummy.x)): the loop of
+ * tensor0.cpp:308-315 runs i, then j upwards, then k, and reads row j - 1
+ * after it has been overwritten, so row j becomes (f_j + new_{j-1}) / 2, the
+ * first interior row taking its ghost neighbour's old value. */
+static void iy_c2f_in_place(or_state *s, double *f)
+{
+  long sy = s->NX;
+#pragma omp parallel for schedule(static)
+  for (int i = s->bs; i < s->NX - s->bs; ++i)
+    for (int j = s->bs; j < s->NY - s->bs; ++j)
+      for (int k = s->bs; k < s->NZ - s->bs; ++k) {
+        long c = G(s, i, j, k);
+        f[c] = (f[c] + f[c - sy]) * 0.5;
+      }
+  fill(s, f);
+}
+
+/* grid::Update_RV_WOQ (grid.cpp:646-692): V = RV_int/Rho_forV; RHS_RV =
+ * eta/3 grad(div V) + eta del2 V - div(I_C2F(RV_int.a) I_a(U)) per component
+ * a (U = RU_int/Rho_face, left by Update_Particle), plus the source (S2_type
+ * 0: V_Source minus its means; else -Iy_C2F of RU_int.x/Rho_face.x on the y
+ * component), then the RK4 updates. */
+void or_update_rv_woq(or_state *s)
+{
+  double dt = s->p.dt, eta = s->p.eta;
+  double **V = T1(s, OR_V), **RVi = T1(s, OR_RV_INT), **RHS = T1(s, OR_RHS_RV), **U = T1(s, OR_U);
+  double **d = T1(s, OR_DUMMY), **d2 = T1(s, OR_DUMMY2), **S2 = T1(s, OR_S2);
+  double *dv = T0(s, OR_DIVERGENCE);
+  for (int c = 0; c < 3; ++c) ew_divs(s, V[c], RVi[c], s->p.rho_forv);
+  div_f2c(s, dv, V);
+  grad_c2f(s, d2, dv);
+  for (int c = 0; c < 3; ++c) del2(s, d[c], V[c]);
+  for (int c = 0; c < 3; ++c) ew_lc2(s, RHS[c], eta / 3., d2[c], eta, d[c]);
+  for (int a = 0; a < 3; ++a) {          /* convection of component a */
+    i_c2f(s, d, RVi[a]);
+    iaxis_c2f_t1(s, d2, U, a);
+    for (int c = 0; c < 3; ++c) ew_mul(s, d[c], d2[c]);
+    div_f2c(s, d2[a], d);
+    ew_sub(s, RHS[a], d2[a]);
+  }
+  if (!s->p.s2_type) {
+    v_source(s, S2);
+    for (int c = 0; c < 3; ++c) ew_sub_s(s, S2[c], red_mean(s, S2[c]));
+    for (int c = 0; c < 3; ++c) ew_add(s, RHS[c], S2[c]);
+  } else {
+    ew_div(s, d[0], T1(s, OR_RU_INT)[0], T1(s, OR_RHO_FACE)[0]);
+    iy_c2f_in_place(s, d[0]);
+    ew_pem(s, RHS[1], -1, d[0]);
+  }
+  for (int c = 0; c < 3; ++c) {
+    ew_pem(s, T1(s, OR_RV_NP1)[c], dt * s->post[s->rk], RHS[c]);
+    if (s->rk != 3) ew_lc2(s, T1(s, OR_RV_NEW)[c], 1, T1(s, OR_RV)[c], dt * s->pre[s->rk], RHS[c]);
+    else ew_copy(s, T1(s, OR_RV_NEW)[c], T1(s, OR_RV_NP1)[c]);
+  }
+}
+
+/* grid::Compute_RHS_Pois_Q (grid.cpp:694-701) */
+void or_compute_rhs_pois_q(or_state *s)
+{
+  double **V = T1(s, OR_V);
+  double *R = T0(s, OR_RHS_POIS_Q);
+  for (int c = 0; c < 3; ++c) ew_divs(s, V[c], T1(s, OR_RV_NEW)[c], s->p.rho_forv);
+  div_f2c(s, R, V);
+  ew_mul_s(s, R, (1. / (s->p.dt * s->pre[s->rk])));
+  make_mean_zero(s, R);
+}
+
+/* grid::Solve_Poisson_Q (grid.cpp:703-707): coefficient 1/Rho_forV on every
+ * face; the solver object is the gas's, so num_iteration is shared */
+void or_solve_poisson_q(or_state *s)
+{
+  double **d = T1(s, OR_DUMMY);
+  for (int c = 0; c < 3; ++c) ew_set(s, d[c], 1. / s->p.rho_forv);
+  poisson_fft(s, d, T0(s, OR_Q), T0(s, OR_RHS_POIS_Q));
+}
+
+/* grid::Update_RV_WQ (grid.cpp:709-715) */
+void or_update_rv_wq(or_state *s)
+{
+  double **d = T1(s, OR_DUMMY), dt = s->p.dt;
+  grad_c2f(s, d, T0(s, OR_Q));
+  for (int c = 0; c < 3; ++c) {
+    ew_pem(s, T1(s, OR_RV_NP1)[c], -(dt * s->post[s->rk]), d[c]);
+    if (s->rk != 3) ew_pem(s, T1(s, OR_RV_NEW)[c], -(dt * s->pre[s->rk]), d[c]);
+    else ew_copy(s, T1(s, OR_RV_NEW)[c], T1(s, OR_RV_NP1)[c]);
+  }
+}
+
 /* G12: tensor1::make_mean_U0 on RU_np1 (tensor1.cpp:406-411, tensor0.cpp:386-396) */
 void or_make_mean_u0(or_state *s, const double a[3])
 {
@@ -1143,6 +1261,7 @@ void or_time_advance(or_state *s)
   ew_copy(s, T0(s, OR_RHO), T0(s, OR_RHO_NP1));
   for (int c = 0; c < 3; ++c) ew_copy(s, T1(s, OR_RU)[c], T1(s, OR_RU_NP1)[c]);
   ew_copy(s, T0(s, OR_PS), T0(s, OR_PS_NP1));  /* grid.cpp:598 */
+  for (int c = 0; c < 3; ++c) ew_copy(s, T1(s, OR_RV)[c], T1(s, OR_RV_NP1)[c]);  /* grid.cpp:600 */
   s->P0 = s->P0_np1;
   for (int a = 0; a < 7; ++a) memcpy(s->q[a][OR_BASE], s->q[a][OR_NP1], sizeof(double) * (size_t)s->np);
   s->T_cur += s->p.dt;
@@ -1162,6 +1281,12 @@ void or_step(or_state *s, int nsteps)
       or_update_p0(s);
       or_update_particle(s);
       if (s->p.solve_for_scalar) or_update_passive_scalar(s);
+      if (s->p.solve_for_vector) {  /* simulation.cpp:66-71 */
+        or_update_rv_woq(s);
+        or_compute_rhs_pois_q(s);
+        or_solve_poisson_q(s);
+        or_update_rv_wq(s);
+      }
       or_update_ru_wop(s);
       or_compute_div_u_new(s);
       or_compute_rhs_pois(s);

@@ -87,6 +87,8 @@ typedef struct or_params {
   double cp, cv, np0, dp, rhop, cvp, epsilonp, nu, i01, i02;
   int solve_for_scalar, s1_type;  /* passive scalar switch and source form */
   double d_m;                     /* its molecular diffusivity */
+  int solve_for_vector, s2_type;  /* passive vector switch and source form (S2_type != 0) */
+  double eta, rho_forv;           /* its diffusivity eta0 and constant density Rho_forV */
 } or_params;
 
 typedef struct or_state or_state;
@@ -94,10 +96,12 @@ typedef struct or_state or_state;
 /* field identifiers */
 enum {
   OR_RU = 0, OR_RU_INT, OR_RU_NEW, OR_RU_NP1, OR_RHS_RU, OR_U, OR_RHO_FACE,
-  OR_DUMMY, OR_DUMMY2, OR_PS_FACE,         /* tensor1 (component 0..2) */
-  OR_P = 16, OR_RHS_POIS, OR_C, OR_RHO, OR_RHO_INT, OR_RHO_NEW, OR_RHO_NP1,
+  OR_DUMMY, OR_DUMMY2, OR_PS_FACE,
+  OR_RV, OR_RV_INT, OR_RV_NEW, OR_RV_NP1, OR_RHS_RV, OR_V, OR_S2,  /* tensor1 (component 0..2) */
+  OR_P = 32, OR_RHS_POIS, OR_C, OR_RHO, OR_RHO_INT, OR_RHO_NEW, OR_RHO_NP1,
   OR_T, OR_DIVERGENCE, OR_RHS_PART_TEMP, OR_RHS_RHO,
-  OR_PS, OR_PS_INT, OR_PS_NEW, OR_PS_NP1, OR_RHS_PS, OR_S1  /* tensor0; PS the passive scalar */
+  OR_PS, OR_PS_INT, OR_PS_NEW, OR_PS_NP1, OR_RHS_PS, OR_S1,  /* tensor0; PS the passive scalar */
+  OR_Q, OR_RHS_POIS_Q                                      /* the passive vector's pressure */
 };
 /* particle stage identifiers */
 enum { OR_BASE = 0, OR_INT = 1, OR_NEW = 2, OR_NP1 = 3 };
@@ -140,6 +144,10 @@ void or_set_rk4_count(or_state *s, int rk);
 void or_update_rho(or_state *s);              /* grid.cpp:608-614 */
 void or_update_p0(or_state *s);               /* grid.cpp:816-830 */
 void or_update_passive_scalar(or_state *s);   /* grid.cpp:711-745 */
+void or_update_rv_woq(or_state *s);           /* grid.cpp:646-692 (V_Source :616-641) */
+void or_compute_rhs_pois_q(or_state *s);      /* grid.cpp:694-701 */
+void or_solve_poisson_q(or_state *s);         /* grid.cpp:703-707 */
+void or_update_rv_wq(or_state *s);            /* grid.cpp:709-715 */
 void or_update_particle(or_state *s);         /* grid.cpp:865-888 */
 void or_update_ru_wop(or_state *s);           /* grid.cpp:776-814 */
 void or_compute_div_u_new(or_state *s);       /* grid.cpp:832-839 */

@@ -29,6 +29,7 @@ _FIELD_FMT = [
     ("cp", "d"), ("cv", "d"), ("np0", "d"), ("dp", "d"), ("rhop", "d"),
     ("cvp", "d"), ("epsilonp", "d"), ("nu", "d"), ("i01", "d"), ("i02", "d"),
     ("solve_for_scalar", "i"), ("s1_type", "i"), ("d_m", "d"),
+    ("solve_for_vector", "i"), ("s2_type", "i"), ("eta", "d"), ("rho_forv", "d"),
 ]
 
 
@@ -37,11 +38,12 @@ class OrParams(ctypes.Structure):
 
 
 T1 = {"RU": 0, "RU_int": 1, "RU_new": 2, "RU_np1": 3, "RHS_RU": 4, "U": 5,
-      "Rho_face": 6, "dummy": 7, "dummy2": 8, "Passive_Scalar_face": 9}
-T0 = {"P": 16, "RHS_Pois": 17, "C": 18, "Rho": 19, "Rho_int": 20, "Rho_new": 21,
-      "Rho_np1": 22, "T": 23, "divergence": 24, "RHS_Part_Temp": 25, "RHS_Rho": 26,
-      "Passive_Scalar": 27, "Passive_Scalar_int": 28, "Passive_Scalar_new": 29,
-      "Passive_Scalar_np1": 30, "RHS_Passive_Scalar": 31, "S1": 32}
+      "Rho_face": 6, "dummy": 7, "dummy2": 8, "Passive_Scalar_face": 9,
+      "RV": 10, "RV_int": 11, "RV_new": 12, "RV_np1": 13, "RHS_RV": 14, "V": 15, "S2": 16}
+T0 = {"P": 32, "RHS_Pois": 33, "C": 34, "Rho": 35, "Rho_int": 36, "Rho_new": 37,
+      "Rho_np1": 38, "T": 39, "divergence": 40, "RHS_Part_Temp": 41, "RHS_Rho": 42,
+      "Passive_Scalar": 43, "Passive_Scalar_int": 44, "Passive_Scalar_new": 45,
+      "Passive_Scalar_np1": 46, "RHS_Passive_Scalar": 47, "S1": 48, "Q": 49, "RHS_Pois_Q": 50}
 STAGE = {"base": 0, "int": 1, "new": 2, "np1": 3}
 AUX = {"RHS_u": 0, "RHS_v": 1, "RHS_w": 2, "ug": 3, "vg": 4, "wg": 5, "Tg": 6,
        "RHS_T": 7, "tau": 8, "eps": 9}
@@ -82,6 +84,7 @@ def lib(path: Optional[str] = None):
         L.or_get_scalars.argtypes = [P, D]
         L.or_set_restart_numbers.argtypes = [P, ctypes.c_double, ctypes.c_double, I]
         for fn in ("or_initialize", "or_params_update", "or_update_rho", "or_update_p0", "or_update_passive_scalar",
+                   "or_update_rv_woq", "or_compute_rhs_pois_q", "or_solve_poisson_q", "or_update_rv_wq",
                    "or_update_particle", "or_update_ru_wop", "or_compute_div_u_new",
                    "or_compute_rhs_pois", "or_solve_poisson", "or_update_ru_wp",
                    "or_time_advance_rk4", "or_time_advance", "or_p_gather_velocity",
@@ -221,6 +224,10 @@ class Oracle:
     def update_p0(self): self.L.or_update_p0(self.h)
     def update_particle(self): self.L.or_update_particle(self.h)
     def update_passive_scalar(self): self.L.or_update_passive_scalar(self.h)
+    def update_rv_woq(self): self.L.or_update_rv_woq(self.h)
+    def compute_rhs_pois_q(self): self.L.or_compute_rhs_pois_q(self.h)
+    def solve_poisson_q(self): self.L.or_solve_poisson_q(self.h)
+    def update_rv_wq(self): self.L.or_update_rv_wq(self.h)
     def update_ru_wop(self): self.L.or_update_ru_wop(self.h)
     def compute_div_u_new(self): self.L.or_compute_div_u_new(self.h)
     def compute_rhs_pois(self): self.L.or_compute_rhs_pois(self.h)
@@ -257,8 +264,8 @@ class Oracle:
         boundary, keyed as tpx.STATS; p is the run's tpx.params.Params.  U is
         the field the last Compute_RHS_Pois left (its mean square is TKE2),
         then U = RU_np1 / Rho_face as the reference recomputes it; C and the
-        particles' ug are what the last Update_Particle left.  The passive
-        vector is off: its entries are 0."""
+        particles' ug are what the last Update_Particle left.  With the
+        passive vector off its entries are 0."""
         nx, ny, nz = p.Nx_tot, p.Ny_tot, p.Nz_tot
         n = nx * ny * nz
         dx, dy, dz = p.Lx / nx, p.Ly / ny, p.Lz / nz
@@ -271,6 +278,13 @@ class Oracle:
         sc = self.scalars()
         npart = self.np
         st = dict.fromkeys(["TKEV_V1", "TKEV_V2", "TKEV_V3", "VV", "Particle_CFL_max"], 0.0)
+        if p.solve_for_vector:
+            # VV: V as the last Compute_RHS_Pois_Q left it; then V = RV_np1 / Rho_forV,
+            # V *= RV_np1 and the component means (grid.cpp:912, 918-933)
+            st["VV"] = float((self.get("V") ** 2).sum()) / n
+            RVn = self.get("RV_np1")
+            for c, key in enumerate(("TKEV_V1", "TKEV_V2", "TKEV_V3")):
+                st[key] = float(((RVn[c] / p.Rho_forV) * RVn[c]).sum()) / n
         st["Passive_Scalar_mean"] = float((self.get("Passive_Scalar") ** 2).sum()) / n  # mean_squares
         st["Rho_mean"], st["Rho_max"], st["Rho_min"] = rho.sum() / n, rho.max(), rho.min()
         st["C_max"], st["C_min"], st["C_mean"] = C.max(), C.min(), C.sum() / n

@@ -23,6 +23,9 @@ def oracle_from_case(case: ic.Case) -> Oracle:
                     tau=case.tau, eps=case.eps)
     if getattr(case, "PS", None) is not None:
         o.set("Passive_Scalar", case.PS)
+    if getattr(case, "RV", None) is not None:
+        o.set("RV", case.RV)
+        o.set("Q", case.Q if getattr(case, "Q", None) is not None else np.zeros(case.RV.shape[1:]))
     o.set_restart_numbers(case.params.P0, 0.0, 0)
     o.initialize()
     return o

@@ -130,3 +130,168 @@ def test_driver_runs_the_passive_scalar(gpu_available, sim, tmp_path):
     e = rel_err(C, o.get("Passive_Scalar"))
     print("passive scalar after 2 steps", e)
     assert e <= 1e-10
+
+
+def driver_case(tmp_path, seed, npart=1000, **kw):
+    case = small_case(n=16, npart=npart, heat=True, variable_rho=True, eps_range=None, seed=seed)
+    p = case.params
+    p.np0 = (npart + 0.5) / p.Vol if npart else 0.0
+    p.Initial = 1
+    p.Data_dir = str(tmp_path) + "/"
+    p.Stat_dir = str(tmp_path) + "/"
+    p.Data_freq_fast = 1
+    p.Data_freq_slow = 2
+    p.T_final = 2.5 * p.dt
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return case
+
+
+def run_sim(sim, tmp_path, case, timeout=120):
+    p = case.params
+    p.write(str(tmp_path / "input.txt"))
+    ic.write_restart(str(tmp_path), case.RU, case.Rho, case.P, p.P0, 0.0, 0, case.particles)
+    r = subprocess.run([sim, "input.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def analytic_rv5(p):
+    """Initial_V = 5 (grid.cpp:256-278)."""
+    n = p.Nx_tot
+    d = p.Lx / n
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    X, Y = d / 2. + i * d, d / 2. + j * d
+    kx, ky = 2 * np.pi / p.Lx, 2 * np.pi / p.Ly
+    RV = np.zeros((3, n, n, n))
+    RV[0] = ky * np.sin(kx * (X - d / 2.)) * np.cos(ky * Y)
+    RV[1] = -kx * np.cos(kx * X) * np.sin(ky * (Y - d / 2.))
+    return RV
+
+
+@pytest.mark.gpu
+def test_driver_runs_the_passive_vector(gpu_available, sim, tmp_path):
+    """solve_for_vector = 1 with Initial_V = 5 and the type-0 source: the
+    driver calls the four vector methods after the scalar's and writes RV_<n>
+    and Q_<n> matching the oracle."""
+    case = driver_case(tmp_path, seed=31, solve_for_vector=1, Initial_V=5, eta=0.05, Rho_forV=1.2, s2t=0.0)
+    p = case.params
+    print("tpx_sim 16^3 + 1000 particles with the passive vector, 3 steps", flush=True)
+    run_sim(sim, tmp_path, case)
+    case.RV, case.Q = analytic_rv5(p), np.zeros((16, 16, 16))
+    o = oracle_from_case(case)
+    o.step(2)
+    n = p.Nx_tot
+    RV = np.fromfile(tmp_path / "RV_2.bin").reshape(3, n, n, n)
+    Q = np.fromfile(tmp_path / "Q_2.bin").reshape(n, n, n)
+    e = {"RV": rel_err(RV, o.get("RV")), "Q": rel_err(Q, o.get("Q"))}
+    print("passive vector after 2 steps", e)
+    assert max(e.values()) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_driver_writes_the_reference_file_set(gpu_available, sim, tmp_path):
+    """grid::Store (grid.cpp:489-593): XYZ, S1, S2 at step 1; RU, RV, C and
+    numbers every fast dump; particles, P, Q and Rho every slow dump -- with
+    the passive scalar and vector off (their files hold the initial fields)."""
+    case = driver_case(tmp_path, seed=33, Initial_C=3, Initial_V=2)
+    p = case.params
+    run_sim(sim, tmp_path, case)
+    got = sorted(f for f in os.listdir(tmp_path) if f.endswith((".bin", ".dat")) and not f.startswith("Restart_"))
+    fast = ["RU_{n}.bin", "RV_{n}.bin", "C_{n}.bin", "numbers_{n}.dat"]
+    slow = [f"Part_{q}_{{n}}.bin" for q in ("x", "y", "z", "u", "v", "w", "T", "flag")] + \
+        ["P_{n}.bin", "Q_{n}.bin", "Rho_{n}.bin"]
+    want = sorted(["XYZ.bin", "S1.bin", "S2.bin"] + [f.format(n=1) for f in fast] + [f.format(n=2) for f in fast] +
+                  [f.format(n=2) for f in slow])
+    assert got == want, (got, want)
+    n = p.Nx_tot
+    d = p.Lx / n
+    xyz = np.fromfile(tmp_path / "XYZ.bin").reshape(3, n, n, n)
+    assert np.array_equal(xyz[0][0, 0], d / 2. + np.arange(n) * d)
+    assert np.array_equal(xyz[2][:, 0, 0], d / 2. + np.arange(n) * d)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    C1 = np.fromfile(tmp_path / "C_1.bin").reshape(n, n, n)
+    Y, Z = d / 2. + j * d, d / 2. + k * d
+    assert np.max(np.abs(C1 - np.sin(2 * np.pi / p.Ly * Y) * np.sin(2 * np.pi / p.Lz * Z))) < 1e-15
+    assert np.max(np.abs(np.fromfile(tmp_path / "RV_2.bin"))) == 0.0
+    assert np.max(np.abs(np.fromfile(tmp_path / "S2.bin"))) == 0.0
+
+
+@pytest.mark.gpu
+def test_driver_stops_on_touch_check(gpu_available, sim, tmp_path):
+    """grid::Touch (grid.cpp:94-108): a 1 written into touch.check while the
+    run goes ends it after that step, with the full dump of Store (the slow
+    files too, whatever Data_freq_slow says)."""
+    import threading
+    import time
+    case = driver_case(tmp_path, seed=35, npart=500, T_final=400 * 1e-3, Data_freq_slow=100000)
+    p = case.params
+    p.dt = 1e-3
+    (tmp_path / "touch.check").write_text("0\n")
+
+    def toucher():
+        t0 = time.time()
+        while time.time() - t0 < 60:
+            if (tmp_path / "numbers_3.dat").exists():
+                (tmp_path / "touch.check").write_text("1\n")
+                return
+            time.sleep(0.005)
+    th = threading.Thread(target=toucher)
+    th.start()
+    r = run_sim(sim, tmp_path, case)
+    th.join()
+    assert "Someone touched me! Simulation stopped." in r.stdout
+    steps = sorted(int(f[8:-4]) for f in os.listdir(tmp_path) if f.startswith("numbers_"))
+    last = steps[-1]
+    assert 3 <= last < 400, steps
+    for f in ("P", "Q", "Rho", "Part_x", "Part_flag"):
+        assert os.path.exists(tmp_path / f"{f}_{last}.bin"), f
+    assert not any(os.path.exists(tmp_path / f"P_{m}.bin") for m in steps[:-1])
+
+
+@pytest.mark.gpu
+def test_driver_initial_0_from_velocity_files(gpu_available, sim, tmp_path):
+    """Initial = 0 (grid.cpp:240-253): U.bin, V.bin, W.bin times Rho0, uniform
+    density, zero pressure; without particles the run is the oracle's from
+    that state."""
+    case = driver_case(tmp_path, seed=37, npart=0, Initial=0)
+    p = case.params
+    rng = np.random.default_rng(5)
+    u = ic.spectral_velocity((16, 16, 16), (p.Lx, p.Ly, p.Lz), seed=5, k0=2.0, uprime=1.0)
+    for c, f in enumerate(("U.bin", "V.bin", "W.bin")):
+        u[c].tofile(tmp_path / f)
+    p.Rho0 = 1.0 + 0.5 * rng.random()
+    p.write(str(tmp_path / "input.txt"))
+    r = subprocess.run([sim, "input.txt"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    case.RU = u * p.Rho0
+    case.Rho = np.full((16, 16, 16), p.Rho0)
+    case.P = np.zeros((16, 16, 16))
+    o = oracle_from_case(case)
+    o.step(2)
+    RU = np.fromfile(tmp_path / "RU_2.bin").reshape(3, 16, 16, 16)
+    e = rel_err(RU, o.get("RU"))
+    print("Initial = 0 after 2 steps", e)
+    assert e <= 1e-10
+
+
+@pytest.mark.gpu
+def test_driver_writes_the_particle_trajectory(gpu_available, sim, tmp_path):
+    """part.trajectory (particle.cpp:717-745) from grid::Statistics: one line a
+    step, T_cur then x y z of the particles with flags 0, K, 2K, ... (K =
+    ceil(N0 / Np_track)), the base positions after that step."""
+    case = driver_case(tmp_path, seed=39, npart=1000, Statistics=1, Np_track=7)
+    p = case.params
+    run_sim(sim, tmp_path, case)
+    lines = (tmp_path / "Part_trajectory.dat").read_text().strip().splitlines()
+    assert len(lines) == 3
+    K = (1000 + 7 - 1) // 7
+    tracked = list(range(0, 1000, K))
+    vals = np.array(lines[1].split(), dtype=float)
+    assert abs(vals[0] - 2 * p.dt) < 1e-12 and len(vals) == 1 + 3 * len(tracked)
+    flag = np.fromfile(tmp_path / "Part_flag_2.bin", dtype=np.int32)
+    for m, f in enumerate(tracked):
+        at = int(np.nonzero(flag == f)[0][0])
+        for c, q in enumerate("xyz"):
+            x = np.fromfile(tmp_path / f"Part_{q}_2.bin")[at]
+            assert abs(vals[1 + 3 * m + c] - x) <= 1e-5 * max(1.0, abs(x))  # 6 significant digits

@@ -1219,14 +1219,16 @@ void launch_residual_coef(tpx_ctx *c, const double *const cf[3], const double *P
 }
 
 void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal,
-                                 const double *res) {
+                                 const double *res, int sum_slot) {
   const Geo &g = c->g;
   int mxh = g.nx / 2 + 1;
   long nlines = (long)g.ny * g.nz;
   int r1, r2;
   timing_begin(c, TPX_K_FFT);
   const double *done = scal + SC_DONE;
-  XArgs xa{g, P, rho_new, rhs, scal, c->spec[0], c->tw[0], (double)g.n, done, res};
+  // XArgs.scal serves only the mean of rhs, read at SC_SUM_RHS: point it so
+  // that slot is sum_slot
+  XArgs xa{g, P, rho_new, rhs, scal + (sum_slot - SC_SUM_RHS), c->spec[0], c->tw[0], (double)g.n, done, res};
   XInvArgs xi{g, P, c->spec[0], c->tw[0], c->partials, (double)g.n, done};
   const bool x4 = four_step(g.nx, r1, r2);
   int y1, y2, z1, z2;

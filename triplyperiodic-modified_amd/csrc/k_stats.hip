@@ -231,6 +231,36 @@ void stats_capture_velocity(tpx_ctx *c, const double *const ru_new[3], const dou
 }
 
 // raw[0..12]: the cell reductions, raw[13..15]: the particle ones
+// particle::trajectory (particle.cpp:717-745): the position of the particle
+// whose flag is m*stride, for m < count, into out[3m..3m+2] (left as filled
+// when no particle has that flag)
+struct k_track {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const double *__restrict__ x, const double *__restrict__ y,
+                                             const double *__restrict__ z, const int32_t *__restrict__ flag, long n,
+                                             int stride, int count, double *__restrict__ out) {
+    const long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (p >= n) return;
+    const int f = flag[p];
+    if (f < 0 || f % stride != 0 || f / stride >= count) return;
+    const int m = f / stride;
+    out[3 * m] = x[p];
+    out[3 * m + 1] = y[p];
+    out[3 * m + 2] = z[p];
+  }
+};
+
+void launch_track(tpx_ctx *c, int stride, int count, double *out) {
+  launch_fill(c, out, NAN, 3L * count);
+  Particles &pt = c->pt;
+  if (pt.n > 0) {
+    tpx_launch<k_track>(grid1d(pt.n, kBlock), kBlock, 0, c->stream, (const double *)pt.q[0][pt.st_base],
+                        (const double *)pt.q[1][pt.st_base], (const double *)pt.q[2][pt.st_base],
+                        (const int32_t *)pt.flag, (long)pt.n, stride, count, out);
+    check_launch("k_track");
+  }
+}
+
 void stats_reduce(tpx_ctx *c, const double *rho, const double *const ru[3], double raw[16]) {
   double *part = stats_buffer(c);
   double *out = part + 16 * c->st_stride;

@@ -153,8 +153,28 @@ void ensure_passive_scalar(tpx_ctx *c) {
   }
 }
 
+// the passive vector's buffers (RV's four sets, Q, the raw right-hand side of
+// Q's solve, the type-1 source's scratch: zero) and the constant density
+// Rho_forV as a field, allocated on first use
+void ensure_passive_vector(tpx_ctx *c) {
+  if (c->Qv) return;
+  const size_t bytes = sizeof(double) * (size_t)c->g.n;
+  auto zero = [&](double *&f) {
+    if (hipMalloc(&f, bytes) != hipSuccess) throw std::bad_alloc();
+    hip_check(hipMemsetAsync(f, 0, bytes, c->stream), "hipMemsetAsync");
+  };
+  for (auto &set : c->rv)
+    for (auto &f : set) zero(f);
+  zero(c->rhs_q);
+  zero(c->rv_filt);
+  if (hipMalloc(&c->rho_forv_field, bytes) != hipSuccess) throw std::bad_alloc();
+  launch_fill(c, c->rho_forv_field, c->p.rho_forv, c->g.n);
+  zero(c->Qv);
+}
+
 double *field_buffer(tpx_ctx *c, int field, int comp) {
   if (field >= TPX_PASSIVE_SCALAR && field <= TPX_PASSIVE_SCALAR_NP1) ensure_passive_scalar(c);
+  if ((field >= TPX_RV && field <= TPX_RV_NP1) || field == TPX_Q || field == TPX_RHS_POIS_Q) ensure_passive_vector(c);
   switch (field) {
   case TPX_RU: return c->ru[c->ru_base][comp];
   case TPX_RU_INT: return c->ru[c->ru_int][comp];
@@ -174,11 +194,17 @@ double *field_buffer(tpx_ctx *c, int field, int comp) {
   case TPX_PASSIVE_SCALAR_INT: return c->ps[c->ps_int];
   case TPX_PASSIVE_SCALAR_NEW: return c->ps[c->ps_new];
   case TPX_PASSIVE_SCALAR_NP1: return c->ps[c->ps_np1];
+  case TPX_RV: return c->rv[c->rv_base][comp];
+  case TPX_RV_INT: return c->rv[c->rv_int][comp];
+  case TPX_RV_NEW: return c->rv[c->rv_new][comp];
+  case TPX_RV_NP1: return c->rv[c->rv_np1][comp];
+  case TPX_Q: return c->Qv;
+  case TPX_RHS_POIS_Q: return c->rhs_q;
   default: return nullptr;
   }
 }
 
-bool is_vector(int field) { return field >= TPX_RU && field <= TPX_RHS_RU; }
+bool is_vector(int field) { return field >= TPX_RU && field <= TPX_RV_NP1; }
 
 void sync(tpx_ctx *c) { hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
 
@@ -333,6 +359,47 @@ void do_solve_coef(tpx_ctx *c, const double *const cf[3], double *P, const doubl
   if (iters) *iters = i;
 }
 
+// grid::Update_RV_WOQ (grid.cpp:646-692); U = RU_int/Rho_face from the stage state
+void do_update_rv_woq(tpx_ctx *c, int rk) {
+  ensure_passive_vector(c);
+  const double *ri[3];
+  ru_ptrs(c, c->ru_int, ri);
+  launch_rv_woq(c, coef(c, rk), ri, c->rho[c->rho_int]);
+  c->div_valid = false;  // the reference's Update_RV_WOQ overwrites divergence with div(V)
+}
+
+// grid::Compute_RHS_Pois_Q (grid.cpp:694-701); its mean is removed where the
+// solve reads it (and on download)
+void do_compute_rhs_pois_q(tpx_ctx *c, int rk) {
+  ensure_passive_vector(c);
+  launch_rhs_pois_q(c, coef(c, rk), c->stats_on && rk == 3);
+}
+
+// grid::Solve_Poisson_Q (grid.cpp:703-707): the FFT iteration with the
+// constant density Rho_forV; the solver's iteration count is shared with the
+// gas's solve, as the reference's single poisson object is
+void do_solve_poisson_q(tpx_ctx *c, int *iters) {
+  ensure_passive_vector(c);
+  const int i = solve_loop(c, 2, [&](int) {
+    launch_poisson_iteration_on(c, c->rho_forv_field, c->Qv, c->rhs_q, c->scal, nullptr, SC_SUM_RHS_Q);
+  });
+  c->num_iteration = i;
+  if (iters) *iters = i;
+}
+
+// grid::Update_RV_WQ (grid.cpp:709-715)
+void do_update_rv_wq(tpx_ctx *c, int rk) {
+  ensure_passive_vector(c);
+  launch_rv_wq(c, coef(c, rk));
+}
+
+void do_passive_vector(tpx_ctx *c, int rk) {
+  do_update_rv_woq(c, rk);
+  do_compute_rhs_pois_q(c, rk);
+  do_solve_poisson_q(c, nullptr);
+  do_update_rv_wq(c, rk);
+}
+
 void do_update_ru_wp(tpx_ctx *c, int rk, bool want_sums) {
   Coef k = coef(c, rk);
   double *on[3], *ow[3];
@@ -348,6 +415,7 @@ void do_time_advance_rk4(tpx_ctx *c) {
   c->ru_new = -1;
   launch_scalar_copy(c, SC_P0_INT, SC_P0_NEW);
   if (c->ps[0]) std::swap(c->ps_int, c->ps_new);  // Passive_Scalar_int = Passive_Scalar_new (grid.cpp:897)
+  if (c->Qv) std::swap(c->rv_int, c->rv_new);     // RV_int = RV_new (grid.cpp:894)
   Particles &pt = c->pt;
   std::swap(pt.st_int, pt.st_new);  // with int_is_base, new receives the spare buffer
   pt.int_is_base = false;
@@ -377,6 +445,8 @@ void do_time_advance(tpx_ctx *c) {
   c->ru_np1_is_base = true;
   launch_scalar_copy(c, SC_P0, SC_P0_NP1);
   if (c->ps[0]) launch_copy(c, c->ps[c->ps_base], c->ps[c->ps_np1], c->g.n);  // grid.cpp:598
+  if (c->Qv)  // RV = RV_np1 (grid.cpp:600)
+    for (int d = 0; d < 3; ++d) launch_copy(c, c->rv[c->rv_base][d], c->rv[c->rv_np1][d], c->g.n);
   c->T_cur += c->p.dt;
   c->num_timestep++;
   Particles &pt = c->pt;
@@ -422,6 +492,7 @@ void fused_substep_pre(tpx_ctx *c, int rk) {
     if (c->p.twc) launch_velocity_deposit(c);
   }
   if (c->p.solve_for_scalar) do_update_passive_scalar(c, rk);  // simulation.cpp:65
+  if (c->p.solve_for_vector) do_passive_vector(c, rk);          // simulation.cpp:66-71
   do_update_ru_wop(c, rk, true);
   if (rebin) {
     launch_sort_new(c);
@@ -485,6 +556,8 @@ void release(tpx_ctx *c) {
   for (auto &b : c->ru) for (auto &p : b) f(p);
   for (auto &p : c->rho) f(p);
   f(c->P); f(c->rhs); f(c->Q); f(c->C); f(c->divergence); f(c->dep_tmp); f(c->coef_stage);
+  for (auto &set : c->rv) for (auto &p : set) f(p);
+  f(c->Qv); f(c->rhs_q); f(c->rho_forv_field); f(c->rv_filt); f(c->track);
   for (auto &p : c->S) f(p);
   for (auto &p : c->spec) f(p);
   for (auto &p : c->lam) f(p);
@@ -673,8 +746,8 @@ int tpx_upload_field(tpx_ctx *ctx, int field, int comp, const double *host, int 
     if (!host || bs < 0) throw std::invalid_argument("bad host array or bs");
     if (is_vector(field) ? (comp < 0 || comp > 2) : comp != 0) throw std::invalid_argument("bad component");
     if (!(field == TPX_RU || field == TPX_RHO || field == TPX_P || field == TPX_RHS_PART_TEMP ||
-          field == TPX_RHS_RU || field == TPX_PASSIVE_SCALAR))
-      throw std::invalid_argument("only RU, Rho, P, RHS_Part_Temp, RHS_RU and Passive_Scalar can be uploaded");
+          field == TPX_RHS_RU || field == TPX_PASSIVE_SCALAR || field == TPX_RV || field == TPX_Q))
+      throw std::invalid_argument("only RU, Rho, P, RHS_Part_Temp, RHS_RU, Passive_Scalar, RV and Q can be uploaded");
     const Geo &g = ctx->g;
     // uploading the base state resets the aliasing of int/np1 to the base
     if (field == TPX_RU && ctx->ru_int != ctx->ru_base) throw std::logic_error("upload RU before the first substep only");
@@ -706,8 +779,8 @@ int tpx_download_field(tpx_ctx *ctx, int field, int comp, double *host, int bs) 
     std::vector<double> tmp((size_t)g.n);
     hip_check(hipMemcpyAsync(tmp.data(), src, sizeof(double) * g.n, hipMemcpyDeviceToHost, ctx->stream), "download");
     sync(ctx);
-    if (field == TPX_RHS_POIS) {  // make_mean_zero applied where the solve reads it
-      double mean = read_scalar(ctx, SC_SUM_RHS) / (double)(g.n);
+    if (field == TPX_RHS_POIS || field == TPX_RHS_POIS_Q) {  // make_mean_zero applied where the solve reads it
+      double mean = read_scalar(ctx, field == TPX_RHS_POIS ? SC_SUM_RHS : SC_SUM_RHS_Q) / (double)(g.n);
       for (auto &v : tmp) v -= mean;
     }
     int GX = g.nx + 2 * bs, GY = g.ny + 2 * bs, GZ = g.nz + 2 * bs;
@@ -834,6 +907,14 @@ int tpx_initialize(tpx_ctx *ctx) {
       launch_copy(c, c->ps[c->ps_np1], c->ps[c->ps_base], c->g.n);
     }
     if (c->p.solve_for_scalar) launch_passive_scalar_source_sum(c);
+    if (c->p.solve_for_vector) ensure_passive_vector(c);
+    if (c->Qv) {  // RV_int = RV, RV_np1 = RV (grid.cpp:463-464)
+      for (int d = 0; d < 3; ++d) {
+        launch_copy(c, c->rv[c->rv_int][d], c->rv[c->rv_base][d], c->g.n);
+        launch_copy(c, c->rv[c->rv_np1][d], c->rv[c->rv_base][d], c->g.n);
+      }
+    }
+    if (c->p.solve_for_vector && c->p.s2_type == 0) launch_rv_source_sum(c);
     Particles &pt = c->pt;
     pt.np1_is_base = true;
     if (pt.n > 0) {
@@ -884,6 +965,42 @@ int tpx_update_passive_scalar(tpx_ctx *ctx, int rk) {
     TPX_RK_CHECK(rk);
     if (!ctx->p.solve_for_scalar) throw std::logic_error("Update_Passive_Scalar with solve_for_scalar = 0");
     do_update_passive_scalar(ctx, rk);
+    return TPX_OK;
+  });
+}
+
+int tpx_update_rv_woq(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (!ctx->p.solve_for_vector) throw std::logic_error("Update_RV_WOQ with solve_for_vector = 0");
+    do_update_rv_woq(ctx, rk);
+    return TPX_OK;
+  });
+}
+
+int tpx_compute_rhs_pois_q(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (!ctx->p.solve_for_vector) throw std::logic_error("Compute_RHS_Pois_Q with solve_for_vector = 0");
+    do_compute_rhs_pois_q(ctx, rk);
+    return TPX_OK;
+  });
+}
+
+int tpx_solve_poisson_q(tpx_ctx *ctx, int rk, int *iterations) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (!ctx->p.solve_for_vector) throw std::logic_error("Solve_Poisson_Q with solve_for_vector = 0");
+    do_solve_poisson_q(ctx, iterations);
+    return TPX_OK;
+  });
+}
+
+int tpx_update_rv_wq(tpx_ctx *ctx, int rk) {
+  return guarded(ctx, [&]() {
+    TPX_RK_CHECK(rk);
+    if (!ctx->p.solve_for_vector) throw std::logic_error("Update_RV_WQ with solve_for_vector = 0");
+    do_update_rv_wq(ctx, rk);
     return TPX_OK;
   });
 }
@@ -1553,6 +1670,7 @@ int tpx_batch_create(tpx_ctx *const *ctxs, int n, tpx_batch **out) {
     else if ((c->p.twc != 0) != (a->p.twc != 0)) why = "TWC differs";
     else if (heat_active(c) != heat_active(a)) why = "heat exchange on in some boxes only";
     else if (c->p.solve_for_scalar) why = "the passive scalar is not batched (use tpx_ensemble_step)";
+    else if (c->p.solve_for_vector) why = "the passive vector is not batched (use tpx_ensemble_step)";
     else if ((c->pt.n > 0) != (a->pt.n > 0)) why = "particles in some boxes only";
     else if (c->pt.has_tau != a->pt.has_tau || c->pt.has_eps != a->pt.has_eps) why = "per-particle tau/eps differ";
     else if (c->fft_passes != a->fft_passes) why = "Poisson pass counts differ";
@@ -1979,12 +2097,42 @@ int tpx_statistics(tpx_ctx *ctx, double *out) {
       launch_reduce(c, c->ps[c->ps_base], 2, SC_RED);
       out[TPX_ST_PASSIVE_SCALAR_MEAN] = read_scalar(c, SC_RED) / n;
     }
+    if (c->p.solve_for_vector && c->Qv) {
+      // V = RV_np1 / Rho_forV; V *= RV_np1; TKEV_V? = V.?.mean() (grid.cpp:918-933),
+      // as sums of RV^2 / Rho_forV; VV = V.mean_squares() of the last
+      // Compute_RHS_Pois_Q's V (grid.cpp:912)
+      const double tkev[3] = {TPX_ST_TKEV_V1, TPX_ST_TKEV_V2, TPX_ST_TKEV_V3};
+      for (int a = 0; a < 3; ++a) {
+        launch_reduce(c, c->rv[c->rv_base][a], 2, SC_RED);
+        out[(int)tkev[a]] = read_scalar(c, SC_RED) / c->p.rho_forv / n;
+      }
+      out[TPX_ST_VV] = sc[SC_ST_VV] / n;
+    }
     out[TPX_ST_P0] = sc[SC_P0];
     out[TPX_ST_PARTICLE_MAX_CFL] = 0.0;
     out[TPX_ST_GAS_MAX_DIFF_CFL] = dt / (c->g.dx * c->g.dx * out[TPX_ST_RHO_MIN] / c->p.mu0) * 6.;
     out[TPX_ST_NUM_ITERATION] = (double)c->num_iteration;
     const double n0 = (double)(int)(c->p.np0 * c->p.lx * c->p.ly * c->p.lz);  // params.h:120
     out[TPX_ST_BALANCE_INDEX] = n0 > 0 ? np / n0 : 0.0;
+    return TPX_OK;
+  });
+}
+
+int tpx_track_particles(tpx_ctx *ctx, int stride, int count, double *xyz) {
+  return guarded(ctx, [&]() {
+    if (stride <= 0 || count < 0 || (count > 0 && !xyz)) throw std::invalid_argument("bad stride, count or array");
+    if (count == 0) return TPX_OK;
+    if (ctx->track_cap < count) {
+      if (ctx->track) (void)hipFree(ctx->track);
+      ctx->track = nullptr;
+      ctx->track_cap = 0;
+      if (hipMalloc((void **)&ctx->track, sizeof(double) * 3 * (size_t)count) != hipSuccess) throw std::bad_alloc();
+      ctx->track_cap = count;
+    }
+    launch_track(ctx, stride, count, ctx->track);
+    hip_check(hipMemcpyAsync(xyz, ctx->track, sizeof(double) * 3 * (size_t)count, hipMemcpyDeviceToHost, ctx->stream),
+              "download");
+    sync(ctx);
     return TPX_OK;
   });
 }

@@ -48,6 +48,9 @@ enum Scal {
   // run's initial P0
   SC_ADJ_P0, SC_ADJ_P0NP1, SC_ADJ_P0NEW, SC_ADJ_LSQ, SC_ADJ_P0I,
   SC_PS_S1,        // passive scalar: sum over the cells of its source C_Source
+  SC_SUM_RHS_Q,    // passive vector: sum of the raw right-hand side of its Q solve
+  SC_RV_S2,        // passive vector: sum over the cells of its source V_Source (y)
+  SC_ST_VV,        // statistics: sum of squares of the last substep's V = RV_new / Rho_forV
   SC_COUNT = 32
 };
 
@@ -239,6 +242,17 @@ struct tpx_ctx {
   // TimeAdvance copies np1 into base
   double *ps[4] = {};
   int ps_base = 0, ps_int = 1, ps_new = 2, ps_np1 = 3;
+  // passive vector (grid.h:65-70, 99): RV's base, int, new and np1 buffer
+  // sets (swapped and copied like the scalar's), its pressure Q, the raw
+  // right-hand side of Q's solve, the constant density Rho_forV as a field
+  // (the Poisson iteration forms its coefficient from a density) and the
+  // type-1 source's filtered field; allocated when the vector is solved or
+  // uploaded
+  double *rv[4][3] = {};
+  int rv_base = 0, rv_int = 1, rv_new = 2, rv_np1 = 3;
+  double *Qv = nullptr, *rhs_q = nullptr, *rho_forv_field = nullptr, *rv_filt = nullptr;
+  double *track = nullptr;        // tracked particle positions (3 x track_cap)
+  int track_cap = 0;
   bool rhs_has_mean = false;      // rhs holds the raw RHS, SC_SUM_RHS its sum
   bool q_zero = false;            // Q is known to be identically zero
   bool div_valid = false;         // divergence holds Compute_Div_U_new's field (the fused
@@ -294,10 +308,20 @@ int fft_setup(tpx_ctx *c);
 // one Poisson iteration; skipped on the device once the solve has stopped
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
 // the same iteration on caller-given P, raw right-hand side and scalar slots
+// sum_slot: the scal slot holding the sum of rhs (its mean is removed in the residual)
 void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal,
-                                 const double *res = nullptr);
+                                 const double *res = nullptr, int sum_slot = SC_SUM_RHS);
 // r = (div_F2C(C grad_C2F P) - RHS) cc for a caller's face coefficient C (cf[0..2])
 void launch_residual_coef(tpx_ctx *c, const double *const cf[3], const double *P, const double *rhs, double *res);
+// the passive vector (k_vector.hip): Update_RV_WOQ, Compute_RHS_Pois_Q (raw
+// right-hand side into rhs_q, its sum into SC_SUM_RHS_Q, with want_vv the sum
+// of squares of V into SC_ST_VV), Update_RV_WQ, and the sum of V_Source
+void launch_rv_woq(tpx_ctx *c, const Coef &k, const double *const ru_int[3], const double *rho_int);
+void launch_rhs_pois_q(tpx_ctx *c, const Coef &k, bool want_vv);
+void launch_rv_wq(tpx_ctx *c, const Coef &k);
+void launch_rv_source_sum(tpx_ctx *c);
+// positions of the particles with flags m*stride, m < count (k_stats.hip)
+void launch_track(tpx_ctx *c, int stride, int count, double *out);
 // grid::Update_Passive_Scalar (k_grid.hip) and the sum of its source
 void launch_passive_scalar(tpx_ctx *c, const Coef &k, const double *const ru_int[3], const double *rho_int);
 void launch_passive_scalar_source_sum(tpx_ctx *c);

@@ -138,6 +138,26 @@ void communicator::read(tensor1 &T, const char *adrs) {
   read_component(*size_, T.z, adrs, 2);
 }
 
+std::vector<double> communicator::read_host(const char *adrs, int comp) {
+  size_t n = (size_t)size_->size_tot();
+  std::vector<double> in;
+  if (!read_raw(adrs, in, n, (size_t)comp * n)) throw std::runtime_error(std::string("cannot read ") + adrs);
+  std::vector<double> gh(ghosted_count(), 0.0);
+  interior_to_ghosted(*size_, in.data(), gh.data());
+  return gh;
+}
+
+void communicator::write_host(const std::vector<double> &ghosted, const char *adrs, bool append) {
+  std::vector<double> out((size_t)size_->size_tot());
+  ghosted_to_interior(*size_, ghosted.data(), out.data());
+  write_raw(adrs, out.data(), out.size(), append);
+}
+
+size_t communicator::ghosted_count() const {
+  size_t b = 2 * (size_t)size_->bs();
+  return ((size_t)size_->Nx() + b) * ((size_t)size_->Ny() + b) * ((size_t)size_->Nz() + b);
+}
+
 void communicator::write(tensor0 &T, const char *adrs) { write_component(*size_, T, adrs, false); }
 
 void communicator::write(tensor1 &T, const char *adrs) {

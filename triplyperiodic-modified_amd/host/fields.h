@@ -91,6 +91,12 @@ class communicator {
   void read(tensor1 &T, const char *adrs);
   void write(tensor0 &T, const char *adrs);
   void write(tensor1 &T, const char *adrs);
+  // the same files for fields kept on the host in the ghosted layout: one
+  // component of a file, and one array written (or appended, for the later
+  // components of a tensor1)
+  std::vector<double> read_host(const char *adrs, int comp = 0);
+  void write_host(const std::vector<double> &ghosted, const char *adrs, bool append = false);
+  size_t ghosted_count() const;
 };
 
 class poisson {

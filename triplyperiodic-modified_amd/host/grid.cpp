@@ -16,6 +16,12 @@ grid::grid(gridsize *size, params *param, proc *pc, communicator *com)
   RU_new.bind(&ctx_, size_, TPX_RU_NEW);
   RU_np1.bind(&ctx_, size_, TPX_RU_NP1);
   RHS_RU.bind(&ctx_, size_, TPX_RHS_RU);
+  RV.bind(&ctx_, size_, TPX_RV);
+  RV_int.bind(&ctx_, size_, TPX_RV_INT);
+  RV_new.bind(&ctx_, size_, TPX_RV_NEW);
+  RV_np1.bind(&ctx_, size_, TPX_RV_NP1);
+  Q.bind(&ctx_, size_, TPX_Q, 0);
+  RHS_Pois_Q.bind(&ctx_, size_, TPX_RHS_POIS_Q, 0);
   P.bind(&ctx_, size_, TPX_P, 0);
   RHS_Pois.bind(&ctx_, size_, TPX_RHS_POIS, 0);
   C.bind(&ctx_, size_, TPX_C, 0);
@@ -57,6 +63,9 @@ grid::grid(gridsize *size, params *param, proc *pc, communicator *com)
       open("Max_CFL_U", stat_GasMaxCFL);
       open("Num_Iteration", stat_NumIteration);
       open("Balance_Index", stat_BalanceIndex);
+      // particle.cpp:34-40: the trajectory stream lives in the stat directory
+      if (param_->Np_track() > 0)
+        stat_trajectory.open(param_->stat_dir() + "Part_trajectory.dat", std::ios::out | std::ios::app);
     }
   }
 }
@@ -128,35 +137,119 @@ void grid::Initialize() {
     P.upload(zero.data());
     part.load_random(g.Lx(), g.Ly(), g.Lz());
     ok(tpx_set_restart_numbers(ctx_, param_->P0(), 0.0, 0), "restart numbers");
+  } else if (init == 0) {
+    // velocity from U.bin, V.bin, W.bin times Rho0, uniform density, zero
+    // pressure, particles at rest from load_random (grid.cpp:240-253)
+    tensor0 *comp[3] = {&RU.x, &RU.y, &RU.z};
+    const char *file[3] = {"U.bin", "V.bin", "W.bin"};
+    for (int c = 0; c < 3; ++c) {
+      std::vector<double> f = com_->read_host(file[c]);
+      for (auto &v : f) v *= param_->Rho0();  // RU *= Rho0 (tensor1 operator*=, all elements)
+      comp[c]->upload(f.data());
+    }
+    std::vector<double> rho(Rho.ghosted_size(), param_->Rho0()), zero(P.ghosted_size(), 0.0);
+    Rho.upload(rho.data());
+    P.upload(zero.data());
+    part.load_random(g.Lx(), g.Ly(), g.Lz());
+    ok(tpx_set_restart_numbers(ctx_, param_->P0(), 0.0, 0), "restart numbers");
   } else {
-    unsupported("Initial = 0 (initial turbulence from U/V/W files)");
+    unsupported("this value of Initial");
   }
+  initial_vector();
   // the passive scalar's initial field (grid.cpp:385-457), read or set when
   // it is solved: Initial_C 0 and 1 read Passive_Scalar.bin and
   // Restart_Passive_Scalar.bin, 3-5 are products of two sines
+  // the passive scalar's initial field (grid.cpp:385-457), set whether or not
+  // it is solved (Store writes it as C_<n>.bin either way): Initial_C 0 and 1
+  // read Passive_Scalar.bin and Restart_Passive_Scalar.bin, 3-5 are products
+  // of two sines; 2 leaves it unset in the reference (zero here).  Unsolved,
+  // it stays on the host.
   int ic = param_->Initial_C();
-  if (param_->solve_for_scalar()) {
-    if (ic == 0) com_->read(Passive_Scalar, "Passive_Scalar.bin");
-    else if (ic == 1) com_->read(Passive_Scalar, "Restart_Passive_Scalar.bin");
-    else if (ic >= 3 && ic <= 5) {
-      std::vector<double> c(Passive_Scalar.ghosted_size(), 0.0);
-      int b = g.bs();
-      long gx = g.Nx() + 2 * b, gy = g.Ny() + 2 * b;
-      double kx = TWO_PI / g.Lx(), ky = TWO_PI / g.Ly(), kz = TWO_PI / g.Lz();
-      for (int k = 0; k < g.Nz(); ++k)
-        for (int j = 0; j < g.Ny(); ++j)
-          for (int i = 0; i < g.Nx(); ++i) {
-            double X = g.dx() / 2. + i * g.dx(), Y = g.dy() / 2. + j * g.dy(), Z = g.dz() / 2. + k * g.dz();
-            double v = ic == 5 ? std::sin(kx * X) * std::sin(ky * Y)
-                     : ic == 4 ? std::sin(kz * Z) * std::sin(kx * X) : std::sin(ky * Y) * std::sin(kz * Z);
-            c[((long)(k + b) * gy + (j + b)) * gx + (i + b)] = v;
-          }
-      Passive_Scalar.upload(c.data());
-    }
+  std::vector<double> c0(Passive_Scalar.ghosted_size(), 0.0);
+  if (ic == 0) c0 = com_->read_host("Passive_Scalar.bin");
+  else if (ic == 1) c0 = com_->read_host("Restart_Passive_Scalar.bin");
+  else if (ic >= 3 && ic <= 5) {
+    int b = g.bs();
+    long gx = g.Nx() + 2 * b, gy = g.Ny() + 2 * b;
+    double kx = TWO_PI / g.Lx(), ky = TWO_PI / g.Ly(), kz = TWO_PI / g.Lz();
+    for (int k = 0; k < g.Nz(); ++k)
+      for (int j = 0; j < g.Ny(); ++j)
+        for (int i = 0; i < g.Nx(); ++i) {
+          double X = g.dx() / 2. + i * g.dx(), Y = g.dy() / 2. + j * g.dy(), Z = g.dz() / 2. + k * g.dz();
+          double v = ic == 5 ? std::sin(kx * X) * std::sin(ky * Y)
+                   : ic == 4 ? std::sin(kz * Z) * std::sin(kx * X) : std::sin(ky * Y) * std::sin(kz * Z);
+          c0[((long)(k + b) * gy + (j + b)) * gx + (i + b)] = v;
+        }
   }
+  if (param_->solve_for_scalar()) Passive_Scalar.upload(c0.data());
+  else ps_host_ = c0;
   ok(tpx_initialize(ctx_), "Initialize");
   pull_scalars();
 }
+
+// the passive vector's initial field and Q (grid.cpp:256-383): Initial_V 0
+// reads V1/V2/V3.bin times Rho_forV, 1 reads Restart_RV.bin and
+// Restart_Q.bin, 2 is zero, 3-5 are the reference's analytic fields; Q is
+// zero except when read.  Unsolved, both stay on the host for Store.
+void grid::initial_vector() {
+  const gridsize &g = *size_;
+  const size_t m = com_->ghosted_count();
+  std::vector<double> rv[3] = {std::vector<double>(m, 0.0), std::vector<double>(m, 0.0), std::vector<double>(m, 0.0)};
+  std::vector<double> q(m, 0.0);
+  int iv = param_->Initial_V();
+  if (iv == 0) {
+    const char *file[3] = {"V1.bin", "V2.bin", "V3.bin"};
+    for (int c = 0; c < 3; ++c) {
+      rv[c] = com_->read_host(file[c]);
+      for (auto &v : rv[c]) v *= param_->Rho_forV();
+    }
+  } else if (iv == 1) {
+    for (int c = 0; c < 3; ++c) rv[c] = com_->read_host("Restart_RV.bin", c);
+    q = com_->read_host("Restart_Q.bin");
+  } else if (iv >= 3 && iv <= 5) {
+    int b = g.bs();
+    long gx = g.Nx() + 2 * b, gy = g.Ny() + 2 * b;
+    double kx = TWO_PI / g.Lx(), ky = TWO_PI / g.Ly(), kz = TWO_PI / g.Lz();
+    for (int k = 0; k < g.Nz(); ++k)
+      for (int j = 0; j < g.Ny(); ++j)
+        for (int i = 0; i < g.Nx(); ++i) {
+          long I = ((long)(k + b) * gy + (j + b)) * gx + (i + b);
+          double X = g.dx() / 2. + i * g.dx(), Y = g.dy() / 2. + j * g.dy(), Z = g.dz() / 2. + k * g.dz();
+          if (iv == 5) {
+            rv[0][I] = ky * std::sin(kx * (X - g.dx() / 2.)) * std::cos(ky * Y);
+            rv[1][I] = -kx * std::cos(kx * X) * std::sin(ky * (Y - g.dy() / 2.));
+          } else if (iv == 4) {
+            rv[0][I] = -kz * std::cos(kz * Z) * std::sin(kx * (X - g.dx() / 2.));
+            rv[2][I] = kx * std::sin(kz * (Z - g.dz() / 2.)) * std::cos(kx * X);
+          } else {
+            rv[1][I] = kz * std::sin(ky * (Y - g.dy() / 2.)) * std::cos(kz * Z);
+            rv[2][I] = -ky * std::cos(ky * Y) * std::sin(kz * (Z - g.dz() / 2.));
+          }
+        }
+  }
+  if (param_->solve_for_vector()) {
+    RV.x.upload(rv[0].data());
+    RV.y.upload(rv[1].data());
+    RV.z.upload(rv[2].data());
+    Q.upload(q.data());
+  } else {
+    for (int c = 0; c < 3; ++c) rv_host_[c] = rv[c];
+    q_host_ = q;
+  }
+}
+
+void grid::write_ghosted(const std::vector<double> &f, const std::string &name) {
+  com_->write_host(f, name.c_str(), false);
+}
+
+void grid::Update_RV_WOQ() { ok(tpx_update_rv_woq(ctx_, RK4_count), "Update_RV_WOQ"); }
+void grid::Compute_RHS_Pois_Q() { ok(tpx_compute_rhs_pois_q(ctx_, RK4_count), "Compute_RHS_Pois_Q"); }
+void grid::Solve_Poisson_Q() {
+  // PS_.Solve(1/Rho_forV, Q, RHS_Pois_Q) (grid.cpp:703-707), on the device
+  int it = 0;
+  ok(tpx_solve_poisson_q(ctx_, RK4_count, &it), "Solve_Poisson_Q");
+}
+void grid::Update_RV_WQ() { ok(tpx_update_rv_wq(ctx_, RK4_count), "Update_RV_WQ"); }
 
 void grid::Update_Rho() {
   // the driver calls PARAM.update(T_cur) before the substeps; the device
@@ -199,7 +292,32 @@ void grid::Statistics() {
   double st[TPX_ST_COUNT];
   ok(tpx_statistics(ctx_, st), "tpx_statistics");
   pull_scalars();
+  if (!param_->solve_for_scalar()) {
+    // Passive_Scalar.mean_squares() of the unsolved field (grid.cpp:913)
+    const gridsize &g = *size_;
+    const int b = g.bs();
+    const long gx = g.Nx() + 2 * b, gy = g.Ny() + 2 * b;
+    double sq = 0.0;
+    for (int i = 0; i < g.Nx(); ++i)
+      for (int j = 0; j < g.Ny(); ++j)
+        for (int k = 0; k < g.Nz(); ++k) {
+          const double v = ps_host_[((long)(k + b) * gy + (j + b)) * gx + (i + b)];
+          sq += v * v;
+        }
+    st[TPX_ST_PASSIVE_SCALAR_MEAN] = sq / (double)g.size_tot();
+  }
   if (!pc_->IsRoot()) return;
+  if (param_->Np_track() > 0) {
+    // part.trajectory(T_cur) (grid.cpp:939, particle.cpp:717-745): flags 0, K,
+    // 2K, ... below N0 with K = ceil(N0 / Np_track)
+    const int n0 = param_->N0(), K = (n0 + param_->Np_track() - 1) / param_->Np_track();
+    const int count = K > 0 ? (n0 + K - 1) / K : 0;
+    std::vector<double> xyz(3 * (size_t)count);
+    if (count > 0) ok(tpx_track_particles(ctx_, K, count, xyz.data()), "part.trajectory");
+    stat_trajectory << T_cur << " ";
+    for (double v : xyz) stat_trajectory << v << " ";
+    stat_trajectory << std::endl;
+  }
   auto line = [&](std::ofstream &f, double v) { f << T_cur << " " << v << std::endl; };
   line(stat_Tg, st[TPX_ST_TG_MEAN]);
   line(stat_Tp, st[TPX_ST_TP_MEAN]);
@@ -263,17 +381,73 @@ void grid::Store() {
     s << param_->data_dir() << stem << "_" << n << ext;
     return s.str();
   };
+  const gridsize &g = *size_;
+  const size_t m = com_->ghosted_count();
+  const int b = g.bs();
+  const long gx = g.Nx() + 2 * b, gy = g.Ny() + 2 * b;
+  auto interior = [&](auto &&fn) {
+    for (int k = 0; k < g.Nz(); ++k)
+      for (int j = 0; j < g.Ny(); ++j)
+        for (int i = 0; i < g.Nx(); ++i) fn(((long)(k + b) * gy + (j + b)) * gx + (i + b), i, j, k);
+  };
+  if (num_timestep == 1) {
+    // grid.cpp:491-520: the cell-centre coordinates XYZ.bin, and the sources
+    // S1.bin (the scalar's C_Source minus its mean when it is solved with
+    // S1_type 0) and S2.bin (V_Source minus its means for the vector with
+    // S2_type 0); zero where the reference never computes them
+    std::vector<double> xyz[3] = {std::vector<double>(m, 0.0), std::vector<double>(m, 0.0), std::vector<double>(m, 0.0)};
+    std::vector<double> s1(m, 0.0), s2y(m, 0.0), zero(m, 0.0);
+    interior([&](long I, int i, int j, int k) {
+      xyz[0][I] = g.dx() / 2. + i * g.dx();
+      xyz[1][I] = g.dy() / 2. + j * g.dy();
+      xyz[2][I] = g.dz() / 2. + k * g.dz();
+    });
+    const std::string xyzf = param_->data_dir() + "XYZ.bin";
+    com_->write_host(xyz[0], xyzf.c_str(), false);
+    com_->write_host(xyz[1], xyzf.c_str(), true);
+    com_->write_host(xyz[2], xyzf.c_str(), true);
+    auto cos_minus_mean = [&](std::vector<double> &f, bool centre_first) {
+      double sum = 0.0;
+      interior([&](long I, int i, int, int) {
+        const double X = centre_first ? g.dx() / 2. + i * g.dx() : i * g.dx() + g.dx() / 2.;
+        f[I] = std::cos(X);
+      });
+      for (int i = 0; i < g.Nx(); ++i)  // tensor0::sum's loop order (i, j, k)
+        for (int j = 0; j < g.Ny(); ++j)
+          for (int k = 0; k < g.Nz(); ++k) sum += f[((long)(k + b) * gy + (j + b)) * gx + (i + b)];
+      const double mean = sum / (double)g.size_tot();
+      interior([&](long I, int, int, int) { f[I] -= mean; });
+    };
+    if (param_->solve_for_scalar() && param_->S1_type() == 0) cos_minus_mean(s1, true);
+    if (param_->solve_for_vector() && param_->S2_type() == 0) cos_minus_mean(s2y, false);
+    write_ghosted(s1, param_->data_dir() + "S1.bin");
+    const std::string s2f = param_->data_dir() + "S2.bin";
+    com_->write_host(zero, s2f.c_str(), false);
+    com_->write_host(s2y, s2f.c_str(), true);
+    com_->write_host(zero, s2f.c_str(), true);
+  }
   if (num_timestep % param_->data_freq_fast() == 0 || Is_touch_) {
+    // grid.cpp:523-557: RU, RV and the passive scalar C every fast dump
     com_->write(RU, name("RU", num_timestep, ".bin").c_str());
-    // the passive scalar as C_n.bin (grid.cpp:538-544; RV_n.bin: no passive vector here)
+    const std::string rvf = name("RV", num_timestep, ".bin");
+    if (param_->solve_for_vector()) com_->write(RV, rvf.c_str());
+    else {
+      com_->write_host(rv_host_[0], rvf.c_str(), false);
+      com_->write_host(rv_host_[1], rvf.c_str(), true);
+      com_->write_host(rv_host_[2], rvf.c_str(), true);
+    }
     if (param_->solve_for_scalar()) com_->write(Passive_Scalar, name("C", num_timestep, ".bin").c_str());
+    else write_ghosted(ps_host_, name("C", num_timestep, ".bin"));
     std::ofstream numbers(name("numbers", num_timestep, ".dat"));
     numbers.precision(16);
     numbers << P0 << std::endl << T_cur << std::endl << num_timestep << std::endl;
   }
   if (num_timestep % param_->data_freq_slow() == 0 || Is_touch_) {
+    // grid.cpp:560-592: particles, P, Q and Rho every slow dump
     part.Store_All(num_timestep);
     com_->write(P, name("P", num_timestep, ".bin").c_str());
+    if (param_->solve_for_vector()) com_->write(Q, name("Q", num_timestep, ".bin").c_str());
+    else write_ghosted(q_host_, name("Q", num_timestep, ".bin"));
     com_->write(Rho, name("Rho", num_timestep, ".bin").c_str());
   }
 }

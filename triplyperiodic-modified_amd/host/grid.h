@@ -24,15 +24,22 @@ class grid {
   std::ofstream stat_Tg, stat_Tp, stat_HT, stat_TKE2, stat_TKE_U, stat_TKE_V, stat_TKE_W,
       stat_Passive_Scalar_mean, stat_TKEV_V1, stat_TKEV_V2, stat_TKEV_V3, stat_P0, stat_CMax, stat_CMin,
       stat_CMean, stat_RhoMax, stat_RhoMin, stat_RhoMean, stat_ParticleMaxCFL, stat_GasMaxCFL,
-      stat_GasMaxDiffCFL, stat_NumIteration, stat_BalanceIndex;
+      stat_GasMaxDiffCFL, stat_NumIteration, stat_BalanceIndex, stat_trajectory;
+  // fields that only Store writes when they are not advanced on the device:
+  // the passive scalar's and vector's initial values (grid.cpp:256-458) and Q
+  std::vector<double> ps_host_, rv_host_[3], q_host_;
   void pull_scalars();
+  void initial_vector();
+  void write_ghosted(const std::vector<double> &f, const std::string &name);
   void ok(int rc, const char *what) { tpx_check(ctx_, rc, what); }
   [[noreturn]] void unsupported(const char *what);
 
  public:
   particle part;
   tensor1 RU, RU_int, RU_new, RU_np1, RHS_RU;
+  tensor1 RV, RV_int, RV_new, RV_np1;
   tensor0 P, RHS_Pois, C, Rho, Rho_int, Rho_new, Rho_np1, divergence, RHS_Part_Temp;
+  tensor0 Q, RHS_Pois_Q;
   tensor0 Passive_Scalar, Passive_Scalar_int, Passive_Scalar_new, Passive_Scalar_np1;
   double P0 = 0, P0_int = 0, P0_new = 0, P0_np1 = 0, dP0_dt = 0;
   double T_cur = 0;
@@ -45,7 +52,7 @@ class grid {
   grid &operator=(const grid &) = delete;
   tpx_ctx *context() { return ctx_; }
 
-  void Initialize();          // grid.cpp:117-486 (Initial = 1..5)
+  void Initialize();          // grid.cpp:117-486 (Initial = 0..5)
   void Store();               // grid.cpp:489-593
   void TimeAdvance();         // grid.cpp:595-606
   void Update_Rho();          // grid.cpp:608-614
@@ -58,14 +65,14 @@ class grid {
   void Update_Particle();     // grid.cpp:865-888
   void Update_Passive_Scalar();  // grid.cpp:711-745
   void TimeAdvance_RK4();     // grid.cpp:890-900
-  void Statistics();          // grid.cpp:902-982 (P0, iterations, density)
+  void Update_RV_WOQ();       // grid.cpp:646-692
+  void Compute_RHS_Pois_Q();  // grid.cpp:694-701
+  void Solve_Poisson_Q();     // grid.cpp:703-707
+  void Update_RV_WQ();        // grid.cpp:709-715
+  void Statistics();          // grid.cpp:902-982 (with part.trajectory)
   bool Touch();               // grid.cpp:94-108
   int num_iteration() const { return PS_.num_iteration(); }
 
-  // passive vector, elongated-box copy: outside this build
-  void Update_RV_WOQ() { unsupported("Update_RV_WOQ"); }
-  void Compute_RHS_Pois_Q() { unsupported("Compute_RHS_Pois_Q"); }
-  void Solve_Poisson_Q() { unsupported("Solve_Poisson_Q"); }
-  void Update_RV_WQ() { unsupported("Update_RV_WQ"); }
+  // elongated-box copy: outside this build
   void CopyBox() { unsupported("CopyBox"); }
 };

@@ -93,6 +93,8 @@ tpx_params params::to_abi() const {
   p.cp = Cp_; p.cv = Cv_; p.np0 = np0_; p.dp = Dp_; p.rhop = Rhop_; p.cvp = Cvp_;
   p.epsilonp = epsilonp_; p.nu = Nu_; p.i01 = I01_; p.i02 = I02_;
   p.solve_for_scalar = solve_for_scalar_ ? 1 : 0; p.s1_type = s1t_ != 0 ? 1 : 0; p.d_m = D_M_;
+  p.solve_for_vector = solve_for_vector_ ? 1 : 0; p.s2_type = s2t_ != 0 ? 1 : 0;
+  p.eta = eta_; p.rho_forv = Rho_forV_;
   return p;
 }
 

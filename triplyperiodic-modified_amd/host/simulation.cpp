@@ -31,7 +31,8 @@ static int print_params(const char *file) {
             << a.gz2 << "\ncp " << a.cp << "\ncv " << a.cv << "\nnp0 " << a.np0 << "\ndp " << a.dp << "\nrhop "
             << a.rhop << "\ncvp " << a.cvp << "\nepsilonp " << a.epsilonp << "\nnu " << a.nu << "\ni01 "
             << a.i01 << "\ni02 " << a.i02 << "\nsolve_for_scalar " << a.solve_for_scalar << "\ns1_type "
-            << a.s1_type << "\nd_m " << a.d_m << "\nT_final " << P.T_final() << "\nN0 " << P.N0() << "\nTp "
+            << a.s1_type << "\nd_m " << a.d_m << "\nsolve_for_vector " << a.solve_for_vector
+            << "\ns2_type " << a.s2_type << "\neta " << a.eta << "\nrho_forv " << a.rho_forv << "\nT_final " << P.T_final() << "\nN0 " << P.N0() << "\nTp "
             << P.Tp() << "\nmp " << P.mp() << "\nP0 " << P.P0() << "\nInitial " << P.Initial() << std::endl;
   return 0;
 }

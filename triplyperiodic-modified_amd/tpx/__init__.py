@@ -20,10 +20,12 @@ LIB_PATH = os.path.join(_PKG_DIR, "libtpx.so")
 TPX_OK, TPX_E_ARG, TPX_E_HIP, TPX_E_STATE, TPX_E_UNSUPP, TPX_E_OOM = 0, -1, -2, -3, -4, -5
 
 FIELD = {"RU": 0, "RU_int": 1, "RU_new": 2, "RU_np1": 3, "RHS_RU": 4,
+         "RV": 5, "RV_int": 6, "RV_new": 7, "RV_np1": 8,
          "P": 16, "RHS_Pois": 17, "C": 18, "Rho": 19, "Rho_int": 20, "Rho_new": 21,
          "Rho_np1": 22, "divergence": 23, "RHS_Part_Temp": 24,
-         "Passive_Scalar": 25, "Passive_Scalar_int": 26, "Passive_Scalar_new": 27, "Passive_Scalar_np1": 28}
-VECTOR = {"RU", "RU_int", "RU_new", "RU_np1", "RHS_RU"}
+         "Passive_Scalar": 25, "Passive_Scalar_int": 26, "Passive_Scalar_new": 27, "Passive_Scalar_np1": 28,
+         "Q": 29, "RHS_Pois_Q": 30}
+VECTOR = {"RU", "RU_int", "RU_new", "RU_np1", "RHS_RU", "RV", "RV_int", "RV_new", "RV_np1"}
 STAGE = {"base": 0, "int": 1, "new": 2, "np1": 3}
 SCALARS = ["P0", "P0_int", "P0_new", "P0_np1", "T_cur", "num_timestep",
            "num_iteration", "mean_energy_transferred", "A", "I0", "Iteration",
@@ -42,10 +44,13 @@ EXPORTS = [
     "tpx_upload_field", "tpx_download_field", "tpx_upload_particles", "tpx_num_particles",
     "tpx_download_particles", "tpx_download_particle_rhs_t", "tpx_set_restart_numbers",
     "tpx_get_scalars", "tpx_initialize", "tpx_params_update", "tpx_update_rho",
-    "tpx_update_p0", "tpx_update_particle", "tpx_update_passive_scalar", "tpx_update_ru_wop", "tpx_compute_div_u_new",
+    "tpx_update_p0", "tpx_update_particle", "tpx_update_passive_scalar",
+    "tpx_update_rv_woq", "tpx_compute_rhs_pois_q", "tpx_solve_poisson_q", "tpx_update_rv_wq",
+    "tpx_update_ru_wop", "tpx_compute_div_u_new",
     "tpx_compute_rhs_pois", "tpx_solve_poisson", "tpx_poisson_solve", "tpx_poisson_solve_device", "tpx_update_ru_wp", "tpx_time_advance_rk4",
     "tpx_make_mean_u0", "tpx_time_advance", "tpx_step", "tpx_ensemble_step",
     "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_gradient_heat", "tpx_adjoint_set_gas_tape", "tpx_statistics_enable", "tpx_statistics",
+    "tpx_track_particles",
     "tpx_tune_tiles", "tpx_get_tiles", "tpx_set_tiles",
     "tpx_adjoint_release",
     "tpx_batch_create", "tpx_batch_step", "tpx_batch_destroy", "tpx_batch_error", "tpx_batch_last_error",
@@ -95,6 +100,9 @@ def load(path: str = LIB_PATH):
     for fn in ("tpx_initialize", "tpx_params_update", "tpx_time_advance_rk4", "tpx_time_advance",
                "tpx_deposit_concentration", "tpx_synchronize"):
         getattr(L, fn).argtypes = [P]
+    for fn in ("tpx_update_rv_woq", "tpx_compute_rhs_pois_q", "tpx_update_rv_wq"):
+        getattr(L, fn).argtypes = [P, I]
+    L.tpx_solve_poisson_q.argtypes = [P, I, ctypes.POINTER(I)]
     for fn in ("tpx_update_rho", "tpx_update_p0", "tpx_update_particle", "tpx_update_passive_scalar", "tpx_update_ru_wop",
                "tpx_compute_div_u_new", "tpx_compute_rhs_pois", "tpx_update_ru_wp", "tpx_step"):
         getattr(L, fn).argtypes = [P, I]
@@ -111,6 +119,7 @@ def load(path: str = LIB_PATH):
     L.tpx_get_tiles.argtypes = [P, IP]
     L.tpx_set_tiles.argtypes = [P, IP]
     L.tpx_statistics.argtypes = [P, DP]
+    L.tpx_track_particles.argtypes = [P, I, I, DP]
     L.tpx_adjoint_release.argtypes = [P]
     L.tpx_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.tpx_batch_step.argtypes = [P, I]
@@ -241,6 +250,15 @@ class Solver:
     def update_particle(self, rk): self._ok(self.L.tpx_update_particle(self.h, rk), "update_particle")
     def update_passive_scalar(self, rk):
         self._ok(self.L.tpx_update_passive_scalar(self.h, rk), "update_passive_scalar")
+    def update_rv_woq(self, rk): self._ok(self.L.tpx_update_rv_woq(self.h, rk), "update_rv_woq")
+    def compute_rhs_pois_q(self, rk): self._ok(self.L.tpx_compute_rhs_pois_q(self.h, rk), "compute_rhs_pois_q")
+
+    def solve_poisson_q(self, rk) -> int:
+        it = ctypes.c_int(0)
+        self._ok(self.L.tpx_solve_poisson_q(self.h, rk, ctypes.byref(it)), "solve_poisson_q")
+        return it.value
+
+    def update_rv_wq(self, rk): self._ok(self.L.tpx_update_rv_wq(self.h, rk), "update_rv_wq")
     def update_ru_wop(self, rk): self._ok(self.L.tpx_update_ru_wop(self.h, rk), "update_ru_wop")
     def compute_div_u_new(self, rk): self._ok(self.L.tpx_compute_div_u_new(self.h, rk), "compute_div_u_new")
     def compute_rhs_pois(self, rk): self._ok(self.L.tpx_compute_rhs_pois(self.h, rk), "compute_rhs_pois")
@@ -372,6 +390,14 @@ class Solver:
         self._ok(self.L.tpx_statistics(self.h, _dp(out)), "statistics")
         return dict(zip(STATS, out.tolist()))
 
+    def track_particles(self, stride: int, count: int) -> np.ndarray:
+        """particle::trajectory's positions: row m is (x, y, z) of the particle
+        with flag m*stride (NaN if none)."""
+        out = np.empty((count, 3))
+        if count:
+            self._ok(self.L.tpx_track_particles(self.h, int(stride), int(count), _dp(out)), "track_particles")
+        return out
+
     def deposit_concentration(self) -> None:
         self._ok(self.L.tpx_deposit_concentration(self.h), "deposit_concentration")
 
@@ -415,10 +441,14 @@ class Solver:
                            tau=case.tau, eps=case.eps)
         if getattr(case, "PS", None) is not None:
             self.set("Passive_Scalar", case.PS)
+        if getattr(case, "RV", None) is not None:
+            self.set("RV", case.RV)
+            self.set("Q", getattr(case, "Q", None) if getattr(case, "Q", None) is not None else np.zeros(self.shape))
         self.set_restart_numbers(case.params.P0, 0.0, 0)
         self.initialize()
 
-    SUBSTEP_METHODS = ("update_rho", "update_p0", "update_particle", "update_passive_scalar", "update_ru_wop",
+    SUBSTEP_METHODS = ("update_rho", "update_p0", "update_particle", "update_passive_scalar",
+                       "update_rv_woq", "compute_rhs_pois_q", "solve_poisson_q", "update_rv_wq", "update_ru_wop",
                        "compute_div_u_new", "compute_rhs_pois", "solve_poisson",
                        "update_ru_wp", "time_advance_rk4")
 
@@ -430,6 +460,9 @@ class Solver:
             for m in self.SUBSTEP_METHODS:
                 if m == "update_passive_scalar" and not p.solve_for_scalar:
                     continue  # simulation.cpp:65
+                if m in ("update_rv_woq", "compute_rhs_pois_q", "solve_poisson_q", "update_rv_wq") and \
+                        not p.solve_for_vector:
+                    continue  # simulation.cpp:66-71
                 fn = getattr(self, m)
                 if m == "time_advance_rk4":
                     fn()

@@ -218,6 +218,8 @@ class Params:
             cp=self.Cp, cv=self.Cv, np0=self.np0, dp=self.Dp, rhop=self.Rhop,
             cvp=self.Cvp, epsilonp=self.epsilonp, nu=self.Nu, i01=self.I01, i02=self.I02,
             solve_for_scalar=int(self.solve_for_scalar), s1_type=int(self.s1t != 0), d_m=self.D_M,
+            solve_for_vector=int(self.solve_for_vector), s2_type=int(self.s2t != 0), eta=self.eta,
+            rho_forv=self.Rho_forV,
         )
 
 
@@ -234,6 +236,7 @@ STRUCT_FIELDS: List[Tuple[str, str]] = [
     ("cp", "d"), ("cv", "d"), ("np0", "d"), ("dp", "d"), ("rhop", "d"),
     ("cvp", "d"), ("epsilonp", "d"), ("nu", "d"), ("i01", "d"), ("i02", "d"),
     ("solve_for_scalar", "i"), ("s1_type", "i"), ("d_m", "d"),
+    ("solve_for_vector", "i"), ("s2_type", "i"), ("eta", "d"), ("rho_forv", "d"),
 ]
 
 
