@@ -166,12 +166,13 @@ def copy_probe(mib):
                       "copy_frac_of_8TBps": round(out[0] / HBM_PEAK_GBS, 4)}))
 
 
-def ensemble_bench(args):
+def ensemble_measure(steps, warmup, with_streams=True):
     """Ensemble throughput: 64 independent 64^3 boxes (nu ~ U[0.01, 0.03],
     St ~ logU[0.1, 10], particles ~ logU[1e4, 2e5] per box, mass loading 0.3)
     stepped by batched kernels (one launch per kernel for all boxes), against
-    the same boxes stepped one after another; the concurrent-streams form
-    (four boxes in flight, each on its own stream) is reported beside it."""
+    the same boxes stepped one after another (each box's own tpx_step); the
+    concurrent-streams form (four boxes in flight, each on its own stream) is
+    reported beside it.  Returns the figures as a dict."""
     import tpx
     from tpx import ic
     t0 = time.perf_counter()
@@ -193,43 +194,59 @@ def ensemble_bench(args):
     log(f"[bench] tile shapes {tiles} tuned on box {ref}")
     batch = tpx.Batch(solvers)
     t0 = time.perf_counter()
-    batch.step(args.warmup)
-    log(f"[bench] warmup {args.warmup} batched steps in {time.perf_counter() - t0:.2f} s")
+    batch.step(warmup)
+    log(f"[bench] warmup {warmup} batched steps in {time.perf_counter() - t0:.2f} s")
     for s in solvers:
         s.synchronize()
     t0 = time.perf_counter()
     for b, s in enumerate(solvers):
-        s.step(args.steps)
-        if b % 16 == 15:
-            log(f"[bench] sequential: {b + 1} boxes in {time.perf_counter() - t0:.2f} s")
+        s.step(steps)
+        s.synchronize()
     t_seq = time.perf_counter() - t0
     t0 = time.perf_counter()
-    batch.step(args.steps)
+    batch.step(steps)
+    for s in solvers:
+        s.synchronize()
     t_bat = time.perf_counter() - t0
-    log(f"[bench] batched {t_bat:.3f} s")
-    t0 = time.perf_counter()
-    tpx.ensemble_step(solvers, args.steps, 4)
-    t_str = time.perf_counter() - t0
+    t_str = None
+    if with_streams:
+        t0 = time.perf_counter()
+        tpx.ensemble_step(solvers, steps, 4)
+        t_str = time.perf_counter() - t0
     batch.close()
-    value = ncells * args.steps / t_bat
-    seq = ncells * args.steps / t_seq
-    log(f"[bench] sequential {t_seq:.3f} s, batched {t_bat:.3f} s, four streams {t_str:.3f} s")
+    for s in solvers:
+        s.close()
+    value = ncells * steps / t_bat
+    seq = ncells * steps / t_seq
+    log(f"[bench] ensemble: sequential {t_seq:.3f} s, batched {t_bat:.3f} s"
+        + (f", four streams {t_str:.3f} s" if t_str else "") + f" -> {value:.4g} cells/s ({value / seq:.2f}x)")
+    out = {"metric": "ensemble cells/s (64 boxes of 64^3)", "value": round(value, 1), "unit": "cells/s",
+           "steps": steps, "warmup": warmup, "ms_per_step": round(1000.0 * t_bat / steps, 3),
+           "boxes": len(cases), "particles": npart,
+           "workload": "ensemble: 64 x 64^3 boxes, particles ~ logU[1e4, 2e5], nu ~ U[0.01, 0.03], "
+                       "St ~ logU[0.1, 10], mass loading 0.3, heat off; batched kernels, deposit tile shapes "
+                       "from the online tuner; sequential = the same boxes stepped one after another",
+           "deposit_tiles": {"heat": list(tiles["heat"]), "velocity": list(tiles["velocity"])},
+           "sequential_cells_per_s": round(seq, 1), "speedup_vs_sequential": round(value / seq, 3)}
+    if t_str:
+        out["four_streams_cells_per_s"] = round(ncells * steps / t_str, 1)
+    return out
+
+
+def ensemble_bench(args):
+    e = ensemble_measure(args.steps, args.warmup)
     print(json.dumps({
-        "metric": "ensemble cells/s (64 boxes of 64^3)", "value": round(value, 1), "unit": "cells/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * t_bat / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "metric": e["metric"], "value": e["value"], "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": e["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "ensemble: 64 x 64^3 boxes, particles ~ logU[1e4, 2e5], nu ~ U[0.01, 0.03], "
-                               "St ~ logU[0.1, 10], mass loading 0.3, heat off; batched kernels, "
-                               "deposit tile shapes from the online tuner",
-                   "boxes": len(cases), "particles": npart,
-                   "deposit_tiles": {"heat": list(tiles["heat"]), "velocity": list(tiles["velocity"])}},
-        "sequential_cells_per_s": round(seq, 1), "speedup_vs_sequential": round(value / seq, 3),
-        "four_streams_cells_per_s": round(ncells * args.steps / t_str, 1),
+        "config": {"workload": e["workload"], "boxes": e["boxes"], "particles": e["particles"],
+                   "deposit_tiles": e["deposit_tiles"]},
+        "sequential_cells_per_s": e["sequential_cells_per_s"], "speedup_vs_sequential": e["speedup_vs_sequential"],
+        "four_streams_cells_per_s": e.get("four_streams_cells_per_s"),
         "roofline": None, "cpu_baseline": None}), flush=True)
 
 
-def adjoint_bench(args):
+def adjoint_measure():
     """Adjoint cost: 128^3, 2e5 particles, St = 2, two-way coupling at mass
     loading 0.5 and the heat exchange on, 50 steps with a particle checkpoint
     every 10.  The whole reverse sweep (per step, backwards: the particles'
@@ -237,9 +254,8 @@ def adjoint_bench(args):
     reversed through the projection, the Poisson iterations and the
     divergence, the heat exchange's gather and deposit, the RK bookkeeping,
     the particles' transposed stage, the momentum and density updates and
-    Update_P0) is timed against the plain forward steps.  With two-way
-    coupling every gradient needs the gas sweep, so the tau-only call costs
-    the same sweep."""
+    Update_P0) is timed against the plain forward steps, with the gas tape
+    kept per substep and with the gas recomputed per step.  Returns a dict."""
     import tpx
     from tpx import ic
     nsteps, every = 50, 10
@@ -290,21 +306,28 @@ def adjoint_bench(args):
     agree = float(np.max(np.abs(full_r["RU"] - full["RU"])) / np.max(np.abs(full["RU"])))
     log(f"[bench] recompute tape: recorded forward {t_rec_r:.3f} s, full sweep {t_rev_r:.3f} s, "
         f"dJ/dRU agrees to {agree:.1e}")
+    return {"metric": "adjoint/forward cost ratio (128^3, 2e5 particles, 50 steps)", "value": round(t_rev / t_fwd, 4),
+            "unit": "ratio", "steps": nsteps, "higher_is_better": False,
+            "workload": "adjoint: 128^3, 2e5 particles, St=2, two-way coupling at mass loading 0.5, heat exchange "
+                        "on (per-particle eps in [0.5, 1]), J = Var(C) of the final positions, gradients w.r.t. "
+                        "every particle's tau, eps and initial state, the forcing coefficient A and the initial "
+                        "RU, Rho, P, P0; particle checkpoint every 10 steps, gas tape kept per substep (it fits "
+                        "in HBM; recompute_*: the gas recomputed per step instead)",
+            "forward_s": round(t_fwd, 4), "recorded_forward_s": round(t_rec, 4), "reverse_s": round(t_rev, 4),
+            "recompute_reverse_s": round(t_rev_r, 4), "recompute_reverse_over_forward": round(t_rev_r / t_fwd, 4),
+            "recompute_recorded_forward_s": round(t_rec_r, 4),
+            "particle_reverse_s": round(t_rev_p, 4), "particle_reverse_over_forward": round(t_rev_p / t_fwd, 4),
+            "recorded_over_plain": round(t_rec / t_fwd, 4), "recompute_agrees_to": agree}
+
+
+def adjoint_bench(args):
+    a = adjoint_measure()
     print(json.dumps({
-        "metric": "adjoint/forward cost ratio (128^3, 2e5 particles, 50 steps)", "value": round(t_rev / t_fwd, 4),
-        "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": 1, "higher_is_better": False,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "adjoint: 128^3, 2e5 particles, St=2, two-way coupling at mass loading 0.5, "
-                               "heat exchange on (per-particle eps in [0.5, 1]), "
-                               "J = Var(C) of the final positions, gradients w.r.t. every particle's tau, eps and "
-                               "initial state (with T), the forcing coefficient A and the initial RU, Rho, P, P0; "
-                               "particle checkpoint every 10 steps, gas tape kept per substep (it fits in "
-                               "HBM; recompute_* fields: the gas recomputed per step instead)"},
-        "forward_s": round(t_fwd, 4), "recorded_forward_s": round(t_rec, 4), "reverse_s": round(t_rev, 4),
-        "recompute_reverse_s": round(t_rev_r, 4), "recompute_reverse_over_forward": round(t_rev_r / t_fwd, 4),
-        "recompute_recorded_forward_s": round(t_rec_r, 4),
-        "particle_reverse_s": round(t_rev_p, 4), "particle_reverse_over_forward": round(t_rev_p / t_fwd, 4),
-        "recorded_over_plain": round(t_rec / t_fwd, 4), "roofline": None, "cpu_baseline": None}), flush=True)
+        "metric": a["metric"], "value": a["value"], "unit": "ratio", "n_gpus": 1, "steps": a["steps"], "warmup": 1,
+        "higher_is_better": False, "dtype": "f64", "data": "synthetic", "config": {"workload": a["workload"]},
+        **{k: v for k, v in a.items() if k not in ("metric", "value", "unit", "steps", "higher_is_better",
+                                                   "workload")},
+        "roofline": None, "cpu_baseline": None}), flush=True)
 
 
 def main():
@@ -314,6 +337,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="heat256", choices=["heat256", "plain128", "plain32", "ensemble", "adjoint"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the compact ensemble and adjoint figures added to the heat256 line")
     ap.add_argument("--cpu-full", action="store_true",
                     help="CPU baseline over 3 timed steps on all threads and a whole step on 1 thread")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -432,6 +457,17 @@ def main():
 
     # whole-step algorithmic bandwidth with the iteration count of the run
     whole = step_bytes(ncell, npart, mean_its) * value / 1e9
+    s.close()
+    del s
+
+    extras = {}
+    if not args.no_extras:
+        # the other two headline figures, measured in the same run (compact:
+        # a few batched ensemble steps; the adjoint configuration in full)
+        log("[bench] ensemble (64 x 64^3) ...")
+        extras["ensemble"] = ensemble_measure(steps=5, warmup=2, with_streams=False)
+        log("[bench] adjoint (128^3, 2e5 particles, 50 steps) ...")
+        extras["adjoint"] = adjoint_measure()
 
     cpu = None
     if not args.no_cpu:
@@ -461,6 +497,7 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = round(value / cpu["value"], 1)
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 

@@ -326,6 +326,24 @@ def test_three_and_five_pass_iterations_agree(gpu_available, monkeypatch):
     assert max(e.values()) <= 1e-12, e
 
 
+@pytest.mark.parametrize("shape", [(16, 32, 128), (32, 16, 256)])
+def test_packed_and_complex_x_transforms_agree(gpu_available, monkeypatch, shape):
+    """At x lengths 128 and 256 the first Poisson pass transforms each real
+    x-row as a half-length complex one (the default) or as a full-length
+    complex one (TPX_XPACK=0): the same half spectrum, so both take the same
+    iterations and give the same fields to round-off."""
+    case = _noncubic_case(shape, seed=43)
+    sp = solver_from_case(case)
+    monkeypatch.setenv("TPX_XPACK", "0")
+    sc = solver_from_case(case)
+    sp.step(3)
+    sc.step(3)
+    assert sp.scalars()["num_iteration"] == sc.scalars()["num_iteration"]
+    e = {q: rel_err(sp.get(q), sc.get(q)) for q in ("RU", "Rho", "P")}
+    print(shape, {k: f"{v:.2e}" for k, v in e.items()}, "iterations", sp.scalars()["num_iteration"])
+    assert max(e.values()) <= 1e-12, e
+
+
 def test_long_run_diagnostics_within_2_percent(gpu_available):
     """Beyond pointwise parity: after 200 steps of the 32^3 configuration the
     energy spectrum (every shell holding at least 1e-4 of the energy), the

@@ -479,6 +479,89 @@ struct RegDft<1, INV> {
   static __device__ __forceinline__ void run(double2 *) {}
 };
 
+// W_16^m for any m (forward; INV conjugates)
+template <bool INV>
+__device__ __forceinline__ double2 w16m(int m) {
+  m &= 15;
+  double2 w = m < 8 ? w16(m) : make_double2(-w16(m - 8).x, -w16(m - 8).y);
+  if (INV) w.y = -w.y;
+  return w;
+}
+
+// a times -i (forward) or +i (INV)
+template <bool INV>
+__device__ __forceinline__ double2 rot_i(double2 a) {
+  return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// radix-4 butterfly in place on a[0..3] (natural order in and out)
+template <bool INV>
+__device__ __forceinline__ void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3) {
+  const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = rot_i<INV>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <bool INV>
+struct RegDft<2, INV> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <bool INV>
+struct RegDft<4, INV> {
+  static __device__ __forceinline__ void run(double2 *v) { dft4<INV>(v[0], v[1], v[2], v[3]); }
+};
+
+// 8 = 2 x 4: a radix-4 transform over n2 for each n1 (x[n1 + 2 n2]), the
+// twiddles W_8^{n1 k1}, then radix-2 over n1: X[k1 + 4 k2]
+template <bool INV>
+struct RegDft<8, INV> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    double2 y[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) y[q] = v[q];
+    dft4<INV>(y[0], y[2], y[4], y[6]);  // n1 = 0: k1 at y[2 k1]
+    dft4<INV>(y[1], y[3], y[5], y[7]);  // n1 = 1: k1 at y[1 + 2 k1]
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) y[1 + 2 * k1] = cmul(y[1 + 2 * k1], w16m<INV>(2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      v[k1] = cadd(y[2 * k1], y[1 + 2 * k1]);
+      v[k1 + 4] = csub(y[2 * k1], y[1 + 2 * k1]);
+    }
+  }
+};
+
+// 16 = 4 x 4: radix-4 over n2 for each n1 (x[n1 + 4 n2]) in place, the
+// twiddles W_16^{n1 k1}, radix-4 over n1 for each k1: X[k1 + 4 k2] is then
+// in slot 4 k1 + k2, renamed into natural order
+template <bool INV>
+struct RegDft<16, INV> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    double2 y[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) y[q] = v[q];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) dft4<INV>(y[n1], y[n1 + 4], y[n1 + 8], y[n1 + 12]);  // k1 at y[n1 + 4 k1]
+#pragma unroll
+    for (int n1 = 1; n1 < 4; ++n1)
+#pragma unroll
+      for (int k1 = 1; k1 < 4; ++k1) y[n1 + 4 * k1] = cmul(y[n1 + 4 * k1], w16m<INV>(n1 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft4<INV>(y[4 * k1], y[4 * k1 + 1], y[4 * k1 + 2], y[4 * k1 + 3]);  // k2 at + k2
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = y[4 * k1 + k2];
+  }
+};
+
 // step A on values v[m] = x[t + R1 m]: transform over m, then W_n^{t k2}
 template <int R2, bool INV>
 __device__ __forceinline__ void step_a(double2 (&v)[R2], int t, const double2 *__restrict__ tw) {
@@ -948,6 +1031,146 @@ struct k_yz_pass {
 }
 };
 
+// ---------------------------------------------------------------------------
+// Passes 1 and 3 with the x transform of the real residual done as a complex
+// transform of half the length (x lengths 2 M = 128, 256; 16 lanes a row):
+// lane ta of a row holds the pairs z[j] = x[2j] + i x[2j+1], j = ta + 16 m
+// (m < R2H, M = 16 R2H), loaded with 16-byte accesses.  The M-point four-step
+// transform Z of z gives the half spectrum of x by
+//   X[k] = E[k] + W_2M^k O[k],  E = (Z[k] + conj Z[M-k]) / 2,
+//                               O = (Z[k] - conj Z[M-k]) / 2i   (k <= M),
+// and the inverse forms Z[k] = (X[k] + conj X[M-k]) + i W_2M^-k (X[k] - conj X[M-k])
+// before an M-point inverse transform whose real and imaginary parts are the
+// even and odd points.  Half the butterflies and half the load instructions
+// of the full-length complex transform of a real row.
+// ---------------------------------------------------------------------------
+
+// the residuals of one x-row in the pair layout: v[m] = (r(x0), r(x0 + 1)),
+// x0 = 2 ta + 32 m.  The x neighbours outside the pair come from the lanes
+// beside it by DPP row rotation, as in row_residuals; the same arithmetic.
+template <int R2H>
+__device__ __forceinline__ void row_residuals_pairs(const XArgs &a, const Ix &ix, int ta, int j, int z, double mean,
+                                                    double2 (&v)[R2H]) {
+  const double *__restrict__ rho = a.rho;
+  const double *__restrict__ P = a.P;
+  const double *__restrict__ rhs = a.rhs;
+  const long rc = (long)(z & ix.mz) * ix.nxy + (long)(j & ix.my) * ix.nx;
+  const long ryl = (long)(z & ix.mz) * ix.nxy + (long)((j - 1) & ix.my) * ix.nx;
+  const long ryh = (long)(z & ix.mz) * ix.nxy + (long)((j + 1) & ix.my) * ix.nx;
+  const long rzl = (long)((z - 1) & ix.mz) * ix.nxy + (long)(j & ix.my) * ix.nx;
+  const long rzh = (long)((z + 1) & ix.mz) * ix.nxy + (long)(j & ix.my) * ix.nx;
+  auto ld2 = [&](const double *f, long row, int m) {
+    return *reinterpret_cast<const double2 *>(f + row + 2 * ta + 32 * m);
+  };
+  // x0 - 1: the odd point of the lane before (lane 15 of the pair before, at ta 0)
+  auto lo_of = [&](double cur_odd, double prev_odd) {
+    const double u = row_rotate<0x121>(cur_odd), w = row_rotate<0x121>(prev_odd);
+    return ta == 0 ? w : u;
+  };
+  // x0 + 2: the even point of the lane after (lane 0 of the pair after, at ta 15)
+  auto hi_of = [&](double cur_even, double next_even) {
+    const double u = row_rotate<0x12F>(cur_even), w = row_rotate<0x12F>(next_even);
+    return ta == 15 ? w : u;
+  };
+  double2 pp = ld2(P, rc, R2H - 1), rp = ld2(rho, rc, R2H - 1);  // the pair before m = 0
+  double2 pc = ld2(P, rc, 0), rcc = ld2(rho, rc, 0);
+  double cxa = rcp_nr((rcc.x + lo_of(rcc.y, rp.y)) * 0.5);  // C on the face below x0
+#pragma unroll
+  for (int m = 0; m < R2H; ++m) {
+    const int mn = (m + 1) % R2H;
+    const double2 pn = ld2(P, rc, mn), rn = ld2(rho, rc, mn);
+    const double2 pym = ld2(P, ryl, m), pyp = ld2(P, ryh, m), pzm = ld2(P, rzl, m), pzp = ld2(P, rzh, m);
+    const double2 qym = ld2(rho, ryl, m), qyp = ld2(rho, ryh, m), qzm = ld2(rho, rzl, m), qzp = ld2(rho, rzh, m);
+    const double2 r0 = ld2(rhs, rc, m);
+    const double cxb = rcp_nr((rcc.y + rcc.x) * 0.5);                    // the face between x0 and x0 + 1
+    const double cxa_n = rcp_nr((rn.x + lo_of(rn.y, rcc.y)) * 0.5);      // below x0 of the next pair
+    const double cxc = hi_of(cxa, cxa_n);                                // above x0 + 1
+    const double cy0a = rcp_nr((rcc.x + qym.x) * 0.5), cy1a = rcp_nr((qyp.x + rcc.x) * 0.5);
+    const double cz0a = rcp_nr((rcc.x + qzm.x) * 0.5), cz1a = rcp_nr((qzp.x + rcc.x) * 0.5);
+    const double cy0b = rcp_nr((rcc.y + qym.y) * 0.5), cy1b = rcp_nr((qyp.y + rcc.y) * 0.5);
+    const double cz0b = rcp_nr((rcc.y + qzm.y) * 0.5), cz1b = rcp_nr((qzp.y + rcc.y) * 0.5);
+    const double ra = residual_of(a.g, pc.x, lo_of(pc.y, pp.y), pc.y, pym.x, pyp.x, pzm.x, pzp.x, cxa, cxb, cy0a,
+                                  cy1a, cz0a, cz1a, r0.x, mean);
+    const double rb = residual_of(a.g, pc.y, pc.x, hi_of(pc.x, pn.x), pym.y, pyp.y, pzm.y, pzp.y, cxb, cxc, cy0b,
+                                  cy1b, cz0b, cz1b, r0.y, mean);
+    v[m] = make_double2(ra, rb);
+    pp = pc;
+    pc = pn;
+    rcc = rn;
+    cxa = cxa_n;
+  }
+}
+
+// pass 1, packed: residual rows, the M-point x transform, the half spectrum
+// recovered on the way out of LDS, y step A
+template <int R2H, int Y2>
+struct k_xy_fwd_rp {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(const XArgs &a, int Y1, const double2 *__restrict__ twy) {
+    if (*a.done != 0.0) return;  // the solve has converged
+    extern __shared__ double ldsd[];
+    constexpr int R1 = 16, M = R1 * R2H, P = R1 + 1, SP = Y2 + 1;
+    const Geo &g = a.g;
+    const int mxh = M + 1;
+    Ix ix{g.nx - 1, g.ny - 1, g.nz - 1, g.nx, g.nxy};
+    const double mean = a.scal[SC_SUM_RHS] / a.ncell;
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int t = (int)(blk % (unsigned)Y1), z = (int)(blk / (unsigned)Y1);
+    const int ta = threadIdx.x % R1, ca = threadIdx.x / R1;
+    double2 v[R2H];
+    row_residuals_pairs<R2H>(a, ix, ta, t + Y1 * ca, z, mean, v);
+    // step A of the M-point transform: over m, then W_M^{ta k2} = W_2M^{2 ta k2}
+    RegDft<R2H, false>::run(v);
+#pragma unroll
+    for (int k2 = 1; k2 < R2H; ++k2) v[k2] = cmul(v[k2], a.tw[2 * ta * k2]);
+    const bool b_item = (int)threadIdx.x < Y2 * R2H;
+    const int k2x = threadIdx.x % R2H, cb = threadIdx.x / R2H;
+    double2 u[R1];
+#pragma unroll
+    for (int q = 0; q < R2H; ++q) ldsd[(ca * R2H + q) * P + ta] = v[q].x;
+    __syncthreads();
+    if (b_item) {
+#pragma unroll
+      for (int q = 0; q < R1; ++q) u[q].x = ldsd[(cb * R2H + k2x) * P + q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R2H; ++q) ldsd[(ca * R2H + q) * P + ta] = v[q].y;
+    __syncthreads();
+    if (b_item) {
+#pragma unroll
+      for (int q = 0; q < R1; ++q) u[q].y = ldsd[(cb * R2H + k2x) * P + q];
+      RegDft<R1, false>::run(u);  // u[k1] = Z[k2x + R2H k1] of row cb
+    }
+    __syncthreads();
+    double2 *S = reinterpret_cast<double2 *>(ldsd);  // S[k SP + row], k < M
+    if (b_item) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) S[(k2x + R2H * k1) * SP + cb] = u[k1];
+    }
+    __syncthreads();
+    const long cs = (long)g.nz * Y1 + kRunPad;
+    const long ks = cs * mxh;
+    for (int kx = threadIdx.x; kx < mxh; kx += block_dim_x()) {
+      const double2 wn = a.tw[kx];  // W_2M^kx
+      const int ka = kx & (M - 1), kb = (M - kx) & (M - 1);
+      double2 w[Y2];
+#pragma unroll
+      for (int m = 0; m < Y2; ++m) {
+        const double2 A = S[ka * SP + m], B = S[kb * SP + m];
+        const double2 E = make_double2((A.x + B.x) * 0.5, (A.y - B.y) * 0.5);   // (A + conj B) / 2
+        const double2 D = make_double2((A.x - B.x) * 0.5, (A.y + B.y) * 0.5);   // (A - conj B) / 2
+        const double2 O = make_double2(D.y, -D.x);                              // D / i
+        w[m] = cadd(E, cmul(wn, O));
+      }
+      step_a<Y2, false>(w, t, twy);
+      double2 *dst = a.spec + kx * cs + z * Y1 + t;
+#pragma unroll
+      for (int k2 = 0; k2 < Y2; ++k2) dst[k2 * ks] = w[k2];
+    }
+  }
+};
+
 template <int R1, int R2, int Y2>
 struct k_xy_inv {
   static constexpr int kBounds = 256;
@@ -1179,6 +1402,15 @@ void xy_fwd_launch(unsigned nb, size_t sh, hipStream_t st, const XArgs &xa, int 
     default: tpx_launch<k_xy_fwd<R1, R2, 16>>(nb, 16 * R1, sh, st, xa, Y1, twy); break;
   }
 }
+// the packed forward form for x lengths 128 and 256 (R2H = 4, 8)
+template <int R2H>
+void xy_fwd_rp_launch(unsigned nb, size_t sh, hipStream_t st, const XArgs &xa, int Y1, int Y2, const double2 *twy) {
+  switch (Y2) {
+    case 4: tpx_launch<k_xy_fwd_rp<R2H, 4>>(nb, 4 * 16, sh, st, xa, Y1, twy); break;
+    case 8: tpx_launch<k_xy_fwd_rp<R2H, 8>>(nb, 8 * 16, sh, st, xa, Y1, twy); break;
+    default: tpx_launch<k_xy_fwd_rp<R2H, 16>>(nb, 16 * 16, sh, st, xa, Y1, twy); break;
+  }
+}
 template <int R1, int R2>
 void xy_inv_launch(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, int Y1, int Y2, const double2 *twy) {
   switch (Y2) {
@@ -1237,8 +1469,16 @@ void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, c
   if (!res && c->fft_passes == 3 && x4 && four_step(g.ny, y1, y2) && four_step(g.nz, z1, z2)) {
     const unsigned nb13 = (unsigned)(g.nz * y1);
     const size_t sh13 = std::max(sizeof(double) * (size_t)y2 * r2 * (r1 + 1), sizeof(double2) * (size_t)mxh * (y2 + 1));
+    // pass 1 at x lengths 128 and 256 takes the packed real-row transform
+    // unless TPX_XPACK=0 asks for the full-length complex one; pass 3 keeps
+    // the full-length inverse (the same half spectrum either way: the packed
+    // inverse, whose final stage leaves half the block idle over the P
+    // update, measured slower)
+    const bool packed = (g.nx == 128 || g.nx == 256) && c->x_packed;
 #define L_XYF(A, B) xy_fwd_launch<A, B>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1])
-    TPX_FOUR_STEP_SWITCH(g.nx, L_XYF)
+    if (packed && g.nx == 256) xy_fwd_rp_launch<8>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
+    else if (packed) xy_fwd_rp_launch<4>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
+    else { TPX_FOUR_STEP_SWITCH(g.nx, L_XYF) }
 #undef L_XYF
     TPX_LAUNCH_CHECK();
     timing_end(c, TPX_K_FFT);

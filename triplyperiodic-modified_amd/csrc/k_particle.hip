@@ -180,10 +180,13 @@ struct k_advance {
       a.base_w[d][p] = bx[d];
     }
   }
+  // the new and np1 stages stream out (read again only in the next substep,
+  // long after they have left the cache): non-temporal stores keep them from
+  // displacing the gas fields the gathers reuse
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
-    a.np1[q][p] = n1[q];
-    a.nw[q][p] = nv[q];
+    __builtin_nontemporal_store(n1[q], &a.np1[q][p]);
+    __builtin_nontemporal_store(nv[q], &a.nw[q][p]);
   }
   int ki = (int)(nv[0] * g.idx), kj = (int)(nv[1] * g.idy), kz = (int)(nv[2] * g.idz);
   long nk = ix(ki, kj, kz);

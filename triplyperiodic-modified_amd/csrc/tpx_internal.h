@@ -223,6 +223,7 @@ struct tpx_ctx {
   int device = 0;
   size_t max_lds = 65536;          // dynamic LDS bytes one workgroup may use on this device
   int fft_passes = 3;              // Poisson iteration: 3 passes where every axis allows, else 5
+  bool x_packed = true;            // x lengths 128, 256: the real rows as half-length complex transforms
   tpx::AdjointTape adj;
   // grid fields
   double *ru[5][3] = {};          // buffer sets for RU
