@@ -238,9 +238,27 @@ int tpx_adjoint_gradient_heat(tpx_ctx *ctx, double *dJ_dT, double *dJ_deps, doub
 /* How tpx_adjoint_record keeps the gas for the gas sweep: 0 (default) keeps
  * every substep's momentum, density and Poisson iterates when they fit in
  * half the free device memory, else recomputes each step from its start;
- * 1 always keeps them; 2 always recomputes.  Applies to the next recording. */
+ * 1 always keeps them; 2 always recomputes; 3 checkpoints the gas state
+ * (RU, RU_int, Rho, P) every `every` steps, as the particles are, keeps no
+ * per-step gas field but the coupling deposits of two-way coupling and the
+ * heat exchange, and has the sweep replay each checkpoint segment's gas from
+ * its checkpoint into a tape of that segment's steps before reversing them
+ * (the sweep then also runs for tpx_adjoint_gradient alone).  Applies to the
+ * next recording. */
 int tpx_adjoint_set_gas_tape(tpx_ctx *ctx, int mode);
 int tpx_adjoint_release(tpx_ctx *ctx);
+/* Transpose checks of the Poisson solve's adjoint, as the sweep runs it (no
+ * counterpart in the reference; for tests).  x, y, Ax, ATy, ATy_rho: n values
+ * each in tpx_download_field's order with bs = 0.  op 0: Ax = M x and
+ * ATy = M y, M the iteration's spectral step (transform, division by the
+ * modified wavenumbers, inverse, -1/ncell), which the sweep takes as its own
+ * transpose.  op 1: Ax = P_1 = x + M(cc (div(C grad x) - 0)), one Poisson
+ * iteration from P = x with a zero right-hand side at the context's Rho
+ * (C = 1/I_C2F(Rho), cc = -1/I_F2C(C)); ATy and ATy_rho = the adjoints of x
+ * and of Rho for the output adjoint y, from the sweep's kernels.  So
+ * <Ax, y> = <x, ATy> and d<Ax, y>/dRho = ATy_rho. */
+int tpx_adjoint_check(tpx_ctx *ctx, int op, const double *x, const double *y, double *Ax, double *ATy,
+                      double *ATy_rho);
 
 typedef struct tpx_batch tpx_batch;
 int tpx_batch_create(tpx_ctx *const *ctxs, int n, tpx_batch **out);

@@ -118,6 +118,37 @@ def test_scalar_tau_gives_the_same_gradient(gpu_available):
     assert np.allclose(g1, g2, rtol=1e-12, atol=0.0)
 
 
+@pytest.mark.parametrize("nsteps,every", [(5, 2), (4, 4)])
+def test_segment_tape_gives_the_kept_gradient(gpu_available, nsteps, every):
+    """With the gas checkpointed every `every` steps and each segment
+    replayed in the sweep (a short last segment included), the particle-only
+    call and the full call give the kept tape's gradients, and the run ends
+    where it ended."""
+    case, _ = _case()
+    out = {}
+    for mode in ("keep", "segments"):
+        s = solver_from_case(case)
+        s.adjoint_set_gas_tape(mode)
+        J = s.adjoint_record(nsteps, every)
+        g_tau = _by_flag(s, s.adjoint_gradient())
+        full = s.adjoint_gradient_full()
+        s.adjoint_release()
+        s.step(1)
+        out[mode] = (J, g_tau, full, s.get("RU"), s.get("P"))
+        s.close()
+    (Ja, ta, fa, rua, pa), (Jb, tb, fb, rub, pb) = out["keep"], out["segments"]
+    assert Ja == Jb
+    assert np.array_equal(rua, rub) and np.array_equal(pa, pb)
+    # the particle-only call of the kept tape reverses whole steps, the
+    # segment mode's the stages between the gas substeps: equal to rounding
+    np.testing.assert_allclose(tb, ta, rtol=0, atol=1e-12 * float(np.max(np.abs(ta))))
+    np.testing.assert_allclose(fb["tau"], fa["tau"], rtol=0, atol=1e-12 * float(np.max(np.abs(ta))))
+    assert abs(fa["A"] - fb["A"]) <= 1e-10 * abs(fa["A"])
+    scale = max(float(np.max(np.abs(fa[q]))) for q in ("RU", "Rho", "P"))
+    for q in ("RU", "Rho", "P"):
+        np.testing.assert_allclose(fb[q], fa[q], rtol=0, atol=1e-10 * scale, err_msg=q)
+
+
 def test_torch_shim_backward(gpu_available):
     torch = pytest.importorskip("torch")
     from tpx.adjoint import var_c

@@ -167,14 +167,16 @@ def test_full_gradient_keeps_the_particle_gradient_and_the_run(gpu_available):
 
 
 @pytest.mark.parametrize("iterations", [6, 2])
-def test_kept_and_recomputed_tapes_agree(gpu_available, iterations):
-    """The sweep from a tape that keeps every substep and the sweep that
-    recomputes each step from its start give the same gradients (to the
-    rounding of the particles' atomic deposits), and both leave the run as
+@pytest.mark.parametrize("other", ["recompute", "segments"])
+def test_kept_and_recomputed_tapes_agree(gpu_available, iterations, other):
+    """The sweep from a tape that keeps every substep, the sweep that
+    recomputes each step from its start and the sweep that replays each
+    checkpoint segment's gas from its checkpoint give the same gradients (to
+    the rounding of the particles' atomic deposits), and all leave the run as
     it ended."""
     case = _variable_density_case(iterations)
     out = {}
-    for mode in ("keep", "recompute"):
+    for mode in ("keep", other):
         s = solver_from_case(case)
         s.adjoint_set_gas_tape(mode)
         J = s.adjoint_record(NSTEPS, EVERY)
@@ -183,7 +185,7 @@ def test_kept_and_recomputed_tapes_agree(gpu_available, iterations):
         s.step(1)
         out[mode] += (s.get("RU"), s.get("P"))
         s.close()
-    (Ja, ga, rua, pa), (Jb, gb, rub, pb) = out["keep"], out["recompute"]
+    (Ja, ga, rua, pa), (Jb, gb, rub, pb) = out["keep"], out[other]
     assert Ja == Jb
     assert np.array_equal(rua, rub) and np.array_equal(pa, pb)
     assert np.array_equal(ga["tau"], gb["tau"])

@@ -177,11 +177,13 @@ def test_particle_velocity_gradient_with_heat(kept):
     assert _worst([float(g["state"][4] @ d) for d in dirs], fds, "v_p") <= FD_REL
 
 
-def test_recomputed_tape_with_heat(kept):
-    """The gas recomputed step by step (with the recorded deposits and P0)
-    gives the kept tape's gradients to the rounding of the atomic deposits."""
+@pytest.mark.parametrize("mode", ["recompute", "segments"])
+def test_recomputed_tape_with_heat(kept, mode):
+    """The gas recomputed step by step, or replayed segment by segment from
+    its checkpoints (with the recorded deposits and P0), gives the kept
+    tape's gradients to the rounding of the atomic deposits."""
     case, _, J, g = kept
-    Jr, gr = _gradients(case, "recompute")
+    Jr, gr = _gradients(case, mode)
     assert Jr == J
     for q in ("tau", "T", "eps"):
         np.testing.assert_allclose(gr[q], g[q], rtol=0, atol=1e-10 * float(np.max(np.abs(g[q]))), err_msg=q)
@@ -192,7 +194,7 @@ def test_recomputed_tape_with_heat(kept):
         assert abs(gr[q] - g[q]) <= 1e-10 * abs(g[q]), q
 
 
-@pytest.mark.parametrize("mode", ["keep", "recompute"])
+@pytest.mark.parametrize("mode", ["keep", "recompute", "segments"])
 def test_sweep_leaves_the_run_where_it_ended(gpu_available, mode):
     """After the sweep one more step equals a plain run of NSTEPS + 1 steps,
     bit for bit, with the thermodynamic pressure and the deposits restored."""

@@ -121,11 +121,13 @@ def test_particle_velocity_gradient_two_way(kept):
     assert _worst([float(g["state"][3] @ d) for d in dirs], fds, "u_p") <= FD_REL
 
 
-def test_recomputed_tape_two_way(kept):
-    """The gas recomputed step by step (with the recorded drag deposits) gives
-    the kept tape's gradients to the rounding of the atomic deposits."""
+@pytest.mark.parametrize("mode", ["recompute", "segments"])
+def test_recomputed_tape_two_way(kept, mode):
+    """The gas recomputed step by step, or replayed segment by segment from
+    its checkpoints (with the recorded drag deposits), gives the kept tape's
+    gradients to the rounding of the atomic deposits."""
     case, _, J, g = kept
-    Jr, gr = _gradients(case, "recompute")
+    Jr, gr = _gradients(case, mode)
     assert Jr == J
     np.testing.assert_allclose(gr["tau"], g["tau"], rtol=0, atol=1e-10 * float(np.max(np.abs(g["tau"]))))
     # one scale for the gas fields: at uniform density dJ/dP0 is rounding alone

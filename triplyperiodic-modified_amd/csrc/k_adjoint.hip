@@ -596,7 +596,7 @@ StepArgs step_args(tpx_ctx *c, int step, const double *const yin[7], double *con
   AdjointTape &t = c->adj;
   StepArgs a{};
   a.g = c->g;
-  a.gas = t.gas + (long)step * 16 * c->g.n;
+  a.gas = t.gas_of(step, c->g.n);
   a.nq = t.nq;
   for (int q = 0; q < t.nq; ++q) {
     a.yin[q] = yin[q];
@@ -625,9 +625,11 @@ void adjoint_tape_substep(tpx_ctx *c, int rk, const double *const ru_int[3], con
   if (t.step < 0) return;
   if (t.step >= t.nsteps) throw std::logic_error("adjoint tape overrun");
   const long n = c->g.n;
-  double *dst = t.gas + ((long)t.step * 4 + rk) * 4 * n;
-  for (int d = 0; d < 3; ++d) launch_copy(c, dst + d * n, ru_int[d], n);
-  launch_copy(c, dst + 3 * n, rho_int, n);
+  if (!t.segmented) {  // the segment mode tapes the gas when it replays a segment
+    double *dst = t.gas_of(t.step, n) + (long)rk * 4 * n;
+    for (int d = 0; d < 3; ++d) launch_copy(c, dst + d * n, ru_int[d], n);
+    launch_copy(c, dst + 3 * n, rho_int, n);
+  }
   if (t.heat) launch_copy(c, t.p0t + (long)t.step * 4 + rk, c->scal + SC_P0_INT, 1);
   if (rk == 0)
     for (int d = 0; d < 3; ++d) t.g3[(size_t)t.step * 3 + d] = (double)c->p.particle_gravity * c->gv[d];
@@ -737,6 +739,7 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
   for (int s = t.nsteps - 1; s >= 0; --s) {
     const int k0 = s / t.every, base = k0 * t.every;
     if (k0 != seg_k) {
+      if (gas && gas->segment) gas->segment(k0);  // the segment's gas tape, replayed from its checkpoint
       for (int q = 0; q < nq; ++q) launch_copy(c, start_of(0, q), t.ckpt + ((long)k0 * nq + q) * np, np);
       for (int r = base; r < s; ++r) {
         for (int q = 0; q < nq; ++q) {
@@ -755,7 +758,7 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
     if (!gas) {
       AdjArgs a{};
       a.g = c->g;
-      a.gas = t.gas + (long)s * 16 * c->g.n;
+      a.gas = t.gas_of(s, c->g.n);
       a.stage = t.stage;
       for (int q = 0; q < 6; ++q) a.lam[q] = lam[q];
       a.dtau = t.dtau;
@@ -783,7 +786,7 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
       const double *const *lrhs = gas->part_rk(s, rk);
       StageArgs a{};
       a.g = c->g;
-      a.gas = t.gas + ((long)s * 4 + rk) * 4 * c->g.n;
+      a.gas = t.gas_of(s, c->g.n) + (long)rk * 4 * c->g.n;
       a.stage = t.stage + (long)rk * nq * np;
       a.lend = t.lam;
       a.lb = t.lb;

@@ -76,21 +76,6 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return r;
 }
 
-// div(C grad f) at the cell, C on the faces at the cell (Cf) and above it
-__device__ __forceinline__ double div_c_grad(const Geo &g, const Cell &x, const double *const Cf[3],
-                                             const double *__restrict__ f, long c) {
-  const double fc = f[c];
-  double s = 0.0;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const long m = x.at(a, -1), p = x.at(a, 1);
-    const double id = inv_d(g, a);
-    const double c0 = Cf[a][c], c1 = Cf[a][p];
-    s += (c1 * ((f[p] - fc) * id) - c0 * ((fc - f[m]) * id)) * id;
-  }
-  return s;
-}
-
 // block sum of v into partials[blockIdx.x]
 __device__ __forceinline__ void block_sum(double v, double *partials) {
   __shared__ double sh[kBlock];
@@ -122,32 +107,18 @@ __device__ __forceinline__ void block_sum2(double u, double v, double *pu, doubl
   }
 }
 
-// the spectral solve of lam_P: a forward iteration from P = 0 with the
-// right-hand side -lam_P/cc has residual lam_P
-__device__ __forceinline__ void spectral_input(long c, double lp, double cc, double *tmp_rhs, double *tmp_p,
-                                               double *scal) {
-  if (c == 0) {
-    scal[SC_SUM_RHS] = 0.0;
-    scal[SC_ITERS] = 0.0;
-    scal[SC_DONE] = 0.0;
-  }
-  tmp_rhs[c] = -lp / cc;
-  tmp_p[c] = 0.0;
-}
-
 // ---- projection (G10) ------------------------------------------------------
 struct GradPArgs {
   Geo g;
   const double *np1[3], *nw[3];
   const double *rho;      // Rho_new
   double *lp;
-  double *Cf[3], *cc;     // out: C on the faces at each cell, and cc, for the Poisson kernels
-  double *tmp_rhs, *tmp_p, *scal;
+  double *cc;             // out: cc of every cell, for the Poisson kernels
   double c_np1, c_new;    // dt post, and dt pre (dt post at the last substep)
 };
 
-// lam_P += div(dt post lam_np1 + dt pre lam_new); also C and cc of the
-// substep's Poisson coefficient, and the first spectral solve's input
+// lam_P += div(dt post lam_np1 + dt pre lam_new) (grad^T = -div), and the
+// substep's cc = -1/I_F2C(C), C = 1/I_C2F(Rho_new)
 struct k_adj_grad_p {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const GradPArgs &a) {
@@ -163,14 +134,10 @@ struct k_adj_grad_p {
       const double q1 = a.c_np1 * a.np1[d][p] + a.c_new * a.nw[d][p];
       s += (q1 - q0) * inv_d(a.g, d);
       const double c0 = rcp_nr((rc + a.rho[m]) * 0.5), c1 = rcp_nr((a.rho[p] + rc) * 0.5);
-      a.Cf[d][c] = c0;
       csum += (c0 + c1) * 0.5;
     }
-    const double cc = -rcp_nr((1. / 3.) * csum);
-    a.cc[c] = cc;
-    const double lp = a.lp[c] + s;
-    a.lp[c] = lp;
-    spectral_input(c, lp, cc, a.tmp_rhs, a.tmp_p, a.scal);
+    a.cc[c] = -rcp_nr((1. / 3.) * csum);
+    a.lp[c] = a.lp[c] + s;
   }
 };
 
@@ -178,78 +145,79 @@ struct k_adj_grad_p {
 struct PoisArgs {
   Geo g;
   const double *rho;       // Rho_new
-  const double *Cf[3], *cc;
-  const double *Pi;        // P before the iteration being reversed
+  const double *cc;
+  const double *nu;        // M lam_P, the spectral solve of the iteration being reversed
+  const double *Pi;        // P before that iteration
   const double *rhs;       // raw RHS_Pois
   const double *rhs_sum;   // its sum (one value)
   double *lp;              // lam_P
-  double *tmp_rhs, *tmp_p; // the spectral solve's input and output
-  double *mu;
-  double *lrhs, *lcc, *lC[3], *lF[3];
-  double *scal;            // slots of the spectral solve
+  double *lrhs, *lcc;
+  double *lrho;            // lam_Rho_new
+  double *lF[3];           // lam_F' (pois_c's first write)
   double *partials;        // block sums of lam_RHS after the last reversed iteration
   int first;               // the first reversed iteration writes the accumulators
   int last;                // the last one (the solve's first iteration)
 };
 
-// mu = cc nu, lam_RHS -= mu, lam_cc += nu (div(C grad P_i) - RHS_P)
-struct k_adj_pois_a {
+// One Poisson iteration reversed, after its spectral solve nu = M lam_P:
+//   mu = cc nu,  lam_RHS -= mu,  lam_cc += nu (div(C grad P_i) - RHS_P),
+//   lam_P += div(C grad mu),
+// and the coefficient's share lam_C = -grad mu . grad P_i on each face taken
+// at once through C = 1/F' (lam_F' = -lam_C C^2) and F' = I_C2F(Rho_new)
+// (each face to its two cells, half each) into lam_Rho_new: the cell gets
+// 1/2 (C^2 grad mu grad P) of its six faces.  C is formed from Rho_new as the
+// forward iteration forms it; mu at the neighbours from their cc and nu.
+struct k_adj_pois_it {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const PoisArgs &a) {
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     double lr = 0.0;
     if (c < a.g.n) {
       const Cell x(a.g, c);
-      const double nu = a.tmp_p[c];
-      const double mu = a.cc[c] * nu;
-      a.mu[c] = mu;
+      const double nuc = a.nu[c];
+      const double muc = a.cc[c] * nuc;
+      const double rc = a.rho[c], pc = a.Pi[c];
+      double dmu = 0.0, dpi = 0.0, lrho = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const long m = x.at(d, -1), p = x.at(d, 1);
+        const double id = inv_d(a.g, d);
+        const double c0 = rcp_nr((rc + a.rho[m]) * 0.5), c1 = rcp_nr((a.rho[p] + rc) * 0.5);
+        const double mum = a.cc[m] * a.nu[m], mup = a.cc[p] * a.nu[p];
+        const double gm0 = (muc - mum) * id, gm1 = (mup - muc) * id;
+        const double gp0 = (pc - a.Pi[m]) * id, gp1 = (a.Pi[p] - pc) * id;
+        dmu += (c1 * gm1 - c0 * gm0) * id;
+        dpi += (c1 * gp1 - c0 * gp0) * id;
+        lrho += 0.5 * ((c0 * c0) * (gm0 * gp0) + (c1 * c1) * (gm1 * gp1));
+      }
       const double mean = a.rhs_sum[0] / (double)a.g.n;
-      const double rr = div_c_grad(a.g, x, a.Cf, a.Pi, c) - (a.rhs[c] - mean);
-      lr = (a.first ? 0.0 : a.lrhs[c]) - mu;
+      const double rr = dpi - (a.rhs[c] - mean);
+      lr = (a.first ? 0.0 : a.lrhs[c]) - muc;
       a.lrhs[c] = lr;
-      a.lcc[c] = (a.first ? 0.0 : a.lcc[c]) + nu * rr;
+      a.lcc[c] = (a.first ? 0.0 : a.lcc[c]) + nuc * rr;
+      a.lp[c] = a.lp[c] + dmu;
+      a.lrho[c] = a.lrho[c] + lrho;
     }
     if (a.last) block_sum(lr, a.partials);
   }
 };
 
-// lam_P += div(C grad mu), lam_C -= grad mu . grad P_i; then the next
-// reversed iteration's spectral solve input
-struct k_adj_pois_b {
-  static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(const PoisArgs &a) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
-    if (c >= a.g.n) return;
-    const Cell x(a.g, c);
-    const double lp = a.lp[c] + div_c_grad(a.g, x, a.Cf, a.mu, c);
-    a.lp[c] = lp;
-    const double muc = a.mu[c], pc = a.Pi[c];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const long m = x.at(d, -1);
-      const double id = inv_d(a.g, d);
-      const double v = -((muc - a.mu[m]) * id) * ((pc - a.Pi[m]) * id);
-      a.lC[d][c] = (a.first ? 0.0 : a.lC[d][c]) + v;
-    }
-    if (!a.last) spectral_input(c, lp, a.cc[c], a.tmp_rhs, a.tmp_p, a.scal);
-  }
-};
-
-// cc's dependence on C, then C = 1/F': lam_F' = -lam_C C^2 (first write of lam_F')
+// cc's dependence on C, then C = 1/F': lam_F' = -lam_C C^2 with lam_C the
+// cc share (first write of lam_F')
 struct k_adj_pois_c {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const PoisArgs &a) {
     const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
     if (c >= a.g.n) return;
     const Cell x(a.g, c);
-    const double cc0 = a.cc[c];
+    const double cc0 = a.cc[c], rc = a.rho[c];
     const double w0 = cc0 * cc0 * a.lcc[c];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const long m = x.at(d, -1);
       const double ccm = a.cc[m];
-      const double lc = a.lC[d][c] + (w0 + ccm * ccm * a.lcc[m]) * (1. / 6.);
-      const double C = a.Cf[d][c];
+      const double lc = (w0 + ccm * ccm * a.lcc[m]) * (1. / 6.);
+      const double C = rcp_nr((rc + a.rho[m]) * 0.5);
       a.lF[d][c] = -lc * C * C;
     }
   }
@@ -322,6 +290,19 @@ struct k_adj_fold {
     const Cell x(g, c);
     const double *lF[3] = {lF0, lF1, lF2};
     out[c] = fold_at(x, c, lF);
+  }
+};
+
+// out += the fold of a face adjoint
+struct k_adj_fold_add {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const Geo &g, const double *lF0, const double *lF1, const double *lF2,
+                                             double *out) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= g.n) return;
+    const Cell x(g, c);
+    const double *lF[3] = {lF0, lF1, lF2};
+    out[c] = out[c] + fold_at(x, c, lF);
   }
 };
 
@@ -569,45 +550,39 @@ void adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
   {
     GradPArgs a{};
     a.g = c->g;
-    for (int d = 0; d < 3; ++d) { a.np1[d] = ga.lnp1[d]; a.nw[d] = ga.lnew[d]; a.Cf[d] = ga.Cf[d]; }
+    for (int d = 0; d < 3; ++d) { a.np1[d] = ga.lnp1[d]; a.nw[d] = ga.lnew[d]; }
     a.rho = s.rho_new;
     a.lp = ga.lp;
     a.cc = ga.cc;
-    a.tmp_rhs = ga.tmp_rhs;
-    a.tmp_p = ga.tmp_p;
-    a.scal = ga.scal;
     a.c_np1 = k.dt * k.post;
     a.c_new = last ? k.dt * k.post : k.dt * k.pre;
     tpx_launch<k_adj_grad_p>(nb, kBlock, 0, c->stream, a);
     TPX_LAUNCH_CHECK();
   }
-  // G7, iterations in reverse
+  // G7, iterations in reverse: each applies the spectral step to lam_P
+  // (M = M^T) and then the rest of the iteration's transpose
   PoisArgs pa{};
   pa.g = c->g;
   pa.rho = s.rho_new;
-  for (int d = 0; d < 3; ++d) { pa.Cf[d] = ga.Cf[d]; pa.lC[d] = ga.lC[d]; pa.lF[d] = ga.lF[d]; }
+  for (int d = 0; d < 3; ++d) pa.lF[d] = ga.lF[d];
   pa.cc = ga.cc;
+  pa.nu = ga.nu;
   pa.rhs = s.rhs;
   pa.rhs_sum = s.rhs_sum;
   pa.lp = ga.lp;
-  pa.tmp_rhs = ga.tmp_rhs;
-  pa.tmp_p = ga.tmp_p;
-  pa.mu = ga.mu;
   pa.lrhs = ga.lrhs_p;
   pa.lcc = ga.lcc;
-  pa.scal = ga.scal;
+  pa.lrho = ga.lrnew;
   pa.partials = c->partials;
   for (int i = s.iters - 1; i >= 0; --i) {
     pa.Pi = s.P[i];
     pa.first = i == s.iters - 1;
     pa.last = i == 0;
-    launch_poisson_iteration_on(c, s.rho_new, ga.tmp_p, ga.tmp_rhs, ga.scal);
-    tpx_launch<k_adj_pois_a>(nb, kBlock, 0, c->stream, pa);
-    TPX_LAUNCH_CHECK();
-    tpx_launch<k_adj_pois_b>(nb, kBlock, 0, c->stream, pa);
+    launch_spectral_apply(c, ga.lp, ga.nu, ga.scal);
+    tpx_launch<k_adj_pois_it>(nb, kBlock, 0, c->stream, pa);
     TPX_LAUNCH_CHECK();
   }
-  launch_finish_sum(c, (int)nb, 0, SC_ADJ_SUM);  // sum of lam_RHS_Pois from the last k_adj_pois_a
+  launch_finish_sum(c, (int)nb, 0, SC_ADJ_SUM);  // sum of lam_RHS_Pois from the last k_adj_pois_it
   tpx_launch<k_adj_pois_c>(nb, kBlock, 0, c->stream, pa);
   TPX_LAUNCH_CHECK();
   // G6 and G5
@@ -702,6 +677,61 @@ void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
                          k.dt * k.pre, -k.R / k.cv / k.Vcell / (double)n);
     TPX_LAUNCH_CHECK();
   }
+}
+
+void adjoint_poisson_check(tpx_ctx *c, int op, const double *rho, const double *x, const double *y, double *ax,
+                           double *aty, double *atr, double *w) {
+  const long n = c->g.n;
+  const unsigned nb = cells(c);
+  double *cc = w, *nu = w + n, *lrhs = w + 2 * n, *lcc = w + 3 * n, *zero = w + 4 * n;
+  double *lF[3] = {w + 5 * n, w + 6 * n, w + 7 * n};
+  double *scal = w + 8 * n;
+  launch_fill(c, w, 0.0, 8 * n + SC_COUNT);
+  if (op == 0) {
+    launch_spectral_apply(c, x, ax, scal);
+    launch_spectral_apply(c, y, aty, scal);
+    return;
+  }
+  // one forward iteration from P = x with a zero right-hand side
+  launch_copy(c, ax, x, n);
+  launch_poisson_iteration_on(c, rho, ax, zero, scal);
+  launch_fill(c, scal, 0.0, SC_COUNT);  // its loop test, undone for the spectral step below
+  // the same iteration reversed for lam = y: cc, nu = M y, then the kernels
+  // of the sweep (the projection term is zero here)
+  GradPArgs ga{};
+  ga.g = c->g;
+  for (int d = 0; d < 3; ++d) { ga.np1[d] = zero; ga.nw[d] = zero; }
+  ga.rho = rho;
+  ga.lp = aty;
+  ga.cc = cc;
+  launch_copy(c, aty, y, n);
+  tpx_launch<k_adj_grad_p>(nb, kBlock, 0, c->stream, ga);
+  TPX_LAUNCH_CHECK();
+  launch_spectral_apply(c, aty, nu, scal);
+  launch_fill(c, atr, 0.0, n);
+  PoisArgs pa{};
+  pa.g = c->g;
+  pa.rho = rho;
+  pa.cc = cc;
+  pa.nu = nu;
+  pa.Pi = x;
+  pa.rhs = zero;
+  pa.rhs_sum = zero;
+  pa.lp = aty;
+  pa.lrhs = lrhs;
+  pa.lcc = lcc;
+  pa.lrho = atr;
+  for (int d = 0; d < 3; ++d) pa.lF[d] = lF[d];
+  pa.partials = c->partials;
+  pa.first = 1;
+  pa.last = 1;
+  tpx_launch<k_adj_pois_it>(nb, kBlock, 0, c->stream, pa);
+  TPX_LAUNCH_CHECK();
+  tpx_launch<k_adj_pois_c>(nb, kBlock, 0, c->stream, pa);
+  TPX_LAUNCH_CHECK();
+  tpx_launch<k_adj_fold_add>(nb, kBlock, 0, c->stream, c->g, (const double *)lF[0], (const double *)lF[1],
+                             (const double *)lF[2], atr);
+  TPX_LAUNCH_CHECK();
 }
 
 }  // namespace tpx

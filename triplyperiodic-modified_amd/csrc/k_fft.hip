@@ -152,6 +152,8 @@ struct XArgs {
   const double *done;    // nonzero: the solve has converged, the iteration is skipped
   const double *res;     // the residual formed by k_residual_coef (a solve with the
                          // caller's face coefficient), or null: formed here from rho
+  const double *in = nullptr;  // three-pass form: transform this field's rows instead of a
+                               // residual (P, rho, rhs unread)
 };
 
 // 1/x from the hardware reciprocal estimate and two Newton steps,
@@ -302,6 +304,7 @@ struct XInvArgs {
   double *partials;
   double ncell;
   const double *done;
+  int overwrite = 0;  // three-pass form: P = dp (P unread) instead of P += dp
 };
 
 // X inverse pass: Hermitian extension, inverse FFT, P += dp, block max|dp|
@@ -862,7 +865,11 @@ struct k_xy_fwd {
   double2 v[R2];
   {
     const int j = t + Y1 * ca;
-    if constexpr (R1 == 16) {
+    if (a.in) {
+      const double *row = a.in + (long)z * g.nxy + (long)j * g.nx + ta;
+#pragma unroll
+      for (int m = 0; m < R2; ++m) v[m] = make_double2(row[R1 * m], 0.0);
+    } else if constexpr (R1 == 16) {
       row_residuals<R2>(a, ix, ta, j, z, mean, v);
     } else {
 #pragma unroll
@@ -1118,7 +1125,13 @@ struct k_xy_fwd_rp {
     const int t = (int)(blk % (unsigned)Y1), z = (int)(blk / (unsigned)Y1);
     const int ta = threadIdx.x % R1, ca = threadIdx.x / R1;
     double2 v[R2H];
-    row_residuals_pairs<R2H>(a, ix, ta, t + Y1 * ca, z, mean, v);
+    if (a.in) {
+      const double *row = a.in + (long)z * g.nxy + (long)(t + Y1 * ca) * g.nx + 2 * ta;
+#pragma unroll
+      for (int m = 0; m < R2H; ++m) v[m] = *reinterpret_cast<const double2 *>(row + 32 * m);
+    } else {
+      row_residuals_pairs<R2H>(a, ix, ta, t + Y1 * ca, z, mean, v);
+    }
     // step A of the M-point transform: over m, then W_M^{ta k2} = W_2M^{2 ta k2}
     RegDft<R2H, false>::run(v);
 #pragma unroll
@@ -1246,7 +1259,7 @@ struct k_xy_inv {
     for (int k1 = 0; k1 < R1; ++k1) {
       double dp = (-1) * u[k1].x / (a.ncell);
       double *Pp = Pl + k2x + R2 * k1;
-      *Pp = *Pp + dp;
+      *Pp = a.overwrite ? dp : *Pp + dp;
       mx = fmax(mx, fabs(dp));
     }
   }
@@ -1450,6 +1463,50 @@ void launch_residual_coef(tpx_ctx *c, const double *const cf[3], const double *P
   TPX_LAUNCH_CHECK();
 }
 
+// Whether one transform pair fits the three-pass form (every axis 16..256);
+// r1, r2 / y1, y2 / z1, z2: the four-step factors of x, y and z.
+bool three_pass_shape(const tpx_ctx *c, int &r1, int &r2, int &y1, int &y2, int &z1, int &z2) {
+  const Geo &g = c->g;
+  return c->fft_passes == 3 && four_step(g.nx, r1, r2) && four_step(g.ny, y1, y2) && four_step(g.nz, z1, z2);
+}
+
+// The three passes of one transform pair (passes 1 and 3 as xa, xi ask);
+// returns the number of pass-1/3 blocks (the partials pass 3 leaves).
+unsigned three_pass_launch(tpx_ctx *c, const XArgs &xa, const XInvArgs &xi, const double *done, int r1, int r2,
+                           int y1, int y2) {
+  const Geo &g = c->g;
+  const int mxh = g.nx / 2 + 1;
+  const unsigned nb13 = (unsigned)(g.nz * y1);
+  const size_t sh13 = std::max(sizeof(double) * (size_t)y2 * r2 * (r1 + 1), sizeof(double2) * (size_t)mxh * (y2 + 1));
+  // pass 1 at x lengths 128 and 256 takes the packed real-row transform
+  // unless TPX_XPACK=0 asks for the full-length complex one; pass 3 keeps
+  // the full-length inverse (the same half spectrum either way: the packed
+  // inverse, whose final stage leaves half the block idle over the P
+  // update, measured slower)
+  const bool packed = (g.nx == 128 || g.nx == 256) && c->x_packed;
+#define L_XYF(A, B) xy_fwd_launch<A, B>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1])
+  if (packed && g.nx == 256) xy_fwd_rp_launch<8>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
+  else if (packed) xy_fwd_rp_launch<4>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
+  else { TPX_FOUR_STEP_SWITCH(g.nx, L_XYF) }
+#undef L_XYF
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_FFT);
+  timing_begin(c, TPX_K_FFT_YZ);
+  const unsigned nb2 = (unsigned)(mxh * y2);
+  const size_t sh2 = sizeof(double2) * (size_t)y1 * (g.nz + g.nz / 16);
+#define L_YZ(A, B) yz_launch<A, B>(nb2, sh2, c->stream, c->spec[0], mxh, y1, y2, c, done)
+  TPX_FOUR_STEP_SWITCH(g.nz, L_YZ)
+#undef L_YZ
+  TPX_LAUNCH_CHECK();
+  timing_end(c, TPX_K_FFT_YZ);
+  timing_begin(c, TPX_K_FFT_INV);
+#define L_XYI(A, B) xy_inv_launch<A, B>(nb13, sh13, c->stream, xi, y1, y2, c->tw[1])
+  TPX_FOUR_STEP_SWITCH(g.nx, L_XYI)
+#undef L_XYI
+  TPX_LAUNCH_CHECK();
+  return nb13;
+}
+
 void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal,
                                  const double *res, int sum_slot) {
   const Geo &g = c->g;
@@ -1466,35 +1523,8 @@ void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, c
   int y1, y2, z1, z2;
   // the three-pass form forms the residual itself from rho; a precomputed
   // residual goes through the five-pass kernels
-  if (!res && c->fft_passes == 3 && x4 && four_step(g.ny, y1, y2) && four_step(g.nz, z1, z2)) {
-    const unsigned nb13 = (unsigned)(g.nz * y1);
-    const size_t sh13 = std::max(sizeof(double) * (size_t)y2 * r2 * (r1 + 1), sizeof(double2) * (size_t)mxh * (y2 + 1));
-    // pass 1 at x lengths 128 and 256 takes the packed real-row transform
-    // unless TPX_XPACK=0 asks for the full-length complex one; pass 3 keeps
-    // the full-length inverse (the same half spectrum either way: the packed
-    // inverse, whose final stage leaves half the block idle over the P
-    // update, measured slower)
-    const bool packed = (g.nx == 128 || g.nx == 256) && c->x_packed;
-#define L_XYF(A, B) xy_fwd_launch<A, B>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1])
-    if (packed && g.nx == 256) xy_fwd_rp_launch<8>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
-    else if (packed) xy_fwd_rp_launch<4>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
-    else { TPX_FOUR_STEP_SWITCH(g.nx, L_XYF) }
-#undef L_XYF
-    TPX_LAUNCH_CHECK();
-    timing_end(c, TPX_K_FFT);
-    timing_begin(c, TPX_K_FFT_YZ);
-    const unsigned nb2 = (unsigned)(mxh * y2);
-    const size_t sh2 = sizeof(double2) * (size_t)y1 * (g.nz + g.nz / 16);
-#define L_YZ(A, B) yz_launch<A, B>(nb2, sh2, c->stream, c->spec[0], mxh, y1, y2, c, done)
-    TPX_FOUR_STEP_SWITCH(g.nz, L_YZ)
-#undef L_YZ
-    TPX_LAUNCH_CHECK();
-    timing_end(c, TPX_K_FFT_YZ);
-    timing_begin(c, TPX_K_FFT_INV);
-#define L_XYI(A, B) xy_inv_launch<A, B>(nb13, sh13, c->stream, xi, y1, y2, c->tw[1])
-    TPX_FOUR_STEP_SWITCH(g.nx, L_XYI)
-#undef L_XYI
-    TPX_LAUNCH_CHECK();
+  if (!res && three_pass_shape(c, r1, r2, y1, y2, z1, z2)) {
+    const unsigned nb13 = three_pass_launch(c, xa, xi, done, r1, r2, y1, y2);
     tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nb13, scal, c->Iteration, c->p.epsilon);
     TPX_LAUNCH_CHECK();
     timing_end(c, TPX_K_FFT_INV);
@@ -1573,6 +1603,28 @@ void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, c
   tpx_launch<k_finish_iter>(1, 1024, 0, c->stream, c->partials, nbx, scal, c->Iteration, c->p.epsilon);
   TPX_LAUNCH_CHECK();
   timing_end(c, TPX_K_FFT_INV);
+}
+
+void launch_spectral_apply(tpx_ctx *c, const double *in, double *out, double *scal) {
+  const Geo &g = c->g;
+  int r1, r2, y1, y2, z1, z2;
+  const double *done = scal + SC_DONE;
+  if (three_pass_shape(c, r1, r2, y1, y2, z1, z2)) {
+    timing_begin(c, TPX_K_FFT);
+    XArgs xa{g, nullptr, nullptr, nullptr, scal, c->spec[0], c->tw[0], (double)g.n, done, nullptr};
+    xa.in = in;
+    XInvArgs xi{g, out, c->spec[0], c->tw[0], c->partials, (double)g.n, done};
+    xi.overwrite = 1;
+    three_pass_launch(c, xa, xi, done, r1, r2, y1, y2);
+    timing_end(c, TPX_K_FFT_INV);
+    return;
+  }
+  // the five-pass form: the input as a precomputed residual, from out = 0;
+  // its loop test is undone so that the next application runs
+  launch_fill(c, out, 0.0, g.n);
+  launch_poisson_iteration_on(c, nullptr, out, nullptr, scal, in);
+  launch_fill(c, scal + SC_ITERS, 0.0, 1);
+  launch_fill(c, scal + SC_DONE, 0.0, 1);
 }
 
 }  // namespace tpx

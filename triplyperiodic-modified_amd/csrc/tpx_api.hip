@@ -537,8 +537,11 @@ void adjoint_free(AdjointTape &t) {
   auto f = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
   f(t.gas); f(t.ckpt); f(t.y); f(t.seg); f(t.lb); f(t.lnext); f(t.S); f(t.stage); f(t.lam); f(t.dtau); f(t.djdc);
   f(t.start); f(t.final_state); f(t.work); f(t.cap); f(t.lup); f(t.iters_seen); f(t.sub); f(t.tau0);
-  f(t.qt); f(t.qs); f(t.p0t); f(t.eps0); f(t.deps); f(t.lh); f(t.hpart); f(t.scal_save);
+  f(t.qt); f(t.qs); f(t.p0t); f(t.eps0); f(t.deps); f(t.lh); f(t.hpart); f(t.scal_save); f(t.gck);
   t.heat = false;
+  t.segmented = false;
+  t.gas_base = 0;
+  t.seg_loaded = -1;
   t.nq = 6;
   t.swept = false;
   auto fi = [](int32_t *&p) { if (p) (void)hipFree(p); p = nullptr; };
@@ -1412,34 +1415,45 @@ void gas_state_copy(tpx_ctx *c, double *dst, bool save) {
   if (!save && c->rho_int != c->rho_base) launch_copy(c, c->rho[c->rho_int], dst + 6 * n, n);
 }
 
-// Recomputes step s of the recorded run, gas only (the gas does not depend on
-// the particles under one-way coupling; under two-way coupling the recorded
-// drag deposits stand in for them, with the heat exchange the recorded
-// RHS_Part_Temp, its sum and P0_new), from its start on the tape, keeping
-// what the reverse sweep reads: RU_new and Rho_new after the momentum and
-// density updates, the raw Poisson right-hand side and its sum, and P before
-// every Poisson iteration.  The arithmetic is the fused step's.
-void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
+// One pool buffer of n values (grown on demand); null when out of memory.
+double *pcap_slot(AdjointTape &t, size_t i, long n) {
+  while (t.pcap.size() <= i) {
+    double *p = nullptr;
+    if (hipMalloc((void **)&p, sizeof(double) * n) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    t.pcap.push_back(p);
+  }
+  return t.pcap[i];
+}
+
+// The gas substeps of step s of the recorded run, gas only (the gas does not
+// depend on the particles under one-way coupling; under two-way coupling the
+// recorded drag deposits stand in for them, with the heat exchange the
+// recorded RHS_Part_Temp, its sum and P0_new), from the gas state in place
+// at the step's start, keeping what the reverse sweep reads: RU_new and
+// Rho_new after the momentum and density updates, the raw Poisson right-hand
+// side and its sum, and P before every Poisson iteration.  The arithmetic is
+// the fused step's.  With `segment` set the step's substep starts (RU_int,
+// Rho_int) are taped too, into the loaded segment's slots, and the captures
+// go to the segment's sub tape; else to the one step's capture buffers.
+void gas_substeps(tpx_ctx *c, int s, GasSubstep sub[4], bool segment) {
   AdjointTape &t = c->adj;
   const long n = c->g.n;
-  if (c->ru_int == c->ru_base) throw std::logic_error("adjoint: RU_int aliases RU at a step boundary");
-  const double *st = t.start + (long)s * 4 * n;
-  const double *g0 = t.gas + (long)s * 16 * n;
-  for (int d = 0; d < 3; ++d) {
-    launch_copy(c, c->ru[c->ru_base][d], st + d * n, n);
-    launch_copy(c, c->ru[c->ru_int][d], g0 + d * n, n);
-  }
-  launch_copy(c, c->rho[c->rho_base], g0 + 3 * n, n);
-  if (c->rho_int != c->rho_base) launch_copy(c, c->rho[c->rho_int], g0 + 3 * n, n);
-  launch_copy(c, c->P, st + 3 * n, n);
   c->A = t.A[s];
   c->Iteration = t.iteration[s];
   for (int d = 0; d < 3; ++d) c->gv[d] = t.gv[(size_t)s * 3 + d];
-  size_t used = 0;
+  const int j = s - t.gas_base;  // slot in a segment's tapes
+  size_t used = segment ? t.pnext : 0;
   for (int rk = 0; rk < 4; ++rk) {
     Coef k = coef(c, rk);
     GasSubstep &g = sub[rk];
-    const double *gs = t.gas + ((long)s * 4 + rk) * 4 * n;
+    double *gs = t.gas_of(s, n) + (long)rk * 4 * n;
+    if (segment) {
+      for (int d = 0; d < 3; ++d) launch_copy(c, gs + d * n, c->ru[c->ru_int][d], n);
+      launch_copy(c, gs + 3 * n, c->rho[c->rho_int], n);
+    }
     for (int d = 0; d < 3; ++d) g.ru_int[d] = gs + d * n;
     g.rho_int = gs + 3 * n;
     const size_t q = (size_t)s * 4 + rk;
@@ -1457,7 +1471,7 @@ void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
     if (t.S)  // two-way coupling: the recorded drag deposit of this substep
       for (int d = 0; d < 3; ++d) launch_copy(c, c->S[d], t.S + (((long)s * 4 + rk) * 3 + d) * n, n);
     do_update_ru_wop(c, rk, true);
-    double *cp = t.cap + (long)rk * 5 * n;
+    double *cp = segment ? t.sub + ((long)j * 4 + rk) * 5 * n : t.cap + (long)rk * 5 * n;
     for (int d = 0; d < 3; ++d) {
       launch_copy(c, cp + d * n, c->ru[c->ru_new][d], n);
       g.ru_new[d] = cp + d * n;
@@ -1469,22 +1483,20 @@ void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
     launch_div_rhs(c, k, 0, rn, c->rho[c->rho_new], c->q_zero ? nullptr : c->Q, nullptr, c->rhs);
     launch_copy(c, cp + 4 * n, c->rhs, n);
     g.rhs = cp + 4 * n;
-    double *sum = t.cap + 20 * n + rk;
+    double *sum = segment ? t.sub + (long)t.tape_steps() * 20 * n + (long)j * 4 + rk : t.cap + 20 * n + rk;
     launch_copy(c, sum, c->scal + SC_SUM_RHS, 1);
     g.rhs_sum = sum;
     // the recorded number of iterations, P kept before each; the count the
     // device's loop test reaches is kept for a check after the sweep
     launch_solve_begin(c);
     g.P.clear();
-    g.iters = t.iters[(size_t)s * 4 + rk];
+    g.iters = t.iters[q];
+    if (segment) t.p_first[q] = used;
     for (int it = 0; it < g.iters; ++it) {
-      if (used == t.pcap.size()) {
-        double *p = nullptr;
-        if (hipMalloc((void **)&p, sizeof(double) * n) != hipSuccess) throw std::bad_alloc();
-        t.pcap.push_back(p);
-      }
-      launch_copy(c, t.pcap[used], c->P, n);
-      g.P.push_back(t.pcap[used++]);
+      double *buf = pcap_slot(t, used++, n);
+      if (!buf) throw std::bad_alloc();
+      launch_copy(c, buf, c->P, n);
+      g.P.push_back(buf);
       launch_poisson_iteration(c, c->rho[c->rho_new]);
     }
     launch_copy(c, t.iters_seen + (long)s * 4 + rk, c->scal + SC_ITERS, 1);
@@ -1495,6 +1507,7 @@ void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
     c->ru_new = -1;
     launch_scalar_copy(c, SC_P0_INT, SC_P0_NEW);
   }
+  if (segment) t.pnext = used;
   const double a[3] = {c->p.u0 * c->p.rho0, c->p.v0 * c->p.rho0, c->p.w0 * c->p.rho0};
   do_make_mean_u0(c, a, true);
   c->rho_base = rho_np1_read(c);  // the gas half of TimeAdvance
@@ -1504,17 +1517,38 @@ void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
   launch_scalar_copy(c, SC_P0, SC_P0_NP1);
 }
 
-// One pool buffer of n values (grown on demand); null when out of memory.
-double *pcap_slot(AdjointTape &t, size_t i, long n) {
-  while (t.pcap.size() <= i) {
-    double *p = nullptr;
-    if (hipMalloc((void **)&p, sizeof(double) * n) != hipSuccess) {
-      (void)hipGetLastError();
-      return nullptr;
-    }
-    t.pcap.push_back(p);
+// Recomputes step s of the recorded run from its start on the tape (RU and P
+// of the step's start, RU_int and Rho of its first substep), capturing into
+// the one step's buffers.
+void gas_recompute_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
+  AdjointTape &t = c->adj;
+  const long n = c->g.n;
+  if (c->ru_int == c->ru_base) throw std::logic_error("adjoint: RU_int aliases RU at a step boundary");
+  const double *st = t.start + (long)s * 4 * n;
+  const double *g0 = t.gas_of(s, n);
+  for (int d = 0; d < 3; ++d) {
+    launch_copy(c, c->ru[c->ru_base][d], st + d * n, n);
+    launch_copy(c, c->ru[c->ru_int][d], g0 + d * n, n);
   }
-  return t.pcap[i];
+  launch_copy(c, c->rho[c->rho_base], g0 + 3 * n, n);
+  if (c->rho_int != c->rho_base) launch_copy(c, c->rho[c->rho_int], g0 + 3 * n, n);
+  launch_copy(c, c->P, st + 3 * n, n);
+  gas_substeps(c, s, sub, false);
+}
+
+// Segment mode: replays checkpoint segment k (its steps, gas only, from the
+// gas checkpoint of its first step), taping every substep's start and what
+// the reverse sweep reads into the segment tapes.
+void gas_replay_segment(tpx_ctx *c, int k) {
+  AdjointTape &t = c->adj;
+  const long n = c->g.n;
+  const int base = k * t.every, end = std::min(t.nsteps, base + t.every);
+  gas_state_copy(c, t.gck + (long)k * 8 * n, false);
+  t.gas_base = base;
+  t.pnext = 0;
+  GasSubstep scratch_sub[4];
+  for (int s = base; s < end; ++s) gas_substeps(c, s, scratch_sub, true);
+  t.seg_loaded = k;
 }
 
 // Step s of the recorded run as the gas sweep reads it, from the tape kept
@@ -1525,10 +1559,11 @@ void taped_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
   c->A = t.A[s];
   c->Iteration = t.iteration[s];
   for (int d = 0; d < 3; ++d) c->gv[d] = t.gv[(size_t)s * 3 + d];
+  const int j = s - t.gas_base;  // slot in the tapes (a segment's, or the whole run's)
   for (int rk = 0; rk < 4; ++rk) {
     GasSubstep &g = sub[rk];
     const size_t q = (size_t)s * 4 + rk;
-    const double *gs = t.gas + (long)q * 4 * n, *sb = t.sub + (long)q * 5 * n;
+    const double *gs = t.gas_of(s, n) + (long)rk * 4 * n, *sb = t.sub + ((long)j * 4 + rk) * 5 * n;
     for (int d = 0; d < 3; ++d) {
       g.ru_int[d] = gs + d * n;
       g.ru_new[d] = sb + d * n;
@@ -1536,7 +1571,7 @@ void taped_step(tpx_ctx *c, int s, GasSubstep sub[4]) {
     g.rho_int = gs + 3 * n;
     g.rho_new = sb + 3 * n;
     g.rhs = sb + 4 * n;
-    g.rhs_sum = t.sub + (long)t.nsteps * 20 * n + q;
+    g.rhs_sum = t.sub + (long)t.tape_steps() * 20 * n + (long)j * 4 + rk;
     if (t.heat) {
       g.Q = t.qt + (long)q * n;
       g.q_sum = t.qs + q;
@@ -1560,17 +1595,16 @@ void gas_adjoint_run(tpx_ctx *c) {
   };
   alloc(&t.work, (size_t)GasAdjoint::kFields * n + SC_COUNT);
   alloc(&t.lup, 3 * (size_t)n);
-  if (!t.store) {
-    alloc(&t.cap, 20 * (size_t)n + 8);
-    alloc(&t.iters_seen, 4 * (size_t)t.nsteps);
-  }
+  if (!t.store) alloc(&t.iters_seen, 4 * (size_t)t.nsteps);
+  if (!t.store && !t.segmented) alloc(&t.cap, 20 * (size_t)n + 8);
+  if (t.segmented) alloc(&t.sub, (size_t)t.tape_steps() * (20 * (size_t)n + 4));
   GasAdjoint &ga = t.ga;
   double *w = t.work;
   auto take = [&]() { double *p = w; w += n; return p; };
-  for (auto *grp : {ga.lru, ga.lnp1, ga.lint, ga.lnew, ga.lrhs, ga.lu, ga.lC, ga.lF, ga.Cf, ga.U})
+  for (auto *grp : {ga.lru, ga.lnp1, ga.lint, ga.lnew, ga.lrhs, ga.lu, ga.lF, ga.U})
     for (int d = 0; d < 3; ++d) grp[d] = take();
-  for (double **p : {&ga.lr, &ga.lrnp1, &ga.lrint, &ga.lrnew, &ga.lp, &ga.lrhs_p, &ga.lcc, &ga.ld, &ga.mu,
-                     &ga.tmp_rhs, &ga.tmp_p, &ga.cc, &ga.lq})
+  for (double **p : {&ga.lr, &ga.lrnp1, &ga.lrint, &ga.lrnew, &ga.lp, &ga.lrhs_p, &ga.lcc, &ga.ld, &ga.nu, &ga.cc,
+                     &ga.lq})
     *p = take();
   ga.scal = w;
   launch_fill(c, t.work, 0.0, (long)GasAdjoint::kFields * n + SC_COUNT);
@@ -1586,10 +1620,11 @@ void gas_adjoint_run(tpx_ctx *c) {
   const bool stored = t.store;
   GasSweep sweep;
   sweep.begin = [&](int s) {
-    if (stored) taped_step(c, s, sub);
+    if (stored || t.segmented) taped_step(c, s, sub);
     else gas_recompute_step(c, s, sub);
     if (s < t.nsteps - 1) adjoint_gas_boundary(c, ga, t.heat);
   };
+  if (t.segmented) sweep.segment = [&](int k) { gas_replay_segment(c, k); };
   sweep.part_a = [&](int, int rk) { adjoint_gas_substep_a(c, ga, sub[rk], coef(c, rk)); };
   sweep.part_rk = [&](int, int rk) { return adjoint_gas_substep_rk(c, ga, sub[rk], coef(c, rk)); };
   sweep.heat_mid = [&](int, int rk) { return GasSweep::HeatMid{sub[rk].rho_new, ga.lq, ga.lrnew}; };
@@ -1765,8 +1800,12 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
     auto alloc = [](double **p, size_t count) {
       if (hipMalloc((void **)p, sizeof(double) * count) != hipSuccess) throw std::bad_alloc();
     };
-    alloc(&t.gas, (size_t)nsteps * 16 * n);
+    t.segmented = t.mode == 3;
+    t.gas_base = 0;
+    t.seg_loaded = -1;
+    alloc(&t.gas, (size_t)t.tape_steps() * 16 * n);
     alloc(&t.ckpt, (size_t)(nsteps / every + 1) * nq * np);
+    if (t.segmented) alloc(&t.gck, (size_t)((nsteps - 1) / every + 1) * 8 * n);
     alloc(&t.y, nq * (size_t)np);
     alloc(&t.stage, 4 * nq * (size_t)np);
     alloc(&t.lam, nq * (size_t)np);
@@ -1794,7 +1833,7 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
       alloc(&t.tau0, (size_t)np);
       launch_copy(c, t.tau0, pt.tau, np);
     }
-    alloc(&t.start, (size_t)nsteps * 4 * n);
+    if (!t.segmented) alloc(&t.start, (size_t)nsteps * 4 * n);
     const bool twc = c->p.twc != 0;
     if (twc) alloc(&t.S, (size_t)nsteps * 12 * n);  // the drag deposits, for a gas recomputation
     alloc(&t.final_state, 8 * (size_t)n);
@@ -1803,7 +1842,7 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
     t.p_first.assign((size_t)nsteps * 4, 0);
     t.pnext = 0;
     t.store = false;
-    if (t.mode != 2) {
+    if (t.mode == 0 || t.mode == 1) {
       size_t free_b = 0, total_b = 0;
       hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
       const double need = (double)nsteps * 4 * (5 + 3) * n * sizeof(double);
@@ -1827,9 +1866,23 @@ int tpx_adjoint_record(tpx_ctx *ctx, int nsteps, int every, double *J) {
       for (int s = 0; s < nsteps; ++s) {
         t.step = s;
         do_params_update(c);
-        // the gas at the step's start (RU_int and Rho are the first substep's tape)
-        for (int d = 0; d < 3; ++d) launch_copy(c, t.start + ((long)s * 4 + d) * n, c->ru[c->ru_base][d], n);
-        launch_copy(c, t.start + ((long)s * 4 + 3) * n, c->P, n);
+        if (t.segmented) {
+          // the gas at a segment's start: RU, RU_int, Rho, P (at the run's
+          // start RU_int is RU's own buffer; the restore writes both)
+          if (s % every == 0) {
+            double *dst = t.gck + (long)(s / every) * 8 * n;
+            for (int d = 0; d < 3; ++d) {
+              launch_copy(c, dst + d * n, c->ru[c->ru_base][d], n);
+              launch_copy(c, dst + (3 + d) * n, c->ru[c->ru_int][d], n);
+            }
+            launch_copy(c, dst + 6 * n, c->rho[c->rho_base], n);
+            launch_copy(c, dst + 7 * n, c->P, n);
+          }
+        } else {
+          // the gas at the step's start (RU_int and Rho are the first substep's tape)
+          for (int d = 0; d < 3; ++d) launch_copy(c, t.start + ((long)s * 4 + d) * n, c->ru[c->ru_base][d], n);
+          launch_copy(c, t.start + ((long)s * 4 + 3) * n, c->P, n);
+        }
         t.A[s] = c->A;
         if (t.heat) t.I0[s] = c->I0;
         t.iteration[s] = c->Iteration;
@@ -1895,7 +1948,8 @@ int tpx_adjoint_gradient_full(tpx_ctx *ctx, double *dJ_dtau, double *dJ_dstate, 
     if (!gas && !dJ_dtau && !dJ_dstate) throw std::invalid_argument("null outputs");
     // under two-way coupling or with the heat exchange the gas depends on
     // the particles, so every gradient needs the gas sweep
-    if (gas || ctx->p.twc || t.heat) gas_adjoint_run(ctx);
+    // the same holds in the segment mode, whose sweep replays the gas tape
+    if (gas || ctx->p.twc || t.heat || t.segmented) gas_adjoint_run(ctx);
     else adjoint_reverse(ctx, nullptr);
     const size_t np = (size_t)ctx->pt.n, n = (size_t)ctx->g.n;
     // the sweep's particle arrays are in the recording's order: give them in
@@ -1960,7 +2014,8 @@ int tpx_adjoint_gradient_heat(tpx_ctx *ctx, double *dJ_dT, double *dJ_deps, doub
 
 int tpx_adjoint_set_gas_tape(tpx_ctx *ctx, int mode) {
   return guarded(ctx, [&]() {
-    if (mode < 0 || mode > 2) throw std::invalid_argument("gas tape mode must be 0 (by memory), 1 (keep) or 2 (recompute)");
+    if (mode < 0 || mode > 3)
+      throw std::invalid_argument("gas tape mode must be 0 (by memory), 1 (keep), 2 (recompute) or 3 (segments)");
     ctx->adj.mode = mode;
     return TPX_OK;
   });
@@ -1970,6 +2025,35 @@ int tpx_adjoint_release(tpx_ctx *ctx) {
   return guarded(ctx, [&]() {
     sync(ctx);
     adjoint_free(ctx->adj);
+    return TPX_OK;
+  });
+}
+
+int tpx_adjoint_check(tpx_ctx *ctx, int op, const double *x, const double *y, double *Ax, double *ATy,
+                      double *ATy_rho) {
+  return guarded(ctx, [&]() {
+    if (op < 0 || op > 1) throw std::invalid_argument("op must be 0 (spectral step) or 1 (Poisson iteration)");
+    if (!x || !y || !Ax || !ATy || (op == 1 && !ATy_rho)) throw std::invalid_argument("null host array");
+    const size_t n = (size_t)ctx->g.n;
+    sync(ctx);
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(double) * (13 * n + SC_COUNT)) != hipSuccess) throw std::bad_alloc();
+    try {
+      hip_check(hipMemcpyAsync(d, x, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
+      hip_check(hipMemcpyAsync(d + n, y, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream), "upload");
+      adjoint_poisson_check(ctx, op, ctx->rho[ctx->rho_base], d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n);
+      hip_check(hipMemcpyAsync(Ax, d + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream), "download");
+      hip_check(hipMemcpyAsync(ATy, d + 3 * n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream), "download");
+      if (op == 1)
+        hip_check(hipMemcpyAsync(ATy_rho, d + 4 * n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream),
+                  "download");
+      sync(ctx);
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipFree(d);
+      throw;
+    }
+    (void)hipFree(d);
     return TPX_OK;
   });
 }

@@ -12,8 +12,9 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <functional>
-
+#include <stdexcept>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -102,11 +103,11 @@ struct Particles {
 // fields F (Rho_face) and C (1/Rho_face); the substep's C, cc and U; the
 // spectral solve's buffers.
 struct GasAdjoint {
-  double *lru[3], *lnp1[3], *lint[3], *lnew[3], *lrhs[3], *lu[3], *lC[3], *lF[3], *Cf[3], *U[3];
-  double *lr, *lrnp1, *lrint, *lrnew, *lp, *lrhs_p, *lcc, *ld, *mu, *tmp_rhs, *tmp_p, *cc;
+  double *lru[3], *lnp1[3], *lint[3], *lnew[3], *lrhs[3], *lu[3], *lF[3], *U[3];
+  double *lr, *lrnp1, *lrint, *lrnew, *lp, *lrhs_p, *lcc, *ld, *nu, *cc;
   double *lq;    // adjoint of the substep's heat deposit RHS_Part_Temp
-  double *scal;  // SC_COUNT slots of the spectral solve
-  static constexpr int kFields = 43;
+  double *scal;  // SC_COUNT slots of the spectral solve (SC_DONE stays 0)
+  static constexpr int kFields = 35;
 };
 
 // One recomputed substep as the gas adjoint reads it.
@@ -125,13 +126,19 @@ struct GasSubstep {
 // The adjoint of a recorded run: the gas stage fields of every substep, each
 // step's gas start (RU, P), the particle base state every `every` steps, and
 // the reverse sweeps' work arrays.  Particle arrays hold nq quantities: x, y,
-// z, u, v, w, and T when the heat exchange is on.
+// z, u, v, w, and T when the heat exchange is on.  In the segment mode the
+// gas is checkpointed every `every` steps too, and the per-substep gas tape
+// covers only the segment being reversed, replayed from its checkpoint.
 struct AdjointTape {
   int nsteps = 0, every = 1;
   int nq = 6;                    // particle quantities taped and reversed (7 with the heat exchange)
   int step = -1;                 // step being recorded (-1: not recording)
   bool ready = false;            // a whole run is recorded and its objective computed
-  double *gas = nullptr;         // nsteps x 4 substeps x (RU_int 3, Rho_int) x n
+  double *gas = nullptr;         // steps x 4 substeps x (RU_int 3, Rho_int) x n: every step, or one segment's
+  int gas_base = 0;              // the step that gas slot 0 holds (a segment's first step, else 0)
+  bool segmented = false;        // gas checkpointed every `every` steps, each segment replayed in the sweep
+  double *gck = nullptr;         // segment mode: (RU 3, RU_int 3, Rho, P) x n at each segment's start
+  int seg_loaded = -1;           // segment mode: the segment whose gas tape is loaded
   std::vector<double> g3;        // particle gravity of each step (3 per step)
   double *ckpt = nullptr;        // (nsteps / every + 1) x nq x np
   double *y = nullptr;           // nq x np
@@ -158,9 +165,11 @@ struct AdjointTape {
   double *cap = nullptr;         // 4 substeps x (RU_new 3, Rho_new, rhs) x n + 4 sums
   double *lup = nullptr;         // 4 substeps x 3 x n
   std::vector<double *> pcap;    // P before each Poisson iteration (grown on demand)
-  int mode = 0;                  // gas tape: 0 keep when it fits in memory, 1 keep, 2 recompute each step
+  int mode = 0;                  // gas tape: 0 keep when it fits in memory, 1 keep, 2 recompute each
+                                 // step, 3 checkpoint every `every` steps and replay each segment
   bool store = false;            // the recorded run kept every substep (sub, pcap): no recomputation
-  double *sub = nullptr;         // nsteps x 4 x (RU_new 3, Rho_new, raw RHS_Pois) x n, then nsteps x 4 sums
+  double *sub = nullptr;         // steps x 4 x (RU_new 3, Rho_new, raw RHS_Pois) x n, then steps x 4 sums
+                                 // (every step, or the loaded segment's)
   std::vector<size_t> p_first;   // first pcap buffer of each substep's solve
   size_t pnext = 0;
   // heat exchange
@@ -176,6 +185,13 @@ struct AdjointTape {
   std::vector<double> I0;        // lamp intensity of each step
   bool swept = false;            // a sweep has run since the recording
   GasAdjoint ga{};
+  // steps the gas and sub tapes hold
+  int tape_steps() const { return segmented ? std::min(every, nsteps) : nsteps; }
+  // the gas tape of step s (4 substeps x 4 fields of n)
+  double *gas_of(int s, long n) const {
+    if (s < gas_base || s >= gas_base + tape_steps()) throw std::logic_error("adjoint: gas tape step not loaded");
+    return gas + (long)(s - gas_base) * 16 * n;
+  }
 };
 
 // Tile shapes of the two deposit kernels (k_heat_fused, k_vel_planes): x-rows
@@ -312,6 +328,10 @@ void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
 // sum_slot: the scal slot holding the sum of rhs (its mean is removed in the residual)
 void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal,
                                  const double *res = nullptr, int sum_slot = SC_SUM_RHS);
+// out = M in, M the Poisson iteration's spectral step (forward transform, the
+// division by the modified wavenumbers, inverse, -1/ncell): the iteration's
+// dp for the residual `in`.  No loop test runs; scal[SC_DONE] must be 0.
+void launch_spectral_apply(tpx_ctx *c, const double *in, double *out, double *scal);
 // r = (div_F2C(C grad_C2F P) - RHS) cc for a caller's face coefficient C (cf[0..2])
 void launch_residual_coef(tpx_ctx *c, const double *const cf[3], const double *P, const double *rhs, double *res);
 // the passive vector (k_vector.hip): Update_RV_WOQ, Compute_RHS_Pois_Q (raw
@@ -372,6 +392,9 @@ struct GasSweep {
   };
   std::function<HeatMid(int, int)> heat_mid;
   HeatMid heat_init{};  // the recorded run's initial exchange (Rho at the start)
+  // segment mode: loads the gas tape of checkpoint segment k before the
+  // particles' stages of that segment are recomputed (null: the tape is whole)
+  std::function<void(int)> segment;
 };
 // dJ/dtau into adj.dtau and dJ/d(initial particle state) into adj.lam; with
 // gas set, the gas sweep runs between the particles' stages
@@ -397,6 +420,13 @@ void adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
 const double *const *adjoint_gas_substep_rk(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
 void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k);
 void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga, bool heat);
+// Transpose checks of the Poisson adjoint (device arrays of n values; w:
+// 8 n + SC_COUNT of work).  op 0: ax = M x, aty = M y (M the spectral step).
+// op 1: ax = one iteration from P = x with a zero right-hand side at the
+// density rho; aty, atr = that iteration reversed for the adjoint y of its
+// output: the adjoint of its input P, and of rho (through C and cc).
+void adjoint_poisson_check(tpx_ctx *c, int op, const double *rho, const double *x, const double *y, double *ax,
+                           double *aty, double *atr, double *w);
 
 // ---- timing helpers (tpx_api.hip) ----
 void timing_begin(tpx_ctx *c, int cls);

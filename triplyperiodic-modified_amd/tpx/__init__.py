@@ -52,7 +52,7 @@ EXPORTS = [
     "tpx_adjoint_record", "tpx_adjoint_gradient", "tpx_adjoint_gradient_full", "tpx_adjoint_gradient_heat", "tpx_adjoint_set_gas_tape", "tpx_statistics_enable", "tpx_statistics",
     "tpx_track_particles",
     "tpx_tune_tiles", "tpx_get_tiles", "tpx_set_tiles",
-    "tpx_adjoint_release",
+    "tpx_adjoint_release", "tpx_adjoint_check",
     "tpx_batch_create", "tpx_batch_step", "tpx_batch_destroy", "tpx_batch_error", "tpx_batch_last_error",
     "tpx_field_sum",
     "tpx_field_max_abs",
@@ -121,6 +121,7 @@ def load(path: str = LIB_PATH):
     L.tpx_statistics.argtypes = [P, DP]
     L.tpx_track_particles.argtypes = [P, I, I, DP]
     L.tpx_adjoint_release.argtypes = [P]
+    L.tpx_adjoint_check.argtypes = [P, I, DP, DP, DP, DP, DP]
     L.tpx_batch_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.tpx_batch_step.argtypes = [P, I]
     L.tpx_batch_destroy.argtypes = [P]
@@ -350,12 +351,25 @@ class Solver:
     def adjoint_set_gas_tape(self, mode: str = "auto") -> None:
         """How the next recording keeps the gas for the gas sweep: "auto"
         (keep every substep when it fits in half the free device memory),
-        "keep", or "recompute" (each step from its start)."""
-        code = {"auto": 0, "keep": 1, "recompute": 2}[mode]
+        "keep", "recompute" (each step from its start), or "segments" (the
+        gas checkpointed with the particles every `every` steps, each
+        segment replayed from its checkpoint in the sweep)."""
+        code = {"auto": 0, "keep": 1, "recompute": 2, "segments": 3}[mode]
         self._ok(self.L.tpx_adjoint_set_gas_tape(self.h, code), "adjoint_set_gas_tape")
 
     def adjoint_release(self) -> None:
         self._ok(self.L.tpx_adjoint_release(self.h), "adjoint_release")
+
+    def adjoint_check(self, op: int, x: np.ndarray, y: np.ndarray):
+        """Transpose checks of the Poisson adjoint (tpx_adjoint_check): op 0
+        gives (M x, M y); op 1 gives (P_1(x), the adjoint of P for y, the
+        adjoint of Rho for y) for one iteration from P = x with a zero
+        right-hand side at the solver's Rho."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.shape)
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(self.shape)
+        ax, aty, atr = np.zeros(self.shape), np.zeros(self.shape), np.zeros(self.shape)
+        self._ok(self.L.tpx_adjoint_check(self.h, op, _dp(x), _dp(y), _dp(ax), _dp(aty), _dp(atr)), "adjoint_check")
+        return (ax, aty) if op == 0 else (ax, aty, atr)
 
     # ---- deposit tile shapes ----
     def tune_tiles(self, max_steps: int = 12) -> dict:
