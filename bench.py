@@ -79,6 +79,36 @@ def step_bytes(ncell, npart, n_it):
     return 4 * per_sub
 
 
+def step_kernel_bytes(ncell, npart, its_per_step, per_eps, per_tau, twc, heat, counted):
+    """Algorithmic bytes of one RK4 step as this build's kernels move them:
+    per substep the particle advance, the fused density/momentum update, the
+    divergence and Poisson right-hand side (6 FP64 a cell), the projection
+    (13 FP64 a cell; 7 at the last substep, where RU_new is RU_np1), the
+    deposits (heat: 44 B a particle + 36 B a cell, then 32 B a cell for the
+    y shifts; drag: 48 B a particle + 28 B a cell) and the binning (16 B a
+    particle for the placement, 12 B a cell + 12 B a particle for the
+    per-cell sort); per Poisson iteration its three passes (P, Rho_new, RHS
+    and the half spectrum; the spectrum twice; the spectrum, P read and
+    written); per step the re-sort of the particle arrays (140 B a
+    particle)."""
+    b = 0.0
+    for rk in range(4):
+        b += advance_bytes_per_launch(ncell, npart, rk, per_eps, per_tau, twc, counted) if npart else 0.0
+        b += grid_rhs_bytes_per_launch(ncell, rk, twc and npart > 0)
+        b += 8.0 * ncell * (6 + (7 if rk == 3 else 13))
+        if heat and npart:
+            b += 44.0 * npart + 36.0 * ncell + 32.0 * ncell
+        if twc and npart:
+            b += 48.0 * npart + 28.0 * ncell
+        if counted:
+            b += 16.0 * npart + 12.0 * ncell + 12.0 * npart
+    spec = (ncell ** (1.0 / 3.0) // 2 + 1) * ncell ** (2.0 / 3.0) * 16.0
+    b += its_per_step * ((3 * 8.0 * ncell + spec) + 2 * spec + (spec + 2 * 8.0 * ncell))
+    if counted:
+        b += 140.0 * npart
+    return b
+
+
 def load_traffic(workload):
     """Per-kernel HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE /
     WRITE_SIZE (scaled by a plain copy of known size) for this workload."""
@@ -456,8 +486,13 @@ def main():
     download_ms = (time.perf_counter() - t0) * 1e3
     log(f"[bench] full state download {download_ms:.1f} ms")
 
-    # whole-step algorithmic bandwidth with the iteration count of the run
+    # whole-step bandwidth: the bytes this build's kernels move by their
+    # algorithmic figures, and the survey's byte model of the reference's
+    # unfused passes (26 field passes a Poisson iteration), both at the
+    # iteration count of the run
     whole = step_bytes(ncell, npart, mean_its) * value / 1e9
+    kbytes = step_kernel_bytes(ncell, npart, 4 * mean_its, per_eps, per_tau, bool(p.TWC), p.Nu > 0, counted)
+    whole_k = kbytes * value / 1e9
     s.close()
     del s
 
@@ -493,7 +528,10 @@ def main():
         "roofline_next": others,
         "cpu_baseline": cpu,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
-        "step_algorithmic_GBps": round(whole, 1),
+        "step_kernel_bytes_GB": round(kbytes / 1e9, 2),
+        "step_GBps_kernel_bytes": round(whole_k, 1),
+        "step_frac_kernel_bytes": round(whole_k / HBM_PEAK_GBS, 4),
+        "step_GBps_survey_byte_model": round(whole, 1),
         "state_download_ms": round(download_ms, 1),
     }
     if cpu:

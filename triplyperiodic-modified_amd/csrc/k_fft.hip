@@ -304,7 +304,7 @@ struct XInvArgs {
   double *partials;
   double ncell;
   const double *done;
-  int overwrite = 0;  // three-pass form: P = dp (P unread) instead of P += dp
+  int overwrite = 0;  // three-pass form: P = dp (P unread, its own kernel instance) instead of P += dp
 };
 
 // X inverse pass: Hermitian extension, inverse FFT, P += dp, block max|dp|
@@ -1184,7 +1184,7 @@ struct k_xy_fwd_rp {
   }
 };
 
-template <int R1, int R2, int Y2>
+template <int R1, int R2, int Y2, bool Overwrite = false>
 struct k_xy_inv {
   static constexpr int kBounds = 256;
   static __device__ __forceinline__ void run(const XInvArgs &a, int Y1, const double2 *__restrict__ twy) {
@@ -1259,7 +1259,8 @@ struct k_xy_inv {
     for (int k1 = 0; k1 < R1; ++k1) {
       double dp = (-1) * u[k1].x / (a.ncell);
       double *Pp = Pl + k2x + R2 * k1;
-      *Pp = a.overwrite ? dp : *Pp + dp;
+      if constexpr (Overwrite) *Pp = dp;
+      else *Pp = *Pp + dp;
       mx = fmax(mx, fabs(dp));
     }
   }
@@ -1424,13 +1425,18 @@ void xy_fwd_rp_launch(unsigned nb, size_t sh, hipStream_t st, const XArgs &xa, i
     default: tpx_launch<k_xy_fwd_rp<R2H, 16>>(nb, 16 * 16, sh, st, xa, Y1, twy); break;
   }
 }
+template <int R1, int R2, bool Ow>
+void xy_inv_launch_t(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, int Y1, int Y2, const double2 *twy) {
+  switch (Y2) {
+    case 4: tpx_launch<k_xy_inv<R1, R2, 4, Ow>>(nb, 4 * R1, sh, st, xi, Y1, twy); break;
+    case 8: tpx_launch<k_xy_inv<R1, R2, 8, Ow>>(nb, 8 * R1, sh, st, xi, Y1, twy); break;
+    default: tpx_launch<k_xy_inv<R1, R2, 16, Ow>>(nb, 16 * R1, sh, st, xi, Y1, twy); break;
+  }
+}
 template <int R1, int R2>
 void xy_inv_launch(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, int Y1, int Y2, const double2 *twy) {
-  switch (Y2) {
-    case 4: tpx_launch<k_xy_inv<R1, R2, 4>>(nb, 4 * R1, sh, st, xi, Y1, twy); break;
-    case 8: tpx_launch<k_xy_inv<R1, R2, 8>>(nb, 8 * R1, sh, st, xi, Y1, twy); break;
-    default: tpx_launch<k_xy_inv<R1, R2, 16>>(nb, 16 * R1, sh, st, xi, Y1, twy); break;
-  }
+  if (xi.overwrite) xy_inv_launch_t<R1, R2, true>(nb, sh, st, xi, Y1, Y2, twy);
+  else xy_inv_launch_t<R1, R2, false>(nb, sh, st, xi, Y1, Y2, twy);
 }
 template <int Z1, int Z2>
 void yz_launch(unsigned nb, size_t sh, hipStream_t st, double2 *spec, int mxh, int Y1, int Y2, const tpx_ctx *c,
