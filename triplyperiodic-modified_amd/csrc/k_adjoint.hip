@@ -237,22 +237,63 @@ struct AdjArgs {
   long np, ncell;
 };
 
-// The transpose of the gather: lug, the adjoint of u_g at x, goes to the two
-// faces each component was interpolated from, with the gather's weights.
-__device__ __forceinline__ void gather_transpose(const Geo &g, const Ix &ix, const double x[3], const double lug[3],
-                                                 double *lup, long n) {
-  int ii = (int)(x[0] * g.idx), jj = (int)(x[1] * g.idy), kk = (int)(x[2] * g.idz);
-  double xl = (x[0] * g.idx - ((int)(x[0] * g.idx)));
-  double yl = (x[1] * g.idy - ((int)(x[1] * g.idy)));
-  double zl = (x[2] * g.idz - ((int)(x[2] * g.idz)));
-  long c000 = ix(ii, jj, kk);
-  atomicAdd(lup + c000, (1 - xl) * lug[0]);
-  atomicAdd(lup + ix(ii + 1, jj, kk), xl * lug[0]);
-  atomicAdd(lup + n + c000, (1 - yl) * lug[1]);
-  atomicAdd(lup + n + ix(ii, jj + 1, kk), yl * lug[1]);
-  atomicAdd(lup + 2 * n + c000, (1 - zl) * lug[2]);
-  atomicAdd(lup + 2 * n + ix(ii, jj, kk + 1), zl * lug[2]);
-}
+// The transpose of the gather (pull form): lug, the adjoint of u_g at each
+// particle (3 x np), goes to the two faces each component was interpolated
+// from, with the gather's weights.  One thread per cell sums the x, y and z
+// faces at its index: from the particles whose gather cell (the floor of
+// x idx, y idy, z idz) is this cell, weight 1 - fraction, then from those of
+// the cell below along that axis, weight fraction; bins in cell order, each
+// cell's particles by index, so the sums do not depend on the run.
+struct k_adj_gather_pull {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const Geo &g, const double *const Y, const double *lug,
+                                             const int32_t *start, const int32_t *perm, double *lup, long np) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= g.n) return;
+    Ix ix = make_ix(g);
+    const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / g.nxy);
+    const double id[3] = {g.idx, g.idy, g.idz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double *pos = Y + (long)d * np, *l = lug + (long)d * np;
+      double acc = 0.0;
+      for (int32_t q = start[c]; q < start[c + 1]; ++q) {
+        const int32_t p = perm[q];
+        const double f = pos[p] * id[d];
+        acc += (1 - (f - ((int)f))) * l[p];
+      }
+      const long cm = ix(i - (d == 0), j - (d == 1), k - (d == 2));
+      for (int32_t q = start[cm]; q < start[cm + 1]; ++q) {
+        const int32_t p = perm[q];
+        const double f = pos[p] * id[d];
+        acc += (f - ((int)f)) * l[p];
+      }
+      lup[(long)d * g.n + c] = acc;
+    }
+  }
+};
+
+// Keys and counts of the bins of positions (x, y, z rows of Y): the gather's
+// cell (floor), or with cic set the cloud-in-cell base corner (round - 1)
+struct k_adj_bin_keys {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const Geo &g, const double *Y, int cic, int32_t *key, int32_t *cnt,
+                                             long np) {
+    const long p = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (p >= np) return;
+    Ix ix = make_ix(g);
+    const double fx = Y[p] * g.idx, fy = Y[np + p] * g.idy, fz = Y[2 * np + p] * g.idz;
+    int ii, jj, kk;
+    if (cic) {
+      ii = (int)(round(fx)) - 1; jj = (int)(round(fy)) - 1; kk = (int)(round(fz)) - 1;
+    } else {
+      ii = (int)fx; jj = (int)fy; kk = (int)fz;
+    }
+    const int32_t kc = (int32_t)ix(ii, jj, kk);
+    key[p] = kc;
+    atomicAdd(cnt + kc, 1);
+  }
+};
 
 // The transposed RK4 step for one particle per thread.
 struct k_adj_step_rev {
@@ -311,7 +352,7 @@ struct StageArgs {
   const double *lend;    // nq x np: adjoint of the step's result
   double *lb, *lnext;    // nq x np each: base accumulator; adjoint of the next stage's state in, of this one's out
   double *dtau;
-  double *lup;           // the gather's transpose of dJ/du_g: 3 face fields of this substep, or null
+  double *lug;           // out, 3 x np: the adjoint of u_g at each particle (for the gather's transpose), or null
   const double *ls[3];   // lam_RHS of this substep's momentum update (3 face fields), or null
   double a_drag;         // -mp/Vcell
   const double *tau;
@@ -385,7 +426,9 @@ struct k_adj_stage_rev {
       a.lh[p] = a.iS * lk[6];
       a.deps[p] += a.iS * lk[6] * a.alpha_eps;
     }
-    if (a.lup) gather_transpose(g, ix, Y, lug, a.lup, a.ncell);
+    if (a.lug)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) a.lug[d * a.np + p] = lug[d];
   }
 };
 
@@ -402,7 +445,7 @@ struct HeatRevArgs {
   const double *lsq;     // adjoint of sum Q (device scalar)
   const double *lh;      // per particle: adjoint of RHS_T through the temperature update
   double *lam;
-  double *lrho;          // adjoint of Rho, accumulated (atomics)
+  double *gt;            // out, per particle: the adjoint of the gathered gas temperature
   double *partials;
   double K, R;
   long np;
@@ -437,8 +480,7 @@ struct k_adj_heat {
 #pragma unroll
       for (int d = 0; d < 3; ++d) a.lam[d * a.np + p] += (gT * dT[d] + h * dq[d]) * id[d];
       a.lam[6 * a.np + p] -= gT;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) atomicAdd(a.lrho + w.c[e], -gT * w.w(e) * T[e] / a.rho[w.c[e]]);
+      a.gt[p] = gT;
       lp0 = gT * Tg / P0;
     }
     __shared__ double sh[kBlock];
@@ -449,6 +491,35 @@ struct k_adj_heat {
       __syncthreads();
     }
     if (threadIdx.x == 0) a.partials[blockIdx.x] = sh[0];
+  }
+};
+
+// The exchange's corner gather transposed into lam_Rho (pull form): the
+// gathered T = (P0/R)/Rho at the eight corners, so each corner cell e of a
+// particle gets -gT w_e T_e / Rho_e.  One thread per cell sums, for the eight
+// corner offsets in turn, the particles whose cloud-in-cell base corner sits
+// that offset below it (bins in cell order, each cell's by index).
+struct k_adj_heat_pull {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const Geo &g, const double *Y, const double *gt, const double *rho,
+                                             const double *p0, double R, const int32_t *start, const int32_t *perm,
+                                             double *lrho, long np) {
+    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    if (c >= g.n) return;
+    Ix ix = make_ix(g);
+    const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / g.nxy);
+    const double rc = rho[c], Tc = (p0[0] / R) / rc;
+    double acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const long cb = ix(i - (e & 1), j - ((e >> 1) & 1), k - (e >> 2));
+      for (int32_t q = start[cb]; q < start[cb + 1]; ++q) {
+        const int32_t p = perm[q];
+        const Cic w(g, ix, Y[p], Y[np + p], Y[2 * np + p]);
+        acc += -gt[p] * w.w(e) * Tc / rc;
+      }
+    }
+    lrho[c] += acc;
   }
 };
 
@@ -675,6 +746,17 @@ double adjoint_objective(tpx_ctx *c) {
   return s[1] / (double)n - mean * mean;
 }
 
+// Bins of the positions of the state Y (rows x, y, z of nq x np) into the
+// tape's bin arrays: by the gather's cell, or (cic) the cloud-in-cell base corner.
+void bin_positions(tpx_ctx *c, const double *Y, bool cic) {
+  AdjointTape &t = c->adj;
+  const long np = c->pt.n;
+  launch_zero_counts(c, t.bstart);
+  tpx_launch<k_adj_bin_keys>(grid1(np), kBlock, 0, c->stream, c->g, Y, (int)cic, t.bkey, t.bstart, np);
+  TPX_LAUNCH_CHECK();
+  launch_bin_by_key(c, t.bkey, t.bperm, t.binv, t.bstart);
+}
+
 // The reverse of one heat exchange at the state Y (nq x np, the recording's
 // order) into lam; its lam_P0 is added to scal[slot].
 void heat_reverse(tpx_ctx *c, const double *Y, const GasSweep::HeatMid &m, const double *p0, double *lam, int slot) {
@@ -688,7 +770,7 @@ void heat_reverse(tpx_ctx *c, const double *Y, const GasSweep::HeatMid &m, const
   a.lsq = c->scal + SC_ADJ_LSQ;
   a.lh = t.lh;
   a.lam = lam;
-  a.lrho = m.lrho;
+  a.gt = t.gt;
   a.partials = t.hpart;
   a.K = c->p.nu * TPX_PI * c->p.dp * c->p.k;
   a.R = c->R;
@@ -697,6 +779,10 @@ void heat_reverse(tpx_ctx *c, const double *Y, const GasSweep::HeatMid &m, const
   tpx_launch<k_adj_heat>(nb, kBlock, 0, c->stream, a);
   TPX_LAUNCH_CHECK();
   tpx_launch<k_adj_acc1>(1, 1024, 0, c->stream, (const double *)t.hpart, (long)nb, c->scal + slot);
+  TPX_LAUNCH_CHECK();
+  bin_positions(c, Y, true);
+  tpx_launch<k_adj_heat_pull>(grid1d(c->g.n, kBlock), kBlock, 0, c->stream, c->g, Y, (const double *)t.gt, m.rho, p0,
+                              c->R, (const int32_t *)t.bstart, (const int32_t *)t.bperm, m.lrho, a.np);
   TPX_LAUNCH_CHECK();
 }
 
@@ -727,6 +813,15 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
   if (gas && !t.lb) {
     if (hipMalloc((void **)&t.lb, sizeof(double) * nq * (size_t)np) != hipSuccess ||
         hipMalloc((void **)&t.lnext, sizeof(double) * nq * (size_t)np) != hipSuccess)
+      throw std::bad_alloc();
+  }
+  if (gas && !t.lug) {  // the pull-form transposes: per-particle adjoints and position bins
+    if (hipMalloc((void **)&t.lug, sizeof(double) * 3 * (size_t)np) != hipSuccess ||
+        hipMalloc((void **)&t.gt, sizeof(double) * (size_t)np) != hipSuccess ||
+        hipMalloc((void **)&t.bkey, sizeof(int32_t) * (size_t)np) != hipSuccess ||
+        hipMalloc((void **)&t.bperm, sizeof(int32_t) * (size_t)np) != hipSuccess ||
+        hipMalloc((void **)&t.binv, sizeof(int32_t) * (size_t)np) != hipSuccess ||
+        hipMalloc((void **)&t.bstart, sizeof(int32_t) * ((size_t)c->g.n + 1)) != hipSuccess)
       throw std::bad_alloc();
   }
   const int seg_len = std::min(t.every, t.nsteps);
@@ -792,8 +887,7 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
       a.lb = t.lb;
       a.lnext = t.lnext;
       a.dtau = t.dtau;
-      a.lup = t.lup;
-      launch_fill(c, t.lup, 0.0, 3 * c->g.n);
+      a.lug = t.lug;
       for (int d = 0; d < 3; ++d) a.ls[d] = gas->a_drag != 0.0 ? lrhs[d] : nullptr;
       a.a_drag = gas->a_drag;
       a.tau = pt.has_tau ? t.tau0 : nullptr;
@@ -811,6 +905,12 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
       a.np = np;
       a.ncell = c->g.n;
       tpx_launch<k_adj_stage_rev>(grid1(np), kBlock, 0, c->stream, a);
+      TPX_LAUNCH_CHECK();
+      // the gather's transpose of lug onto the faces, pulled per cell
+      bin_positions(c, a.stage, false);
+      tpx_launch<k_adj_gather_pull>(grid1d(c->g.n, kBlock), kBlock, 0, c->stream, c->g, a.stage,
+                                    (const double *)t.lug, (const int32_t *)t.bstart, (const int32_t *)t.bperm,
+                                    t.lup, np);
       TPX_LAUNCH_CHECK();
       gas->part_b(s, rk, t.lup);
     }

@@ -835,6 +835,12 @@ static void zero_counts(tpx_ctx *c, int32_t *cnt) {
   TPX_LAUNCH_CHECK();
 }
 
+void launch_bin_by_key(tpx_ctx *c, const int32_t *key, int32_t *perm, int32_t *inv, int32_t *start) {
+  bin_particles(c, key, perm, inv, start);
+}
+
+void launch_zero_counts(tpx_ctx *c, int32_t *cnt) { zero_counts(c, cnt); }
+
 void launch_particle_advance(tpx_ctx *c, const Coef &k, const double *const ru_int[3],
                              const double *rho_int, bool count_new) {
   Particles &pt = c->pt;

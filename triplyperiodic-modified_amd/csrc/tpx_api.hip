@@ -546,6 +546,8 @@ void adjoint_free(AdjointTape &t) {
   t.swept = false;
   auto fi = [](int32_t *&p) { if (p) (void)hipFree(p); p = nullptr; };
   fi(t.cum); fi(t.cum_tmp);
+  f(t.lug); f(t.gt);
+  fi(t.bkey); fi(t.bperm); fi(t.binv); fi(t.bstart);
   t.store = false;
   for (auto &p : t.pcap) f(p);
   t.pcap.clear();

@@ -181,6 +181,12 @@ struct AdjointTape {
   double *deps = nullptr;        // np: dJ/deps
   double *lh = nullptr;          // np: adjoint of RHS_T through the next particle temperature update
   double *hpart = nullptr;       // block partials of the exchange's reversal (one per 256 particles)
+  // the pull-form transposes of the gather (face fields) and of the exchange's
+  // corner gather (lam_Rho): per-particle adjoints, and bins of the positions
+  double *lug = nullptr;         // 3 x np: adjoint of the gathered velocity at the stage's positions
+  double *gt = nullptr;          // np: adjoint of the gathered gas temperature of an exchange
+  int32_t *bkey = nullptr, *bperm = nullptr, *binv = nullptr;  // np each
+  int32_t *bstart = nullptr;     // cells + 1
   double *scal_save = nullptr;   // the Update_P0 slots of the run's end, kept over a recomputing sweep
   std::vector<double> I0;        // lamp intensity of each step
   bool swept = false;            // a sweep has run since the recording
@@ -364,6 +370,12 @@ void launch_concentration(tpx_ctx *c);
 void launch_particle_permute(tpx_ctx *c);
 void launch_sort_stage(tpx_ctx *c, int stage, bool into_int);
 void launch_iota(tpx_ctx *c, int32_t *p, int64_t n);
+// Deterministic cell bins of the particles' keys (the particle path's
+// binning): start (cells + 1) holds each cell's count on entry and its first
+// slot on return, perm the particles in cell order (each cell's by index),
+// inv its inverse.  launch_zero_counts clears the cells + 1 counts.
+void launch_bin_by_key(tpx_ctx *c, const int32_t *key, int32_t *perm, int32_t *inv, int32_t *start);
+void launch_zero_counts(tpx_ctx *c, int32_t *cnt);
 void launch_exclusive_scan(tpx_ctx *c, int32_t *data, int64_t n);  // in place, n+1 output
 
 // k_adjoint.hip
