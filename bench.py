@@ -238,11 +238,27 @@ def ensemble_measure(steps, warmup, with_streams=True):
     for s in solvers:
         s.synchronize()
     t_bat = time.perf_counter() - t0
-    t_str = None
+    t_str = t_ng = None
     if with_streams:
         t0 = time.perf_counter()
         tpx.ensemble_step(solvers, steps, 4)
         t_str = time.perf_counter() - t0
+        # the same batch with its segments launched kernel by kernel instead
+        # of replayed from captured graphs
+        os.environ["TPX_BATCH_GRAPH"] = "0"
+        try:
+            b2 = tpx.Batch(solvers)
+        finally:
+            del os.environ["TPX_BATCH_GRAPH"]
+        b2.step(1)
+        for s in solvers:
+            s.synchronize()
+        t0 = time.perf_counter()
+        b2.step(steps)
+        for s in solvers:
+            s.synchronize()
+        t_ng = time.perf_counter() - t0
+        b2.close()
     batch.close()
     for s in solvers:
         s.close()
@@ -260,6 +276,9 @@ def ensemble_measure(steps, warmup, with_streams=True):
            "sequential_cells_per_s": round(seq, 1), "speedup_vs_sequential": round(value / seq, 3)}
     if t_str:
         out["four_streams_cells_per_s"] = round(ncells * steps / t_str, 1)
+    if t_ng:
+        out["batched_without_graphs_cells_per_s"] = round(ncells * steps / t_ng, 1)
+        log(f"[bench] ensemble batched without graphs {t_ng:.3f} s")
     return out
 
 

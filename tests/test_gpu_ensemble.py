@@ -41,12 +41,17 @@ def test_tuner_returns_a_candidate(gpu_available):
     assert c in (1, 2, 4)
 
 
+@pytest.mark.parametrize("graphs", ["1", "0"])
 @pytest.mark.parametrize("heat", [False, True])
-def test_batched_boxes_equal_their_own_runs(gpu_available, heat):
+def test_batched_boxes_equal_their_own_runs(gpu_available, heat, graphs, monkeypatch):
     """One launch per kernel for all boxes (box in the grid's third dimension,
     arguments from a table): boxes with different particle counts, viscosities,
     Stokes numbers and time steps each equal their own solo run bit for bit,
-    Poisson iteration counts included."""
+    Poisson iteration counts included.  With graphs "1" (the default) the
+    segments between the solves are replayed from captured graphs (each
+    captured once, replayed at every later step, with the host's own work and
+    waits between the replays); with "0" they are launched kernel by kernel."""
+    monkeypatch.setenv("TPX_BATCH_GRAPH", graphs)
     cases = ic.ensemble_cases(n_boxes=5, n=16, seed=7, np_range=(200, 3000))
     if heat:
         cases = [ic.make_case(16, npart, nu=0.01 + 0.004 * b, seed=20 + b, St=0.5 + b, heat=True, nsteps=4)

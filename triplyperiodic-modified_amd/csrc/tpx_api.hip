@@ -1201,6 +1201,18 @@ struct tpx_batch {
   size_t iter_cap = 0;
   hipGraphExec_t iter_exec = nullptr;
   std::vector<char> iter_sig;  // the launches the graph was captured from
+  // the segments between the solves (a substep's kernels before and after
+  // its solve, the step's end) as captured graphs, one per distinct launch
+  // sequence; each keeps its argument table at a fixed device address,
+  // refreshed (stream-ordered) before each replay
+  struct Seg {
+    std::vector<char> sig;
+    char *h = nullptr, *d = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t copied = nullptr;  // the last table copy has read h
+  };
+  std::vector<Seg> segs;
+  bool graphs = true;  // TPX_BATCH_GRAPH=0: launch the segments kernel by kernel
   std::string err;
 };
 
@@ -1218,6 +1230,12 @@ void batch_release(tpx_batch *b) {
   if (b->d_state) (void)hipFree(b->d_state);
   if (b->h_state) (void)hipHostFree(b->h_state);
   if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
+  for (auto &e : b->segs) {
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    if (e.h) (void)hipHostFree(e.h);
+    if (e.d) (void)hipFree(e.d);
+    if (e.copied) (void)hipEventDestroy(e.copied);
+  }
   if (b->h_iter) (void)hipHostFree(b->h_iter);
   if (b->d_iter) (void)hipFree(b->d_iter);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1318,6 +1336,61 @@ void batch_run(tpx_batch *b, Fn &&fn) {
   hip_check(hipMemcpyAsync(b->d_tab[s], b->h_tab[s], plan.total, hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
   hip_check(hipEventRecord(b->ev[s], b->stream), "hipEventRecord");
   batch_launch(b, plan, b->d_tab[s]);
+}
+
+// the launch sequence of a plan (what a graph captured from it depends on)
+std::vector<char> plan_signature(const BatchPlan &plan) {
+  std::vector<char> sig(sizeof(BatchPlan::K) * plan.k.size() + sizeof(size_t));
+  for (size_t j = 0; j < plan.k.size(); ++j) {
+    BatchPlan::K k = plan.k[j];
+    memcpy(sig.data() + j * sizeof(k), &k, sizeof(k));
+  }
+  memcpy(sig.data() + sizeof(BatchPlan::K) * plan.k.size(), &plan.total, sizeof(size_t));
+  return sig;
+}
+
+// batch_run as a graph replay: the segment's graph (captured the first time
+// its launch sequence occurs) launched after a copy of this occurrence's
+// argument table to the graph's fixed table
+template <class Fn>
+void batch_run_graph(tpx_batch *b, Fn &&fn) {
+  if (!b->graphs) {
+    batch_run(b, fn);
+    return;
+  }
+  BatchPlan plan;
+  batch_record(b, fn, plan);
+  if (plan.k.empty()) return;
+  std::vector<char> sig = plan_signature(plan);
+  tpx_batch::Seg *e = nullptr;
+  for (auto &x : b->segs)
+    if (x.sig == sig) e = &x;
+  if (!e) {
+    b->segs.emplace_back();
+    e = &b->segs.back();
+    e->sig = sig;
+    hip_check(hipHostMalloc((void **)&e->h, plan.total), "hipHostMalloc");
+    if (hipMalloc((void **)&e->d, plan.total) != hipSuccess) throw std::bad_alloc();
+    hip_check(hipEventCreateWithFlags(&e->copied, hipEventDisableTiming), "hipEventCreate");
+    hipGraph_t graph = nullptr;
+    hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      batch_launch(b, plan, e->d);
+    } catch (...) {
+      (void)hipStreamEndCapture(b->stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(b->stream, &graph), "hipStreamEndCapture");
+    const hipError_t err = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    hip_check(err, "hipGraphInstantiate");
+  }
+  hip_check(hipEventSynchronize(e->copied), "hipEventSynchronize");  // the last copy from e->h is done
+  batch_pack(b, plan, e->h);
+  hip_check(hipMemcpyAsync(e->d, e->h, plan.total, hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
+  hip_check(hipEventRecord(e->copied, b->stream), "hipEventRecord");
+  hip_check(hipGraphLaunch(e->exec, b->stream), "hipGraphLaunch");
 }
 
 // The Poisson iteration of every box, its table refreshed for this solve and
@@ -1723,6 +1796,7 @@ int tpx_batch_create(tpx_ctx *const *ctxs, int n, tpx_batch **out) {
     b->ctx.assign(ctxs, ctxs + n);
     b->rec.resize(n);
     b->device = a->device;
+    if (const char *e = getenv("TPX_BATCH_GRAPH")) b->graphs = atoi(e) != 0;
     hip_check(hipSetDevice(b->device), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto &e : b->ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
@@ -1753,11 +1827,11 @@ int tpx_batch_step(tpx_batch *b, int nsteps) {
     for (int s = 0; s < nsteps; ++s) {
       for (tpx_ctx *c : b->ctx) do_params_update(c);
       for (int rk = 0; rk < 4; ++rk) {
-        batch_run(b, [rk](tpx_ctx *c, int) { fused_substep_pre(c, rk); });
+        batch_run_graph(b, [rk](tpx_ctx *c, int) { fused_substep_pre(c, rk); });
         batch_solve(b, rk);
-        batch_run(b, [rk](tpx_ctx *c, int) { fused_substep_post(c, rk); });
+        batch_run_graph(b, [rk](tpx_ctx *c, int) { fused_substep_post(c, rk); });
       }
-      batch_run(b, [](tpx_ctx *c, int) { step_end(c); });
+      batch_run_graph(b, [](tpx_ctx *c, int) { step_end(c); });
     }
     hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
   } catch (const std::logic_error &e) {
