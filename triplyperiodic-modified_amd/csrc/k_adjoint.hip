@@ -825,36 +825,56 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
       throw std::bad_alloc();
   }
   const int seg_len = std::min(t.every, t.nsteps);
-  if (!t.seg && hipMalloc((void **)&t.seg, sizeof(double) * nq * (size_t)np * seg_len) != hipSuccess)
+  // the start of every step of the segment and the result of its last; when
+  // memory allows, also the four stage states of every step of the segment,
+  // kept from the segment's one recomputation (else each step's stages are
+  // recomputed from its start when it is reversed)
+  if (!t.seg && hipMalloc((void **)&t.seg, sizeof(double) * nq * (size_t)np * (seg_len + 1)) != hipSuccess)
     throw std::bad_alloc();
+  if (!t.segstage && !t.segstage_tried) {
+    t.segstage_tried = true;
+    if (hipMalloc((void **)&t.segstage, sizeof(double) * 4 * nq * (size_t)np * seg_len) != hipSuccess) {
+      (void)hipGetLastError();
+      t.segstage = nullptr;
+    }
+  }
   const double *in[7] = {};
   double *out[7] = {};
   auto start_of = [&](int j, int q) { return t.seg + ((long)j * nq + q) * np; };
+  auto stages_of = [&](int j) { return t.segstage + (long)j * 4 * nq * np; };
   int seg_k = -1;
   for (int s = t.nsteps - 1; s >= 0; --s) {
     const int k0 = s / t.every, base = k0 * t.every;
     if (k0 != seg_k) {
       if (gas && gas->segment) gas->segment(k0);  // the segment's gas tape, replayed from its checkpoint
       for (int q = 0; q < nq; ++q) launch_copy(c, start_of(0, q), t.ckpt + ((long)k0 * nq + q) * np, np);
-      for (int r = base; r < s; ++r) {
+      const int last = t.segstage ? s : s - 1;
+      for (int r = base; r <= last; ++r) {
         for (int q = 0; q < nq; ++q) {
           in[q] = start_of(r - base, q);
           out[q] = start_of(r - base + 1, q);
         }
-        tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, r, in, out, nullptr));
+        tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream,
+                                   step_args(c, r, in, out, t.segstage ? stages_of(r - base) : nullptr));
         TPX_LAUNCH_CHECK();
       }
       seg_k = k0;
     }
-    // the stage states of step s
-    for (int q = 0; q < nq; ++q) in[q] = start_of(s - base, q);
-    tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, s, in, y, t.stage));
-    TPX_LAUNCH_CHECK();
+    // the stage states of step s and its result
+    double *stg = t.stage, *yres = t.y;
+    if (t.segstage) {
+      stg = stages_of(s - base);
+      yres = start_of(s - base + 1, 0);
+    } else {
+      for (int q = 0; q < nq; ++q) in[q] = start_of(s - base, q);
+      tpx_launch<k_adj_step_fwd>(grid1(np), kBlock, 0, c->stream, step_args(c, s, in, y, t.stage));
+      TPX_LAUNCH_CHECK();
+    }
     if (!gas) {
       AdjArgs a{};
       a.g = c->g;
       a.gas = t.gas_of(s, c->g.n);
-      a.stage = t.stage;
+      a.stage = stg;
       for (int q = 0; q < 6; ++q) a.lam[q] = lam[q];
       a.dtau = t.dtau;
       a.tau = pt.has_tau ? t.tau0 : nullptr;
@@ -874,7 +894,7 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
     for (int rk = 3; rk >= 0; --rk) {
       gas->part_a(s, rk);
       if (t.heat) {  // the exchange at the end of this substep, at the next stage's state
-        const double *Y = rk < 3 ? t.stage + (long)(rk + 1) * nq * np : t.y;
+        const double *Y = rk < 3 ? stg + (long)(rk + 1) * nq * np : yres;
         heat_reverse(c, Y, gas->heat_mid(s, rk), t.p0t + (long)s * 4 + rk + 1, rk < 3 ? t.lnext : t.lam,
                      SC_ADJ_P0NEW);
       }
@@ -882,7 +902,7 @@ void adjoint_reverse(tpx_ctx *c, const GasSweep *gas) {
       StageArgs a{};
       a.g = c->g;
       a.gas = t.gas_of(s, c->g.n) + (long)rk * 4 * c->g.n;
-      a.stage = t.stage + (long)rk * nq * np;
+      a.stage = stg + (long)rk * nq * np;
       a.lend = t.lam;
       a.lb = t.lb;
       a.lnext = t.lnext;

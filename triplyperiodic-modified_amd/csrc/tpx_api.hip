@@ -546,7 +546,8 @@ void adjoint_free(AdjointTape &t) {
   t.swept = false;
   auto fi = [](int32_t *&p) { if (p) (void)hipFree(p); p = nullptr; };
   fi(t.cum); fi(t.cum_tmp);
-  f(t.lug); f(t.gt);
+  f(t.lug); f(t.gt); f(t.segstage);
+  t.segstage_tried = false;
   fi(t.bkey); fi(t.bperm); fi(t.binv); fi(t.bstart);
   t.store = false;
   for (auto &p : t.pcap) f(p);
@@ -1532,9 +1533,8 @@ void gas_substeps(tpx_ctx *c, int s, GasSubstep sub[4], bool segment) {
     for (int d = 0; d < 3; ++d) g.ru_int[d] = gs + d * n;
     g.rho_int = gs + 3 * n;
     const size_t q = (size_t)s * 4 + rk;
-    if (t.heat) {  // Update_P0's P0_new, and the substep's deposit for the divergence
+    if (t.heat) {  // Update_P0's P0_new, and the substep's deposit for the divergence (read from the tape)
       launch_copy(c, c->scal + SC_P0_NEW, t.p0t + q + 1, 1);
-      launch_copy(c, c->Q, t.qt + (long)q * n, n);
       launch_copy(c, c->scal + SC_SUM_Q, t.qs + q, 1);
       c->q_zero = false;
       g.Q = t.qt + (long)q * n;
@@ -1543,9 +1543,21 @@ void gas_substeps(tpx_ctx *c, int s, GasSubstep sub[4], bool segment) {
     } else {
       launch_update_p0(c, k);
     }
-    if (t.S)  // two-way coupling: the recorded drag deposit of this substep
-      for (int d = 0; d < 3; ++d) launch_copy(c, c->S[d], t.S + (((long)s * 4 + rk) * 3 + d) * n, n);
-    do_update_ru_wop(c, rk, true);
+    {
+      // two-way coupling: the momentum update reads the recorded drag deposit
+      // of this substep in place (the context's S is pointed at the tape for
+      // the launch; nothing in a gas-only step writes S)
+      double *keep[3] = {c->S[0], c->S[1], c->S[2]};
+      if (t.S)
+        for (int d = 0; d < 3; ++d) c->S[d] = t.S + (((long)s * 4 + rk) * 3 + d) * n;
+      try {
+        do_update_ru_wop(c, rk, true);
+      } catch (...) {
+        for (int d = 0; d < 3; ++d) c->S[d] = keep[d];
+        throw;
+      }
+      for (int d = 0; d < 3; ++d) c->S[d] = keep[d];
+    }
     double *cp = segment ? t.sub + ((long)j * 4 + rk) * 5 * n : t.cap + (long)rk * 5 * n;
     for (int d = 0; d < 3; ++d) {
       launch_copy(c, cp + d * n, c->ru[c->ru_new][d], n);
@@ -1555,7 +1567,7 @@ void gas_substeps(tpx_ctx *c, int s, GasSubstep sub[4], bool segment) {
     g.rho_new = cp + 3 * n;
     const double *rn[3];
     ru_ptrs(c, c->ru_new, rn);
-    launch_div_rhs(c, k, 0, rn, c->rho[c->rho_new], c->q_zero ? nullptr : c->Q, nullptr, c->rhs);
+    launch_div_rhs(c, k, 0, rn, c->rho[c->rho_new], t.heat ? g.Q : (c->q_zero ? nullptr : c->Q), nullptr, c->rhs);
     launch_copy(c, cp + 4 * n, c->rhs, n);
     g.rhs = cp + 4 * n;
     double *sum = segment ? t.sub + (long)t.tape_steps() * 20 * n + (long)j * 4 + rk : t.cap + 20 * n + rk;

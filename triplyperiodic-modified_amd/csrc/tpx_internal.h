@@ -142,7 +142,9 @@ struct AdjointTape {
   std::vector<double> g3;        // particle gravity of each step (3 per step)
   double *ckpt = nullptr;        // (nsteps / every + 1) x nq x np
   double *y = nullptr;           // nq x np
-  double *seg = nullptr;         // min(every, nsteps) x nq x np: step starts of the segment being reversed
+  double *seg = nullptr;         // (min(every, nsteps) + 1) x nq x np: step starts of the segment being reversed
+  double *segstage = nullptr;    // min(every, nsteps) x 4 x nq x np: their stage states (null: recomputed per step)
+  bool segstage_tried = false;
   double *lb = nullptr, *lnext = nullptr;  // nq x np each: the staged sweep's base and next-stage adjoints
   double *S = nullptr;           // two-way coupling: the drag deposit of every substep (nsteps x 4 x 3 n)
   // particle order: with re-binning (two-way coupling) the particle arrays are
