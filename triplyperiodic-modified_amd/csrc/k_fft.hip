@@ -937,7 +937,23 @@ struct k_yz_pass {
   const int k2 = (int)(blk / (unsigned)mxh), kx = (int)(blk % (unsigned)mxh);
   double2 *base = spec + (long)blk * (NE + kRunPad);  // element (z, t) at z Y1 + t
   const int tid = threadIdx.x;
-  for (int e = tid; e < NE; e += block_dim_x()) L[pad(e)] = base[e];
+  // the run into LDS: every load of a thread issued before the first LDS
+  // write (a loop over the block width read at run time waited on each load
+  // in turn); yz_launch starts kBounds threads a block
+  constexpr int NU = (NE + kBounds - 1) / kBounds;
+  {
+    double2 r[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int e = tid + u * kBounds;
+      if (NE % kBounds == 0 || e < NE) r[u] = base[e];
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int e = tid + u * kBounds;
+      if (NE % kBounds == 0 || e < NE) L[pad(e)] = r[u];
+    }
+  }
   __syncthreads();
   // y step B: item z, DFT over t
   const bool zi = tid < nz;
@@ -1034,7 +1050,11 @@ struct k_yz_pass {
     for (int q = 0; q < Y1; ++q) L[pad(tid * Y1 + q)] = u[q];
   }
   __syncthreads();
-  for (int e = tid; e < NE; e += block_dim_x()) base[e] = L[pad(e)];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int e = tid + u * kBounds;
+    if (NE % kBounds == 0 || e < NE) base[e] = L[pad(e)];
+  }
 }
 };
 
