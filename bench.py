@@ -116,7 +116,18 @@ def load_traffic(workload):
     if not os.path.exists(path):
         return {}
     with open(path) as fh:
-        return json.load(fh).get("kernels", {})
+        kernels = json.load(fh).get("kernels", {})
+    # by kernel name without template arguments: the dispatch-weighted mean
+    # over its instances (e.g. the momentum kernel's rk 0-2 and rk 3 forms)
+    out = {}
+    for name, e in kernels.items():
+        base = name.split("<")[0]
+        a = out.setdefault(base, {"dispatches": 0, "traffic_bytes": 0.0})
+        a["traffic_bytes"] += e["traffic_bytes"] * e["dispatches"]
+        a["dispatches"] += e["dispatches"]
+    for a in out.values():
+        a["traffic_bytes"] /= max(a["dispatches"], 1)
+    return out
 
 
 def cpu_baseline(case, threads, full=False):
@@ -474,8 +485,11 @@ def main():
     # solve has stopped return at once, so a pass's time is divided by the
     # iterations that ran
     spec = (p.Nx_tot // 2 + 1) * p.Ny_tot * p.Nz_tot * 16.0
+    # pass 1 at x lengths 128 and 256 is the packed real-row form unless TPX_XPACK=0
+    packed = p.Nx_tot in (128, 256) and os.environ.get("TPX_XPACK", "1") != "0"
     fft = {
-        "fft": ("k_xy_fwd", "Poisson iteration pass 1: residual, x transform, first y step", 3 * 8.0 * ncell + spec),
+        "fft": ("k_xy_fwd_rp" if packed else "k_xy_fwd",
+                "Poisson iteration pass 1: residual, x transform, first y step", 3 * 8.0 * ncell + spec),
         "fft_yz": ("k_yz_pass", "Poisson iteration pass 2: y and z transforms, division by lambda", 2 * spec),
         "fft_inv": ("k_xy_inv", "Poisson iteration pass 3: inverse y and x transforms, P += dp", spec + 2 * 8.0 * ncell),
     }
