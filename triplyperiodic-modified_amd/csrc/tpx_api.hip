@@ -1367,25 +1367,35 @@ void batch_run_graph(tpx_batch *b, Fn &&fn) {
   for (auto &x : b->segs)
     if (x.sig == sig) e = &x;
   if (!e) {
-    b->segs.emplace_back();
-    e = &b->segs.back();
-    e->sig = sig;
-    hip_check(hipHostMalloc((void **)&e->h, plan.total), "hipHostMalloc");
-    if (hipMalloc((void **)&e->d, plan.total) != hipSuccess) throw std::bad_alloc();
-    hip_check(hipEventCreateWithFlags(&e->copied, hipEventDisableTiming), "hipEventCreate");
+    // a new launch sequence: its table and graph, or nothing cached on failure
+    tpx_batch::Seg n;
+    n.sig = sig;
     hipGraph_t graph = nullptr;
-    hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    bool capturing = false;
     try {
-      batch_launch(b, plan, e->d);
+      hip_check(hipHostMalloc((void **)&n.h, plan.total), "hipHostMalloc");
+      if (hipMalloc((void **)&n.d, plan.total) != hipSuccess) throw std::bad_alloc();
+      hip_check(hipEventCreateWithFlags(&n.copied, hipEventDisableTiming), "hipEventCreate");
+      hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      capturing = true;
+      batch_launch(b, plan, n.d);
+      capturing = false;
+      hip_check(hipStreamEndCapture(b->stream, &graph), "hipStreamEndCapture");
+      const hipError_t err = hipGraphInstantiate(&n.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      graph = nullptr;
+      hip_check(err, "hipGraphInstantiate");
     } catch (...) {
-      (void)hipStreamEndCapture(b->stream, &graph);
+      if (capturing) (void)hipStreamEndCapture(b->stream, &graph);
       if (graph) (void)hipGraphDestroy(graph);
+      if (n.exec) (void)hipGraphExecDestroy(n.exec);
+      if (n.h) (void)hipHostFree(n.h);
+      if (n.d) (void)hipFree(n.d);
+      if (n.copied) (void)hipEventDestroy(n.copied);
       throw;
     }
-    hip_check(hipStreamEndCapture(b->stream, &graph), "hipStreamEndCapture");
-    const hipError_t err = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    hip_check(err, "hipGraphInstantiate");
+    b->segs.push_back(std::move(n));
+    e = &b->segs.back();
   }
   hip_check(hipEventSynchronize(e->copied), "hipEventSynchronize");  // the last copy from e->h is done
   batch_pack(b, plan, e->h);
@@ -1633,8 +1643,8 @@ void gas_replay_segment(tpx_ctx *c, int k) {
   gas_state_copy(c, t.gck + (long)k * 8 * n, false);
   t.gas_base = base;
   t.pnext = 0;
-  GasSubstep scratch_sub[4];
-  for (int s = base; s < end; ++s) gas_substeps(c, s, scratch_sub, true);
+  GasSubstep sub[4];  // filled per step; the sweep reads the segment tapes through taped_step
+  for (int s = base; s < end; ++s) gas_substeps(c, s, sub, true);
   t.seg_loaded = k;
 }
 
