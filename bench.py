@@ -485,8 +485,8 @@ def main():
     # solve has stopped return at once, so a pass's time is divided by the
     # iterations that ran
     spec = (p.Nx_tot // 2 + 1) * p.Ny_tot * p.Nz_tot * 16.0
-    # pass 1 at x lengths 128 and 256 is the packed real-row form unless TPX_XPACK=0
-    packed = p.Nx_tot in (128, 256) and os.environ.get("TPX_XPACK", "1") != "0"
+    # pass 1 at x lengths 64, 128 and 256 is the packed real-row form unless TPX_XPACK=0
+    packed = p.Nx_tot in (64, 128, 256) and os.environ.get("TPX_XPACK", "1") != "0"
     fft = {
         "fft": ("k_xy_fwd_rp" if packed else "k_xy_fwd",
                 "Poisson iteration pass 1: residual, x transform, first y step", 3 * 8.0 * ncell + spec),

@@ -1059,17 +1059,16 @@ struct k_yz_pass {
 };
 
 // ---------------------------------------------------------------------------
-// Passes 1 and 3 with the x transform of the real residual done as a complex
-// transform of half the length (x lengths 2 M = 128, 256; 16 lanes a row):
+// Pass 1 with the x transform of the real residual done as a complex
+// transform of half the length (x lengths 2 M = 64, 128, 256; 16 lanes a row):
 // lane ta of a row holds the pairs z[j] = x[2j] + i x[2j+1], j = ta + 16 m
 // (m < R2H, M = 16 R2H), loaded with 16-byte accesses.  The M-point four-step
 // transform Z of z gives the half spectrum of x by
 //   X[k] = E[k] + W_2M^k O[k],  E = (Z[k] + conj Z[M-k]) / 2,
-//                               O = (Z[k] - conj Z[M-k]) / 2i   (k <= M),
-// and the inverse forms Z[k] = (X[k] + conj X[M-k]) + i W_2M^-k (X[k] - conj X[M-k])
-// before an M-point inverse transform whose real and imaginary parts are the
-// even and odd points.  Half the butterflies and half the load instructions
-// of the full-length complex transform of a real row.
+//                               O = (Z[k] - conj Z[M-k]) / 2i   (k <= M).
+// Half the butterflies and half the load instructions of the full-length
+// complex transform of a real row; the residual's x neighbours come from the
+// adjacent lanes of the row by DPP, as in the full-length form.
 // ---------------------------------------------------------------------------
 
 // the residuals of one x-row in the pair layout: v[m] = (r(x0), r(x0 + 1)),
@@ -1436,7 +1435,7 @@ void xy_fwd_launch(unsigned nb, size_t sh, hipStream_t st, const XArgs &xa, int 
     default: tpx_launch<k_xy_fwd<R1, R2, 16>>(nb, 16 * R1, sh, st, xa, Y1, twy); break;
   }
 }
-// the packed forward form for x lengths 128 and 256 (R2H = 4, 8)
+// the packed forward form for x lengths 64, 128 and 256 (R2H = 2, 4, 8)
 template <int R2H>
 void xy_fwd_rp_launch(unsigned nb, size_t sh, hipStream_t st, const XArgs &xa, int Y1, int Y2, const double2 *twy) {
   switch (Y2) {
@@ -1504,15 +1503,16 @@ unsigned three_pass_launch(tpx_ctx *c, const XArgs &xa, const XInvArgs &xi, cons
   const int mxh = g.nx / 2 + 1;
   const unsigned nb13 = (unsigned)(g.nz * y1);
   const size_t sh13 = std::max(sizeof(double) * (size_t)y2 * r2 * (r1 + 1), sizeof(double2) * (size_t)mxh * (y2 + 1));
-  // pass 1 at x lengths 128 and 256 takes the packed real-row transform
+  // pass 1 at x lengths 64, 128 and 256 takes the packed real-row transform
   // unless TPX_XPACK=0 asks for the full-length complex one; pass 3 keeps
   // the full-length inverse (the same half spectrum either way: the packed
   // inverse, whose final stage leaves half the block idle over the P
   // update, measured slower)
-  const bool packed = (g.nx == 128 || g.nx == 256) && c->x_packed;
+  const bool packed = (g.nx == 64 || g.nx == 128 || g.nx == 256) && c->x_packed;
 #define L_XYF(A, B) xy_fwd_launch<A, B>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1])
   if (packed && g.nx == 256) xy_fwd_rp_launch<8>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
-  else if (packed) xy_fwd_rp_launch<4>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
+  else if (packed && g.nx == 128) xy_fwd_rp_launch<4>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
+  else if (packed) xy_fwd_rp_launch<2>(nb13, sh13, c->stream, xa, y1, y2, c->tw[1]);
   else { TPX_FOUR_STEP_SWITCH(g.nx, L_XYF) }
 #undef L_XYF
   TPX_LAUNCH_CHECK();

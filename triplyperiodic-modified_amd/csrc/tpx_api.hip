@@ -660,7 +660,7 @@ int tpx_create(const tpx_params *p, int64_t np_capacity, tpx_ctx **out) {
     // TPX_FFT_PASSES=5 selects the five-pass Poisson iteration everywhere
     // (the three-pass form is the default where the axis lengths allow it)
     if (const char *e = getenv("TPX_FFT_PASSES")) c->fft_passes = atoi(e) == 5 ? 5 : 3;
-    // TPX_XPACK=0 keeps the full-length complex x transform at x lengths 128 and 256
+    // TPX_XPACK=0 keeps the full-length complex x transform at x lengths 64, 128 and 256
     if (const char *e = getenv("TPX_XPACK")) c->x_packed = atoi(e) != 0;
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     Geo &g = c->g;

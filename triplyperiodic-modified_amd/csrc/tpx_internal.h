@@ -247,7 +247,7 @@ struct tpx_ctx {
   int device = 0;
   size_t max_lds = 65536;          // dynamic LDS bytes one workgroup may use on this device
   int fft_passes = 3;              // Poisson iteration: 3 passes where every axis allows, else 5
-  bool x_packed = true;            // x lengths 128, 256: the real rows as half-length complex transforms
+  bool x_packed = true;            // x lengths 64, 128, 256: pass 1 transforms the real rows as half-length complex ones
   tpx::AdjointTape adj;
   // grid fields
   double *ru[5][3] = {};          // buffer sets for RU
@@ -515,23 +515,32 @@ void launch_batched(dim3 grid, dim3 block, size_t shmem, hipStream_t st, const c
   hipLaunchKernelGGL((k_batched<F, A...>), dim3(grid.x, grid.y, nbox), block, shmem, st, tbl, stride);
 }
 
+// records one launch into the batch being recorded (g_rec): `launch` starts
+// the kernel for every box with the arguments a... of each box read from the
+// table as a std::tuple<A...>
+template <class... A>
+inline void tpx_record(Recorder *r, void (*launch)(dim3, dim3, size_t, hipStream_t, const char *, unsigned, unsigned),
+                       dim3 grid, dim3 block, size_t shmem, A... a) {
+  static_assert((std::is_trivially_copyable<A>::value && ...), "kernel arguments are copied as bytes");
+  using T = std::tuple<A...>;
+  Recorder::Rec rec;
+  rec.launch = launch;
+  rec.grid = grid;
+  rec.block = block;
+  rec.shmem = shmem;
+  rec.size = sizeof(T);
+  rec.align = alignof(T);
+  rec.off = (r->bytes.size() + alignof(T) - 1) / alignof(T) * alignof(T);
+  r->bytes.resize(rec.off + sizeof(T));
+  const T t(a...);
+  memcpy(r->bytes.data() + rec.off, static_cast<const void *>(&t), sizeof(T));
+  r->recs.push_back(rec);
+}
+
 template <class F, class... A>
 inline void tpx_launch(dim3 grid, dim3 block, size_t shmem, hipStream_t st, A... a) {
-  static_assert((std::is_trivially_copyable<A>::value && ...), "kernel arguments are copied as bytes");
   if (Recorder *r = g_rec) {
-    using T = std::tuple<A...>;
-    Recorder::Rec rec;
-    rec.launch = &launch_batched<F, A...>;
-    rec.grid = grid;
-    rec.block = block;
-    rec.shmem = shmem;
-    rec.size = sizeof(T);
-    rec.align = alignof(T);
-    rec.off = (r->bytes.size() + alignof(T) - 1) / alignof(T) * alignof(T);
-    r->bytes.resize(rec.off + sizeof(T));
-    const T t(a...);
-    memcpy(r->bytes.data() + rec.off, static_cast<const void *>(&t), sizeof(T));
-    r->recs.push_back(rec);
+    tpx_record(r, &launch_batched<F, A...>, grid, block, shmem, a...);
     return;
   }
   hipLaunchKernelGGL((k_single<F, A...>), grid, block, shmem, st, a...);
