@@ -118,29 +118,21 @@ struct RhsArgs {
 // ---------------------------------------------------------------------------
 struct ZmGeom { int TX, TY, KC; };
 
+template <bool WITH_RHO, bool HAS_S, bool RK3>
+struct k_rhs_ru {
+  static constexpr int kBounds = 256;
+  static __device__ __forceinline__ void run(const Geo &g, const Coef &kc, const RhsArgs &a, const ZmGeom &zg) {
+#include "k_rhs_ru_body.inc"
+  }
+};
+
 // The single-box launch keeps the body in the kernel itself: inlined from a
-// device function it allocates scalar registers worse (measured 10 % slower).
+// device function it allocates scalar registers worse (measured 10 % slower;
+// for a batch, whose kernels read their arguments from a table, the same
+// kernel-resident body measured the same as the struct form).
 template <bool WITH_RHO, bool HAS_S, bool RK3>
 __global__ __launch_bounds__(256) void k_rhs_ru_kernel(Geo g, Coef kc, RhsArgs a, ZmGeom zg) {
 #include "k_rhs_ru_body.inc"
-}
-
-// The same for a batch (box = blockIdx.z): the body in the kernel itself, its
-// box's arguments copied out of the table by value, as kernel arguments are
-template <bool WITH_RHO, bool HAS_S, bool RK3>
-__global__ __launch_bounds__(256) void k_rhs_ru_batched(const char *__restrict__ tbl, unsigned stride) {
-  const auto &t = *reinterpret_cast<const std::tuple<Geo, Coef, RhsArgs, ZmGeom> *>(tbl + (size_t)blockIdx.z * stride);
-  const Geo g = std::get<0>(t);
-  const Coef kc = std::get<1>(t);
-  const RhsArgs a = std::get<2>(t);
-  const ZmGeom zg = std::get<3>(t);
-#include "k_rhs_ru_body.inc"
-}
-
-template <bool WR, bool HS, bool R3>
-void launch_rhs_ru_batched(dim3 grid, dim3 block, size_t shmem, hipStream_t st, const char *tbl, unsigned stride,
-                           unsigned nbox) {
-  hipLaunchKernelGGL((k_rhs_ru_batched<WR, HS, R3>), dim3(grid.x, grid.y, nbox), block, shmem, st, tbl, stride);
 }
 
 // G1 alone: Update_Rho (grid.cpp:608-614)
@@ -407,7 +399,16 @@ void launch_rhs_ru(tpx_ctx *c, const Coef &k, bool with_rho, const double *const
   ZmGeom zg;
   zg.TX = g.nx < 32 ? g.nx : 32;
   zg.TY = 256 / zg.TX < g.ny ? 256 / zg.TX : g.ny;
+  // planes per block: 32, halved while the launch (over every box of a
+  // batch) has fewer than 1024 blocks, down to 2.  Each block re-reads two
+  // halo planes of the four raw fields, so a short march costs bandwidth, but
+  // a small launch needs the blocks: one 64^3 box runs 128 -> 20 us at 2
+  // planes, one 128^3 box 170 -> 84 us at 8; 64 boxes of 64^3 keep 32.
   zg.KC = g.nz < 32 ? g.nz : 32;
+  {
+    const long cols = (long)(g.nx / zg.TX) * (g.ny / zg.TY) * (g_rec ? g_rec->nbox : 1u);
+    while (zg.KC > 2 && zg.KC % 2 == 0 && cols * (g.nz / zg.KC) < 1024) zg.KC /= 2;
+  }
   unsigned nb = (unsigned)((g.nx / zg.TX) * (g.ny / zg.TY) * (g.nz / zg.KC));
   int threads = zg.TX * zg.TY;
   if ((zg.TX + 3) * (zg.TY + 3) > 2 * threads) throw std::logic_error("z-march tile exceeds two points per thread");
@@ -416,7 +417,7 @@ void launch_rhs_ru(tpx_ctx *c, const Coef &k, bool with_rho, const double *const
   timing_begin(c, TPX_K_GRID_RHS);
 #define TPX_K1(WR, HS, R3)                                                            \
   do {                                                                                \
-    if (g_rec) tpx_record(g_rec, &launch_rhs_ru_batched<WR, HS, R3>, nb, threads, sh, g, k, a, zg); \
+    if (g_rec) tpx_launch<k_rhs_ru<WR, HS, R3>>(nb, threads, sh, c->stream, g, k, a, zg); \
     else k_rhs_ru_kernel<WR, HS, R3><<<nb, threads, sh, c->stream>>>(g, k, a, zg);         \
   } while (0)
   if (with_rho) {

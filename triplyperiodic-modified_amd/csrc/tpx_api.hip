@@ -1261,6 +1261,7 @@ void batch_record(tpx_batch *b, Fn &&fn, BatchPlan &plan) {
   const int nb = (int)b->ctx.size();
   for (int i = 0; i < nb; ++i) {
     b->rec[i].clear();
+    b->rec[i].nbox = (unsigned)nb;
     g_rec = &b->rec[i];
     try {
       fn(b->ctx[i], i);

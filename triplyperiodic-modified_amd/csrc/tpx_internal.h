@@ -505,6 +505,7 @@ struct Recorder {
   };
   std::vector<Rec> recs;
   std::vector<char> bytes;
+  unsigned nbox = 1;  // boxes of the batch: a launch's blocks are this many times its own grid
   void clear() { recs.clear(); bytes.clear(); }
 };
 extern thread_local Recorder *g_rec;
@@ -515,32 +516,23 @@ void launch_batched(dim3 grid, dim3 block, size_t shmem, hipStream_t st, const c
   hipLaunchKernelGGL((k_batched<F, A...>), dim3(grid.x, grid.y, nbox), block, shmem, st, tbl, stride);
 }
 
-// records one launch into the batch being recorded (g_rec): `launch` starts
-// the kernel for every box with the arguments a... of each box read from the
-// table as a std::tuple<A...>
-template <class... A>
-inline void tpx_record(Recorder *r, void (*launch)(dim3, dim3, size_t, hipStream_t, const char *, unsigned, unsigned),
-                       dim3 grid, dim3 block, size_t shmem, A... a) {
-  static_assert((std::is_trivially_copyable<A>::value && ...), "kernel arguments are copied as bytes");
-  using T = std::tuple<A...>;
-  Recorder::Rec rec;
-  rec.launch = launch;
-  rec.grid = grid;
-  rec.block = block;
-  rec.shmem = shmem;
-  rec.size = sizeof(T);
-  rec.align = alignof(T);
-  rec.off = (r->bytes.size() + alignof(T) - 1) / alignof(T) * alignof(T);
-  r->bytes.resize(rec.off + sizeof(T));
-  const T t(a...);
-  memcpy(r->bytes.data() + rec.off, static_cast<const void *>(&t), sizeof(T));
-  r->recs.push_back(rec);
-}
-
 template <class F, class... A>
 inline void tpx_launch(dim3 grid, dim3 block, size_t shmem, hipStream_t st, A... a) {
+  static_assert((std::is_trivially_copyable<A>::value && ...), "kernel arguments are copied as bytes");
   if (Recorder *r = g_rec) {
-    tpx_record(r, &launch_batched<F, A...>, grid, block, shmem, a...);
+    using T = std::tuple<A...>;
+    Recorder::Rec rec;
+    rec.launch = &launch_batched<F, A...>;
+    rec.grid = grid;
+    rec.block = block;
+    rec.shmem = shmem;
+    rec.size = sizeof(T);
+    rec.align = alignof(T);
+    rec.off = (r->bytes.size() + alignof(T) - 1) / alignof(T) * alignof(T);
+    r->bytes.resize(rec.off + sizeof(T));
+    const T t(a...);
+    memcpy(r->bytes.data() + rec.off, static_cast<const void *>(&t), sizeof(T));
+    r->recs.push_back(rec);
     return;
   }
   hipLaunchKernelGGL((k_single<F, A...>), grid, block, shmem, st, a...);
