@@ -248,7 +248,7 @@ struct k_adj_gather_pull {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const Geo &g, const double *const Y, const double *lug,
                                              const int32_t *start, const int32_t *perm, double *lup, long np) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)xcd_block(blockIdx.x, gridDim.x) * block_dim_x() + threadIdx.x;  // neighbour bins in this XCD's L2
     if (c >= g.n) return;
     Ix ix = make_ix(g);
     const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / g.nxy);
@@ -504,7 +504,7 @@ struct k_adj_heat_pull {
   static __device__ __forceinline__ void run(const Geo &g, const double *Y, const double *gt, const double *rho,
                                              const double *p0, double R, const int32_t *start, const int32_t *perm,
                                              double *lrho, long np) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)xcd_block(blockIdx.x, gridDim.x) * block_dim_x() + threadIdx.x;  // neighbour bins in this XCD's L2
     if (c >= g.n) return;
     Ix ix = make_ix(g);
     const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / g.nxy);

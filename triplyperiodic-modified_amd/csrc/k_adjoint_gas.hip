@@ -76,7 +76,14 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return r;
 }
 
-// block sum of v into partials[blockIdx.x]
+// The cell block of a work group: blocks renumbered so that each XCD takes
+// one contiguous range of cells, and the stencils' neighbouring rows and
+// planes are in that XCD's L2.  Block partial sums are stored under this
+// number, so they cover the same cells at the same index as a plain
+// numbering would.
+__device__ __forceinline__ unsigned cell_block() { return xcd_block(blockIdx.x, gridDim.x); }
+
+// block sum of v into partials[cell_block()]
 __device__ __forceinline__ void block_sum(double v, double *partials) {
   __shared__ double sh[kBlock];
   sh[threadIdx.x] = v;
@@ -85,10 +92,23 @@ __device__ __forceinline__ void block_sum(double v, double *partials) {
     if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) partials[cell_block()] = sh[0];
 }
 
-// block sums of u and v into pu[blockIdx.x] and pv[blockIdx.x]
+// block sum of v into partials[cell_block()] for a group of any power-of-two
+// width up to kBlock
+__device__ __forceinline__ void block_sum_bd(double v, double *partials) {
+  __shared__ double sh[kBlock];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = (int)block_dim_x() / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[cell_block()] = sh[0];
+}
+
+// block sums of u and v into pu[cell_block()] and pv[cell_block()]
 __device__ __forceinline__ void block_sum2(double u, double v, double *pu, double *pv) {
   __shared__ double sh[2][kBlock];
   sh[0][threadIdx.x] = u;
@@ -102,8 +122,8 @@ __device__ __forceinline__ void block_sum2(double u, double v, double *pu, doubl
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    pu[blockIdx.x] = sh[0][0];
-    pv[blockIdx.x] = sh[1][0];
+    pu[cell_block()] = sh[0][0];
+    pv[cell_block()] = sh[1][0];
   }
 }
 
@@ -122,7 +142,7 @@ struct GradPArgs {
 struct k_adj_grad_p {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const GradPArgs &a) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     if (c >= a.g.n) return;
     const Cell x(a.g, c);
     double s = 0.0, csum = 0.0;
@@ -170,7 +190,7 @@ struct PoisArgs {
 struct k_adj_pois_it {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const PoisArgs &a) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     double lr = 0.0;
     if (c < a.g.n) {
       const Cell x(a.g, c);
@@ -207,7 +227,7 @@ struct k_adj_pois_it {
 struct k_adj_pois_c {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const PoisArgs &a) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     if (c >= a.g.n) return;
     const Cell x(a.g, c);
     const double cc0 = a.cc[c], rc = a.rho[c];
@@ -243,7 +263,7 @@ struct RhsPArgs {
 struct k_adj_rhs_pois {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const RhsPArgs &a) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     double ta = 0.0, tb = 0.0;
     if (c < a.g.n) {
       const Cell x(a.g, c);
@@ -285,7 +305,7 @@ struct k_adj_fold {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const Geo &g, const double *lF0, const double *lF1, const double *lF2,
                                              double *out) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     if (c >= g.n) return;
     const Cell x(g, c);
     const double *lF[3] = {lF0, lF1, lF2};
@@ -298,7 +318,7 @@ struct k_adj_fold_add {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const Geo &g, const double *lF0, const double *lF1, const double *lF2,
                                              double *out) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     if (c >= g.n) return;
     const Cell x(g, c);
     const double *lF[3] = {lF0, lF1, lF2};
@@ -321,7 +341,7 @@ struct RkArgs {
 struct k_adj_rk {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const RkArgs &a) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     if (c >= a.g.n) return;
     const Cell x(a.g, c);
     const double rc = a.rho_int[c];
@@ -359,90 +379,169 @@ struct MomArgs {
   const double *U[3];       // RU_int / I_C2F(Rho_int)
   const double *lup[3];     // the particles' gather adjoint on the faces, or null
   const double *ld;         // adjoint of div RU_int (G1)
-  double *lu[3];            // adjoint of U
   double *lint[3], *lF[3];  // adjoint of RU_int, of F (first writes)
   double *partials;         // per block: sum lam_RHS . RU_int (the forcing coefficient's adjoint)
   double b_visc, c_visc, A, gv[3];
 };
 
-// adjoint of U = RU_int/F: viscous terms, the convective term's U factor
-// and the particles' gather
-struct k_adj_mom_u {
-  static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(const MomArgs &a) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
-    if (c >= a.g.n) return;
-    const Cell x(a.g, c);
-    const Geo &g = a.g;
-    // div lam_RHS at the cell and at its lower neighbour along each axis
-    auto divl = [&](int di, int dj, int dk) {
-      const long q = x.off(di, dj, dk);
-      return (a.lrhs[0][x.off(di + 1, dj, dk)] - a.lrhs[0][q]) * g.idx +
-             (a.lrhs[1][x.off(di, dj + 1, dk)] - a.lrhs[1][q]) * g.idy +
-             (a.lrhs[2][x.off(di, dj, dk + 1)] - a.lrhs[2][q]) * g.idz;
-    };
-    const double dv0 = divl(0, 0, 0);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {  // component d of U
-      const double *L = a.lrhs[d];
-      const double lc = L[c];
-      double v = a.b_visc * (dv0 - divl(-(d == 0), -(d == 1), -(d == 2))) * inv_d(g, d);
-      double lap = 0.0;
-#pragma unroll
-      for (int b = 0; b < 3; ++b) lap += (L[x.at(b, 1)] - 2 * lc + L[x.at(b, -1)]) * inv_d2(g, b);
-      v += a.c_visc * lap;
-      // -div_F2C(I_C2F(U_d) * I_d(RU_int)): the U_d factor, through every axis b
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        const double idb = inv_d(g, b);
-        const double *R = a.rui[b];
-        const long pb = x.at(b, 1);
-        const double w0 = (lc - L[x.at(b, -1)]) * idb;
-        const double w1 = (L[pb] - lc) * idb;
-        const double d0 = (R[c] + R[x.at(d, -1)]) * 0.5;
-        const double d1 = (R[pb] + R[x.at(b, 1, d, -1)]) * 0.5;
-        v += 0.5 * (w0 * d0 + w1 * d1);
-      }
-      if (a.lup[0]) v += a.lup[d][c];
-      a.lu[d][c] = v;
-    }
-  }
-};
+// Tile of the momentum adjoint's z-march: TX x TY cells of a plane per work
+// group (one thread each), KC planes marched per group.
+struct AdjMomGeom { int TX, TY, KC; };
 
-// adjoint of RU_int (forcing, convection's RU factor, U = RU/F, div RU_int of
-// the density update) and of F (body force, U = RU/F); the forcing
-// coefficient's adjoint as block partial sums
-struct k_adj_mom_r {
+// The momentum right-hand side reversed, one cell per thread: the adjoint of
+// component d of U = RU_int/F (viscous terms, the convective term's U factor
+// through every axis b, the particles' gather), used at the cell only, then
+// the adjoints of RU_int (forcing, convection's RU factor, U = RU/F, div
+// RU_int of the density update) and of F (body force, U = RU/F), and the
+// cell's share of the forcing coefficient's adjoint (lam_RHS . RU_int).
+// val(q, ox, oy, oz) is field q at offset (ox, oy, oz) from the cell: q = 0-2
+// lam_RHS, 3-5 RU_int, 6-8 U, 9 Rho_int, 10 the adjoint of div RU_int.
+template <class V>
+__device__ __forceinline__ void adj_mom_cell(const MomArgs &a, const V &val, const double (&lup)[3], double (&lint)[3],
+                                             double (&lF)[3], double &part) {
+  const Geo &g = a.g;
+  // div lam_RHS at offset (di, dj, dk)
+  auto divl = [&](int di, int dj, int dk) {
+    return (val(0, di + 1, dj, dk) - val(0, di, dj, dk)) * g.idx + (val(1, di, dj + 1, dk) - val(1, di, dj, dk)) * g.idy +
+           (val(2, di, dj, dk + 1) - val(2, di, dj, dk)) * g.idz;
+  };
+  // field q at da along axis a and db along axis b
+  auto at = [&](int q, int ax, int da, int bx = 0, int db = 0) {
+    return val(q, (ax == 0) * da + (bx == 0) * db, (ax == 1) * da + (bx == 1) * db, (ax == 2) * da + (bx == 2) * db);
+  };
+  const double dv0 = divl(0, 0, 0);
+  auto adj_u = [&](int d) {
+    const double lc = val(d, 0, 0, 0);
+    double v = a.b_visc * (dv0 - divl(-(d == 0), -(d == 1), -(d == 2))) * inv_d(g, d);
+    double lap = 0.0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) lap += (at(d, b, 1) - 2 * lc + at(d, b, -1)) * inv_d2(g, b);
+    v += a.c_visc * lap;
+    // -div_F2C(I_C2F(U_d) * I_d(RU_int)): the U_d factor
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const double idb = inv_d(g, b);
+      const double w0 = (lc - at(d, b, -1)) * idb;
+      const double w1 = (at(d, b, 1) - lc) * idb;
+      const double d0 = (val(3 + b, 0, 0, 0) + at(3 + b, d, -1)) * 0.5;
+      const double d1 = (at(3 + b, b, 1) + at(3 + b, b, 1, d, -1)) * 0.5;
+      v += 0.5 * (w0 * d0 + w1 * d1);
+    }
+    if (a.lup[0]) v += lup[d];
+    return v;
+  };
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {  // component b of RU_int, face b at the cell
+    const double idb = inv_d(g, b);
+    const double iF = 1.0 / ((val(9, 0, 0, 0) + at(9, b, -1)) * 0.5);
+    const double lrb = val(b, 0, 0, 0), lub = adj_u(b);
+    double v = lub * iF + a.A * lrb - (val(10, 0, 0, 0) - at(10, b, -1)) * idb;
+    // the RU_b factor of every component d's convective flux along b
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double w0 = (val(d, 0, 0, 0) - at(d, b, -1)) * idb;
+      const double w1 = (at(d, d, 1) - at(d, d, 1, b, -1)) * idb;
+      // U_d on its faces at the cell, - e_b, + e_d, + e_d - e_b
+      v += 0.5 * (w0 * (val(6 + d, 0, 0, 0) + at(6 + d, b, -1)) * 0.5 +
+                  w1 * (at(6 + d, d, 1) + at(6 + d, d, 1, b, -1)) * 0.5);
+    }
+    lint[b] = v;
+    lF[b] = a.gv[b] * lrb - lub * val(6 + b, 0, 0, 0) * iF;
+    part += lrb * val(3 + b, 0, 0, 0);
+  }
+}
+
+// The momentum adjoint as a z-march: a work group holds planes k and k + 1
+// of the eleven fields on its tile (with a one-cell x/y halo) in LDS and
+// plane k + 2 in flight in registers; of plane k - 1 each thread keeps what
+// the stencil reads there (its own column's values, and the x + 1 or y + 1
+// neighbour's for the three fields read there) from when that plane was in
+// LDS.  Every field value is read from HBM about once per march instead of
+// up to seven times per cell from L2.
+struct k_adj_mom {
   static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(const MomArgs &a) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
-    double part = 0.0;
-    if (c < a.g.n) {
-      const Cell x(a.g, c);
-      const Geo &g = a.g;
+  static __device__ __forceinline__ void run(const MomArgs &a, const AdjMomGeom zg) {
+    constexpr int NF = 11;
+    extern __shared__ double sm[];
+    const Geo &g = a.g;
+    const int TX = zg.TX, TY = zg.TY, KC = zg.KC;
+    const int RW = TX + 2, RN = RW * (TY + 2);  // x from i0 - 1 to i0 + TX, y from j0 - 1 to j0 + TY
+    const unsigned blk = cell_block();
+    const int ntx = g.nx / TX, nty = g.ny / TY;
+    const int i0 = (int)(blk % (unsigned)ntx) * TX, j0 = (int)((blk / (unsigned)ntx) % (unsigned)nty) * TY;
+    const int k0 = (int)(blk / (unsigned)(ntx * nty)) * KC;
+    const int tid = (int)threadIdx.x, bd = (int)block_dim_x(), tx = tid % TX, ty = tid / TX;
+    const int mx = g.nx - 1, my = g.ny - 1, mz = g.nz - 1;
+    const int i = i0 + tx, j = j0 + ty;
+    const double *f[NF] = {a.lrhs[0], a.lrhs[1], a.lrhs[2], a.rui[0], a.rui[1], a.rui[2],
+                           a.U[0],    a.U[1],    a.U[2],    a.rho,    a.ld};
+    auto gidx = [&](int x, int y, int z) { return (long)(z & mz) * g.nxy + (long)(y & my) * g.nx + (x & mx); };
+    // the tile point e of plane kz, its offset in the field
+    auto tile_off = [&](int e, int kz) { return gidx(i0 + e % RW - 1, j0 + e / RW - 1, kz); };
+    auto T = [&](int slot, int q, int x, int y) -> double { return sm[(slot * NF + q) * RN + (y + 1) * RW + (x + 1)]; };
+    auto load_tile = [&](int kz, int slot) {
+      for (int e = tid; e < RN; e += bd) {
+        const long o = tile_off(e, kz);
 #pragma unroll
-      for (int b = 0; b < 3; ++b) {  // component b of RU_int, face b at c
-        const double idb = inv_d(g, b);
-        const long mb = x.at(b, -1);
-        const double iF = 1.0 / ((a.rho[c] + a.rho[mb]) * 0.5);
-        const double lrb = a.lrhs[b][c], lub = a.lu[b][c];
-        double v = lub * iF + a.A * lrb - (a.ld[c] - a.ld[mb]) * idb;
-        // the RU_b factor of every component d's convective flux along b
+        for (int q = 0; q < NF; ++q) sm[(slot * NF + q) * RN + e] = f[q][o];
+      }
+    };
+    // plane k - 1: zc[q] at the cell, zx[0..2] of fields 0, 3, 6 at x + 1,
+    // zy[0..2] of fields 1, 4, 7 at y + 1
+    double zc[NF], zx[3], zy[3];
+    {
+      const long o = gidx(i, j, k0 - 1), ox = gidx(i + 1, j, k0 - 1), oy = gidx(i, j + 1, k0 - 1);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const double *L = a.lrhs[d], *Ud = a.U[d];
-          const long pd = x.at(d, 1), pdmb = x.at(d, 1, b, -1);
-          const double w0 = (L[c] - L[mb]) * idb;
-          const double w1 = (L[pd] - L[pdmb]) * idb;
-          // U_d on its faces at c, c - e_b, c + e_d, c + e_d - e_b
-          v += 0.5 * (w0 * (Ud[c] + Ud[mb]) * 0.5 + w1 * (Ud[pd] + Ud[pdmb]) * 0.5);
-        }
-        a.lint[b][c] = v;
-        a.lF[b][c] = a.gv[b] * lrb - lub * a.U[b][c] * iF;
-        part += lrb * a.rui[b][c];
+      for (int q = 0; q < NF; ++q) zc[q] = f[q][o];
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        zx[u] = f[3 * u][ox];
+        zy[u] = f[3 * u + 1][oy];
       }
     }
-    block_sum(part, a.partials);
+    load_tile(k0, 0);
+    load_tile(k0 + 1, 1);
+    __syncthreads();
+    double part = 0.0;
+    int cur = 0;
+    for (int kk = k0; kk < k0 + KC; ++kk) {
+      const int nxt = cur ^ 1;
+      const bool more = kk + 1 < k0 + KC;
+      auto val = [&](int q, int ox, int oy, int oz) -> double {
+        if (oz == 0) return T(cur, q, tx + ox, ty + oy);
+        if (oz == 1) return T(nxt, q, tx + ox, ty + oy);
+        if (ox == 0 && oy == 0) return zc[q];
+        if (ox == 1 && oy == 0 && q < 9 && q % 3 == 0) return zx[q / 3];
+        if (ox == 0 && oy == 1 && q < 9 && q % 3 == 1) return zy[q / 3];
+        return __builtin_nan("");  // not read by the stencil
+      };
+      const long c = gidx(i, j, kk);
+      double lup[3] = {0.0, 0.0, 0.0};
+      if (a.lup[0]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) lup[d] = a.lup[d][c];
+      }
+      double lint[3], lF[3];
+      adj_mom_cell(a, val, lup, lint, lF, part);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        a.lint[d][c] = lint[d];
+        a.lF[d][c] = lF[d];
+      }
+      // this plane becomes plane k - 1 of the next
+#pragma unroll
+      for (int q = 0; q < NF; ++q) zc[q] = T(cur, q, tx, ty);
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        zx[u] = T(cur, 3 * u, tx + 1, ty);
+        zy[u] = T(cur, 3 * u + 1, tx, ty + 1);
+      }
+      __syncthreads();
+      if (more) load_tile(kk + 2, cur);
+      __syncthreads();
+      cur = nxt;
+    }
+    block_sum_bd(part, a.partials);
   }
 };
 
@@ -470,7 +569,7 @@ struct k_adj_boundary {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(long n, double *l0, double *l1, double *l2, double *b0, double *b1,
                                              double *b2, double *lr, double *br) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     if (c >= n) return;
     l0[c] += b0[c]; b0[c] = 0.0;
     l1[c] += b1[c]; b1[c] = 0.0;
@@ -483,7 +582,7 @@ struct k_adj_boundary {
 struct k_adj_sub_mean {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(long n, double *f, const double *scal, int slot) {
-    const long c = (long)blockIdx.x * block_dim_x() + threadIdx.x;
+    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     if (c >= n) return;
     f[c] -= scal[slot] / (double)n;
   }
@@ -652,7 +751,6 @@ void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
     m.rui[d] = s.ru_int[d];
     m.U[d] = ga.U[d];
     m.lup[d] = s.lup ? s.lup + (long)d * n : nullptr;
-    m.lu[d] = ga.lu[d];
     m.lint[d] = ga.lint[d];
     m.lF[d] = ga.lF[d];
     m.gv[d] = k.g[d];
@@ -663,11 +761,24 @@ void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
   m.b_visc = k.b_visc;
   m.c_visc = k.c_visc;
   m.A = k.A;
-  tpx_launch<k_adj_mom_u>(nb, kBlock, 0, c->stream, m);
+  // the z-march's tile: 32 x 8 cells (all of x and more rows when nx < 32),
+  // 32 planes a group, halved while the launch has fewer than 1024 groups
+  // (down to 2)
+  const Geo &g = c->g;
+  AdjMomGeom zg;
+  zg.TX = g.nx < 32 ? g.nx : 32;
+  zg.TY = kBlock / zg.TX < g.ny ? kBlock / zg.TX : g.ny;
+  zg.KC = g.nz < 32 ? g.nz : 32;
+  const long cols = (long)(g.nx / zg.TX) * (g.ny / zg.TY);
+  while (zg.KC > 2 && zg.KC % 2 == 0 && cols * (g.nz / zg.KC) < 1024) zg.KC /= 2;
+  const int threads = zg.TX * zg.TY;
+  const long nbm = cols * (g.nz / zg.KC);
+  if ((zg.TX + 2) * (zg.TY + 2) > 2 * threads || (threads & (threads - 1)) != 0 || nbm > c->n_partials)
+    throw std::logic_error("momentum adjoint: the grid does not fit the z-march tile");
+  const size_t sh = sizeof(double) * 2 * 11 * (size_t)((zg.TX + 2) * (zg.TY + 2));
+  tpx_launch<k_adj_mom>((unsigned)nbm, threads, sh, c->stream, m, zg);
   TPX_LAUNCH_CHECK();
-  tpx_launch<k_adj_mom_r>(nb, kBlock, 0, c->stream, m);
-  TPX_LAUNCH_CHECK();
-  tpx_launch<k_adj_acc>(1, 1024, 0, c->stream, (const double *)c->partials, (long)nb, c->scal + SC_ADJ_A);
+  tpx_launch<k_adj_acc>(1, 1024, 0, c->stream, (const double *)c->partials, nbm, c->scal + SC_ADJ_A);
   TPX_LAUNCH_CHECK();
   tpx_launch<k_adj_fold>(nb, kBlock, 0, c->stream, c->g, (const double *)ga.lF[0], (const double *)ga.lF[1],
                          (const double *)ga.lF[2], ga.lrint);
