@@ -323,18 +323,26 @@ def adjoint_measure():
     nsteps, every = 50, 10
     case = ic.make_case(128, 200_000, nu=0.01, seed=0, St=2.0, phi=0.5, heat=True, twc=True, nsteps=nsteps,
                         eps_range=(0.5, 1.0))
-    plain = tpx.Solver(case.params, np_capacity=200_000)
-    plain.load_case(case)
-    plain.step(1)
-    plain.synchronize()
-    t0 = time.perf_counter()
-    plain.step(nsteps)
-    plain.synchronize()
-    t_fwd = time.perf_counter() - t0
-    plain.close()
+    # the same configuration without the heat exchange (uniform temperature,
+    # so the Poisson solves take one or two iterations a substep)
+    case_cold = ic.make_case(128, 200_000, nu=0.01, seed=0, St=2.0, phi=0.5, heat=False, twc=True, nsteps=nsteps)
+
+    def forward(case):
+        plain = tpx.Solver(case.params, np_capacity=200_000)
+        plain.load_case(case)
+        plain.step(1)
+        plain.synchronize()
+        t0 = time.perf_counter()
+        plain.step(nsteps)
+        plain.synchronize()
+        t = time.perf_counter() - t0
+        plain.close()
+        return t
+
+    t_fwd = forward(case)
     log(f"[bench] forward {nsteps} steps {t_fwd:.3f} s")
 
-    def run(mode, particle_call=False):
+    def run(mode, particle_call=False, case=case):
         s = tpx.Solver(case.params, np_capacity=200_000)
         s.load_case(case)
         s.step(1)
@@ -354,13 +362,16 @@ def adjoint_measure():
         s.close()
         log(f"[bench] gas tape {mode}: recorded forward {t_rec:.3f} s, full sweep {t_rev:.3f} s"
             + (f", particle-only sweep {t_p:.3f} s" if t_p is not None else "")
-            + f"; J = {J:.6e}, dJ/dA = {full['A']:.6e}, |dJ/dRU| = {np.linalg.norm(full['RU']):.3e}, "
-            f"|dJ/deps| = {np.linalg.norm(full['eps']):.3e}, dJ/dP0 = {full['P0']:.6e}")
+            + f"; J = {J:.6e}, dJ/dA = {full['A']:.6e}, |dJ/dRU| = {np.linalg.norm(full['RU']):.3e}"
+            + (f", |dJ/deps| = {np.linalg.norm(full['eps']):.3e}, dJ/dP0 = {full['P0']:.6e}" if "eps" in full else ""))
         return t_rec, t_rev, t_p, full
 
     t_rec, t_rev, _, full = run("segments")
     t_rec_k, t_rev_k, t_rev_p, full_k = run("keep", particle_call=True)
     t_rec_r, t_rev_r, _, full_r = run("recompute")
+    t_fwd_c = forward(case_cold)
+    log(f"[bench] forward {nsteps} steps without the heat exchange {t_fwd_c:.3f} s")
+    _, t_rev_c, _, _ = run("segments", case=case_cold)
     ref = np.max(np.abs(full_k["RU"]))
     agree = max(float(np.max(np.abs(f["RU"] - full_k["RU"])) / ref) for f in (full, full_r))
     return {"metric": "adjoint/forward cost ratio (128^3, 2e5 particles, 50 steps)", "value": round(t_rev / t_fwd, 4),
@@ -370,7 +381,8 @@ def adjoint_measure():
                         "every particle's tau, eps and initial state, the forcing coefficient A and the initial "
                         "RU, Rho, P, P0; gas and particles checkpointed every 10 steps, each segment recomputed "
                         "in the sweep (value: that sweep over the plain forward; keep_*: the gas tape kept per "
-                        "substep, recompute_*: the gas recomputed from every step's start)",
+                        "substep, recompute_*: the gas recomputed from every step's start; no_heat_*: the same "
+                        "segment sweep and forward without the heat exchange)",
             "forward_s": round(t_fwd, 4), "recorded_forward_s": round(t_rec, 4), "reverse_s": round(t_rev, 4),
             "record_and_reverse_over_forward": round((t_rec + t_rev) / t_fwd, 4),
             "keep_reverse_s": round(t_rev_k, 4), "keep_reverse_over_forward": round(t_rev_k / t_fwd, 4),
@@ -378,7 +390,9 @@ def adjoint_measure():
             "recompute_reverse_s": round(t_rev_r, 4), "recompute_reverse_over_forward": round(t_rev_r / t_fwd, 4),
             "recompute_recorded_forward_s": round(t_rec_r, 4),
             "particle_reverse_s": round(t_rev_p, 4), "particle_reverse_over_forward": round(t_rev_p / t_fwd, 4),
-            "recorded_over_plain": round(t_rec / t_fwd, 4), "tapes_agree_to": agree}
+            "recorded_over_plain": round(t_rec / t_fwd, 4), "tapes_agree_to": agree,
+            "no_heat_forward_s": round(t_fwd_c, 4), "no_heat_reverse_s": round(t_rev_c, 4),
+            "no_heat_reverse_over_forward": round(t_rev_c / t_fwd_c, 4)}
 
 
 def adjoint_bench(args):
