@@ -332,8 +332,7 @@ struct RkArgs {
   double *lnp1[3], *lnew[3], *lru[3], *lrhs[3];
   double *lr, *lrnp1, *lrnew, *ld;
   const double *lF[3];       // the Poisson and right-hand side's face adjoint, folded into lam_Rho_new first
-  const double *rui[3], *rho_int;
-  double *U[3];              // out: U = RU_int / I_C2F(Rho_int) for the momentum kernels
+  const double *rho_int;
   double c_post, c_pre;
   int last;  // the fourth substep: new = np1
 };
@@ -344,7 +343,6 @@ struct k_adj_rk {
     const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
     if (c >= a.g.n) return;
     const Cell x(a.g, c);
-    const double rc = a.rho_int[c];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double ln = a.lnew[d][c];
@@ -356,7 +354,6 @@ struct k_adj_rk {
         a.lrhs[d][c] = a.c_post * a.lnp1[d][c] + a.c_pre * ln;
         a.lru[d][c] += ln;
       }
-      a.U[d][c] = a.rui[d][c] / ((rc + a.rho_int[x.at(d, -1)]) * 0.5);
     }
     const double lrn = a.lrnew[c] + fold_at(x, c, a.lF);
     if (a.last) {
@@ -376,7 +373,6 @@ struct MomArgs {
   const double *lrhs[3];
   const double *rui[3];
   const double *rho;        // Rho_int
-  const double *U[3];       // RU_int / I_C2F(Rho_int)
   const double *lup[3];     // the particles' gather adjoint on the faces, or null
   const double *ld;         // adjoint of div RU_int (G1)
   double *lint[3], *lF[3];  // adjoint of RU_int, of F (first writes)
@@ -395,7 +391,8 @@ struct AdjMomGeom { int TX, TY, KC; };
 // RU_int of the density update) and of F (body force, U = RU/F), and the
 // cell's share of the forcing coefficient's adjoint (lam_RHS . RU_int).
 // val(q, ox, oy, oz) is field q at offset (ox, oy, oz) from the cell: q = 0-2
-// lam_RHS, 3-5 RU_int, 6-8 U, 9 Rho_int, 10 the adjoint of div RU_int.
+// lam_RHS, 3-5 RU_int, 6-8 U = RU_int/I_C2F(Rho_int), 9 Rho_int, 10 the
+// adjoint of div RU_int.
 template <class V>
 __device__ __forceinline__ void adj_mom_cell(const MomArgs &a, const V &val, const double (&lup)[3], double (&lint)[3],
                                              double (&lF)[3], double &part) {
@@ -452,12 +449,12 @@ __device__ __forceinline__ void adj_mom_cell(const MomArgs &a, const V &val, con
 }
 
 // The momentum adjoint as a z-march: a work group holds planes k and k + 1
-// of the eleven fields on its tile (with a one-cell x/y halo) in LDS and
-// plane k + 2 in flight in registers; of plane k - 1 each thread keeps what
-// the stencil reads there (its own column's values, and the x + 1 or y + 1
-// neighbour's for the three fields read there) from when that plane was in
-// LDS.  Every field value is read from HBM about once per march instead of
-// up to seven times per cell from L2.
+// of the eleven fields on its tile (with a one-cell x/y halo) in LDS, U
+// formed from RU_int and Rho_int as the tile is loaded; of plane k - 1 each
+// thread keeps what the stencil reads there (its own column's values, and the
+// x + 1 or y + 1 neighbour's for the fields read there) from when that plane
+// was in LDS.  Every field value is read from HBM about once per march
+// instead of up to seven times per cell from L2.
 struct k_adj_mom {
   static constexpr int kBounds = kBlock;
   static __device__ __forceinline__ void run(const MomArgs &a, const AdjMomGeom zg) {
@@ -474,30 +471,34 @@ struct k_adj_mom {
     const int mx = g.nx - 1, my = g.ny - 1, mz = g.nz - 1;
     const int i = i0 + tx, j = j0 + ty;
     const double *f[NF] = {a.lrhs[0], a.lrhs[1], a.lrhs[2], a.rui[0], a.rui[1], a.rui[2],
-                           a.U[0],    a.U[1],    a.U[2],    a.rho,    a.ld};
+                           nullptr,   nullptr,   nullptr,   a.rho,    a.ld};
     auto gidx = [&](int x, int y, int z) { return (long)(z & mz) * g.nxy + (long)(y & my) * g.nx + (x & mx); };
+    // field q at (x, y, z); U_d = RU_int_d / I_C2F(Rho_int) on face d
+    // (q = 6 + d) is formed from RU_int and Rho_int as the forward forms it
+    auto field = [&](int q, int x, int y, int z) -> double {
+      if (q < 6 || q > 8) return f[q][gidx(x, y, z)];
+      const int d = q - 6;
+      const long o = gidx(x, y, z), m = gidx(x - (d == 0), y - (d == 1), z - (d == 2));
+      return a.rui[d][o] / ((a.rho[o] + a.rho[m]) * 0.5);
+    };
     // the tile point e of plane kz, its offset in the field
-    auto tile_off = [&](int e, int kz) { return gidx(i0 + e % RW - 1, j0 + e / RW - 1, kz); };
     auto T = [&](int slot, int q, int x, int y) -> double { return sm[(slot * NF + q) * RN + (y + 1) * RW + (x + 1)]; };
     auto load_tile = [&](int kz, int slot) {
       for (int e = tid; e < RN; e += bd) {
-        const long o = tile_off(e, kz);
+        const int x = i0 + e % RW - 1, y = j0 + e / RW - 1;
 #pragma unroll
-        for (int q = 0; q < NF; ++q) sm[(slot * NF + q) * RN + e] = f[q][o];
+        for (int q = 0; q < NF; ++q) sm[(slot * NF + q) * RN + e] = field(q, x, y, kz);
       }
     };
     // plane k - 1: zc[q] at the cell, zx[0..2] of fields 0, 3, 6 at x + 1,
     // zy[0..2] of fields 1, 4, 7 at y + 1
     double zc[NF], zx[3], zy[3];
-    {
-      const long o = gidx(i, j, k0 - 1), ox = gidx(i + 1, j, k0 - 1), oy = gidx(i, j + 1, k0 - 1);
 #pragma unroll
-      for (int q = 0; q < NF; ++q) zc[q] = f[q][o];
+    for (int q = 0; q < NF; ++q) zc[q] = field(q, i, j, k0 - 1);
 #pragma unroll
-      for (int u = 0; u < 3; ++u) {
-        zx[u] = f[3 * u][ox];
-        zy[u] = f[3 * u + 1][oy];
-      }
+    for (int u = 0; u < 3; ++u) {
+      zx[u] = field(3 * u, i + 1, j, k0 - 1);
+      zy[u] = field(3 * u + 1, i, j + 1, k0 - 1);
     }
     load_tile(k0, 0);
     load_tile(k0 + 1, 1);
@@ -728,10 +729,9 @@ const double *const *adjoint_gas_substep_rk(tpx_ctx *c, GasAdjoint &ga, const Ga
     a.g = c->g;
     for (int d = 0; d < 3; ++d) {
       a.lnp1[d] = ga.lnp1[d]; a.lnew[d] = ga.lnew[d]; a.lru[d] = ga.lru[d]; a.lrhs[d] = ga.lrhs[d];
-      a.lF[d] = ga.lF[d]; a.rui[d] = s.ru_int[d]; a.U[d] = ga.U[d];
+      a.lF[d] = ga.lF[d];
     }
     a.lr = ga.lr; a.lrnp1 = ga.lrnp1; a.lrnew = ga.lrnew; a.ld = ga.ld;
-    a.rho_int = s.rho_int;
     a.c_post = k.dt * k.post;
     a.c_pre = k.dt * k.pre;
     a.last = last;
@@ -749,7 +749,6 @@ void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
   for (int d = 0; d < 3; ++d) {
     m.lrhs[d] = ga.lrhs[d];
     m.rui[d] = s.ru_int[d];
-    m.U[d] = ga.U[d];
     m.lup[d] = s.lup ? s.lup + (long)d * n : nullptr;
     m.lint[d] = ga.lint[d];
     m.lF[d] = ga.lF[d];

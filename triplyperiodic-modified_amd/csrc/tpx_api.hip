@@ -1699,7 +1699,7 @@ void gas_adjoint_run(tpx_ctx *c) {
   GasAdjoint &ga = t.ga;
   double *w = t.work;
   auto take = [&]() { double *p = w; w += n; return p; };
-  for (auto *grp : {ga.lru, ga.lnp1, ga.lint, ga.lnew, ga.lrhs, ga.lF, ga.U})
+  for (auto *grp : {ga.lru, ga.lnp1, ga.lint, ga.lnew, ga.lrhs, ga.lF})
     for (int d = 0; d < 3; ++d) grp[d] = take();
   for (double **p : {&ga.lr, &ga.lrnp1, &ga.lrint, &ga.lrnew, &ga.lp, &ga.lrhs_p, &ga.lcc, &ga.ld, &ga.nu, &ga.cc,
                      &ga.lq})

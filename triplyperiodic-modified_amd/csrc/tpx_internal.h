@@ -100,14 +100,14 @@ struct Particles {
 // Work arrays of the gas adjoint sweep (n values each): adjoints of the
 // base, np1, int and new momentum and density, of P, of the Poisson
 // right-hand side, of the momentum right-hand side and of the face fields F
-// (Rho_face) and C (1/Rho_face); the substep's C, cc and U; the spectral
+// (Rho_face) and C (1/Rho_face); the substep's C and cc; the spectral
 // solve's buffers.
 struct GasAdjoint {
-  double *lru[3], *lnp1[3], *lint[3], *lnew[3], *lrhs[3], *lF[3], *U[3];
+  double *lru[3], *lnp1[3], *lint[3], *lnew[3], *lrhs[3], *lF[3];
   double *lr, *lrnp1, *lrint, *lrnew, *lp, *lrhs_p, *lcc, *ld, *nu, *cc;
   double *lq;    // adjoint of the substep's heat deposit RHS_Part_Temp
   double *scal;  // SC_COUNT slots of the spectral solve (SC_DONE stays 0)
-  static constexpr int kFields = 32;
+  static constexpr int kFields = 29;
 };
 
 // One recomputed substep as the gas adjoint reads it.
