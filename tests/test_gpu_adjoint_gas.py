@@ -23,14 +23,18 @@ FD_REL = 1e-6  # |FD - adjoint| / max(|FD|, |adjoint|, median |adjoint|) per dir
 NSTEPS, EVERY = 4, 2
 
 
-def _variable_density_case(iterations=6):
+def _variable_density_case(iterations=6, shape=None):
     """Non-uniform initial density (so the Poisson coefficient, the
     conduction term k/cp del2(1/Rho) and the body force are all active),
-    gravity on the gas, fixed Poisson iteration counts."""
-    case = ic.make_case(16, 400, nu=0.02, seed=2, St=2.0, phi=0.5, heat=False, twc=False, nsteps=6)
-    n = 16
-    x = (np.arange(n) + 0.5) * (2 * np.pi / n)
-    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    gravity on the gas, fixed Poisson iteration counts; a 16^3 box, or
+    shape = (nz, ny, nx) cells of one spacing."""
+    case = ic.make_case(16, 400, nu=0.02, seed=2, St=2.0, phi=0.5, heat=False, twc=False, nsteps=6, shape=shape)
+    nz, ny, nx = case.Rho.shape
+    d = 2 * np.pi / nx
+    z, y, x = ((np.arange(m) + 0.5) * d for m in (nz, ny, nx))
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    # one period along each side
+    Y, Z = Y * (nx / ny), Z * (nx / nz)
     case.Rho = 1.0 + 0.1 * np.sin(X + 0.3) * np.cos(2 * Y) + 0.05 * np.sin(Z - 0.7)
     case.RU = case.RU * 1.0
     rng = np.random.default_rng(4)
@@ -130,6 +134,19 @@ def test_gas_gradient_two_iteration_solves(two_iterations, which):
     worst = _check(case, g, which, h, 3, seed=29, smooth=which == "P")
     print("J", J, "worst", worst)
     assert worst <= FD_REL
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 32), (8, 16, 32)])
+def test_gas_gradient_on_a_box_of_unequal_sides(gpu_available, shape):
+    """The momentum right-hand side's adjoint marches tiles of 32 x 8 cells
+    along z (all of x and more rows when nx < 32); on boxes of unequal sides
+    (the three-pass transform at 16 x 16 x 32, the five-pass one where an
+    axis is 8 long) dJ/dA, dJ/dRU and dJ/dRho match central differences."""
+    case, J, g = _record(_variable_density_case(6, shape))
+    for which, h in (("A", 1e-4 * case.params.A1), ("RU", 1e-6), ("Rho", 1e-6)):
+        worst = _check(case, g, which, h, 2, seed=31)
+        print(shape, which, "J", J, "worst", worst)
+        assert worst <= FD_REL
 
 
 def test_gas_gradient_uniform_density_adaptive_solve(gpu_available):

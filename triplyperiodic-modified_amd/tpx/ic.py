@@ -249,19 +249,24 @@ class Case:
 def make_case(n: int, n_particles: int, nu: float, seed: int = 0, St=1.0, phi: float = 0.2,
               heat: bool = False, twc: bool = True, k0: float = 4.0, cfl: float = 0.3,
               eps_range: Optional[Tuple[float, float]] = None, forcing: bool = True,
-              L: float = 2 * PI, nsteps: int = 20, drag_limit: bool = False) -> Case:
+              L: float = 2 * PI, nsteps: int = 20, drag_limit: bool = False,
+              shape: Optional[Tuple[int, int, int]] = None) -> Case:
     """Synthetic case following the benchmark recipe: spectral IC (k0, u'=1),
     CFL-0.3 dt, Lundgren forcing A = eps0/(3 u'^2), particles at seeded
     uniform positions with the gas velocity interpolated to them, Stokes
-    number St (scalar, or an array for per-particle tau), mass loading phi."""
-    shape = (n, n, n)
-    Ls = (L, L, L)
+    number St (scalar, or an array for per-particle tau), mass loading phi.
+    The box is n^3 of side L, or shape = (nz, ny, nx) cells of one spacing
+    L / nx."""
+    if shape is None:
+        shape = (n, n, n)
+    nz, ny, nx = shape
+    Ls = (L, L * ny / nx, L * nz / nx)
     rho0, T0 = 1.0, 1.0
     u = spectral_velocity(shape, Ls, seed=seed, k0=k0, uprime=1.0)
     eps0 = dissipation(u, nu, Ls)
     tau_eta = math.sqrt(nu / eps0)
     dt = cfl / max_cfl_rate(u, Ls)
-    p = Params(Nx_tot=n, Ny_tot=n, Nz_tot=n, Lx=L, Ly=L, Lz=L, bs=2)
+    p = Params(Nx_tot=nx, Ny_tot=ny, Nz_tot=nz, Lx=Ls[0], Ly=Ls[1], Lz=Ls[2], bs=2)
     p.dt = dt
     p.T_final = dt * nsteps
     p.Mu0 = nu * rho0
@@ -276,7 +281,7 @@ def make_case(n: int, n_particles: int, nu: float, seed: int = 0, St=1.0, phi: f
     # straight into dp (poisson.cpp:442), so a smaller tolerance would never
     # be met and every solve would run all Iteration passes.
     p.epsilon = 1e-10
-    vol = L ** 3
+    vol = Ls[0] * Ls[1] * Ls[2]
     npart = int(n_particles)
     st_arr = np.asarray(St, dtype=float)
     st_ref = float(np.exp(np.mean(np.log(st_arr)))) if st_arr.ndim else float(st_arr)
@@ -296,7 +301,7 @@ def make_case(n: int, n_particles: int, nu: float, seed: int = 0, St=1.0, phi: f
         # rate at 1, well inside RK4's real-axis stability limit (2.78) to
         # leave room for cells holding several particles, so boxes whose
         # particles outweigh a cell's gas do not blow up.
-        mcell = rho0 * vol / (n ** 3)
+        mcell = rho0 * vol / (nx * ny * nz)
         tau_min = float(np.min(st_arr)) * tau_eta if st_arr.ndim else tau_ref
         p.dt = min(p.dt, tau_min / (1.0 + (phi * rho0 * vol / npart) / mcell))
         p.T_final = p.dt * nsteps
