@@ -305,6 +305,7 @@ struct XInvArgs {
   double ncell;
   const double *done;
   int overwrite = 0;  // three-pass form: P = dp (P unread, its own kernel instance) instead of P += dp
+  double *Pout = nullptr;  // non-null: P + dp is written here and P is left as it was
 };
 
 // X inverse pass: Hermitian extension, inverse FFT, P += dp, block max|dp|
@@ -335,8 +336,8 @@ struct k_xinv {
     long line = line0 + l;
     if (line < nlines) {
       double dp = (-1) * lds[e].x / (a.ncell);
-      double *Pp = a.P + line * nx + i;
-      *Pp = *Pp + dp;
+      const long o = line * nx + i;
+      (a.Pout ? a.Pout : a.P)[o] = a.P[o] + dp;
       mx = fmax(mx, fabs(dp));
     }
   }
@@ -684,8 +685,8 @@ struct k_xinv4 {
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) {
         double dp = (-1) * u[k1].x / (a.ncell);
-        double *Pp = a.P + line * n + k2 + R2 * k1;
-        *Pp = *Pp + dp;
+        const long o = line * n + k2 + R2 * k1;
+        (a.Pout ? a.Pout : a.P)[o] = a.P[o] + dp;
         mx = fmax(mx, fabs(dp));
       }
     }
@@ -1203,7 +1204,7 @@ struct k_xy_fwd_rp {
   }
 };
 
-template <int R1, int R2, int Y2, bool Overwrite = false>
+template <int R1, int R2, int Y2, bool Overwrite = false, bool OutOfPlace = false>
 struct k_xy_inv {
   static constexpr int kBounds = 256;
   static __device__ __forceinline__ void run(const XInvArgs &a, int Y1, const double2 *__restrict__ twy) {
@@ -1273,12 +1274,16 @@ struct k_xy_inv {
 #pragma unroll
     for (int q = 0; q < R1; ++q) u[q].y = ldsd[(cb * R2 + k2x) * P + q];
     RegDft<R1, true>::run(u);
-    double *Pl = a.P + ((long)z * g.ny + t + Y1 * cb) * n;
+    const long l0 = ((long)z * g.ny + t + Y1 * cb) * n;
+    double *Pl = a.P + l0;
+    const double *__restrict__ Pin = a.P + l0;
+    double *__restrict__ Po = (OutOfPlace ? a.Pout : a.P) + l0;
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       double dp = (-1) * u[k1].x / (a.ncell);
       double *Pp = Pl + k2x + R2 * k1;
       if constexpr (Overwrite) *Pp = dp;
+      else if constexpr (OutOfPlace) Po[k2x + R2 * k1] = Pin[k2x + R2 * k1] + dp;
       else *Pp = *Pp + dp;
       mx = fmax(mx, fabs(dp));
     }
@@ -1444,18 +1449,19 @@ void xy_fwd_rp_launch(unsigned nb, size_t sh, hipStream_t st, const XArgs &xa, i
     default: tpx_launch<k_xy_fwd_rp<R2H, 16>>(nb, 16 * 16, sh, st, xa, Y1, twy); break;
   }
 }
-template <int R1, int R2, bool Ow>
+template <int R1, int R2, bool Ow, bool Oop>
 void xy_inv_launch_t(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, int Y1, int Y2, const double2 *twy) {
   switch (Y2) {
-    case 4: tpx_launch<k_xy_inv<R1, R2, 4, Ow>>(nb, 4 * R1, sh, st, xi, Y1, twy); break;
-    case 8: tpx_launch<k_xy_inv<R1, R2, 8, Ow>>(nb, 8 * R1, sh, st, xi, Y1, twy); break;
-    default: tpx_launch<k_xy_inv<R1, R2, 16, Ow>>(nb, 16 * R1, sh, st, xi, Y1, twy); break;
+    case 4: tpx_launch<k_xy_inv<R1, R2, 4, Ow, Oop>>(nb, 4 * R1, sh, st, xi, Y1, twy); break;
+    case 8: tpx_launch<k_xy_inv<R1, R2, 8, Ow, Oop>>(nb, 8 * R1, sh, st, xi, Y1, twy); break;
+    default: tpx_launch<k_xy_inv<R1, R2, 16, Ow, Oop>>(nb, 16 * R1, sh, st, xi, Y1, twy); break;
   }
 }
 template <int R1, int R2>
 void xy_inv_launch(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, int Y1, int Y2, const double2 *twy) {
-  if (xi.overwrite) xy_inv_launch_t<R1, R2, true>(nb, sh, st, xi, Y1, Y2, twy);
-  else xy_inv_launch_t<R1, R2, false>(nb, sh, st, xi, Y1, Y2, twy);
+  if (xi.overwrite) xy_inv_launch_t<R1, R2, true, false>(nb, sh, st, xi, Y1, Y2, twy);
+  else if (xi.Pout) xy_inv_launch_t<R1, R2, false, true>(nb, sh, st, xi, Y1, Y2, twy);
+  else xy_inv_launch_t<R1, R2, false, false>(nb, sh, st, xi, Y1, Y2, twy);
 }
 template <int Z1, int Z2>
 void yz_launch(unsigned nb, size_t sh, hipStream_t st, double2 *spec, int mxh, int Y1, int Y2, const tpx_ctx *c,
@@ -1534,7 +1540,7 @@ unsigned three_pass_launch(tpx_ctx *c, const XArgs &xa, const XInvArgs &xi, cons
 }
 
 void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal,
-                                 const double *res, int sum_slot) {
+                                 const double *res, int sum_slot, double *Pout) {
   const Geo &g = c->g;
   int mxh = g.nx / 2 + 1;
   long nlines = (long)g.ny * g.nz;
@@ -1545,6 +1551,7 @@ void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, c
   // that slot is sum_slot
   XArgs xa{g, P, rho_new, rhs, scal + (sum_slot - SC_SUM_RHS), c->spec[0], c->tw[0], (double)g.n, done, res};
   XInvArgs xi{g, P, c->spec[0], c->tw[0], c->partials, (double)g.n, done};
+  xi.Pout = Pout;
   const bool x4 = four_step(g.nx, r1, r2);
   int y1, y2, z1, z2;
   // the three-pass form forms the residual itself from rho; a precomputed

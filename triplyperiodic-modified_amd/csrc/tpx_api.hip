@@ -1584,8 +1584,12 @@ void gas_substeps(tpx_ctx *c, int s, GasSubstep sub[4], bool segment) {
     double *sum = segment ? t.sub + (long)t.tape_steps() * 20 * n + (long)j * 4 + rk : t.cap + 20 * n + rk;
     launch_copy(c, sum, c->scal + SC_SUM_RHS, 1);
     g.rhs_sum = sum;
-    // the recorded number of iterations, P kept before each; the count the
-    // device's loop test reaches is kept for a check after the sweep
+    // the recorded number of iterations, P kept before each: the first P is
+    // copied into its slot, and every iteration writes its result into the
+    // next slot (the last into P), out of place, so P before each stays; the
+    // count the device's loop test reaches is kept for a check after the
+    // sweep (the replay is the recorded run bit for bit, so no iteration is
+    // skipped on the device before the recorded count)
     launch_solve_begin(c);
     g.P.clear();
     g.iters = t.iters[q];
@@ -1593,9 +1597,11 @@ void gas_substeps(tpx_ctx *c, int s, GasSubstep sub[4], bool segment) {
     for (int it = 0; it < g.iters; ++it) {
       double *buf = pcap_slot(t, used++, n);
       if (!buf) throw std::bad_alloc();
-      launch_copy(c, buf, c->P, n);
+      if (it == 0) launch_copy(c, buf, c->P, n);
       g.P.push_back(buf);
-      launch_poisson_iteration(c, c->rho[c->rho_new]);
+      double *out = c->P;
+      if (it + 1 < g.iters && !(out = pcap_slot(t, used, n))) throw std::bad_alloc();
+      launch_poisson_iteration_on(c, c->rho[c->rho_new], buf, c->rhs, c->scal, nullptr, SC_SUM_RHS, out);
     }
     launch_copy(c, t.iters_seen + (long)s * 4 + rk, c->scal + SC_ITERS, 1);
     do_update_ru_wp(c, rk, rk == 3);

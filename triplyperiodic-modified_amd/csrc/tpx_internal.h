@@ -333,9 +333,11 @@ int fft_setup(tpx_ctx *c);
 // one Poisson iteration; skipped on the device once the solve has stopped
 void launch_poisson_iteration(tpx_ctx *c, const double *rho_new);
 // the same iteration on caller-given P, raw right-hand side and scalar slots
-// sum_slot: the scal slot holding the sum of rhs (its mean is removed in the residual)
+// sum_slot: the scal slot holding the sum of rhs (its mean is removed in the residual);
+// Pout: non-null, P + dp is written there and P is left as it was (an iteration
+// skipped on the device then writes nothing)
 void launch_poisson_iteration_on(tpx_ctx *c, const double *rho_new, double *P, const double *rhs, double *scal,
-                                 const double *res = nullptr, int sum_slot = SC_SUM_RHS);
+                                 const double *res = nullptr, int sum_slot = SC_SUM_RHS, double *Pout = nullptr);
 // out = M in, M the Poisson iteration's spectral step (forward transform, the
 // division by the modified wavenumbers, inverse, -1/ncell): the iteration's
 // dp for the residual `in`.  No loop test runs; scal[SC_DONE] must be 0.
