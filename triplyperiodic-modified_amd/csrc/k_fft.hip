@@ -1275,17 +1275,25 @@ struct k_xy_inv {
     for (int q = 0; q < R1; ++q) u[q].y = ldsd[(cb * R2 + k2x) * P + q];
     RegDft<R1, true>::run(u);
     const long l0 = ((long)z * g.ny + t + Y1 * cb) * n;
-    double *Pl = a.P + l0;
-    const double *__restrict__ Pin = a.P + l0;
-    double *__restrict__ Po = (OutOfPlace ? a.Pout : a.P) + l0;
+    if constexpr (OutOfPlace) {
+      const double *__restrict__ Pin = a.P + l0;
+      double *__restrict__ Po = a.Pout + l0;
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) {
-      double dp = (-1) * u[k1].x / (a.ncell);
-      double *Pp = Pl + k2x + R2 * k1;
-      if constexpr (Overwrite) *Pp = dp;
-      else if constexpr (OutOfPlace) Po[k2x + R2 * k1] = Pin[k2x + R2 * k1] + dp;
-      else *Pp = *Pp + dp;
-      mx = fmax(mx, fabs(dp));
+      for (int k1 = 0; k1 < R1; ++k1) {
+        double dp = (-1) * u[k1].x / (a.ncell);
+        Po[k2x + R2 * k1] = Pin[k2x + R2 * k1] + dp;
+        mx = fmax(mx, fabs(dp));
+      }
+    } else {
+      double *Pl = a.P + l0;
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        double dp = (-1) * u[k1].x / (a.ncell);
+        double *Pp = Pl + k2x + R2 * k1;
+        if constexpr (Overwrite) *Pp = dp;
+        else *Pp = *Pp + dp;
+        mx = fmax(mx, fabs(dp));
+      }
     }
   }
   red[threadIdx.x] = mx;
