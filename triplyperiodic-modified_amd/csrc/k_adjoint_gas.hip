@@ -252,6 +252,9 @@ struct RhsPArgs {
   const double *scal;     // SC_ADJ_SUM: sum of lam_RHS
   double s, kappa;        // 1/(dt pre), k/cp
   double *lnew[3], *lF[3], *lrho;
+  // non-null: the Poisson coefficient's share of lam_F' (k_adj_pois_c's) is
+  // formed here from cc and lam_cc and lam_F' written, not accumulated
+  const double *cc, *lcc;
   // heat exchange (Q null: none)
   const double *Q, *q_sum, *p0n;
   double *lq;
@@ -270,6 +273,7 @@ struct k_adj_rhs_pois {
       const double mean = a.scal[SC_ADJ_SUM] / (double)a.g.n;
       auto nu = [&](long p) { return a.s * (a.lrhs[p] - mean); };
       const double nc = nu(c), rc = a.rho[c];
+      const double w0 = a.lcc ? a.cc[c] * a.cc[c] * a.lcc[c] : 0.0;
       double lap = 0.0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -279,7 +283,15 @@ struct k_adj_rhs_pois {
         const double F = (rc + a.rho[m]) * 0.5;
         const double lu = -((nc - nm) * inv_d(a.g, d));
         a.lnew[d][c] += lu / F;
-        a.lF[d][c] += -lu * a.ru_new[d][c] / (F * F);
+        if (a.lcc) {
+          const double ccm = a.cc[m];
+          const double lc = (w0 + ccm * ccm * a.lcc[m]) * (1. / 6.);
+          const double C = rcp_nr((rc + a.rho[m]) * 0.5);
+          const double pc = -lc * C * C;
+          a.lF[d][c] = pc + -lu * a.ru_new[d][c] / (F * F);
+        } else {
+          a.lF[d][c] += -lu * a.ru_new[d][c] / (F * F);
+        }
       }
       a.lrho[c] += a.kappa * lap / (rc * rc);
       if (a.Q) {
@@ -683,15 +695,15 @@ void adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
     TPX_LAUNCH_CHECK();
   }
   launch_finish_sum(c, (int)nb, 0, SC_ADJ_SUM);  // sum of lam_RHS_Pois from the last k_adj_pois_it
-  tpx_launch<k_adj_pois_c>(nb, kBlock, 0, c->stream, pa);
-  TPX_LAUNCH_CHECK();
-  // G6 and G5
+  // G6 and G5, with the Poisson coefficient's share of lam_F' (cc(C), C = 1/F')
   {
     RhsPArgs a{};
     a.g = c->g;
     a.rho = s.rho_new;
     for (int d = 0; d < 3; ++d) { a.ru_new[d] = s.ru_new[d]; a.lnew[d] = ga.lnew[d]; a.lF[d] = ga.lF[d]; }
     a.lrhs = ga.lrhs_p;
+    a.cc = ga.cc;
+    a.lcc = ga.lcc;
     a.scal = c->scal;
     a.s = 1. / (k.dt * k.pre);
     a.kappa = k.k_over_cp;
