@@ -203,3 +203,14 @@ def test_adjoint_config_against_finite_differences(gpu_available):
         fds.append((_J(cp) - _J(cm)) / 2e-5)
     w_RU = _worst([float(np.sum(g["RU"] * d)) for d in dirs], fds, "RU")
     assert max(w_tau, w_A, w_RU) <= FD_REL, (w_tau, w_A, w_RU)
+    # the configuration's own scheme, the gas checkpointed and each segment
+    # replayed in the sweep, gives the same gradients bit for bit
+    s = solver_from_case(case)
+    s.adjoint_set_gas_tape("segments")
+    assert s.adjoint_record(A_STEPS, A_EVERY) == J
+    gs = s.adjoint_gradient_full()
+    s.adjoint_release()
+    s.close()
+    assert gs["A"] == g["A"]
+    for q in ("tau", "state", "RU", "Rho", "P"):
+        assert np.array_equal(gs[q], g[q]), q
