@@ -450,6 +450,34 @@ def test_b1_spectral_initial_condition():
     assert np.max(np.abs(o.apply("div_f2c", u[0], u[1], u[2]))) < 1e-11
 
 
+def test_b1_case_on_a_box_of_unequal_sides():
+    """make_case with shape = (nz, ny, nx): one grid spacing L / nx, side
+    lengths to match, a discretely solenoidal u' = 1 field, particles inside
+    the box, the parameter file's sizes and lengths; shape (n, n, n) is the
+    cube bit for bit."""
+    a = ic.make_case(16, 300, nu=0.02, seed=2, St=2.0, phi=0.5, heat=True, twc=True, nsteps=4,
+                     eps_range=(0.5, 1.0))
+    b = ic.make_case(16, 300, nu=0.02, seed=2, St=2.0, phi=0.5, heat=True, twc=True, nsteps=4,
+                     eps_range=(0.5, 1.0), shape=(16, 16, 16))
+    assert np.array_equal(a.RU, b.RU) and a.params.dt == b.params.dt and a.params.Dp == b.params.Dp
+    for q in "xyzuvw":
+        assert np.array_equal(a.particles[q], b.particles[q])
+    c = ic.make_case(0, 500, nu=0.02, seed=3, St=1.0, phi=0.3, shape=(8, 16, 32))
+    p = c.params
+    assert (p.Nz_tot, p.Ny_tot, p.Nx_tot) == (8, 16, 32)
+    assert (p.Lx, p.Ly, p.Lz) == (L, L / 2, L / 4)
+    assert c.RU.shape == (3, 8, 16, 32) and c.Rho.shape == (8, 16, 32)
+    div = ic.discrete_divergence(c.RU, (p.Lx, p.Ly, p.Lz))
+    assert np.max(np.abs(div)) < 1e-12 * np.max(np.abs(c.RU)) * 32
+    uprime = math.sqrt(float(np.mean(c.RU[0] ** 2 + c.RU[1] ** 2 + c.RU[2] ** 2)) / 3)
+    assert abs(uprime - 1.0) < 1e-12
+    for q, side in (("x", p.Lx), ("y", p.Ly), ("z", p.Lz)):
+        assert np.all(c.particles[q] >= 0) and np.all(c.particles[q] < side)
+    # mass loading over the box's own volume
+    mp = p.Rhop * math.pi * p.Dp ** 3 / 6
+    assert abs(mp * 500 / (p.Lx * p.Ly * p.Lz) - 0.3) < 1e-12
+
+
 # ---------------------------------------------------------------- G5 / G6 / G10 / G14
 def prepared(p, npart=0, parts=None, Rho=None):
     o = new_oracle(p, npart)
