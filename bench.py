@@ -80,7 +80,7 @@ def step_bytes(ncell, npart, n_it):
 
 
 def step_kernel_bytes(ncell, npart, its_per_step, per_eps, per_tau, twc, heat, counted):
-    """Algorithmic bytes of one RK4 step as this build's kernels move them:
+    """Algorithmic bytes of one RK4 step as the library's kernels move them:
     per substep the particle advance, the fused density/momentum update, the
     divergence and Poisson right-hand side (6 FP64 a cell), the projection
     (13 FP64 a cell; 7 at the last substep, where RU_new is RU_np1), the
@@ -425,7 +425,7 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the step is measured at N=1 only; "
+        print(json.dumps({"error": "this library runs on one GPU: the step is measured at N=1 only; "
                                    "multi-GPU decomposition is a design (see the sharding notes), "
                                    f"not a runnable mode (asked for --gpus {args.gpus})"}))
         sys.exit(2)
@@ -533,9 +533,9 @@ def main():
     download_ms = (time.perf_counter() - t0) * 1e3
     log(f"[bench] full state download {download_ms:.1f} ms")
 
-    # whole-step bandwidth: the bytes this build's kernels move by their
-    # algorithmic figures, and the survey's byte model of the reference's
-    # unfused passes (26 field passes a Poisson iteration), both at the
+    # whole-step bandwidth: the bytes these kernels move by their
+    # algorithmic figures, and a byte model of the reference's unfused
+    # whole-array passes (26 field passes a Poisson iteration), both at the
     # iteration count of the run
     whole = step_bytes(ncell, npart, mean_its) * value / 1e9
     kbytes = step_kernel_bytes(ncell, npart, 4 * mean_its, per_eps, per_tau, bool(p.TWC), p.Nu > 0, counted)
@@ -578,7 +578,7 @@ def main():
         "step_kernel_bytes_GB": round(kbytes / 1e9, 2),
         "step_GBps_kernel_bytes": round(whole_k, 1),
         "step_frac_kernel_bytes": round(whole_k / HBM_PEAK_GBS, 4),
-        "step_GBps_survey_byte_model": round(whole, 1),
+        "step_GBps_unfused_pass_model": round(whole, 1),
         "state_download_ms": round(download_ms, 1),
     }
     if cpu:

@@ -35,7 +35,7 @@ extern "C" {
 #define TPX_E_ARG (-1)      /* invalid argument */
 #define TPX_E_HIP (-2)      /* HIP runtime error */
 #define TPX_E_STATE (-3)    /* call not valid in the current state */
-#define TPX_E_UNSUPP (-4)   /* feature outside this build (HYPRE, scalar, ...) */
+#define TPX_E_UNSUPP (-4)   /* feature outside this library (HYPRE, ...) */
 #define TPX_E_OOM (-5)      /* device allocation failed */
 
 /* The input-file parameters the step reads (params.cpp:23-59, params.h:8-158).

@@ -476,7 +476,7 @@ static void update_position(or_state *s)
 }
 
 /* P3: particle::update_velocity (particle.cpp:362-399); per-particle tau is
- * the build's generalisation of the scalar Tp. */
+ * the library's generalisation of the scalar Tp. */
 static void update_velocity(or_state *s)
 {
   double iTp = s->Tp != 0 ? 1. / s->Tp : 0.;
@@ -505,7 +505,7 @@ static void update_velocity(or_state *s)
 }
 
 /* P4: particle::update_Temp (particle.cpp:477-492); per-particle epsilon is
- * the build's generalisation of the scalar epsilonp. */
+ * the library's generalisation of the scalar epsilonp. */
 static void update_temp(or_state *s)
 {
   double dt = s->p.dt, post = s->post[s->rk], pre = s->pre[s->rk];

@@ -1,5 +1,5 @@
 """Why the benchmark cases stop the Poisson iteration at max|dp| <= 1e-10 and
-not at the survey's 1e-13.
+not at the 1e-13 the benchmark configurations name.
 
 The stopping rule (poisson.cpp:391-411) compares the absolute max|dp| of an
 iteration with epsilon.  Once the iteration has converged, dp is the zero mode

@@ -57,7 +57,7 @@ void tensor0::upload(const double *ghosted) {
 }
 
 void tensor0::kill_strong_modes() {
-  throw std::runtime_error("kill_strong_modes (elongated box) is outside this build");
+  throw std::runtime_error("kill_strong_modes (elongated box) is outside this library");
 }
 
 // ---------------------------------------------------------------- tensor1

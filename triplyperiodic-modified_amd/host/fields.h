@@ -40,7 +40,7 @@ class tensor0 {
   std::vector<double> host() const;
   void download(double *ghosted) const;
   void upload(const double *ghosted);
-  // elongated-box mode filter (tensor0.cpp) is outside this build
+  // elongated-box mode filter (tensor0.cpp) is outside this library
   void kill_strong_modes();
 };
 

@@ -75,7 +75,7 @@ grid::~grid() {
 }
 
 void grid::unsupported(const char *what) {
-  throw std::runtime_error(std::string(what) + " is outside this build (the RK4 step of the particle-laden gas only)");
+  throw std::runtime_error(std::string(what) + " is outside this library (the RK4 step of the particle-laden gas only)");
 }
 
 void grid::pull_scalars() {
@@ -285,8 +285,8 @@ void grid::TimeAdvance() {
 }
 
 // grid::Statistics (grid.cpp:902-982): the reductions run on the device
-// (tpx_statistics); the particle trajectory file (part.trajectory) is not
-// written by this build.
+// (tpx_statistics), the tracked particles' positions for the trajectory file
+// (part.trajectory, particle.cpp:717-745) come from tpx_track_particles.
 void grid::Statistics() {
   if (!param_->Statistics()) return;
   double st[TPX_ST_COUNT];

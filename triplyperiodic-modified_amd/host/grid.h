@@ -73,6 +73,6 @@ class grid {
   bool Touch();               // grid.cpp:94-108
   int num_iteration() const { return PS_.num_iteration(); }
 
-  // elongated-box copy: outside this build
+  // elongated-box copy: outside this library
   void CopyBox() { unsupported("CopyBox"); }
 };

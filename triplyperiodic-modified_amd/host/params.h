@@ -139,7 +139,7 @@ class params {
 };
 
 // A single-process stand-in for the reference's MPI process grid (proc.h):
-// this build runs one context per GPU with the whole box on it.
+// this library runs one context per GPU with the whole box on it.
 class proc {
  public:
   bool IsRoot() const { return true; }
