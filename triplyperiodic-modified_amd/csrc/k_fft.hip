@@ -938,6 +938,17 @@ struct k_yz_pass {
   const int k2 = (int)(blk / (unsigned)mxh), kx = (int)(blk % (unsigned)mxh);
   double2 *base = spec + (long)blk * (NE + kRunPad);  // element (z, t) at z Y1 + t
   const int tid = threadIdx.x;
+  // the z twiddles and lambda_z beside the run (yz_shared_bytes), copied in
+  // with it: the transform stages then wait on LDS for them, not on global
+  // memory (74 against 98 us a pass at 256^3); lambda_x + lambda_y of the item
+  // this thread takes in z step B, read up front
+  double2 *twl = L + Y1 * RS;
+  double *lzl = reinterpret_cast<double *>(twl + nz);
+  const double lxy = tid < Y1 * Z2 ? lamx[kx] + lamy[k2 + Y2 * (tid / Z2)] : 0.0;
+  for (int e = tid; e < nz; e += kBounds) {
+    twl[e] = twz[e];
+    lzl[e] = lamz[e];
+  }
   // the run into LDS: every load of a thread issued before the first LDS
   // write (a loop over the block width read at run time waited on each load
   // in turn); yz_launch starts kBounds threads a block
@@ -978,7 +989,7 @@ struct k_yz_pass {
     if (ai) {
 #pragma unroll
       for (int m = 0; m < Z2; ++m) v[m] = L[ka * RS + pad(tz + Z1 * m)];
-      step_a<Z2, false>(v, tz, twz);
+      step_a<Z2, false>(v, tz, twl);
     }
     __syncthreads();
     if (ai) {
@@ -998,12 +1009,11 @@ struct k_yz_pass {
       for (int q = 0; q < Z1; ++q) w[q] = L[kb * RS + pad(kz2 * Z1 + q)];
       RegDft<Z1, false>::run(w);  // w[kz1] at kz = kz2 + Z2 kz1
       const int ky = k2 + Y2 * kb;
-      const double lxy = lamx[kx] + lamy[ky];
 #pragma unroll
       for (int kz1 = 0; kz1 < Z1; ++kz1) {
         const int kz = kz2 + Z2 * kz1;
         if (kx + ky + kz > 0) {
-          const double lam = lxy + lamz[kz];
+          const double lam = lxy + lzl[kz];
           w[kz1].y /= lam;
           w[kz1].x /= lam;
         }
@@ -1011,7 +1021,7 @@ struct k_yz_pass {
       RegDft<Z1, true>::run(w);
 #pragma unroll
       for (int q = 1; q < Z1; ++q) {
-        double2 tw = twz[kz2 * q];
+        double2 tw = twl[kz2 * q];
         tw.y = -tw.y;
         w[q] = cmul(w[q], tw);
       }
@@ -1471,6 +1481,12 @@ void xy_inv_launch(unsigned nb, size_t sh, hipStream_t st, const XInvArgs &xi, i
   else if (xi.Pout) xy_inv_launch_t<R1, R2, false, true>(nb, sh, st, xi, Y1, Y2, twy);
   else xy_inv_launch_t<R1, R2, false, false>(nb, sh, st, xi, Y1, Y2, twy);
 }
+// LDS of a pass-2 block: the run of y1 nz elements, one spare slot every 16,
+// then the nz z twiddles and the nz lambda_z (76 KB at nz = 256, y1 = 16: two
+// blocks a CU)
+size_t yz_shared_bytes(int y1, int nz) {
+  return sizeof(double2) * (size_t)y1 * (nz + nz / 16) + (sizeof(double2) + sizeof(double)) * (size_t)nz;
+}
 template <int Z1, int Z2>
 void yz_launch(unsigned nb, size_t sh, hipStream_t st, double2 *spec, int mxh, int Y1, int Y2, const tpx_ctx *c,
                const double *done) {
@@ -1533,7 +1549,7 @@ unsigned three_pass_launch(tpx_ctx *c, const XArgs &xa, const XInvArgs &xi, cons
   timing_end(c, TPX_K_FFT);
   timing_begin(c, TPX_K_FFT_YZ);
   const unsigned nb2 = (unsigned)(mxh * y2);
-  const size_t sh2 = sizeof(double2) * (size_t)y1 * (g.nz + g.nz / 16);
+  const size_t sh2 = yz_shared_bytes(y1, g.nz);
 #define L_YZ(A, B) yz_launch<A, B>(nb2, sh2, c->stream, c->spec[0], mxh, y1, y2, c, done)
   TPX_FOUR_STEP_SWITCH(g.nz, L_YZ)
 #undef L_YZ
