@@ -924,7 +924,11 @@ struct k_xy_fwd {
 
 template <int Y1, int Z1, int Z2>
 struct k_yz_pass {
-  static constexpr int kBounds = 256;
+  // one lane an item of the largest stage, whole waves: 64 lanes at nz = 64
+  // and Y1 = 8 (the 64^3 boxes), where 256 would leave three waves of four idle
+  // in every transform stage
+  static constexpr int kItems = std::max(Z1 * Z2, Y1 * std::max(Z1, Z2));
+  static constexpr int kBounds = kItems <= 64 ? 64 : (kItems <= 128 ? 128 : 256);
   static __device__ __forceinline__ void run(double2 *spec, int mxh, int Y2, const double2 *__restrict__ twz,
                                                  const double *__restrict__ lamx,
                                                  const double *__restrict__ lamy,
@@ -936,7 +940,16 @@ struct k_yz_pass {
   auto pad = [](int e) { return e + (e >> 4); };  // one spare slot every 16: fewer bank conflicts
   const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const int k2 = (int)(blk / (unsigned)mxh), kx = (int)(blk % (unsigned)mxh);
-  double2 *base = spec + (long)blk * (NE + kRunPad);  // element (z, t) at z Y1 + t
+  // element (z, t) at z Y1 + t.  The run and the tables are read through
+  // global-address-space pointers: in the batched launch the arguments come
+  // from a table in memory as generic pointers, whose flat loads also count
+  // against the LDS wait counter (and the register staging below went
+  // through scratch); a kernel's own pointer arguments are known to be global.
+  typedef double dv2 __attribute__((ext_vector_type(2)));  // a builtin pair: copied through any address space
+  typedef __attribute__((address_space(1))) dv2 gdv2;
+  typedef __attribute__((address_space(1))) const dv2 gcdv2;
+  typedef __attribute__((address_space(1))) const double gcdouble;
+  gdv2 *base = (gdv2 *)(spec + (long)blk * (NE + kRunPad));
   const int tid = threadIdx.x;
   // the z twiddles and lambda_z beside the run (yz_shared_bytes), copied in
   // with it: the transform stages then wait on LDS for them, not on global
@@ -944,10 +957,11 @@ struct k_yz_pass {
   // this thread takes in z step B, read up front
   double2 *twl = L + Y1 * RS;
   double *lzl = reinterpret_cast<double *>(twl + nz);
-  const double lxy = tid < Y1 * Z2 ? lamx[kx] + lamy[k2 + Y2 * (tid / Z2)] : 0.0;
+  const double lxy = tid < Y1 * Z2 ? ((gcdouble *)lamx)[kx] + ((gcdouble *)lamy)[k2 + Y2 * (tid / Z2)] : 0.0;
   for (int e = tid; e < nz; e += kBounds) {
-    twl[e] = twz[e];
-    lzl[e] = lamz[e];
+    const dv2 w = ((gcdv2 *)twz)[e];
+    twl[e] = make_double2(w.x, w.y);
+    lzl[e] = ((gcdouble *)lamz)[e];
   }
   // the run into LDS: every load of a thread issued before the first LDS
   // write (a loop over the block width read at run time waited on each load
@@ -958,7 +972,10 @@ struct k_yz_pass {
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
       const int e = tid + u * kBounds;
-      if (NE % kBounds == 0 || e < NE) r[u] = base[e];
+      if (NE % kBounds == 0 || e < NE) {
+        const dv2 q = base[e];
+        r[u] = make_double2(q.x, q.y);
+      }
     }
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
@@ -1064,7 +1081,10 @@ struct k_yz_pass {
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
     const int e = tid + u * kBounds;
-    if (NE % kBounds == 0 || e < NE) base[e] = L[pad(e)];
+    if (NE % kBounds == 0 || e < NE) {
+      const double2 q = L[pad(e)];
+      base[e] = dv2{q.x, q.y};
+    }
   }
 }
 };
@@ -1491,9 +1511,9 @@ template <int Z1, int Z2>
 void yz_launch(unsigned nb, size_t sh, hipStream_t st, double2 *spec, int mxh, int Y1, int Y2, const tpx_ctx *c,
                const double *done) {
   switch (Y1) {
-    case 4: tpx_launch<k_yz_pass<4, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], done); break;
-    case 8: tpx_launch<k_yz_pass<8, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], done); break;
-    default: tpx_launch<k_yz_pass<16, Z1, Z2>>(nb, 256, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], done); break;
+    case 4: tpx_launch<k_yz_pass<4, Z1, Z2>>(nb, k_yz_pass<4, Z1, Z2>::kBounds, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], done); break;
+    case 8: tpx_launch<k_yz_pass<8, Z1, Z2>>(nb, k_yz_pass<8, Z1, Z2>::kBounds, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], done); break;
+    default: tpx_launch<k_yz_pass<16, Z1, Z2>>(nb, k_yz_pass<16, Z1, Z2>::kBounds, sh, st, spec, mxh, Y2, c->tw[2], c->lam[0], c->lam[1], c->lam[2], done); break;
   }
 }
 
