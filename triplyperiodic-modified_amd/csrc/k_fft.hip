@@ -154,6 +154,17 @@ struct XArgs {
                          // caller's face coefficient), or null: formed here from rho
   const double *in = nullptr;  // three-pass form: transform this field's rows instead of a
                                // residual (P, rho, rhs unread)
+  __device__ void to_global() {
+    P = gptr(P);
+    rho = gptr(rho);
+    rhs = gptr(rhs);
+    scal = gptr(scal);
+    spec = gptr(spec);
+    tw = gptr(tw);
+    done = gptr(done);
+    res = gptr(res);
+    in = gptr(in);
+  }
 };
 
 // 1/x from the hardware reciprocal estimate and two Newton steps,
@@ -306,6 +317,14 @@ struct XInvArgs {
   const double *done;
   int overwrite = 0;  // three-pass form: P = dp (P unread, its own kernel instance) instead of P += dp
   double *Pout = nullptr;  // non-null: P + dp is written here and P is left as it was
+  __device__ void to_global() {
+    P = gptr(P);
+    spec = gptr(spec);
+    tw = gptr(tw);
+    partials = gptr(partials);
+    done = gptr(done);
+    Pout = gptr(Pout);
+  }
 };
 
 // X inverse pass: Hermitian extension, inverse FFT, P += dp, block max|dp|

@@ -96,6 +96,21 @@ struct RhsArgs {
   double *rho_np1;
   double *rho_new;
   double c_np1, c_new, c_rho_np1, c_rho_new;
+  __device__ void to_global() {
+    for (int c = 0; c < 3; ++c) {
+      ru_int.a[c] = gptr(ru_int.a[c]);
+      ru_base[c] = gptr(ru_base[c]);
+      ru_np1_in[c] = gptr(ru_np1_in[c]);
+      S[c] = gptr(S[c]);
+      ru_np1[c] = gptr(ru_np1[c]);
+      ru_new[c] = gptr(ru_new[c]);
+    }
+    rho_int = gptr(rho_int);
+    rho_base = gptr(rho_base);
+    rho_np1_in = gptr(rho_np1_in);
+    rho_np1 = gptr(rho_np1);
+    rho_new = gptr(rho_new);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -165,6 +180,15 @@ struct DivArgs {
   const double *scal;
   double inv_dtpre;
   double ncell;
+  __device__ void to_global() {
+    for (int q = 0; q < 3; ++q) ru_new.a[q] = gptr(ru_new.a[q]);
+    rho_new = gptr(rho_new);
+    Q = gptr(Q);
+    divergence = gptr(divergence);
+    rhs = gptr(rhs);
+    partials = gptr(partials);
+    scal = gptr(scal);
+  }
 };
 
 // (k/Cp) del2(1/Rho_new) - R/(Cp P0_new Vcell) Q' (grid.cpp:832-839)
@@ -223,6 +247,12 @@ struct ProjArgs {
   double *partials;
   long npart;
   double c_np1, c_new;
+  __device__ void to_global() {
+    P = gptr(P);
+    for (int q = 0; q < 3; ++q) np1[q] = gptr(np1[q]);
+    for (int q = 0; q < 3; ++q) nw[q] = gptr(nw[q]);
+    partials = gptr(partials);
+  }
 };
 
 // G10: Update_RU_WP (grid.cpp:857-863); at RK4_count==3 RU_new aliases
@@ -446,6 +476,15 @@ struct PsArgs {
   const double *scal;
   double d_m, c_post, c_pre, ncell;
   int s1_type, rk3;
+  __device__ void to_global() {
+    for (int q = 0; q < 3; ++q) ru_int.a[q] = gptr(ru_int.a[q]);
+    rho_int = gptr(rho_int);
+    cint = gptr(cint);
+    cbase = gptr(cbase);
+    cnp1 = gptr(cnp1);
+    cnew = gptr(cnew);
+    scal = gptr(scal);
+  }
 };
 
 struct k_passive_scalar {

@@ -80,6 +80,23 @@ struct AdvArgs {
   double epsp;              // scalar epsilonp
   double L[3];
   int rk3;
+  __device__ void to_global() {
+    for (int q = 0; q < 3; ++q) ru[q] = gptr(ru[q]);
+    rho = gptr(rho);
+    for (int q = 0; q < 7; ++q) base[q] = gptr(base[q]);
+    for (int q = 0; q < 7; ++q) in_int[q] = gptr(in_int[q]);
+    for (int q = 0; q < 7; ++q) np1_in[q] = gptr(np1_in[q]);
+    for (int q = 0; q < 7; ++q) np1[q] = gptr(np1[q]);
+    for (int q = 0; q < 7; ++q) nw[q] = gptr(nw[q]);
+    for (int q = 0; q < 3; ++q) base_w[q] = gptr(base_w[q]);
+    rhs_t = gptr(rhs_t);
+    tau = gptr(tau);
+    eps = gptr(eps);
+    contrib = gptr(contrib);
+    inv_int = gptr(inv_int);
+    cnt = gptr(cnt);
+    key = gptr(key);
+  }
 };
 
 // P1-P4, P6 shares and P7 wrap for one particle (particle.cpp:314-399,
@@ -385,6 +402,18 @@ struct HeatFusedArgs {
   double *rhs_t;             // by particle
   double *t3;                // 3 x n: contribution of row j to row j - 1, j, j + 1
   int rows, zc;
+  __device__ void to_global() {
+    start = gptr(start);
+    perm = gptr(perm);
+    x = gptr(x);
+    y = gptr(y);
+    z = gptr(z);
+    Tp = gptr(Tp);
+    rho = gptr(rho);
+    scal = gptr(scal);
+    rhs_t = gptr(rhs_t);
+    t3 = gptr(t3);
+  }
 };
 
 // The block works on whole x-rows of one base plane at a time; the cells of
@@ -705,6 +734,15 @@ struct CicDepArgs {
   const int32_t *start;
   double *out;
   double *partials;
+  __device__ void to_global() {
+    x = gptr(x);
+    y = gptr(y);
+    z = gptr(z);
+    perm = gptr(perm);
+    start = gptr(start);
+    out = gptr(out);
+    partials = gptr(partials);
+  }
 };
 
 // P5 pull (statistics only): concentration at x_int, weight one per particle
@@ -758,6 +796,15 @@ struct PermArgs {
   const int32_t *flag_s;
   int32_t *flag_d;
   long n;
+  __device__ void to_global() {
+    perm = gptr(perm);
+    for (int q = 0; q < 7; ++q) src[q] = gptr(src[q]);
+    for (int q = 0; q < 7; ++q) dst[q] = gptr(dst[q]);
+    for (int q = 0; q < 3; ++q) s1[q] = gptr(s1[q]);
+    for (int q = 0; q < 3; ++q) d1[q] = gptr(d1[q]);
+    flag_s = gptr(flag_s);
+    flag_d = gptr(flag_d);
+  }
 };
 
 // physical re-sort into cell order: sorted position q gathers the base state

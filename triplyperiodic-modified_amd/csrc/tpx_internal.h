@@ -477,9 +477,42 @@ __device__ __forceinline__ unsigned block_dim_x() {
   return *(const uint16_t *)((const char *)__builtin_amdgcn_implicitarg_ptr() + 12);
 }
 
+// A pointer read from an argument table is a generic (flat) address to the
+// compiler, where a kernel's own pointer argument is known to be global: its
+// accesses become flat loads and stores, which also count against the LDS
+// wait counter, so every LDS wait of the body waits on them too.  Every table
+// pointer is device (or pinned host) memory: gptr hands it on as a global
+// pointer (the empty asm keeps the compiler from folding the two casts away),
+// and the accesses compile as for a kernel argument.  An argument struct with
+// a to_global() member has its pointers passed through gptr the same way.
+template <class V>
+__device__ __forceinline__ V *gptr(V *p) {
+  __attribute__((address_space(1))) V *g = (__attribute__((address_space(1))) V *)p;
+  asm("" : "+s"(g));
+  return (V *)g;
+}
+template <class V, class = void>
+struct has_to_global : std::false_type {};
+template <class V>
+struct has_to_global<V, std::void_t<decltype(std::declval<V &>().to_global())>> : std::true_type {};
+template <class V>
+__device__ __forceinline__ V *as_global(V *p) {
+  return gptr(p);
+}
+template <class V, std::enable_if_t<has_to_global<V>::value, int> = 0>
+__device__ __forceinline__ V as_global(const V &v) {
+  V c = v;
+  c.to_global();
+  return c;
+}
+template <class V, std::enable_if_t<!has_to_global<V>::value && !std::is_pointer_v<V>, int> = 0>
+__device__ __forceinline__ const V &as_global(const V &v) {
+  return v;
+}
+
 template <class F, class T, size_t... I>
 __device__ __forceinline__ void run_args(const T &t, std::index_sequence<I...>) {
-  F::run(std::get<I>(t)...);
+  F::run(as_global(std::get<I>(t))...);
 }
 
 // Struct arguments reach the body by reference to the kernel argument itself
