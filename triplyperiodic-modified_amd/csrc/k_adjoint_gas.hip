@@ -179,48 +179,6 @@ struct PoisArgs {
   int last;                // the last one (the solve's first iteration)
 };
 
-// One Poisson iteration reversed, after its spectral solve nu = M lam_P:
-//   mu = cc nu,  lam_RHS -= mu,  lam_cc += nu (div(C grad P_i) - RHS_P),
-//   lam_P += div(C grad mu),
-// and the coefficient's share lam_C = -grad mu . grad P_i on each face taken
-// at once through C = 1/F' (lam_F' = -lam_C C^2) and F' = I_C2F(Rho_new)
-// (each face to its two cells, half each) into lam_Rho_new: the cell gets
-// 1/2 (C^2 grad mu grad P) of its six faces.  C is formed from Rho_new as the
-// forward iteration forms it; mu at the neighbours from their cc and nu.
-struct k_adj_pois_it {
-  static constexpr int kBounds = kBlock;
-  static __device__ __forceinline__ void run(const PoisArgs &a) {
-    const long c = (long)cell_block() * block_dim_x() + threadIdx.x;
-    double lr = 0.0;
-    if (c < a.g.n) {
-      const Cell x(a.g, c);
-      const double nuc = a.nu[c];
-      const double muc = a.cc[c] * nuc;
-      const double rc = a.rho[c], pc = a.Pi[c];
-      double dmu = 0.0, dpi = 0.0, lrho = 0.0;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const long m = x.at(d, -1), p = x.at(d, 1);
-        const double id = inv_d(a.g, d);
-        const double c0 = rcp_nr((rc + a.rho[m]) * 0.5), c1 = rcp_nr((a.rho[p] + rc) * 0.5);
-        const double mum = a.cc[m] * a.nu[m], mup = a.cc[p] * a.nu[p];
-        const double gm0 = (muc - mum) * id, gm1 = (mup - muc) * id;
-        const double gp0 = (pc - a.Pi[m]) * id, gp1 = (a.Pi[p] - pc) * id;
-        dmu += (c1 * gm1 - c0 * gm0) * id;
-        dpi += (c1 * gp1 - c0 * gp0) * id;
-        lrho += 0.5 * ((c0 * c0) * (gm0 * gp0) + (c1 * c1) * (gm1 * gp1));
-      }
-      const double mean = a.rhs_sum[0] / (double)a.g.n;
-      const double rr = dpi - (a.rhs[c] - mean);
-      lr = (a.first ? 0.0 : a.lrhs[c]) - muc;
-      a.lrhs[c] = lr;
-      a.lcc[c] = (a.first ? 0.0 : a.lcc[c]) + nuc * rr;
-      a.lp[c] = a.lp[c] + dmu;
-      a.lrho[c] = a.lrho[c] + lrho;
-    }
-    if (a.last) block_sum(lr, a.partials);
-  }
-};
 
 // cc's dependence on C, then C = 1/F': lam_F' = -lam_C C^2 with lam_C the
 // cc share (first write of lam_F')
@@ -558,6 +516,98 @@ struct k_adj_mom {
   }
 };
 
+// One Poisson iteration reversed, after its spectral solve nu = M lam_P:
+//   mu = cc nu,  lam_RHS -= mu,  lam_cc += nu (div(C grad P_i) - RHS_P),
+//   lam_P += div(C grad mu),
+// and the coefficient's share lam_C = -grad mu . grad P_i on each face taken
+// at once through C = 1/F' (lam_F' = -lam_C C^2) and F' = I_C2F(Rho_new)
+// (each face to its two cells, half each) into lam_Rho_new: the cell gets
+// 1/2 (C^2 grad mu grad P) of its six faces.  C is formed from Rho_new as the
+// forward iteration forms it; mu at the neighbours from their cc and nu.
+// An LDS z-march: a group takes a TX x TY column of KC planes; each plane's
+// nu, cc, Rho_new and P_i with a one-cell x/y halo come into LDS once (two
+// slots, this plane and the next), the plane below's values at the cell stay
+// in registers and the plane above's are read from the next slot: six loads a
+// cell instead of 28 neighbour loads with one thread a cell (51 -> 44 us at
+// 128^3).  The block sums of lam_RHS (the last reversed iteration) cover a
+// column of cells each.
+struct k_adj_pois_zm {
+  static constexpr int kBounds = kBlock;
+  static __device__ __forceinline__ void run(const PoisArgs &a, const AdjMomGeom zg) {
+    constexpr int NF = 4;  // nu, cc, Rho_new, P_i
+    extern __shared__ double sm[];
+    const Geo &g = a.g;
+    const int TX = zg.TX, TY = zg.TY, KC = zg.KC;
+    const int RW = TX + 2, RN = RW * (TY + 2);
+    const unsigned blk = cell_block();
+    const int ntx = g.nx / TX, nty = g.ny / TY;
+    const int i0 = (int)(blk % (unsigned)ntx) * TX, j0 = (int)((blk / (unsigned)ntx) % (unsigned)nty) * TY;
+    const int k0 = (int)(blk / (unsigned)(ntx * nty)) * KC;
+    const int tid = (int)threadIdx.x, bd = (int)block_dim_x(), tx = tid % TX, ty = tid / TX;
+    const int mx = g.nx - 1, my = g.ny - 1, mz = g.nz - 1;
+    const int i = i0 + tx, j = j0 + ty;
+    const double *f[NF] = {a.nu, a.cc, a.rho, a.Pi};
+    auto gidx = [&](int x, int y, int z) { return (long)(z & mz) * g.nxy + (long)(y & my) * g.nx + (x & mx); };
+    auto T = [&](int slot, int q, int x, int y) -> double { return sm[(slot * NF + q) * RN + (y + 1) * RW + (x + 1)]; };
+    auto load_tile = [&](int kz, int slot) {
+      for (int e = tid; e < RN; e += bd) {
+        const long o = gidx(i0 + e % RW - 1, j0 + e / RW - 1, kz);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) sm[(slot * NF + q) * RN + e] = f[q][o];
+      }
+    };
+    double zb[NF];  // the plane below, at the cell
+    {
+      const long o = gidx(i, j, k0 - 1);
+#pragma unroll
+      for (int q = 0; q < NF; ++q) zb[q] = f[q][o];
+    }
+    load_tile(k0, 0);
+    load_tile(k0 + 1, 1);
+    __syncthreads();
+    const double mean = a.rhs_sum[0] / (double)g.n;
+    double part = 0.0;
+    int cur = 0;
+    for (int kk = k0; kk < k0 + KC; ++kk) {
+      const int nxt = cur ^ 1;
+      const long c = gidx(i, j, kk);
+      const double nuc = T(cur, 0, tx, ty);
+      const double muc = T(cur, 1, tx, ty) * nuc;
+      const double rc = T(cur, 2, tx, ty), pc = T(cur, 3, tx, ty);
+      double dmu = 0.0, dpi = 0.0, lrho = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        // field q at the low (m) and high (p) neighbour along d
+        auto vm = [&](int q) { return d == 0 ? T(cur, q, tx - 1, ty) : (d == 1 ? T(cur, q, tx, ty - 1) : zb[q]); };
+        auto vp = [&](int q) { return d == 0 ? T(cur, q, tx + 1, ty) : (d == 1 ? T(cur, q, tx, ty + 1) : T(nxt, q, tx, ty)); };
+        const double id = inv_d(g, d);
+        const double c0 = rcp_nr((rc + vm(2)) * 0.5), c1 = rcp_nr((vp(2) + rc) * 0.5);
+        const double mum = vm(1) * vm(0), mup = vp(1) * vp(0);
+        const double gm0 = (muc - mum) * id, gm1 = (mup - muc) * id;
+        const double gp0 = (pc - vm(3)) * id, gp1 = (vp(3) - pc) * id;
+        dmu += (c1 * gm1 - c0 * gm0) * id;
+        dpi += (c1 * gp1 - c0 * gp0) * id;
+        lrho += 0.5 * ((c0 * c0) * (gm0 * gp0) + (c1 * c1) * (gm1 * gp1));
+      }
+      const double rr = dpi - (a.rhs[c] - mean);
+      const double lr = (a.first ? 0.0 : a.lrhs[c]) - muc;
+      a.lrhs[c] = lr;
+      a.lcc[c] = (a.first ? 0.0 : a.lcc[c]) + nuc * rr;
+      a.lp[c] = a.lp[c] + dmu;
+      a.lrho[c] = a.lrho[c] + lrho;
+      part += lr;
+      // this plane becomes the plane below of the next
+#pragma unroll
+      for (int q = 0; q < NF; ++q) zb[q] = T(cur, q, tx, ty);
+      __syncthreads();
+      if (kk + 1 < k0 + KC) load_tile(kk + 2, cur);
+      __syncthreads();
+      cur = nxt;
+    }
+    if (a.last) block_sum_bd(part, a.partials);
+  }
+};
+
 // acc[0] += sum of partials (one block)
 struct k_adj_acc {
   static constexpr int kBounds = 1024;
@@ -635,6 +685,8 @@ unsigned cells(const tpx_ctx *c) { return grid1d(c->g.n, kBlock); }
 
 }  // namespace
 
+static long launch_adj_pois_it(tpx_ctx *c, const PoisArgs &pa);
+
 void adjoint_gas_boundary(tpx_ctx *c, GasAdjoint &ga, bool heat) {
   const long n = c->g.n;
   if (heat) {
@@ -686,15 +738,15 @@ void adjoint_gas_substep_a(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
   pa.lcc = ga.lcc;
   pa.lrho = ga.lrnew;
   pa.partials = c->partials;
+  long nbz = 0;
   for (int i = s.iters - 1; i >= 0; --i) {
     pa.Pi = s.P[i];
     pa.first = i == s.iters - 1;
     pa.last = i == 0;
     launch_spectral_apply(c, ga.lp, ga.nu, ga.scal);
-    tpx_launch<k_adj_pois_it>(nb, kBlock, 0, c->stream, pa);
-    TPX_LAUNCH_CHECK();
+    nbz = launch_adj_pois_it(c, pa);
   }
-  launch_finish_sum(c, (int)nb, 0, SC_ADJ_SUM);  // sum of lam_RHS_Pois from the last k_adj_pois_it
+  launch_finish_sum(c, (int)nbz, 0, SC_ADJ_SUM);  // sum of lam_RHS_Pois from the last reversed iteration
   // G6 and G5, with the Poisson coefficient's share of lam_F' (cc(C), C = 1/F')
   {
     RhsPArgs a{};
@@ -753,6 +805,36 @@ const double *const *adjoint_gas_substep_rk(tpx_ctx *c, GasAdjoint &ga, const Ga
   return ga.lrhs;
 }
 
+
+// the z-march tile of the adjoint stencils: 32 x 8 cells (all of x and more
+// rows when nx < 32), 32 planes a group, halved while the launch has fewer
+// than 1024 groups (down to 2); *nblocks the launch's groups
+static AdjMomGeom adj_zm_geom(const tpx_ctx *c, long *nblocks, const char *what) {
+  const Geo &g = c->g;
+  AdjMomGeom zg;
+  zg.TX = g.nx < 32 ? g.nx : 32;
+  zg.TY = kBlock / zg.TX < g.ny ? kBlock / zg.TX : g.ny;
+  zg.KC = g.nz < 32 ? g.nz : 32;
+  const long cols = (long)(g.nx / zg.TX) * (g.ny / zg.TY);
+  while (zg.KC > 2 && zg.KC % 2 == 0 && cols * (g.nz / zg.KC) < 1024) zg.KC /= 2;
+  const int threads = zg.TX * zg.TY;
+  *nblocks = cols * (g.nz / zg.KC);
+  if ((zg.TX + 2) * (zg.TY + 2) > 2 * threads || (threads & (threads - 1)) != 0 || *nblocks > c->n_partials)
+    throw std::logic_error(std::string(what) + ": the grid does not fit the z-march tile");
+  return zg;
+}
+
+// one reversed Poisson iteration (k_adj_pois_zm); returns its groups (the
+// number of lam_RHS block sums it leaves when pa.last is set)
+static long launch_adj_pois_it(tpx_ctx *c, const PoisArgs &pa) {
+  long nbz = 0;
+  const AdjMomGeom zg = adj_zm_geom(c, &nbz, "Poisson adjoint");
+  const size_t sh = sizeof(double) * 2 * 4 * (size_t)((zg.TX + 2) * (zg.TY + 2));
+  tpx_launch<k_adj_pois_zm>((unsigned)nbz, zg.TX * zg.TY, sh, c->stream, pa, zg);
+  TPX_LAUNCH_CHECK();
+  return nbz;
+}
+
 void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, const Coef &k) {
   const long n = c->g.n;
   const unsigned nb = cells(c);
@@ -772,22 +854,10 @@ void adjoint_gas_substep_b(tpx_ctx *c, GasAdjoint &ga, const GasSubstep &s, cons
   m.b_visc = k.b_visc;
   m.c_visc = k.c_visc;
   m.A = k.A;
-  // the z-march's tile: 32 x 8 cells (all of x and more rows when nx < 32),
-  // 32 planes a group, halved while the launch has fewer than 1024 groups
-  // (down to 2)
-  const Geo &g = c->g;
-  AdjMomGeom zg;
-  zg.TX = g.nx < 32 ? g.nx : 32;
-  zg.TY = kBlock / zg.TX < g.ny ? kBlock / zg.TX : g.ny;
-  zg.KC = g.nz < 32 ? g.nz : 32;
-  const long cols = (long)(g.nx / zg.TX) * (g.ny / zg.TY);
-  while (zg.KC > 2 && zg.KC % 2 == 0 && cols * (g.nz / zg.KC) < 1024) zg.KC /= 2;
-  const int threads = zg.TX * zg.TY;
-  const long nbm = cols * (g.nz / zg.KC);
-  if ((zg.TX + 2) * (zg.TY + 2) > 2 * threads || (threads & (threads - 1)) != 0 || nbm > c->n_partials)
-    throw std::logic_error("momentum adjoint: the grid does not fit the z-march tile");
+  long nbm = 0;
+  const AdjMomGeom zg = adj_zm_geom(c, &nbm, "momentum adjoint");
   const size_t sh = sizeof(double) * 2 * 11 * (size_t)((zg.TX + 2) * (zg.TY + 2));
-  tpx_launch<k_adj_mom>((unsigned)nbm, threads, sh, c->stream, m, zg);
+  tpx_launch<k_adj_mom>((unsigned)nbm, zg.TX * zg.TY, sh, c->stream, m, zg);
   TPX_LAUNCH_CHECK();
   tpx_launch<k_adj_acc>(1, 1024, 0, c->stream, (const double *)c->partials, nbm, c->scal + SC_ADJ_A);
   TPX_LAUNCH_CHECK();
@@ -847,8 +917,7 @@ void adjoint_poisson_check(tpx_ctx *c, int op, const double *rho, const double *
   pa.partials = c->partials;
   pa.first = 1;
   pa.last = 1;
-  tpx_launch<k_adj_pois_it>(nb, kBlock, 0, c->stream, pa);
-  TPX_LAUNCH_CHECK();
+  launch_adj_pois_it(c, pa);
   tpx_launch<k_adj_pois_c>(nb, kBlock, 0, c->stream, pa);
   TPX_LAUNCH_CHECK();
   tpx_launch<k_adj_fold_add>(nb, kBlock, 0, c->stream, c->g, (const double *)lF[0], (const double *)lF[1],
